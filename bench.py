@@ -1,0 +1,146 @@
+#!/usr/bin/env python3
+"""Detected coincidences/s of the MI355X photon-transport library on the head workload.
+
+Workload (one step = one full simulation): 128x128x96 head phantom at 2 mm (8 tissues),
+18F activity grey 4 : white 1 : other 0.5, 1e9 expected decays over a 10 s acquisition,
+cylindrical scanner R = 40 cm, H = 25 cm, 12% energy resolution, 350-600 keV window,
+3D sinogram 168 x 168 x 1293 (int32), emission images, energy spectrum. Inputs are
+resident on the device before timing; every step clears the tallies and simulates all
+decays with a new seed.
+
+Prints one JSON line (see DESIGN.md for the field definitions).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "mcgpu-pet_amd"))
+
+MATERIALS = os.path.join(ROOT, "tests", "golden", "sample_simulation", "materials")
+
+# Algorithmic work units (VALU lane-ops, HBM bytes) per event, counted from the reference
+# source: RANECU draw 25 ops; Woodcock step 83 ops incl. its 2 draws, 4 B packed voxel word;
+# material/energy change 24 B of (a,b); Compton 40 + 35 per shell evaluation; detected
+# coincidence 32 B of atomic read-modify-write.
+U_DRAW, U_STEP, U_COMPTON, U_SHELL = 25, 83, 40, 35
+B_STEP, B_MATCHANGE, B_COINC = 4, 24, 32
+VALU_PEAK = 39.3e12      # 256 CU x 4 SIMD x 16 lanes x 2.4 GHz, non-packed lane-ops/s
+HBM_PEAK = 8.0e12        # B/s
+
+
+def algorithmic(stats):
+    steps = stats["woodcock_steps"]
+    other_draws = max(0, stats["rn_draws"] - 2 * steps)
+    ops = (U_STEP * steps + U_DRAW * other_draws + U_COMPTON * stats["compton"] + U_SHELL * stats["compton_shell_evals"])
+    bytes_ = B_STEP * steps + B_MATCHANGE * stats["material_changes"] + B_COINC * stats["coincidences"]
+    return ops, bytes_
+
+
+def load_traffic():
+    p = os.path.join(ROOT, "profiles", "transport_head_traffic.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    return None
+
+
+def cpu_baseline(cfg, ph, vs, names, threads, stride):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py
+    from mcpet import phantoms
+    tab = oracle_py.OracleTables(names + [""] * (15 - len(names)), ph.density_max(), "libm")
+    o = oracle_py.Oracle(phantoms.config_dict(cfg), ph.voxel_mat_dens(), ph.shape, vs, tab, "libm")
+    r = o.run(nthreads=threads, voxel_stride=stride)
+    c = r["counters"]
+    return {"value": c["coincidences"] / r["wall_seconds"], "unit": "coincidences/s", "cores": threads, "kind": "port",
+            "sample": f"every {stride}th voxel of the same phantom, full 10 s acquisition: {c['decays']} decays, "
+                      f"{c['coincidences']} coincidences in {r['wall_seconds']:.1f} s (C restatement, libm, "
+                      f"{threads} threads)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--decays", type=float, default=1.0e9)
+    ap.add_argument("--tpb", type=int, default=32, help="decay streams per emitting voxel")
+    ap.add_argument("--kernel", choices=["persistent", "plain"], default="persistent")
+    ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--block-size", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-stride", type=int, default=5)
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one MI355X only; multi-GPU runs are not supported "
+              "(see DESIGN.md for how the path shards)", file=sys.stderr)
+        sys.exit(2)
+
+    import mcpet
+    from mcpet import phantoms
+    mat, dens, act, vs = phantoms.head(n_decays=a.decays)
+    names = phantoms.head_materials(MATERIALS)
+    cfg = phantoms.make_config(seed=20261202, threads_per_block=a.tpb, acquisition_time_s=10.0, nrad=168,
+                               materials=names)
+    ph = mcpet.phantom_from_arrays(mat, dens, act, vs)
+    tab = mcpet.load_materials(names, ph.density_max())
+    kernel = mcpet.KERNEL_PERSISTENT if a.kernel == "persistent" else mcpet.KERNEL_PLAIN
+    sim = mcpet.Simulation(cfg, ph, tab)
+    run_kw = dict(kernel=kernel, block_size=a.block_size, blocks_per_cu=a.blocks_per_cu)
+    for w in range(a.warmup):
+        sim.run(seed=1000 + w, **run_kw)
+        sim.sync()
+    sim.sync()
+    coinc = decays = 0
+    kernel_ms = []
+    agg = None
+    t0 = time.perf_counter()
+    for s in range(a.steps):
+        sim.run(seed=2000 + s, **run_kw)
+        st = sim.stats()               # synchronises; counters of this step
+        coinc += st["coincidences"]
+        decays += st["decays"]
+        kernel_ms.append(st["kernel_ms"])
+        agg = st if agg is None else {k: agg[k] + st[k] for k in agg}
+    sim.sync()
+    elapsed = time.perf_counter() - t0
+    sim.close()
+
+    ops, byts = algorithmic(agg)
+    kt = sum(kernel_ms) / 1e3
+    avg_ms = sum(kernel_ms) / len(kernel_ms)
+    valu = ops / kt
+    hbm = byts / kt
+    traffic = load_traffic()
+    out = {
+        "metric": "detected coincidences/s on 1 MI355X", "value": coinc / elapsed, "unit": "coincidences/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * elapsed / a.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "head phantom 128x128x96 at 2 mm, 8 tissues, 18F, 1e9 decays in 10 s, "
+                               "sinogram 168x168x1293, fused transport + tallies",
+                   "decays_per_step": decays / a.steps, "threads_per_block": a.tpb, "kernel": a.kernel,
+                   "parallelism": "single GPU"},
+        "decays_per_s": decays / elapsed, "photon_histories_per_s": 2 * decays / elapsed,
+        "kernel_ms_avg": avg_ms,
+        "roofline": {"bound": "valu", "achieved": valu / 1e12, "peak": VALU_PEAK / 1e12, "unit": "Tlane-op/s",
+                     "frac": valu / VALU_PEAK, "traffic": traffic,
+                     "hbm": {"achieved": hbm / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": hbm / HBM_PEAK,
+                             "algorithmic_bytes_per_launch": byts / a.steps},
+                     "algorithmic_lane_ops_per_launch": ops / a.steps, "kernel": "k_transport (persistent)"},
+        "events_per_decay": {k: agg[k] / max(1, decays) for k in ("woodcock_steps", "material_changes", "compton",
+                                                                   "rayleigh", "photoelectric", "coincidences",
+                                                                   "rn_draws")},
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(cfg, ph, vs, names, a.cpu_threads, a.cpu_stride)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

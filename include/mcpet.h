@@ -1,0 +1,294 @@
+/*
+ * mcpet.h — C ABI of the MI355X Monte Carlo PET photon-transport library (libmcpet.so).
+ *
+ * The reference (MCGPU-PET v0.1) has no library or FFI: its photon-transport path
+ * sits behind (1) the launch of track_particles (MCGPU-PET.cu:452-487) with the
+ * device buffers prepared by init_CUDA_device (MCGPU-PET.cu:1842-2146) and
+ * (2) the CLI/file contract (MCGPU-PET.in, .vox, .mcgpu.gz in; sinograms, images,
+ * spectrum, PSF, dose out). Every entry point below replaces one piece of that,
+ * cited next to it. Layouts of the inputs are the reference's, so its files drop in.
+ *
+ * Conventions
+ *   - Plain C types only; no exceptions cross the boundary; every call returns a
+ *     status code (the reference calls exit(-1|-2|-3) instead).
+ *   - The library owns device memory; the caller owns host buffers it passes in.
+ *   - One context per host thread; a context owns one HIP stream and is not
+ *     thread-safe. Distinct contexts may be used from distinct threads.
+ */
+#ifndef MCPET_H
+#define MCPET_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Compile-time limits of the reference (MCGPU-PET.h:96-99). */
+#define MCPET_MAX_MATERIALS 15
+#define MCPET_MAX_SHELLS 30
+#define MCPET_NP_RAYLEIGH 128
+#define MCPET_MAX_ENERGYBINS_RAYLEIGH 25005
+#define MCPET_NAME_LEN 250
+
+typedef enum mcpet_status {
+  MCPET_OK = 0,
+  MCPET_E_ARG = 1,      /* invalid argument / inconsistent sizes          */
+  MCPET_E_IO = 2,       /* file missing or unreadable   (ref exit(-1|-2)) */
+  MCPET_E_FORMAT = 3,   /* malformed input file          (ref exit(-2))   */
+  MCPET_E_RANGE = 4,    /* value out of the supported range              */
+  MCPET_E_NOMEM = 5,    /* host or device allocation failed              */
+  MCPET_E_HIP = 6,      /* HIP runtime error (no device, launch failure) */
+  MCPET_E_STATE = 7,    /* call out of order (e.g. fetch before run)     */
+  MCPET_E_CAPACITY = 8  /* a fixed-capacity buffer was exceeded          */
+} mcpet_status;
+
+/* ------------------------------------------------------------------ data layouts */
+
+/* Cylindrical detector: reference detector_struct (MCGPU-PET.h:220-232). */
+typedef struct mcpet_detector {
+  float center[3];            /* PSF_center (cm)                                 */
+  float radius;               /* PSF_radius (cm); <0 in the .in means "centre on bbox" */
+  float height;               /* PSF_height (cm)                                 */
+  int32_t psf_size;           /* PSF capacity in records                          */
+  int32_t tally_psf_sinogram; /* 0 both, 1 PSF only, 2 sinogram only              */
+  int32_t tally_type;         /* 0 both, 1 trues only, 2 scatter only             */
+} mcpet_detector;
+
+/* Phase-space record: reference PSF_element_struct, packed, 44 bytes (MCGPU-PET.h:235-247). */
+#pragma pack(push, 4)
+typedef struct mcpet_psf_record {
+  uint64_t emission_time_ps;
+  float travel_time_ps;
+  int32_t emission_absvox;
+  float energy, z, phi, vx, vy, vz;
+  int16_t index1;             /* scatter state 0 none,1 Compton,2 Rayleigh,3 multiple */
+  int16_t index2;             /* 0 for a 511 keV annihilation photon                  */
+} mcpet_psf_record;
+#pragma pack(pop)
+
+/* Rayleigh sampling data: reference rayleigh_struct (MCGPU-PET.h:347-360). */
+typedef struct mcpet_rayleigh_table {
+  float xco[MCPET_NP_RAYLEIGH * MCPET_MAX_MATERIALS];
+  float pco[MCPET_NP_RAYLEIGH * MCPET_MAX_MATERIALS];
+  float aco[MCPET_NP_RAYLEIGH * MCPET_MAX_MATERIALS];
+  float bco[MCPET_NP_RAYLEIGH * MCPET_MAX_MATERIALS];
+  float pmax[MCPET_MAX_ENERGYBINS_RAYLEIGH * MCPET_MAX_MATERIALS];
+  uint8_t itlco[MCPET_NP_RAYLEIGH * MCPET_MAX_MATERIALS];
+  uint8_t ituco[MCPET_NP_RAYLEIGH * MCPET_MAX_MATERIALS];
+} mcpet_rayleigh_table;
+
+/* Compton shell data: reference compton_struct (MCGPU-PET.h:334-344). */
+typedef struct mcpet_compton_table {
+  float fco[MCPET_MAX_MATERIALS * MCPET_MAX_SHELLS];
+  float uico[MCPET_MAX_MATERIALS * MCPET_MAX_SHELLS];
+  float fj0[MCPET_MAX_MATERIALS * MCPET_MAX_SHELLS];
+  int32_t noscco[MCPET_MAX_MATERIALS];
+} mcpet_compton_table;
+
+/* Everything read_input returns (MCGPU-PET.cu:866-1280). */
+typedef struct mcpet_config {
+  int32_t seed;                  /* RANECU seed; 0 in the file means time-based (C:920-928) */
+  int32_t gpu_id;
+  int32_t threads_per_block;     /* "GPU THREADS PER CUDA BLOCK": decay streams per voxel */
+  float fact_object;             /* density scale factor (C:938,1494)                      */
+  uint64_t acquisition_time_ps;  /* (uint64)(T*1e12+0.5) (C:977)                            */
+  float mean_life;               /* isotope mean life (s)                                   */
+  float activity[MCPET_MAX_MATERIALS]; /* parsed but unused by the transport (C:987-995)    */
+  char psf_file[MCPET_NAME_LEN];
+  mcpet_detector detector;
+  int32_t material_dose;         /* 1 tally dose per material                               */
+  int32_t voxel_dose;            /* 1 tally dose per voxel in the ROI                       */
+  char dose_file[MCPET_NAME_LEN];
+  int16_t roi_x_min, roi_x_max, roi_y_min, roi_y_max, roi_z_min, roi_z_max; /* 0-based */
+  float e_resol, e_low, e_high;  /* energy resolution (FWHM fraction), window in eV         */
+  float fovz;
+  int32_t nrows, ncrystals, nangles, nrad, nzs, res, ne, mrd, span;
+  int32_t nseg, nsinos, nbins, nvoxs; /* derived (C:1176-1189)                               */
+  char voxel_file[MCPET_NAME_LEN];
+  char material_files[MCPET_MAX_MATERIALS][MCPET_NAME_LEN];
+  int32_t bg_mode;               /* 1: second photon isotropic ("BG" run, C:81-84)          */
+} mcpet_config;
+
+/* Voxelized phantom (load_voxels, MCGPU-PET.cu:1381-1533). */
+typedef struct mcpet_phantom {
+  int32_t nx, ny, nz;
+  float voxel_size[3];
+  float inv_voxel_size[3];       /* 1/voxel_size as float (C:1460-1462)      */
+  float size_bbox[3];            /* n*voxel_size as float (C:1433-1435)      */
+  float density_max[MCPET_MAX_MATERIALS]; /* -999 where a material is absent */
+  float* voxel_mat_dens;         /* 3*nx*ny*nz floats, x fastest: (material+0.0001, density, activity Bq) */
+  int32_t owned;                 /* 1 if voxel_mat_dens was allocated by the library */
+} mcpet_phantom;
+
+/* Interaction tables (load_material, MCGPU-PET.cu:1565-1831). */
+typedef struct mcpet_tables {
+  int32_t num_values;            /* energy grid points (10200 in the sample files) */
+  float e0, ide;                 /* first energy (eV), 1/bin width (1/eV)          */
+  float density_nominal[MCPET_MAX_MATERIALS]; /* -1 for unused slots             */
+  float* woodcock;               /* 2*num_values: (a, b), mfp_W(E) = a + E b         */
+  float* mfp_a;                  /* 3*num_values*15, [bin][mat]: (total, Compton, Rayleigh) */
+  float* mfp_b;                  /* same layout, slopes                             */
+  mcpet_rayleigh_table* rayleigh;
+  mcpet_compton_table* compton;
+  int32_t owned;
+} mcpet_tables;
+
+/* Results copied back by mcpet_fetch (reference D2H copies, MCGPU-PET.cu:499-535,642-652). */
+typedef struct mcpet_tallies {
+  int32_t* trues;                /* nbins     (may be NULL to skip)    */
+  int32_t* scatter;              /* nbins                              */
+  int32_t* image_trues;          /* nx*ny*nz                           */
+  int32_t* image_scatter;        /* nx*ny*nz                           */
+  int32_t* spectrum;             /* ne                                 */
+  mcpet_psf_record* psf;         /* psf_capacity records (may be NULL) */
+  int64_t psf_capacity;
+  uint64_t* voxel_edep;          /* 2*N_ROI (Edep, Edep^2) pairs       */
+  uint64_t* material_edep;       /* 2*15                                */
+  /* scalars, always filled */
+  uint64_t total_histories;      /* photons: 2 per decay (K:282)       */
+  int64_t psf_count;             /* records written (index_PSF, clamped to capacity) */
+  int64_t psf_requested;         /* records that reached the PSF stage (index_PSF before clamp) */
+  int64_t sum_trues, sum_scatter;
+  int32_t final_seed;            /* seed.x of the last stream (K:454-455); 0 if not written */
+} mcpet_tallies;
+
+/* Transport launch variants. */
+typedef enum mcpet_kernel_kind {
+  MCPET_KERNEL_AUTO = 0,
+  MCPET_KERNEL_PLAIN = 1,        /* one workgroup per emitting voxel, lanes = decay streams */
+  MCPET_KERNEL_PERSISTENT = 2    /* persistent waves pulling streams from a device queue    */
+} mcpet_kernel_kind;
+
+typedef struct mcpet_run_params {
+  int32_t kernel;                /* mcpet_kernel_kind                                  */
+  int32_t block_size;            /* workgroup size for the persistent kernel (0: auto) */
+  int32_t blocks_per_cu;         /* persistent grid = CUs * blocks_per_cu (0: auto)    */
+  int32_t zero_tallies;          /* 1: clear tallies before the run (default)          */
+  int32_t seed;                  /* 0: use the config seed                             */
+  uint64_t acquisition_time_ps;  /* 0: use the config value                            */
+} mcpet_run_params;
+
+/* Per-run statistics: HIP-event kernel time and algorithmic event counts. */
+typedef struct mcpet_stats {
+  double kernel_ms;              /* transport kernel(s), HIP events on the context stream */
+  double zero_ms;                /* tally clearing                                       */
+  uint64_t decays;
+  uint64_t streams;
+  uint64_t woodcock_steps;
+  uint64_t material_changes;     /* (a,b) table reloads                                  */
+  uint64_t compton;
+  uint64_t compton_shell_evals;  /* sum over Compton events of n_shell * n_trials        */
+  uint64_t rayleigh;
+  uint64_t photoelectric;
+  uint64_t pairs_tallied;        /* coincidences reaching tally_Sinogram/PSF             */
+  uint64_t coincidences;         /* trues + scatter binned                               */
+  uint64_t rn_draws;
+} mcpet_stats;
+
+/* ------------------------------------------------------------------ host-side input (boundary) */
+
+/* Parse a MCGPU-PET.in file. Replaces read_input (MCGPU-PET.cu:866-1280), fgets_trimmed
+ * (C:1320-1350), trim_name (C:1292-1311) and the sinogram-size formula (C:1176-1189).
+ * The seed 0 -> time-based rule is applied. err (may be NULL) receives a message. */
+mcpet_status mcpet_read_input(const char* path, mcpet_config* cfg, char* err, size_t errlen);
+
+/* Load a .vox[.gz] phantom. Replaces load_voxels (MCGPU-PET.cu:1381-1533): applies
+ * fact_object, clamps the ROI maxima of cfg (may be NULL) to the phantom size. */
+mcpet_status mcpet_load_voxels(const char* path, mcpet_config* cfg, mcpet_phantom* out,
+                               char* err, size_t errlen);
+
+/* Build a phantom from caller arrays (material id 1..15, density g/cm3, activity Bq), x fastest.
+ * Equivalent to writing and re-reading a .vox file. */
+mcpet_status mcpet_phantom_from_arrays(int32_t nx, int32_t ny, int32_t nz, const float voxel_size[3],
+                                       const int32_t* material, const float* density,
+                                       const float* activity, float fact_object, mcpet_phantom* out,
+                                       char* err, size_t errlen);
+void mcpet_free_phantom(mcpet_phantom* ph);
+
+/* Read material files and build the interaction tables. Replaces load_material
+ * (MCGPU-PET.cu:1565-1831). Slots with an empty name are skipped; slots absent from
+ * the phantom (density_max <= 0) are skipped except slot 0 (C:1608-1621). */
+mcpet_status mcpet_load_materials(const char names[MCPET_MAX_MATERIALS][MCPET_NAME_LEN],
+                                  const float density_max[MCPET_MAX_MATERIALS], mcpet_tables* out,
+                                  char* err, size_t errlen);
+void mcpet_free_tables(mcpet_tables* t);
+
+/* Sinogram sizes from (MRD, SPAN, NZS, NRAD, NANGLES): NSEG, NSINOS, NBINS (C:1176-1189). */
+void mcpet_sinogram_sizes(mcpet_config* cfg);
+
+/* ------------------------------------------------------------------ device path */
+
+typedef struct mcpet_ctx mcpet_ctx;
+
+/* Select the device, upload phantom and tables, allocate tallies. Replaces
+ * init_CUDA_device (MCGPU-PET.cu:1842-2146) and the zeroing at C:408-413.
+ * Applies the "negative radius -> centre the cylinder on the bbox" rule (C:316-322)
+ * and the 511 keV table-range check (C:344-351). */
+mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phantom* ph, const mcpet_tables* tab,
+                          mcpet_ctx** out);
+
+/* Launch the transport (asynchronous on the context stream). Replaces
+ * track_particles<<<blocks,threads>>> (MCGPU-PET.cu:452-487; kernel K:153-463). */
+mcpet_status mcpet_run(mcpet_ctx* ctx, const mcpet_run_params* params);
+
+/* Wait for the context stream; reports kernel faults (C:486-487). */
+mcpet_status mcpet_sync(mcpet_ctx* ctx);
+
+/* Copy tallies to caller buffers (NULL pointers are skipped). Replaces the D2H copies
+ * at MCGPU-PET.cu:503-535,646,652. Implies mcpet_sync. */
+mcpet_status mcpet_fetch(mcpet_ctx* ctx, mcpet_tallies* out);
+
+/* Statistics of the last run (implies mcpet_sync). */
+mcpet_status mcpet_get_stats(mcpet_ctx* ctx, mcpet_stats* out);
+
+/* Free device memory and the stream (MCGPU-PET.cu:654-669). */
+void mcpet_destroy(mcpet_ctx* ctx);
+
+/* Message of the last failing call on ctx (or of the last context-free call if ctx is NULL). */
+const char* mcpet_last_error(const mcpet_ctx* ctx);
+
+/* Effective detector after mcpet_create (centre/radius rule applied). */
+mcpet_status mcpet_get_detector(const mcpet_ctx* ctx, mcpet_detector* out);
+
+/* ------------------------------------------------------------------ host-side output (boundary) */
+
+/* Write the reference's output files into directory dir (NULL = cwd):
+ * sinogram_Trues/Scatter.raw.gz and image_Trues/Scatter.raw.gz (C:544-562; BG_ names in
+ * BG mode, C:580-594), Energy_Sinogram_Spectrum.dat (C:612-616), the PSF text and .raw
+ * (report_PSF, C:3193-3322) and the voxel dose files (report_voxels_dose, C:2389-2650).
+ * Which files are written follows tally_psf_sinogram and the dose flags, as in main. */
+mcpet_status mcpet_write_outputs(const char* dir, const mcpet_config* cfg, const mcpet_phantom* ph,
+                                 const mcpet_tables* tab, const mcpet_tallies* t,
+                                 double time_elapsed_s, char* err, size_t errlen);
+
+/* RANECU seeds (x, y) at the start of stream ids[i] for input seed `seed`, computed on the
+ * host with the same jump-table skip-ahead as the kernels; equal to init_PRNG(ids[i], 555555,
+ * seed) (K:924-978). Lets a caller resume or reproduce any decay stream. */
+mcpet_status mcpet_stream_seeds(int32_t seed, const int64_t* ids, int32_t n, int32_t* sx, int32_t* sy);
+
+/* ------------------------------------------------------------------ self-test entry points */
+
+/* Draw n RANECU floats for each of nstreams streams on the device, stream i initialised as
+ * init_PRNG(stream_ids[i], 555555, seed) (K:924-978, K:1044-1061); out is [nstreams][n].
+ * If as_double, draws ranecu_double (K:1067-1083) into out_d instead. */
+mcpet_status mcpet_selftest_ranecu(int32_t seed, const int64_t* stream_ids, int32_t nstreams,
+                                   int32_t n, float* out, double* out_d, int32_t as_double);
+
+/* Evaluate the device elementary functions (log, exp, sincos, atan2, acos, logf, expf,
+ * powf, sincosf, atan2f, acosf) on n inputs; used to check host/device bit equality. */
+mcpet_status mcpet_selftest_fmath(const double* x, const double* y, int32_t n, double* out_d /* 7*n */,
+                                  float* out_f /* 7*n */);
+
+/* Device copy of nbytes (for HBM-counter calibration: FETCH_SIZE/WRITE_SIZE of a known copy). */
+mcpet_status mcpet_selftest_copy(size_t nbytes, int32_t iters, double* ms_out);
+
+/* Library version string. */
+const char* mcpet_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MCPET_H */

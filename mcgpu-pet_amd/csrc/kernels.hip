@@ -1,0 +1,821 @@
+// kernels.hip — gfx950 photon-transport kernels.
+//
+// One lane runs one decay stream at a time: the reference's per-thread Poisson
+// process (MCGPU-PET_kernel.cu:229-292) with its own RANECU stream. The kernels
+// differ only in how lanes get streams:
+//   plain      : lane i of the grid owns stream i (one pass, no queue);
+//   persistent : a fixed grid of waves pulls stream indices from a device counter and
+//                refills a lane as soon as its stream ends (streams are ordered by
+//                decreasing voxel activity, so the long streams start first).
+// Per lane the sequence of random numbers and the floating-point operations are those
+// of the reference thread with the same id; integer tallies are order-independent, so
+// both kernels produce the same tallies bit for bit.
+//
+// Memory layout (MI355X-first):
+//   - voxels: one 8-byte word (density, material) per voxel; activity is only read
+//     once per stream from a compacted, activity-sorted list of emitting voxels;
+//   - cross sections: total (a,b) pairs in an 8-byte table read in the Woodcock loop,
+//     Compton/Rayleigh (a,b) in a separate 16-byte table read only at real events;
+//   - Compton shell table (5460 B), the energy spectrum and the per-material dose
+//     are kept in LDS and flushed once per workgroup;
+//   - emission-image counts are kept per stream in registers and flushed once per stream.
+#include <hip/hip_runtime.h>
+#include <climits>
+
+#include "fmath.h"
+#include "kparams.h"
+
+#define K_PI 3.14159265358979323846
+#define K_EPS 0.00015f
+#define K_INV_C (1.0 / 29979245800e-12)
+#define K_E511 510998.95f
+#define K_MAXMAT 15
+// Upper bound on the iterations of each rejection loop (GRAa, GCOa). Valid tables accept
+// within a few trials; the bound only guarantees that no lane can loop forever on
+// corrupt input. The CPU checker applies the same bound.
+#define K_MAX_TRIALS (1 << 20)
+
+namespace {
+
+struct Rng {
+  int x, y;
+};
+
+struct Counters {
+  unsigned c[KC_N];
+};
+
+__device__ __forceinline__ float rn(Rng& r, Counters& C)
+{
+  C.c[KC_RNDRAWS]++;
+  return mcr_ranecu(&r.x, &r.y);
+}
+__device__ __forceinline__ double rnd(Rng& r, Counters& C)
+{
+  C.c[KC_RNDRAWS]++;
+  return mcr_ranecu_double(&r.x, &r.y);
+}
+
+// __float2int_rd with NaN -> INT_MIN and saturation (matches the host checker).
+__device__ __forceinline__ int f2i_rd(float x)
+{
+  float f = floorf(x);
+  if (!(f >= -2147483648.0f)) return INT_MIN;
+  if (f >= 2147483648.0f) return INT_MAX;
+  return (int)f;
+}
+__device__ __forceinline__ unsigned long long d2ull_rn(double x)
+{
+  double r = rint(x);
+  if (!(r > 0.0)) return 0ull;
+  if (r >= 18446744073709551616.0) return ~0ull;
+  return (unsigned long long)r;
+}
+__device__ __forceinline__ unsigned long long f2ull_rn(float x)
+{
+  float r = rintf(x);
+  if (!(r > 0.0f)) return 0ull;
+  if (r >= 18446744073709551616.0f) return ~0ull;
+  return (unsigned long long)r;
+}
+
+// locate_voxel (K:1097-1124). NaN positions count as outside and indices are clamped
+// to the grid (the reference would read out of bounds in both cases).
+__device__ __forceinline__ int locate(const KParams& P, const float* p)
+{
+  if ((p[1] < K_EPS) || (p[1] > (P.by - K_EPS)) || (p[0] < K_EPS) || (p[0] > (P.bx - K_EPS)) || (p[2] < K_EPS) ||
+      (p[2] > (P.bz - K_EPS)))
+    return -100;
+  if (!(p[0] == p[0] && p[1] == p[1] && p[2] == p[2])) return -100;
+  int vx = min(f2i_rd(p[0] * P.ivx), P.nx - 1);
+  int vy = min(f2i_rd(p[1] * P.ivy), P.ny - 1);
+  int vz = min(f2i_rd(p[2] * P.ivz), P.nz - 1);
+  return (vx + vy * (P.nx) + vz * (P.nx) * (P.ny));
+}
+
+// rotate_double (K:1159-1200): PENELOPE DIRECT in double precision on a float vector.
+__device__ __forceinline__ void rotate_double(float* d, double costh, double phi)
+{
+  double DXY, NORM, cosphi, sinphi, SDT;
+  DXY = (double)(d[0] * d[0] + d[1] * d[1]);
+  mcf_sincos(phi, &sinphi, &cosphi);
+  NORM = DXY + (double)(d[2] * d[2]);
+  if (fabs(NORM - 1.0) > 1.0e-14) {
+    NORM = 1.0 / sqrt(NORM);
+    d[0] = (float)(NORM * d[0]);
+    d[1] = (float)(NORM * d[1]);
+    d[2] = (float)(NORM * d[2]);
+    DXY = (double)(d[0] * d[0] + d[1] * d[1]);
+  }
+  if (DXY > 1.0e-28) {
+    SDT = sqrt((1.0 - costh * costh) / DXY);
+    float xin = d[0];
+    float yz = d[1] * d[2];
+    float xz = xin * d[2];
+    d[0] = (float)(d[0] * costh + SDT * ((double)xz * cosphi - d[1] * sinphi));
+    d[1] = (float)(d[1] * costh + SDT * ((double)yz * cosphi + xin * sinphi));
+    d[2] = (float)(d[2] * costh - DXY * SDT * cosphi);
+  } else {
+    SDT = sqrt(1.0 - costh * costh);
+    d[1] = (float)(SDT * sinphi);
+    if (d[2] > 0.0) {
+      d[0] = (float)(SDT * cosphi);
+      d[2] = (float)costh;
+    } else {
+      d[0] = (float)(-SDT * cosphi);
+      d[2] = (float)(-costh);
+    }
+  }
+}
+
+// GRAa (K:1230-1296): Rayleigh angle by RITA inverse cumulative + Thomson rejection.
+__device__ void graa(float energy, double* costh, int mat, float pmax_current, Rng& r, Counters& C,
+                     const mcpet_rayleigh_table* __restrict__ cgra)
+{
+  double xmax = ((double)energy) * 8.065535669099010e-5;
+  double x2lim = (double)cgra->xco[(mat + 1) * MCPET_NP_RAYLEIGH - 1];
+  double x2max = (xmax * xmax) < x2lim ? (xmax * xmax) : x2lim;
+  if (xmax < 0.01) {
+    int trials = 0;
+    do {
+      *costh = 1.0 - rnd(r, C) * 2.0;
+    } while (rnd(r, C) > (((*costh) * (*costh) + 1.0) * 0.5) && ++trials < K_MAX_TRIALS);
+    return;
+  }
+  *costh = 1.0;
+  for (int trial = 0; trial < K_MAX_TRIALS; trial++) {
+    double ru = rnd(r, C) * (double)pmax_current;
+    int itn = (int)(ru * (MCPET_NP_RAYLEIGH - 1));
+    itn = min(max(itn, 0), MCPET_NP_RAYLEIGH - 1);
+    int i__ = (int)cgra->itlco[itn + mat * MCPET_NP_RAYLEIGH];
+    int j = (int)cgra->ituco[itn + mat * MCPET_NP_RAYLEIGH];
+    if ((j - i__) > 1) {
+      do {
+        int k = (i__ + j) >> 1;
+        if (ru > cgra->pco[k - 1 + mat * MCPET_NP_RAYLEIGH]) i__ = k;
+        else j = k;
+      } while ((j - i__) > 1);
+    }
+    int index = i__ - 1 + mat * MCPET_NP_RAYLEIGH;
+    double rr = ru - cgra->pco[index];
+    double xx;
+    if (rr > 1e-16) {
+      double d__ = (double)(cgra->pco[index + 1] - cgra->pco[index]);
+      float aco_index = cgra->aco[index], bco_index = cgra->bco[index], xco_index = cgra->xco[index];
+      xx = (double)xco_index + (double)(aco_index + 1.0f + bco_index) * d__ * rr /
+                                   (d__ * d__ + (aco_index * d__ + bco_index * rr) * rr) *
+                                   (double)(cgra->xco[index + 1] - xco_index);
+    } else {
+      xx = cgra->xco[index];
+    }
+    if (xx < x2max) {
+      *costh = 1.0 - 2.0 * xx / x2max;
+      if (rnd(r, C) < (((*costh) * (*costh) + 1.0) * 0.5)) break;
+    }
+  }
+}
+
+// Compton profile term of shell i at momentum transfer factor `cdtf` (the inner body of
+// the incoherent-scattering-function loops, K:1407-1449); 0 for inaccessible shells.
+__device__ __forceinline__ float shell_term(const mcpet_compton_table& cg, int mat, int i, float energy, float cdtf, bool use_cdt)
+{
+  float temp = cg.uico[mat + i * K_MAXMAT];
+  if (!(temp < energy)) return 0.0f;
+  float pzomc;
+  if (!use_cdt) {
+    float aux = energy * (energy - temp) * 2.f;
+    pzomc = cg.fj0[mat + i * K_MAXMAT] * (aux - temp * 510998.918f) / (sqrtf(aux + aux + temp * temp) * 510998.918f);
+    if (pzomc > 0.0f)
+      temp = (0.707106781186545f + pzomc * 1.4142135623731f) * (0.707106781186545f + pzomc * 1.4142135623731f);
+    else
+      temp = (0.707106781186545f - pzomc * 1.4142135623731f) * (0.707106781186545f - pzomc * 1.4142135623731f);
+    temp = 0.5f * mcf_expf(0.5f - temp);
+    if (pzomc > 0.0f) temp = 1.0f - temp;
+    return temp;
+  }
+  float aux = (energy) * (energy - temp) * cdtf;
+  if ((aux > 1.0e-12f) || (temp > 1.0e-12f))
+    pzomc = cg.fj0[mat + i * K_MAXMAT] * (aux - temp * 510998.918f) / (sqrtf(aux + aux + temp * temp) * 510998.918f);
+  else
+    pzomc = 0.002f;
+  temp = pzomc * 1.4142135623731f;
+  if (pzomc > 0.0f)
+    temp = 0.5f - (temp + 0.70710678118654502f) * (temp + 0.70710678118654502f);
+  else
+    temp = 0.5f - (0.70710678118654502f - temp) * (0.70710678118654502f - temp);
+  temp = 0.5f * mcf_expf(temp);
+  if (pzomc > 0.0f) temp = 1.0f - temp;
+  return temp;
+}
+
+// GCOa (K:1334-1563): Compton scattering, relativistic impulse approximation with
+// analytical one-electron profiles. The per-shell values rn[i] of the last trial are
+// recomputed when the shell is selected instead of being kept in a 30-entry array
+// (same values, no scratch memory).
+__device__ void gcoa(float* energy, double* costh, int mat, Rng& r, Counters& C, const mcpet_compton_table& cg)
+{
+  float s, a1, s0, af, ek, ek2, ek3, tau, pzomc, taumin;
+  double cdt1;
+  const int my_noscco = cg.noscco[mat];
+  const float E = *energy;
+
+  ek = E * 1.956951306108245e-6f;
+  ek2 = ek * 2.f + 1.f;
+  ek3 = ek * ek;
+  taumin = 1.f / ek2;
+  a1 = mcf_logf(ek2);
+
+  s0 = 0.0f;
+  for (int i = 0; i < my_noscco; i++) {
+    if (cg.uico[mat + i * K_MAXMAT] < E) s0 += cg.fco[mat + i * K_MAXMAT] * shell_term(cg, mat, i, E, 0.0f, false);
+  }
+  float cdtf;
+  int trials = 0;
+  do {
+    if (rn(r, C) * (a1 + 2. * ek * (ek + 1.f) * taumin * taumin) < a1)
+      tau = mcf_powf(taumin, rn(r, C));
+    else
+      tau = sqrtf(1.f + rn(r, C) * (taumin * taumin - 1.f));
+    cdt1 = (double)(1.f - tau) / (((double)tau) * ((double)E) * 1.956951306108245e-6);
+    if (cdt1 > 2.0) cdt1 = 1.99999999;
+    cdtf = (float)cdt1;
+    s = 0.0f;
+    C.c[KC_SHELLEVAL] += (unsigned)my_noscco;
+    for (int i = 0; i < my_noscco; i++) {
+      if (cg.uico[mat + i * K_MAXMAT] < E) s += cg.fco[mat + i * K_MAXMAT] * shell_term(cg, mat, i, E, cdtf, true);
+    }
+  } while ((rn(r, C) * s0) > (s * (1.0f + tau * ((ek3 - ek2 - 1.0f) + tau * (ek2 + tau * ek3))) / (ek3 * tau * (tau * tau + 1.0f))) &&
+           ++trials < K_MAX_TRIALS);
+
+  *costh = 1.0 - cdt1;
+
+  pzomc = 0.0f;
+  for (int trial = 0; trial < K_MAX_TRIALS; trial++) {
+    float temp = s * rn(r, C);
+    float pac = 0.0f;
+    int ishell = my_noscco - 1;
+    for (int i = 0; i < (my_noscco - 1); i++) {
+      pac += cg.fco[mat + i * K_MAXMAT] * shell_term(cg, mat, i, E, cdtf, true);
+      if (pac > temp) {
+        ishell = i;
+        break;
+      }
+    }
+    temp = rn(r, C) * shell_term(cg, mat, ishell, E, cdtf, true);
+    const float fj = cg.fj0[mat + ishell * K_MAXMAT];
+    if (temp < 0.5f)
+      pzomc = (0.70710678118654502f - sqrtf(0.5f - mcf_logf(temp + temp))) / (fj * 1.4142135623731f);
+    else
+      pzomc = (sqrtf(0.5f - mcf_logf(2.0f - 2.0f * temp)) - 0.70710678118654502f) / (fj * 1.4142135623731f);
+    if (pzomc < -1.0f) continue;
+    temp = (float)(tau * (tau - (*costh) * 2.f) + 1.f);
+    if (temp > 1.0e-20f)
+      af = sqrtf(temp) * (tau * (tau - ((float)(*costh))) / temp + 1.f);
+    else
+      af = 0.00200f;
+    if (af > 0.0f) temp = af * 0.2f + 1.f;
+    else temp = 1.f - af * 0.2f;
+    float pzc = pzomc < 0.2f ? pzomc : 0.2f;
+    pzc = pzc > -0.2f ? pzc : -0.2f;
+    if (rn(r, C) * temp < (af * pzc + 1.f)) break;
+  }
+  {
+    float t, b1, b2, temp;
+    t = pzomc * pzomc;
+    b1 = 1.f - t * tau * tau;
+    b2 = 1.f - t * tau * ((float)(*costh));
+    temp = sqrtf(fabsf(b2 * b2 - b1 * (1.0f - t)));
+    if (pzomc < 0.0f) temp *= -1.0f;
+    t = (tau / b1) * (b2 + temp);
+    if (t > 1.0f) t = 1.0f;
+    (*energy) *= t;
+  }
+}
+
+struct Stream {
+  Rng r;
+  unsigned long long remaining;  // acquisition time left (ps)
+  double inv_act;                // tpb / activity, decayed per emission
+  unsigned long long hist;
+  long long tid;                 // reference thread id
+  int absvox, ix, iy, iz;
+  int img_t, img_sc;
+};
+
+// Cylinder intersection of both photons (K:1645-1707 / K:536-577).
+__device__ __forceinline__ float cyl_dist(const KParams& P, float x0, float y0, const float* dir)
+{
+  float A = dir[0] * dir[0] + dir[1] * dir[1];
+  float B = 2.0f * (x0 * dir[0] + y0 * dir[1]);
+  float Cc = x0 * x0 + y0 * y0 - P.radius * P.radius;
+  if (A == 0.0f) return 99999999.9f;
+  return (-B + sqrtf(B * B - 4.0f * A * Cc)) / (2.0f * A);
+}
+
+// Michelogram bin (K:1749-1788); -1 if rejected. ith >= nangles is rejected as well
+// (only possible when NANGLES < NCRYSTALS/2, where the reference writes out of bounds).
+__device__ int sino_bin(const KParams& P, float phi, float zn, float phi1, float zfn)
+{
+  int ix1 = f2i_rd((float)(((double)phi / (2.0 * K_PI) + 0.5) * (double)P.ncrystals));
+  int ix2 = f2i_rd((float)(((double)phi1 / (2.0 * K_PI) + 0.5) * (double)P.ncrystals));
+  int iz1 = f2i_rd((float)(((double)(zn / P.fovz) + 0.5) * (double)P.nrows));
+  int iz2 = f2i_rd((float)(((double)(zfn / P.fovz) + 0.5) * (double)P.nrows));
+  if (iz1 < 0 || iz2 < 0 || iz1 >= P.nrows || iz2 >= P.nrows) return -1;
+  if (abs(iz2 - iz1) > P.mrd) return -1;
+  float delta = (float)abs(iz2 - iz1);
+  int incl = f2i_rd((delta - 1 + ((P.span + 1) / 2)) / (float)P.span);
+  int iseg = 2 * incl;
+  if (iz2 < iz1 && iseg > 0) iseg = iseg - 1;
+  int ofseg = iseg * P.nzs;
+  for (int kka = 2; kka <= iseg; kka++) {
+    ofseg -= (P.span + 1);
+    if (kka > 3) ofseg -= f2i_rd((float)((kka - 2) * 0.5)) * 2 * P.span;
+  }
+  int ofz = 0;
+  if (incl > 0) ofz = ((P.span + 1) / 2) + (incl - 1) * P.span;
+  int izm = ofseg + (iz1 + iz2) - ofz;
+  if (ix2 < ix1) {
+    int t = ix2;
+    ix2 = ix1;
+    ix1 = t;
+  }
+  int ith = ((ix1 + ix2 + P.nangles + 1) % P.ncrystals) / 2;
+  int ir = abs(ix2 - ix1 - P.nangles);
+  if ((ix1 < ith) || (ix2 >= (ith + P.nangles))) ir = -ir;
+  ir = ir + P.nrad / 2;
+  if (izm < 0 || izm >= P.nsinos || ir < 0 || ir >= P.nrad || ith < 0 || ith >= P.nangles) return -1;
+  return izm * P.nangles * P.nrad + ith * P.nrad + ir;
+}
+
+__device__ __forceinline__ void spectrum_add(const KParams& P, int* s_spec, int iE)
+{
+  if (P.lds_spectrum) atomicAdd(&s_spec[iE], 1);
+  else atomicAdd(&P.spectrum[iE], 1);
+}
+
+// tally_Sinogram (K:1623-1836).
+__device__ void tally_sinogram(const KParams& P, float energy, float energyFirst, const float* pos, const float* posf,
+                               const float* dir, const float* dirf, int ss, int ssf, Stream& st, Counters& C, int* s_spec)
+{
+  float x0 = pos[0] - P.cx, y0 = pos[1] - P.cy;
+  float dist = cyl_dist(P, x0, y0, dir);
+  float pxn = x0 + dist * dir[0];
+  float pyn = y0 + dist * dir[1];
+  float pzn = (pos[2] - P.cz) + dist * dir[2];
+  float phi = mcf_atan2f(pyn, pxn);
+  if (!(fabsf(pzn) < 0.5f * P.height)) return;
+  x0 = posf[0] - P.cx;
+  y0 = posf[1] - P.cy;
+  dist = cyl_dist(P, x0, y0, dirf);
+  float pfxn = x0 + dist * dirf[0];
+  float pfyn = y0 + dist * dirf[1];
+  float pfzn = (posf[2] - P.cz) + dist * dirf[2];
+  float phi1 = mcf_atan2f(pfyn, pfxn);
+  if (!(fabsf(pfzn) < 0.5f * P.height)) return;
+  int ibin = sino_bin(P, phi, pzn, phi1, pfzn);
+  if (ibin < 0) return;
+  float randno1 = rn(st.r, C);
+  float randno2 = rn(st.r, C);
+  float g1 = sqrtf((float)(-2.0 * (double)mcf_logf(randno1 + 1e-8f))) * mcf_cosf((float)(2.0f * K_PI * (double)randno2));
+  float randno3 = rn(st.r, C);
+  float randno4 = rn(st.r, C);
+  float g2 = sqrtf((float)(-2.0 * (double)mcf_logf(randno3 + 1.0e-8f))) * mcf_cosf((float)(2.0f * K_PI * (double)randno4));
+  float Energia1 = energy + ((P.e_resol) / 2.35f) * energy * g1;
+  float Energia2 = energyFirst + ((P.e_resol) / 2.35f) * energyFirst * g2;
+  int iE1 = f2i_rd(Energia1 / 1000.0f);
+  int iE2 = f2i_rd(Energia2 / 1000.0f);
+  if ((iE1 < P.ne) && (iE1 >= 0)) spectrum_add(P, s_spec, iE1);
+  if ((iE2 < P.ne) && (iE2 >= 0)) spectrum_add(P, s_spec, iE2);
+  C.c[KC_PAIRS]++;
+  if ((Energia1) > (P.e_low) && (Energia1) < (P.e_high) && (Energia2) > (P.e_low) && (Energia2) < (P.e_high)) {
+    if (ss == 0 && ssf == 0) {
+      atomicAdd(&P.trues[ibin], 1);
+      st.img_t++;
+      C.c[KC_COINC]++;
+    }
+    if (ss > 0 || ssf > 0) {
+      atomicAdd(&P.scatter[ibin], 1);
+      st.img_sc++;
+      C.c[KC_COINC]++;
+    }
+  }
+}
+
+__device__ __forceinline__ void write_psf(mcpet_psf_record* rec, unsigned long long t_ps, float travel_ps, int vox, float e,
+                                          float z, float phi, const float* v, int idx1)
+{
+  rec->emission_time_ps = t_ps;
+  rec->travel_time_ps = travel_ps;
+  rec->emission_absvox = vox;
+  rec->energy = e;
+  rec->z = z;
+  rec->phi = phi;
+  rec->vx = v[0];
+  rec->vy = v[1];
+  rec->vz = v[2];
+  rec->index1 = (int16_t)idx1;
+  rec->index2 = 0;
+}
+
+// tally_PSF_coincidences (K:530-644).
+__device__ void tally_psf(const KParams& P, float energy, float energyFirst, const float* pos_in, const float* posf_in,
+                          const float* dir, const float* dirf, int ss, int ssf, float travel, float travelFirst, Stream& st,
+                          Counters& C)
+{
+  float position[3], positionFirst[3];
+  float x0 = pos_in[0] - P.cx, y0 = pos_in[1] - P.cy;
+  float dist = cyl_dist(P, x0, y0, dir);
+  travel += dist;
+  position[0] = x0 + dist * dir[0];
+  position[1] = y0 + dist * dir[1];
+  position[2] = (pos_in[2] - P.cz) + dist * dir[2];
+  float phi = mcf_atan2f(position[1], position[0]);
+  if (!(fabsf(position[2]) < 0.5f * P.height)) return;
+  x0 = posf_in[0] - P.cx;
+  y0 = posf_in[1] - P.cy;
+  dist = cyl_dist(P, x0, y0, dirf);
+  travelFirst += dist;
+  positionFirst[0] = x0 + dist * dirf[0];
+  positionFirst[1] = y0 + dist * dirf[1];
+  positionFirst[2] = (posf_in[2] - P.cz) + dist * dirf[2];
+  float phi1 = mcf_atan2f(positionFirst[1], positionFirst[0]);
+  float randno1 = rn(st.r, C);
+  float randno2 = rn(st.r, C);
+  float g1 = sqrtf(-2.0f * mcf_logf(randno1 + 1e-8f)) * mcf_cosf((float)(2.0f * K_PI * (double)randno2));
+  float randno3 = rn(st.r, C);
+  float randno4 = rn(st.r, C);
+  float g2 = sqrtf(-2.0f * mcf_logf(randno3 + 1.0e-8f)) * mcf_cosf((float)(2.0f * K_PI * (double)randno4));
+  float Energia1 = energy + ((P.e_resol) * (1.0f / 2.35f)) * energy * g1;
+  float Energia2 = energyFirst + ((P.e_resol) * (1.0f / 2.35f)) * energyFirst * g2;
+  if (fabsf(positionFirst[2]) < 0.5f * P.height && (Energia1 > P.e_low) && (Energia1 < P.e_high) && (Energia2 > P.e_low) &&
+      (Energia2 < P.e_high)) {
+    int index = atomicAdd(P.psf_index, 2);
+    if (index > 2000000000) atomicMin(P.psf_index, 2000000000);
+    if (index < P.psf_size && P.psf) {
+      int i0, i1;
+      if (travel < travelFirst) {
+        i0 = index;
+        i1 = index + 1;
+      } else {
+        i0 = index + 1;
+        i1 = index;
+      }
+      unsigned long long t_ps = P.acq_ps - st.remaining;
+      if (i0 < P.psf_size)
+        write_psf(P.psf + i0, t_ps, (float)(travel * K_INV_C), st.absvox, Energia1, position[2], phi, dir, ss);
+      if (i1 < P.psf_size)
+        write_psf(P.psf + i1, t_ps, (float)(travelFirst * K_INV_C), st.absvox, Energia2, positionFirst[2], phi1, dirf, ssf);
+    }
+  }
+}
+
+__device__ __forceinline__ void tally_dose(const KParams& P, float Edep, int mat, const float* pos, unsigned long long* s_mat)
+{
+  if (P.material_dose) {
+    atomicAdd(&s_mat[2 * mat], f2ull_rn(Edep * 1.0f));
+    atomicAdd(&s_mat[2 * mat + 1], f2ull_rn(Edep * Edep));
+  }
+  if (P.voxel_dose) {
+    short vx = (short)f2i_rd(pos[0] * P.ivx);
+    short vy = (short)f2i_rd(pos[1] * P.ivy);
+    short vz = (short)f2i_rd(pos[2] * P.ivz);
+    if ((vx < P.roi[0]) || (vx > P.roi[1]) || (vy < P.roi[2]) || (vy > P.roi[3]) || (vz < P.roi[4]) || (vz > P.roi[5])) return;
+    int DX = 1 + (int)(P.roi[1] - P.roi[0]);
+    int num = (int)(vx - P.roi[0]) + ((int)(vy - P.roi[2])) * DX + ((int)(vz - P.roi[4])) * DX * (1 + (int)(P.roi[3] - P.roi[2]));
+    atomicAdd(&P.voxel_edep[2 * (size_t)num], f2ull_rn(Edep * 1.0f));
+    atomicAdd(&P.voxel_edep[2 * (size_t)num + 1], f2ull_rn(Edep * Edep));
+  }
+}
+
+// Photon transport until escape or absorption (K:297-415). Returns the final energy
+// table index (< 0 = absorbed) and updates position/direction/scatter state/travel.
+__device__ int transport_photon(const KParams& P, float& energy, float* position, float* direction, int& scatter_state,
+                                float& travel_distance, Stream& st, Counters& C, const mcpet_compton_table& cg,
+                                unsigned long long* s_mat)
+{
+  int index = f2i_rd((energy - P.e0) * P.ide);
+  float2 w = P.woodcock[index];
+  float mfp_Woodcock = w.x + energy * w.y;
+  int material_old = -1, material0 = 0, absvox;
+  float prob = 0.0f, randno = 0.0f, mfp_density = 0.0f;
+  float2 tab = make_float2(0.0f, 0.0f);
+  for (;;) {
+    absvox = locate(P, position);
+    if (absvox < 0) break;
+    do {
+      float step = -(mfp_Woodcock)*mcf_logf(rn(st.r, C));
+      C.c[KC_WOODCOCK]++;
+      travel_distance += step;
+      position[0] += step * direction[0];
+      position[1] += step * direction[1];
+      position[2] += step * direction[2];
+      absvox = locate(P, position);
+      if (absvox < 0) break;
+      float2 md = P.vox[absvox];
+      material0 = __float_as_int(md.y);
+      if (material0 != material_old) {
+        tab = P.tot[index * K_MAXMAT + material0];
+        material_old = material0;
+        C.c[KC_MATCHANGE]++;
+      }
+      mfp_density = mfp_Woodcock * md.x;
+      prob = 1.0f - mfp_density * (tab.x + energy * tab.y);
+      randno = rn(st.r, C);
+    } while (randno < prob);
+    if (absvox < 0) break;
+    const float4 cr = P.cr[index * K_MAXMAT + material0];
+    prob += mfp_density * (cr.x + energy * cr.y);
+    if (randno < prob) {
+      double costh_Compton;
+      randno = energy;
+      C.c[KC_COMPTON]++;
+      gcoa(&energy, &costh_Compton, material0, st.r, C, cg);
+      rotate_double(direction, costh_Compton, 6.28318530717958647693 * rnd(st.r, C));
+      randno = energy - randno;
+      index = f2i_rd((energy - P.e0) * P.ide);
+      if (index > -1) {
+        w = P.woodcock[index];
+        mfp_Woodcock = w.x + energy * w.y;
+        material_old = -2;
+        scatter_state = (scatter_state == 0) ? 1 : 3;
+      }
+    } else {
+      prob += mfp_density * (cr.z + energy * cr.w);
+      if (randno < prob) {
+        double costh_Rayleigh;
+        C.c[KC_RAYLEIGH]++;
+        float pmax_current = P.ray->pmax[(index + 1) * K_MAXMAT + material0];
+        graa(energy, &costh_Rayleigh, material0, pmax_current, st.r, C, P.ray);
+        rotate_double(direction, costh_Rayleigh, 6.28318530717958647693 * rnd(st.r, C));
+        scatter_state = (scatter_state == 0) ? 2 : 3;
+      } else {
+        C.c[KC_PHOTO]++;
+        randno = -energy;
+        index = -11;
+      }
+    }
+    if (randno < -0.001f) tally_dose(P, -1.0f * randno, material0, position, s_mat);
+    if (index < 0) break;
+  }
+  return index;
+}
+
+// Pair emission at a uniform point of the stream's voxel (K:665-773).
+__device__ __forceinline__ void source_first(const KParams& P, Stream& st, Counters& C, float* pos, float* dir)
+{
+  dir[2] = 1.0f - 2.0f * rn(st.r, C);
+  float phi_sampled = 6.28318530716f * rn(st.r, C);
+  float sin_theta = sqrtf(1.0f - dir[2] * dir[2]);
+  float sp, cp;
+  mcf_sincosf(phi_sampled, &sp, &cp);
+  dir[1] = sin_theta * sp;
+  dir[0] = sin_theta * cp;
+  float r1 = rn(st.r, C);
+  float r2 = rn(st.r, C);
+  float r3 = rn(st.r, C);
+  r1 = (K_EPS > r1 - K_EPS) ? K_EPS : r1 - K_EPS;
+  r2 = (K_EPS > r2 - K_EPS) ? K_EPS : r2 - K_EPS;
+  r3 = (K_EPS > r3 - K_EPS) ? K_EPS : r3 - K_EPS;
+  pos[0] = ((float)st.ix + r1) / P.ivx;
+  pos[1] = ((float)st.iy + r2) / P.ivy;
+  pos[2] = ((float)st.iz + r3) / P.ivz;
+}
+
+__device__ __forceinline__ void source_second(const KParams& P, Stream& st, Counters& C, const float* dir1, float* dir)
+{
+  if (P.bg_mode) {
+    dir[2] = 1.0f - 2.0f * rn(st.r, C);
+    float phi_sampled = 6.28318530716f * rn(st.r, C);
+    float sin_theta = sqrtf(1.0f - dir[2] * dir[2]);
+    float sp, cp;
+    mcf_sincosf(phi_sampled, &sp, &cp);
+    dir[1] = sin_theta * sp;
+    dir[0] = sin_theta * cp;
+    return;
+  }
+  float phi_sampled2 = (float)((double)mcf_atan2f(dir1[1], dir1[0]) + K_PI);
+  float theta_sampled2 = (float)(K_PI - mcf_acos((double)dir1[2]));
+  float V1 = rn(st.r, C);
+  float V2 = rn(st.r, C);
+  float THETA_NC = (float)((2.0 * 0.21276 * K_PI / 180.0) * (double)sqrtf(-2.0f * mcf_logf(V2 + 1e-8f)) *
+                           (double)mcf_cosf((float)((2.0 * K_PI) * (double)V1)));
+  float PHI_NC = (float)((2.0 * K_PI) * (double)rn(st.r, C));
+  float ST, CT, CP, SP, ST1, CT1, CP1, SP1;
+  mcf_sincosf(THETA_NC, &ST, &CT);
+  mcf_sincosf(PHI_NC, &SP, &CP);
+  mcf_sincosf(theta_sampled2, &ST1, &CT1);
+  mcf_sincosf(phi_sampled2, &SP1, &CP1);
+  float GX = CP * ST;
+  float GY = SP * ST;
+  float GZ = CT;
+  dir[0] = GX * CT1 * CP1 - GY * SP1 + GZ * ST1 * CP1;
+  dir[1] = GX * CT1 * SP1 + GY * CP1 + GZ * ST1 * SP1;
+  dir[2] = -GX * ST1 + GZ * CT1;
+}
+
+// One annihilation of the stream (reference main loop body, K:259-449, both photons).
+// Returns false when the stream's acquisition time is exhausted.
+__device__ bool decay(const KParams& P, Stream& st, Counters& C, const mcpet_compton_table& cg, int* s_spec,
+                      unsigned long long* s_mat)
+{
+  {
+    double Dt = -mcf_log(rnd(st.r, C)) * st.inv_act;
+    unsigned long long Dt_ps = d2ull_rn(Dt * 1e12);
+    if (!(Dt_ps < st.remaining)) return false;
+    st.remaining -= Dt_ps;
+    st.hist += 2;
+    C.c[KC_DECAYS]++;
+    st.inv_act = st.inv_act * mcf_exp(Dt * P.inv_mean_life);
+  }
+  // photon 1
+  float pos1[3], dir1[3];
+  source_first(P, st, C, pos1, dir1);
+  float position[3] = {pos1[0], pos1[1], pos1[2]};
+  float direction[3] = {dir1[0], dir1[1], dir1[2]};
+  float energy = K_E511;
+  int ss = 0;
+  float travel = 0.0f;
+  int index = transport_photon(P, energy, position, direction, ss, travel, st, C, cg, s_mat);
+  const bool first_alive = index > -1;
+  const float energyFirst = first_alive ? energy : -10.0f;
+  const float posF[3] = {position[0], position[1], position[2]};
+  const float dirF[3] = {direction[0], direction[1], direction[2]};
+  const int ssF = ss;
+  const float travelF = travel;
+  // photon 2
+  source_second(P, st, C, dir1, direction);
+  position[0] = pos1[0];
+  position[1] = pos1[1];
+  position[2] = pos1[2];
+  energy = K_E511;
+  ss = 0;
+  travel = 0.0f;
+  index = transport_photon(P, energy, position, direction, ss, travel, st, C, cg, s_mat);
+  if (index > -1 && energyFirst > -0.5f) {
+    if (P.type == 0 || (P.type == 1 && ss == 0 && ssF == 0) || (P.type == 2 && (ss != 0 || ssF != 0))) {
+      if (P.mode == 0 || P.mode == 2) tally_sinogram(P, energy, energyFirst, position, posF, direction, dirF, ss, ssF, st, C, s_spec);
+      if (P.mode == 0 || P.mode == 1) tally_psf(P, energy, energyFirst, position, posF, direction, dirF, ss, ssF, travel, travelF, st, C);
+    }
+  }
+  return true;
+}
+
+__device__ __forceinline__ void stream_init(const KParams& P, long long s, Stream& st)
+{
+  const int k = (int)(s / P.tpb);
+  const int t = (int)(s - (long long)k * P.tpb);
+  st.absvox = P.active_vox[k];
+  const float act = P.active_act[k];
+  st.ix = st.absvox % P.nx;
+  st.iy = (st.absvox / P.nx) % P.ny;
+  st.iz = st.absvox / (P.nx * P.ny);
+  st.tid = (long long)st.absvox * P.tpb + t;
+  mcr_init_stream(st.tid, P.seed, P.j1, P.j2, &st.r.x, &st.r.y);
+  st.remaining = P.acq_ps;
+  st.inv_act = ((double)P.tpb / (double)act);
+  st.hist = 0;
+  st.img_t = 0;
+  st.img_sc = 0;
+}
+
+__device__ __forceinline__ void stream_finish(const KParams& P, Stream& st, unsigned long long& hist_acc)
+{
+  if (st.img_t) atomicAdd(&P.img_t[st.absvox], st.img_t);
+  if (st.img_sc) atomicAdd(&P.img_sc[st.absvox], st.img_sc);
+  hist_acc += st.hist;
+  if (st.tid == P.last_thread_id) *P.final_seed = st.r.x;
+}
+
+template <bool PERSISTENT>
+__global__ __launch_bounds__(256) void k_transport(KParams P)
+{
+  __shared__ mcpet_compton_table s_cmp;
+  __shared__ unsigned long long s_mat[2 * K_MAXMAT];
+  extern __shared__ int s_spec[];
+  {
+    const int* src = reinterpret_cast<const int*>(P.cmp);
+    int* dst = reinterpret_cast<int*>(&s_cmp);
+    for (int i = threadIdx.x; i < (int)(sizeof(mcpet_compton_table) / 4); i += blockDim.x) dst[i] = src[i];
+    if (threadIdx.x < 2 * K_MAXMAT) s_mat[threadIdx.x] = 0ull;
+    if (P.lds_spectrum)
+      for (int i = threadIdx.x; i < P.ne; i += blockDim.x) s_spec[i] = 0;
+  }
+  __syncthreads();
+
+  Counters C;
+#pragma unroll
+  for (int i = 0; i < KC_N; i++) C.c[i] = 0;
+  unsigned long long hist_acc = 0;
+  Stream st;
+  bool have = false;
+  long long sid = PERSISTENT ? -1 : (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  bool used = false;
+  for (;;) {
+    if (!have) {
+      if (PERSISTENT) {
+        sid = (long long)atomicAdd(P.queue, 1ull);
+      } else if (used) {
+        break;
+      }
+      used = true;
+      if (sid >= P.n_streams) break;
+      stream_init(P, sid, st);
+      C.c[KC_STREAMS]++;
+      have = true;
+    }
+    if (!decay(P, st, C, s_cmp, s_spec, s_mat)) {
+      stream_finish(P, st, hist_acc);
+      have = false;
+    }
+  }
+  // Flush per-lane counters (the atomic optimiser combines them per wave).
+  if (hist_acc) atomicAdd(P.total_hist, hist_acc);
+#pragma unroll
+  for (int i = 0; i < KC_N; i++)
+    if (C.c[i]) atomicAdd(&P.counters[i], (unsigned long long)C.c[i]);
+  __syncthreads();
+  if (P.lds_spectrum)
+    for (int i = threadIdx.x; i < P.ne; i += blockDim.x)
+      if (s_spec[i]) atomicAdd(&P.spectrum[i], s_spec[i]);
+  if (threadIdx.x < 2 * K_MAXMAT && s_mat[threadIdx.x]) atomicAdd(&P.mat_edep[threadIdx.x], s_mat[threadIdx.x]);
+}
+
+// ---------------------------------------------------------------------------------- self tests
+
+__global__ void k_selftest_ranecu(int seed, const long long* ids, int nstreams, int n, float* out, double* out_d, int as_double,
+                                  const uint32_t* j1, const uint32_t* j2)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nstreams) return;
+  int sx, sy;
+  mcr_init_stream(ids[i], seed, j1, j2, &sx, &sy);
+  for (int k = 0; k < n; k++) {
+    if (as_double) out_d[(size_t)i * n + k] = mcr_ranecu_double(&sx, &sy);
+    else out[(size_t)i * n + k] = mcr_ranecu(&sx, &sy);
+  }
+}
+
+__global__ void k_selftest_fmath(const double* x, const double* y, int n, double* od, float* of)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double a = x[i], b = y[i];
+  double s, c;
+  mcf_sincos(a, &s, &c);
+  od[0 * (size_t)n + i] = mcf_log(fabs(a) + 1e-300);
+  od[1 * (size_t)n + i] = mcf_exp(a);
+  od[2 * (size_t)n + i] = s;
+  od[3 * (size_t)n + i] = c;
+  od[4 * (size_t)n + i] = mcf_atan2(a, b);
+  od[5 * (size_t)n + i] = mcf_acos(fmin(fmax(b, -1.0), 1.0));
+  od[6 * (size_t)n + i] = sqrt(fabs(a));
+  float af = (float)a, bf = (float)b, sf, cf;
+  mcf_sincosf(af, &sf, &cf);
+  of[0 * (size_t)n + i] = mcf_logf(fabsf(af) + 1e-30f);
+  of[1 * (size_t)n + i] = mcf_expf(af);
+  of[2 * (size_t)n + i] = mcf_powf(fminf(fabsf(bf), 1.0f), fabsf(af));
+  of[3 * (size_t)n + i] = sf;
+  of[4 * (size_t)n + i] = cf;
+  of[5 * (size_t)n + i] = mcf_atan2f(af, bf);
+  of[6 * (size_t)n + i] = sqrtf(fabsf(af)) / (bf + 3.0f);
+}
+
+__global__ void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16)
+{
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n16; i += stride) dst[i] = src[i];
+}
+
+}  // namespace
+
+hipError_t mcpet_launch_transport(const KParams& P, int kind, int blocks, int block_size, hipStream_t stream)
+{
+  size_t lds = P.lds_spectrum ? (size_t)P.ne * sizeof(int) : 0;
+  if (kind == MCPET_KERNEL_PERSISTENT)
+    hipLaunchKernelGGL(k_transport<true>, dim3(blocks), dim3(block_size), lds, stream, P);
+  else
+    hipLaunchKernelGGL(k_transport<false>, dim3(blocks), dim3(block_size), lds, stream, P);
+  return hipGetLastError();
+}
+
+hipError_t mcpet_launch_selftest_ranecu(int seed, const long long* d_ids, int nstreams, int n, float* d_out, double* d_out_d,
+                                        int as_double, const uint32_t* j1, const uint32_t* j2, hipStream_t stream)
+{
+  int blocks = (nstreams + 255) / 256;
+  hipLaunchKernelGGL(k_selftest_ranecu, dim3(blocks), dim3(256), 0, stream, seed, d_ids, nstreams, n, d_out, d_out_d, as_double,
+                     j1, j2);
+  return hipGetLastError();
+}
+
+hipError_t mcpet_launch_selftest_fmath(const double* d_x, const double* d_y, int n, double* d_od, float* d_of, hipStream_t stream)
+{
+  hipLaunchKernelGGL(k_selftest_fmath, dim3((n + 255) / 256), dim3(256), 0, stream, d_x, d_y, n, d_od, d_of);
+  return hipGetLastError();
+}
+
+hipError_t mcpet_launch_copy(const uint4* src, uint4* dst, size_t n16, hipStream_t stream)
+{
+  hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, stream, src, dst, n16);
+  return hipGetLastError();
+}

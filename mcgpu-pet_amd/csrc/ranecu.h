@@ -1,0 +1,107 @@
+/*
+ * ranecu.h — RANECU (L'Ecuyer 1988 combined MLCG) with sequence-splitting skip-ahead,
+ * as used by the reference transport (MCGPU-PET_kernel.cu:898-1083).
+ *
+ * Stream layout (reference K:251-253, K:931): stream i starts at the position
+ * (i+1) * 555555 * 256 of both MLCGs, both seeded with the same input seed.
+ * The multiplier a^((i+1)L) mod m is computed here from a host-side table of
+ * a^(L 2^k) mod m (k = 0..62), one modular product per set bit of i+1, instead of
+ * the reference's square-and-multiply over the full leap; the value is the same
+ * integer. The final product with the input seed uses the reference's own
+ * Russian-peasant/approximate-factoring routine so that any int seed maps exactly
+ * as in the reference.
+ */
+#ifndef MCPET_RANECU_H
+#define MCPET_RANECU_H
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define MCR_HD static inline __host__ __device__
+#else
+#define MCR_HD static inline
+#endif
+
+#define MCR_A1 40014
+#define MCR_M1 2147483563
+#define MCR_A2 40692
+#define MCR_M2 2147483399
+#define MCR_LEAP_DISTANCE 256
+#define MCR_HIST_PER_THREAD 555555
+#define MCR_JUMP_BITS 63
+
+/* (a*s) mod m for 0 < a, s < m by L'Ecuyer & Cote's Russian-peasant + approximate
+ * factoring (exact in 32-bit arithmetic; identical to the reference's abMODm for all int inputs). */
+MCR_HD int mcr_abmodm(int m, int a, int s)
+{
+  int q, k;
+  int p = -m;
+  while (a > 32768) {
+    if (0 != (a & 1)) {
+      p += s;
+      if (p > 0) p -= m;
+    }
+    a >>= 1;
+    s = (s - m) + s;
+    if (s < 0) s += m;
+  }
+  q = m / a;
+  k = s / q;
+  s = a * (s - k * q) - k * (m - q * a);
+  while (s < 0) s += m;
+  p += s;
+  if (p < 0) p += m;
+  return p;
+}
+
+MCR_HD uint32_t mcr_mulmod(uint32_t a, uint32_t b, uint32_t m) { return (uint32_t)(((uint64_t)a * b) % m); }
+
+/* Host: table[k] = a^(L*2^k) mod m, L = 555555*256. */
+static inline void mcr_build_jump_table(int a, int m, uint32_t* table)
+{
+  uint64_t L = (uint64_t)MCR_HIST_PER_THREAD * MCR_LEAP_DISTANCE;
+  /* a^L by square-and-multiply */
+  uint64_t base = (uint64_t)a, r = 1, e = L;
+  while (e) {
+    if (e & 1) r = (r * base) % (uint64_t)m;
+    base = (base * base) % (uint64_t)m;
+    e >>= 1;
+  }
+  table[0] = (uint32_t)r;
+  for (int k = 1; k < MCR_JUMP_BITS; k++) table[k] = mcr_mulmod(table[k - 1], table[k - 1], (uint32_t)m);
+}
+
+/* Seeds of stream `id` (reference thread id) from the jump tables. */
+MCR_HD void mcr_init_stream(int64_t id, int seed_input, const uint32_t* j1, const uint32_t* j2, int* sx, int* sy)
+{
+  uint64_t n = (uint64_t)(id + 1);
+  uint32_t y1 = 1, y2 = 1;
+  for (int k = 0; k < MCR_JUMP_BITS && n; k++, n >>= 1) {
+    if (n & 1ull) {
+      y1 = mcr_mulmod(y1, j1[k], (uint32_t)MCR_M1);
+      y2 = mcr_mulmod(y2, j2[k], (uint32_t)MCR_M2);
+    }
+  }
+  *sx = mcr_abmodm(MCR_M1, seed_input, (int)y1);
+  *sy = mcr_abmodm(MCR_M2, seed_input, (int)y2);
+}
+
+/* One step of both MLCGs (Schrage decomposition), combined output in [1, m1-1]. */
+MCR_HD int mcr_step(int* sx, int* sy)
+{
+  int i1 = *sx / 53668;
+  *sx = 40014 * (*sx - i1 * 53668) - i1 * 12211;
+  int i2 = *sy / 52774;
+  *sy = 40692 * (*sy - i2 * 52774) - i2 * 3791;
+  if (*sx < 0) *sx += 2147483563;
+  if (*sy < 0) *sy += 2147483399;
+  i2 = *sx - *sy;
+  if (i2 < 1) i2 += 2147483562;
+  return i2;
+}
+
+/* float_rn(i2) * 2^-31 (the reference constant 4.65661305739e-10f is exactly 2^-31 in binary32). */
+MCR_HD float mcr_ranecu(int* sx, int* sy) { return (float)mcr_step(sx, sy) * 4.65661305739e-10f; }
+MCR_HD double mcr_ranecu_double(int* sx, int* sy) { return (double)mcr_step(sx, sy) * 4.6566130573917692e-10; }
+
+#endif /* MCPET_RANECU_H */

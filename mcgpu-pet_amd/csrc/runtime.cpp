@@ -1,0 +1,588 @@
+// runtime.cpp — C ABI of the device path (mcpet_create / run / sync / fetch / destroy).
+//
+// Replaces the reference's init_CUDA_device (MCGPU-PET.cu:1842-2146), the single launch
+// of track_particles (C:452-487) and the result copies (C:499-535, 642-652). The
+// library owns all device memory; the caller's host buffers are only read or written
+// during the calls.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "host_internal.h"
+#include "kparams.h"
+#include "mcpet.h"
+
+struct mcpet_ctx {
+  int device = 0;
+  int num_cus = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_k0 = nullptr, ev_k1 = nullptr, ev_z0 = nullptr;
+  mcpet_config cfg{};
+  mcpet_detector det{};
+  int nx = 0, ny = 0, nz = 0;
+  long long nvox = 0;
+  long long n_active = 0;
+  long long n_roi = 0;
+  KParams P{};
+  // device allocations
+  float2* d_vox = nullptr;
+  int* d_active_vox = nullptr;
+  float* d_active_act = nullptr;
+  float2* d_woodcock = nullptr;
+  float2* d_tot = nullptr;
+  float4* d_cr = nullptr;
+  mcpet_rayleigh_table* d_ray = nullptr;
+  mcpet_compton_table* d_cmp = nullptr;
+  int* d_trues = nullptr;
+  int* d_scatter = nullptr;
+  int* d_img_t = nullptr;
+  int* d_img_sc = nullptr;
+  int* d_spectrum = nullptr;
+  mcpet_psf_record* d_psf = nullptr;
+  int* d_psf_index = nullptr;
+  unsigned long long* d_voxel_edep = nullptr;
+  unsigned long long* d_mat_edep = nullptr;
+  unsigned long long* d_counters = nullptr;  // KC_N counters + queue + total histories
+  int* d_final_seed = nullptr;
+  bool ran = false;
+  bool timed = false;
+  double zero_ms = 0.0;
+  std::string err;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+mcpet_status set_err(mcpet_ctx* c, mcpet_status st, const std::string& msg)
+{
+  if (c) c->err = msg;
+  g_err = msg;
+  mcpet_host_set_error(msg.c_str());
+  return st;
+}
+
+#define HIPCHK(ctx, call)                                                                              \
+  do {                                                                                                 \
+    hipError_t e_ = (call);                                                                            \
+    if (e_ != hipSuccess)                                                                              \
+      return set_err(ctx, MCPET_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));              \
+  } while (0)
+
+template <class T>
+mcpet_status dalloc(mcpet_ctx* c, T** p, size_t n)
+{
+  *p = nullptr;
+  if (n == 0) return MCPET_OK;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return set_err(c, MCPET_E_NOMEM, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed: " + hipGetErrorString(e));
+  }
+  return MCPET_OK;
+}
+
+void free_all(mcpet_ctx* c)
+{
+  void* ptrs[] = {c->d_vox, c->d_active_vox, c->d_active_act, c->d_woodcock, c->d_tot, c->d_cr, c->d_ray, c->d_cmp,
+                  c->d_trues, c->d_scatter, c->d_img_t, c->d_img_sc, c->d_spectrum, c->d_psf, c->d_psf_index,
+                  c->d_voxel_edep, c->d_mat_edep, c->d_counters, c->d_final_seed};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (c->ev_k0) (void)hipEventDestroy(c->ev_k0);
+  if (c->ev_k1) (void)hipEventDestroy(c->ev_k1);
+  if (c->ev_z0) (void)hipEventDestroy(c->ev_z0);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+}  // namespace
+
+extern "C" const char* mcpet_version(void) { return "mcpet-mi355x 0.1 (gfx950)"; }
+
+extern "C" const char* mcpet_last_error(const mcpet_ctx* ctx)
+{
+  if (ctx) return ctx->err.c_str();
+  const char* h = mcpet_host_last_error();
+  if (h && h[0]) return h;
+  return g_err.c_str();
+}
+
+extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phantom* ph, const mcpet_tables* tab, mcpet_ctx** out)
+{
+  if (!out) return set_err(nullptr, MCPET_E_ARG, "mcpet_create: null output pointer");
+  *out = nullptr;
+  if (!cfg || !ph || !tab || !ph->voxel_mat_dens || !tab->woodcock || !tab->mfp_a || !tab->mfp_b || !tab->rayleigh || !tab->compton)
+    return set_err(nullptr, MCPET_E_ARG, "mcpet_create: null argument");
+  if (cfg->threads_per_block < 1) return set_err(nullptr, MCPET_E_RANGE, "mcpet_create: threads_per_block must be >= 1");
+  if (cfg->nbins < 1 || cfg->ne < 0 || cfg->nrows < 1 || cfg->ncrystals < 1 || cfg->nangles < 1 || cfg->nrad < 1 || cfg->span < 1)
+    return set_err(nullptr, MCPET_E_RANGE, "mcpet_create: invalid sinogram parameters");
+  if (tab->num_values < 2 || !(tab->ide > 0.0f))
+    return set_err(nullptr, MCPET_E_RANGE, "mcpet_create: invalid interaction tables");
+  // C:344-351: the tables must reach the annihilation energy.
+  if (510998.95f > (tab->e0 + (tab->num_values - 1) / tab->ide) || !(tab->e0 < 510998.95f))
+    return set_err(nullptr, MCPET_E_RANGE, "mcpet_create: the material files do not have data at 510.999 keV");
+
+  {
+    // The primary's energy bin must lie inside the tables (index+1 is read for Rayleigh).
+    float fi = floorf((510998.95f - tab->e0) * tab->ide);
+    if (!(fi >= 0.0f && fi <= (float)(tab->num_values - 2)))
+      return set_err(nullptr, MCPET_E_RANGE, "mcpet_create: 511 keV outside the interaction tables");
+  }
+  mcpet_ctx* c = new mcpet_ctx();
+  c->cfg = *cfg;
+  c->nx = ph->nx;
+  c->ny = ph->ny;
+  c->nz = ph->nz;
+  c->nvox = (long long)ph->nx * ph->ny * ph->nz;
+
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    delete c;
+    return set_err(nullptr, MCPET_E_HIP, "mcpet_create: no HIP device available");
+  }
+  c->device = (cfg->gpu_id >= 0 && cfg->gpu_id < ndev) ? cfg->gpu_id : 0;   // C:1940-1952 fallback
+  mcpet_status st;
+#define CK(x)                  \
+  do {                         \
+    st = (x);                  \
+    if (st != MCPET_OK) {      \
+      std::string m = c->err;  \
+      free_all(c);             \
+      delete c;                \
+      set_err(nullptr, st, m); \
+      return st;               \
+    }                          \
+  } while (0)
+#define HK(x) CK(([&]() -> mcpet_status { HIPCHK(c, (x)); return MCPET_OK; })())
+
+  HK(hipSetDevice(c->device));
+  hipDeviceProp_t prop;
+  HK(hipGetDeviceProperties(&prop, c->device));
+  c->num_cus = prop.multiProcessorCount;
+  HK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HK(hipEventCreate(&c->ev_k0));
+  HK(hipEventCreate(&c->ev_k1));
+  HK(hipEventCreate(&c->ev_z0));
+
+  // Detector: negative radius -> centre on the bounding box (C:316-322).
+  c->det = cfg->detector;
+  if (c->det.radius < 0.0f) {
+    c->det.center[0] = ph->size_bbox[0] * 0.5f;
+    c->det.center[1] = ph->size_bbox[1] * 0.5f;
+    c->det.center[2] = ph->size_bbox[2] * 0.5f;
+    c->det.radius = -c->det.radius;
+  }
+
+  // Voxels -> packed (density, material index) words; emitting voxels sorted by activity.
+  std::vector<float2> hv((size_t)c->nvox);
+  std::vector<int> act_idx;
+  for (long long v = 0; v < c->nvox; v++) {
+    const float* p = ph->voxel_mat_dens + 3 * v;
+    int m0 = (int)(p[0] - 1);
+    if (m0 < 0 || m0 >= MCPET_MAX_MATERIALS) {
+      free_all(c);
+      delete c;
+      return set_err(nullptr, MCPET_E_RANGE, "mcpet_create: voxel material out of range");
+    }
+    float bits;
+    std::memcpy(&bits, &m0, 4);
+    hv[(size_t)v] = make_float2(p[1], bits);
+    if (p[2] > 1.0e-7f) act_idx.push_back((int)v);   // K:232-237
+  }
+  std::stable_sort(act_idx.begin(), act_idx.end(), [&](int a, int b) {
+    return ph->voxel_mat_dens[3 * (size_t)a + 2] > ph->voxel_mat_dens[3 * (size_t)b + 2];
+  });
+  std::vector<float> act_val(act_idx.size());
+  for (size_t i = 0; i < act_idx.size(); i++) act_val[i] = ph->voxel_mat_dens[3 * (size_t)act_idx[i] + 2];
+  c->n_active = (long long)act_idx.size();
+
+  // Tables: split (a,b) of the total cross section (read per Woodcock step) from the
+  // Compton/Rayleigh pairs (read per real interaction).
+  const int N = tab->num_values;
+  std::vector<float2> hw((size_t)N), htot((size_t)N * MCPET_MAX_MATERIALS);
+  std::vector<float4> hcr((size_t)N * MCPET_MAX_MATERIALS);
+  for (int i = 0; i < N; i++) hw[i] = make_float2(tab->woodcock[2 * i], tab->woodcock[2 * i + 1]);
+  for (size_t b = 0; b < (size_t)N * MCPET_MAX_MATERIALS; b++) {
+    const float* a = tab->mfp_a + 3 * b;
+    const float* s = tab->mfp_b + 3 * b;
+    htot[b] = make_float2(a[0], s[0]);
+    hcr[b] = make_float4(a[1], s[1], a[2], s[2]);
+  }
+
+  // ROI (dose) size.
+  const bool voxel_dose = cfg->voxel_dose && cfg->roi_x_max > -1;
+  if (voxel_dose) {
+    if (cfg->roi_x_min > cfg->roi_x_max || cfg->roi_y_min > cfg->roi_y_max || cfg->roi_z_min > cfg->roi_z_max ||
+        cfg->roi_x_min < 0 || cfg->roi_y_min < 0 || cfg->roi_z_min < 0 || cfg->roi_x_max >= ph->nx || cfg->roi_y_max >= ph->ny ||
+        cfg->roi_z_max >= ph->nz) {
+      free_all(c);
+      delete c;
+      return set_err(nullptr, MCPET_E_RANGE, "mcpet_create: dose region of interest outside the phantom");
+    }
+    c->n_roi = (long long)(cfg->roi_x_max - cfg->roi_x_min + 1) * (cfg->roi_y_max - cfg->roi_y_min + 1) *
+               (cfg->roi_z_max - cfg->roi_z_min + 1);
+  }
+  const int mode = cfg->detector.tally_psf_sinogram;
+  const bool want_psf = (mode == 0 || mode == 1) && cfg->detector.psf_size > 0;
+
+  CK(dalloc(c, &c->d_vox, (size_t)c->nvox));
+  CK(dalloc(c, &c->d_active_vox, std::max<size_t>(1, act_idx.size())));
+  CK(dalloc(c, &c->d_active_act, std::max<size_t>(1, act_idx.size())));
+  CK(dalloc(c, &c->d_woodcock, (size_t)N));
+  CK(dalloc(c, &c->d_tot, htot.size()));
+  CK(dalloc(c, &c->d_cr, hcr.size()));
+  CK(dalloc(c, &c->d_ray, 1));
+  CK(dalloc(c, &c->d_cmp, 1));
+  CK(dalloc(c, &c->d_trues, (size_t)cfg->nbins));
+  CK(dalloc(c, &c->d_scatter, (size_t)cfg->nbins));
+  CK(dalloc(c, &c->d_img_t, (size_t)c->nvox));
+  CK(dalloc(c, &c->d_img_sc, (size_t)c->nvox));
+  CK(dalloc(c, &c->d_spectrum, (size_t)std::max(1, cfg->ne)));
+  if (want_psf) CK(dalloc(c, &c->d_psf, (size_t)cfg->detector.psf_size));
+  CK(dalloc(c, &c->d_psf_index, 1));
+  if (voxel_dose) CK(dalloc(c, &c->d_voxel_edep, 2 * (size_t)c->n_roi));
+  CK(dalloc(c, &c->d_mat_edep, 2 * MCPET_MAX_MATERIALS));
+  CK(dalloc(c, &c->d_counters, KC_N + 2));
+  CK(dalloc(c, &c->d_final_seed, 1));
+
+  HK(hipMemcpy(c->d_vox, hv.data(), hv.size() * sizeof(float2), hipMemcpyHostToDevice));
+  if (!act_idx.empty()) {
+    HK(hipMemcpy(c->d_active_vox, act_idx.data(), act_idx.size() * sizeof(int), hipMemcpyHostToDevice));
+    HK(hipMemcpy(c->d_active_act, act_val.data(), act_val.size() * sizeof(float), hipMemcpyHostToDevice));
+  }
+  HK(hipMemcpy(c->d_woodcock, hw.data(), hw.size() * sizeof(float2), hipMemcpyHostToDevice));
+  HK(hipMemcpy(c->d_tot, htot.data(), htot.size() * sizeof(float2), hipMemcpyHostToDevice));
+  HK(hipMemcpy(c->d_cr, hcr.data(), hcr.size() * sizeof(float4), hipMemcpyHostToDevice));
+  HK(hipMemcpy(c->d_ray, tab->rayleigh, sizeof(mcpet_rayleigh_table), hipMemcpyHostToDevice));
+  HK(hipMemcpy(c->d_cmp, tab->compton, sizeof(mcpet_compton_table), hipMemcpyHostToDevice));
+
+  KParams& P = c->P;
+  P.vox = c->d_vox;
+  P.nx = ph->nx;
+  P.ny = ph->ny;
+  P.nz = ph->nz;
+  P.ivx = ph->inv_voxel_size[0];
+  P.ivy = ph->inv_voxel_size[1];
+  P.ivz = ph->inv_voxel_size[2];
+  P.bx = ph->size_bbox[0];
+  P.by = ph->size_bbox[1];
+  P.bz = ph->size_bbox[2];
+  P.active_vox = c->d_active_vox;
+  P.active_act = c->d_active_act;
+  P.tpb = cfg->threads_per_block;
+  P.n_streams = c->n_active * (long long)cfg->threads_per_block;
+  P.queue = c->d_counters + KC_N;
+  P.total_hist = c->d_counters + KC_N + 1;
+  P.seed = cfg->seed;
+  P.bg_mode = cfg->bg_mode;
+  P.acq_ps = cfg->acquisition_time_ps;
+  P.inv_mean_life = 1.0 / (double)cfg->mean_life;
+  mcr_build_jump_table(MCR_A1, MCR_M1, P.j1);
+  mcr_build_jump_table(MCR_A2, MCR_M2, P.j2);
+  P.woodcock = c->d_woodcock;
+  P.tot = c->d_tot;
+  P.cr = c->d_cr;
+  P.ray = c->d_ray;
+  P.cmp = c->d_cmp;
+  P.num_values = N;
+  P.e0 = tab->e0;
+  P.ide = tab->ide;
+  P.cx = c->det.center[0];
+  P.cy = c->det.center[1];
+  P.cz = c->det.center[2];
+  P.radius = c->det.radius;
+  P.height = c->det.height;
+  P.psf_size = want_psf ? cfg->detector.psf_size : 0;
+  P.mode = mode;
+  P.type = cfg->detector.tally_type;
+  P.e_resol = cfg->e_resol;
+  P.e_low = cfg->e_low;
+  P.e_high = cfg->e_high;
+  P.fovz = cfg->fovz;
+  P.nrows = cfg->nrows;
+  P.ncrystals = cfg->ncrystals;
+  P.nangles = cfg->nangles;
+  P.nrad = cfg->nrad;
+  P.nsinos = cfg->nsinos;
+  P.nzs = cfg->nzs;
+  P.ne = cfg->ne;
+  P.mrd = cfg->mrd;
+  P.span = cfg->span;
+  P.material_dose = cfg->material_dose ? 1 : 0;
+  P.voxel_dose = voxel_dose ? 1 : 0;
+  P.roi[0] = cfg->roi_x_min;
+  P.roi[1] = cfg->roi_x_max;
+  P.roi[2] = cfg->roi_y_min;
+  P.roi[3] = cfg->roi_y_max;
+  P.roi[4] = cfg->roi_z_min;
+  P.roi[5] = cfg->roi_z_max;
+  P.trues = c->d_trues;
+  P.scatter = c->d_scatter;
+  P.img_t = c->d_img_t;
+  P.img_sc = c->d_img_sc;
+  P.spectrum = c->d_spectrum;
+  P.psf = c->d_psf;
+  P.psf_index = c->d_psf_index;
+  P.voxel_edep = c->d_voxel_edep;
+  P.mat_edep = c->d_mat_edep;
+  P.counters = c->d_counters;
+  P.final_seed = c->d_final_seed;
+  P.last_thread_id = (c->nvox - 1) * (long long)cfg->threads_per_block + (cfg->threads_per_block - 1);
+  P.lds_spectrum = (cfg->ne > 0 && cfg->ne <= 8192) ? 1 : 0;
+  *out = c;
+#undef CK
+#undef HK
+  return MCPET_OK;
+}
+
+static mcpet_status zero_tallies(mcpet_ctx* c)
+{
+  const KParams& P = c->P;
+  HIPCHK(c, hipMemsetAsync(c->d_trues, 0, sizeof(int) * (size_t)c->cfg.nbins, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->d_scatter, 0, sizeof(int) * (size_t)c->cfg.nbins, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->d_img_t, 0, sizeof(int) * (size_t)c->nvox, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->d_img_sc, 0, sizeof(int) * (size_t)c->nvox, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->d_spectrum, 0, sizeof(int) * (size_t)std::max(1, c->cfg.ne), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->d_psf_index, 0, sizeof(int), c->stream));
+  if (c->d_voxel_edep) HIPCHK(c, hipMemsetAsync(c->d_voxel_edep, 0, 16 * (size_t)c->n_roi, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->d_mat_edep, 0, 16 * MCPET_MAX_MATERIALS, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->d_final_seed, 0, sizeof(int), c->stream));
+  (void)P;
+  return MCPET_OK;
+}
+
+extern "C" mcpet_status mcpet_run(mcpet_ctx* c, const mcpet_run_params* rp)
+{
+  if (!c) return set_err(nullptr, MCPET_E_ARG, "mcpet_run: null context");
+  mcpet_run_params p{};
+  p.zero_tallies = 1;
+  if (rp) p = *rp;
+  HIPCHK(c, hipSetDevice(c->device));
+  KParams P = c->P;
+  if (p.seed != 0) P.seed = p.seed;
+  if (p.acquisition_time_ps != 0) P.acq_ps = p.acquisition_time_ps;
+  int kind = p.kernel == MCPET_KERNEL_AUTO ? MCPET_KERNEL_PERSISTENT : p.kernel;
+  int bs = p.block_size > 0 ? p.block_size : 256;
+  if (bs % 64 != 0 || bs > 256) return set_err(c, MCPET_E_RANGE, "mcpet_run: block_size must be a multiple of 64, <= 256");
+  HIPCHK(c, hipEventRecord(c->ev_z0, c->stream));
+  if (p.zero_tallies) {
+    mcpet_status st = zero_tallies(c);
+    if (st != MCPET_OK) return st;
+  }
+  HIPCHK(c, hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * (KC_N + 2), c->stream));
+  HIPCHK(c, hipEventRecord(c->ev_k0, c->stream));
+  long long blocks;
+  if (kind == MCPET_KERNEL_PERSISTENT) {
+    int per_cu = p.blocks_per_cu > 0 ? p.blocks_per_cu : 8;
+    blocks = (long long)c->num_cus * per_cu;
+    long long need = (P.n_streams + bs - 1) / bs;
+    if (blocks > need) blocks = std::max(1LL, need);
+  } else {
+    blocks = std::max(1LL, (P.n_streams + bs - 1) / bs);
+    if (blocks > 0x7fffffffLL) return set_err(c, MCPET_E_RANGE, "mcpet_run: too many streams for one launch");
+  }
+  if (P.n_streams > 0) {
+    hipError_t e = mcpet_launch_transport(P, kind, (int)blocks, bs, c->stream);
+    if (e != hipSuccess) return set_err(c, MCPET_E_HIP, std::string("transport launch failed: ") + hipGetErrorString(e));
+  }
+  HIPCHK(c, hipEventRecord(c->ev_k1, c->stream));
+  c->ran = true;
+  c->timed = true;
+  return MCPET_OK;
+}
+
+extern "C" mcpet_status mcpet_sync(mcpet_ctx* c)
+{
+  if (!c) return set_err(nullptr, MCPET_E_ARG, "mcpet_sync: null context");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MCPET_OK;
+}
+
+extern "C" mcpet_status mcpet_fetch(mcpet_ctx* c, mcpet_tallies* t)
+{
+  if (!c || !t) return set_err(c, MCPET_E_ARG, "mcpet_fetch: null argument");
+  if (!c->ran) return set_err(c, MCPET_E_STATE, "mcpet_fetch: no run to fetch");
+  mcpet_status st = mcpet_sync(c);
+  if (st != MCPET_OK) return st;
+  const size_t nb = (size_t)c->cfg.nbins, nv = (size_t)c->nvox;
+  if (t->trues) HIPCHK(c, hipMemcpy(t->trues, c->d_trues, 4 * nb, hipMemcpyDeviceToHost));
+  if (t->scatter) HIPCHK(c, hipMemcpy(t->scatter, c->d_scatter, 4 * nb, hipMemcpyDeviceToHost));
+  if (t->image_trues) HIPCHK(c, hipMemcpy(t->image_trues, c->d_img_t, 4 * nv, hipMemcpyDeviceToHost));
+  if (t->image_scatter) HIPCHK(c, hipMemcpy(t->image_scatter, c->d_img_sc, 4 * nv, hipMemcpyDeviceToHost));
+  if (t->spectrum && c->cfg.ne > 0) HIPCHK(c, hipMemcpy(t->spectrum, c->d_spectrum, 4 * (size_t)c->cfg.ne, hipMemcpyDeviceToHost));
+  int idx = 0;
+  HIPCHK(c, hipMemcpy(&idx, c->d_psf_index, sizeof(int), hipMemcpyDeviceToHost));
+  t->psf_requested = idx;
+  long long n = std::min<long long>(idx, c->P.psf_size);
+  if (n < 0) n = 0;
+  t->psf_count = n;
+  if (t->psf && n > 0) {
+    if (t->psf_capacity < n) return set_err(c, MCPET_E_CAPACITY, "mcpet_fetch: psf buffer smaller than the records written");
+    HIPCHK(c, hipMemcpy(t->psf, c->d_psf, (size_t)n * sizeof(mcpet_psf_record), hipMemcpyDeviceToHost));
+  }
+  if (t->voxel_edep && c->d_voxel_edep)
+    HIPCHK(c, hipMemcpy(t->voxel_edep, c->d_voxel_edep, 16 * (size_t)c->n_roi, hipMemcpyDeviceToHost));
+  if (t->material_edep) HIPCHK(c, hipMemcpy(t->material_edep, c->d_mat_edep, 16 * MCPET_MAX_MATERIALS, hipMemcpyDeviceToHost));
+  unsigned long long cnt[KC_N + 2];
+  HIPCHK(c, hipMemcpy(cnt, c->d_counters, sizeof cnt, hipMemcpyDeviceToHost));
+  t->total_histories = cnt[KC_N + 1];
+  HIPCHK(c, hipMemcpy(&t->final_seed, c->d_final_seed, sizeof(int), hipMemcpyDeviceToHost));
+  // Sums over the sinograms (host copies if fetched, else device-side counters).
+  if (t->trues && t->scatter) {
+    long long a = 0, b = 0;
+    for (size_t i = 0; i < nb; i++) {
+      a += t->trues[i];
+      b += t->scatter[i];
+    }
+    t->sum_trues = a;
+    t->sum_scatter = b;
+  } else {
+    t->sum_trues = -1;
+    t->sum_scatter = -1;
+  }
+  return MCPET_OK;
+}
+
+extern "C" mcpet_status mcpet_get_stats(mcpet_ctx* c, mcpet_stats* s)
+{
+  if (!c || !s) return set_err(c, MCPET_E_ARG, "mcpet_get_stats: null argument");
+  if (!c->ran) return set_err(c, MCPET_E_STATE, "mcpet_get_stats: no run");
+  mcpet_status st = mcpet_sync(c);
+  if (st != MCPET_OK) return st;
+  std::memset(s, 0, sizeof *s);
+  float ms = 0.0f, zms = 0.0f;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev_k0, c->ev_k1));
+  HIPCHK(c, hipEventElapsedTime(&zms, c->ev_z0, c->ev_k0));
+  s->kernel_ms = ms;
+  s->zero_ms = zms;
+  unsigned long long cnt[KC_N + 2];
+  HIPCHK(c, hipMemcpy(cnt, c->d_counters, sizeof cnt, hipMemcpyDeviceToHost));
+  s->decays = cnt[KC_DECAYS];
+  s->streams = cnt[KC_STREAMS];
+  s->woodcock_steps = cnt[KC_WOODCOCK];
+  s->material_changes = cnt[KC_MATCHANGE];
+  s->compton = cnt[KC_COMPTON];
+  s->compton_shell_evals = cnt[KC_SHELLEVAL];
+  s->rayleigh = cnt[KC_RAYLEIGH];
+  s->photoelectric = cnt[KC_PHOTO];
+  s->pairs_tallied = cnt[KC_PAIRS];
+  s->coincidences = cnt[KC_COINC];
+  s->rn_draws = cnt[KC_RNDRAWS];
+  return MCPET_OK;
+}
+
+extern "C" mcpet_status mcpet_get_detector(const mcpet_ctx* c, mcpet_detector* out)
+{
+  if (!c || !out) return MCPET_E_ARG;
+  *out = c->det;
+  return MCPET_OK;
+}
+
+extern "C" void mcpet_destroy(mcpet_ctx* c)
+{
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_all(c);
+  delete c;
+}
+
+extern "C" mcpet_status mcpet_stream_seeds(int32_t seed, const int64_t* ids, int32_t n, int32_t* sx, int32_t* sy)
+{
+  if (!ids || !sx || !sy || n < 0) return set_err(nullptr, MCPET_E_ARG, "mcpet_stream_seeds: bad args");
+  uint32_t j1[MCR_JUMP_BITS], j2[MCR_JUMP_BITS];
+  mcr_build_jump_table(MCR_A1, MCR_M1, j1);
+  mcr_build_jump_table(MCR_A2, MCR_M2, j2);
+  for (int32_t i = 0; i < n; i++) mcr_init_stream(ids[i], seed, j1, j2, &sx[i], &sy[i]);
+  return MCPET_OK;
+}
+
+// ------------------------------------------------------------------------------ self tests
+
+extern "C" mcpet_status mcpet_selftest_ranecu(int32_t seed, const int64_t* ids, int32_t nstreams, int32_t n, float* out,
+                                              double* out_d, int32_t as_double)
+{
+  if (!ids || nstreams < 1 || n < 1 || (as_double ? !out_d : !out)) return set_err(nullptr, MCPET_E_ARG, "selftest_ranecu: bad args");
+  uint32_t j[2][MCR_JUMP_BITS];
+  mcr_build_jump_table(MCR_A1, MCR_M1, j[0]);
+  mcr_build_jump_table(MCR_A2, MCR_M2, j[1]);
+  long long* d_ids = nullptr;
+  uint32_t* d_j = nullptr;
+  void* d_out = nullptr;
+  const size_t elem = as_double ? 8 : 4;
+  const size_t bytes = (size_t)nstreams * n * elem;
+  mcpet_status rc = MCPET_OK;
+  hipError_t e = hipMalloc((void**)&d_ids, sizeof(long long) * nstreams);
+  if (e == hipSuccess) e = hipMalloc((void**)&d_j, sizeof j);
+  if (e == hipSuccess) e = hipMalloc(&d_out, bytes);
+  if (e == hipSuccess) e = hipMemcpy(d_ids, ids, sizeof(long long) * nstreams, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d_j, j, sizeof j, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = mcpet_launch_selftest_ranecu(seed, d_ids, nstreams, n, (float*)d_out, (double*)d_out, as_double, d_j, d_j + MCR_JUMP_BITS, 0);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(as_double ? (void*)out_d : (void*)out, d_out, bytes, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) rc = set_err(nullptr, MCPET_E_HIP, std::string("selftest_ranecu: ") + hipGetErrorString(e));
+  if (d_ids) (void)hipFree(d_ids);
+  if (d_j) (void)hipFree(d_j);
+  if (d_out) (void)hipFree(d_out);
+  return rc;
+}
+
+extern "C" mcpet_status mcpet_selftest_fmath(const double* x, const double* y, int32_t n, double* od, float* of)
+{
+  if (!x || !y || !od || !of || n < 1) return set_err(nullptr, MCPET_E_ARG, "selftest_fmath: bad args");
+  double *dx = nullptr, *dy = nullptr, *dod = nullptr;
+  float* dof = nullptr;
+  mcpet_status rc = MCPET_OK;
+  hipError_t e = hipMalloc((void**)&dx, 8 * (size_t)n);
+  if (e == hipSuccess) e = hipMalloc((void**)&dy, 8 * (size_t)n);
+  if (e == hipSuccess) e = hipMalloc((void**)&dod, 7 * 8 * (size_t)n);
+  if (e == hipSuccess) e = hipMalloc((void**)&dof, 7 * 4 * (size_t)n);
+  if (e == hipSuccess) e = hipMemcpy(dx, x, 8 * (size_t)n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dy, y, 8 * (size_t)n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = mcpet_launch_selftest_fmath(dx, dy, n, dod, dof, 0);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(od, dod, 7 * 8 * (size_t)n, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(of, dof, 7 * 4 * (size_t)n, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) rc = set_err(nullptr, MCPET_E_HIP, std::string("selftest_fmath: ") + hipGetErrorString(e));
+  if (dx) (void)hipFree(dx);
+  if (dy) (void)hipFree(dy);
+  if (dod) (void)hipFree(dod);
+  if (dof) (void)hipFree(dof);
+  return rc;
+}
+
+extern "C" mcpet_status mcpet_selftest_copy(size_t nbytes, int32_t iters, double* ms_out)
+{
+  if (nbytes < 16 || iters < 1) return set_err(nullptr, MCPET_E_ARG, "selftest_copy: bad args");
+  size_t n16 = nbytes / 16;
+  uint4 *a = nullptr, *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  mcpet_status rc = MCPET_OK;
+  hipError_t e = hipMalloc((void**)&a, n16 * 16);
+  if (e == hipSuccess) e = hipMalloc((void**)&b, n16 * 16);
+  if (e == hipSuccess) e = hipMemset(a, 1, n16 * 16);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = mcpet_launch_copy(a, b, n16, 0);
+  if (e == hipSuccess) e = hipEventRecord(e0, 0);
+  for (int i = 0; i < iters && e == hipSuccess; i++) e = mcpet_launch_copy(a, b, n16, 0);
+  if (e == hipSuccess) e = hipEventRecord(e1, 0);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (ms_out) *ms_out = ms / iters;
+  if (e != hipSuccess) rc = set_err(nullptr, MCPET_E_HIP, std::string("selftest_copy: ") + hipGetErrorString(e));
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  return rc;
+}

@@ -1,0 +1,452 @@
+"""Python mirror of the libmcpet C ABI (include/mcpet.h), via ctypes.
+
+The product is the shared library libmcpet.so (gfx950 HIP kernels + host C++); this
+module only marshals arguments. It never computes tallies itself and raises if the
+library is missing: there is no CPU fallback.
+
+Reference correspondence (MCGPU-PET): read_input -> read_input (MCGPU-PET.cu:866),
+load_voxels -> load_voxels (C:1381), load_materials -> load_material (C:1565),
+Simulation.run -> track_particles launch (C:452-487), Simulation.fetch -> result copies
+(C:499-535), write_outputs -> output writers (C:544-616, 3193-3322, 2389-2650).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "lib", "libmcpet.so")
+
+MAX_MATERIALS = 15
+MAX_SHELLS = 30
+NP_RAYLEIGH = 128
+MAX_ENERGYBINS_RAYLEIGH = 25005
+NAME_LEN = 250
+
+KERNEL_AUTO, KERNEL_PLAIN, KERNEL_PERSISTENT = 0, 1, 2
+
+STATUS = {0: "OK", 1: "E_ARG", 2: "E_IO", 3: "E_FORMAT", 4: "E_RANGE", 5: "E_NOMEM", 6: "E_HIP", 7: "E_STATE",
+          8: "E_CAPACITY"}
+
+
+class McpetError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Detector(C.Structure):
+    _fields_ = [("center", C.c_float * 3), ("radius", C.c_float), ("height", C.c_float), ("psf_size", C.c_int32),
+                ("tally_psf_sinogram", C.c_int32), ("tally_type", C.c_int32)]
+
+
+class PsfRecord(C.Structure):
+    _pack_ = 4
+    _fields_ = [("emission_time_ps", C.c_uint64), ("travel_time_ps", C.c_float), ("emission_absvox", C.c_int32),
+                ("energy", C.c_float), ("z", C.c_float), ("phi", C.c_float), ("vx", C.c_float), ("vy", C.c_float),
+                ("vz", C.c_float), ("index1", C.c_int16), ("index2", C.c_int16)]
+
+
+# numpy view of the 44-byte PSF record (MCGPU-PET.h:235-247)
+PSF_DTYPE = np.dtype([("emission_time_ps", "<u8"), ("travel_time_ps", "<f4"), ("emission_absvox", "<i4"),
+                      ("energy", "<f4"), ("z", "<f4"), ("phi", "<f4"), ("vx", "<f4"), ("vy", "<f4"), ("vz", "<f4"),
+                      ("index1", "<i2"), ("index2", "<i2")])
+assert PSF_DTYPE.itemsize == 44
+
+
+class RayleighTable(C.Structure):
+    _fields_ = [("xco", C.c_float * (NP_RAYLEIGH * MAX_MATERIALS)), ("pco", C.c_float * (NP_RAYLEIGH * MAX_MATERIALS)),
+                ("aco", C.c_float * (NP_RAYLEIGH * MAX_MATERIALS)), ("bco", C.c_float * (NP_RAYLEIGH * MAX_MATERIALS)),
+                ("pmax", C.c_float * (MAX_ENERGYBINS_RAYLEIGH * MAX_MATERIALS)),
+                ("itlco", C.c_uint8 * (NP_RAYLEIGH * MAX_MATERIALS)), ("ituco", C.c_uint8 * (NP_RAYLEIGH * MAX_MATERIALS))]
+
+
+class ComptonTable(C.Structure):
+    _fields_ = [("fco", C.c_float * (MAX_MATERIALS * MAX_SHELLS)), ("uico", C.c_float * (MAX_MATERIALS * MAX_SHELLS)),
+                ("fj0", C.c_float * (MAX_MATERIALS * MAX_SHELLS)), ("noscco", C.c_int32 * MAX_MATERIALS)]
+
+
+class Config(C.Structure):
+    _fields_ = [("seed", C.c_int32), ("gpu_id", C.c_int32), ("threads_per_block", C.c_int32), ("fact_object", C.c_float),
+                ("acquisition_time_ps", C.c_uint64), ("mean_life", C.c_float), ("activity", C.c_float * MAX_MATERIALS),
+                ("psf_file", C.c_char * NAME_LEN), ("detector", Detector), ("material_dose", C.c_int32),
+                ("voxel_dose", C.c_int32), ("dose_file", C.c_char * NAME_LEN), ("roi_x_min", C.c_int16),
+                ("roi_x_max", C.c_int16), ("roi_y_min", C.c_int16), ("roi_y_max", C.c_int16), ("roi_z_min", C.c_int16),
+                ("roi_z_max", C.c_int16), ("e_resol", C.c_float), ("e_low", C.c_float), ("e_high", C.c_float),
+                ("fovz", C.c_float), ("nrows", C.c_int32), ("ncrystals", C.c_int32), ("nangles", C.c_int32),
+                ("nrad", C.c_int32), ("nzs", C.c_int32), ("res", C.c_int32), ("ne", C.c_int32), ("mrd", C.c_int32),
+                ("span", C.c_int32), ("nseg", C.c_int32), ("nsinos", C.c_int32), ("nbins", C.c_int32),
+                ("nvoxs", C.c_int32), ("voxel_file", C.c_char * NAME_LEN),
+                ("material_files", (C.c_char * NAME_LEN) * MAX_MATERIALS), ("bg_mode", C.c_int32)]
+
+
+class Phantom(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("voxel_size", C.c_float * 3),
+                ("inv_voxel_size", C.c_float * 3), ("size_bbox", C.c_float * 3),
+                ("density_max", C.c_float * MAX_MATERIALS), ("voxel_mat_dens", C.POINTER(C.c_float)),
+                ("owned", C.c_int32)]
+
+
+class Tables(C.Structure):
+    _fields_ = [("num_values", C.c_int32), ("e0", C.c_float), ("ide", C.c_float),
+                ("density_nominal", C.c_float * MAX_MATERIALS), ("woodcock", C.POINTER(C.c_float)),
+                ("mfp_a", C.POINTER(C.c_float)), ("mfp_b", C.POINTER(C.c_float)),
+                ("rayleigh", C.POINTER(RayleighTable)), ("compton", C.POINTER(ComptonTable)), ("owned", C.c_int32)]
+
+
+class Tallies(C.Structure):
+    _fields_ = [("trues", C.POINTER(C.c_int32)), ("scatter", C.POINTER(C.c_int32)),
+                ("image_trues", C.POINTER(C.c_int32)), ("image_scatter", C.POINTER(C.c_int32)),
+                ("spectrum", C.POINTER(C.c_int32)), ("psf", C.c_void_p), ("psf_capacity", C.c_int64),
+                ("voxel_edep", C.POINTER(C.c_uint64)), ("material_edep", C.POINTER(C.c_uint64)),
+                ("total_histories", C.c_uint64), ("psf_count", C.c_int64), ("psf_requested", C.c_int64),
+                ("sum_trues", C.c_int64), ("sum_scatter", C.c_int64), ("final_seed", C.c_int32)]
+
+
+class RunParams(C.Structure):
+    _fields_ = [("kernel", C.c_int32), ("block_size", C.c_int32), ("blocks_per_cu", C.c_int32),
+                ("zero_tallies", C.c_int32), ("seed", C.c_int32), ("acquisition_time_ps", C.c_uint64)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("zero_ms", C.c_double), ("decays", C.c_uint64), ("streams", C.c_uint64),
+                ("woodcock_steps", C.c_uint64), ("material_changes", C.c_uint64), ("compton", C.c_uint64),
+                ("compton_shell_evals", C.c_uint64), ("rayleigh", C.c_uint64), ("photoelectric", C.c_uint64),
+                ("pairs_tallied", C.c_uint64), ("coincidences", C.c_uint64), ("rn_draws", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Load libmcpet.so (built by mcgpu-pet_amd/build.py). Raises if absent: no fallback."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise McpetError(2, f"{LIB_PATH} not built (run python mcgpu-pet_amd/build.py)")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    L.mcpet_read_input.argtypes = [C.c_char_p, P(Config), C.c_char_p, C.c_size_t]
+    L.mcpet_load_voxels.argtypes = [C.c_char_p, P(Config), P(Phantom), C.c_char_p, C.c_size_t]
+    L.mcpet_phantom_from_arrays.argtypes = [C.c_int32, C.c_int32, C.c_int32, P(C.c_float), P(C.c_int32), P(C.c_float),
+                                            P(C.c_float), C.c_float, P(Phantom), C.c_char_p, C.c_size_t]
+    L.mcpet_free_phantom.argtypes = [P(Phantom)]
+    L.mcpet_free_phantom.restype = None
+    L.mcpet_load_materials.argtypes = [C.c_void_p, P(C.c_float), P(Tables), C.c_char_p, C.c_size_t]
+    L.mcpet_free_tables.argtypes = [P(Tables)]
+    L.mcpet_free_tables.restype = None
+    L.mcpet_sinogram_sizes.argtypes = [P(Config)]
+    L.mcpet_sinogram_sizes.restype = None
+    L.mcpet_create.argtypes = [P(Config), P(Phantom), P(Tables), P(C.c_void_p)]
+    L.mcpet_run.argtypes = [C.c_void_p, P(RunParams)]
+    L.mcpet_sync.argtypes = [C.c_void_p]
+    L.mcpet_fetch.argtypes = [C.c_void_p, P(Tallies)]
+    L.mcpet_get_stats.argtypes = [C.c_void_p, P(Stats)]
+    L.mcpet_destroy.argtypes = [C.c_void_p]
+    L.mcpet_destroy.restype = None
+    L.mcpet_last_error.argtypes = [C.c_void_p]
+    L.mcpet_last_error.restype = C.c_char_p
+    L.mcpet_get_detector.argtypes = [C.c_void_p, P(Detector)]
+    L.mcpet_write_outputs.argtypes = [C.c_char_p, P(Config), P(Phantom), P(Tables), P(Tallies), C.c_double, C.c_char_p,
+                                      C.c_size_t]
+    L.mcpet_selftest_ranecu.argtypes = [C.c_int32, P(C.c_int64), C.c_int32, C.c_int32, P(C.c_float), P(C.c_double),
+                                        C.c_int32]
+    L.mcpet_selftest_fmath.argtypes = [P(C.c_double), P(C.c_double), C.c_int32, P(C.c_double), P(C.c_float)]
+    L.mcpet_selftest_copy.argtypes = [C.c_size_t, C.c_int32, P(C.c_double)]
+    L.mcpet_stream_seeds.argtypes = [C.c_int32, P(C.c_int64), C.c_int32, P(C.c_int32), P(C.c_int32)]
+    L.mcpet_version.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def _check(st, ctx=None, err=None):
+    if st != 0:
+        msg = err.value.decode(errors="replace") if err is not None and err.value else \
+            lib().mcpet_last_error(ctx).decode(errors="replace")
+        raise McpetError(st, msg)
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def read_input(path):
+    cfg = Config()
+    err = C.create_string_buffer(1024)
+    _check(lib().mcpet_read_input(os.fsencode(path), C.byref(cfg), err, 1024), err=err)
+    return cfg
+
+
+def sinogram_sizes(cfg):
+    lib().mcpet_sinogram_sizes(C.byref(cfg))
+    return cfg
+
+
+def load_voxels(path, cfg=None):
+    ph = Phantom()
+    err = C.create_string_buffer(1024)
+    _check(lib().mcpet_load_voxels(os.fsencode(path), C.byref(cfg) if cfg is not None else None, C.byref(ph), err, 1024),
+           err=err)
+    return PhantomHandle(ph)
+
+
+def phantom_from_arrays(material, density, activity, voxel_size, fact_object=1.0):
+    """material/density/activity: arrays shaped (nz, ny, nx) (x fastest in memory)."""
+    material = np.ascontiguousarray(material, dtype=np.int32)
+    density = np.ascontiguousarray(density, dtype=np.float32)
+    activity = np.ascontiguousarray(activity, dtype=np.float32)
+    nz, ny, nx = material.shape
+    vs = (C.c_float * 3)(*[float(v) for v in voxel_size])
+    ph = Phantom()
+    err = C.create_string_buffer(1024)
+    _check(lib().mcpet_phantom_from_arrays(nx, ny, nz, vs, material.ctypes.data_as(C.POINTER(C.c_int32)),
+                                           _fptr(density), _fptr(activity), float(fact_object), C.byref(ph), err, 1024),
+           err=err)
+    return PhantomHandle(ph)
+
+
+class PhantomHandle:
+    """Owns a library-allocated phantom."""
+
+    def __init__(self, ph):
+        self.ph = ph
+
+    @property
+    def shape(self):
+        return (self.ph.nz, self.ph.ny, self.ph.nx)
+
+    @property
+    def nvox(self):
+        return self.ph.nx * self.ph.ny * self.ph.nz
+
+    def voxel_mat_dens(self):
+        return np.ctypeslib.as_array(self.ph.voxel_mat_dens, shape=(self.nvox, 3)).copy()
+
+    def density_max(self):
+        return np.array(self.ph.density_max[:], dtype=np.float32)
+
+    def close(self):
+        if self.ph is not None and self.ph.owned:
+            lib().mcpet_free_phantom(C.byref(self.ph))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _names_buffer(names):
+    buf = ((C.c_char * NAME_LEN) * MAX_MATERIALS)()
+    for i, n in enumerate(names[:MAX_MATERIALS]):
+        b = os.fsencode(n) if n else b""
+        buf[i].value = b[:NAME_LEN - 1]
+    return buf
+
+
+def load_materials(names, density_max):
+    """names: up to 15 material file paths ('' for unused slots)."""
+    if isinstance(names, Config):
+        names = [names.material_files[i].value.decode() for i in range(MAX_MATERIALS)]
+    names = list(names) + [""] * (MAX_MATERIALS - len(names))
+    dm = np.ascontiguousarray(density_max, dtype=np.float32)
+    t = Tables()
+    err = C.create_string_buffer(1024)
+    buf = _names_buffer(names)
+    _check(lib().mcpet_load_materials(C.cast(buf, C.c_void_p), _fptr(dm), C.byref(t), err, 1024), err=err)
+    return TablesHandle(t)
+
+
+class TablesHandle:
+    def __init__(self, t):
+        self.t = t
+
+    def arrays(self):
+        t = self.t
+        n = t.num_values
+        return {
+            "num_values": n, "e0": t.e0, "ide": t.ide,
+            "density_nominal": np.array(t.density_nominal[:], dtype=np.float32),
+            "woodcock": np.ctypeslib.as_array(t.woodcock, shape=(n, 2)).copy(),
+            "mfp_a": np.ctypeslib.as_array(t.mfp_a, shape=(n, MAX_MATERIALS, 3)).copy(),
+            "mfp_b": np.ctypeslib.as_array(t.mfp_b, shape=(n, MAX_MATERIALS, 3)).copy(),
+        }
+
+    def close(self):
+        if self.t is not None and self.t.owned:
+            lib().mcpet_free_tables(C.byref(self.t))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Simulation:
+    """A device context: phantom and tables uploaded once, tallies device-resident."""
+
+    def __init__(self, cfg, phantom, tables):
+        self.cfg = cfg
+        self.phantom = phantom
+        self.tables = tables
+        self.ctx = C.c_void_p()
+        st = lib().mcpet_create(C.byref(cfg), C.byref(phantom.ph), C.byref(tables.t), C.byref(self.ctx))
+        if st != 0:
+            raise McpetError(st, lib().mcpet_last_error(None).decode(errors="replace"))
+
+    def run(self, kernel=KERNEL_AUTO, block_size=0, blocks_per_cu=0, zero_tallies=True, seed=0, acquisition_time_ps=0):
+        rp = RunParams(kernel, block_size, blocks_per_cu, 1 if zero_tallies else 0, seed, acquisition_time_ps)
+        _check(lib().mcpet_run(self.ctx, C.byref(rp)), self.ctx)
+
+    def sync(self):
+        _check(lib().mcpet_sync(self.ctx), self.ctx)
+
+    def stats(self):
+        s = Stats()
+        _check(lib().mcpet_get_stats(self.ctx, C.byref(s)), self.ctx)
+        return s.as_dict()
+
+    def detector(self):
+        d = Detector()
+        _check(lib().mcpet_get_detector(self.ctx, C.byref(d)), self.ctx)
+        return d
+
+    def fetch(self, sinograms=True, images=True, spectrum=True, psf=True, dose=True):
+        cfg = self.cfg
+        nv = self.phantom.nvox
+        out = {}
+        t = Tallies()
+        if sinograms:
+            out["trues"] = np.zeros(cfg.nbins, np.int32)
+            out["scatter"] = np.zeros(cfg.nbins, np.int32)
+            t.trues = out["trues"].ctypes.data_as(C.POINTER(C.c_int32))
+            t.scatter = out["scatter"].ctypes.data_as(C.POINTER(C.c_int32))
+        if images:
+            out["image_trues"] = np.zeros(nv, np.int32)
+            out["image_scatter"] = np.zeros(nv, np.int32)
+            t.image_trues = out["image_trues"].ctypes.data_as(C.POINTER(C.c_int32))
+            t.image_scatter = out["image_scatter"].ctypes.data_as(C.POINTER(C.c_int32))
+        if spectrum and cfg.ne > 0:
+            out["spectrum"] = np.zeros(cfg.ne, np.int32)
+            t.spectrum = out["spectrum"].ctypes.data_as(C.POINTER(C.c_int32))
+        psf_arr = None
+        if psf and cfg.detector.tally_psf_sinogram in (0, 1) and cfg.detector.psf_size > 0:
+            psf_arr = np.zeros(cfg.detector.psf_size, PSF_DTYPE)
+            t.psf = psf_arr.ctypes.data
+            t.psf_capacity = cfg.detector.psf_size
+        if dose:
+            out["material_edep"] = np.zeros((MAX_MATERIALS, 2), np.uint64)
+            t.material_edep = out["material_edep"].ctypes.data_as(C.POINTER(C.c_uint64))
+            if cfg.voxel_dose and cfg.roi_x_max > -1:
+                n_roi = (cfg.roi_x_max - cfg.roi_x_min + 1) * (cfg.roi_y_max - cfg.roi_y_min + 1) * \
+                        (cfg.roi_z_max - cfg.roi_z_min + 1)
+                out["voxel_edep"] = np.zeros((n_roi, 2), np.uint64)
+                t.voxel_edep = out["voxel_edep"].ctypes.data_as(C.POINTER(C.c_uint64))
+        _check(lib().mcpet_fetch(self.ctx, C.byref(t)), self.ctx)
+        if psf_arr is not None:
+            out["psf"] = psf_arr[:t.psf_count].copy()
+        out["total_histories"] = t.total_histories
+        out["psf_count"] = t.psf_count
+        out["psf_requested"] = t.psf_requested
+        out["sum_trues"] = t.sum_trues
+        out["sum_scatter"] = t.sum_scatter
+        out["final_seed"] = t.final_seed
+        self._last_tallies = t
+        return out
+
+    def close(self):
+        if self.ctx:
+            lib().mcpet_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def write_outputs(directory, cfg, phantom, tables, tallies, time_elapsed=0.0):
+    """Write the reference output files for a fetched result dict (see Simulation.fetch)."""
+    t = Tallies()
+    keep = []
+
+    def ip(name):
+        a = tallies.get(name)
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=np.int32)
+        keep.append(a)
+        return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+    t.trues, t.scatter = ip("trues"), ip("scatter")
+    t.image_trues, t.image_scatter, t.spectrum = ip("image_trues"), ip("image_scatter"), ip("spectrum")
+    if tallies.get("psf") is not None:
+        p = np.ascontiguousarray(tallies["psf"])
+        keep.append(p)
+        t.psf = p.ctypes.data
+        t.psf_capacity = len(p)
+        t.psf_count = len(p)
+    if tallies.get("voxel_edep") is not None:
+        v = np.ascontiguousarray(tallies["voxel_edep"], dtype=np.uint64)
+        keep.append(v)
+        t.voxel_edep = v.ctypes.data_as(C.POINTER(C.c_uint64))
+    t.total_histories = int(tallies.get("total_histories", 0))
+    err = C.create_string_buffer(1024)
+    _check(lib().mcpet_write_outputs(os.fsencode(directory), C.byref(cfg), C.byref(phantom.ph), C.byref(tables.t),
+                                     C.byref(t), float(time_elapsed), err, 1024), err=err)
+
+
+def selftest_ranecu(seed, stream_ids, n, as_double=False):
+    ids = np.ascontiguousarray(stream_ids, dtype=np.int64)
+    if as_double:
+        out = np.zeros((len(ids), n), np.float64)
+        _check(lib().mcpet_selftest_ranecu(seed, ids.ctypes.data_as(C.POINTER(C.c_int64)), len(ids), n, None,
+                                           out.ctypes.data_as(C.POINTER(C.c_double)), 1))
+    else:
+        out = np.zeros((len(ids), n), np.float32)
+        _check(lib().mcpet_selftest_ranecu(seed, ids.ctypes.data_as(C.POINTER(C.c_int64)), len(ids), n,
+                                           out.ctypes.data_as(C.POINTER(C.c_float)), None, 0))
+    return out
+
+
+def stream_seeds(seed, stream_ids):
+    """Host-side RANECU seeds at the start of each stream (same skip-ahead as the kernels)."""
+    ids = np.ascontiguousarray(stream_ids, dtype=np.int64)
+    sx = np.zeros(len(ids), np.int32)
+    sy = np.zeros(len(ids), np.int32)
+    _check(lib().mcpet_stream_seeds(seed, ids.ctypes.data_as(C.POINTER(C.c_int64)), len(ids),
+                                    sx.ctypes.data_as(C.POINTER(C.c_int32)), sy.ctypes.data_as(C.POINTER(C.c_int32))))
+    return sx, sy
+
+
+def selftest_fmath(x, y):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    n = len(x)
+    od = np.zeros((7, n), np.float64)
+    of = np.zeros((7, n), np.float32)
+    _check(lib().mcpet_selftest_fmath(x.ctypes.data_as(C.POINTER(C.c_double)), y.ctypes.data_as(C.POINTER(C.c_double)),
+                                      n, od.ctypes.data_as(C.POINTER(C.c_double)), of.ctypes.data_as(C.POINTER(C.c_float))))
+    return od, of
+
+
+def selftest_copy(nbytes, iters=10):
+    ms = C.c_double()
+    _check(lib().mcpet_selftest_copy(int(nbytes), int(iters), C.byref(ms)))
+    return ms.value
+
+
+def version():
+    return lib().mcpet_version().decode()

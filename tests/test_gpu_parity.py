@@ -1,0 +1,152 @@
+"""GPU parity: the MI355X library against the CPU oracle on the same inputs.
+
+Bit-exact tier: the oracle built with the library's elementary functions ("fmath") follows
+the reference algorithm stream by stream, so every integer tally, the PSF multiset, the
+dose sums, the final seed and the event counters must be identical.
+Statistical tier: the oracle built on the C library ("libm") is an independent arithmetic;
+totals must agree within Poisson errors and the energy spectrum by a chi-square test.
+"""
+import sys
+
+import numpy as np
+import pytest
+from scipy import stats
+
+import mcpet
+import oracle_py
+from harness import assert_tallies_equal, head_scenario, sample_scenario, water_cube_scenario
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def test_ranecu_bit_exact():
+    _say("ranecu: 12 streams x 4096 draws, float and double")
+    ids = np.array([0, 1, 2, 31, 1000, 123456, 999999, 2**31 - 2, 2**31 - 1, 2**31 + 7, 5 * 2**33 + 3, 10**11], np.int64)
+    for seed in (1, 12345, 2147483562, -17):
+        g = mcpet.selftest_ranecu(seed, ids, 4096)
+        o = oracle_py.ranecu_block(seed, ids, 4096)
+        assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
+        gd = mcpet.selftest_ranecu(seed, ids, 512, as_double=True)
+        od = oracle_py.ranecu_block(seed, ids, 512, as_double=True)
+        assert np.array_equal(gd.view(np.uint64), od.view(np.uint64))
+
+
+def test_fmath_host_device_bit_exact():
+    _say("fmath: 200k inputs, 14 functions, device vs gcc host")
+    rng = np.random.default_rng(5)
+    n = 200_000
+    x = np.concatenate([rng.uniform(-20, 20, n // 4), rng.uniform(-1, 1, n // 4),
+                        np.exp(rng.uniform(-30, 6, n // 4)) * rng.choice([-1, 1], n // 4),
+                        rng.uniform(-700, 700, n - 3 * (n // 4))])
+    y = rng.uniform(-1.5, 1.5, n)
+    gd, gf = mcpet.selftest_fmath(x, y)
+    hd, hf = oracle_py.fmath_eval(x, y)
+    for i in range(7):
+        bad = np.flatnonzero(gd[i].view(np.uint64) != hd[i].view(np.uint64))
+        assert bad.size == 0, f"double fn {i}: {bad.size} mismatches, first x={x[bad[0]]!r} y={y[bad[0]]!r}"
+        badf = np.flatnonzero(gf[i].view(np.uint32) != hf[i].view(np.uint32))
+        assert badf.size == 0, f"float fn {i}: {badf.size} mismatches, first x={x[badf[0]]!r} y={y[badf[0]]!r}"
+
+
+@pytest.mark.parametrize("kernel", [mcpet.KERNEL_PLAIN, mcpet.KERNEL_PERSISTENT])
+def test_water_cube_bit_exact(kernel):
+    _say(f"water cube 2e5 decays, tpb 32, sinogram+PSF+doses, kernel {kernel}")
+    sc = water_cube_scenario(n_decays=2e5, tpb=32, tally_psf_sinogram=0, psf_size=400_000, material_dose=True,
+                             voxel_dose=True, roi=(1, 7, 0, 8, 2, 6))
+    g = sc.run_gpu(kernel=kernel)
+    o = sc.run_oracle("fmath")
+    assert o["trues"].sum() > 1000 and o["scatter"].sum() > 100
+    assert_tallies_equal(g, o)
+
+
+def test_sample_simulation_bit_exact():
+    _say("reference sample .in (R 9.05 cm), 2 s acquisition, PSF + sinogram + material dose")
+    sc = sample_scenario(acq_s=2.0)
+    g = sc.run_gpu()
+    o = sc.run_oracle("fmath")
+    assert o["total_histories"] > 40000
+    assert_tallies_equal(g, o)
+
+
+@pytest.mark.parametrize("extra", [dict(bg_mode=1), dict(tally_type=1, tally_psf_sinogram=0, psf_size=100_000),
+                                   dict(tally_type=2, tally_psf_sinogram=1, psf_size=100_000)])
+def test_modes_bit_exact(extra):
+    _say(f"water cube modes {extra}")
+    bg = extra.pop("bg_mode", 0)
+    sc = water_cube_scenario(n_decays=5e4, tpb=16, **extra)
+    sc.cfg.bg_mode = bg
+    g = sc.run_gpu()
+    o = sc.run_oracle("fmath")
+    assert_tallies_equal(g, o)
+
+
+def test_head_small_bit_exact():
+    _say("head phantom 32x32x24 at 8 mm, 8 tissues, 1e5 decays")
+    sc = head_scenario(n_decays=1e5, material_dose=True, tally_psf_sinogram=0, psf_size=200_000)
+    g = sc.run_gpu()
+    o = sc.run_oracle("fmath")
+    assert o["counters"]["compton"] > 10000
+    assert_tallies_equal(g, o)
+
+
+def test_statistical_vs_libm_oracle():
+    _say("water cube 1e6 decays: GPU vs libm oracle on an independent seed, Poisson totals, spectrum chi2, image z")
+    sc = water_cube_scenario(n_decays=1e6, tpb=64, seed=99, material_dose=True)
+    g = sc.run_gpu()
+    sc_o = water_cube_scenario(n_decays=1e6, tpb=64, seed=1234567, material_dose=True)
+    o = sc_o.run_oracle("libm")
+    for k in ("trues", "scatter"):
+        a, b = int(g[k].sum()), int(o[k].sum())
+        z = (a - b) / np.sqrt(a + b)
+        assert abs(z) < 4.0, f"{k}: gpu {a} oracle {b} z={z:.2f}"
+    ga, oa = g["spectrum"].astype(float), o["spectrum"].astype(float)
+    m = (ga + oa) > 20
+    chi2 = (((ga[m] - oa[m]) ** 2) / (ga[m] + oa[m])).sum()
+    p = stats.chi2.sf(chi2, m.sum() - 1)
+    assert p > 0.001, f"spectrum chi2={chi2:.1f} dof={m.sum() - 1} p={p:.2g}"
+    # per-bin z-scores of the emission image (coarse bins, well populated)
+    gi = g["image_trues"].astype(float)
+    oi = o["image_trues"].astype(float)
+    mm = (gi + oi) > 50
+    zz = (gi[mm] - oi[mm]) / np.sqrt(gi[mm] + oi[mm])
+    assert abs(zz.mean()) < 0.2 and 0.8 < zz.std() < 1.25, f"image z mean {zz.mean():.3f} std {zz.std():.3f}"
+
+
+def test_head_full_size_invariants():
+    """The bench workload (128x128x96 head, 2 mm, 1e9 decays in 10 s) on the GPU: size-independent
+    checks (trues/scatter sums equal image sums, photon count = 2 x decays with the expected
+    Poisson mean, counters consistent) plus a statistical match of the per-decay rates against
+    the oracle on a sample of the same phantom."""
+    from mcpet import phantoms
+    from conftest import MATERIALS
+    _say("head 128x128x96, 1e9 decays: invariants")
+    mat, dens, act, vs = phantoms.head()
+    names = phantoms.head_materials(MATERIALS)
+    cfg = phantoms.make_config(seed=20261203, threads_per_block=32, acquisition_time_s=10.0, nrad=168, materials=names)
+    ph = mcpet.phantom_from_arrays(mat, dens, act, vs)
+    tab = mcpet.load_materials(names, ph.density_max())
+    with mcpet.Simulation(cfg, ph, tab) as sim:
+        sim.run()
+        out = sim.fetch(psf=False)
+        st = sim.stats()
+    decays = st["decays"]
+    assert out["total_histories"] == 2 * decays
+    expected = float(act.astype(np.float64).sum()) * phantoms.F18_MEAN_LIFE * (1 - np.exp(-10.0 / phantoms.F18_MEAN_LIFE))
+    assert abs(decays - expected) < 6 * np.sqrt(expected), (decays, expected)
+    assert out["trues"].sum() == out["image_trues"].sum()
+    assert out["scatter"].sum() == out["image_scatter"].sum()
+    assert st["coincidences"] == out["trues"].sum() + out["scatter"].sum()
+    assert out["spectrum"].sum() <= 2 * st["pairs_tallied"]
+    # oracle on every 499th voxel of the same phantom and seed
+    sc_o = oracle_py.Oracle(phantoms.config_dict(cfg), ph.voxel_mat_dens(), ph.shape, vs,
+                            oracle_py.OracleTables(names + [""] * (15 - len(names)), ph.density_max(), "libm"), "libm")
+    o = sc_o.run(nthreads=16, voxel_stride=499)
+    od = o["counters"]["decays"]
+    for key in ("coincidences", "compton", "woodcock_steps"):
+        rg, ro = st[key] / decays, o["counters"][key] / od
+        assert abs(rg - ro) / ro < 0.05, f"{key}/decay gpu {rg:.4f} oracle-sample {ro:.4f}"
