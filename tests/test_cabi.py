@@ -1,0 +1,44 @@
+"""The C ABI library loads and exports every function include/mcpet.h declares; the struct
+layouts seen from Python match the header (no device calls)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import mcpet
+from conftest import ROOT
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "mcpet.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(mcpet_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_every_declared_symbol_is_exported():
+    names = declared_functions()
+    assert len(names) >= 20
+    out = subprocess.run(["nm", "-D", "--defined-only", mcpet.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (mcpet_[a-z0-9_]+)", out))
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    lib = mcpet.lib()
+    for n in names:
+        assert getattr(lib, n) is not None
+
+
+def test_struct_layouts():
+    assert ctypes.sizeof(mcpet.PsfRecord) == 44
+    assert ctypes.sizeof(mcpet.ComptonTable) == 5460
+    assert ctypes.sizeof(mcpet.RayleighTable) == 4 * 4 * 128 * 15 + 4 * 25005 * 15 + 2 * 128 * 15
+    assert mcpet.version().startswith("mcpet-mi355x")
+
+
+def test_create_rejects_bad_arguments_without_touching_the_device():
+    cfg = mcpet.Config()
+    ph = mcpet.Phantom()
+    tab = mcpet.Tables()
+    ctx = ctypes.c_void_p()
+    st = mcpet.lib().mcpet_create(ctypes.byref(cfg), ctypes.byref(ph), ctypes.byref(tab), ctypes.byref(ctx))
+    assert st == 1 and not ctx.value
+    assert b"null argument" in mcpet.lib().mcpet_last_error(None)

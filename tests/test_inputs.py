@@ -1,0 +1,239 @@
+"""The file-format boundary: .in parser, .vox loader, material table builder and output writers
+(MCGPU-PET.cu:866-1831, 544-616, 2389-2650, 3193-3322), checked against the reference's own
+sample fixtures, known answers derived from them, an independent restatement in the oracle,
+and closed forms of the table arithmetic."""
+import ctypes
+import gzip
+import hashlib
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import mcpet
+import oracle_py
+from conftest import AIR, MATERIALS, ROOT, SAMPLE, WATER, names15
+from mcpet import phantoms
+
+
+def test_read_sample_input():
+    cfg = mcpet.read_input(os.path.join(SAMPLE, "MCGPU-PET.in"))
+    assert cfg.seed != 0                       # 0 in the file -> time-based seed (C:920-928)
+    assert (cfg.gpu_id, cfg.threads_per_block) == (0, 32)
+    assert cfg.fact_object == np.float32(1.0)
+    assert cfg.acquisition_time_ps == 1_000_000_000_000
+    assert cfg.mean_life == np.float32(70000.0)
+    assert cfg.psf_file == b"MCGPU_PET.psf"
+    d = cfg.detector
+    assert list(d.center) == [0.0, 0.0, 0.0]
+    assert d.height == np.float32(12.656) and d.radius == np.float32(-9.05)
+    assert (d.psf_size, d.tally_type, d.tally_psf_sinogram) == (150_000_000, 0, 0)
+    assert cfg.material_dose == 1 and cfg.voxel_dose == 0
+    assert (cfg.roi_x_min, cfg.roi_x_max) == (32500, -32500)
+    assert (cfg.e_resol, cfg.e_low, cfg.e_high) == (np.float32(0.12), np.float32(350000.0), np.float32(600000.0))
+    assert cfg.fovz == np.float32(12.656)
+    assert (cfg.nrows, cfg.ncrystals, cfg.nangles, cfg.nrad, cfg.nzs, cfg.res, cfg.ne, cfg.mrd, cfg.span) == \
+           (80, 336, 168, 147, 159, 128, 700, 79, 11)
+    assert (cfg.nseg, cfg.nsinos, cfg.nbins) == (15, 1293, 31_931_928)
+    assert cfg.voxel_file == b"phantom_9x9x9cm.vox"
+    assert cfg.material_files[0].value == b"./materials/air_5-515keV.mcgpu.gz"
+    assert cfg.material_files[1].value == b"./materials/water_5-515keV.mcgpu.gz"
+    assert all(cfg.material_files[i].value == b"" for i in range(2, 15))
+
+
+def _distinct_izm(o, nrows, mrd):
+    vals = set()
+    for iz1 in range(nrows):
+        for iz2 in range(nrows):
+            if abs(iz1 - iz2) <= mrd:
+                b = o.sinogram_bin(10, iz1, 10 + o.p["nangles"], iz2)
+                assert b >= 0
+                vals.add(b // (o.p["nangles"] * o.p["nrad"]))
+    return vals
+
+
+@pytest.mark.parametrize("nrows,mrd,span", [(80, 79, 11), (20, 19, 3), (24, 11, 5), (16, 13, 9), (33, 17, 7), (40, 37, 5)])
+def test_nsinos_equals_number_of_michelogram_planes(nrows, mrd, span):
+    """NSINOS (C:1176-1189) equals the number of distinct axial indices izm the binning
+    (K:1758-1773) produces over all ring pairs within MRD, and they are 0..NSINOS-1.
+    Holds for odd SPAN >= 3 and MRD <= (NSEG-1)/2*SPAN + (SPAN-1)/2, i.e. when the last
+    segment is complete (the sample scanner: 80 rings, MRD 79, SPAN 11)."""
+    assert mrd <= ((2 * (mrd // span) + 1) - 1) // 2 * span + (span - 1) // 2
+    cfg = phantoms.make_config(nrows=nrows, nzs=2 * nrows - 1, mrd=mrd, span=span, nrad=31, nangles=168, ncrystals=336)
+    nseg, nsinos, nbins = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    oracle_py.load("libm").orc_sinogram_sizes(mrd, span, 2 * nrows - 1, 31, 168, ctypes.byref(nseg),
+                                              ctypes.byref(nsinos), ctypes.byref(nbins))
+    assert (cfg.nseg, cfg.nsinos, cfg.nbins) == (nseg.value, nsinos.value, nbins.value)
+    mat, dens, act, vs = phantoms.water_cube(n_decays=10)
+    tabs = oracle_py.OracleTables(names15([AIR, WATER]), np.array([-999, 1.0] + [-999] * 13, np.float32), "libm")
+    o = oracle_py.Oracle(phantoms.config_dict(cfg), np.stack([mat.ravel() + 1e-4, dens.ravel(), act.ravel()], 1),
+                         mat.shape, vs, tabs, "libm")
+    vals = _distinct_izm(o, nrows, mrd)
+    assert vals == set(range(cfg.nsinos))
+
+
+def test_load_sample_voxels():
+    cfg = mcpet.read_input(os.path.join(SAMPLE, "MCGPU-PET.in"))
+    ph = mcpet.load_voxels(os.path.join(SAMPLE, "phantom_9x9x9cm.vox"), cfg)
+    v = ph.voxel_mat_dens()
+    assert ph.shape == (9, 9, 9) and len(v) == 729
+    mat = v[:, 0].astype(np.int32)
+    assert (mat == 1).sum() == 605 and (mat == 2).sum() == 124
+    assert np.all(v[:, 0] == (mat.astype(np.float32) + np.float32(0.0001)))
+    act = v[:, 2]
+    assert act.sum() == 12050.0 and (act == 50).sum() == 121
+    idx = lambda i, j, k: i + 9 * j + 81 * k
+    assert (act[idx(3, 3, 4)], act[idx(5, 3, 4)], act[idx(5, 5, 4)], act[idx(3, 5, 4)]) == (1000, 2000, 3000, 0)
+    assert mat[idx(3, 5, 4)] == 1
+    dm = ph.density_max()
+    assert dm[0] == np.float32(0.0012) and dm[1] == np.float32(1.0) and np.all(dm[2:] == -999.0)
+    assert list(ph.ph.size_bbox) == [9.0, 9.0, 9.0] and list(ph.ph.inv_voxel_size) == [1.0, 1.0, 1.0]
+
+
+def test_sample_vox_writer_is_byte_identical(tmp_path):
+    p = tmp_path / "cube.vox"
+    phantoms.write_sample_vox(str(p))
+    ref = open(os.path.join(SAMPLE, "phantom_9x9x9cm.vox"), "rb").read()
+    assert hashlib.md5(ref).hexdigest() == "835c9bb86a8db9971d61dc6c5fae43f0"
+    assert p.read_bytes() == ref
+
+
+def test_vox_round_trip_gz(tmp_path):
+    mat, dens, act, vs = phantoms.head(shape=(6, 8, 7), voxel_cm=3.3, n_decays=1e4)
+    p = tmp_path / "h.vox"
+    phantoms.write_vox(str(p), mat, dens, act, vs)
+    with open(p, "rb") as f, gzip.open(str(p) + ".gz", "wb") as g:
+        g.write(f.read())
+    for path in (str(p), str(p) + ".gz"):
+        ph = mcpet.load_voxels(path)
+        v = ph.voxel_mat_dens()
+        assert np.array_equal(v[:, 0].astype(np.int32), mat.ravel())
+        assert np.array_equal(v[:, 1], dens.ravel()) and np.array_equal(v[:, 2], act.ravel())
+
+
+def _raw_mfp(path):
+    rows = []
+    with gzip.open(path, "rt") as f:
+        for line in f:
+            if line.startswith("#"):
+                if rows:
+                    break
+                continue
+            rows.append([float(x) for x in line.split()])
+    return np.array(rows)
+
+
+def test_tables_bit_identical_to_oracle_and_closed_forms():
+    dm = np.full(15, -999.0, np.float32)
+    dm[0], dm[1], dm[2] = 0.0012, 1.85, 1.0      # air, water as bone, water
+    names = [AIR, WATER, WATER]
+    t = mcpet.load_materials(names, dm).arrays()
+    o = oracle_py.OracleTables(names15(names), dm, "libm")
+    assert t["num_values"] == o.num_values == 10200
+    assert t["e0"] == o.e0 == 5000.0 and t["ide"] == o.ide == np.float32(1.0 / 50.0)
+    for k in ("woodcock", "mfp_a", "mfp_b"):
+        assert np.array_equal(t[k].view(np.uint32), getattr(o, k).view(np.uint32)), k
+    assert int(np.floor((np.float32(510998.95) - np.float32(t["e0"])) * np.float32(t["ide"]))) == 10119
+    E = 5000.0 + 50.0 * np.arange(10200)
+    for slot, path, rho in ((0, AIR, 0.00120479), (1, WATER, 1.0), (2, WATER, 1.0)):
+        raw = _raw_mfp(path)
+        assert raw.shape == (10200, 6) and np.allclose(raw[:, 0], E)
+        for c, col in ((0, 4), (1, 2), (2, 1)):     # total, Compton, Rayleigh
+            val = t["mfp_a"][:, slot, c].astype(np.float64) + E * t["mfp_b"][:, slot, c]
+            want = 1.0 / (raw[:, col] * rho)
+            assert np.max(np.abs(val / want - 1)) < 2e-5, (slot, c)
+    # Woodcock: minimum over used slots of mfp_total * rho_nominal / rho_max (C:1682-1684)
+    raws = {p: _raw_mfp(p) for p in (AIR, WATER)}
+    wood = np.minimum.reduce([raws[AIR][:, 4] * 0.00120479 / 0.0012, raws[WATER][:, 4] / 1.85, raws[WATER][:, 4] / 1.0])
+    val = t["woodcock"][:, 0].astype(np.float64) + E * t["woodcock"][:, 1]
+    assert np.max(np.abs(val[:-1] / wood[:-1] - 1)) < 2e-5
+
+
+@pytest.mark.parametrize("kind", ["missing", "nosection", "badyes", "roi"])
+def test_read_input_errors(tmp_path, kind):
+    src = open(os.path.join(SAMPLE, "MCGPU-PET.in")).read()
+    if kind == "missing":
+        path = str(tmp_path / "nope.in")
+    else:
+        if kind == "nosection":
+            src = src.replace("SECTION ENERGY PARAMETERS", "SECTION ENERGY PARAM")
+        elif kind == "badyes":
+            src = src.replace("YES                             # TALLY MATERIAL", "MAYBE                           # TALLY MATERIAL")
+        elif kind == "roi":
+            src = src.replace("NO                              # TALLY 3D VOXEL", "YES                             # TALLY 3D VOXEL")
+            src = src.replace("  1  128                        # VOXEL DOSE ROI: X", "  9  3                          # VOXEL DOSE ROI: X")
+        path = str(tmp_path / "x.in")
+        open(path, "w").write(src)
+    with pytest.raises(mcpet.McpetError) as e:
+        mcpet.read_input(path)
+    assert e.value.status == {"missing": 2, "nosection": 3, "badyes": 3, "roi": 4}[kind]
+
+
+@pytest.mark.parametrize("bad,status", [("0 1.0 5.0", 4), ("16 1.0 5.0", 4), ("2 0.0 5.0", 4), ("2 1.0", 3), (None, 3)])
+def test_load_voxels_errors(tmp_path, bad, status):
+    lines = open(os.path.join(SAMPLE, "phantom_9x9x9cm.vox")).read().splitlines(True)
+    first = next(i for i, l in enumerate(lines) if l.startswith("1 0.0012"))
+    if bad is None:
+        lines = lines[:first + 100]          # truncated file
+    else:
+        lines[first + 5] = bad + "\n"
+    p = tmp_path / "bad.vox"
+    p.write_text("".join(lines))
+    with pytest.raises(mcpet.McpetError) as e:
+        mcpet.load_voxels(str(p))
+    assert e.value.status == status
+
+
+def test_material_errors(tmp_path):
+    dm = np.full(15, -999.0, np.float32)
+    dm[0] = 1.0
+    with pytest.raises(mcpet.McpetError) as e:
+        mcpet.load_materials([str(tmp_path / "none.gz")], dm)
+    assert e.value.status == 2
+    txt = gzip.open(WATER, "rt").read().replace("#    128\n", "#    127\n")
+    p = tmp_path / "w.mcgpu.gz"
+    with gzip.open(p, "wt") as f:
+        f.write(txt)
+    with pytest.raises(mcpet.McpetError) as e:
+        mcpet.load_materials([str(p)], dm)
+    assert e.value.status == 3
+
+
+def test_output_writers(tmp_path):
+    cfg = phantoms.make_config(materials=[AIR, WATER], tally_psf_sinogram=0, psf_size=10, nrad=15, nangles=12,
+                               ncrystals=24, nrows=4, nzs=7, mrd=3, span=1, ne=50)
+    mat, dens, act, vs = phantoms.water_cube(n_decays=10)
+    ph = mcpet.phantom_from_arrays(mat, dens, act, vs)
+    tab = mcpet.load_materials([AIR, WATER], ph.density_max())
+    rng = np.random.default_rng(0)
+    psf = np.zeros(3, mcpet.PSF_DTYPE)
+    psf["emission_time_ps"] = [5, 6, 7]
+    psf["energy"] = [511e3, 400e3, 300e3]
+    t = {"trues": rng.integers(0, 9, cfg.nbins, dtype=np.int32), "scatter": rng.integers(0, 9, cfg.nbins, dtype=np.int32),
+         "image_trues": rng.integers(0, 9, 729, dtype=np.int32), "image_scatter": rng.integers(0, 9, 729, dtype=np.int32),
+         "spectrum": np.arange(50, dtype=np.int32), "psf": psf, "total_histories": 1234}
+    mcpet.write_outputs(str(tmp_path), cfg, ph, tab, t, 1.5)
+    for name, key in (("sinogram_Trues.raw.gz", "trues"), ("sinogram_Scatter.raw.gz", "scatter"),
+                      ("image_Trues.raw.gz", "image_trues"), ("image_Scatter.raw.gz", "image_scatter")):
+        got = np.frombuffer(gzip.open(tmp_path / name).read(), np.int32)
+        assert np.array_equal(got, t[key])
+    lines = open(tmp_path / "Energy_Sinogram_Spectrum.dat").read().splitlines()
+    assert lines[0].startswith("# HISTOGRAM OF ENERGIES") and len(lines) == 52
+    assert lines[2] == "%10d  %10d " % (0, 0) and lines[51] == "%10d  %10d " % (49, 49)
+    raw = (tmp_path / "MCGPU_PET.psf.raw").read_bytes()
+    assert len(raw) == 3 * 44 and np.array_equal(np.frombuffer(raw, mcpet.PSF_DTYPE), psf)
+    txt = open(tmp_path / "MCGPU_PET.psf").read()
+    assert "Number of detections in the PSF = 3" in txt and "\n5 0.000000 0 511000.000000" in txt
+    cfg.bg_mode = 1
+    mcpet.write_outputs(str(tmp_path), cfg, ph, tab, t, 1.5)
+    for n in ("sinogram_BG_Trues.raw.gz", "sinogram_BG_Scatter.raw.gz", "BG_image_True.raw.gz", "BG_image_Scatter.raw.gz"):
+        assert (tmp_path / n).exists()
+
+
+def test_cli_missing_input_exit_code(tmp_path):
+    exe = os.path.join(ROOT, "mcgpu-pet_amd", "bin", "mcgpu_pet")
+    r = subprocess.run([exe, str(tmp_path / "none.in")], capture_output=True, text=True)
+    assert r.returncode == 255 and "not found" in r.stdout
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 255
