@@ -73,6 +73,8 @@ def main():
     ap.add_argument("--kernel", choices=["persistent", "plain"], default="persistent")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--block-size", type=int, default=0)
+    ap.add_argument("--phase-policy", type=int, default=0, help="0 auto, 1 most lanes, 2 least waste")
+    ap.add_argument("--phase-cost", type=str, default="", help="5 comma-separated relative phase costs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-stride", type=int, default=5)
@@ -92,7 +94,8 @@ def main():
     tab = mcpet.load_materials(names, ph.density_max())
     kernel = mcpet.KERNEL_PERSISTENT if a.kernel == "persistent" else mcpet.KERNEL_PLAIN
     sim = mcpet.Simulation(cfg, ph, tab)
-    run_kw = dict(kernel=kernel, block_size=a.block_size, blocks_per_cu=a.blocks_per_cu)
+    run_kw = dict(kernel=kernel, block_size=a.block_size, blocks_per_cu=a.blocks_per_cu, phase_policy=a.phase_policy,
+                  phase_cost=[int(x) for x in a.phase_cost.split(",")] if a.phase_cost else None)
     for w in range(a.warmup):
         sim.run(seed=1000 + w, **run_kw)
         sim.sync()
@@ -107,7 +110,8 @@ def main():
         coinc += st["coincidences"]
         decays += st["decays"]
         kernel_ms.append(st["kernel_ms"])
-        agg = st if agg is None else {k: agg[k] + st[k] for k in agg}
+        agg = st if agg is None else {k: (agg[k] + st[k]) if not isinstance(st[k], list) else
+                                      [x + y for x, y in zip(agg[k], st[k])] for k in agg}
     sim.sync()
     elapsed = time.perf_counter() - t0
     sim.close()
@@ -137,6 +141,10 @@ def main():
                                                                    "rayleigh", "photoelectric", "coincidences",
                                                                    "rn_draws")},
     }
+    if any(agg["phase_exec"]):
+        out["phase_stats"] = {n: {"exec_per_decay": e / max(1, decays), "mean_lanes": l / max(1, e)}
+                              for n, e, l in zip(("decay", "source2", "step", "interact", "tally"),
+                                                 agg["phase_exec"], agg["phase_lanes"])}
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, ph, vs, names, a.cpu_threads, a.cpu_stride)
     print(json.dumps(out))

@@ -168,6 +168,8 @@ typedef struct mcpet_run_params {
   int32_t zero_tallies;          /* 1: clear tallies before the run (default)          */
   int32_t seed;                  /* 0: use the config seed                             */
   uint64_t acquisition_time_ps;  /* 0: use the config value                            */
+  int32_t phase_policy;          /* persistent kernel: 0 auto, 1 most lanes, 2 least waste */
+  int32_t phase_cost[5];         /* relative phase costs for policy 2 (0: built-in)    */
 } mcpet_run_params;
 
 /* Per-run statistics: HIP-event kernel time and algorithmic event counts. */
@@ -185,6 +187,10 @@ typedef struct mcpet_stats {
   uint64_t pairs_tallied;        /* coincidences reaching tally_Sinogram/PSF             */
   uint64_t coincidences;         /* trues + scatter binned                               */
   uint64_t rn_draws;
+  /* persistent kernel, diagnostic builds only: executions and active lanes per phase
+     (DECAY, SOURCE2, STEP, INTERACT, TALLY); zero otherwise */
+  uint64_t phase_exec[5];
+  uint64_t phase_lanes[5];
 } mcpet_stats;
 
 /* ------------------------------------------------------------------ host-side input (boundary) */

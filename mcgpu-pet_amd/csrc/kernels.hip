@@ -35,6 +35,29 @@
 // corrupt input. The CPU checker applies the same bound.
 #define K_MAX_TRIALS (1 << 20)
 
+// Occupancy hint for the transport kernel (waves per SIMD); 0 leaves it to the compiler.
+#ifndef MCPET_WAVES_PER_EU
+#define MCPET_WAVES_PER_EU 0
+#endif
+#if MCPET_WAVES_PER_EU > 0
+#define K_OCCUPANCY __attribute__((amdgpu_waves_per_eu(MCPET_WAVES_PER_EU, 8)))
+#else
+#define K_OCCUPANCY
+#endif
+// Rarely executed physics (Compton, Rayleigh, tallies) as real calls instead of inlined code.
+// Per-phase execution statistics of the persistent kernel (diagnostic builds only).
+#ifndef MCPET_SM_STATS
+#define MCPET_SM_STATS 0
+#endif
+#ifndef MCPET_NOINLINE_RARE
+#define MCPET_NOINLINE_RARE 0
+#endif
+#if MCPET_NOINLINE_RARE
+#define K_RARE __device__ __noinline__
+#else
+#define K_RARE __device__
+#endif
+
 namespace {
 
 struct Rng {
@@ -129,7 +152,7 @@ __device__ __forceinline__ void rotate_double(float* d, double costh, double phi
 }
 
 // GRAa (K:1230-1296): Rayleigh angle by RITA inverse cumulative + Thomson rejection.
-__device__ void graa(float energy, double* costh, int mat, float pmax_current, Rng& r, Counters& C,
+K_RARE void graa(float energy, double* costh, int mat, float pmax_current, Rng& r, Counters& C,
                      const mcpet_rayleigh_table* __restrict__ cgra)
 {
   double xmax = ((double)energy) * 8.065535669099010e-5;
@@ -209,10 +232,10 @@ __device__ __forceinline__ float shell_term(const mcpet_compton_table& cg, int m
 }
 
 // GCOa (K:1334-1563): Compton scattering, relativistic impulse approximation with
-// analytical one-electron profiles. The per-shell values rn[i] of the last trial are
-// recomputed when the shell is selected instead of being kept in a 30-entry array
-// (same values, no scratch memory).
-__device__ void gcoa(float* energy, double* costh, int mat, Rng& r, Counters& C, const mcpet_compton_table& cg)
+// analytical one-electron profiles. The per-shell values rn[i] of the accepted trial are
+// kept in registers for the first four shells and recomputed for later shells instead of
+// a 30-entry array (same values, no scratch memory).
+K_RARE void gcoa(float* energy, double* costh, int mat, Rng& r, Counters& C, const mcpet_compton_table& cg)
 {
   float s, a1, s0, af, ek, ek2, ek3, tau, pzomc, taumin;
   double cdt1;
@@ -230,6 +253,12 @@ __device__ void gcoa(float* energy, double* costh, int mat, Rng& r, Counters& C,
     if (cg.uico[mat + i * K_MAXMAT] < E) s0 += cg.fco[mat + i * K_MAXMAT] * shell_term(cg, mat, i, E, 0.0f, false);
   }
   float cdtf;
+  // rn[] of the accepted trial: shells 0..3 kept in registers, later shells recomputed.
+  float rc0 = 0.0f, rc1 = 0.0f, rc2 = 0.0f, rc3 = 0.0f;
+  auto rnv = [&](int i) -> float {
+    if (i < 4) return i == 0 ? rc0 : i == 1 ? rc1 : i == 2 ? rc2 : rc3;
+    return shell_term(cg, mat, i, E, cdtf, true);
+  };
   int trials = 0;
   do {
     if (rn(r, C) * (a1 + 2. * ek * (ek + 1.f) * taumin * taumin) < a1)
@@ -241,8 +270,16 @@ __device__ void gcoa(float* energy, double* costh, int mat, Rng& r, Counters& C,
     cdtf = (float)cdt1;
     s = 0.0f;
     C.c[KC_SHELLEVAL] += (unsigned)my_noscco;
+    rc0 = rc1 = rc2 = rc3 = 0.0f;
     for (int i = 0; i < my_noscco; i++) {
-      if (cg.uico[mat + i * K_MAXMAT] < E) s += cg.fco[mat + i * K_MAXMAT] * shell_term(cg, mat, i, E, cdtf, true);
+      if (cg.uico[mat + i * K_MAXMAT] < E) {
+        const float t = shell_term(cg, mat, i, E, cdtf, true);
+        s += cg.fco[mat + i * K_MAXMAT] * t;
+        rc0 = (i == 0) ? t : rc0;
+        rc1 = (i == 1) ? t : rc1;
+        rc2 = (i == 2) ? t : rc2;
+        rc3 = (i == 3) ? t : rc3;
+      }
     }
   } while ((rn(r, C) * s0) > (s * (1.0f + tau * ((ek3 - ek2 - 1.0f) + tau * (ek2 + tau * ek3))) / (ek3 * tau * (tau * tau + 1.0f))) &&
            ++trials < K_MAX_TRIALS);
@@ -255,13 +292,13 @@ __device__ void gcoa(float* energy, double* costh, int mat, Rng& r, Counters& C,
     float pac = 0.0f;
     int ishell = my_noscco - 1;
     for (int i = 0; i < (my_noscco - 1); i++) {
-      pac += cg.fco[mat + i * K_MAXMAT] * shell_term(cg, mat, i, E, cdtf, true);
+      pac += cg.fco[mat + i * K_MAXMAT] * rnv(i);
       if (pac > temp) {
         ishell = i;
         break;
       }
     }
-    temp = rn(r, C) * shell_term(cg, mat, ishell, E, cdtf, true);
+    temp = rn(r, C) * rnv(ishell);
     const float fj = cg.fj0[mat + ishell * K_MAXMAT];
     if (temp < 0.5f)
       pzomc = (0.70710678118654502f - sqrtf(0.5f - mcf_logf(temp + temp))) / (fj * 1.4142135623731f);
@@ -354,7 +391,7 @@ __device__ __forceinline__ void spectrum_add(const KParams& P, int* s_spec, int 
 }
 
 // tally_Sinogram (K:1623-1836).
-__device__ void tally_sinogram(const KParams& P, float energy, float energyFirst, const float* pos, const float* posf,
+K_RARE void tally_sinogram(const KParams& P, float energy, float energyFirst, const float* pos, const float* posf,
                                const float* dir, const float* dirf, int ss, int ssf, Stream& st, Counters& C, int* s_spec)
 {
   float x0 = pos[0] - P.cx, y0 = pos[1] - P.cy;
@@ -418,7 +455,7 @@ __device__ __forceinline__ void write_psf(mcpet_psf_record* rec, unsigned long l
 }
 
 // tally_PSF_coincidences (K:530-644).
-__device__ void tally_psf(const KParams& P, float energy, float energyFirst, const float* pos_in, const float* posf_in,
+K_RARE void tally_psf(const KParams& P, float energy, float energyFirst, const float* pos_in, const float* posf_in,
                           const float* dir, const float* dirf, int ss, int ssf, float travel, float travelFirst, Stream& st,
                           Counters& C)
 {
@@ -487,77 +524,109 @@ __device__ __forceinline__ void tally_dose(const KParams& P, float Edep, int mat
   }
 }
 
-// Photon transport until escape or absorption (K:297-415). Returns the final energy
-// table index (< 0 = absorbed) and updates position/direction/scatter state/travel.
-__device__ int transport_photon(const KParams& P, float& energy, float* position, float* direction, int& scatter_state,
-                                float& travel_distance, Stream& st, Counters& C, const mcpet_compton_table& cg,
-                                unsigned long long* s_mat)
+// ---------------------------------------------------------------------------------- photon transport
+// Transport is split into three per-lane building blocks so that the same arithmetic
+// serves the nested (plain) kernel and the state-machine (persistent) kernel:
+// photon_start (energy bin, Woodcock mean free path), woodcock_step (one delta-tracking
+// step, K:319-342) and real_interaction (K:348-413).
+
+struct Photon {
+  float pos[3], dir[3];
+  float E, travel, mfpW, mfp_density, prob, randno;
+  float2 tab;            // (a, b) of the total cross section at (index, mat_old)
+  int index, mat_old, mat0, ss;
+};
+
+__device__ __forceinline__ void photon_start(const KParams& P, Photon& ph)
 {
-  int index = f2i_rd((energy - P.e0) * P.ide);
-  float2 w = P.woodcock[index];
-  float mfp_Woodcock = w.x + energy * w.y;
-  int material_old = -1, material0 = 0, absvox;
-  float prob = 0.0f, randno = 0.0f, mfp_density = 0.0f;
-  float2 tab = make_float2(0.0f, 0.0f);
-  for (;;) {
-    absvox = locate(P, position);
-    if (absvox < 0) break;
-    do {
-      float step = -(mfp_Woodcock)*mcf_logf(rn(st.r, C));
-      C.c[KC_WOODCOCK]++;
-      travel_distance += step;
-      position[0] += step * direction[0];
-      position[1] += step * direction[1];
-      position[2] += step * direction[2];
-      absvox = locate(P, position);
-      if (absvox < 0) break;
-      float2 md = P.vox[absvox];
-      material0 = __float_as_int(md.y);
-      if (material0 != material_old) {
-        tab = P.tot[index * K_MAXMAT + material0];
-        material_old = material0;
-        C.c[KC_MATCHANGE]++;
-      }
-      mfp_density = mfp_Woodcock * md.x;
-      prob = 1.0f - mfp_density * (tab.x + energy * tab.y);
-      randno = rn(st.r, C);
-    } while (randno < prob);
-    if (absvox < 0) break;
-    const float4 cr = P.cr[index * K_MAXMAT + material0];
-    prob += mfp_density * (cr.x + energy * cr.y);
-    if (randno < prob) {
-      double costh_Compton;
-      randno = energy;
-      C.c[KC_COMPTON]++;
-      gcoa(&energy, &costh_Compton, material0, st.r, C, cg);
-      rotate_double(direction, costh_Compton, 6.28318530717958647693 * rnd(st.r, C));
-      randno = energy - randno;
-      index = f2i_rd((energy - P.e0) * P.ide);
-      if (index > -1) {
-        w = P.woodcock[index];
-        mfp_Woodcock = w.x + energy * w.y;
-        material_old = -2;
-        scatter_state = (scatter_state == 0) ? 1 : 3;
-      }
-    } else {
-      prob += mfp_density * (cr.z + energy * cr.w);
-      if (randno < prob) {
-        double costh_Rayleigh;
-        C.c[KC_RAYLEIGH]++;
-        float pmax_current = P.ray->pmax[(index + 1) * K_MAXMAT + material0];
-        graa(energy, &costh_Rayleigh, material0, pmax_current, st.r, C, P.ray);
-        rotate_double(direction, costh_Rayleigh, 6.28318530717958647693 * rnd(st.r, C));
-        scatter_state = (scatter_state == 0) ? 2 : 3;
-      } else {
-        C.c[KC_PHOTO]++;
-        randno = -energy;
-        index = -11;
-      }
-    }
-    if (randno < -0.001f) tally_dose(P, -1.0f * randno, material0, position, s_mat);
-    if (index < 0) break;
+  ph.E = K_E511;
+  ph.ss = 0;
+  ph.travel = 0.0f;
+  ph.index = f2i_rd((ph.E - P.e0) * P.ide);
+  const float2 w = P.woodcock[ph.index];
+  ph.mfpW = w.x + ph.E * w.y;
+  ph.mat_old = -1;
+}
+
+enum StepOutcome { STEP_ESCAPED = 0, STEP_VIRTUAL = 1, STEP_REAL = 2 };
+
+__device__ __forceinline__ int woodcock_step(const KParams& P, Photon& ph, Rng& r, Counters& C)
+{
+  const float step = -(ph.mfpW)*mcf_logf(rn(r, C));
+  C.c[KC_WOODCOCK]++;
+  ph.travel += step;
+  ph.pos[0] += step * ph.dir[0];
+  ph.pos[1] += step * ph.dir[1];
+  ph.pos[2] += step * ph.dir[2];
+  const int absvox = locate(P, ph.pos);
+  if (absvox < 0) return STEP_ESCAPED;
+  const float2 md = P.vox[absvox];
+  ph.mat0 = __float_as_int(md.y);
+  if (ph.mat0 != ph.mat_old) {
+    ph.tab = P.tot[ph.index * K_MAXMAT + ph.mat0];
+    ph.mat_old = ph.mat0;
+    C.c[KC_MATCHANGE]++;
   }
-  return index;
+  ph.mfp_density = ph.mfpW * md.x;
+  ph.prob = 1.0f - ph.mfp_density * (ph.tab.x + ph.E * ph.tab.y);
+  ph.randno = rn(r, C);
+  return (ph.randno < ph.prob) ? STEP_VIRTUAL : STEP_REAL;
+}
+
+// Real event at the photon's position: Compton, Rayleigh or photoelectric, with the dose
+// tallies. Returns false when the photon is absorbed (index < 0).
+__device__ __forceinline__ bool real_interaction(const KParams& P, Photon& ph, Rng& r, Counters& C, const mcpet_compton_table& cg,
+                                                 unsigned long long* s_mat)
+{
+  const float4 cr = P.cr[ph.index * K_MAXMAT + ph.mat0];
+  float prob = ph.prob + ph.mfp_density * (cr.x + ph.E * cr.y);
+  float randno = ph.randno;
+  if (randno < prob) {
+    double costh_Compton;
+    randno = ph.E;
+    C.c[KC_COMPTON]++;
+    gcoa(&ph.E, &costh_Compton, ph.mat0, r, C, cg);
+    rotate_double(ph.dir, costh_Compton, 6.28318530717958647693 * rnd(r, C));
+    randno = ph.E - randno;
+    ph.index = f2i_rd((ph.E - P.e0) * P.ide);
+    if (ph.index > -1) {
+      const float2 w = P.woodcock[ph.index];
+      ph.mfpW = w.x + ph.E * w.y;
+      ph.mat_old = -2;
+      ph.ss = (ph.ss == 0) ? 1 : 3;
+    }
+  } else {
+    prob += ph.mfp_density * (cr.z + ph.E * cr.w);
+    if (randno < prob) {
+      double costh_Rayleigh;
+      C.c[KC_RAYLEIGH]++;
+      const float pmax_current = P.ray->pmax[(ph.index + 1) * K_MAXMAT + ph.mat0];
+      graa(ph.E, &costh_Rayleigh, ph.mat0, pmax_current, r, C, P.ray);
+      rotate_double(ph.dir, costh_Rayleigh, 6.28318530717958647693 * rnd(r, C));
+      ph.ss = (ph.ss == 0) ? 2 : 3;
+    } else {
+      C.c[KC_PHOTO]++;
+      randno = -ph.E;
+      ph.index = -11;
+    }
+  }
+  if (randno < -0.001f) tally_dose(P, -1.0f * randno, ph.mat0, ph.pos, s_mat);
+  return ph.index >= 0;
+}
+
+// Whole photon (K:297-415), nested loops: escape or absorption.
+__device__ __forceinline__ void transport_photon(const KParams& P, Photon& ph, Stream& st, Counters& C,
+                                                 const mcpet_compton_table& cg, unsigned long long* s_mat)
+{
+  for (;;) {
+    if (locate(P, ph.pos) < 0) return;
+    int o;
+    do {
+      o = woodcock_step(P, ph, st.r, C);
+    } while (o == STEP_VIRTUAL);
+    if (o == STEP_ESCAPED) return;
+    if (!real_interaction(P, ph, st.r, C, cg, s_mat)) return;
+  }
 }
 
 // Pair emission at a uniform point of the stream's voxel (K:665-773).
@@ -613,50 +682,74 @@ __device__ __forceinline__ void source_second(const KParams& P, Stream& st, Coun
   dir[2] = -GX * ST1 + GZ * CT1;
 }
 
-// One annihilation of the stream (reference main loop body, K:259-449, both photons).
-// Returns false when the stream's acquisition time is exhausted.
+// Time to the next annihilation of the stream (K:261-292). False when the stream's
+// acquisition time is exhausted.
+__device__ __forceinline__ bool decay_time(const KParams& P, Stream& st, Counters& C)
+{
+  const double Dt = -mcf_log(rnd(st.r, C)) * st.inv_act;
+  const unsigned long long Dt_ps = d2ull_rn(Dt * 1e12);
+  if (!(Dt_ps < st.remaining)) return false;
+  st.remaining -= Dt_ps;
+  st.hist += 2;
+  C.c[KC_DECAYS]++;
+  st.inv_act = st.inv_act * mcf_exp(Dt * P.inv_mean_life);
+  return true;
+}
+
+// Pair state kept between the two photons of an annihilation (K:200-203, 418-441).
+struct Pair {
+  float p1[3], d1[3];              // emission point and photon-1 direction
+  float EF, travelF;               // photon 1 at escape (EF = -10 if absorbed)
+  float posF[3], dirF[3];
+  int ssF;
+};
+
+__device__ __forceinline__ bool pair_selected(const KParams& P, int ss, int ssF)
+{
+  return P.type == 0 || (P.type == 1 && ss == 0 && ssF == 0) || (P.type == 2 && (ss != 0 || ssF != 0));
+}
+
+__device__ __forceinline__ void tally_pair(const KParams& P, const Photon& ph, const Pair& pr, Stream& st, Counters& C, int* s_spec)
+{
+  if (P.mode == 0 || P.mode == 2) tally_sinogram(P, ph.E, pr.EF, ph.pos, pr.posF, ph.dir, pr.dirF, ph.ss, pr.ssF, st, C, s_spec);
+  if (P.mode == 0 || P.mode == 1)
+    tally_psf(P, ph.E, pr.EF, ph.pos, pr.posF, ph.dir, pr.dirF, ph.ss, pr.ssF, ph.travel, pr.travelF, st, C);
+}
+
+__device__ __forceinline__ void keep_first(Pair& pr, const Photon& ph)
+{
+  if (ph.index > -1) {
+    pr.EF = ph.E;
+    pr.travelF = ph.travel;
+    pr.ssF = ph.ss;
+    for (int k = 0; k < 3; k++) {
+      pr.posF[k] = ph.pos[k];
+      pr.dirF[k] = ph.dir[k];
+    }
+  }
+}
+
+// One annihilation, both photons, nested loops (reference main loop body, K:259-449).
 __device__ bool decay(const KParams& P, Stream& st, Counters& C, const mcpet_compton_table& cg, int* s_spec,
                       unsigned long long* s_mat)
 {
-  {
-    double Dt = -mcf_log(rnd(st.r, C)) * st.inv_act;
-    unsigned long long Dt_ps = d2ull_rn(Dt * 1e12);
-    if (!(Dt_ps < st.remaining)) return false;
-    st.remaining -= Dt_ps;
-    st.hist += 2;
-    C.c[KC_DECAYS]++;
-    st.inv_act = st.inv_act * mcf_exp(Dt * P.inv_mean_life);
+  if (!decay_time(P, st, C)) return false;
+  Pair pr;
+  Photon ph;
+  source_first(P, st, C, pr.p1, pr.d1);
+  pr.EF = -10.0f;
+  for (int k = 0; k < 3; k++) {
+    ph.pos[k] = pr.p1[k];
+    ph.dir[k] = pr.d1[k];
   }
-  // photon 1
-  float pos1[3], dir1[3];
-  source_first(P, st, C, pos1, dir1);
-  float position[3] = {pos1[0], pos1[1], pos1[2]};
-  float direction[3] = {dir1[0], dir1[1], dir1[2]};
-  float energy = K_E511;
-  int ss = 0;
-  float travel = 0.0f;
-  int index = transport_photon(P, energy, position, direction, ss, travel, st, C, cg, s_mat);
-  const bool first_alive = index > -1;
-  const float energyFirst = first_alive ? energy : -10.0f;
-  const float posF[3] = {position[0], position[1], position[2]};
-  const float dirF[3] = {direction[0], direction[1], direction[2]};
-  const int ssF = ss;
-  const float travelF = travel;
-  // photon 2
-  source_second(P, st, C, dir1, direction);
-  position[0] = pos1[0];
-  position[1] = pos1[1];
-  position[2] = pos1[2];
-  energy = K_E511;
-  ss = 0;
-  travel = 0.0f;
-  index = transport_photon(P, energy, position, direction, ss, travel, st, C, cg, s_mat);
-  if (index > -1 && energyFirst > -0.5f) {
-    if (P.type == 0 || (P.type == 1 && ss == 0 && ssF == 0) || (P.type == 2 && (ss != 0 || ssF != 0))) {
-      if (P.mode == 0 || P.mode == 2) tally_sinogram(P, energy, energyFirst, position, posF, direction, dirF, ss, ssF, st, C, s_spec);
-      if (P.mode == 0 || P.mode == 1) tally_psf(P, energy, energyFirst, position, posF, direction, dirF, ss, ssF, travel, travelF, st, C);
-    }
-  }
+  photon_start(P, ph);
+  transport_photon(P, ph, st, C, cg, s_mat);
+  keep_first(pr, ph);
+  source_second(P, st, C, pr.d1, ph.dir);
+  for (int k = 0; k < 3; k++) ph.pos[k] = pr.p1[k];
+  photon_start(P, ph);
+  transport_photon(P, ph, st, C, cg, s_mat);
+  if (ph.index > -1 && pr.EF > -0.5f && pair_selected(P, ph.ss, pr.ssF)) tally_pair(P, ph, pr, st, C, s_spec);
   return true;
 }
 
@@ -686,58 +779,165 @@ __device__ __forceinline__ void stream_finish(const KParams& P, Stream& st, unsi
   if (st.tid == P.last_thread_id) *P.final_seed = st.r.x;
 }
 
-template <bool PERSISTENT>
-__global__ __launch_bounds__(256) void k_transport(KParams P)
-{
-  __shared__ mcpet_compton_table s_cmp;
-  __shared__ unsigned long long s_mat[2 * K_MAXMAT];
-  extern __shared__ int s_spec[];
-  {
-    const int* src = reinterpret_cast<const int*>(P.cmp);
-    int* dst = reinterpret_cast<int*>(&s_cmp);
-    for (int i = threadIdx.x; i < (int)(sizeof(mcpet_compton_table) / 4); i += blockDim.x) dst[i] = src[i];
-    if (threadIdx.x < 2 * K_MAXMAT) s_mat[threadIdx.x] = 0ull;
-    if (P.lds_spectrum)
-      for (int i = threadIdx.x; i < P.ne; i += blockDim.x) s_spec[i] = 0;
-  }
-  __syncthreads();
-
-  Counters C;
-#pragma unroll
-  for (int i = 0; i < KC_N; i++) C.c[i] = 0;
+// Workgroup prologue/epilogue shared by the kernels: Compton table, spectrum and
+// material dose in LDS; counters flushed once per lane / per workgroup.
+#define K_PROLOGUE                                                                               \
+  __shared__ mcpet_compton_table s_cmp;                                                          \
+  __shared__ unsigned long long s_mat[2 * K_MAXMAT];                                             \
+  extern __shared__ int s_spec[];                                                                \
+  {                                                                                              \
+    const int* src = reinterpret_cast<const int*>(P.cmp);                                        \
+    int* dst = reinterpret_cast<int*>(&s_cmp);                                                   \
+    for (int i = threadIdx.x; i < (int)(sizeof(mcpet_compton_table) / 4); i += blockDim.x)       \
+      dst[i] = src[i];                                                                           \
+    if (threadIdx.x < 2 * K_MAXMAT) s_mat[threadIdx.x] = 0ull;                                   \
+    if (P.lds_spectrum)                                                                          \
+      for (int i = threadIdx.x; i < P.ne; i += blockDim.x) s_spec[i] = 0;                        \
+  }                                                                                              \
+  __syncthreads();                                                                               \
+  Counters C;                                                                                    \
+  _Pragma("unroll") for (int i = 0; i < KC_N; i++) C.c[i] = 0;                                   \
   unsigned long long hist_acc = 0;
-  Stream st;
-  bool have = false;
-  long long sid = PERSISTENT ? -1 : (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  bool used = false;
-  for (;;) {
-    if (!have) {
-      if (PERSISTENT) {
-        sid = (long long)atomicAdd(P.queue, 1ull);
-      } else if (used) {
-        break;
-      }
-      used = true;
-      if (sid >= P.n_streams) break;
-      stream_init(P, sid, st);
-      C.c[KC_STREAMS]++;
-      have = true;
+
+#define K_EPILOGUE                                                                               \
+  if (hist_acc) atomicAdd(P.total_hist, hist_acc);                                               \
+  _Pragma("unroll") for (int i = 0; i < KC_N; i++) if (C.c[i])                                   \
+    atomicAdd(&P.counters[i], (unsigned long long)C.c[i]);                                       \
+  __syncthreads();                                                                               \
+  if (P.lds_spectrum)                                                                            \
+    for (int i = threadIdx.x; i < P.ne; i += blockDim.x)                                         \
+      if (s_spec[i]) atomicAdd(&P.spectrum[i], s_spec[i]);                                       \
+  if (threadIdx.x < 2 * K_MAXMAT && s_mat[threadIdx.x]) atomicAdd(&P.mat_edep[threadIdx.x], s_mat[threadIdx.x]);
+
+// Plain kernel: lane i of the grid runs stream i to the end (nested loops, no queue).
+__global__ __launch_bounds__(256) K_OCCUPANCY void k_transport_plain(KParams P)
+{
+  K_PROLOGUE
+  const long long sid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sid < P.n_streams) {
+    Stream st;
+    stream_init(P, sid, st);
+    C.c[KC_STREAMS]++;
+    while (decay(P, st, C, s_cmp, s_spec, s_mat)) {
     }
-    if (!decay(P, st, C, s_cmp, s_spec, s_mat)) {
-      stream_finish(P, st, hist_acc);
-      have = false;
+    stream_finish(P, st, hist_acc);
+  }
+  K_EPILOGUE
+}
+
+// Persistent kernel: every lane is a small state machine over the phases of a decay
+//   DECAY    : (refill a stream from the device queue,) sample the decay time, emit photon 1
+//   STEP     : one Woodcock step of the current photon
+//   INTERACT : one real interaction (Compton / Rayleigh / photoelectric + dose)
+//   SOURCE2  : emit photon 2 of the pair
+//   TALLY    : detector, energy blur, sinogram / PSF tallies of the pair
+// and each iteration the wave executes the one phase held by the most lanes (ballot
+// counts). Lanes in other phases wait, so expensive phases (Compton rejection loops,
+// detector geometry) run with many lanes at once instead of one or two. A lane's own
+// sequence of random numbers and operations is unchanged.
+enum LaneState { LS_DECAY = 0, LS_SOURCE2 = 1, LS_STEP = 2, LS_INTERACT = 3, LS_TALLY = 4, LS_N = 5, LS_DONE = 7 };
+
+__device__ __forceinline__ int after_photon(const KParams& P, int photon_no, const Photon& ph, Pair& pr)
+{
+  if (photon_no == 1) {
+    keep_first(pr, ph);
+    return LS_SOURCE2;
+  }
+  if (ph.index > -1 && pr.EF > -0.5f && pair_selected(P, ph.ss, pr.ssF)) return LS_TALLY;
+  return LS_DECAY;
+}
+
+__global__ __launch_bounds__(256) K_OCCUPANCY void k_transport_sm(KParams P)
+{
+  K_PROLOGUE
+  Stream st;
+  Photon ph;
+  Pair pr;
+  bool have = false;
+  int photon_no = 1;
+  int state = LS_DECAY;
+  for (;;) {
+    int best = -1;
+    if (P.sm_policy == 0) {
+      int bestc = 0;
+#pragma unroll
+      for (int sidx = 0; sidx < LS_N; sidx++) {
+        const int c = __popcll(__ballot(state == sidx));
+        if (c > bestc) {
+          bestc = c;
+          best = sidx;
+        }
+      }
+    } else {
+      int bestw = 0x7fffffff;
+#pragma unroll
+      for (int sidx = 0; sidx < LS_N; sidx++) {
+        const int c = __popcll(__ballot(state == sidx));
+        const int w = (64 - c) * P.sm_cost[sidx];
+        if (c > 0 && w < bestw) {
+          bestw = w;
+          best = sidx;
+        }
+      }
+    }
+    best = __builtin_amdgcn_readfirstlane(best);
+    if (best < 0) break;
+#if MCPET_SM_STATS
+    {
+      const int lanes = __popcll(__ballot(state == best));
+      if (__lane_id() == 0) {
+        C.c[KC_PHASE_EXEC + best]++;
+        C.c[KC_PHASE_LANES + best] += lanes;
+      }
+    }
+#endif
+    if (state != best) continue;
+    if (best == LS_STEP) {
+      const int o = woodcock_step(P, ph, st.r, C);
+      if (o == STEP_ESCAPED) state = after_photon(P, photon_no, ph, pr);
+      else if (o == STEP_REAL) state = LS_INTERACT;
+    } else if (best == LS_INTERACT) {
+      if (!real_interaction(P, ph, st.r, C, s_cmp, s_mat) || locate(P, ph.pos) < 0) state = after_photon(P, photon_no, ph, pr);
+      else state = LS_STEP;
+    } else if (best == LS_DECAY) {
+      for (;;) {
+        if (!have) {
+          const long long sid = (long long)atomicAdd(P.queue, 1ull);
+          if (sid >= P.n_streams) {
+            state = LS_DONE;
+            break;
+          }
+          stream_init(P, sid, st);
+          C.c[KC_STREAMS]++;
+          have = true;
+        }
+        if (decay_time(P, st, C)) break;
+        stream_finish(P, st, hist_acc);
+        have = false;
+      }
+      if (state != LS_DONE) {
+        source_first(P, st, C, pr.p1, pr.d1);
+        pr.EF = -10.0f;
+        for (int k = 0; k < 3; k++) {
+          ph.pos[k] = pr.p1[k];
+          ph.dir[k] = pr.d1[k];
+        }
+        photon_start(P, ph);
+        photon_no = 1;
+        state = (locate(P, ph.pos) < 0) ? after_photon(P, 1, ph, pr) : LS_STEP;
+      }
+    } else if (best == LS_SOURCE2) {
+      source_second(P, st, C, pr.d1, ph.dir);
+      for (int k = 0; k < 3; k++) ph.pos[k] = pr.p1[k];
+      photon_start(P, ph);
+      photon_no = 2;
+      state = (locate(P, ph.pos) < 0) ? after_photon(P, 2, ph, pr) : LS_STEP;
+    } else {  // LS_TALLY
+      tally_pair(P, ph, pr, st, C, s_spec);
+      state = LS_DECAY;
     }
   }
-  // Flush per-lane counters (the atomic optimiser combines them per wave).
-  if (hist_acc) atomicAdd(P.total_hist, hist_acc);
-#pragma unroll
-  for (int i = 0; i < KC_N; i++)
-    if (C.c[i]) atomicAdd(&P.counters[i], (unsigned long long)C.c[i]);
-  __syncthreads();
-  if (P.lds_spectrum)
-    for (int i = threadIdx.x; i < P.ne; i += blockDim.x)
-      if (s_spec[i]) atomicAdd(&P.spectrum[i], s_spec[i]);
-  if (threadIdx.x < 2 * K_MAXMAT && s_mat[threadIdx.x]) atomicAdd(&P.mat_edep[threadIdx.x], s_mat[threadIdx.x]);
+  K_EPILOGUE
 }
 
 // ---------------------------------------------------------------------------------- self tests
@@ -793,9 +993,9 @@ hipError_t mcpet_launch_transport(const KParams& P, int kind, int blocks, int bl
 {
   size_t lds = P.lds_spectrum ? (size_t)P.ne * sizeof(int) : 0;
   if (kind == MCPET_KERNEL_PERSISTENT)
-    hipLaunchKernelGGL(k_transport<true>, dim3(blocks), dim3(block_size), lds, stream, P);
+    hipLaunchKernelGGL(k_transport_sm, dim3(blocks), dim3(block_size), lds, stream, P);
   else
-    hipLaunchKernelGGL(k_transport<false>, dim3(blocks), dim3(block_size), lds, stream, P);
+    hipLaunchKernelGGL(k_transport_plain, dim3(blocks), dim3(block_size), lds, stream, P);
   return hipGetLastError();
 }
 

@@ -21,7 +21,9 @@ enum KCounter {
   KC_PAIRS,
   KC_COINC,
   KC_RNDRAWS,
-  KC_N = 16
+  KC_PHASE_EXEC = 16,   // per-phase executions of the persistent kernel (5 entries, MCPET_SM_STATS builds)
+  KC_PHASE_LANES = 24,  // per-phase sum of active lanes (5 entries)
+  KC_N = 32
 };
 
 struct KParams {
@@ -76,6 +78,10 @@ struct KParams {
   int* final_seed;
   long long last_thread_id;      // reference thread id whose final seed is stored (K:454-455)
   int lds_spectrum;              // 1: privatise the spectrum in LDS
+  // Phase selection of the persistent kernel: 0 = phase held by most lanes;
+  // 1 = phase with the fewest wasted lane-slots, (64 - lanes) * phase_cost.
+  int sm_policy;
+  int sm_cost[8];                // relative cost per phase (DECAY, SOURCE2, STEP, INTERACT, TALLY)
 };
 
 // Launchers implemented in kernels.hip.

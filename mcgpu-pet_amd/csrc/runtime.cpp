@@ -367,6 +367,11 @@ extern "C" mcpet_status mcpet_run(mcpet_ctx* c, const mcpet_run_params* rp)
   KParams P = c->P;
   if (p.seed != 0) P.seed = p.seed;
   if (p.acquisition_time_ps != 0) P.acq_ps = p.acquisition_time_ps;
+  // Phase selection (persistent kernel): most lanes by default (measured best on the head
+  // workload); policy 2 weighs the phases by relative cost (DECAY, SOURCE2, STEP, INTERACT, TALLY).
+  P.sm_policy = (p.phase_policy == 2) ? 1 : 0;
+  static const int default_cost[5] = {4, 6, 1, 10, 6};
+  for (int i = 0; i < 5; i++) P.sm_cost[i] = p.phase_cost[i] > 0 ? p.phase_cost[i] : default_cost[i];
   int kind = p.kernel == MCPET_KERNEL_AUTO ? MCPET_KERNEL_PERSISTENT : p.kernel;
   int bs = p.block_size > 0 ? p.block_size : 256;
   if (bs % 64 != 0 || bs > 256) return set_err(c, MCPET_E_RANGE, "mcpet_run: block_size must be a multiple of 64, <= 256");
@@ -475,6 +480,10 @@ extern "C" mcpet_status mcpet_get_stats(mcpet_ctx* c, mcpet_stats* s)
   s->pairs_tallied = cnt[KC_PAIRS];
   s->coincidences = cnt[KC_COINC];
   s->rn_draws = cnt[KC_RNDRAWS];
+  for (int i = 0; i < 5; i++) {
+    s->phase_exec[i] = cnt[KC_PHASE_EXEC + i];
+    s->phase_lanes[i] = cnt[KC_PHASE_LANES + i];
+  }
   return MCPET_OK;
 }
 

@@ -15,7 +15,7 @@ import os
 import numpy as np
 
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(_PKG, "lib", "libmcpet.so")
+LIB_PATH = os.environ.get("MCPET_LIB") or os.path.join(_PKG, "lib", "libmcpet.so")
 
 MAX_MATERIALS = 15
 MAX_SHELLS = 30
@@ -105,17 +105,22 @@ class Tallies(C.Structure):
 
 class RunParams(C.Structure):
     _fields_ = [("kernel", C.c_int32), ("block_size", C.c_int32), ("blocks_per_cu", C.c_int32),
-                ("zero_tallies", C.c_int32), ("seed", C.c_int32), ("acquisition_time_ps", C.c_uint64)]
+                ("zero_tallies", C.c_int32), ("seed", C.c_int32), ("acquisition_time_ps", C.c_uint64),
+                ("phase_policy", C.c_int32), ("phase_cost", C.c_int32 * 5)]
 
 
 class Stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("zero_ms", C.c_double), ("decays", C.c_uint64), ("streams", C.c_uint64),
                 ("woodcock_steps", C.c_uint64), ("material_changes", C.c_uint64), ("compton", C.c_uint64),
                 ("compton_shell_evals", C.c_uint64), ("rayleigh", C.c_uint64), ("photoelectric", C.c_uint64),
-                ("pairs_tallied", C.c_uint64), ("coincidences", C.c_uint64), ("rn_draws", C.c_uint64)]
+                ("pairs_tallied", C.c_uint64), ("coincidences", C.c_uint64), ("rn_draws", C.c_uint64),
+                ("phase_exec", C.c_uint64 * 5), ("phase_lanes", C.c_uint64 * 5)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["phase_exec"] = list(self.phase_exec)
+        d["phase_lanes"] = list(self.phase_lanes)
+        return d
 
 
 _lib = None
@@ -299,8 +304,10 @@ class Simulation:
         if st != 0:
             raise McpetError(st, lib().mcpet_last_error(None).decode(errors="replace"))
 
-    def run(self, kernel=KERNEL_AUTO, block_size=0, blocks_per_cu=0, zero_tallies=True, seed=0, acquisition_time_ps=0):
-        rp = RunParams(kernel, block_size, blocks_per_cu, 1 if zero_tallies else 0, seed, acquisition_time_ps)
+    def run(self, kernel=KERNEL_AUTO, block_size=0, blocks_per_cu=0, zero_tallies=True, seed=0, acquisition_time_ps=0,
+            phase_policy=0, phase_cost=None):
+        rp = RunParams(kernel, block_size, blocks_per_cu, 1 if zero_tallies else 0, seed, acquisition_time_ps,
+                       phase_policy, (C.c_int32 * 5)(*(phase_cost or [0] * 5)))
         _check(lib().mcpet_run(self.ctx, C.byref(rp)), self.ctx)
 
     def sync(self):
