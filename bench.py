@@ -142,9 +142,11 @@ def main():
                                                                    "rn_draws")},
     }
     if any(agg["phase_exec"]):
-        out["phase_stats"] = {n: {"exec_per_decay": e / max(1, decays), "mean_lanes": l / max(1, e)}
-                              for n, e, l in zip(("decay", "source2", "step", "interact", "tally"),
-                                                 agg["phase_exec"], agg["phase_lanes"])}
+        tot_cyc = max(1, sum(agg["phase_cycles"]))
+        out["phase_stats"] = {n: {"exec_per_decay": e / max(1, decays), "mean_lanes": l / max(1, e),
+                                  "cycle_share": c / tot_cyc, "cycles_per_exec": c / max(1, e)}
+                              for n, e, l, c in zip(("decay", "source2", "step", "interact", "tally"),
+                                                    agg["phase_exec"], agg["phase_lanes"], agg["phase_cycles"])}
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, ph, vs, names, a.cpu_threads, a.cpu_stride)
     print(json.dumps(out))

@@ -191,6 +191,7 @@ typedef struct mcpet_stats {
      (DECAY, SOURCE2, STEP, INTERACT, TALLY); zero otherwise */
   uint64_t phase_exec[5];
   uint64_t phase_lanes[5];
+  uint64_t phase_cycles[5];      /* shader clock cycles per phase, summed over waves     */
 } mcpet_stats;
 
 /* ------------------------------------------------------------------ host-side input (boundary) */
@@ -285,7 +286,7 @@ mcpet_status mcpet_selftest_ranecu(int32_t seed, const int64_t* stream_ids, int3
 /* Evaluate the device elementary functions (log, exp, sincos, atan2, acos, logf, expf,
  * powf, sincosf, atan2f, acosf) on n inputs; used to check host/device bit equality. */
 mcpet_status mcpet_selftest_fmath(const double* x, const double* y, int32_t n, double* out_d /* 7*n */,
-                                  float* out_f /* 7*n */);
+                                  float* out_f /* 8*n */);
 
 /* Device copy of nbytes (for HBM-counter calibration: FETCH_SIZE/WRITE_SIZE of a known copy). */
 mcpet_status mcpet_selftest_copy(size_t nbytes, int32_t iters, double* ms_out);

@@ -399,7 +399,6 @@ K_RARE void tally_sinogram(const KParams& P, float energy, float energyFirst, co
   float pxn = x0 + dist * dir[0];
   float pyn = y0 + dist * dir[1];
   float pzn = (pos[2] - P.cz) + dist * dir[2];
-  float phi = mcf_atan2f(pyn, pxn);
   if (!(fabsf(pzn) < 0.5f * P.height)) return;
   x0 = posf[0] - P.cx;
   y0 = posf[1] - P.cy;
@@ -407,8 +406,10 @@ K_RARE void tally_sinogram(const KParams& P, float energy, float energyFirst, co
   float pfxn = x0 + dist * dirf[0];
   float pfyn = y0 + dist * dirf[1];
   float pfzn = (posf[2] - P.cz) + dist * dirf[2];
-  float phi1 = mcf_atan2f(pfyn, pfxn);
   if (!(fabsf(pfzn) < 0.5f * P.height)) return;
+  // the azimuths only matter once both hits are inside the detector height (K:1669,1700)
+  const float phi = mcf_atan2f(pyn, pxn);
+  const float phi1 = mcf_atan2f(pfyn, pfxn);
   int ibin = sino_bin(P, phi, pzn, phi1, pfzn);
   if (ibin < 0) return;
   float randno1 = rn(st.r, C);
@@ -466,7 +467,6 @@ K_RARE void tally_psf(const KParams& P, float energy, float energyFirst, const f
   position[0] = x0 + dist * dir[0];
   position[1] = y0 + dist * dir[1];
   position[2] = (pos_in[2] - P.cz) + dist * dir[2];
-  float phi = mcf_atan2f(position[1], position[0]);
   if (!(fabsf(position[2]) < 0.5f * P.height)) return;
   x0 = posf_in[0] - P.cx;
   y0 = posf_in[1] - P.cy;
@@ -475,7 +475,6 @@ K_RARE void tally_psf(const KParams& P, float energy, float energyFirst, const f
   positionFirst[0] = x0 + dist * dirf[0];
   positionFirst[1] = y0 + dist * dirf[1];
   positionFirst[2] = (posf_in[2] - P.cz) + dist * dirf[2];
-  float phi1 = mcf_atan2f(positionFirst[1], positionFirst[0]);
   float randno1 = rn(st.r, C);
   float randno2 = rn(st.r, C);
   float g1 = sqrtf(-2.0f * mcf_logf(randno1 + 1e-8f)) * mcf_cosf((float)(2.0f * K_PI * (double)randno2));
@@ -498,6 +497,9 @@ K_RARE void tally_psf(const KParams& P, float energy, float energyFirst, const f
         i1 = index;
       }
       unsigned long long t_ps = P.acq_ps - st.remaining;
+      // azimuths of the two records (K:554,577), needed only for a stored pair
+      const float phi = mcf_atan2f(position[1], position[0]);
+      const float phi1 = mcf_atan2f(positionFirst[1], positionFirst[0]);
       if (i0 < P.psf_size)
         write_psf(P.psf + i0, t_ps, (float)(travel * K_INV_C), st.absvox, Energia1, position[2], phi, dir, ss);
       if (i1 < P.psf_size)
@@ -663,7 +665,7 @@ __device__ __forceinline__ void source_second(const KParams& P, Stream& st, Coun
     return;
   }
   float phi_sampled2 = (float)((double)mcf_atan2f(dir1[1], dir1[0]) + K_PI);
-  float theta_sampled2 = (float)(K_PI - mcf_acos((double)dir1[2]));
+  float theta_sampled2 = (float)(K_PI - (double)mcf_acosf(dir1[2]));
   float V1 = rn(st.r, C);
   float V2 = rn(st.r, C);
   float THETA_NC = (float)((2.0 * 0.21276 * K_PI / 180.0) * (double)sqrtf(-2.0f * mcf_logf(V2 + 1e-8f)) *
@@ -883,6 +885,7 @@ __global__ __launch_bounds__(256) K_OCCUPANCY void k_transport_sm(KParams P)
     best = __builtin_amdgcn_readfirstlane(best);
     if (best < 0) break;
 #if MCPET_SM_STATS
+    const long long t_phase = clock64();
     {
       const int lanes = __popcll(__ballot(state == best));
       if (__lane_id() == 0) {
@@ -891,8 +894,8 @@ __global__ __launch_bounds__(256) K_OCCUPANCY void k_transport_sm(KParams P)
       }
     }
 #endif
-    if (state != best) continue;
-    if (best == LS_STEP) {
+    if (state != best) {
+    } else if (best == LS_STEP) {
       const int o = woodcock_step(P, ph, st.r, C);
       if (o == STEP_ESCAPED) state = after_photon(P, photon_no, ph, pr);
       else if (o == STEP_REAL) state = LS_INTERACT;
@@ -936,6 +939,9 @@ __global__ __launch_bounds__(256) K_OCCUPANCY void k_transport_sm(KParams P)
       tally_pair(P, ph, pr, st, C, s_spec);
       state = LS_DECAY;
     }
+#if MCPET_SM_STATS
+    if (__lane_id() == 0) C.c[KC_PHASE_CYCLES + best] += (unsigned long long)(clock64() - t_phase);
+#endif
   }
   K_EPILOGUE
 }
@@ -978,6 +984,7 @@ __global__ void k_selftest_fmath(const double* x, const double* y, int n, double
   of[4 * (size_t)n + i] = cf;
   of[5 * (size_t)n + i] = mcf_atan2f(af, bf);
   of[6 * (size_t)n + i] = sqrtf(fabsf(af)) / (bf + 3.0f);
+  of[7 * (size_t)n + i] = mcf_acosf(fminf(fmaxf(bf, -1.0f), 1.0f));
 }
 
 __global__ void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16)

@@ -21,6 +21,7 @@ enum KCounter {
   KC_PAIRS,
   KC_COINC,
   KC_RNDRAWS,
+  KC_PHASE_CYCLES = 11, // per-phase shader clock cycles, summed over waves (5 entries, MCPET_SM_STATS builds)
   KC_PHASE_EXEC = 16,   // per-phase executions of the persistent kernel (5 entries, MCPET_SM_STATS builds)
   KC_PHASE_LANES = 24,  // per-phase sum of active lanes (5 entries)
   KC_N = 32

@@ -56,6 +56,19 @@ MCR_HD int mcr_abmodm(int m, int a, int s)
 
 MCR_HD uint32_t mcr_mulmod(uint32_t a, uint32_t b, uint32_t m) { return (uint32_t)(((uint64_t)a * b) % m); }
 
+/* (a*b) mod m for m = 2^31 - c (both RANECU moduli), a, b < m: two folds of the high
+ * part (2^31 = c mod m) and one conditional subtraction; the same integer as the
+ * generic remainder, without a 64-bit division. */
+MCR_HD uint32_t mcr_mulmod_pm(uint32_t a, uint32_t b, uint32_t c)
+{
+  const uint64_t x = (uint64_t)a * b;                       /* < 2^62 */
+  const uint64_t y = (x >> 31) * c + (x & 0x7fffffffull);   /* < 250 * 2^31 */
+  uint32_t z = (uint32_t)((y >> 31) * c + (y & 0x7fffffffull)); /* < 2^31 + 62250 */
+  const uint32_t m = 0x80000000u - c;
+  if (z >= m) z -= m;
+  return z;
+}
+
 /* Host: table[k] = a^(L*2^k) mod m, L = 555555*256. */
 static inline void mcr_build_jump_table(int a, int m, uint32_t* table)
 {
@@ -78,12 +91,18 @@ MCR_HD void mcr_init_stream(int64_t id, int seed_input, const uint32_t* j1, cons
   uint32_t y1 = 1, y2 = 1;
   for (int k = 0; k < MCR_JUMP_BITS && n; k++, n >>= 1) {
     if (n & 1ull) {
-      y1 = mcr_mulmod(y1, j1[k], (uint32_t)MCR_M1);
-      y2 = mcr_mulmod(y2, j2[k], (uint32_t)MCR_M2);
+      y1 = mcr_mulmod_pm(y1, j1[k], 0x80000000u - (uint32_t)MCR_M1);
+      y2 = mcr_mulmod_pm(y2, j2[k], 0x80000000u - (uint32_t)MCR_M2);
     }
   }
-  *sx = mcr_abmodm(MCR_M1, seed_input, (int)y1);
-  *sy = mcr_abmodm(MCR_M2, seed_input, (int)y2);
+  if (seed_input > 0 && seed_input < MCR_M2) {
+    /* abMODm(m, a, s) = a*s mod m for 0 < a, s < m */
+    *sx = (int)mcr_mulmod_pm((uint32_t)seed_input, y1, 0x80000000u - (uint32_t)MCR_M1);
+    *sy = (int)mcr_mulmod_pm((uint32_t)seed_input, y2, 0x80000000u - (uint32_t)MCR_M2);
+  } else {
+    *sx = mcr_abmodm(MCR_M1, seed_input, (int)y1);
+    *sy = mcr_abmodm(MCR_M2, seed_input, (int)y2);
+  }
 }
 
 /* One step of both MLCGs (Schrage decomposition), combined output in [1, m1-1]. */

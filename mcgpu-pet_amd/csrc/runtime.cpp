@@ -68,6 +68,15 @@ mcpet_status set_err(mcpet_ctx* c, mcpet_status st, const std::string& msg)
   return st;
 }
 
+// Seeds for which the reference's abMODm (K:1000-1032) divides by zero: 0 (replaced by a
+// time-based seed when read from a file) and values below -m2.
+static bool seed_usable(int s) { return s != 0 && s >= -MCR_M2; }
+static std::string seed_msg(int s)
+{
+  return "seed " + std::to_string(s) + " is not a usable RANECU seed (must be non-zero and >= -" +
+         std::to_string(MCR_M2) + ")";
+}
+
 #define HIPCHK(ctx, call)                                                                              \
   do {                                                                                                 \
     hipError_t e_ = (call);                                                                            \
@@ -366,6 +375,7 @@ extern "C" mcpet_status mcpet_run(mcpet_ctx* c, const mcpet_run_params* rp)
   HIPCHK(c, hipSetDevice(c->device));
   KParams P = c->P;
   if (p.seed != 0) P.seed = p.seed;
+  if (!seed_usable(P.seed)) return set_err(c, MCPET_E_ARG, seed_msg(P.seed));
   if (p.acquisition_time_ps != 0) P.acq_ps = p.acquisition_time_ps;
   // Phase selection (persistent kernel): most lanes by default (measured best on the head
   // workload); policy 2 weighs the phases by relative cost (DECAY, SOURCE2, STEP, INTERACT, TALLY).
@@ -483,6 +493,7 @@ extern "C" mcpet_status mcpet_get_stats(mcpet_ctx* c, mcpet_stats* s)
   for (int i = 0; i < 5; i++) {
     s->phase_exec[i] = cnt[KC_PHASE_EXEC + i];
     s->phase_lanes[i] = cnt[KC_PHASE_LANES + i];
+    s->phase_cycles[i] = cnt[KC_PHASE_CYCLES + i];
   }
   return MCPET_OK;
 }
@@ -505,6 +516,7 @@ extern "C" void mcpet_destroy(mcpet_ctx* c)
 
 extern "C" mcpet_status mcpet_stream_seeds(int32_t seed, const int64_t* ids, int32_t n, int32_t* sx, int32_t* sy)
 {
+  if (!seed_usable(seed)) return set_err(nullptr, MCPET_E_ARG, seed_msg(seed));
   if (!ids || !sx || !sy || n < 0) return set_err(nullptr, MCPET_E_ARG, "mcpet_stream_seeds: bad args");
   uint32_t j1[MCR_JUMP_BITS], j2[MCR_JUMP_BITS];
   mcr_build_jump_table(MCR_A1, MCR_M1, j1);
@@ -519,6 +531,7 @@ extern "C" mcpet_status mcpet_selftest_ranecu(int32_t seed, const int64_t* ids, 
                                               double* out_d, int32_t as_double)
 {
   if (!ids || nstreams < 1 || n < 1 || (as_double ? !out_d : !out)) return set_err(nullptr, MCPET_E_ARG, "selftest_ranecu: bad args");
+  if (!seed_usable(seed)) return set_err(nullptr, MCPET_E_ARG, seed_msg(seed));
   uint32_t j[2][MCR_JUMP_BITS];
   mcr_build_jump_table(MCR_A1, MCR_M1, j[0]);
   mcr_build_jump_table(MCR_A2, MCR_M2, j[1]);
@@ -553,13 +566,13 @@ extern "C" mcpet_status mcpet_selftest_fmath(const double* x, const double* y, i
   hipError_t e = hipMalloc((void**)&dx, 8 * (size_t)n);
   if (e == hipSuccess) e = hipMalloc((void**)&dy, 8 * (size_t)n);
   if (e == hipSuccess) e = hipMalloc((void**)&dod, 7 * 8 * (size_t)n);
-  if (e == hipSuccess) e = hipMalloc((void**)&dof, 7 * 4 * (size_t)n);
+  if (e == hipSuccess) e = hipMalloc((void**)&dof, 8 * 4 * (size_t)n);
   if (e == hipSuccess) e = hipMemcpy(dx, x, 8 * (size_t)n, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(dy, y, 8 * (size_t)n, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = mcpet_launch_selftest_fmath(dx, dy, n, dod, dof, 0);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e == hipSuccess) e = hipMemcpy(od, dod, 7 * 8 * (size_t)n, hipMemcpyDeviceToHost);
-  if (e == hipSuccess) e = hipMemcpy(of, dof, 7 * 4 * (size_t)n, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(of, dof, 8 * 4 * (size_t)n, hipMemcpyDeviceToHost);
   if (e != hipSuccess) rc = set_err(nullptr, MCPET_E_HIP, std::string("selftest_fmath: ") + hipGetErrorString(e));
   if (dx) (void)hipFree(dx);
   if (dy) (void)hipFree(dy);

@@ -114,12 +114,14 @@ class Stats(C.Structure):
                 ("woodcock_steps", C.c_uint64), ("material_changes", C.c_uint64), ("compton", C.c_uint64),
                 ("compton_shell_evals", C.c_uint64), ("rayleigh", C.c_uint64), ("photoelectric", C.c_uint64),
                 ("pairs_tallied", C.c_uint64), ("coincidences", C.c_uint64), ("rn_draws", C.c_uint64),
-                ("phase_exec", C.c_uint64 * 5), ("phase_lanes", C.c_uint64 * 5)]
+                ("phase_exec", C.c_uint64 * 5), ("phase_lanes", C.c_uint64 * 5),
+                ("phase_cycles", C.c_uint64 * 5)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
         d["phase_exec"] = list(self.phase_exec)
         d["phase_lanes"] = list(self.phase_lanes)
+        d["phase_cycles"] = list(self.phase_cycles)
         return d
 
 
@@ -443,7 +445,7 @@ def selftest_fmath(x, y):
     y = np.ascontiguousarray(y, dtype=np.float64)
     n = len(x)
     od = np.zeros((7, n), np.float64)
-    of = np.zeros((7, n), np.float32)
+    of = np.zeros((8, n), np.float32)
     _check(lib().mcpet_selftest_fmath(x.ctypes.data_as(C.POINTER(C.c_double)), y.ctypes.data_as(C.POINTER(C.c_double)),
                                       n, od.ctypes.data_as(C.POINTER(C.c_double)), of.ctypes.data_as(C.POINTER(C.c_float))))
     return od, of

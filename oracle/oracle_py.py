@@ -323,7 +323,7 @@ def fmath_eval(x, y):
     y = np.ascontiguousarray(y, np.float64)
     n = len(x)
     od = np.zeros((7, n), np.float64)
-    of = np.zeros((7, n), np.float32)
+    of = np.zeros((8, n), np.float32)
     L.orc_fmath_eval(x.ctypes.data_as(C.POINTER(C.c_double)), y.ctypes.data_as(C.POINTER(C.c_double)), n,
                      od.ctypes.data_as(C.POINTER(C.c_double)), _f(of))
     return od, of

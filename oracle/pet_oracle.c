@@ -42,7 +42,7 @@ static inline float o_atan2f(float y, float x) { return mcf_atan2f(y, x); }
 static inline double o_log(double x) { return mcf_log(x); }
 static inline double o_exp(double x) { return mcf_exp(x); }
 static inline void o_sincos(double x, double* s, double* c) { mcf_sincos(x, s, c); }
-static inline double o_acos(double x) { return mcf_acos(x); }
+static inline float o_acosf(float x) { return mcf_acosf(x); }
 #else
 static inline float o_logf(float x) { return logf(x); }
 static inline float o_expf(float x) { return expf(x); }
@@ -53,7 +53,7 @@ static inline float o_atan2f(float y, float x) { return atan2f(y, x); }
 static inline double o_log(double x) { return log(x); }
 static inline double o_exp(double x) { return exp(x); }
 static inline void o_sincos(double x, double* s, double* c) { sincos(x, s, c); }
-static inline double o_acos(double x) { return acos(x); }
+static inline float o_acosf(float x) { return acosf(x); }
 #endif
 
 #define O_PI 3.14159265358979323846            /* H:105 */
@@ -449,7 +449,7 @@ static void o_source(const orc_sim* s, int ix, int iy, int iz, float* pos, float
     dir[0] = sin_theta * cp;
   } else {
     float phi_sampled2 = (float)((double)o_atan2f(dir1[1], dir1[0]) + O_PI);
-    float theta_sampled2 = (float)(O_PI - o_acos((double)dir1[2]));
+    float theta_sampled2 = (float)(O_PI - (double)o_acosf(dir1[2]));
     float V1 = RN(r);
     float V2 = RN(r);
     float THETA_NC = (float)((2.0 * 0.21276 * O_PI / 180.0) * (double)sqrtf(-2.0f * o_logf(V2 + 1e-8f)) *
@@ -1137,6 +1137,7 @@ void orc_fmath_eval(const double* x, const double* y, int n, double* od, float* 
     of[4 * (size_t)n + i] = cf;
     of[5 * (size_t)n + i] = mcf_atan2f(af, bf);
     of[6 * (size_t)n + i] = sqrtf(fabsf(af)) / (bf + 3.0f);
+    of[7 * (size_t)n + i] = mcf_acosf(fminf(fmaxf(bf, -1.0f), 1.0f));
   }
 }
 #endif

@@ -37,18 +37,19 @@ def test_ranecu_bit_exact():
 
 
 def test_fmath_host_device_bit_exact():
-    _say("fmath: 200k inputs, 14 functions, device vs gcc host")
+    _say("fmath: 200k inputs, 15 functions, device vs gcc host")
     rng = np.random.default_rng(5)
     n = 200_000
     x = np.concatenate([rng.uniform(-20, 20, n // 4), rng.uniform(-1, 1, n // 4),
                         np.exp(rng.uniform(-30, 6, n // 4)) * rng.choice([-1, 1], n // 4),
-                        rng.uniform(-700, 700, n - 3 * (n // 4))])
+                        rng.uniform(-700, 700, n // 8), rng.uniform(-3e4, 3e4, n - 3 * (n // 4) - n // 8)])
     y = rng.uniform(-1.5, 1.5, n)
     gd, gf = mcpet.selftest_fmath(x, y)
     hd, hf = oracle_py.fmath_eval(x, y)
     for i in range(7):
         bad = np.flatnonzero(gd[i].view(np.uint64) != hd[i].view(np.uint64))
         assert bad.size == 0, f"double fn {i}: {bad.size} mismatches, first x={x[bad[0]]!r} y={y[bad[0]]!r}"
+    for i in range(8):
         badf = np.flatnonzero(gf[i].view(np.uint32) != hf[i].view(np.uint32))
         assert badf.size == 0, f"float fn {i}: {badf.size} mismatches, first x={x[badf[0]]!r} y={y[badf[0]]!r}"
 

@@ -4,6 +4,7 @@ skip-ahead checked against the oracle."""
 import ctypes as C
 
 import numpy as np
+import pytest
 
 import mcpet
 import oracle_py
@@ -87,7 +88,8 @@ def test_library_jump_table_matches_reference_skip_ahead():
     square-and-multiply init_PRNG for ids across the int32 range and beyond."""
     L = oracle_py.load("libm")
     ids = np.array([0, 1, 2, 3, 63, 64, 1000, 65535, 123456789, 2**31 - 1, 2**31, 3 * 2**33 + 1, 10**11], np.int64)
-    for seed in (1, 12345, -3, 2147483562):
+    rs = np.random.default_rng(9).integers(1, 2**31 - 1, 6).tolist()
+    for seed in [1, 12345, 32768, 32769, -3, -M2, M2 - 1, M2, M2 + 1, M1 - 1, M1, 2**31 - 1] + rs:
         sx, sy = mcpet.stream_seeds(seed, ids)
         for k, i in enumerate(ids.tolist()):
             ox, oy = C.c_int(), C.c_int()
@@ -97,3 +99,22 @@ def test_library_jump_table_matches_reference_skip_ahead():
                 leap = (i + 1) * 555555 * 256
                 assert int(sx[k]) == (seed * pow(A1, leap, M1)) % M1
                 assert int(sy[k]) == (seed * pow(A2, leap, M2)) % M2
+
+
+def test_pseudo_mersenne_mulmod_brute_force():
+    """The library's stream set-up reduces products modulo 2^31 - c by folding; check the
+    folded product against Python integers over random and extreme operands, through
+    the public seed routine: seeds of stream 0 are seed * a^L mod m."""
+    rng = np.random.default_rng(21)
+    seeds = np.concatenate([rng.integers(1, M2, 3000), [1, 2, M2 - 1, 2**30, 2**31 - 250]]).astype(np.int64)
+    L1, L2 = pow(A1, 555555 * 256, M1), pow(A2, 555555 * 256, M2)
+    for seed in seeds.tolist():
+        sx, sy = mcpet.stream_seeds(int(seed), np.array([0], np.int64))
+        assert int(sx[0]) == seed * L1 % M1 and int(sy[0]) == seed * L2 % M2
+
+
+def test_seeds_the_reference_cannot_use_are_rejected():
+    """0 and seeds below -m2 make abMODm divide by zero (K:1000-1032); the library refuses them."""
+    for seed in (0, -M2 - 1, -2**31):
+        with pytest.raises(mcpet.McpetError):
+            mcpet.stream_seeds(seed, np.array([0], np.int64))
