@@ -74,7 +74,7 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--block-size", type=int, default=0)
     ap.add_argument("--phase-policy", type=int, default=0, help="0 auto, 1 most lanes, 2 least waste")
-    ap.add_argument("--phase-cost", type=str, default="", help="5 comma-separated relative phase costs")
+    ap.add_argument("--phase-cost", type=str, default="", help="comma-separated relative phase costs (DECAY, SOURCE2, STEP, INTERACT, TALLY, CTAU, CPZ)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-stride", type=int, default=5)
@@ -102,18 +102,27 @@ def main():
     sim.sync()
     coinc = decays = 0
     kernel_ms = []
-    agg = None
+    per_step = []
     t0 = time.perf_counter()
     for s in range(a.steps):
         sim.run(seed=2000 + s, **run_kw)
-        st = sim.stats()               # synchronises; counters of this step
+        st = sim.stats()               # synchronises; decays and coincidences of this step
         coinc += st["coincidences"]
         decays += st["decays"]
         kernel_ms.append(st["kernel_ms"])
-        agg = st if agg is None else {k: (agg[k] + st[k]) if not isinstance(st[k], list) else
-                                      [x + y for x, y in zip(agg[k], st[k])] for k in agg}
+        per_step.append((st["decays"], st["coincidences"]))
     sim.sync()
     elapsed = time.perf_counter() - t0
+    # Algorithmic event counts of the timed steps: the same seeds replayed (untimed) with the
+    # instrumented kernel, which performs the identical simulation and counts every event.
+    agg = None
+    for s in range(a.steps):
+        sim.run(seed=2000 + s, count_events=True, **run_kw)
+        st = sim.stats()
+        if (st["decays"], st["coincidences"]) != per_step[s]:
+            raise SystemExit(f"bench.py: instrumented replay of step {s} differs from the timed run")
+        agg = st if agg is None else {k: (agg[k] + st[k]) if not isinstance(st[k], list) else
+                                      [x + y for x, y in zip(agg[k], st[k])] for k in agg}
     sim.close()
 
     ops, byts = algorithmic(agg)
@@ -145,7 +154,7 @@ def main():
         tot_cyc = max(1, sum(agg["phase_cycles"]))
         out["phase_stats"] = {n: {"exec_per_decay": e / max(1, decays), "mean_lanes": l / max(1, e),
                                   "cycle_share": c / tot_cyc, "cycles_per_exec": c / max(1, e)}
-                              for n, e, l, c in zip(("decay", "source2", "step", "interact", "tally"),
+                              for n, e, l, c in zip(("decay", "source2", "step", "interact", "tally", "ctau", "cpz"),
                                                     agg["phase_exec"], agg["phase_lanes"], agg["phase_cycles"])}
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, ph, vs, names, a.cpu_threads, a.cpu_stride)

@@ -31,6 +31,7 @@ extern "C" {
 #define MCPET_NP_RAYLEIGH 128
 #define MCPET_MAX_ENERGYBINS_RAYLEIGH 25005
 #define MCPET_NAME_LEN 250
+#define MCPET_NPHASES 8                /* phase slots of the persistent kernel's statistics */
 
 typedef enum mcpet_status {
   MCPET_OK = 0,
@@ -169,10 +170,15 @@ typedef struct mcpet_run_params {
   int32_t seed;                  /* 0: use the config seed                             */
   uint64_t acquisition_time_ps;  /* 0: use the config value                            */
   int32_t phase_policy;          /* persistent kernel: 0 auto, 1 most lanes, 2 least waste */
-  int32_t phase_cost[5];         /* relative phase costs for policy 2 (0: built-in)    */
+  int32_t phase_cost[MCPET_NPHASES]; /* relative phase costs for policy 2 (0: built-in) */
+  int32_t count_events;          /* 1: instrumented persistent kernel that counts every event
+                                    in mcpet_stats (same tallies; slower). 0: only decays,
+                                    streams, pairs_tallied and coincidences are counted. The
+                                    plain kernel always counts every event.              */
 } mcpet_run_params;
 
-/* Per-run statistics: HIP-event kernel time and algorithmic event counts. */
+/* Per-run statistics: HIP-event kernel time and algorithmic event counts (see
+ * mcpet_run_params.count_events for which counts a run fills). */
 typedef struct mcpet_stats {
   double kernel_ms;              /* transport kernel(s), HIP events on the context stream */
   double zero_ms;                /* tally clearing                                       */
@@ -187,11 +193,12 @@ typedef struct mcpet_stats {
   uint64_t pairs_tallied;        /* coincidences reaching tally_Sinogram/PSF             */
   uint64_t coincidences;         /* trues + scatter binned                               */
   uint64_t rn_draws;
-  /* persistent kernel, diagnostic builds only: executions and active lanes per phase
-     (DECAY, SOURCE2, STEP, INTERACT, TALLY); zero otherwise */
-  uint64_t phase_exec[5];
-  uint64_t phase_lanes[5];
-  uint64_t phase_cycles[5];      /* shader clock cycles per phase, summed over waves     */
+  /* persistent kernel, diagnostic builds only: executions, active lanes and shader clock
+     cycles (summed over waves) per phase: DECAY, SOURCE2, STEP, INTERACT, TALLY, CTAU (one
+     trial of GCOa's tau loop), CPZ (one trial of its p_z loop); zero otherwise */
+  uint64_t phase_exec[MCPET_NPHASES];
+  uint64_t phase_lanes[MCPET_NPHASES];
+  uint64_t phase_cycles[MCPET_NPHASES];
 } mcpet_stats;
 
 /* ------------------------------------------------------------------ host-side input (boundary) */

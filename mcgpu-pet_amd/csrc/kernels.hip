@@ -18,7 +18,9 @@
 //     Compton/Rayleigh (a,b) in a separate 16-byte table read only at real events;
 //   - Compton shell table (5460 B), the energy spectrum and the per-material dose
 //     are kept in LDS and flushed once per workgroup;
-//   - emission-image counts are kept per stream in registers and flushed once per stream.
+//   - per-lane state touched by only some phases of the persistent kernel (the pair kept
+//     between the two photons, the Compton sampler between rejection trials) lives in
+//     LDS slots, so that registers hold only the stream and the current photon.
 #include <hip/hip_runtime.h>
 #include <climits>
 
@@ -35,20 +37,23 @@
 // corrupt input. The CPU checker applies the same bound.
 #define K_MAX_TRIALS (1 << 20)
 
-// Occupancy hint for the transport kernel (waves per SIMD); 0 leaves it to the compiler.
+// Occupancy target of the transport kernels (waves per SIMD; 0 leaves it to the compiler).
+// The persistent kernel is bound by the latency of dependent gathers and transcendental
+// chains; 3 waves per SIMD (168 VGPRs, a few dwords of spill) hides enough of it to beat
+// the compiler's own choice (2 waves at ~180 VGPRs) by 1.35x on the head workload.
 #ifndef MCPET_WAVES_PER_EU
-#define MCPET_WAVES_PER_EU 0
+#define MCPET_WAVES_PER_EU 3
 #endif
 #if MCPET_WAVES_PER_EU > 0
 #define K_OCCUPANCY __attribute__((amdgpu_waves_per_eu(MCPET_WAVES_PER_EU, 8)))
 #else
 #define K_OCCUPANCY
 #endif
-// Rarely executed physics (Compton, Rayleigh, tallies) as real calls instead of inlined code.
 // Per-phase execution statistics of the persistent kernel (diagnostic builds only).
 #ifndef MCPET_SM_STATS
 #define MCPET_SM_STATS 0
 #endif
+// Rarely executed physics (Compton, Rayleigh, tallies) as real calls instead of inlined code.
 #ifndef MCPET_NOINLINE_RARE
 #define MCPET_NOINLINE_RARE 0
 #endif
@@ -64,18 +69,48 @@ struct Rng {
   int x, y;
 };
 
-struct Counters {
+// Event counters, a compile-time policy of every device function below.
+// FullCounters (instrumented instantiation): every event count, per lane in registers,
+// flushed once per lane. LiteCounters (production): only decays, streams, tallied pairs
+// and coincidences, as workgroup counters in LDS, so that the transport keeps no counter
+// registers. Both instantiations perform the same arithmetic on the same random numbers.
+struct FullCounters {
   unsigned c[KC_N];
+  __device__ __forceinline__ void init(unsigned*)
+  {
+#pragma unroll
+    for (int i = 0; i < KC_N; i++) c[i] = 0;
+  }
+  __device__ __forceinline__ void inc(int k) { c[k]++; }
+  __device__ __forceinline__ void add(int k, unsigned v) { c[k] += v; }
+  __device__ __forceinline__ void flush(const KParams& P, const unsigned*)
+  {
+#pragma unroll
+    for (int i = 0; i < KC_N; i++)
+      if (c[i]) atomicAdd(&P.counters[i], (unsigned long long)c[i]);
+  }
+};
+struct LiteCounters {
+  unsigned* s;  // KC_N workgroup counters in LDS
+  __device__ __forceinline__ void init(unsigned* lds) { s = lds; }
+  __device__ __forceinline__ void inc(int k)
+  {
+    if (k == KC_DECAYS || k == KC_STREAMS || k == KC_PAIRS || k == KC_COINC) atomicAdd(&s[k], 1u);
+  }
+  __device__ __forceinline__ void add(int, unsigned) {}
+  __device__ __forceinline__ void flush(const KParams&, const unsigned*) {}
 };
 
-__device__ __forceinline__ float rn(Rng& r, Counters& C)
+template <class CN>
+__device__ __forceinline__ float rn(Rng& r, CN& C)
 {
-  C.c[KC_RNDRAWS]++;
+  C.inc(KC_RNDRAWS);
   return mcr_ranecu(&r.x, &r.y);
 }
-__device__ __forceinline__ double rnd(Rng& r, Counters& C)
+template <class CN>
+__device__ __forceinline__ double rnd(Rng& r, CN& C)
 {
-  C.c[KC_RNDRAWS]++;
+  C.inc(KC_RNDRAWS);
   return mcr_ranecu_double(&r.x, &r.y);
 }
 
@@ -152,7 +187,8 @@ __device__ __forceinline__ void rotate_double(float* d, double costh, double phi
 }
 
 // GRAa (K:1230-1296): Rayleigh angle by RITA inverse cumulative + Thomson rejection.
-K_RARE void graa(float energy, double* costh, int mat, float pmax_current, Rng& r, Counters& C,
+template <class CN>
+K_RARE void graa(float energy, double* costh, int mat, float pmax_current, Rng& r, CN& C,
                      const mcpet_rayleigh_table* __restrict__ cgra)
 {
   double xmax = ((double)energy) * 8.065535669099010e-5;
@@ -232,111 +268,152 @@ __device__ __forceinline__ float shell_term(const mcpet_compton_table& cg, int m
 }
 
 // GCOa (K:1334-1563): Compton scattering, relativistic impulse approximation with
-// analytical one-electron profiles. The per-shell values rn[i] of the accepted trial are
-// kept in registers for the first four shells and recomputed for later shells instead of
-// a 30-entry array (same values, no scratch memory).
-K_RARE void gcoa(float* energy, double* costh, int mat, Rng& r, Counters& C, const mcpet_compton_table& cg)
-{
-  float s, a1, s0, af, ek, ek2, ek3, tau, pzomc, taumin;
+// analytical one-electron profiles, split into resumable pieces so that the persistent
+// kernel can run one rejection trial per phase: gcoa_init (S(theta=pi) over the shells),
+// gcoa_tau_trial (one trial of the tau/shell-profile loop, K:1389-1459), gcoa_pz_trial
+// (one trial of the p_z loop, K:1463-1539) and gcoa_finish (the final energy, K:1543-1557).
+// The per-shell values rn[i] of the accepted tau trial are kept for the first four shells
+// and recomputed for later shells instead of a 30-entry array (same values).
+struct ComptonState {
+  float a1, s0, taumin;            // set by gcoa_init
+  float tau, s, cdtf;              // accepted tau trial
+  float rc0, rc1, rc2, rc3;        // its profile terms of shells 0..3
+  float pzomc;
   double cdt1;
+  int trials;
+};
+
+__device__ __forceinline__ void gcoa_init(float E, int mat, const mcpet_compton_table& cg, ComptonState& cs)
+{
   const int my_noscco = cg.noscco[mat];
-  const float E = *energy;
-
-  ek = E * 1.956951306108245e-6f;
-  ek2 = ek * 2.f + 1.f;
-  ek3 = ek * ek;
-  taumin = 1.f / ek2;
-  a1 = mcf_logf(ek2);
-
-  s0 = 0.0f;
+  const float ek = E * 1.956951306108245e-6f;
+  const float ek2 = ek * 2.f + 1.f;
+  cs.taumin = 1.f / ek2;
+  cs.a1 = mcf_logf(ek2);
+  float s0 = 0.0f;
   for (int i = 0; i < my_noscco; i++) {
     if (cg.uico[mat + i * K_MAXMAT] < E) s0 += cg.fco[mat + i * K_MAXMAT] * shell_term(cg, mat, i, E, 0.0f, false);
   }
-  float cdtf;
-  // rn[] of the accepted trial: shells 0..3 kept in registers, later shells recomputed.
+  cs.s0 = s0;
+  cs.trials = 0;
+}
+
+// One trial of the tau loop; true when the loop is over (accepted or trial cap).
+template <class CN>
+__device__ __forceinline__ bool gcoa_tau_trial(float E, int mat, Rng& r, CN& C, const mcpet_compton_table& cg,
+                                               ComptonState& cs)
+{
+  const int my_noscco = cg.noscco[mat];
+  const float ek = E * 1.956951306108245e-6f;
+  const float ek2 = ek * 2.f + 1.f;
+  const float ek3 = ek * ek;
+  const float taumin = cs.taumin, a1 = cs.a1;
+  float tau;
+  if (rn(r, C) * (a1 + 2. * ek * (ek + 1.f) * taumin * taumin) < a1)
+    tau = mcf_powf(taumin, rn(r, C));
+  else
+    tau = sqrtf(1.f + rn(r, C) * (taumin * taumin - 1.f));
+  double cdt1 = (double)(1.f - tau) / (((double)tau) * ((double)E) * 1.956951306108245e-6);
+  if (cdt1 > 2.0) cdt1 = 1.99999999;
+  const float cdtf = (float)cdt1;
+  float s = 0.0f;
+  C.add(KC_SHELLEVAL, (unsigned)my_noscco);
   float rc0 = 0.0f, rc1 = 0.0f, rc2 = 0.0f, rc3 = 0.0f;
-  auto rnv = [&](int i) -> float {
-    if (i < 4) return i == 0 ? rc0 : i == 1 ? rc1 : i == 2 ? rc2 : rc3;
-    return shell_term(cg, mat, i, E, cdtf, true);
-  };
-  int trials = 0;
-  do {
-    if (rn(r, C) * (a1 + 2. * ek * (ek + 1.f) * taumin * taumin) < a1)
-      tau = mcf_powf(taumin, rn(r, C));
-    else
-      tau = sqrtf(1.f + rn(r, C) * (taumin * taumin - 1.f));
-    cdt1 = (double)(1.f - tau) / (((double)tau) * ((double)E) * 1.956951306108245e-6);
-    if (cdt1 > 2.0) cdt1 = 1.99999999;
-    cdtf = (float)cdt1;
-    s = 0.0f;
-    C.c[KC_SHELLEVAL] += (unsigned)my_noscco;
-    rc0 = rc1 = rc2 = rc3 = 0.0f;
-    for (int i = 0; i < my_noscco; i++) {
-      if (cg.uico[mat + i * K_MAXMAT] < E) {
-        const float t = shell_term(cg, mat, i, E, cdtf, true);
-        s += cg.fco[mat + i * K_MAXMAT] * t;
-        rc0 = (i == 0) ? t : rc0;
-        rc1 = (i == 1) ? t : rc1;
-        rc2 = (i == 2) ? t : rc2;
-        rc3 = (i == 3) ? t : rc3;
-      }
+  for (int i = 0; i < my_noscco; i++) {
+    if (cg.uico[mat + i * K_MAXMAT] < E) {
+      const float t = shell_term(cg, mat, i, E, cdtf, true);
+      s += cg.fco[mat + i * K_MAXMAT] * t;
+      rc0 = (i == 0) ? t : rc0;
+      rc1 = (i == 1) ? t : rc1;
+      rc2 = (i == 2) ? t : rc2;
+      rc3 = (i == 3) ? t : rc3;
     }
-  } while ((rn(r, C) * s0) > (s * (1.0f + tau * ((ek3 - ek2 - 1.0f) + tau * (ek2 + tau * ek3))) / (ek3 * tau * (tau * tau + 1.0f))) &&
-           ++trials < K_MAX_TRIALS);
+  }
+  cs.tau = tau;
+  cs.cdt1 = cdt1;
+  cs.cdtf = cdtf;
+  cs.s = s;
+  cs.rc0 = rc0;
+  cs.rc1 = rc1;
+  cs.rc2 = rc2;
+  cs.rc3 = rc3;
+  const bool reject =
+      (rn(r, C) * cs.s0) > (s * (1.0f + tau * ((ek3 - ek2 - 1.0f) + tau * (ek2 + tau * ek3))) / (ek3 * tau * (tau * tau + 1.0f)));
+  if (!reject) return true;
+  return ++cs.trials >= K_MAX_TRIALS;
+}
 
-  *costh = 1.0 - cdt1;
+__device__ __forceinline__ float gcoa_rnv(float E, int mat, const mcpet_compton_table& cg, const ComptonState& cs, int i)
+{
+  if (i < 4) return i == 0 ? cs.rc0 : i == 1 ? cs.rc1 : i == 2 ? cs.rc2 : cs.rc3;
+  return shell_term(cg, mat, i, E, cs.cdtf, true);
+}
 
-  pzomc = 0.0f;
-  for (int trial = 0; trial < K_MAX_TRIALS; trial++) {
-    float temp = s * rn(r, C);
-    float pac = 0.0f;
-    int ishell = my_noscco - 1;
-    for (int i = 0; i < (my_noscco - 1); i++) {
-      pac += cg.fco[mat + i * K_MAXMAT] * rnv(i);
-      if (pac > temp) {
-        ishell = i;
-        break;
-      }
+// One trial of the p_z loop; true when the loop is over (accepted or trial cap).
+template <class CN>
+__device__ __forceinline__ bool gcoa_pz_trial(float E, int mat, Rng& r, CN& C, const mcpet_compton_table& cg,
+                                              ComptonState& cs)
+{
+  const int my_noscco = cg.noscco[mat];
+  const double costh = 1.0 - cs.cdt1;
+  const float tau = cs.tau;
+  float temp = cs.s * rn(r, C);
+  float pac = 0.0f;
+  int ishell = my_noscco - 1;
+  for (int i = 0; i < (my_noscco - 1); i++) {
+    pac += cg.fco[mat + i * K_MAXMAT] * gcoa_rnv(E, mat, cg, cs, i);
+    if (pac > temp) {
+      ishell = i;
+      break;
     }
-    temp = rn(r, C) * rnv(ishell);
-    const float fj = cg.fj0[mat + ishell * K_MAXMAT];
-    if (temp < 0.5f)
-      pzomc = (0.70710678118654502f - sqrtf(0.5f - mcf_logf(temp + temp))) / (fj * 1.4142135623731f);
-    else
-      pzomc = (sqrtf(0.5f - mcf_logf(2.0f - 2.0f * temp)) - 0.70710678118654502f) / (fj * 1.4142135623731f);
-    if (pzomc < -1.0f) continue;
-    temp = (float)(tau * (tau - (*costh) * 2.f) + 1.f);
+  }
+  temp = rn(r, C) * gcoa_rnv(E, mat, cg, cs, ishell);
+  const float fj = cg.fj0[mat + ishell * K_MAXMAT];
+  float pzomc;
+  if (temp < 0.5f)
+    pzomc = (0.70710678118654502f - sqrtf(0.5f - mcf_logf(temp + temp))) / (fj * 1.4142135623731f);
+  else
+    pzomc = (sqrtf(0.5f - mcf_logf(2.0f - 2.0f * temp)) - 0.70710678118654502f) / (fj * 1.4142135623731f);
+  cs.pzomc = pzomc;
+  if (!(pzomc < -1.0f)) {
+    float af;
+    temp = (float)(tau * (tau - costh * 2.f) + 1.f);
     if (temp > 1.0e-20f)
-      af = sqrtf(temp) * (tau * (tau - ((float)(*costh))) / temp + 1.f);
+      af = sqrtf(temp) * (tau * (tau - ((float)costh)) / temp + 1.f);
     else
       af = 0.00200f;
     if (af > 0.0f) temp = af * 0.2f + 1.f;
     else temp = 1.f - af * 0.2f;
     float pzc = pzomc < 0.2f ? pzomc : 0.2f;
     pzc = pzc > -0.2f ? pzc : -0.2f;
-    if (rn(r, C) * temp < (af * pzc + 1.f)) break;
+    if (rn(r, C) * temp < (af * pzc + 1.f)) return true;
   }
-  {
-    float t, b1, b2, temp;
-    t = pzomc * pzomc;
-    b1 = 1.f - t * tau * tau;
-    b2 = 1.f - t * tau * ((float)(*costh));
-    temp = sqrtf(fabsf(b2 * b2 - b1 * (1.0f - t)));
-    if (pzomc < 0.0f) temp *= -1.0f;
-    t = (tau / b1) * (b2 + temp);
-    if (t > 1.0f) t = 1.0f;
-    (*energy) *= t;
-  }
+  return ++cs.trials >= K_MAX_TRIALS;
+}
+
+// Final energy (K:1543-1557); returns cos(theta) of the accepted trial.
+__device__ __forceinline__ double gcoa_finish(float* energy, const ComptonState& cs)
+{
+  const double costh = 1.0 - cs.cdt1;
+  const float tau = cs.tau, pzomc = cs.pzomc;
+  float t, b1, b2, temp;
+  t = pzomc * pzomc;
+  b1 = 1.f - t * tau * tau;
+  b2 = 1.f - t * tau * ((float)costh);
+  temp = sqrtf(fabsf(b2 * b2 - b1 * (1.0f - t)));
+  if (pzomc < 0.0f) temp *= -1.0f;
+  t = (tau / b1) * (b2 + temp);
+  if (t > 1.0f) t = 1.0f;
+  (*energy) *= t;
+  return costh;
 }
 
 struct Stream {
   Rng r;
   unsigned long long remaining;  // acquisition time left (ps)
   double inv_act;                // tpb / activity, decayed per emission
-  unsigned long long hist;
-  long long tid;                 // reference thread id
   int absvox, ix, iy, iz;
-  int img_t, img_sc;
+  int last;                      // 1 for the reference thread whose final seed is reported
 };
 
 // Cylinder intersection of both photons (K:1645-1707 / K:536-577).
@@ -391,8 +468,9 @@ __device__ __forceinline__ void spectrum_add(const KParams& P, int* s_spec, int 
 }
 
 // tally_Sinogram (K:1623-1836).
+template <class CN>
 K_RARE void tally_sinogram(const KParams& P, float energy, float energyFirst, const float* pos, const float* posf,
-                               const float* dir, const float* dirf, int ss, int ssf, Stream& st, Counters& C, int* s_spec)
+                               const float* dir, const float* dirf, int ss, int ssf, Stream& st, CN& C, int* s_spec)
 {
   float x0 = pos[0] - P.cx, y0 = pos[1] - P.cy;
   float dist = cyl_dist(P, x0, y0, dir);
@@ -424,17 +502,17 @@ K_RARE void tally_sinogram(const KParams& P, float energy, float energyFirst, co
   int iE2 = f2i_rd(Energia2 / 1000.0f);
   if ((iE1 < P.ne) && (iE1 >= 0)) spectrum_add(P, s_spec, iE1);
   if ((iE2 < P.ne) && (iE2 >= 0)) spectrum_add(P, s_spec, iE2);
-  C.c[KC_PAIRS]++;
+  C.inc(KC_PAIRS);
   if ((Energia1) > (P.e_low) && (Energia1) < (P.e_high) && (Energia2) > (P.e_low) && (Energia2) < (P.e_high)) {
     if (ss == 0 && ssf == 0) {
       atomicAdd(&P.trues[ibin], 1);
-      st.img_t++;
-      C.c[KC_COINC]++;
+      atomicAdd(&P.img_t[st.absvox], 1);
+      C.inc(KC_COINC);
     }
     if (ss > 0 || ssf > 0) {
       atomicAdd(&P.scatter[ibin], 1);
-      st.img_sc++;
-      C.c[KC_COINC]++;
+      atomicAdd(&P.img_sc[st.absvox], 1);
+      C.inc(KC_COINC);
     }
   }
 }
@@ -456,9 +534,10 @@ __device__ __forceinline__ void write_psf(mcpet_psf_record* rec, unsigned long l
 }
 
 // tally_PSF_coincidences (K:530-644).
+template <class CN>
 K_RARE void tally_psf(const KParams& P, float energy, float energyFirst, const float* pos_in, const float* posf_in,
                           const float* dir, const float* dirf, int ss, int ssf, float travel, float travelFirst, Stream& st,
-                          Counters& C)
+                          CN& C)
 {
   float position[3], positionFirst[3];
   float x0 = pos_in[0] - P.cx, y0 = pos_in[1] - P.cy;
@@ -552,10 +631,11 @@ __device__ __forceinline__ void photon_start(const KParams& P, Photon& ph)
 
 enum StepOutcome { STEP_ESCAPED = 0, STEP_VIRTUAL = 1, STEP_REAL = 2 };
 
-__device__ __forceinline__ int woodcock_step(const KParams& P, Photon& ph, Rng& r, Counters& C)
+template <class CN>
+__device__ __forceinline__ int woodcock_step(const KParams& P, Photon& ph, Rng& r, CN& C)
 {
   const float step = -(ph.mfpW)*mcf_logf(rn(r, C));
-  C.c[KC_WOODCOCK]++;
+  C.inc(KC_WOODCOCK);
   ph.travel += step;
   ph.pos[0] += step * ph.dir[0];
   ph.pos[1] += step * ph.dir[1];
@@ -567,7 +647,7 @@ __device__ __forceinline__ int woodcock_step(const KParams& P, Photon& ph, Rng& 
   if (ph.mat0 != ph.mat_old) {
     ph.tab = P.tot[ph.index * K_MAXMAT + ph.mat0];
     ph.mat_old = ph.mat0;
-    C.c[KC_MATCHANGE]++;
+    C.inc(KC_MATCHANGE);
   }
   ph.mfp_density = ph.mfpW * md.x;
   ph.prob = 1.0f - ph.mfp_density * (ph.tab.x + ph.E * ph.tab.y);
@@ -575,49 +655,82 @@ __device__ __forceinline__ int woodcock_step(const KParams& P, Photon& ph, Rng& 
   return (ph.randno < ph.prob) ? STEP_VIRTUAL : STEP_REAL;
 }
 
-// Real event at the photon's position: Compton, Rayleigh or photoelectric, with the dose
-// tallies. Returns false when the photon is absorbed (index < 0).
-__device__ __forceinline__ bool real_interaction(const KParams& P, Photon& ph, Rng& r, Counters& C, const mcpet_compton_table& cg,
-                                                 unsigned long long* s_mat)
+// Real event at the photon's position (K:348-413), in two halves around GCOa's loops:
+// interaction_begin picks the process; Rayleigh and photoelectric events complete there
+// (with their dose), a Compton event starts GCOa and is completed by compton_end after
+// its rejection loops. interaction_begin returns IA_ALIVE, IA_ABSORBED or IA_COMPTON.
+enum InteractionOutcome { IA_ALIVE = 0, IA_ABSORBED = 1, IA_COMPTON = 2 };
+
+template <class CN>
+__device__ __forceinline__ int interaction_begin(const KParams& P, Photon& ph, Rng& r, CN& C,
+                                                 const mcpet_compton_table& cg, ComptonState& cs, unsigned long long* s_mat)
 {
   const float4 cr = P.cr[ph.index * K_MAXMAT + ph.mat0];
   float prob = ph.prob + ph.mfp_density * (cr.x + ph.E * cr.y);
-  float randno = ph.randno;
+  const float randno = ph.randno;
   if (randno < prob) {
-    double costh_Compton;
-    randno = ph.E;
-    C.c[KC_COMPTON]++;
-    gcoa(&ph.E, &costh_Compton, ph.mat0, r, C, cg);
-    rotate_double(ph.dir, costh_Compton, 6.28318530717958647693 * rnd(r, C));
-    randno = ph.E - randno;
-    ph.index = f2i_rd((ph.E - P.e0) * P.ide);
-    if (ph.index > -1) {
-      const float2 w = P.woodcock[ph.index];
-      ph.mfpW = w.x + ph.E * w.y;
-      ph.mat_old = -2;
-      ph.ss = (ph.ss == 0) ? 1 : 3;
-    }
-  } else {
-    prob += ph.mfp_density * (cr.z + ph.E * cr.w);
-    if (randno < prob) {
-      double costh_Rayleigh;
-      C.c[KC_RAYLEIGH]++;
-      const float pmax_current = P.ray->pmax[(ph.index + 1) * K_MAXMAT + ph.mat0];
-      graa(ph.E, &costh_Rayleigh, ph.mat0, pmax_current, r, C, P.ray);
-      rotate_double(ph.dir, costh_Rayleigh, 6.28318530717958647693 * rnd(r, C));
-      ph.ss = (ph.ss == 0) ? 2 : 3;
-    } else {
-      C.c[KC_PHOTO]++;
-      randno = -ph.E;
-      ph.index = -11;
-    }
+    C.inc(KC_COMPTON);
+    gcoa_init(ph.E, ph.mat0, cg, cs);
+    return IA_COMPTON;
+  }
+  prob += ph.mfp_density * (cr.z + ph.E * cr.w);
+  if (randno < prob) {
+    double costh_Rayleigh;
+    C.inc(KC_RAYLEIGH);
+    const float pmax_current = P.ray->pmax[(ph.index + 1) * K_MAXMAT + ph.mat0];
+    graa(ph.E, &costh_Rayleigh, ph.mat0, pmax_current, r, C, P.ray);
+    rotate_double(ph.dir, costh_Rayleigh, 6.28318530717958647693 * rnd(r, C));
+    ph.ss = (ph.ss == 0) ? 2 : 3;
+    return IA_ALIVE;
+  }
+  C.inc(KC_PHOTO);
+  const float randno_pe = -ph.E;
+  ph.index = -11;
+  if (randno_pe < -0.001f) tally_dose(P, -1.0f * randno_pe, ph.mat0, ph.pos, s_mat);
+  return IA_ABSORBED;
+}
+
+// Completion of a Compton event once both GCOa loops are done: new energy and direction,
+// energy bin, Woodcock mean free path, scatter state and dose. False if absorbed (energy
+// below the tables).
+template <class CN>
+__device__ __forceinline__ bool compton_end(const KParams& P, Photon& ph, Rng& r, CN& C, const ComptonState& cs,
+                                            unsigned long long* s_mat)
+{
+  const float e_before = ph.E;
+  const double costh_Compton = gcoa_finish(&ph.E, cs);
+  rotate_double(ph.dir, costh_Compton, 6.28318530717958647693 * rnd(r, C));
+  const float randno = ph.E - e_before;
+  ph.index = f2i_rd((ph.E - P.e0) * P.ide);
+  if (ph.index > -1) {
+    const float2 w = P.woodcock[ph.index];
+    ph.mfpW = w.x + ph.E * w.y;
+    ph.mat_old = -2;
+    ph.ss = (ph.ss == 0) ? 1 : 3;
   }
   if (randno < -0.001f) tally_dose(P, -1.0f * randno, ph.mat0, ph.pos, s_mat);
   return ph.index >= 0;
 }
 
+// Whole real event, nested form. False when the photon is absorbed.
+template <class CN>
+__device__ __forceinline__ bool real_interaction(const KParams& P, Photon& ph, Rng& r, CN& C, const mcpet_compton_table& cg,
+                                                 unsigned long long* s_mat)
+{
+  ComptonState cs;
+  const int o = interaction_begin(P, ph, r, C, cg, cs, s_mat);
+  if (o != IA_COMPTON) return o == IA_ALIVE;
+  while (!gcoa_tau_trial(ph.E, ph.mat0, r, C, cg, cs)) {
+  }
+  cs.trials = 0;
+  while (!gcoa_pz_trial(ph.E, ph.mat0, r, C, cg, cs)) {
+  }
+  return compton_end(P, ph, r, C, cs, s_mat);
+}
+
 // Whole photon (K:297-415), nested loops: escape or absorption.
-__device__ __forceinline__ void transport_photon(const KParams& P, Photon& ph, Stream& st, Counters& C,
+template <class CN>
+__device__ __forceinline__ void transport_photon(const KParams& P, Photon& ph, Stream& st, CN& C,
                                                  const mcpet_compton_table& cg, unsigned long long* s_mat)
 {
   for (;;) {
@@ -632,7 +745,8 @@ __device__ __forceinline__ void transport_photon(const KParams& P, Photon& ph, S
 }
 
 // Pair emission at a uniform point of the stream's voxel (K:665-773).
-__device__ __forceinline__ void source_first(const KParams& P, Stream& st, Counters& C, float* pos, float* dir)
+template <class CN>
+__device__ __forceinline__ void source_first(const KParams& P, Stream& st, CN& C, float* pos, float* dir)
 {
   dir[2] = 1.0f - 2.0f * rn(st.r, C);
   float phi_sampled = 6.28318530716f * rn(st.r, C);
@@ -652,7 +766,8 @@ __device__ __forceinline__ void source_first(const KParams& P, Stream& st, Count
   pos[2] = ((float)st.iz + r3) / P.ivz;
 }
 
-__device__ __forceinline__ void source_second(const KParams& P, Stream& st, Counters& C, const float* dir1, float* dir)
+template <class CN>
+__device__ __forceinline__ void source_second(const KParams& P, Stream& st, CN& C, const float* dir1, float* dir)
 {
   if (P.bg_mode) {
     dir[2] = 1.0f - 2.0f * rn(st.r, C);
@@ -686,14 +801,14 @@ __device__ __forceinline__ void source_second(const KParams& P, Stream& st, Coun
 
 // Time to the next annihilation of the stream (K:261-292). False when the stream's
 // acquisition time is exhausted.
-__device__ __forceinline__ bool decay_time(const KParams& P, Stream& st, Counters& C)
+template <class CN>
+__device__ __forceinline__ bool decay_time(const KParams& P, Stream& st, CN& C)
 {
   const double Dt = -mcf_log(rnd(st.r, C)) * st.inv_act;
   const unsigned long long Dt_ps = d2ull_rn(Dt * 1e12);
   if (!(Dt_ps < st.remaining)) return false;
   st.remaining -= Dt_ps;
-  st.hist += 2;
-  C.c[KC_DECAYS]++;
+  C.inc(KC_DECAYS);              // 2 photon histories per decay (K:282)
   st.inv_act = st.inv_act * mcf_exp(Dt * P.inv_mean_life);
   return true;
 }
@@ -711,7 +826,8 @@ __device__ __forceinline__ bool pair_selected(const KParams& P, int ss, int ssF)
   return P.type == 0 || (P.type == 1 && ss == 0 && ssF == 0) || (P.type == 2 && (ss != 0 || ssF != 0));
 }
 
-__device__ __forceinline__ void tally_pair(const KParams& P, const Photon& ph, const Pair& pr, Stream& st, Counters& C, int* s_spec)
+template <class CN>
+__device__ __forceinline__ void tally_pair(const KParams& P, const Photon& ph, const Pair& pr, Stream& st, CN& C, int* s_spec)
 {
   if (P.mode == 0 || P.mode == 2) tally_sinogram(P, ph.E, pr.EF, ph.pos, pr.posF, ph.dir, pr.dirF, ph.ss, pr.ssF, st, C, s_spec);
   if (P.mode == 0 || P.mode == 1)
@@ -732,7 +848,8 @@ __device__ __forceinline__ void keep_first(Pair& pr, const Photon& ph)
 }
 
 // One annihilation, both photons, nested loops (reference main loop body, K:259-449).
-__device__ bool decay(const KParams& P, Stream& st, Counters& C, const mcpet_compton_table& cg, int* s_spec,
+template <class CN>
+__device__ bool decay(const KParams& P, Stream& st, CN& C, const mcpet_compton_table& cg, int* s_spec,
                       unsigned long long* s_mat)
 {
   if (!decay_time(P, st, C)) return false;
@@ -764,28 +881,24 @@ __device__ __forceinline__ void stream_init(const KParams& P, long long s, Strea
   st.ix = st.absvox % P.nx;
   st.iy = (st.absvox / P.nx) % P.ny;
   st.iz = st.absvox / (P.nx * P.ny);
-  st.tid = (long long)st.absvox * P.tpb + t;
-  mcr_init_stream(st.tid, P.seed, P.j1, P.j2, &st.r.x, &st.r.y);
+  const long long tid = (long long)st.absvox * P.tpb + t;
+  mcr_init_stream(tid, P.seed, P.j1, P.j2, &st.r.x, &st.r.y);
+  st.last = (tid == P.last_thread_id);
   st.remaining = P.acq_ps;
   st.inv_act = ((double)P.tpb / (double)act);
-  st.hist = 0;
-  st.img_t = 0;
-  st.img_sc = 0;
 }
 
-__device__ __forceinline__ void stream_finish(const KParams& P, Stream& st, unsigned long long& hist_acc)
+__device__ __forceinline__ void stream_finish(const KParams& P, const Stream& st)
 {
-  if (st.img_t) atomicAdd(&P.img_t[st.absvox], st.img_t);
-  if (st.img_sc) atomicAdd(&P.img_sc[st.absvox], st.img_sc);
-  hist_acc += st.hist;
-  if (st.tid == P.last_thread_id) *P.final_seed = st.r.x;
+  if (st.last) *P.final_seed = st.r.x;
 }
 
-// Workgroup prologue/epilogue shared by the kernels: Compton table, spectrum and
-// material dose in LDS; counters flushed once per lane / per workgroup.
-#define K_PROLOGUE                                                                               \
+// Workgroup prologue/epilogue shared by the kernels: Compton table, spectrum, material
+// dose and workgroup counters in LDS; counters flushed once per lane / per workgroup.
+#define K_PROLOGUE(CN)                                                                           \
   __shared__ mcpet_compton_table s_cmp;                                                          \
   __shared__ unsigned long long s_mat[2 * K_MAXMAT];                                             \
+  __shared__ unsigned s_cnt[KC_N];                                                               \
   extern __shared__ int s_spec[];                                                                \
   {                                                                                              \
     const int* src = reinterpret_cast<const int*>(P.cmp);                                        \
@@ -793,19 +906,19 @@ __device__ __forceinline__ void stream_finish(const KParams& P, Stream& st, unsi
     for (int i = threadIdx.x; i < (int)(sizeof(mcpet_compton_table) / 4); i += blockDim.x)       \
       dst[i] = src[i];                                                                           \
     if (threadIdx.x < 2 * K_MAXMAT) s_mat[threadIdx.x] = 0ull;                                   \
+    if (threadIdx.x < KC_N) s_cnt[threadIdx.x] = 0u;                                             \
     if (P.lds_spectrum)                                                                          \
       for (int i = threadIdx.x; i < P.ne; i += blockDim.x) s_spec[i] = 0;                        \
   }                                                                                              \
   __syncthreads();                                                                               \
-  Counters C;                                                                                    \
-  _Pragma("unroll") for (int i = 0; i < KC_N; i++) C.c[i] = 0;                                   \
-  unsigned long long hist_acc = 0;
+  CN C;                                                                                          \
+  C.init(s_cnt);
 
 #define K_EPILOGUE                                                                               \
-  if (hist_acc) atomicAdd(P.total_hist, hist_acc);                                               \
-  _Pragma("unroll") for (int i = 0; i < KC_N; i++) if (C.c[i])                                   \
-    atomicAdd(&P.counters[i], (unsigned long long)C.c[i]);                                       \
+  C.flush(P, s_cnt);                                                                             \
   __syncthreads();                                                                               \
+  if (threadIdx.x < KC_N && s_cnt[threadIdx.x])                                                  \
+    atomicAdd(&P.counters[threadIdx.x], (unsigned long long)s_cnt[threadIdx.x]);                 \
   if (P.lds_spectrum)                                                                            \
     for (int i = threadIdx.x; i < P.ne; i += blockDim.x)                                         \
       if (s_spec[i]) atomicAdd(&P.spectrum[i], s_spec[i]);                                       \
@@ -814,15 +927,15 @@ __device__ __forceinline__ void stream_finish(const KParams& P, Stream& st, unsi
 // Plain kernel: lane i of the grid runs stream i to the end (nested loops, no queue).
 __global__ __launch_bounds__(256) K_OCCUPANCY void k_transport_plain(KParams P)
 {
-  K_PROLOGUE
+  K_PROLOGUE(FullCounters)
   const long long sid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (sid < P.n_streams) {
     Stream st;
     stream_init(P, sid, st);
-    C.c[KC_STREAMS]++;
+    C.inc(KC_STREAMS);
     while (decay(P, st, C, s_cmp, s_spec, s_mat)) {
     }
-    stream_finish(P, st, hist_acc);
+    stream_finish(P, st);
   }
   K_EPILOGUE
 }
@@ -833,28 +946,126 @@ __global__ __launch_bounds__(256) K_OCCUPANCY void k_transport_plain(KParams P)
 //   INTERACT : one real interaction (Compton / Rayleigh / photoelectric + dose)
 //   SOURCE2  : emit photon 2 of the pair
 //   TALLY    : detector, energy blur, sinogram / PSF tallies of the pair
+//   CTAU     : one trial of GCOa's tau loop
+//   CPZ      : one trial of GCOa's p_z loop (+ completion of the Compton event)
 // and each iteration the wave executes the one phase held by the most lanes (ballot
-// counts). Lanes in other phases wait, so expensive phases (Compton rejection loops,
-// detector geometry) run with many lanes at once instead of one or two. A lane's own
+// counts). Lanes in other phases wait, so expensive phases (Compton rejection trials,
+// detector geometry) run with many lanes at once instead of one or two, and a lane that
+// needs more rejection trials than its neighbours does not hold them back. A lane's own
 // sequence of random numbers and operations is unchanged.
-enum LaneState { LS_DECAY = 0, LS_SOURCE2 = 1, LS_STEP = 2, LS_INTERACT = 3, LS_TALLY = 4, LS_N = 5, LS_DONE = 7 };
+enum LaneState {
+  LS_DECAY = 0, LS_SOURCE2 = 1, LS_STEP = 2, LS_INTERACT = 3, LS_TALLY = 4, LS_CTAU = 5, LS_CPZ = 6,
+  LS_N = 7, LS_DONE = 8
+};
 
-__device__ __forceinline__ int after_photon(const KParams& P, int photon_no, const Photon& ph, Pair& pr)
+// Per-lane state that only some phases touch lives in LDS slots instead of registers
+// (structure of arrays, slot k of lane i at [k][i]: consecutive lanes, consecutive banks):
+// the pair kept between the two photons (15 words) and the Compton sampler's state
+// between its rejection trials (14 words). Only the stream and the current photon stay
+// in registers across iterations, which keeps the kernel at 3 waves per SIMD.
+#define K_BLOCK 256
+enum PairSlot { PS_P1 = 0, PS_D1 = 3, PS_EF = 6, PS_TRAVELF = 7, PS_POSF = 8, PS_DIRF = 11, PS_SSF = 14, PS_N = 15 };
+enum ComptonSlot {
+  CS_A1 = PS_N, CS_S0, CS_TAUMIN, CS_TAU, CS_S, CS_CDTF, CS_RC0, CS_RC1, CS_RC2, CS_RC3, CS_PZOMC, CS_CDT1_LO, CS_CDT1_HI,
+  CS_TRIALS, LANE_SLOTS
+};
+
+struct LaneSlots {
+  float* p;  // &slots[0][lane]
+  __device__ __forceinline__ float& operator[](int k) const { return p[k * K_BLOCK]; }
+};
+
+__device__ __forceinline__ void pair_store_source(const LaneSlots& L, const Pair& pr)
+{
+  for (int k = 0; k < 3; k++) {
+    L[PS_P1 + k] = pr.p1[k];
+    L[PS_D1 + k] = pr.d1[k];
+  }
+  L[PS_EF] = pr.EF;
+}
+__device__ __forceinline__ void pair_store_first(const LaneSlots& L, const Photon& ph)
+{
+  Pair pr;
+  pr.EF = L[PS_EF];
+  keep_first(pr, ph);
+  if (ph.index > -1) {
+    L[PS_EF] = pr.EF;
+    L[PS_TRAVELF] = pr.travelF;
+    L[PS_SSF] = __int_as_float(pr.ssF);
+    for (int k = 0; k < 3; k++) {
+      L[PS_POSF + k] = pr.posF[k];
+      L[PS_DIRF + k] = pr.dirF[k];
+    }
+  }
+}
+__device__ __forceinline__ void pair_load(const LaneSlots& L, Pair& pr)
+{
+  for (int k = 0; k < 3; k++) {
+    pr.p1[k] = L[PS_P1 + k];
+    pr.d1[k] = L[PS_D1 + k];
+    pr.posF[k] = L[PS_POSF + k];
+    pr.dirF[k] = L[PS_DIRF + k];
+  }
+  pr.EF = L[PS_EF];
+  pr.travelF = L[PS_TRAVELF];
+  pr.ssF = __float_as_int(L[PS_SSF]);
+}
+__device__ __forceinline__ void cs_store(const LaneSlots& L, const ComptonState& cs)
+{
+  L[CS_A1] = cs.a1;
+  L[CS_S0] = cs.s0;
+  L[CS_TAUMIN] = cs.taumin;
+  L[CS_TAU] = cs.tau;
+  L[CS_S] = cs.s;
+  L[CS_CDTF] = cs.cdtf;
+  L[CS_RC0] = cs.rc0;
+  L[CS_RC1] = cs.rc1;
+  L[CS_RC2] = cs.rc2;
+  L[CS_RC3] = cs.rc3;
+  L[CS_PZOMC] = cs.pzomc;
+  const unsigned long long b = (unsigned long long)__double_as_longlong(cs.cdt1);
+  L[CS_CDT1_LO] = __uint_as_float((unsigned)b);
+  L[CS_CDT1_HI] = __uint_as_float((unsigned)(b >> 32));
+  L[CS_TRIALS] = __int_as_float(cs.trials);
+}
+__device__ __forceinline__ void cs_load(const LaneSlots& L, ComptonState& cs)
+{
+  cs.a1 = L[CS_A1];
+  cs.s0 = L[CS_S0];
+  cs.taumin = L[CS_TAUMIN];
+  cs.tau = L[CS_TAU];
+  cs.s = L[CS_S];
+  cs.cdtf = L[CS_CDTF];
+  cs.rc0 = L[CS_RC0];
+  cs.rc1 = L[CS_RC1];
+  cs.rc2 = L[CS_RC2];
+  cs.rc3 = L[CS_RC3];
+  cs.pzomc = L[CS_PZOMC];
+  const unsigned long long b = (unsigned long long)__float_as_uint(L[CS_CDT1_LO]) |
+                               ((unsigned long long)__float_as_uint(L[CS_CDT1_HI]) << 32);
+  cs.cdt1 = __longlong_as_double((long long)b);
+  cs.trials = __float_as_int(L[CS_TRIALS]);
+}
+
+// Next state once the current photon has escaped or was absorbed (K:418-442).
+__device__ __forceinline__ int after_photon(const KParams& P, int photon_no, const Photon& ph, const LaneSlots& L)
 {
   if (photon_no == 1) {
-    keep_first(pr, ph);
+    pair_store_first(L, ph);
     return LS_SOURCE2;
   }
-  if (ph.index > -1 && pr.EF > -0.5f && pair_selected(P, ph.ss, pr.ssF)) return LS_TALLY;
+  if (ph.index > -1 && L[PS_EF] > -0.5f && pair_selected(P, ph.ss, __float_as_int(L[PS_SSF]))) return LS_TALLY;
   return LS_DECAY;
 }
 
-__global__ __launch_bounds__(256) K_OCCUPANCY void k_transport_sm(KParams P)
+template <class CN>
+__global__ __launch_bounds__(K_BLOCK) K_OCCUPANCY void k_transport_sm(KParams P)
 {
-  K_PROLOGUE
+  K_PROLOGUE(CN)
+  __shared__ float s_slots[LANE_SLOTS][K_BLOCK];
+  const LaneSlots L{&s_slots[0][threadIdx.x]};
   Stream st;
   Photon ph;
-  Pair pr;
   bool have = false;
   int photon_no = 1;
   int state = LS_DECAY;
@@ -889,19 +1100,44 @@ __global__ __launch_bounds__(256) K_OCCUPANCY void k_transport_sm(KParams P)
     {
       const int lanes = __popcll(__ballot(state == best));
       if (__lane_id() == 0) {
-        C.c[KC_PHASE_EXEC + best]++;
-        C.c[KC_PHASE_LANES + best] += lanes;
+        C.add(KC_PHASE_EXEC + best, 1u);
+        C.add(KC_PHASE_LANES + best, (unsigned)lanes);
       }
     }
 #endif
     if (state != best) {
     } else if (best == LS_STEP) {
       const int o = woodcock_step(P, ph, st.r, C);
-      if (o == STEP_ESCAPED) state = after_photon(P, photon_no, ph, pr);
+      if (o == STEP_ESCAPED) state = after_photon(P, photon_no, ph, L);
       else if (o == STEP_REAL) state = LS_INTERACT;
     } else if (best == LS_INTERACT) {
-      if (!real_interaction(P, ph, st.r, C, s_cmp, s_mat) || locate(P, ph.pos) < 0) state = after_photon(P, photon_no, ph, pr);
-      else state = LS_STEP;
+      ComptonState cs;
+      const int o = interaction_begin(P, ph, st.r, C, s_cmp, cs, s_mat);
+      if (o == IA_COMPTON) {
+        cs_store(L, cs);
+        state = LS_CTAU;
+      } else if (o == IA_ABSORBED || locate(P, ph.pos) < 0) {
+        state = after_photon(P, photon_no, ph, L);
+      } else {
+        state = LS_STEP;
+      }
+    } else if (best == LS_CTAU) {
+      ComptonState cs;
+      cs_load(L, cs);
+      if (gcoa_tau_trial(ph.E, ph.mat0, st.r, C, s_cmp, cs)) {
+        cs.trials = 0;
+        state = LS_CPZ;
+      }
+      cs_store(L, cs);
+    } else if (best == LS_CPZ) {
+      ComptonState cs;
+      cs_load(L, cs);
+      if (gcoa_pz_trial(ph.E, ph.mat0, st.r, C, s_cmp, cs)) {
+        if (!compton_end(P, ph, st.r, C, cs, s_mat) || locate(P, ph.pos) < 0) state = after_photon(P, photon_no, ph, L);
+        else state = LS_STEP;
+      } else {
+        L[CS_TRIALS] = __int_as_float(cs.trials);
+      }
     } else if (best == LS_DECAY) {
       for (;;) {
         if (!have) {
@@ -911,36 +1147,45 @@ __global__ __launch_bounds__(256) K_OCCUPANCY void k_transport_sm(KParams P)
             break;
           }
           stream_init(P, sid, st);
-          C.c[KC_STREAMS]++;
+          C.inc(KC_STREAMS);
           have = true;
         }
         if (decay_time(P, st, C)) break;
-        stream_finish(P, st, hist_acc);
+        stream_finish(P, st);
         have = false;
       }
       if (state != LS_DONE) {
+        Pair pr;
         source_first(P, st, C, pr.p1, pr.d1);
         pr.EF = -10.0f;
+        pair_store_source(L, pr);
         for (int k = 0; k < 3; k++) {
           ph.pos[k] = pr.p1[k];
           ph.dir[k] = pr.d1[k];
         }
         photon_start(P, ph);
         photon_no = 1;
-        state = (locate(P, ph.pos) < 0) ? after_photon(P, 1, ph, pr) : LS_STEP;
+        state = (locate(P, ph.pos) < 0) ? after_photon(P, 1, ph, L) : LS_STEP;
       }
     } else if (best == LS_SOURCE2) {
-      source_second(P, st, C, pr.d1, ph.dir);
-      for (int k = 0; k < 3; k++) ph.pos[k] = pr.p1[k];
+      float p1[3], d1[3];
+      for (int k = 0; k < 3; k++) {
+        p1[k] = L[PS_P1 + k];
+        d1[k] = L[PS_D1 + k];
+      }
+      source_second(P, st, C, d1, ph.dir);
+      for (int k = 0; k < 3; k++) ph.pos[k] = p1[k];
       photon_start(P, ph);
       photon_no = 2;
-      state = (locate(P, ph.pos) < 0) ? after_photon(P, 2, ph, pr) : LS_STEP;
+      state = (locate(P, ph.pos) < 0) ? after_photon(P, 2, ph, L) : LS_STEP;
     } else {  // LS_TALLY
+      Pair pr;
+      pair_load(L, pr);
       tally_pair(P, ph, pr, st, C, s_spec);
       state = LS_DECAY;
     }
 #if MCPET_SM_STATS
-    if (__lane_id() == 0) C.c[KC_PHASE_CYCLES + best] += (unsigned long long)(clock64() - t_phase);
+    if (__lane_id() == 0) C.add(KC_PHASE_CYCLES + best, (unsigned)(clock64() - t_phase));
 #endif
   }
   K_EPILOGUE
@@ -996,11 +1241,14 @@ __global__ void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, s
 
 }  // namespace
 
-hipError_t mcpet_launch_transport(const KParams& P, int kind, int blocks, int block_size, hipStream_t stream)
+hipError_t mcpet_launch_transport(const KParams& P, int kind, int count_events, int blocks, int block_size, hipStream_t stream)
 {
+  if (block_size > K_BLOCK || block_size % 64 != 0) return hipErrorInvalidValue;
   size_t lds = P.lds_spectrum ? (size_t)P.ne * sizeof(int) : 0;
-  if (kind == MCPET_KERNEL_PERSISTENT)
-    hipLaunchKernelGGL(k_transport_sm, dim3(blocks), dim3(block_size), lds, stream, P);
+  if (kind == MCPET_KERNEL_PERSISTENT && count_events)
+    hipLaunchKernelGGL(k_transport_sm<FullCounters>, dim3(blocks), dim3(block_size), lds, stream, P);
+  else if (kind == MCPET_KERNEL_PERSISTENT)
+    hipLaunchKernelGGL(k_transport_sm<LiteCounters>, dim3(blocks), dim3(block_size), lds, stream, P);
   else
     hipLaunchKernelGGL(k_transport_plain, dim3(blocks), dim3(block_size), lds, stream, P);
   return hipGetLastError();

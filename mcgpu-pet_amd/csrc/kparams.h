@@ -21,10 +21,10 @@ enum KCounter {
   KC_PAIRS,
   KC_COINC,
   KC_RNDRAWS,
-  KC_PHASE_CYCLES = 11, // per-phase shader clock cycles, summed over waves (5 entries, MCPET_SM_STATS builds)
-  KC_PHASE_EXEC = 16,   // per-phase executions of the persistent kernel (5 entries, MCPET_SM_STATS builds)
-  KC_PHASE_LANES = 24,  // per-phase sum of active lanes (5 entries)
-  KC_N = 32
+  KC_PHASE_CYCLES = 16, // per-phase shader clock cycles, summed over waves (MCPET_SM_STATS builds)
+  KC_PHASE_EXEC = 24,   // per-phase executions of the persistent kernel (MCPET_SM_STATS builds)
+  KC_PHASE_LANES = 32,  // per-phase sum of active lanes (MCPET_SM_STATS builds)
+  KC_N = 40
 };
 
 struct KParams {
@@ -75,18 +75,17 @@ struct KParams {
   unsigned long long* voxel_edep;
   unsigned long long* mat_edep;
   unsigned long long* counters;  // KC_N
-  unsigned long long* total_hist;
   int* final_seed;
   long long last_thread_id;      // reference thread id whose final seed is stored (K:454-455)
   int lds_spectrum;              // 1: privatise the spectrum in LDS
   // Phase selection of the persistent kernel: 0 = phase held by most lanes;
   // 1 = phase with the fewest wasted lane-slots, (64 - lanes) * phase_cost.
   int sm_policy;
-  int sm_cost[8];                // relative cost per phase (DECAY, SOURCE2, STEP, INTERACT, TALLY)
+  int sm_cost[8];                // relative cost per phase (DECAY, SOURCE2, STEP, INTERACT, TALLY, CTAU, CPZ)
 };
 
 // Launchers implemented in kernels.hip.
-hipError_t mcpet_launch_transport(const KParams& P, int kind, int blocks, int block_size, hipStream_t stream);
+hipError_t mcpet_launch_transport(const KParams& P, int kind, int count_events, int blocks, int block_size, hipStream_t stream);
 hipError_t mcpet_launch_selftest_ranecu(int seed, const long long* d_ids, int nstreams, int n, float* d_out, double* d_out_d,
                                         int as_double, const uint32_t* j1, const uint32_t* j2, hipStream_t stream);
 hipError_t mcpet_launch_selftest_fmath(const double* d_x, const double* d_y, int n, double* d_od, float* d_of, hipStream_t stream);

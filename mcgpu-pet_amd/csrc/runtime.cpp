@@ -48,7 +48,7 @@ struct mcpet_ctx {
   int* d_psf_index = nullptr;
   unsigned long long* d_voxel_edep = nullptr;
   unsigned long long* d_mat_edep = nullptr;
-  unsigned long long* d_counters = nullptr;  // KC_N counters + queue + total histories
+  unsigned long long* d_counters = nullptr;  // KC_N counters + stream queue
   int* d_final_seed = nullptr;
   bool ran = false;
   bool timed = false;
@@ -257,7 +257,7 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
   CK(dalloc(c, &c->d_psf_index, 1));
   if (voxel_dose) CK(dalloc(c, &c->d_voxel_edep, 2 * (size_t)c->n_roi));
   CK(dalloc(c, &c->d_mat_edep, 2 * MCPET_MAX_MATERIALS));
-  CK(dalloc(c, &c->d_counters, KC_N + 2));
+  CK(dalloc(c, &c->d_counters, KC_N + 1));
   CK(dalloc(c, &c->d_final_seed, 1));
 
   HK(hipMemcpy(c->d_vox, hv.data(), hv.size() * sizeof(float2), hipMemcpyHostToDevice));
@@ -287,7 +287,6 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
   P.tpb = cfg->threads_per_block;
   P.n_streams = c->n_active * (long long)cfg->threads_per_block;
   P.queue = c->d_counters + KC_N;
-  P.total_hist = c->d_counters + KC_N + 1;
   P.seed = cfg->seed;
   P.bg_mode = cfg->bg_mode;
   P.acq_ps = cfg->acquisition_time_ps;
@@ -380,8 +379,8 @@ extern "C" mcpet_status mcpet_run(mcpet_ctx* c, const mcpet_run_params* rp)
   // Phase selection (persistent kernel): most lanes by default (measured best on the head
   // workload); policy 2 weighs the phases by relative cost (DECAY, SOURCE2, STEP, INTERACT, TALLY).
   P.sm_policy = (p.phase_policy == 2) ? 1 : 0;
-  static const int default_cost[5] = {4, 6, 1, 10, 6};
-  for (int i = 0; i < 5; i++) P.sm_cost[i] = p.phase_cost[i] > 0 ? p.phase_cost[i] : default_cost[i];
+  static const int default_cost[MCPET_NPHASES] = {4, 6, 1, 3, 6, 3, 2, 1};
+  for (int i = 0; i < MCPET_NPHASES; i++) P.sm_cost[i] = p.phase_cost[i] > 0 ? p.phase_cost[i] : default_cost[i];
   int kind = p.kernel == MCPET_KERNEL_AUTO ? MCPET_KERNEL_PERSISTENT : p.kernel;
   int bs = p.block_size > 0 ? p.block_size : 256;
   if (bs % 64 != 0 || bs > 256) return set_err(c, MCPET_E_RANGE, "mcpet_run: block_size must be a multiple of 64, <= 256");
@@ -390,7 +389,7 @@ extern "C" mcpet_status mcpet_run(mcpet_ctx* c, const mcpet_run_params* rp)
     mcpet_status st = zero_tallies(c);
     if (st != MCPET_OK) return st;
   }
-  HIPCHK(c, hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * (KC_N + 2), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * (KC_N + 1), c->stream));
   HIPCHK(c, hipEventRecord(c->ev_k0, c->stream));
   long long blocks;
   if (kind == MCPET_KERNEL_PERSISTENT) {
@@ -403,7 +402,7 @@ extern "C" mcpet_status mcpet_run(mcpet_ctx* c, const mcpet_run_params* rp)
     if (blocks > 0x7fffffffLL) return set_err(c, MCPET_E_RANGE, "mcpet_run: too many streams for one launch");
   }
   if (P.n_streams > 0) {
-    hipError_t e = mcpet_launch_transport(P, kind, (int)blocks, bs, c->stream);
+    hipError_t e = mcpet_launch_transport(P, kind, p.count_events, (int)blocks, bs, c->stream);
     if (e != hipSuccess) return set_err(c, MCPET_E_HIP, std::string("transport launch failed: ") + hipGetErrorString(e));
   }
   HIPCHK(c, hipEventRecord(c->ev_k1, c->stream));
@@ -445,9 +444,9 @@ extern "C" mcpet_status mcpet_fetch(mcpet_ctx* c, mcpet_tallies* t)
   if (t->voxel_edep && c->d_voxel_edep)
     HIPCHK(c, hipMemcpy(t->voxel_edep, c->d_voxel_edep, 16 * (size_t)c->n_roi, hipMemcpyDeviceToHost));
   if (t->material_edep) HIPCHK(c, hipMemcpy(t->material_edep, c->d_mat_edep, 16 * MCPET_MAX_MATERIALS, hipMemcpyDeviceToHost));
-  unsigned long long cnt[KC_N + 2];
+  unsigned long long cnt[KC_N + 1];
   HIPCHK(c, hipMemcpy(cnt, c->d_counters, sizeof cnt, hipMemcpyDeviceToHost));
-  t->total_histories = cnt[KC_N + 1];
+  t->total_histories = 2ull * cnt[KC_DECAYS];  // two photon histories per decay (K:282)
   HIPCHK(c, hipMemcpy(&t->final_seed, c->d_final_seed, sizeof(int), hipMemcpyDeviceToHost));
   // Sums over the sinograms (host copies if fetched, else device-side counters).
   if (t->trues && t->scatter) {
@@ -477,7 +476,7 @@ extern "C" mcpet_status mcpet_get_stats(mcpet_ctx* c, mcpet_stats* s)
   HIPCHK(c, hipEventElapsedTime(&zms, c->ev_z0, c->ev_k0));
   s->kernel_ms = ms;
   s->zero_ms = zms;
-  unsigned long long cnt[KC_N + 2];
+  unsigned long long cnt[KC_N + 1];
   HIPCHK(c, hipMemcpy(cnt, c->d_counters, sizeof cnt, hipMemcpyDeviceToHost));
   s->decays = cnt[KC_DECAYS];
   s->streams = cnt[KC_STREAMS];
@@ -490,7 +489,7 @@ extern "C" mcpet_status mcpet_get_stats(mcpet_ctx* c, mcpet_stats* s)
   s->pairs_tallied = cnt[KC_PAIRS];
   s->coincidences = cnt[KC_COINC];
   s->rn_draws = cnt[KC_RNDRAWS];
-  for (int i = 0; i < 5; i++) {
+  for (int i = 0; i < MCPET_NPHASES; i++) {
     s->phase_exec[i] = cnt[KC_PHASE_EXEC + i];
     s->phase_lanes[i] = cnt[KC_PHASE_LANES + i];
     s->phase_cycles[i] = cnt[KC_PHASE_CYCLES + i];

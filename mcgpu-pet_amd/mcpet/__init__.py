@@ -19,6 +19,7 @@ LIB_PATH = os.environ.get("MCPET_LIB") or os.path.join(_PKG, "lib", "libmcpet.so
 
 MAX_MATERIALS = 15
 MAX_SHELLS = 30
+NPHASES = 8
 NP_RAYLEIGH = 128
 MAX_ENERGYBINS_RAYLEIGH = 25005
 NAME_LEN = 250
@@ -106,7 +107,8 @@ class Tallies(C.Structure):
 class RunParams(C.Structure):
     _fields_ = [("kernel", C.c_int32), ("block_size", C.c_int32), ("blocks_per_cu", C.c_int32),
                 ("zero_tallies", C.c_int32), ("seed", C.c_int32), ("acquisition_time_ps", C.c_uint64),
-                ("phase_policy", C.c_int32), ("phase_cost", C.c_int32 * 5)]
+                ("phase_policy", C.c_int32), ("phase_cost", C.c_int32 * NPHASES),
+                ("count_events", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -114,8 +116,8 @@ class Stats(C.Structure):
                 ("woodcock_steps", C.c_uint64), ("material_changes", C.c_uint64), ("compton", C.c_uint64),
                 ("compton_shell_evals", C.c_uint64), ("rayleigh", C.c_uint64), ("photoelectric", C.c_uint64),
                 ("pairs_tallied", C.c_uint64), ("coincidences", C.c_uint64), ("rn_draws", C.c_uint64),
-                ("phase_exec", C.c_uint64 * 5), ("phase_lanes", C.c_uint64 * 5),
-                ("phase_cycles", C.c_uint64 * 5)]
+                ("phase_exec", C.c_uint64 * NPHASES), ("phase_lanes", C.c_uint64 * NPHASES),
+                ("phase_cycles", C.c_uint64 * NPHASES)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
@@ -307,9 +309,11 @@ class Simulation:
             raise McpetError(st, lib().mcpet_last_error(None).decode(errors="replace"))
 
     def run(self, kernel=KERNEL_AUTO, block_size=0, blocks_per_cu=0, zero_tallies=True, seed=0, acquisition_time_ps=0,
-            phase_policy=0, phase_cost=None):
+            phase_policy=0, phase_cost=None, count_events=False):
+        """Launch the transport (asynchronous). count_events selects the instrumented
+        persistent kernel, which fills every event count of stats()."""
         rp = RunParams(kernel, block_size, blocks_per_cu, 1 if zero_tallies else 0, seed, acquisition_time_ps,
-                       phase_policy, (C.c_int32 * 5)(*(phase_cost or [0] * 5)))
+                       phase_policy, (C.c_int32 * NPHASES)(*(phase_cost or [0] * NPHASES)), 1 if count_events else 0)
         _check(lib().mcpet_run(self.ctx, C.byref(rp)), self.ctx)
 
     def sync(self):

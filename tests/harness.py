@@ -28,9 +28,9 @@ class Scenario:
     def run_oracle(self, flavor="fmath", nthreads=8, **kw):
         return self.oracle(flavor).run(nthreads=nthreads, **kw)
 
-    def run_gpu(self, kernel=mcpet.KERNEL_AUTO, **kw):
+    def run_gpu(self, kernel=mcpet.KERNEL_AUTO, count_events=True, **kw):
         with mcpet.Simulation(self.cfg, self.phantom, self.tables) as sim:
-            sim.run(kernel=kernel, **kw)
+            sim.run(kernel=kernel, count_events=count_events, **kw)
             out = sim.fetch()
             out["stats"] = sim.stats()
         return out
@@ -80,7 +80,7 @@ def psf_sorted(p):
     return raw[order]
 
 
-def assert_tallies_equal(g, o):
+def assert_tallies_equal(g, o, counters="all"):
     for k in ("trues", "scatter", "image_trues", "image_scatter", "spectrum"):
         assert np.array_equal(g[k], o[k]), f"{k} differs: gpu sum {g[k].sum()} oracle sum {o[k].sum()}"
     assert int(g["total_histories"]) == int(o["total_histories"])
@@ -91,5 +91,7 @@ def assert_tallies_equal(g, o):
         assert g["psf_requested"] == o["psf_requested"]
         assert np.array_equal(psf_sorted(g["psf"]), psf_sorted(o["psf"]))
     assert g["final_seed"] == o["final_seed"]
+    lite = ("decays", "streams", "pairs_tallied", "coincidences")
     for k, v in o["counters"].items():
-        assert g["stats"][k] == v, f"counter {k}: gpu {g['stats'][k]} oracle {v}"
+        if counters == "all" or k in lite:
+            assert g["stats"][k] == v, f"counter {k}: gpu {g['stats'][k]} oracle {v}"

@@ -132,7 +132,7 @@ def test_head_full_size_invariants():
     ph = mcpet.phantom_from_arrays(mat, dens, act, vs)
     tab = mcpet.load_materials(names, ph.density_max())
     with mcpet.Simulation(cfg, ph, tab) as sim:
-        sim.run()
+        sim.run(count_events=True)
         out = sim.fetch(psf=False)
         st = sim.stats()
     decays = st["decays"]
@@ -151,3 +151,11 @@ def test_head_full_size_invariants():
     for key in ("coincidences", "compton", "woodcock_steps"):
         rg, ro = st[key] / decays, o["counters"][key] / od
         assert abs(rg - ro) / ro < 0.05, f"{key}/decay gpu {rg:.4f} oracle-sample {ro:.4f}"
+
+
+def test_production_kernel_equals_instrumented_and_oracle():
+    _say("water cube 2e5 decays: uninstrumented persistent kernel vs oracle (tallies, PSF, doses, lite counters)")
+    sc = water_cube_scenario(n_decays=2e5, tpb=32, tally_psf_sinogram=0, psf_size=500_000, material_dose=True)
+    o = sc.run_oracle("fmath")
+    g = sc.run_gpu(kernel=mcpet.KERNEL_PERSISTENT, count_events=False)
+    assert_tallies_equal(g, o, counters="lite")
