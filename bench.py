@@ -75,9 +75,12 @@ def main():
     ap.add_argument("--block-size", type=int, default=0)
     ap.add_argument("--phase-policy", type=int, default=0, help="0 auto, 1 most lanes, 2 least waste")
     ap.add_argument("--phase-cost", type=str, default="", help="comma-separated relative phase costs (DECAY, SOURCE2, STEP, INTERACT, TALLY, CTAU, CPZ)")
+    ap.add_argument("--voxel-format", type=int, default=0, help="0 auto (palette indices), 1 8-byte voxel words")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-stride", type=int, default=5)
+    ap.add_argument("--copy-calibration", type=float, default=0,
+                    help="only run the library's device copy of this many bytes (HBM counter calibration) and exit")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one MI355X only; multi-GPU runs are not supported "
@@ -85,11 +88,17 @@ def main():
         sys.exit(2)
 
     import mcpet
+    if a.copy_calibration:
+        n = int(a.copy_calibration)
+        ms = mcpet.selftest_copy(n, iters=5)
+        print(json.dumps({"copy_bytes": n, "iters": 5, "ms_per_copy": ms, "GB/s": 2 * n / ms / 1e6}))
+        return
     from mcpet import phantoms
     mat, dens, act, vs = phantoms.head(n_decays=a.decays)
     names = phantoms.head_materials(MATERIALS)
     cfg = phantoms.make_config(seed=20261202, threads_per_block=a.tpb, acquisition_time_s=10.0, nrad=168,
                                materials=names)
+    cfg.voxel_format = a.voxel_format
     ph = mcpet.phantom_from_arrays(mat, dens, act, vs)
     tab = mcpet.load_materials(names, ph.density_max())
     kernel = mcpet.KERNEL_PERSISTENT if a.kernel == "persistent" else mcpet.KERNEL_PLAIN
@@ -154,7 +163,7 @@ def main():
         tot_cyc = max(1, sum(agg["phase_cycles"]))
         out["phase_stats"] = {n: {"exec_per_decay": e / max(1, decays), "mean_lanes": l / max(1, e),
                                   "cycle_share": c / tot_cyc, "cycles_per_exec": c / max(1, e)}
-                              for n, e, l, c in zip(("decay", "source2", "step", "interact", "tally", "ctau", "cpz"),
+                              for n, e, l, c in zip(("decay", "source2", "step", "interact", "tally", "ctau", "cpz", "detect", "source3"),
                                                     agg["phase_exec"], agg["phase_lanes"], agg["phase_cycles"])}
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, ph, vs, names, a.cpu_threads, a.cpu_stride)

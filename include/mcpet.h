@@ -31,7 +31,7 @@ extern "C" {
 #define MCPET_NP_RAYLEIGH 128
 #define MCPET_MAX_ENERGYBINS_RAYLEIGH 25005
 #define MCPET_NAME_LEN 250
-#define MCPET_NPHASES 8                /* phase slots of the persistent kernel's statistics */
+#define MCPET_NPHASES 10               /* phase slots of the persistent kernel's statistics */
 
 typedef enum mcpet_status {
   MCPET_OK = 0,
@@ -110,7 +110,34 @@ typedef struct mcpet_config {
   char voxel_file[MCPET_NAME_LEN];
   char material_files[MCPET_MAX_MATERIALS][MCPET_NAME_LEN];
   int32_t bg_mode;               /* 1: second photon isotropic ("BG" run, C:81-84)          */
+  int32_t voxel_format;          /* device voxel layout: 0 auto (palette indices when the phantom
+                                    has <= 65536 distinct (material, density) pairs), 1 always the
+                                    8-byte (density, material) word. Results are identical.    */
+  /* Extensions beyond the reference; all-zero values give the reference's behaviour.
+   * tally_psf_sinogram = MCPET_TALLY_SINGLES selects the singles pipeline: the transport
+   * writes every detected photon to a singles list, which is sorted by detection time and
+   * paired with a coincidence timing window into trues, scatter, randoms and spurious
+   * (prompt-gamma) coincidences. The reference's files cannot request it. */
+  float positron_fraction;       /* fraction of decays emitting an annihilation pair (<= 0: 1) */
+  float prompt_gamma_fraction;   /* fraction of decays emitting a prompt gamma (0: none)       */
+  float prompt_gamma_energy;     /* eV, inside the interaction tables                          */
+  int32_t reserved0;
+  int64_t singles_capacity;      /* records of the device singles list                         */
+  uint64_t coincidence_window_ps;/* coincidence timing window (ps)                             */
 } mcpet_config;
+
+/* tally_psf_sinogram value of the singles pipeline (the reference uses 0, 1, 2). */
+#define MCPET_TALLY_SINGLES 3
+
+/* One detected photon of the singles pipeline (32 bytes). */
+typedef struct mcpet_single {
+  uint64_t time_ps;              /* emission time + flight time to the detector (ps)           */
+  uint64_t event;                /* decay identifier: (reference thread id << 32) | decay number */
+  float energy;                  /* energy after the detector's energy blur (eV)               */
+  float z, phi;                  /* hit on the detector cylinder (cm, rad), relative to its centre */
+  int16_t scatter;               /* 0 none, 1 Compton, 2 Rayleigh, 3 multiple                  */
+  int16_t kind;                  /* 0, 1: annihilation photon 1, 2; 2: prompt gamma             */
+} mcpet_single;
 
 /* Voxelized phantom (load_voxels, MCGPU-PET.cu:1381-1533). */
 typedef struct mcpet_phantom {
@@ -153,6 +180,16 @@ typedef struct mcpet_tallies {
   int64_t psf_requested;         /* records that reached the PSF stage (index_PSF before clamp) */
   int64_t sum_trues, sum_scatter;
   int32_t final_seed;            /* seed.x of the last stream (K:454-455); 0 if not written */
+  /* singles pipeline (tally_psf_sinogram = MCPET_TALLY_SINGLES); trues/scatter above hold
+     its true and scattered coincidences */
+  int32_t* randoms;              /* nbins: coincidences of singles from different decays    */
+  int32_t* spurious;             /* nbins: a prompt gamma paired with a photon of its decay  */
+  mcpet_single* singles;         /* the singles list sorted by detection time (may be NULL) */
+  int64_t singles_buffer;        /* capacity of `singles` in records                          */
+  int64_t singles_count;         /* singles stored (clamped to the device capacity)          */
+  int64_t singles_requested;     /* singles detected (before the clamp)                      */
+  int64_t coincidences[5];       /* binned trues, scatter, randoms, spurious; clusters of more
+                                    than two singles (multiples, discarded)                  */
 } mcpet_tallies;
 
 /* Transport launch variants. */
@@ -195,10 +232,17 @@ typedef struct mcpet_stats {
   uint64_t rn_draws;
   /* persistent kernel, diagnostic builds only: executions, active lanes and shader clock
      cycles (summed over waves) per phase: DECAY, SOURCE2, STEP, INTERACT, TALLY, CTAU (one
-     trial of GCOa's tau loop), CPZ (one trial of its p_z loop); zero otherwise */
+     trial of GCOa's tau loop), CPZ (one trial of its p_z loop), DETECT (singles list),
+     SOURCE3 (prompt gamma); zero otherwise */
   uint64_t phase_exec[MCPET_NPHASES];
   uint64_t phase_lanes[MCPET_NPHASES];
   uint64_t phase_cycles[MCPET_NPHASES];
+  /* singles pipeline: time of the sort and pairing (HIP events), singles detected; in this
+     mode `coincidences` counts the binned true and scattered coincidences of the pairing */
+  double coinc_ms;
+  uint64_t singles;
+  uint64_t pairs_emitted;        /* decays that emitted an annihilation pair              */
+  uint64_t prompts_emitted;      /* decays that emitted a prompt gamma                    */
 } mcpet_stats;
 
 /* ------------------------------------------------------------------ host-side input (boundary) */
@@ -295,11 +339,20 @@ mcpet_status mcpet_selftest_ranecu(int32_t seed, const int64_t* stream_ids, int3
 mcpet_status mcpet_selftest_fmath(const double* x, const double* y, int32_t n, double* out_d /* 7*n */,
                                   float* out_f /* 8*n */);
 
+/* Stable device radix sort of n 64-bit keys by their low 8*passes bits (passes 1..8), the
+ * sort of the singles pipeline; perm receives the sorted order of the input indices. */
+mcpet_status mcpet_selftest_sort(const uint64_t* keys, int64_t n, int32_t passes, uint32_t* perm);
+
 /* Device copy of nbytes (for HBM-counter calibration: FETCH_SIZE/WRITE_SIZE of a known copy). */
 mcpet_status mcpet_selftest_copy(size_t nbytes, int32_t iters, double* ms_out);
 
 /* Library version string. */
 const char* mcpet_version(void);
+
+/* sizeof of the ABI structs, in this order: mcpet_config, mcpet_phantom, mcpet_tables,
+ * mcpet_tallies, mcpet_run_params, mcpet_stats, mcpet_psf_record, mcpet_single (lets a
+ * binding check its struct layouts). Returns the number of sizes written (<= n). */
+int32_t mcpet_struct_sizes(size_t* out, int32_t n);
 
 #ifdef __cplusplus
 }

@@ -53,6 +53,7 @@ def build_lib(jobs):
     os.makedirs(BIN, exist_ok=True)
     units = [
         ([HIPCC, "-x", "hip", *HIP_FLAGS, "-c", os.path.join(CSRC, "kernels.hip"), "-o", os.path.join(BUILD, "kernels.o")]),
+        ([HIPCC, "-x", "hip", *HIP_FLAGS, "-c", os.path.join(CSRC, "coinc.hip"), "-o", os.path.join(BUILD, "coinc.o")]),
         ([HIPCC, "-x", "hip", *HIP_FLAGS, "-c", os.path.join(CSRC, "runtime.cpp"), "-o", os.path.join(BUILD, "runtime.o")]),
         ([CXX, *HOST_FLAGS, "-c", os.path.join(CSRC, "host_io.cpp"), "-o", os.path.join(BUILD, "host_io.o")]),
     ]
@@ -61,7 +62,8 @@ def build_lib(jobs):
     so = os.path.join(LIB, "libmcpet.so")
     tmp = so + ".tmp"
     run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp,
-         os.path.join(BUILD, "kernels.o"), os.path.join(BUILD, "runtime.o"), os.path.join(BUILD, "host_io.o"), "-lz"])
+         os.path.join(BUILD, "kernels.o"), os.path.join(BUILD, "coinc.o"), os.path.join(BUILD, "runtime.o"),
+         os.path.join(BUILD, "host_io.o"), "-lz"])
     os.replace(tmp, so)
     cli = os.path.join(BIN, "mcgpu_pet")
     run([CXX, *HOST_FLAGS, "-o", cli + ".tmp", os.path.join(CSRC, "cli.cpp"), "-L" + LIB, "-lmcpet",

@@ -1,0 +1,281 @@
+// coinc.hip — singles pipeline after the transport: sort the singles by detection time and
+// pair them with a coincidence timing window (no counterpart in the reference, which only
+// tallies the two photons of each annihilation, MCGPU-PET_kernel.cu:1623-1836).
+//
+// Sort: least-significant-digit radix sort of the 64-bit detection times with the record
+// index as payload, 8-bit digits, as many passes as the largest time needs. Each pass is
+// three kernels over tiles of RS_TILE keys: per-tile digit histograms (LDS atomics), an
+// exclusive scan of the digit-major histogram table (global offsets), and a stable scatter
+// in which every wave ranks its 64 keys by digit with eight ballots and the workgroup adds
+// the counts of earlier waves and earlier rounds of the tile from LDS. Stable at every
+// level, so equal times keep the order of the singles list.
+//
+// Pairing: a coincidence is a cluster of exactly two singles whose times differ by at most
+// the window, with gaps larger than the window on both sides; a cluster of three or more
+// singles is a multiple and is discarded. Every single decides locally whether it opens such
+// a cluster (at most two neighbours read), so the kernel is one parallel pass. A pair is
+// true (same decay, both unscattered annihilation photons), scattered (same decay, either
+// scattered), spurious (same decay, one photon is the prompt gamma) or random (different
+// decays), and is binned with the reference's Michelogram arithmetic (earlier single first;
+// singles with equal times in the order of (event, kind)).
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "devutil.h"
+#include "kparams.h"
+
+#define RS_THREADS 256
+#define RS_ROUNDS 16
+#define RS_TILE (RS_THREADS * RS_ROUNDS)
+#define SCAN_THREADS 256
+#define SCAN_ITEMS 4
+#define SCAN_TILE (SCAN_THREADS * SCAN_ITEMS)
+
+namespace {
+
+__global__ void k_keys_from_singles(const mcpet_single* __restrict__ s, long long n, unsigned long long* __restrict__ keys,
+                                    unsigned* __restrict__ vals)
+{
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    keys[i] = s[i].time_ps;
+    vals[i] = (unsigned)i;
+  }
+}
+
+// hist[d * ntiles + tile] = occurrences of digit d in the tile.
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const unsigned long long* __restrict__ keys, long long n, int shift,
+                                                        unsigned* __restrict__ hist, int ntiles)
+{
+  __shared__ unsigned h[256];
+  h[threadIdx.x] = 0u;
+  __syncthreads();
+  const long long base = (long long)blockIdx.x * RS_TILE;
+  for (int r = 0; r < RS_ROUNDS; r++) {
+    const long long i = base + (long long)r * RS_THREADS + threadIdx.x;
+    if (i < n) atomicAdd(&h[(unsigned)(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter of one tile by digit.
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const unsigned long long* __restrict__ kin,
+                                                           const unsigned* __restrict__ vin, unsigned long long* __restrict__ kout,
+                                                           unsigned* __restrict__ vout, long long n, int shift,
+                                                           const unsigned* __restrict__ offs, int ntiles)
+{
+  __shared__ unsigned s_base[256];
+  __shared__ unsigned s_wcnt[RS_THREADS / 64][256];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  s_base[tid] = offs[(size_t)tid * ntiles + blockIdx.x];
+  const long long base = (long long)blockIdx.x * RS_TILE;
+  for (int r = 0; r < RS_ROUNDS; r++) {
+    const long long i = base + (long long)r * RS_THREADS + tid;
+    const bool valid = i < n;
+    const unsigned long long key = valid ? kin[i] : 0ull;
+    const unsigned val = valid ? vin[i] : 0u;
+    const unsigned d = (unsigned)(key >> shift) & 255u;
+    unsigned long long m = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      const bool bit = (d >> b) & 1u;
+      const unsigned long long bb = __ballot(valid && bit);
+      m &= bit ? bb : ~bb;
+    }
+    const unsigned rank = (unsigned)__popcll(m & lt);
+#pragma unroll
+    for (int w = 0; w < RS_THREADS / 64; w++) s_wcnt[w][tid] = 0u;
+    __syncthreads();
+    if (valid && rank == 0u) s_wcnt[wave][d] = (unsigned)__popcll(m);
+    __syncthreads();
+    if (valid) {
+      unsigned pos = s_base[d] + rank;
+      for (int w = 0; w < wave; w++) pos += s_wcnt[w][d];
+      kout[pos] = key;
+      vout[pos] = val;
+    }
+    __syncthreads();
+    unsigned add = 0u;
+#pragma unroll
+    for (int w = 0; w < RS_THREADS / 64; w++) add += s_wcnt[w][tid];
+    s_base[tid] += add;
+    __syncthreads();
+  }
+}
+
+// Exclusive scan of one tile of SCAN_TILE values; the tile total goes to sums[tile].
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_tile(const unsigned* __restrict__ in, unsigned* __restrict__ out,
+                                                             long long m, unsigned* __restrict__ sums)
+{
+  __shared__ unsigned s[SCAN_THREADS];
+  const long long base = (long long)blockIdx.x * SCAN_TILE + (long long)threadIdx.x * SCAN_ITEMS;
+  unsigned v[SCAN_ITEMS], t = 0u;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; k++) {
+    v[k] = (base + k < m) ? in[base + k] : 0u;
+    t += v[k];
+  }
+  s[threadIdx.x] = t;
+  __syncthreads();
+  for (int off = 1; off < SCAN_THREADS; off <<= 1) {   // inclusive Hillis-Steele scan of thread totals
+    const unsigned x = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0u;
+    __syncthreads();
+    s[threadIdx.x] += x;
+    __syncthreads();
+  }
+  unsigned run = s[threadIdx.x] - t;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; k++) {
+    if (base + k < m) out[base + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == SCAN_THREADS - 1) sums[blockIdx.x] = s[threadIdx.x];
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_add(unsigned* __restrict__ out, long long m, const unsigned* __restrict__ sums)
+{
+  const long long base = (long long)blockIdx.x * SCAN_TILE;
+  const unsigned a = sums[blockIdx.x];
+  for (int k = threadIdx.x; k < SCAN_TILE; k += SCAN_THREADS)
+    if (base + k < m) out[base + k] += a;
+}
+
+// One single per lane: does it open a coincidence cluster, and of what kind?
+__global__ __launch_bounds__(256) void k_pair(KParams P, const unsigned long long* __restrict__ t, const unsigned* __restrict__ perm,
+                                              long long n)
+{
+  __shared__ unsigned s_cnt[5];
+  if (threadIdx.x < 5) s_cnt[threadIdx.x] = 0u;
+  __syncthreads();
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned long long W = P.window_ps;
+  if (i < n && (i == 0 || t[i] - t[i - 1] > W) && i + 1 < n && t[i + 1] - t[i] <= W) {
+    if (i + 2 < n && t[i + 2] - t[i + 1] <= W) {
+      atomicAdd(&s_cnt[4], 1u);                       // three or more singles: multiple
+    } else {
+      mcpet_single a = P.singles[perm[i]];
+      mcpet_single b = P.singles[perm[i + 1]];
+      // equal detection times: order by (event, kind), so that the binning (earlier single
+      // first) does not depend on the order in which the singles were appended
+      if (t[i] == t[i + 1] && (b.event < a.event || (b.event == a.event && b.kind < a.kind))) {
+        const mcpet_single x = a;
+        a = b;
+        b = x;
+      }
+      int cls;
+      if (a.event != b.event) cls = 2;                                  // random
+      else if (a.kind == 2 || b.kind == 2) cls = 3;                     // spurious (prompt gamma)
+      else cls = (a.scatter == 0 && b.scatter == 0) ? 0 : 1;            // true / scattered
+      const int ibin = sino_bin(P, a.phi, a.z, b.phi, b.z);
+      if (ibin >= 0) {
+        int* sino = cls == 0 ? P.trues : cls == 1 ? P.scatter : cls == 2 ? P.randoms : P.spurious;
+        atomicAdd(&sino[ibin], 1);
+        atomicAdd(&s_cnt[cls], 1u);
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 5 && s_cnt[threadIdx.x]) atomicAdd(&P.coinc_counts[threadIdx.x], (unsigned long long)s_cnt[threadIdx.x]);
+}
+
+__global__ void k_gather_singles(const mcpet_single* __restrict__ s, const unsigned* __restrict__ perm, long long n,
+                                 mcpet_single* __restrict__ out)
+{
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = s[perm[i]];
+}
+
+__global__ void k_iota(unsigned* __restrict__ vals, long long n)
+{
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) vals[i] = (unsigned)i;
+}
+
+inline int nblocks(long long n, int per) { return (int)((n + per - 1) / per); }
+
+}  // namespace
+
+long long mcpet_rs_tiles(long long n) { return (n + RS_TILE - 1) / RS_TILE; }
+
+// Scratch sizes (in unsigned words) of the digit-offset scan for up to n keys: the table itself
+// and the per-level tile sums.
+void mcpet_scan_levels(long long m, std::vector<long long>& levels)
+{
+  levels.clear();
+  while (true) {
+    const long long t = (m + SCAN_TILE - 1) / SCAN_TILE;
+    levels.push_back(t);
+    if (t <= 1) break;
+    m = t;
+  }
+}
+
+static hipError_t scan_exclusive(unsigned* data, long long m, unsigned* const* level_bufs, int level, hipStream_t st)
+{
+  const long long tiles = (m + SCAN_TILE - 1) / SCAN_TILE;
+  hipLaunchKernelGGL(k_scan_tile, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, st, data, data, m, level_bufs[level]);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || tiles <= 1) return e;
+  e = scan_exclusive(level_bufs[level], tiles, level_bufs, level + 1, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, st, data, m, level_bufs[level]);
+  return hipGetLastError();
+}
+
+hipError_t mcpet_launch_keys_from_singles(const mcpet_single* singles, long long n, unsigned long long* keys, unsigned* vals,
+                                         hipStream_t st)
+{
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_keys_from_singles, dim3(nblocks(n, 256)), dim3(256), 0, st, singles, n, keys, vals);
+  return hipGetLastError();
+}
+
+hipError_t mcpet_launch_iota(unsigned* vals, long long n, hipStream_t st)
+{
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_iota, dim3(nblocks(n, 256)), dim3(256), 0, st, vals, n);
+  return hipGetLastError();
+}
+
+// Radix sort of keys[0] (payload vals[0]) by the low 8*passes bits; the result is in
+// keys[*which], vals[*which].
+hipError_t mcpet_radix_sort(long long n, int passes, unsigned long long* keys[2], unsigned* vals[2], unsigned* hist,
+                            unsigned* const* level_bufs, int* which, hipStream_t st)
+{
+  *which = 0;
+  if (n <= 0) return hipSuccess;
+  hipError_t e = hipSuccess;
+  const int ntiles = (int)mcpet_rs_tiles(n);
+  int cur = 0;
+  for (int p = 0; p < passes && e == hipSuccess; p++) {
+    const int shift = 8 * p;
+    hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_THREADS), 0, st, keys[cur], n, shift, hist, ntiles);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = scan_exclusive(hist, 256ll * ntiles, level_bufs, 0, st);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(RS_THREADS), 0, st, keys[cur], vals[cur], keys[cur ^ 1], vals[cur ^ 1], n,
+                         shift, hist, ntiles);
+      e = hipGetLastError();
+    }
+    cur ^= 1;
+  }
+  *which = cur;
+  return e;
+}
+
+hipError_t mcpet_launch_pairing(const KParams& P, const unsigned long long* keys, const unsigned* perm, long long n, hipStream_t st)
+{
+  if (n < 2) return hipSuccess;
+  hipLaunchKernelGGL(k_pair, dim3(nblocks(n, 256)), dim3(256), 0, st, P, keys, perm, n);
+  return hipGetLastError();
+}
+
+hipError_t mcpet_launch_gather_singles(const mcpet_single* s, const unsigned* perm, long long n, mcpet_single* out, hipStream_t st)
+{
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_singles, dim3(nblocks(n, 256)), dim3(256), 0, st, s, perm, n, out);
+  return hipGetLastError();
+}

@@ -1,0 +1,72 @@
+// devutil.h — device helpers shared by the transport and coincidence kernels.
+#ifndef MCPET_DEVUTIL_H
+#define MCPET_DEVUTIL_H
+
+#include <hip/hip_runtime.h>
+#include <climits>
+
+#define DU_PI 3.14159265358979323846
+
+// __float2int_rd with NaN -> INT_MIN and saturation (matches the host checker).
+static __device__ __forceinline__ int f2i_rd(float x)
+{
+  float f = floorf(x);
+  if (!(f >= -2147483648.0f)) return INT_MIN;
+  if (f >= 2147483648.0f) return INT_MAX;
+  return (int)f;
+}
+static __device__ __forceinline__ unsigned long long d2ull_rn(double x)
+{
+  double r = rint(x);
+  if (!(r > 0.0)) return 0ull;
+  if (r >= 18446744073709551616.0) return ~0ull;
+  return (unsigned long long)r;
+}
+static __device__ __forceinline__ unsigned long long f2ull_rn(float x)
+{
+  float r = rintf(x);
+  if (!(r > 0.0f)) return 0ull;
+  if (r >= 18446744073709551616.0f) return ~0ull;
+  return (unsigned long long)r;
+}
+
+// Michelogram bin of a pair of detector hits (azimuth phi, axial z relative to the detector
+// centre), for any parameter block G with the sinogram fields (ncrystals, nrows, fovz, mrd,
+// span, nzs, nangles, nrad, nsinos): the reference's binning (K:1749-1788); -1 if rejected.
+// ith >= nangles is rejected as well (only possible when NANGLES < NCRYSTALS/2, where the
+// reference writes out of bounds).
+template <class G>
+static __device__ int sino_bin(const G& P, float phi, float zn, float phi1, float zfn)
+{
+  int ix1 = f2i_rd((float)(((double)phi / (2.0 * DU_PI) + 0.5) * (double)P.ncrystals));
+  int ix2 = f2i_rd((float)(((double)phi1 / (2.0 * DU_PI) + 0.5) * (double)P.ncrystals));
+  int iz1 = f2i_rd((float)(((double)(zn / P.fovz) + 0.5) * (double)P.nrows));
+  int iz2 = f2i_rd((float)(((double)(zfn / P.fovz) + 0.5) * (double)P.nrows));
+  if (iz1 < 0 || iz2 < 0 || iz1 >= P.nrows || iz2 >= P.nrows) return -1;
+  if (abs(iz2 - iz1) > P.mrd) return -1;
+  float delta = (float)abs(iz2 - iz1);
+  int incl = f2i_rd((delta - 1 + ((P.span + 1) / 2)) / (float)P.span);
+  int iseg = 2 * incl;
+  if (iz2 < iz1 && iseg > 0) iseg = iseg - 1;
+  int ofseg = iseg * P.nzs;
+  for (int kka = 2; kka <= iseg; kka++) {
+    ofseg -= (P.span + 1);
+    if (kka > 3) ofseg -= f2i_rd((float)((kka - 2) * 0.5)) * 2 * P.span;
+  }
+  int ofz = 0;
+  if (incl > 0) ofz = ((P.span + 1) / 2) + (incl - 1) * P.span;
+  int izm = ofseg + (iz1 + iz2) - ofz;
+  if (ix2 < ix1) {
+    int t = ix2;
+    ix2 = ix1;
+    ix1 = t;
+  }
+  int ith = ((ix1 + ix2 + P.nangles + 1) % P.ncrystals) / 2;
+  int ir = abs(ix2 - ix1 - P.nangles);
+  if ((ix1 < ith) || (ix2 >= (ith + P.nangles))) ir = -ir;
+  ir = ir + P.nrad / 2;
+  if (izm < 0 || izm >= P.nsinos || ir < 0 || ir >= P.nrad || ith < 0 || ith >= P.nangles) return -1;
+  return izm * P.nangles * P.nrad + ith * P.nrad + ir;
+}
+
+#endif

@@ -12,8 +12,10 @@
 // both kernels produce the same tallies bit for bit.
 //
 // Memory layout (MI355X-first):
-//   - voxels: one 8-byte word (density, material) per voxel; activity is only read
-//     once per stream from a compacted, activity-sorted list of emitting voxels;
+//   - voxels: a u8 (or u16) index per voxel into a palette of (density, material)
+//     words when the phantom has few distinct pairs (the volume then stays in each XCD's
+//     L2), else one 8-byte word per voxel; activity is only read once per stream from a
+//     compacted, activity-sorted list of emitting voxels;
 //   - cross sections: total (a,b) pairs in an 8-byte table read in the Woodcock loop,
 //     Compton/Rayleigh (a,b) in a separate 16-byte table read only at real events;
 //   - Compton shell table (5460 B), the energy spectrum and the per-material dose
@@ -24,6 +26,7 @@
 #include <hip/hip_runtime.h>
 #include <climits>
 
+#include "devutil.h"
 #include "fmath.h"
 #include "kparams.h"
 
@@ -112,29 +115,6 @@ __device__ __forceinline__ double rnd(Rng& r, CN& C)
 {
   C.inc(KC_RNDRAWS);
   return mcr_ranecu_double(&r.x, &r.y);
-}
-
-// __float2int_rd with NaN -> INT_MIN and saturation (matches the host checker).
-__device__ __forceinline__ int f2i_rd(float x)
-{
-  float f = floorf(x);
-  if (!(f >= -2147483648.0f)) return INT_MIN;
-  if (f >= 2147483648.0f) return INT_MAX;
-  return (int)f;
-}
-__device__ __forceinline__ unsigned long long d2ull_rn(double x)
-{
-  double r = rint(x);
-  if (!(r > 0.0)) return 0ull;
-  if (r >= 18446744073709551616.0) return ~0ull;
-  return (unsigned long long)r;
-}
-__device__ __forceinline__ unsigned long long f2ull_rn(float x)
-{
-  float r = rintf(x);
-  if (!(r > 0.0f)) return 0ull;
-  if (r >= 18446744073709551616.0f) return ~0ull;
-  return (unsigned long long)r;
 }
 
 // locate_voxel (K:1097-1124). NaN positions count as outside and indices are clamped
@@ -413,7 +393,7 @@ struct Stream {
   unsigned long long remaining;  // acquisition time left (ps)
   double inv_act;                // tpb / activity, decayed per emission
   int absvox, ix, iy, iz;
-  int last;                      // 1 for the reference thread whose final seed is reported
+  unsigned long long event;      // (reference thread id << 32) | number of the current decay
 };
 
 // Cylinder intersection of both photons (K:1645-1707 / K:536-577).
@@ -424,41 +404,6 @@ __device__ __forceinline__ float cyl_dist(const KParams& P, float x0, float y0, 
   float Cc = x0 * x0 + y0 * y0 - P.radius * P.radius;
   if (A == 0.0f) return 99999999.9f;
   return (-B + sqrtf(B * B - 4.0f * A * Cc)) / (2.0f * A);
-}
-
-// Michelogram bin (K:1749-1788); -1 if rejected. ith >= nangles is rejected as well
-// (only possible when NANGLES < NCRYSTALS/2, where the reference writes out of bounds).
-__device__ int sino_bin(const KParams& P, float phi, float zn, float phi1, float zfn)
-{
-  int ix1 = f2i_rd((float)(((double)phi / (2.0 * K_PI) + 0.5) * (double)P.ncrystals));
-  int ix2 = f2i_rd((float)(((double)phi1 / (2.0 * K_PI) + 0.5) * (double)P.ncrystals));
-  int iz1 = f2i_rd((float)(((double)(zn / P.fovz) + 0.5) * (double)P.nrows));
-  int iz2 = f2i_rd((float)(((double)(zfn / P.fovz) + 0.5) * (double)P.nrows));
-  if (iz1 < 0 || iz2 < 0 || iz1 >= P.nrows || iz2 >= P.nrows) return -1;
-  if (abs(iz2 - iz1) > P.mrd) return -1;
-  float delta = (float)abs(iz2 - iz1);
-  int incl = f2i_rd((delta - 1 + ((P.span + 1) / 2)) / (float)P.span);
-  int iseg = 2 * incl;
-  if (iz2 < iz1 && iseg > 0) iseg = iseg - 1;
-  int ofseg = iseg * P.nzs;
-  for (int kka = 2; kka <= iseg; kka++) {
-    ofseg -= (P.span + 1);
-    if (kka > 3) ofseg -= f2i_rd((float)((kka - 2) * 0.5)) * 2 * P.span;
-  }
-  int ofz = 0;
-  if (incl > 0) ofz = ((P.span + 1) / 2) + (incl - 1) * P.span;
-  int izm = ofseg + (iz1 + iz2) - ofz;
-  if (ix2 < ix1) {
-    int t = ix2;
-    ix2 = ix1;
-    ix1 = t;
-  }
-  int ith = ((ix1 + ix2 + P.nangles + 1) % P.ncrystals) / 2;
-  int ir = abs(ix2 - ix1 - P.nangles);
-  if ((ix1 < ith) || (ix2 >= (ith + P.nangles))) ir = -ir;
-  ir = ir + P.nrad / 2;
-  if (izm < 0 || izm >= P.nsinos || ir < 0 || ir >= P.nrad || ith < 0 || ith >= P.nangles) return -1;
-  return izm * P.nangles * P.nrad + ith * P.nrad + ir;
 }
 
 __device__ __forceinline__ void spectrum_add(const KParams& P, int* s_spec, int iE)
@@ -587,6 +532,38 @@ K_RARE void tally_psf(const KParams& P, float energy, float energyFirst, const f
   }
 }
 
+// Singles list (no reference counterpart; the reference tallies pairs only): a photon that
+// leaves the phantom and hits the detector cylinder within its height gets the detector's
+// energy blur (the PSF path's form, K:580-587, two draws), joins the singles spectrum and,
+// inside the energy window, is appended with its detection time = emission time + flight
+// time to the cylinder.
+template <class CN>
+__device__ __forceinline__ void record_single(const KParams& P, const float* pos, const float* dir, float energy, float travel,
+                                              int ss, int kind, Stream& st, CN& C, int* s_spec)
+{
+  const float x0 = pos[0] - P.cx, y0 = pos[1] - P.cy;
+  const float dist = cyl_dist(P, x0, y0, dir);
+  const float z = (pos[2] - P.cz) + dist * dir[2];
+  if (!(fabsf(z) < 0.5f * P.height)) return;
+  const float randno1 = rn(st.r, C);
+  const float randno2 = rn(st.r, C);
+  const float g = sqrtf(-2.0f * mcf_logf(randno1 + 1e-8f)) * mcf_cosf((float)(2.0f * K_PI * (double)randno2));
+  const float e = energy + ((P.e_resol) * (1.0f / 2.35f)) * energy * g;
+  const int iE = f2i_rd(e / 1000.0f);
+  if ((iE < P.ne) && (iE >= 0)) spectrum_add(P, s_spec, iE);
+  if (!(e > P.e_low && e < P.e_high)) return;
+  const float phi = mcf_atan2f(y0 + dist * dir[1], x0 + dist * dir[0]);
+  const unsigned long long flight = d2ull_rn((double)(travel + dist) * K_INV_C);
+  const unsigned long long t = (P.acq_ps - st.remaining) + (flight < (1ull << 40) ? flight : (1ull << 40) - 1ull);
+  const unsigned long long k = atomicAdd(P.singles_n, 1ull);
+  if ((long long)k < P.singles_cap) {
+    uint4* rec = reinterpret_cast<uint4*>(P.singles + k);
+    rec[0] = make_uint4((unsigned)t, (unsigned)(t >> 32), (unsigned)st.event, (unsigned)(st.event >> 32));
+    rec[1] = make_uint4(__float_as_uint(e), __float_as_uint(z), __float_as_uint(phi),
+                        ((unsigned)ss & 0xffffu) | ((unsigned)kind << 16));
+  }
+}
+
 __device__ __forceinline__ void tally_dose(const KParams& P, float Edep, int mat, const float* pos, unsigned long long* s_mat)
 {
   if (P.material_dose) {
@@ -618,9 +595,9 @@ struct Photon {
   int index, mat_old, mat0, ss;
 };
 
-__device__ __forceinline__ void photon_start(const KParams& P, Photon& ph)
+__device__ __forceinline__ void photon_start(const KParams& P, Photon& ph, float energy = K_E511)
 {
-  ph.E = K_E511;
+  ph.E = energy;
   ph.ss = 0;
   ph.travel = 0.0f;
   ph.index = f2i_rd((ph.E - P.e0) * P.ide);
@@ -630,6 +607,18 @@ __device__ __forceinline__ void photon_start(const KParams& P, Photon& ph)
 }
 
 enum StepOutcome { STEP_ESCAPED = 0, STEP_VIRTUAL = 1, STEP_REAL = 2 };
+
+// Palette of (density, material) words for u8-indexed phantoms, filled by each
+// workgroup's prologue.
+__shared__ float2 k_palette[256];
+
+// The (density, material) word of voxel absvox in the device layout chosen at set-up.
+__device__ __forceinline__ float2 voxel_word(const KParams& P, int absvox)
+{
+  if (P.vox_fmt == 1) return k_palette[P.vox8[absvox]];
+  if (P.vox_fmt == 2) return P.palette[P.vox16[absvox]];
+  return P.vox[absvox];
+}
 
 template <class CN>
 __device__ __forceinline__ int woodcock_step(const KParams& P, Photon& ph, Rng& r, CN& C)
@@ -642,7 +631,7 @@ __device__ __forceinline__ int woodcock_step(const KParams& P, Photon& ph, Rng& 
   ph.pos[2] += step * ph.dir[2];
   const int absvox = locate(P, ph.pos);
   if (absvox < 0) return STEP_ESCAPED;
-  const float2 md = P.vox[absvox];
+  const float2 md = voxel_word(P, absvox);
   ph.mat0 = __float_as_int(md.y);
   if (ph.mat0 != ph.mat_old) {
     ph.tab = P.tot[ph.index * K_MAXMAT + ph.mat0];
@@ -766,17 +755,24 @@ __device__ __forceinline__ void source_first(const KParams& P, Stream& st, CN& C
   pos[2] = ((float)st.iz + r3) / P.ivz;
 }
 
+// Isotropic direction (K:711-720, the BG-mode second photon).
+template <class CN>
+__device__ __forceinline__ void isotropic_dir(Stream& st, CN& C, float* dir)
+{
+  dir[2] = 1.0f - 2.0f * rn(st.r, C);
+  float phi_sampled = 6.28318530716f * rn(st.r, C);
+  float sin_theta = sqrtf(1.0f - dir[2] * dir[2]);
+  float sp, cp;
+  mcf_sincosf(phi_sampled, &sp, &cp);
+  dir[1] = sin_theta * sp;
+  dir[0] = sin_theta * cp;
+}
+
 template <class CN>
 __device__ __forceinline__ void source_second(const KParams& P, Stream& st, CN& C, const float* dir1, float* dir)
 {
   if (P.bg_mode) {
-    dir[2] = 1.0f - 2.0f * rn(st.r, C);
-    float phi_sampled = 6.28318530716f * rn(st.r, C);
-    float sin_theta = sqrtf(1.0f - dir[2] * dir[2]);
-    float sp, cp;
-    mcf_sincosf(phi_sampled, &sp, &cp);
-    dir[1] = sin_theta * sp;
-    dir[0] = sin_theta * cp;
+    isotropic_dir(st, C, dir);
     return;
   }
   float phi_sampled2 = (float)((double)mcf_atan2f(dir1[1], dir1[0]) + K_PI);
@@ -808,6 +804,7 @@ __device__ __forceinline__ bool decay_time(const KParams& P, Stream& st, CN& C)
   const unsigned long long Dt_ps = d2ull_rn(Dt * 1e12);
   if (!(Dt_ps < st.remaining)) return false;
   st.remaining -= Dt_ps;
+  st.event++;
   C.inc(KC_DECAYS);              // 2 photon histories per decay (K:282)
   st.inv_act = st.inv_act * mcf_exp(Dt * P.inv_mean_life);
   return true;
@@ -853,6 +850,7 @@ __device__ bool decay(const KParams& P, Stream& st, CN& C, const mcpet_compton_t
                       unsigned long long* s_mat)
 {
   if (!decay_time(P, st, C)) return false;
+  C.inc(KC_EMIT_PAIR);
   Pair pr;
   Photon ph;
   source_first(P, st, C, pr.p1, pr.d1);
@@ -883,14 +881,14 @@ __device__ __forceinline__ void stream_init(const KParams& P, long long s, Strea
   st.iz = st.absvox / (P.nx * P.ny);
   const long long tid = (long long)st.absvox * P.tpb + t;
   mcr_init_stream(tid, P.seed, P.j1, P.j2, &st.r.x, &st.r.y);
-  st.last = (tid == P.last_thread_id);
+  st.event = (unsigned long long)tid << 32;
   st.remaining = P.acq_ps;
   st.inv_act = ((double)P.tpb / (double)act);
 }
 
 __device__ __forceinline__ void stream_finish(const KParams& P, const Stream& st)
 {
-  if (st.last) *P.final_seed = st.r.x;
+  if ((long long)(st.event >> 32) == P.last_thread_id) *P.final_seed = st.r.x;
 }
 
 // Workgroup prologue/epilogue shared by the kernels: Compton table, spectrum, material
@@ -907,6 +905,8 @@ __device__ __forceinline__ void stream_finish(const KParams& P, const Stream& st
       dst[i] = src[i];                                                                           \
     if (threadIdx.x < 2 * K_MAXMAT) s_mat[threadIdx.x] = 0ull;                                   \
     if (threadIdx.x < KC_N) s_cnt[threadIdx.x] = 0u;                                             \
+    if (P.vox_fmt == 1)                                                                          \
+      for (int i = threadIdx.x; i < P.n_palette; i += blockDim.x) k_palette[i] = P.palette[i];   \
     if (P.lds_spectrum)                                                                          \
       for (int i = threadIdx.x; i < P.ne; i += blockDim.x) s_spec[i] = 0;                        \
   }                                                                                              \
@@ -948,14 +948,16 @@ __global__ __launch_bounds__(256) K_OCCUPANCY void k_transport_plain(KParams P)
 //   TALLY    : detector, energy blur, sinogram / PSF tallies of the pair
 //   CTAU     : one trial of GCOa's tau loop
 //   CPZ      : one trial of GCOa's p_z loop (+ completion of the Compton event)
+//   DETECT   : singles list: detector hit, energy blur and record of the photon just ended
+//   SOURCE3  : emit the decay's prompt gamma (isotropic)
 // and each iteration the wave executes the one phase held by the most lanes (ballot
 // counts). Lanes in other phases wait, so expensive phases (Compton rejection trials,
 // detector geometry) run with many lanes at once instead of one or two, and a lane that
 // needs more rejection trials than its neighbours does not hold them back. A lane's own
 // sequence of random numbers and operations is unchanged.
 enum LaneState {
-  LS_DECAY = 0, LS_SOURCE2 = 1, LS_STEP = 2, LS_INTERACT = 3, LS_TALLY = 4, LS_CTAU = 5, LS_CPZ = 6,
-  LS_N = 7, LS_DONE = 8
+  LS_DECAY = 0, LS_SOURCE2 = 1, LS_STEP = 2, LS_INTERACT = 3, LS_TALLY = 4, LS_CTAU = 5, LS_CPZ = 6, LS_DETECT = 7,
+  LS_SOURCE3 = 8, LS_N = 9, LS_DONE = 10
 };
 
 // Per-lane state that only some phases touch lives in LDS slots instead of registers
@@ -1047,15 +1049,19 @@ __device__ __forceinline__ void cs_load(const LaneSlots& L, ComptonState& cs)
   cs.trials = __float_as_int(L[CS_TRIALS]);
 }
 
-// Next state once the current photon has escaped or was absorbed (K:418-442).
-__device__ __forceinline__ int after_photon(const KParams& P, int photon_no, const Photon& ph, const LaneSlots& L)
+// Next state once the current photon has escaped or was absorbed (K:418-442). Photons are
+// numbered 1, 2 (annihilation pair) and 3 (prompt gamma); the singles pipeline records every
+// photon in DETECT instead of pairing them here.
+__device__ __forceinline__ int after_photon(const KParams& P, int photon_no, bool prompt, const Photon& ph, const LaneSlots& L)
 {
+  if (P.mode == MCPET_TALLY_SINGLES) return LS_DETECT;
   if (photon_no == 1) {
     pair_store_first(L, ph);
     return LS_SOURCE2;
   }
-  if (ph.index > -1 && L[PS_EF] > -0.5f && pair_selected(P, ph.ss, __float_as_int(L[PS_SSF]))) return LS_TALLY;
-  return LS_DECAY;
+  if (photon_no == 2 && ph.index > -1 && L[PS_EF] > -0.5f && pair_selected(P, ph.ss, __float_as_int(L[PS_SSF])))
+    return LS_TALLY;
+  return (photon_no == 2 && prompt) ? LS_SOURCE3 : LS_DECAY;
 }
 
 template <class CN>
@@ -1067,6 +1073,7 @@ __global__ __launch_bounds__(K_BLOCK) K_OCCUPANCY void k_transport_sm(KParams P)
   Stream st;
   Photon ph;
   bool have = false;
+  bool prompt = false;           // the current decay still has its prompt gamma to emit
   int photon_no = 1;
   int state = LS_DECAY;
   for (;;) {
@@ -1108,7 +1115,7 @@ __global__ __launch_bounds__(K_BLOCK) K_OCCUPANCY void k_transport_sm(KParams P)
     if (state != best) {
     } else if (best == LS_STEP) {
       const int o = woodcock_step(P, ph, st.r, C);
-      if (o == STEP_ESCAPED) state = after_photon(P, photon_no, ph, L);
+      if (o == STEP_ESCAPED) state = after_photon(P, photon_no, prompt, ph, L);
       else if (o == STEP_REAL) state = LS_INTERACT;
     } else if (best == LS_INTERACT) {
       ComptonState cs;
@@ -1117,7 +1124,7 @@ __global__ __launch_bounds__(K_BLOCK) K_OCCUPANCY void k_transport_sm(KParams P)
         cs_store(L, cs);
         state = LS_CTAU;
       } else if (o == IA_ABSORBED || locate(P, ph.pos) < 0) {
-        state = after_photon(P, photon_no, ph, L);
+        state = after_photon(P, photon_no, prompt, ph, L);
       } else {
         state = LS_STEP;
       }
@@ -1133,7 +1140,7 @@ __global__ __launch_bounds__(K_BLOCK) K_OCCUPANCY void k_transport_sm(KParams P)
       ComptonState cs;
       cs_load(L, cs);
       if (gcoa_pz_trial(ph.E, ph.mat0, st.r, C, s_cmp, cs)) {
-        if (!compton_end(P, ph, st.r, C, cs, s_mat) || locate(P, ph.pos) < 0) state = after_photon(P, photon_no, ph, L);
+        if (!compton_end(P, ph, st.r, C, cs, s_mat) || locate(P, ph.pos) < 0) state = after_photon(P, photon_no, prompt, ph, L);
         else state = LS_STEP;
       } else {
         L[CS_TRIALS] = __int_as_float(cs.trials);
@@ -1155,17 +1162,28 @@ __global__ __launch_bounds__(K_BLOCK) K_OCCUPANCY void k_transport_sm(KParams P)
         have = false;
       }
       if (state != LS_DONE) {
+        // decay branches (no draws for a pure positron emitter without prompt gamma)
+        bool pair = true;
+        if (P.positron_frac < 1.0f) pair = rn(st.r, C) < P.positron_frac;
+        prompt = false;
+        if (P.prompt_frac > 0.0f) prompt = rn(st.r, C) < P.prompt_frac;
         Pair pr;
         source_first(P, st, C, pr.p1, pr.d1);
         pr.EF = -10.0f;
         pair_store_source(L, pr);
-        for (int k = 0; k < 3; k++) {
-          ph.pos[k] = pr.p1[k];
-          ph.dir[k] = pr.d1[k];
+        if (prompt) C.inc(KC_EMIT_PROMPT);
+        if (pair) {
+          C.inc(KC_EMIT_PAIR);
+          for (int k = 0; k < 3; k++) {
+            ph.pos[k] = pr.p1[k];
+            ph.dir[k] = pr.d1[k];
+          }
+          photon_start(P, ph);
+          photon_no = 1;
+          state = (locate(P, ph.pos) < 0) ? after_photon(P, 1, prompt, ph, L) : LS_STEP;
+        } else {
+          state = prompt ? LS_SOURCE3 : LS_DECAY;
         }
-        photon_start(P, ph);
-        photon_no = 1;
-        state = (locate(P, ph.pos) < 0) ? after_photon(P, 1, ph, L) : LS_STEP;
       }
     } else if (best == LS_SOURCE2) {
       float p1[3], d1[3];
@@ -1177,12 +1195,22 @@ __global__ __launch_bounds__(K_BLOCK) K_OCCUPANCY void k_transport_sm(KParams P)
       for (int k = 0; k < 3; k++) ph.pos[k] = p1[k];
       photon_start(P, ph);
       photon_no = 2;
-      state = (locate(P, ph.pos) < 0) ? after_photon(P, 2, ph, L) : LS_STEP;
+      state = (locate(P, ph.pos) < 0) ? after_photon(P, 2, prompt, ph, L) : LS_STEP;
+    } else if (best == LS_DETECT) {
+      if (ph.index > -1) record_single(P, ph.pos, ph.dir, ph.E, ph.travel, ph.ss, photon_no - 1, st, C, s_spec);
+      state = (photon_no == 1) ? LS_SOURCE2 : (photon_no == 2 && prompt) ? LS_SOURCE3 : LS_DECAY;
+    } else if (best == LS_SOURCE3) {
+      for (int k = 0; k < 3; k++) ph.pos[k] = L[PS_P1 + k];
+      isotropic_dir(st, C, ph.dir);
+      photon_start(P, ph, P.prompt_energy);
+      photon_no = 3;
+      prompt = false;
+      state = (locate(P, ph.pos) < 0) ? after_photon(P, 3, false, ph, L) : LS_STEP;
     } else {  // LS_TALLY
       Pair pr;
       pair_load(L, pr);
       tally_pair(P, ph, pr, st, C, s_spec);
-      state = LS_DECAY;
+      state = prompt ? LS_SOURCE3 : LS_DECAY;
     }
 #if MCPET_SM_STATS
     if (__lane_id() == 0) C.add(KC_PHASE_CYCLES + best, (unsigned)(clock64() - t_phase));

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "mcpet.h"
 #include "ranecu.h"
 
@@ -21,15 +23,23 @@ enum KCounter {
   KC_PAIRS,
   KC_COINC,
   KC_RNDRAWS,
+  KC_EMIT_PAIR,         // annihilation pairs emitted
+  KC_EMIT_PROMPT,       // prompt gammas emitted
   KC_PHASE_CYCLES = 16, // per-phase shader clock cycles, summed over waves (MCPET_SM_STATS builds)
-  KC_PHASE_EXEC = 24,   // per-phase executions of the persistent kernel (MCPET_SM_STATS builds)
-  KC_PHASE_LANES = 32,  // per-phase sum of active lanes (MCPET_SM_STATS builds)
-  KC_N = 40
+  KC_PHASE_EXEC = 16 + MCPET_NPHASES,       // per-phase executions (MCPET_SM_STATS builds)
+  KC_PHASE_LANES = 16 + 2 * MCPET_NPHASES,  // per-phase sum of active lanes (MCPET_SM_STATS builds)
+  KC_N = 16 + 3 * MCPET_NPHASES
 };
 
 struct KParams {
-  // phantom: packed voxel words (density, material index 0..14 as int bits), x fastest
+  // phantom, x fastest. vox_fmt 0: one packed word (density, material index 0..14 as int
+  // bits) per voxel; 1: u8 index into a palette of such words (palette copied to LDS);
+  // 2: u16 index into the palette (read from global memory).
   const float2* vox;
+  const uint8_t* vox8;
+  const uint16_t* vox16;
+  const float2* palette;
+  int vox_fmt, n_palette;
   int nx, ny, nz;
   float ivx, ivy, ivz;         // inverse voxel size
   float bx, by, bz;            // bounding box size
@@ -78,10 +88,22 @@ struct KParams {
   int* final_seed;
   long long last_thread_id;      // reference thread id whose final seed is stored (K:454-455)
   int lds_spectrum;              // 1: privatise the spectrum in LDS
+  // Decay branching (drawn per decay only when active) and the singles list.
+  float positron_frac;           // < 1: a decay emits the annihilation pair with this probability
+  float prompt_frac;             // > 0: a decay emits a prompt gamma with this probability
+  float prompt_energy;           // eV
+  mcpet_single* singles;
+  unsigned long long* singles_n; // singles detected (may exceed the capacity)
+  long long singles_cap;
+  // coincidence pairing of the sorted singles
+  unsigned long long window_ps;
+  int* randoms;
+  int* spurious;
+  unsigned long long* coinc_counts;  // binned trues, scatter, randoms, spurious; multiples
   // Phase selection of the persistent kernel: 0 = phase held by most lanes;
   // 1 = phase with the fewest wasted lane-slots, (64 - lanes) * phase_cost.
   int sm_policy;
-  int sm_cost[8];                // relative cost per phase (DECAY, SOURCE2, STEP, INTERACT, TALLY, CTAU, CPZ)
+  int sm_cost[MCPET_NPHASES];    // relative cost per phase (DECAY, SOURCE2, STEP, INTERACT, TALLY, CTAU, CPZ, DETECT, SOURCE3)
 };
 
 // Launchers implemented in kernels.hip.
@@ -91,5 +113,16 @@ hipError_t mcpet_launch_selftest_ranecu(int seed, const long long* d_ids, int ns
 hipError_t mcpet_launch_selftest_fmath(const double* d_x, const double* d_y, int n, double* d_od, float* d_of, hipStream_t stream);
 hipError_t mcpet_launch_copy(const uint4* src, uint4* dst, size_t n16, hipStream_t stream);
 int mcpet_transport_regs_hint();
+
+// Singles pipeline (coinc.hip).
+long long mcpet_rs_tiles(long long n);
+void mcpet_scan_levels(long long m, std::vector<long long>& levels);
+hipError_t mcpet_launch_keys_from_singles(const mcpet_single* singles, long long n, unsigned long long* keys, unsigned* vals,
+                                         hipStream_t st);
+hipError_t mcpet_launch_iota(unsigned* vals, long long n, hipStream_t st);
+hipError_t mcpet_radix_sort(long long n, int passes, unsigned long long* keys[2], unsigned* vals[2], unsigned* hist,
+                            unsigned* const* level_bufs, int* which, hipStream_t st);
+hipError_t mcpet_launch_pairing(const KParams& P, const unsigned long long* keys, const unsigned* perm, long long n, hipStream_t st);
+hipError_t mcpet_launch_gather_singles(const mcpet_single* s, const unsigned* perm, long long n, mcpet_single* out, hipStream_t st);
 
 #endif

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <numeric>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "host_internal.h"
@@ -32,6 +33,9 @@ struct mcpet_ctx {
   KParams P{};
   // device allocations
   float2* d_vox = nullptr;
+  uint8_t* d_vox8 = nullptr;
+  uint16_t* d_vox16 = nullptr;
+  float2* d_palette = nullptr;
   int* d_active_vox = nullptr;
   float* d_active_act = nullptr;
   float2* d_woodcock = nullptr;
@@ -50,6 +54,21 @@ struct mcpet_ctx {
   unsigned long long* d_mat_edep = nullptr;
   unsigned long long* d_counters = nullptr;  // KC_N counters + stream queue
   int* d_final_seed = nullptr;
+  // singles pipeline
+  bool singles_mode = false;
+  mcpet_single* d_singles = nullptr;
+  unsigned long long* d_singles_n = nullptr;
+  unsigned long long* d_keys[2] = {nullptr, nullptr};
+  unsigned* d_vals[2] = {nullptr, nullptr};
+  unsigned* d_hist = nullptr;
+  std::vector<unsigned*> d_scan;
+  int* d_randoms = nullptr;
+  int* d_spurious = nullptr;
+  unsigned long long* d_coinc = nullptr;
+  hipEvent_t ev_c1 = nullptr;
+  unsigned long long singles_req = 0;
+  long long n_singles = 0;
+  int sorted = 0;
   bool ran = false;
   bool timed = false;
   double zero_ms = 0.0;
@@ -99,11 +118,16 @@ mcpet_status dalloc(mcpet_ctx* c, T** p, size_t n)
 
 void free_all(mcpet_ctx* c)
 {
-  void* ptrs[] = {c->d_vox, c->d_active_vox, c->d_active_act, c->d_woodcock, c->d_tot, c->d_cr, c->d_ray, c->d_cmp,
+  void* ptrs[] = {c->d_vox, c->d_vox8, c->d_vox16, c->d_palette, c->d_active_vox, c->d_active_act, c->d_woodcock, c->d_tot, c->d_cr, c->d_ray, c->d_cmp,
                   c->d_trues, c->d_scatter, c->d_img_t, c->d_img_sc, c->d_spectrum, c->d_psf, c->d_psf_index,
-                  c->d_voxel_edep, c->d_mat_edep, c->d_counters, c->d_final_seed};
+                  c->d_voxel_edep, c->d_mat_edep, c->d_counters, c->d_final_seed, c->d_singles, c->d_singles_n,
+                  c->d_keys[0], c->d_keys[1], c->d_vals[0], c->d_vals[1], c->d_hist, c->d_randoms, c->d_spurious,
+                  c->d_coinc};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  for (unsigned* p : c->d_scan)
+    if (p) (void)hipFree(p);
+  if (c->ev_c1) (void)hipEventDestroy(c->ev_c1);
   if (c->ev_k0) (void)hipEventDestroy(c->ev_k0);
   if (c->ev_k1) (void)hipEventDestroy(c->ev_k1);
   if (c->ev_z0) (void)hipEventDestroy(c->ev_z0);
@@ -112,7 +136,17 @@ void free_all(mcpet_ctx* c)
 
 }  // namespace
 
-extern "C" const char* mcpet_version(void) { return "mcpet-mi355x 0.1 (gfx950)"; }
+extern "C" const char* mcpet_version(void) { return "mcpet-mi355x 0.2 (gfx950)"; }
+
+extern "C" int32_t mcpet_struct_sizes(size_t* out, int32_t n)
+{
+  const size_t sz[] = {sizeof(mcpet_config),      sizeof(mcpet_phantom), sizeof(mcpet_tables),
+                       sizeof(mcpet_tallies),     sizeof(mcpet_run_params), sizeof(mcpet_stats),
+                       sizeof(mcpet_psf_record),  sizeof(mcpet_single)};
+  int32_t k = 0;
+  for (; out && k < n && k < (int32_t)(sizeof(sz) / sizeof(sz[0])); k++) out[k] = sz[k];
+  return k;
+}
 
 extern "C" const char* mcpet_last_error(const mcpet_ctx* ctx)
 {
@@ -142,6 +176,25 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
     float fi = floorf((510998.95f - tab->e0) * tab->ide);
     if (!(fi >= 0.0f && fi <= (float)(tab->num_values - 2)))
       return set_err(nullptr, MCPET_E_RANGE, "mcpet_create: 511 keV outside the interaction tables");
+  }
+  const int tmode = cfg->detector.tally_psf_sinogram;
+  if (tmode < 0 || tmode > MCPET_TALLY_SINGLES)
+    return set_err(nullptr, MCPET_E_RANGE, "mcpet_create: tally_psf_sinogram must be 0, 1, 2 or 3 (singles)");
+  if (!(cfg->prompt_gamma_fraction >= 0.0f && cfg->prompt_gamma_fraction <= 1.0f))
+    return set_err(nullptr, MCPET_E_RANGE, "mcpet_create: prompt_gamma_fraction must lie in [0, 1]");
+  if (cfg->prompt_gamma_fraction > 0.0f) {
+    // the prompt gamma's energy bin (and the next one, read by Rayleigh sampling) must exist
+    const float fi = floorf((cfg->prompt_gamma_energy - tab->e0) * tab->ide);
+    if (!(cfg->prompt_gamma_energy > 0.0f && fi >= 0.0f && fi <= (float)(tab->num_values - 2)))
+      return set_err(nullptr, MCPET_E_RANGE, "mcpet_create: the prompt gamma energy lies outside the interaction tables");
+  }
+  if (tmode == MCPET_TALLY_SINGLES) {
+    if (cfg->singles_capacity < 1 || cfg->singles_capacity > 0x7fffffffLL)
+      return set_err(nullptr, MCPET_E_RANGE, "mcpet_create: singles_capacity must lie in [1, 2^31-1]");
+    if (cfg->coincidence_window_ps == 0)
+      return set_err(nullptr, MCPET_E_RANGE, "mcpet_create: coincidence_window_ps must be > 0");
+    if ((long long)ph->nx * ph->ny * ph->nz * (long long)cfg->threads_per_block >= (1LL << 32))
+      return set_err(nullptr, MCPET_E_RANGE, "mcpet_create: voxels x threads_per_block must be < 2^32 for singles event ids");
   }
   mcpet_ctx* c = new mcpet_ctx();
   c->cfg = *cfg;
@@ -204,6 +257,32 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
     hv[(size_t)v] = make_float2(p[1], bits);
     if (p[2] > 1.0e-7f) act_idx.push_back((int)v);   // K:232-237
   }
+  // Palette of the distinct (density, material) words: a u8 or u16 index per voxel keeps
+  // the volume small enough to stay in the L2 of each XCD (the Woodcock gather is random).
+  std::vector<float2> palette;
+  std::vector<uint16_t> pidx;
+  if (cfg->voxel_format == 0) {
+    std::unordered_map<uint64_t, int> seen;
+    pidx.resize((size_t)c->nvox);
+    for (long long v = 0; v < c->nvox && palette.size() <= 65536; v++) {
+      uint32_t d, m;
+      std::memcpy(&d, &hv[(size_t)v].x, 4);
+      std::memcpy(&m, &hv[(size_t)v].y, 4);
+      const uint64_t key = ((uint64_t)m << 32) | d;
+      auto it = seen.find(key);
+      if (it == seen.end()) {
+        it = seen.emplace(key, (int)palette.size()).first;
+        palette.push_back(hv[(size_t)v]);
+      }
+      pidx[(size_t)v] = (uint16_t)it->second;
+    }
+    if (palette.size() > 65536) {
+      palette.clear();
+      pidx.clear();
+    }
+  }
+  c->P.vox_fmt = palette.empty() ? 0 : (palette.size() <= 256 ? 1 : 2);
+  c->P.n_palette = (int)palette.size();
   std::stable_sort(act_idx.begin(), act_idx.end(), [&](int a, int b) {
     return ph->voxel_mat_dens[3 * (size_t)a + 2] > ph->voxel_mat_dens[3 * (size_t)b + 2];
   });
@@ -240,7 +319,10 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
   const int mode = cfg->detector.tally_psf_sinogram;
   const bool want_psf = (mode == 0 || mode == 1) && cfg->detector.psf_size > 0;
 
-  CK(dalloc(c, &c->d_vox, (size_t)c->nvox));
+  if (c->P.vox_fmt == 0) CK(dalloc(c, &c->d_vox, (size_t)c->nvox));
+  if (c->P.vox_fmt == 1) CK(dalloc(c, &c->d_vox8, (size_t)c->nvox));
+  if (c->P.vox_fmt == 2) CK(dalloc(c, &c->d_vox16, (size_t)c->nvox));
+  if (c->P.vox_fmt != 0) CK(dalloc(c, &c->d_palette, palette.size()));
   CK(dalloc(c, &c->d_active_vox, std::max<size_t>(1, act_idx.size())));
   CK(dalloc(c, &c->d_active_act, std::max<size_t>(1, act_idx.size())));
   CK(dalloc(c, &c->d_woodcock, (size_t)N));
@@ -258,9 +340,36 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
   if (voxel_dose) CK(dalloc(c, &c->d_voxel_edep, 2 * (size_t)c->n_roi));
   CK(dalloc(c, &c->d_mat_edep, 2 * MCPET_MAX_MATERIALS));
   CK(dalloc(c, &c->d_counters, KC_N + 1));
+  c->singles_mode = (mode == MCPET_TALLY_SINGLES);
+  if (c->singles_mode) {
+    const long long cap = cfg->singles_capacity;
+    CK(dalloc(c, &c->d_singles, (size_t)cap));
+    CK(dalloc(c, &c->d_singles_n, 1));
+    for (int k = 0; k < 2; k++) {
+      CK(dalloc(c, &c->d_keys[k], (size_t)cap));
+      CK(dalloc(c, &c->d_vals[k], (size_t)cap));
+    }
+    const long long m = 256LL * mcpet_rs_tiles(cap);
+    CK(dalloc(c, &c->d_hist, (size_t)m));
+    std::vector<long long> levels;
+    mcpet_scan_levels(m, levels);
+    c->d_scan.assign(levels.size(), nullptr);
+    for (size_t k = 0; k < levels.size(); k++) CK(dalloc(c, &c->d_scan[k], (size_t)levels[k]));
+    CK(dalloc(c, &c->d_randoms, (size_t)cfg->nbins));
+    CK(dalloc(c, &c->d_spurious, (size_t)cfg->nbins));
+    CK(dalloc(c, &c->d_coinc, 5));
+    HK(hipEventCreate(&c->ev_c1));
+  }
   CK(dalloc(c, &c->d_final_seed, 1));
 
-  HK(hipMemcpy(c->d_vox, hv.data(), hv.size() * sizeof(float2), hipMemcpyHostToDevice));
+  if (c->P.vox_fmt == 0) HK(hipMemcpy(c->d_vox, hv.data(), hv.size() * sizeof(float2), hipMemcpyHostToDevice));
+  if (c->P.vox_fmt == 1) {
+    std::vector<uint8_t> p8(pidx.begin(), pidx.end());
+    HK(hipMemcpy(c->d_vox8, p8.data(), p8.size(), hipMemcpyHostToDevice));
+  }
+  if (c->P.vox_fmt == 2) HK(hipMemcpy(c->d_vox16, pidx.data(), pidx.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+  if (c->P.vox_fmt != 0)
+    HK(hipMemcpy(c->d_palette, palette.data(), palette.size() * sizeof(float2), hipMemcpyHostToDevice));
   if (!act_idx.empty()) {
     HK(hipMemcpy(c->d_active_vox, act_idx.data(), act_idx.size() * sizeof(int), hipMemcpyHostToDevice));
     HK(hipMemcpy(c->d_active_act, act_val.data(), act_val.size() * sizeof(float), hipMemcpyHostToDevice));
@@ -272,7 +381,20 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
   HK(hipMemcpy(c->d_cmp, tab->compton, sizeof(mcpet_compton_table), hipMemcpyHostToDevice));
 
   KParams& P = c->P;
+  P.positron_frac = (cfg->positron_fraction > 0.0f && cfg->positron_fraction < 1.0f) ? cfg->positron_fraction : 1.0f;
+  P.prompt_frac = cfg->prompt_gamma_fraction;
+  P.prompt_energy = cfg->prompt_gamma_energy;
+  P.singles = c->d_singles;
+  P.singles_n = c->d_singles_n;
+  P.singles_cap = c->singles_mode ? cfg->singles_capacity : 0;
+  P.window_ps = cfg->coincidence_window_ps;
+  P.randoms = c->d_randoms;
+  P.spurious = c->d_spurious;
+  P.coinc_counts = c->d_coinc;
   P.vox = c->d_vox;
+  P.vox8 = c->d_vox8;
+  P.vox16 = c->d_vox16;
+  P.palette = c->d_palette;
   P.nx = ph->nx;
   P.ny = ph->ny;
   P.nz = ph->nz;
@@ -361,6 +483,10 @@ static mcpet_status zero_tallies(mcpet_ctx* c)
   if (c->d_voxel_edep) HIPCHK(c, hipMemsetAsync(c->d_voxel_edep, 0, 16 * (size_t)c->n_roi, c->stream));
   HIPCHK(c, hipMemsetAsync(c->d_mat_edep, 0, 16 * MCPET_MAX_MATERIALS, c->stream));
   HIPCHK(c, hipMemsetAsync(c->d_final_seed, 0, sizeof(int), c->stream));
+  if (c->singles_mode) {
+    HIPCHK(c, hipMemsetAsync(c->d_randoms, 0, sizeof(int) * (size_t)c->cfg.nbins, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_spurious, 0, sizeof(int) * (size_t)c->cfg.nbins, c->stream));
+  }
   (void)P;
   return MCPET_OK;
 }
@@ -379,9 +505,12 @@ extern "C" mcpet_status mcpet_run(mcpet_ctx* c, const mcpet_run_params* rp)
   // Phase selection (persistent kernel): most lanes by default (measured best on the head
   // workload); policy 2 weighs the phases by relative cost (DECAY, SOURCE2, STEP, INTERACT, TALLY).
   P.sm_policy = (p.phase_policy == 2) ? 1 : 0;
-  static const int default_cost[MCPET_NPHASES] = {4, 6, 1, 3, 6, 3, 2, 1};
+  static const int default_cost[MCPET_NPHASES] = {4, 6, 1, 3, 6, 3, 2, 4, 4, 1};
   for (int i = 0; i < MCPET_NPHASES; i++) P.sm_cost[i] = p.phase_cost[i] > 0 ? p.phase_cost[i] : default_cost[i];
   int kind = p.kernel == MCPET_KERNEL_AUTO ? MCPET_KERNEL_PERSISTENT : p.kernel;
+  if (kind == MCPET_KERNEL_PLAIN && (c->singles_mode || P.positron_frac < 1.0f || P.prompt_frac > 0.0f))
+    return set_err(c, MCPET_E_ARG, "mcpet_run: the plain kernel supports the reference's pair tallies only (no singles, "
+                                   "positron branching or prompt gammas)");
   int bs = p.block_size > 0 ? p.block_size : 256;
   if (bs % 64 != 0 || bs > 256) return set_err(c, MCPET_E_RANGE, "mcpet_run: block_size must be a multiple of 64, <= 256");
   HIPCHK(c, hipEventRecord(c->ev_z0, c->stream));
@@ -390,6 +519,10 @@ extern "C" mcpet_status mcpet_run(mcpet_ctx* c, const mcpet_run_params* rp)
     if (st != MCPET_OK) return st;
   }
   HIPCHK(c, hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * (KC_N + 1), c->stream));
+  if (c->singles_mode) {
+    HIPCHK(c, hipMemsetAsync(c->d_singles_n, 0, sizeof(unsigned long long), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_coinc, 0, 5 * sizeof(unsigned long long), c->stream));
+  }
   HIPCHK(c, hipEventRecord(c->ev_k0, c->stream));
   long long blocks;
   if (kind == MCPET_KERNEL_PERSISTENT) {
@@ -408,6 +541,21 @@ extern "C" mcpet_status mcpet_run(mcpet_ctx* c, const mcpet_run_params* rp)
   HIPCHK(c, hipEventRecord(c->ev_k1, c->stream));
   c->ran = true;
   c->timed = true;
+  if (!c->singles_mode) return MCPET_OK;
+  // Singles pipeline: the sort needs the number of singles, so wait for the transport.
+  HIPCHK(c, hipMemcpyAsync(&c->singles_req, c->d_singles_n, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->n_singles = (long long)std::min<unsigned long long>(c->singles_req, (unsigned long long)P.singles_cap);
+  // detection times are below acq + 2^40 ps (flight times are clamped there)
+  const unsigned long long tmax = P.acq_ps + (1ULL << 40);
+  int bits = 64;
+  while (bits > 1 && !((tmax >> (bits - 1)) & 1ULL)) bits--;
+  const int passes = (bits + 7) / 8;
+  hipError_t e = mcpet_launch_keys_from_singles(c->d_singles, c->n_singles, c->d_keys[0], c->d_vals[0], c->stream);
+  if (e == hipSuccess) e = mcpet_radix_sort(c->n_singles, passes, c->d_keys, c->d_vals, c->d_hist, c->d_scan.data(), &c->sorted, c->stream);
+  if (e == hipSuccess) e = mcpet_launch_pairing(P, c->d_keys[c->sorted], c->d_vals[c->sorted], c->n_singles, c->stream);
+  if (e != hipSuccess) return set_err(c, MCPET_E_HIP, std::string("coincidence sorting failed: ") + hipGetErrorString(e));
+  HIPCHK(c, hipEventRecord(c->ev_c1, c->stream));
   return MCPET_OK;
 }
 
@@ -448,6 +596,27 @@ extern "C" mcpet_status mcpet_fetch(mcpet_ctx* c, mcpet_tallies* t)
   HIPCHK(c, hipMemcpy(cnt, c->d_counters, sizeof cnt, hipMemcpyDeviceToHost));
   t->total_histories = 2ull * cnt[KC_DECAYS];  // two photon histories per decay (K:282)
   HIPCHK(c, hipMemcpy(&t->final_seed, c->d_final_seed, sizeof(int), hipMemcpyDeviceToHost));
+  t->singles_count = c->singles_mode ? c->n_singles : 0;
+  t->singles_requested = c->singles_mode ? (long long)c->singles_req : 0;
+  for (int k = 0; k < 5; k++) t->coincidences[k] = 0;
+  if (c->singles_mode) {
+    if (t->randoms) HIPCHK(c, hipMemcpy(t->randoms, c->d_randoms, 4 * nb, hipMemcpyDeviceToHost));
+    if (t->spurious) HIPCHK(c, hipMemcpy(t->spurious, c->d_spurious, 4 * nb, hipMemcpyDeviceToHost));
+    unsigned long long cc[5];
+    HIPCHK(c, hipMemcpy(cc, c->d_coinc, sizeof cc, hipMemcpyDeviceToHost));
+    for (int k = 0; k < 5; k++) t->coincidences[k] = (int64_t)cc[k];
+    if (t->singles && c->n_singles > 0) {
+      if (t->singles_buffer < c->n_singles)
+        return set_err(c, MCPET_E_CAPACITY, "mcpet_fetch: singles buffer smaller than the singles stored");
+      mcpet_single* tmp = nullptr;
+      HIPCHK(c, hipMalloc((void**)&tmp, (size_t)c->n_singles * sizeof(mcpet_single)));
+      hipError_t e = mcpet_launch_gather_singles(c->d_singles, c->d_vals[c->sorted], c->n_singles, tmp, c->stream);
+      if (e == hipSuccess) e = hipMemcpyAsync(t->singles, tmp, (size_t)c->n_singles * sizeof(mcpet_single), hipMemcpyDeviceToHost, c->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      (void)hipFree(tmp);
+      if (e != hipSuccess) return set_err(c, MCPET_E_HIP, std::string("mcpet_fetch: singles: ") + hipGetErrorString(e));
+    }
+  }
   // Sums over the sinograms (host copies if fetched, else device-side counters).
   if (t->trues && t->scatter) {
     long long a = 0, b = 0;
@@ -489,6 +658,17 @@ extern "C" mcpet_status mcpet_get_stats(mcpet_ctx* c, mcpet_stats* s)
   s->pairs_tallied = cnt[KC_PAIRS];
   s->coincidences = cnt[KC_COINC];
   s->rn_draws = cnt[KC_RNDRAWS];
+  s->pairs_emitted = cnt[KC_EMIT_PAIR];
+  s->prompts_emitted = cnt[KC_EMIT_PROMPT];
+  if (c->singles_mode) {
+    float cms = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&cms, c->ev_k1, c->ev_c1));
+    s->coinc_ms = cms;
+    s->singles = c->singles_req;
+    unsigned long long cc[5];
+    HIPCHK(c, hipMemcpy(cc, c->d_coinc, sizeof cc, hipMemcpyDeviceToHost));
+    s->coincidences = cc[0] + cc[1];
+  }
   for (int i = 0; i < MCPET_NPHASES; i++) {
     s->phase_exec[i] = cnt[KC_PHASE_EXEC + i];
     s->phase_lanes[i] = cnt[KC_PHASE_LANES + i];
@@ -577,6 +757,43 @@ extern "C" mcpet_status mcpet_selftest_fmath(const double* x, const double* y, i
   if (dy) (void)hipFree(dy);
   if (dod) (void)hipFree(dod);
   if (dof) (void)hipFree(dof);
+  return rc;
+}
+
+extern "C" mcpet_status mcpet_selftest_sort(const uint64_t* keys, int64_t n, int32_t passes, uint32_t* perm)
+{
+  if (!keys || !perm || n < 0 || n > 0x7fffffffLL || passes < 1 || passes > 8)
+    return set_err(nullptr, MCPET_E_ARG, "selftest_sort: bad args");
+  if (n == 0) return MCPET_OK;
+  unsigned long long* k[2] = {nullptr, nullptr};
+  unsigned* v[2] = {nullptr, nullptr};
+  unsigned* hist = nullptr;
+  const long long m = 256LL * mcpet_rs_tiles(n);
+  std::vector<long long> levels;
+  mcpet_scan_levels(m, levels);
+  std::vector<unsigned*> scan(levels.size(), nullptr);
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < 2 && e == hipSuccess; i++) {
+    e = hipMalloc((void**)&k[i], 8 * (size_t)n);
+    if (e == hipSuccess) e = hipMalloc((void**)&v[i], 4 * (size_t)n);
+  }
+  if (e == hipSuccess) e = hipMalloc((void**)&hist, 4 * (size_t)m);
+  for (size_t i = 0; i < levels.size() && e == hipSuccess; i++) e = hipMalloc((void**)&scan[i], 4 * (size_t)levels[i]);
+  int which = 0;
+  if (e == hipSuccess) e = hipMemcpy(k[0], keys, 8 * (size_t)n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = mcpet_launch_iota(v[0], n, 0);
+  if (e == hipSuccess) e = mcpet_radix_sort(n, passes, k, v, hist, scan.data(), &which, 0);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(perm, v[which], 4 * (size_t)n, hipMemcpyDeviceToHost);
+  mcpet_status rc = MCPET_OK;
+  if (e != hipSuccess) rc = set_err(nullptr, MCPET_E_HIP, std::string("selftest_sort: ") + hipGetErrorString(e));
+  for (int i = 0; i < 2; i++) {
+    if (k[i]) (void)hipFree(k[i]);
+    if (v[i]) (void)hipFree(v[i]);
+  }
+  if (hist) (void)hipFree(hist);
+  for (unsigned* p : scan)
+    if (p) (void)hipFree(p);
   return rc;
 }
 

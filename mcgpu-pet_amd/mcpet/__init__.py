@@ -19,7 +19,7 @@ LIB_PATH = os.environ.get("MCPET_LIB") or os.path.join(_PKG, "lib", "libmcpet.so
 
 MAX_MATERIALS = 15
 MAX_SHELLS = 30
-NPHASES = 8
+NPHASES = 10
 NP_RAYLEIGH = 128
 MAX_ENERGYBINS_RAYLEIGH = 25005
 NAME_LEN = 250
@@ -78,7 +78,10 @@ class Config(C.Structure):
                 ("nrad", C.c_int32), ("nzs", C.c_int32), ("res", C.c_int32), ("ne", C.c_int32), ("mrd", C.c_int32),
                 ("span", C.c_int32), ("nseg", C.c_int32), ("nsinos", C.c_int32), ("nbins", C.c_int32),
                 ("nvoxs", C.c_int32), ("voxel_file", C.c_char * NAME_LEN),
-                ("material_files", (C.c_char * NAME_LEN) * MAX_MATERIALS), ("bg_mode", C.c_int32)]
+                ("material_files", (C.c_char * NAME_LEN) * MAX_MATERIALS), ("bg_mode", C.c_int32), ("voxel_format", C.c_int32),
+                ("positron_fraction", C.c_float), ("prompt_gamma_fraction", C.c_float),
+                ("prompt_gamma_energy", C.c_float), ("reserved0", C.c_int32), ("singles_capacity", C.c_int64),
+                ("coincidence_window_ps", C.c_uint64)]
 
 
 class Phantom(C.Structure):
@@ -101,7 +104,10 @@ class Tallies(C.Structure):
                 ("spectrum", C.POINTER(C.c_int32)), ("psf", C.c_void_p), ("psf_capacity", C.c_int64),
                 ("voxel_edep", C.POINTER(C.c_uint64)), ("material_edep", C.POINTER(C.c_uint64)),
                 ("total_histories", C.c_uint64), ("psf_count", C.c_int64), ("psf_requested", C.c_int64),
-                ("sum_trues", C.c_int64), ("sum_scatter", C.c_int64), ("final_seed", C.c_int32)]
+                ("sum_trues", C.c_int64), ("sum_scatter", C.c_int64), ("final_seed", C.c_int32),
+                ("randoms", C.POINTER(C.c_int32)), ("spurious", C.POINTER(C.c_int32)), ("singles", C.c_void_p),
+                ("singles_buffer", C.c_int64), ("singles_count", C.c_int64), ("singles_requested", C.c_int64),
+                ("coincidences", C.c_int64 * 5)]
 
 
 class RunParams(C.Structure):
@@ -117,7 +123,8 @@ class Stats(C.Structure):
                 ("compton_shell_evals", C.c_uint64), ("rayleigh", C.c_uint64), ("photoelectric", C.c_uint64),
                 ("pairs_tallied", C.c_uint64), ("coincidences", C.c_uint64), ("rn_draws", C.c_uint64),
                 ("phase_exec", C.c_uint64 * NPHASES), ("phase_lanes", C.c_uint64 * NPHASES),
-                ("phase_cycles", C.c_uint64 * NPHASES)]
+                ("phase_cycles", C.c_uint64 * NPHASES), ("coinc_ms", C.c_double), ("singles", C.c_uint64),
+                ("pairs_emitted", C.c_uint64), ("prompts_emitted", C.c_uint64)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
@@ -168,6 +175,9 @@ def lib():
     L.mcpet_selftest_copy.argtypes = [C.c_size_t, C.c_int32, P(C.c_double)]
     L.mcpet_stream_seeds.argtypes = [C.c_int32, P(C.c_int64), C.c_int32, P(C.c_int32), P(C.c_int32)]
     L.mcpet_version.restype = C.c_char_p
+    L.mcpet_selftest_sort.argtypes = [P(C.c_uint64), C.c_int64, C.c_int32, P(C.c_uint32)]
+    L.mcpet_struct_sizes.argtypes = [C.POINTER(C.c_size_t), C.c_int32]
+    L.mcpet_struct_sizes.restype = C.c_int32
     _lib = L
     return L
 
@@ -329,7 +339,7 @@ class Simulation:
         _check(lib().mcpet_get_detector(self.ctx, C.byref(d)), self.ctx)
         return d
 
-    def fetch(self, sinograms=True, images=True, spectrum=True, psf=True, dose=True):
+    def fetch(self, sinograms=True, images=True, spectrum=True, psf=True, dose=True, singles=True):
         cfg = self.cfg
         nv = self.phantom.nvox
         out = {}
@@ -360,7 +370,25 @@ class Simulation:
                         (cfg.roi_z_max - cfg.roi_z_min + 1)
                 out["voxel_edep"] = np.zeros((n_roi, 2), np.uint64)
                 t.voxel_edep = out["voxel_edep"].ctypes.data_as(C.POINTER(C.c_uint64))
+        singles_arr = None
+        if cfg.detector.tally_psf_sinogram == TALLY_SINGLES:
+            if sinograms:
+                out["randoms"] = np.zeros(cfg.nbins, np.int32)
+                out["spurious"] = np.zeros(cfg.nbins, np.int32)
+                t.randoms = out["randoms"].ctypes.data_as(C.POINTER(C.c_int32))
+                t.spurious = out["spurious"].ctypes.data_as(C.POINTER(C.c_int32))
+            if singles:
+                singles_arr = np.zeros(cfg.singles_capacity, SINGLE_DTYPE)
+                t.singles = singles_arr.ctypes.data
+                t.singles_buffer = cfg.singles_capacity
         _check(lib().mcpet_fetch(self.ctx, C.byref(t)), self.ctx)
+        if singles_arr is not None:
+            out["singles"] = singles_arr[:t.singles_count].copy()
+        if cfg.detector.tally_psf_sinogram == TALLY_SINGLES:
+            out["singles_count"] = t.singles_count
+            out["singles_requested"] = t.singles_requested
+            out["coincidences"] = dict(zip(("trues", "scatter", "randoms", "spurious", "multiples"),
+                                           [int(x) for x in t.coincidences]))
         if psf_arr is not None:
             out["psf"] = psf_arr[:t.psf_count].copy()
         out["total_histories"] = t.total_histories
@@ -455,10 +483,36 @@ def selftest_fmath(x, y):
     return od, of
 
 
+def selftest_sort(keys, passes=8):
+    """Device radix sort of uint64 keys (low 8*passes bits); returns the stable sorted order."""
+    k = np.ascontiguousarray(keys, dtype=np.uint64)
+    perm = np.zeros(len(k), np.uint32)
+    _check(lib().mcpet_selftest_sort(k.ctypes.data_as(C.POINTER(C.c_uint64)), len(k), int(passes),
+                                     perm.ctypes.data_as(C.POINTER(C.c_uint32))))
+    return perm
+
+
 def selftest_copy(nbytes, iters=10):
     ms = C.c_double()
     _check(lib().mcpet_selftest_copy(int(nbytes), int(iters), C.byref(ms)))
     return ms.value
+
+
+class Single(C.Structure):
+    _fields_ = [("time_ps", C.c_uint64), ("event", C.c_uint64), ("energy", C.c_float), ("z", C.c_float),
+                ("phi", C.c_float), ("scatter", C.c_int16), ("kind", C.c_int16)]
+
+
+SINGLE_DTYPE = np.dtype([("time_ps", "<u8"), ("event", "<u8"), ("energy", "<f4"), ("z", "<f4"), ("phi", "<f4"),
+                         ("scatter", "<i2"), ("kind", "<i2")])
+TALLY_SINGLES = 3
+
+
+def struct_sizes():
+    """sizeof of the C structs as compiled into the library (see mcpet_struct_sizes)."""
+    out = (C.c_size_t * 8)()
+    n = lib().mcpet_struct_sizes(out, 8)
+    return list(out)[:n]
 
 
 def version():

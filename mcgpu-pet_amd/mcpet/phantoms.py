@@ -83,8 +83,10 @@ def make_config(*, seed=12345, threads_per_block=64, acquisition_time_s=1.0, mea
                 radius=-40.0, height=25.0, center=(0.0, 0.0, 0.0), psf_size=0, tally_psf_sinogram=2, tally_type=0,
                 material_dose=False, voxel_dose=False, roi=None, e_resol=0.12, e_low=350000.0, e_high=600000.0,
                 fovz=25.0, nrows=80, ncrystals=336, nangles=168, nrad=147, nzs=159, res=128, ne=700, mrd=79, span=11,
-                materials=(), bg_mode=False, gpu_id=0):
-    """A Config equivalent to a MCGPU-PET.in file with these values."""
+                materials=(), bg_mode=False, gpu_id=0, positron_fraction=0.0, prompt_gamma_fraction=0.0,
+                prompt_gamma_energy=0.0, singles_capacity=0, coincidence_window_ps=0):
+    """A Config equivalent to a MCGPU-PET.in file with these values (plus the library's
+    extensions: decay branching, prompt gamma and the singles pipeline, tally_psf_sinogram=3)."""
     c = Config()
     c.seed = seed
     c.gpu_id = gpu_id
@@ -114,6 +116,11 @@ def make_config(*, seed=12345, threads_per_block=64, acquisition_time_s=1.0, mea
     for i, m in enumerate(list(materials)[:MAX_MATERIALS]):
         c.material_files[i].value = os.fsencode(m)[:NAME_LEN - 1]
     c.bg_mode = 1 if bg_mode else 0
+    c.positron_fraction = positron_fraction
+    c.prompt_gamma_fraction = prompt_gamma_fraction
+    c.prompt_gamma_energy = prompt_gamma_energy
+    c.singles_capacity = int(singles_capacity)
+    c.coincidence_window_ps = int(coincidence_window_ps)
     return c
 
 
@@ -129,6 +136,9 @@ def config_dict(c):
         "mrd": c.mrd, "span": c.span, "nsinos": c.nsinos, "material_dose": c.material_dose,
         "roi": (c.roi_x_min, c.roi_x_max, c.roi_y_min, c.roi_y_max, c.roi_z_min, c.roi_z_max) if c.voxel_dose
         else (32500, -32500, 32500, -32500, 32500, -32500),
+        "positron_fraction": c.positron_fraction, "prompt_gamma_fraction": c.prompt_gamma_fraction,
+        "prompt_gamma_energy": c.prompt_gamma_energy, "singles_capacity": c.singles_capacity,
+        "coincidence_window_ps": c.coincidence_window_ps,
     }
 
 

@@ -17,7 +17,8 @@ MAXBINS_RAY = 25005
 MAXSH = 30
 
 COUNTER_NAMES = ["decays", "streams", "woodcock_steps", "material_changes", "compton", "compton_shell_evals",
-                 "rayleigh", "photoelectric", "pairs_tallied", "coincidences", "rn_draws"]
+                 "rayleigh", "photoelectric", "pairs_tallied", "coincidences", "rn_draws", "pairs_emitted",
+                 "prompts_emitted"]
 
 
 class OrcRayleigh(C.Structure):
@@ -52,8 +53,12 @@ class OrcSim(C.Structure):
         ("voxel_edep", C.POINTER(C.c_uint64)), ("material_edep", C.POINTER(C.c_uint64)),
         ("psf_index", C.c_int), ("total_histories", C.c_uint64), ("final_seed", C.c_int), ("final_seed_written", C.c_int),
         ("counters", C.c_uint64 * 16), ("nthreads", C.c_int), ("voxel_stride", C.c_int), ("voxel_offset", C.c_int),
-        ("wall_seconds", C.c_double)]
+        ("wall_seconds", C.c_double), ("positron_fraction", C.c_float), ("prompt_fraction", C.c_float),
+        ("prompt_energy", C.c_float), ("singles", C.c_void_p), ("singles_cap", C.c_longlong), ("singles_n", C.c_longlong)]
 
+
+SINGLE_DTYPE = np.dtype([("time_ps", "<u8"), ("event", "<u8"), ("energy", "<f4"), ("z", "<f4"), ("phi", "<f4"),
+                         ("scatter", "<i2"), ("kind", "<i2")])
 
 _libs = {}
 
@@ -203,6 +208,11 @@ class Oracle:
         s.material_dose = p.get("material_dose", 0)
         roi = p.get("roi", (32500, -32500, 32500, -32500, 32500, -32500))
         s.roi[:] = list(roi)
+        pf = p.get("positron_fraction", 0.0)
+        s.positron_fraction = pf if 0.0 < pf < 1.0 else 1.0
+        s.prompt_fraction = p.get("prompt_gamma_fraction", 0.0)
+        s.prompt_energy = p.get("prompt_gamma_energy", 0.0)
+        s.singles_cap = int(p.get("singles_capacity", 0))
         self.s = s
 
     def locate(self, x, y, z):
@@ -234,6 +244,12 @@ class Oracle:
             s.voxel_edep = out["voxel_edep"].ctypes.data_as(C.POINTER(C.c_uint64))
         else:
             s.voxel_edep = None
+        singles_buf = None
+        if s.tally_psf_sinogram == 3 and s.singles_cap > 0:
+            singles_buf = np.zeros(s.singles_cap * 32, np.uint8)
+            s.singles = singles_buf.ctypes.data
+        else:
+            s.singles = None
         s.nthreads = nthreads
         s.voxel_stride = voxel_stride
         s.voxel_offset = voxel_offset
@@ -245,6 +261,10 @@ class Oracle:
         if psf_buf is not None:
             n = min(s.psf_index, s.psf_size)
             out["psf"] = psf_buf[:n * 44].view(_psf_dtype()).copy()
+        if singles_buf is not None:
+            n = min(s.singles_n, s.singles_cap)
+            out["singles"] = singles_buf[:n * 32].view(SINGLE_DTYPE).copy()
+            out["singles_requested"] = s.singles_n
         out["final_seed"] = s.final_seed if s.final_seed_written else 0
         out["counters"] = {k: int(s.counters[i]) for i, k in enumerate(COUNTER_NAMES)}
         out["wall_seconds"] = s.wall_seconds
@@ -327,3 +347,55 @@ def fmath_eval(x, y):
     L.orc_fmath_eval(x.ctypes.data_as(C.POINTER(C.c_double)), y.ctypes.data_as(C.POINTER(C.c_double)), n,
                      od.ctypes.data_as(C.POINTER(C.c_double)), _f(of))
     return od, of
+
+
+def pair_singles(singles, window_ps, geom, oracle=None):
+    """Coincidence pairing of a singles list (the CPU statement of the library's pairing):
+    singles ordered by (time, event, kind); a coincidence is a cluster of exactly two singles
+    with consecutive gap <= window and gaps > window on both sides; clusters of three or more
+    are multiples. Classes: 0 true, 1 scattered, 2 random, 3 spurious (prompt gamma of the
+    same decay). Pairs are binned with the reference's Michelogram arithmetic (earlier single
+    first) through the oracle's binning, crystal indices as in tally_Sinogram (K:1749-1752).
+    geom: dict with ncrystals, nrows, fovz, nbins. Returns (sinograms[4, nbins], binned counts
+    per class + multiples [5], unbinned counts per class + multiples [5]); with oracle=None
+    nothing is binned."""
+    s = np.sort(np.asarray(singles), order=["time_ps", "event", "kind"], kind="stable")
+    t = s["time_ps"].astype(np.uint64)
+    n = len(s)
+    sinos = np.zeros((4, geom["nbins"]), np.int64)
+    counts = np.zeros(5, np.int64)
+    raw = np.zeros(5, np.int64)
+    if n < 2:
+        return sinos, counts, raw
+    W = np.uint64(window_ps)
+    gap = t[1:] - t[:-1]                                    # gap[i] = t[i+1] - t[i]
+    idx = np.arange(n - 1)
+    start = gap <= W
+    start &= np.concatenate([[True], gap[:-1] > W])         # left gap > W (or first)
+    multiple = start & np.concatenate([gap[1:] <= W, [False]])
+    counts[4] = int(multiple.sum())
+    pairs = idx[start & ~multiple]
+    a, b = s[pairs], s[pairs + 1]
+    same = a["event"] == b["event"]
+    prompt = (a["kind"] == 2) | (b["kind"] == 2)
+    unsc = (a["scatter"] == 0) & (b["scatter"] == 0)
+    cls = np.where(~same, 2, np.where(prompt, 3, np.where(unsc, 0, 1)))
+    raw[:4] = np.bincount(cls, minlength=4)[:4]
+    raw[4] = counts[4]
+    if oracle is None:
+        return sinos, counts, raw
+
+    def crystal(phi):
+        return np.floor(((phi.astype(np.float64) / (2.0 * np.pi) + 0.5) * geom["ncrystals"]).astype(np.float32)).astype(np.int64)
+
+    def ring(z):
+        q = (z.astype(np.float32) / np.float32(geom["fovz"])).astype(np.float64)
+        return np.floor(((q + 0.5) * geom["nrows"]).astype(np.float32)).astype(np.int64)
+
+    ix1, iz1, ix2, iz2 = crystal(a["phi"]), ring(a["z"]), crystal(b["phi"]), ring(b["z"])
+    for k in range(len(pairs)):
+        ibin = oracle.sinogram_bin(int(ix1[k]), int(iz1[k]), int(ix2[k]), int(iz2[k]))
+        if ibin >= 0:
+            sinos[cls[k], ibin] += 1
+            counts[cls[k]] += 1
+    return sinos, counts, raw

@@ -732,6 +732,7 @@ static void o_thread(orc_sim* s, int absvox_block, int tid, o_local* L)
         acq_thread = acq_thread - Dt_ps;
         histories_thread = histories_thread + 2;
         L->c[ORC_C_DECAYS]++;
+        L->c[ORC_C_EMIT_PAIR]++;
         inv_activity_thread = inv_activity_thread * o_exp(Dt * inv_mean_life);
       } else {
         break;
@@ -839,6 +840,233 @@ static void o_thread(orc_sim* s, int absvox_block, int tid, o_local* L)
   L->c[ORC_C_RNDRAWS] += r.draws;
 }
 
+/* ------------------------------------------------------------------ extended decay model
+ * Restatement of the library's extensions (no reference counterpart): positron and
+ * prompt-gamma branching per decay and the singles list. The photon transport and the
+ * pair tallies are the reference's (K:297-442) as in o_thread; the random-number order
+ * per decay is: decay time, positron branch (only if positron_fraction < 1), prompt
+ * branch (only if prompt_fraction > 0), source() of photon 1, [photon 1, its single,
+ * photon 2's direction, photon 2, its single or the pair tallies], [prompt gamma's
+ * isotropic direction, prompt gamma, its single]. */
+
+/* One photon from pos/dir with the given energy until it escapes or is absorbed
+ * (K:297-415); the body of o_thread's inner loop. */
+static void o_transport(orc_sim* s, float* position, float* direction, float* energy_io, int* index_io, int* ss_io,
+                        float* travel_io, o_rng* r, o_local* L)
+{
+  const float* vmd = s->voxel_mat_dens;
+  float energy = *energy_io, step, prob = 0.0f, randno = 0.0f, mfp_density = 0.0f, mfp_Woodcock;
+  float ra[3] = {0, 0, 0}, rb[3] = {0, 0, 0};
+  int material0 = 0, material_old, absvox;
+  int scatter_state = 0;
+  float travel_distance = 0.0f;
+  int index = o_f2i_rd((energy - s->e0) * s->ide);
+  mfp_Woodcock = s->woodcock[2 * index] + energy * s->woodcock[2 * index + 1];
+  material_old = -1;
+  for (;;) {
+    absvox = o_locate(s, position);
+    if (absvox < 0) break;
+    do {
+      step = -(mfp_Woodcock)*o_logf(RN(r));
+      L->c[ORC_C_WOODCOCK]++;
+      travel_distance += step;
+      position[0] += step * direction[0];
+      position[1] += step * direction[1];
+      position[2] += step * direction[2];
+      absvox = o_locate(s, position);
+      if (absvox < 0) break;
+      float mdx = vmd[3 * (size_t)absvox], mdy = vmd[3 * (size_t)absvox + 1];
+      material0 = (int)(mdx - 1);
+      if (material0 != material_old) {
+        const size_t b = 3 * ((size_t)index * O_MAXMAT + material0);
+        ra[0] = s->mfp_a[b]; ra[1] = s->mfp_a[b + 1]; ra[2] = s->mfp_a[b + 2];
+        rb[0] = s->mfp_b[b]; rb[1] = s->mfp_b[b + 1]; rb[2] = s->mfp_b[b + 2];
+        material_old = material0;
+        L->c[ORC_C_MATCHANGE]++;
+      }
+      mfp_density = mfp_Woodcock * mdy;
+      prob = 1.0f - mfp_density * (ra[0] + energy * rb[0]);
+      randno = RN(r);
+    } while (randno < prob);
+    if (absvox < 0) break;
+    prob += mfp_density * (ra[1] + energy * rb[1]);
+    if (randno < prob) {
+      double costh_Compton;
+      randno = energy;
+      L->c[ORC_C_COMPTON]++;
+      o_GCOa(&energy, &costh_Compton, material0, r, s->compton, &L->c[ORC_C_SHELLEVAL]);
+      o_rotate_double(direction, costh_Compton, 6.28318530717958647693 * RND(r));
+      randno = energy - randno;
+      index = o_f2i_rd((energy - s->e0) * s->ide);
+      if (index > -1) {
+        mfp_Woodcock = s->woodcock[2 * index] + energy * s->woodcock[2 * index + 1];
+        material_old = -2;
+        scatter_state = (scatter_state == 0) ? 1 : 3;
+      }
+    } else {
+      prob += mfp_density * (ra[2] + energy * rb[2]);
+      if (randno < prob) {
+        double costh_Rayleigh;
+        L->c[ORC_C_RAYLEIGH]++;
+        float pmax_current = s->rayleigh->pmax[(index + 1) * O_MAXMAT + material0];
+        o_GRAa(energy, &costh_Rayleigh, material0, pmax_current, r, s->rayleigh);
+        o_rotate_double(direction, costh_Rayleigh, 6.28318530717958647693 * RND(r));
+        scatter_state = (scatter_state == 0) ? 2 : 3;
+      } else {
+        L->c[ORC_C_PHOTO]++;
+        randno = -energy;
+        index = -11;
+      }
+    }
+    if (randno < -0.001f) o_tally_dose(s, -1.0f * randno, material0, position);
+    if (index < 0) break;
+  }
+  *energy_io = energy;
+  *index_io = index;
+  *ss_io = scatter_state;
+  *travel_io = travel_distance;
+}
+
+/* A photon that left the phantom: detector hit within the height, energy blur (the PSF
+ * path's form, K:580-587), singles spectrum, energy window, and the 32-byte record with
+ * detection time = emission time + flight time to the cylinder. */
+static void o_record_single(orc_sim* s, const float* pos, const float* dir, float energy, float travel, int ss, int kind,
+                            unsigned long long remaining, unsigned long long event, o_rng* r)
+{
+  float x0 = pos[0] - s->det_center[0];
+  float y0 = pos[1] - s->det_center[1];
+  float A = dir[0] * dir[0] + dir[1] * dir[1];
+  float B = 2.0f * (x0 * dir[0] + y0 * dir[1]);
+  float Cq = x0 * x0 + y0 * y0 - s->det_radius * s->det_radius;
+  float dist;
+  if (A == 0.0f) dist = 99999999.9f;
+  else dist = (-B + sqrtf(B * B - 4.0f * A * Cq)) / (2.0f * A);
+  float z = (pos[2] - s->det_center[2]) + dist * dir[2];
+  if (!(fabsf(z) < 0.5f * s->det_height)) return;
+  float randno1 = RN(r);
+  float randno2 = RN(r);
+  float g = sqrtf(-2.0f * o_logf(randno1 + 1e-8f)) * o_cosf((float)(2.0f * O_PI * (double)randno2));
+  float e = energy + ((s->e_resol) * (1.0f / 2.35f)) * energy * g;
+  int iE = o_f2i_rd(e / 1000.0f);
+  if ((iE < s->ne) && (iE >= 0)) __atomic_fetch_add(&s->spectrum[iE], 1, __ATOMIC_RELAXED);
+  if (!(e > s->e_low && e < s->e_high)) return;
+  float phi = o_atan2f(y0 + dist * dir[1], x0 + dist * dir[0]);
+  unsigned long long flight = o_d2ull_rn((double)(travel + dist) * O_INV_SPEEDOFLIGHT);
+  if (flight >= (1ull << 40)) flight = (1ull << 40) - 1ull;
+  unsigned long long t = (s->acquisition_time_ps - remaining) + flight;
+  long long k = __atomic_fetch_add(&s->singles_n, 1, __ATOMIC_RELAXED);
+  if (k < s->singles_cap && s->singles) {
+    unsigned char* p = s->singles + (size_t)k * 32;
+    short s16 = (short)ss, k16 = (short)kind;
+    memcpy(p + 0, &t, 8);
+    memcpy(p + 8, &event, 8);
+    memcpy(p + 16, &e, 4);
+    memcpy(p + 20, &z, 4);
+    memcpy(p + 24, &phi, 4);
+    memcpy(p + 28, &s16, 2);
+    memcpy(p + 30, &k16, 2);
+  }
+}
+
+static void o_thread_ext(orc_sim* s, int absvox_block, int tid, o_local* L)
+{
+  const float* vmd = s->voxel_mat_dens;
+  float voxel_activity = vmd[3 * (size_t)absvox_block + 2];
+  unsigned long long acq_thread = s->acquisition_time_ps;
+  unsigned long long histories_thread = 0;
+  double inv_activity_thread;
+  if (voxel_activity > 1.0e-7f) inv_activity_thread = ((double)s->threads_per_block / (double)voxel_activity);
+  else return;
+  const double inv_mean_life = 1.0 / (double)s->mean_life;
+  const int ix = absvox_block % s->nx;
+  const int iy = (absvox_block / s->nx) % s->ny;
+  const int iz = absvox_block / (s->nx * s->ny);
+  const int singles = s->tally_psf_sinogram == 3;
+  o_rng r;
+  r.draws = 0;
+  long long thread_id = (long long)absvox_block * s->threads_per_block + tid;
+  orc_init_PRNG(thread_id, 555555, s->seed, &r.x, &r.y);
+  L->c[ORC_C_STREAMS]++;
+  unsigned long long event = (unsigned long long)thread_id << 32;
+  long long img_t0 = L->img_t, img_sc0 = L->img_sc;
+  for (;;) {
+    double Dt = -o_log(RND(&r)) * inv_activity_thread;
+    unsigned long long Dt_ps = o_d2ull_rn(Dt * 1e12);
+    if (!(Dt_ps < acq_thread)) break;
+    acq_thread = acq_thread - Dt_ps;
+    histories_thread = histories_thread + 2;
+    L->c[ORC_C_DECAYS]++;
+    inv_activity_thread = inv_activity_thread * o_exp(Dt * inv_mean_life);
+    event++;
+    int pair = 1, prompt = 0;
+    if (s->positron_fraction < 1.0f) pair = RN(&r) < s->positron_fraction;
+    if (s->prompt_fraction > 0.0f) prompt = RN(&r) < s->prompt_fraction;
+    float position[3], direction[3], position1[3], direction1[3] = {11.0f, 0.0f, 0.0f}, energyFirst;
+    o_source(s, ix, iy, iz, position, direction, position1, direction1, &energyFirst, &r);
+    if (prompt) L->c[ORC_C_EMIT_PROMPT]++;
+    if (pair) {
+      L->c[ORC_C_EMIT_PAIR]++;
+      float energy = O_E511, travel, travelFirst = 0.0f;
+      int index, ss, ssFirst = 0;
+      float positionFirst[3] = {0, 0, 0}, directionFirst[3] = {0, 0, 0};
+      o_transport(s, position, direction, &energy, &index, &ss, &travel, &r, L);
+      if (index > -1) {
+        if (singles) {
+          o_record_single(s, position, direction, energy, travel, ss, 0, acq_thread, event, &r);
+        } else {
+          energyFirst = energy;
+          memcpy(positionFirst, position, sizeof position);
+          memcpy(directionFirst, direction, sizeof direction);
+          ssFirst = ss;
+          travelFirst = travel;
+        }
+      }
+      o_source(s, ix, iy, iz, position, direction, position1, direction1, &energyFirst, &r);
+      energy = O_E511;
+      o_transport(s, position, direction, &energy, &index, &ss, &travel, &r, L);
+      if (index > -1) {
+        if (singles) {
+          o_record_single(s, position, direction, energy, travel, ss, 1, acq_thread, event, &r);
+        } else if (energyFirst > -0.5f &&
+                   (s->tally_type == 0 || (s->tally_type == 1 && ss == 0 && ssFirst == 0) ||
+                    (s->tally_type == 2 && (ss != 0 || ssFirst != 0)))) {
+          if ((s->tally_psf_sinogram == 0) || (s->tally_psf_sinogram == 2))
+            o_tally_sinogram(s, energy, energyFirst, position, positionFirst, direction, directionFirst, ss, ssFirst,
+                             absvox_block, &r, L);
+          if ((s->tally_psf_sinogram == 0) || (s->tally_psf_sinogram == 1)) {
+            float p2[3] = {position[0], position[1], position[2]};
+            float pf[3] = {positionFirst[0], positionFirst[1], positionFirst[2]};
+            o_tally_psf(s, energy, energyFirst, p2, pf, direction, directionFirst, ss, ssFirst, acq_thread, travel,
+                        travelFirst, absvox_block, &r);
+          }
+        }
+      }
+    }
+    if (prompt) {
+      float dir3[3], pos3[3] = {position1[0], position1[1], position1[2]};
+      dir3[2] = 1.0f - 2.0f * RN(&r);
+      float phi_sampled = 6.28318530716f * RN(&r);
+      float sin_theta = sqrtf(1.0f - dir3[2] * dir3[2]);
+      float sp, cp;
+      o_sincosf(phi_sampled, &sp, &cp);
+      dir3[1] = sin_theta * sp;
+      dir3[0] = sin_theta * cp;
+      float energy = s->prompt_energy, travel;
+      int index, ss;
+      o_transport(s, pos3, dir3, &energy, &index, &ss, &travel, &r, L);
+      if (index > -1 && singles) o_record_single(s, pos3, dir3, energy, travel, ss, 2, acq_thread, event, &r);
+    }
+  }
+  if (absvox_block == s->nx * s->ny * s->nz - 1 && tid == s->threads_per_block - 1) {  /* K:454-455 */
+    s->final_seed = r.x;
+    s->final_seed_written = 1;
+  }
+  __atomic_fetch_add(&s->total_histories, histories_thread, __ATOMIC_RELAXED);
+  if (L->img_t != img_t0) __atomic_fetch_add(&s->image_trues[absvox_block], (int)(L->img_t - img_t0), __ATOMIC_RELAXED);
+  if (L->img_sc != img_sc0) __atomic_fetch_add(&s->image_scatter[absvox_block], (int)(L->img_sc - img_sc0), __ATOMIC_RELAXED);
+  L->c[ORC_C_RNDRAWS] += r.draws;
+}
+
 typedef struct o_worker {
   orc_sim* s;
   long long* next;
@@ -857,7 +1085,11 @@ static void* o_worker_main(void* arg)
     if (k >= nsel) break;
     long long v = s->voxel_offset + k * stride;
     if (!(s->voxel_mat_dens[3 * v + 2] > 1.0e-7f)) continue;
-    for (int t = 0; t < s->threads_per_block; t++) o_thread(s, (int)v, t, &w->L);
+    const int ext = s->tally_psf_sinogram == 3 || s->positron_fraction < 1.0f || s->prompt_fraction > 0.0f;
+    for (int t = 0; t < s->threads_per_block; t++) {
+      if (ext) o_thread_ext(s, (int)v, t, &w->L);
+      else o_thread(s, (int)v, t, &w->L);
+    }
   }
   return NULL;
 }
@@ -870,6 +1102,7 @@ int orc_run(orc_sim* s)
   if (!ws || !th) { free(ws); free(th); return -1; }
   long long next = 0;
   s->psf_index = 0;
+  s->singles_n = 0;
   s->total_histories = 0;
   s->final_seed = 0;
   s->final_seed_written = 0;

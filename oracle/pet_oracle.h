@@ -47,7 +47,8 @@ typedef struct orc_compton {
 /* Counters reported by orc_run (same meaning as the library's mcpet_stats). */
 enum {
   ORC_C_DECAYS = 0, ORC_C_STREAMS, ORC_C_WOODCOCK, ORC_C_MATCHANGE, ORC_C_COMPTON, ORC_C_SHELLEVAL,
-  ORC_C_RAYLEIGH, ORC_C_PHOTO, ORC_C_PAIRS, ORC_C_COINC, ORC_C_RNDRAWS, ORC_C_N = 16
+  ORC_C_RAYLEIGH, ORC_C_PHOTO, ORC_C_PAIRS, ORC_C_COINC, ORC_C_RNDRAWS, ORC_C_EMIT_PAIR, ORC_C_EMIT_PROMPT,
+  ORC_C_N = 16
 };
 
 typedef struct orc_sim {
@@ -98,6 +99,15 @@ typedef struct orc_sim {
   int voxel_stride;                      /* simulate every k-th voxel (1 = all) */
   int voxel_offset;
   double wall_seconds;                   /* transport wall time */
+  /* extended decay model and singles list (no reference counterpart; the library's
+   * mcpet_config extensions): effective positron branching (1 = every decay), prompt-gamma
+   * branching and energy, and the singles list (32-byte records, see mcpet_single) */
+  float positron_fraction;
+  float prompt_fraction;
+  float prompt_energy;
+  unsigned char* singles;
+  long long singles_cap;
+  long long singles_n;
 } orc_sim;
 
 /* RANECU and skip-ahead (K:898-1083, C:2906-2935) */

@@ -159,3 +159,32 @@ def test_production_kernel_equals_instrumented_and_oracle():
     o = sc.run_oracle("fmath")
     g = sc.run_gpu(kernel=mcpet.KERNEL_PERSISTENT, count_events=False)
     assert_tallies_equal(g, o, counters="lite")
+
+
+@pytest.mark.parametrize("k_distinct,fmt", [(6, "u8 palette"), (3000, "u16 palette"), (0, "8-byte words")])
+def test_voxel_layouts_bit_exact(k_distinct, fmt):
+    """Phantoms with few, many and all-distinct (density, material) pairs select the u8-palette,
+    u16-palette and 8-byte device layouts; each equals the oracle bit for bit, and forcing the
+    8-byte layout gives the same tallies."""
+    from conftest import AIR, WATER
+    from harness import Scenario
+    from mcpet import phantoms
+    _say(f"48^3 voxels, {k_distinct or 'all'} distinct densities ({fmt}), 1e5 decays")
+    n = 48
+    rng = np.random.default_rng(100 + k_distinct)
+    mat = rng.choice([1, 2], size=(n, n, n), p=[0.1, 0.9]).astype(np.int32)
+    if k_distinct:
+        vals = rng.uniform(0.3, 1.9, k_distinct).astype(np.float32)
+        dens = vals[rng.integers(0, k_distinct, mat.shape)]
+    else:
+        dens = rng.uniform(0.3, 1.9, mat.shape).astype(np.float32)
+    frac = phantoms.F18_MEAN_LIFE * (1.0 - np.exp(-1.0 / phantoms.F18_MEAN_LIFE))
+    act = np.full(mat.shape, 1e5 / (mat.size * frac), np.float32)
+    cfg = phantoms.make_config(seed=777, threads_per_block=4, materials=[AIR, WATER], material_dose=True)
+    sc = Scenario(cfg, mat, dens, act, (0.3, 0.3, 0.3), [AIR, WATER])
+    o = sc.run_oracle("fmath")
+    g = sc.run_gpu(kernel=mcpet.KERNEL_PERSISTENT)
+    assert_tallies_equal(g, o)
+    sc.cfg.voxel_format = 1
+    g8 = sc.run_gpu(kernel=mcpet.KERNEL_PERSISTENT, count_events=False)
+    assert_tallies_equal(g8, o, counters="lite")
