@@ -591,8 +591,10 @@ __device__ __forceinline__ void tally_dose(const KParams& P, float Edep, int mat
 struct Photon {
   float pos[3], dir[3];
   float E, travel, mfpW, mfp_density, prob, randno;
-  float2 tab;            // (a, b) of the total cross section at (index, mat_old)
-  int index, mat_old, mat0, ss;
+  float2 tab;            // (a, b) of the total cross section at (index, material of mat_old)
+  int index;
+  int mat_old;           // voxel material word of the last step (< 0: reload the cross section)
+  int mat0, ss;
 };
 
 __device__ __forceinline__ void photon_start(const KParams& P, Photon& ph, float energy = K_E511)
@@ -632,10 +634,14 @@ __device__ __forceinline__ int woodcock_step(const KParams& P, Photon& ph, Rng& 
   const int absvox = locate(P, ph.pos);
   if (absvox < 0) return STEP_ESCAPED;
   const float2 md = voxel_word(P, absvox);
-  ph.mat0 = __float_as_int(md.y);
-  if (ph.mat0 != ph.mat_old) {
-    ph.tab = P.tot[ph.index * K_MAXMAT + ph.mat0];
-    ph.mat_old = ph.mat0;
+  // material word: slot (bits 0-7) | id of its total cross-section table (bits 8-15); slots
+  // whose tables are bitwise identical share an id, so a change between them (a material
+  // change of the reference, K:331-336) keeps the (a, b) already loaded: the same values
+  const int w = __float_as_int(md.y);
+  ph.mat0 = w & 0xff;
+  if (w != ph.mat_old) {
+    if (ph.mat_old < 0 || (w >> 8) != (ph.mat_old >> 8)) ph.tab = P.tot[ph.index * K_MAXMAT + ph.mat0];
+    ph.mat_old = w;
     C.inc(KC_MATCHANGE);
   }
   ph.mfp_density = ph.mfpW * md.x;
@@ -1064,9 +1070,12 @@ __device__ __forceinline__ int after_photon(const KParams& P, int photon_no, boo
   return (photon_no == 2 && prompt) ? LS_SOURCE3 : LS_DECAY;
 }
 
-template <class CN>
+// kExt: the singles list or prompt gammas are in use (phases DETECT and SOURCE3 exist);
+// otherwise the wave only ballots over the seven phases of the reference's pair tallies.
+template <class CN, bool kExt>
 __global__ __launch_bounds__(K_BLOCK) K_OCCUPANCY void k_transport_sm(KParams P)
 {
+  constexpr int kPhases = kExt ? LS_N : LS_CPZ + 1;
   K_PROLOGUE(CN)
   __shared__ float s_slots[LANE_SLOTS][K_BLOCK];
   const LaneSlots L{&s_slots[0][threadIdx.x]};
@@ -1081,7 +1090,7 @@ __global__ __launch_bounds__(K_BLOCK) K_OCCUPANCY void k_transport_sm(KParams P)
     if (P.sm_policy == 0) {
       int bestc = 0;
 #pragma unroll
-      for (int sidx = 0; sidx < LS_N; sidx++) {
+      for (int sidx = 0; sidx < kPhases; sidx++) {
         const int c = __popcll(__ballot(state == sidx));
         if (c > bestc) {
           bestc = c;
@@ -1091,7 +1100,7 @@ __global__ __launch_bounds__(K_BLOCK) K_OCCUPANCY void k_transport_sm(KParams P)
     } else {
       int bestw = 0x7fffffff;
 #pragma unroll
-      for (int sidx = 0; sidx < LS_N; sidx++) {
+      for (int sidx = 0; sidx < kPhases; sidx++) {
         const int c = __popcll(__ballot(state == sidx));
         const int w = (64 - c) * P.sm_cost[sidx];
         if (c > 0 && w < bestw) {
@@ -1196,10 +1205,10 @@ __global__ __launch_bounds__(K_BLOCK) K_OCCUPANCY void k_transport_sm(KParams P)
       photon_start(P, ph);
       photon_no = 2;
       state = (locate(P, ph.pos) < 0) ? after_photon(P, 2, prompt, ph, L) : LS_STEP;
-    } else if (best == LS_DETECT) {
+    } else if (kExt && best == LS_DETECT) {
       if (ph.index > -1) record_single(P, ph.pos, ph.dir, ph.E, ph.travel, ph.ss, photon_no - 1, st, C, s_spec);
       state = (photon_no == 1) ? LS_SOURCE2 : (photon_no == 2 && prompt) ? LS_SOURCE3 : LS_DECAY;
-    } else if (best == LS_SOURCE3) {
+    } else if (kExt && best == LS_SOURCE3) {
       for (int k = 0; k < 3; k++) ph.pos[k] = L[PS_P1 + k];
       isotropic_dir(st, C, ph.dir);
       photon_start(P, ph, P.prompt_energy);
@@ -1273,10 +1282,15 @@ hipError_t mcpet_launch_transport(const KParams& P, int kind, int count_events, 
 {
   if (block_size > K_BLOCK || block_size % 64 != 0) return hipErrorInvalidValue;
   size_t lds = P.lds_spectrum ? (size_t)P.ne * sizeof(int) : 0;
-  if (kind == MCPET_KERNEL_PERSISTENT && count_events)
-    hipLaunchKernelGGL(k_transport_sm<FullCounters>, dim3(blocks), dim3(block_size), lds, stream, P);
+  const bool ext = P.mode == MCPET_TALLY_SINGLES || P.prompt_frac > 0.0f;
+  if (kind == MCPET_KERNEL_PERSISTENT && count_events && ext)
+    hipLaunchKernelGGL((k_transport_sm<FullCounters, true>), dim3(blocks), dim3(block_size), lds, stream, P);
+  else if (kind == MCPET_KERNEL_PERSISTENT && count_events)
+    hipLaunchKernelGGL((k_transport_sm<FullCounters, false>), dim3(blocks), dim3(block_size), lds, stream, P);
+  else if (kind == MCPET_KERNEL_PERSISTENT && ext)
+    hipLaunchKernelGGL((k_transport_sm<LiteCounters, true>), dim3(blocks), dim3(block_size), lds, stream, P);
   else if (kind == MCPET_KERNEL_PERSISTENT)
-    hipLaunchKernelGGL(k_transport_sm<LiteCounters>, dim3(blocks), dim3(block_size), lds, stream, P);
+    hipLaunchKernelGGL((k_transport_sm<LiteCounters, false>), dim3(blocks), dim3(block_size), lds, stream, P);
   else
     hipLaunchKernelGGL(k_transport_plain, dim3(blocks), dim3(block_size), lds, stream, P);
   return hipGetLastError();

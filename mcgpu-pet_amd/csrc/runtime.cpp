@@ -241,7 +241,24 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
     c->det.radius = -c->det.radius;
   }
 
-  // Voxels -> packed (density, material index) words; emitting voxels sorted by activity.
+  // Table id per material slot: slots whose total cross-section columns are bitwise
+  // identical (the same material file) share the smallest such slot's id.
+  int table_id[MCPET_MAX_MATERIALS];
+  for (int m = 0; m < MCPET_MAX_MATERIALS; m++) {
+    table_id[m] = m;
+    for (int m2 = 0; m2 < m; m2++) {
+      bool same = true;
+      for (int b = 0; b < tab->num_values && same; b++) {
+        const size_t i1 = 3 * ((size_t)b * MCPET_MAX_MATERIALS + m), i2 = 3 * ((size_t)b * MCPET_MAX_MATERIALS + m2);
+        same = std::memcmp(&tab->mfp_a[i1], &tab->mfp_a[i2], 4) == 0 && std::memcmp(&tab->mfp_b[i1], &tab->mfp_b[i2], 4) == 0;
+      }
+      if (same) {
+        table_id[m] = table_id[m2];
+        break;
+      }
+    }
+  }
+  // Voxels -> packed (density, material word) pairs; emitting voxels sorted by activity.
   std::vector<float2> hv((size_t)c->nvox);
   std::vector<int> act_idx;
   for (long long v = 0; v < c->nvox; v++) {
@@ -253,7 +270,8 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
       return set_err(nullptr, MCPET_E_RANGE, "mcpet_create: voxel material out of range");
     }
     float bits;
-    std::memcpy(&bits, &m0, 4);
+    const int word = m0 | (table_id[m0] << 8);
+    std::memcpy(&bits, &word, 4);
     hv[(size_t)v] = make_float2(p[1], bits);
     if (p[2] > 1.0e-7f) act_idx.push_back((int)v);   // K:232-237
   }
