@@ -63,6 +63,67 @@ def cpu_baseline(cfg, ph, vs, names, threads, stride):
                       f"{threads} threads)"}
 
 
+def bench_batch(a):
+    """Many-phantom workload: a.batch_size varied phantoms (random_phantoms: 32-160 x 32-160 x
+    32-96 voxels at 2 mm, 1e6-1e8 photon histories each, 10 s), sinogram 168x168x1293 each.
+    Rates of three ways to run it: the plain kernel launched per phantom, the persistent
+    kernel launched per phantom, and the batched persistent kernel (one launch for all
+    phantoms, replayed as a HIP graph with the autotuned launch shape)."""
+    import mcpet
+    from mcpet import phantoms
+    specs = phantoms.random_phantoms(a.batch_size)
+    names = phantoms.head_materials(MATERIALS)
+    cfg = phantoms.make_config(seed=20261202, threads_per_block=a.tpb, acquisition_time_s=10.0, nrad=168, materials=names)
+    phs = [mcpet.phantom_from_arrays(*sp) for sp in specs]
+    dmax = np.max([p.density_max() for p in phs], axis=0)
+    tab = mcpet.load_materials(names, dmax)
+
+    def per_phantom(kernel):
+        coinc, secs = 0, 0.0
+        for ph in phs:
+            with mcpet.Simulation(cfg, ph, tab) as sim:
+                sim.run(kernel=kernel, seed=1000)      # warm-up of this context
+                sim.sync()
+                t0 = time.perf_counter()
+                sim.run(kernel=kernel, seed=2000)
+                st = sim.stats()
+                secs += time.perf_counter() - t0
+                coinc += st["coincidences"]
+        return coinc, secs
+
+    plain_c, plain_s = per_phantom(mcpet.KERNEL_PLAIN)
+    pers_c, pers_s = per_phantom(mcpet.KERNEL_PERSISTENT)
+    with mcpet.Batch(cfg, phs, tab) as b:
+        tune = b.autotune(0.1)
+        for w in range(a.warmup):
+            b.run(seed=1000 + w, use_graph=True)
+            b.sync()
+        coinc, decays, kms = 0, 0, []
+        t0 = time.perf_counter()
+        for s in range(a.steps):
+            b.run(seed=2000 + s, use_graph=True)
+            b.sync()
+            sts = [b.stats(k) for k in range(len(phs))]
+            coinc += sum(x["coincidences"] for x in sts)
+            decays += sum(x["decays"] for x in sts)
+            kms.append(sts[0]["kernel_ms"])
+        elapsed = time.perf_counter() - t0
+    value = coinc / elapsed
+    out = {"metric": "detected coincidences/s on 1 MI355X", "value": value, "unit": "coincidences/s", "n_gpus": 1,
+           "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * elapsed / a.steps, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"{len(phs)} varied phantoms (32-160 x 32-160 x 32-96 voxels at 2 mm, 1e6-1e8 "
+                                  "photon histories each, 10 s), sinogram 168x168x1293 per phantom, one batched "
+                                  "persistent launch replayed as a HIP graph", "decays_per_step": decays / a.steps,
+                      "threads_per_block": a.tpb, "parallelism": "single GPU"},
+           "kernel_ms_avg": sum(kms) / len(kms),
+           "plain_per_phantom": {"value": plain_c / plain_s, "seconds": plain_s},
+           "persistent_per_phantom": {"value": pers_c / pers_s, "seconds": pers_s},
+           "speedup_vs_plain_per_phantom": value / (plain_c / plain_s),
+           "autotune": tune}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +137,9 @@ def main():
     ap.add_argument("--phase-policy", type=int, default=0, help="0 auto, 1 most lanes, 2 least waste")
     ap.add_argument("--phase-cost", type=str, default="", help="comma-separated relative phase costs (DECAY, SOURCE2, STEP, INTERACT, TALLY, CTAU, CPZ)")
     ap.add_argument("--voxel-format", type=int, default=0, help="0 auto (palette indices), 1 8-byte voxel words")
+    ap.add_argument("--workload", choices=["head", "batch"], default="head",
+                    help="head: the 1e9-decay head phantom; batch: 48 varied phantoms in one launch")
+    ap.add_argument("--batch-size", type=int, default=48)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-stride", type=int, default=5)
@@ -94,6 +158,8 @@ def main():
         print(json.dumps({"copy_bytes": n, "iters": 5, "ms_per_copy": ms, "GB/s": 2 * n / ms / 1e6}))
         return
     from mcpet import phantoms
+    if a.workload == "batch":
+        return bench_batch(a)
     mat, dens, act, vs = phantoms.head(n_decays=a.decays)
     names = phantoms.head_materials(MATERIALS)
     cfg = phantoms.make_config(seed=20261202, threads_per_block=a.tpb, acquisition_time_s=10.0, nrad=168,

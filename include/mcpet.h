@@ -212,6 +212,7 @@ typedef struct mcpet_run_params {
                                     in mcpet_stats (same tallies; slower). 0: only decays,
                                     streams, pairs_tallied and coincidences are counted. The
                                     plain kernel always counts every event.              */
+  int32_t use_graph;             /* batches: replay the clearing + launch as a HIP graph  */
 } mcpet_run_params;
 
 /* Per-run statistics: HIP-event kernel time and algorithmic event counts (see
@@ -288,7 +289,9 @@ mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phantom* ph, cons
                           mcpet_ctx** out);
 
 /* Launch the transport (asynchronous on the context stream). Replaces
- * track_particles<<<blocks,threads>>> (MCGPU-PET.cu:452-487; kernel K:153-463). */
+ * track_particles<<<blocks,threads>>> (MCGPU-PET.cu:452-487; kernel K:153-463).
+ * Waits for the context's previous run to finish first (its tallies are overwritten);
+ * in the singles pipeline it also waits for the transport, to size the sort. */
 mcpet_status mcpet_run(mcpet_ctx* ctx, const mcpet_run_params* params);
 
 /* Wait for the context stream; reports kernel faults (C:486-487). */
@@ -309,6 +312,37 @@ const char* mcpet_last_error(const mcpet_ctx* ctx);
 
 /* Effective detector after mcpet_create (centre/radius rule applied). */
 mcpet_status mcpet_get_detector(const mcpet_ctx* ctx, mcpet_detector* out);
+
+/* ------------------------------------------------------------------ batches of phantoms
+ * No reference counterpart (the reference runs one phantom per process). A batch simulates
+ * n phantoms that share the scanner, tally settings and interaction tables (cfg, tab), each
+ * with its own voxels, activity and tallies, in ONE persistent launch with a stream queue
+ * per phantom. Phantom k's tallies equal those of mcpet_create(cfg, &phantoms[k], tab) +
+ * mcpet_run with the same run parameters, bit for bit. The pair tallies (modes 0-2) only. */
+typedef struct mcpet_batch mcpet_batch;
+
+typedef struct mcpet_tune_result {
+  int32_t block_size, blocks_per_cu;
+  double ms;                     /* batch kernel time (HIP events)                        */
+  double decays_per_ms;
+} mcpet_tune_result;
+
+mcpet_status mcpet_batch_create(const mcpet_config* cfg, const mcpet_phantom* phantoms, int32_t n, const mcpet_tables* tab,
+                                mcpet_batch** out);
+/* kernel is ignored (always the persistent batch kernel); block_size/blocks_per_cu 0 take the
+ * autotuned values if mcpet_batch_autotune ran, else the defaults; use_graph replays the
+ * clearing and the launch as a captured HIP graph (rebuilt when the launch shape changes). */
+mcpet_status mcpet_batch_run(mcpet_batch* b, const mcpet_run_params* params);
+mcpet_status mcpet_batch_sync(mcpet_batch* b);
+mcpet_status mcpet_batch_fetch(mcpet_batch* b, int32_t k, mcpet_tallies* out);
+/* Counts of phantom k; kernel_ms is the whole batch launch. */
+mcpet_status mcpet_batch_get_stats(mcpet_batch* b, int32_t k, mcpet_stats* out);
+/* Time the batch at a fraction of the acquisition for each launch shape (block size x
+ * workgroups per CU), keep the fastest for later runs; writes up to max_results timings. */
+mcpet_status mcpet_batch_autotune(mcpet_batch* b, double acquisition_fraction, mcpet_tune_result* results,
+                                  int32_t max_results, int32_t* n_results);
+void mcpet_batch_destroy(mcpet_batch* b);
+const char* mcpet_batch_last_error(const mcpet_batch* b);
 
 /* ------------------------------------------------------------------ host-side output (boundary) */
 

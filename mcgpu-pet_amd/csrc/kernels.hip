@@ -617,7 +617,7 @@ __shared__ float2 k_palette[256];
 // The (density, material) word of voxel absvox in the device layout chosen at set-up.
 __device__ __forceinline__ float2 voxel_word(const KParams& P, int absvox)
 {
-  if (P.vox_fmt == 1) return k_palette[P.vox8[absvox]];
+  if (P.vox_fmt == 1) return P.lds_palette ? k_palette[P.vox8[absvox]] : P.palette[P.vox8[absvox]];
   if (P.vox_fmt == 2) return P.palette[P.vox16[absvox]];
   return P.vox[absvox];
 }
@@ -911,7 +911,7 @@ __device__ __forceinline__ void stream_finish(const KParams& P, const Stream& st
       dst[i] = src[i];                                                                           \
     if (threadIdx.x < 2 * K_MAXMAT) s_mat[threadIdx.x] = 0ull;                                   \
     if (threadIdx.x < KC_N) s_cnt[threadIdx.x] = 0u;                                             \
-    if (P.vox_fmt == 1)                                                                          \
+    if (P.vox_fmt == 1 && P.lds_palette)                                                         \
       for (int i = threadIdx.x; i < P.n_palette; i += blockDim.x) k_palette[i] = P.palette[i];   \
     if (P.lds_spectrum)                                                                          \
       for (int i = threadIdx.x; i < P.ne; i += blockDim.x) s_spec[i] = 0;                        \
@@ -1070,15 +1070,15 @@ __device__ __forceinline__ int after_photon(const KParams& P, int photon_no, boo
   return (photon_no == 2 && prompt) ? LS_SOURCE3 : LS_DECAY;
 }
 
-// kExt: the singles list or prompt gammas are in use (phases DETECT and SOURCE3 exist);
-// otherwise the wave only ballots over the seven phases of the reference's pair tallies.
+// The state machine of one wave on one phantom, until the phantom's stream queue is empty
+// and every lane of the wave has finished its stream. kExt: the singles list or prompt
+// gammas are in use (phases DETECT and SOURCE3 exist); otherwise the wave only ballots
+// over the seven phases of the reference's pair tallies.
 template <class CN, bool kExt>
-__global__ __launch_bounds__(K_BLOCK) K_OCCUPANCY void k_transport_sm(KParams P)
+__device__ __forceinline__ void sm_waves(const KParams& P, CN& C, const mcpet_compton_table& s_cmp, int* s_spec,
+                                         unsigned long long* s_mat, const LaneSlots& L)
 {
   constexpr int kPhases = kExt ? LS_N : LS_CPZ + 1;
-  K_PROLOGUE(CN)
-  __shared__ float s_slots[LANE_SLOTS][K_BLOCK];
-  const LaneSlots L{&s_slots[0][threadIdx.x]};
   Stream st;
   Photon ph;
   bool have = false;
@@ -1225,7 +1225,69 @@ __global__ __launch_bounds__(K_BLOCK) K_OCCUPANCY void k_transport_sm(KParams P)
     if (__lane_id() == 0) C.add(KC_PHASE_CYCLES + best, (unsigned)(clock64() - t_phase));
 #endif
   }
+}
+
+// Persistent kernel over one phantom.
+template <class CN, bool kExt>
+// Parameters are read through a pointer to device memory: only the fields a phase uses are
+// loaded (as scalars), instead of a by-value block that would be kept in scalar registers
+// (and spilled) or copied to private memory.
+__global__ __launch_bounds__(K_BLOCK) K_OCCUPANCY void k_transport_sm(const KParams* __restrict__ Pd)
+{
+  const KParams& P = *Pd;
+  K_PROLOGUE(CN)
+  __shared__ float s_slots[LANE_SLOTS][K_BLOCK];
+  const LaneSlots L{&s_slots[0][threadIdx.x]};
+  sm_waves<CN, kExt>(P, C, s_cmp, s_spec, s_mat, L);
   K_EPILOGUE
+}
+
+// Batched persistent kernel over n phantoms that share their interaction tables and
+// scanner (params Ps[0..n-1], one per phantom, each with its own stream queue, voxels and
+// tallies). Every wave works on one phantom at a time, so all parameters stay wave-uniform:
+// it pulls streams from that phantom's queue, and when the queue is empty and its lanes are
+// done, it flushes its own counters and material dose (per-wave LDS slices) to that phantom
+// and moves to the next one. Waves start at different phantoms and visit every phantom
+// once, so the phantoms' queues drain concurrently and no phantom waits for a launch of its
+// own. The spectrum goes straight to global memory and the palette is read from global
+// memory (both are per phantom). Per phantom, streams and random numbers are those of a
+// single-phantom run, so each phantom's tallies equal its own run's bit for bit.
+template <class CN, bool kExt>
+__global__ __launch_bounds__(K_BLOCK) K_OCCUPANCY void k_transport_batch(const KParams* __restrict__ Ps, int n)
+{
+  __shared__ mcpet_compton_table s_cmp;
+  __shared__ unsigned long long s_matw[K_BLOCK / 64][2 * K_MAXMAT];
+  __shared__ unsigned s_cntw[K_BLOCK / 64][KC_N];
+  __shared__ float s_slots[LANE_SLOTS][K_BLOCK];
+  {
+    const int* src = reinterpret_cast<const int*>(Ps[0].cmp);
+    int* dst = reinterpret_cast<int*>(&s_cmp);
+    for (int i = threadIdx.x; i < (int)(sizeof(mcpet_compton_table) / 4); i += blockDim.x) dst[i] = src[i];
+    for (int i = threadIdx.x; i < (K_BLOCK / 64) * 2 * K_MAXMAT; i += blockDim.x) (&s_matw[0][0])[i] = 0ull;
+    for (int i = threadIdx.x; i < (K_BLOCK / 64) * KC_N; i += blockDim.x) (&s_cntw[0][0])[i] = 0u;
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const LaneSlots L{&s_slots[0][threadIdx.x]};
+  const int first = (int)(((long long)blockIdx.x * (blockDim.x >> 6) + wave) % n);
+  for (int j = 0; j < n; j++) {
+    const int k = __builtin_amdgcn_readfirstlane((first + j) % n);
+    const KParams& P = Ps[k];
+    CN C;
+    C.init(s_cntw[wave]);
+    sm_waves<CN, kExt>(P, C, s_cmp, nullptr, s_matw[wave], L);
+    C.flush(P, s_cntw[wave]);
+    __builtin_amdgcn_wave_barrier();
+    if (lane < KC_N && s_cntw[wave][lane]) {
+      atomicAdd(&P.counters[lane], (unsigned long long)s_cntw[wave][lane]);
+      s_cntw[wave][lane] = 0u;
+    }
+    if (lane < 2 * K_MAXMAT && s_matw[wave][lane]) {
+      atomicAdd(&P.mat_edep[lane], s_matw[wave][lane]);
+      s_matw[wave][lane] = 0ull;
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
 }
 
 // ---------------------------------------------------------------------------------- self tests
@@ -1269,6 +1331,15 @@ __global__ void k_selftest_fmath(const double* x, const double* y, int n, double
   of[7 * (size_t)n + i] = mcf_acosf(fminf(fmaxf(bf, -1.0f), 1.0f));
 }
 
+// Clear n words (a memset that a captured graph may hold).
+__global__ void k_fill_u32(unsigned* __restrict__ p, size_t n)
+{
+  const size_t base = (size_t)blockIdx.x * 256 * 16 + threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < 16; k++)
+    if (base + (size_t)k * 256 < n) p[base + (size_t)k * 256] = 0u;
+}
+
 __global__ void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16)
 {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1278,21 +1349,45 @@ __global__ void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, s
 
 }  // namespace
 
-hipError_t mcpet_launch_transport(const KParams& P, int kind, int count_events, int blocks, int block_size, hipStream_t stream)
+hipError_t mcpet_launch_transport(const KParams& P, const KParams* d_P, int kind, int count_events, int blocks, int block_size,
+                                  hipStream_t stream)
 {
   if (block_size > K_BLOCK || block_size % 64 != 0) return hipErrorInvalidValue;
   size_t lds = P.lds_spectrum ? (size_t)P.ne * sizeof(int) : 0;
   const bool ext = P.mode == MCPET_TALLY_SINGLES || P.prompt_frac > 0.0f;
   if (kind == MCPET_KERNEL_PERSISTENT && count_events && ext)
-    hipLaunchKernelGGL((k_transport_sm<FullCounters, true>), dim3(blocks), dim3(block_size), lds, stream, P);
+    hipLaunchKernelGGL((k_transport_sm<FullCounters, true>), dim3(blocks), dim3(block_size), lds, stream, d_P);
   else if (kind == MCPET_KERNEL_PERSISTENT && count_events)
-    hipLaunchKernelGGL((k_transport_sm<FullCounters, false>), dim3(blocks), dim3(block_size), lds, stream, P);
+    hipLaunchKernelGGL((k_transport_sm<FullCounters, false>), dim3(blocks), dim3(block_size), lds, stream, d_P);
   else if (kind == MCPET_KERNEL_PERSISTENT && ext)
-    hipLaunchKernelGGL((k_transport_sm<LiteCounters, true>), dim3(blocks), dim3(block_size), lds, stream, P);
+    hipLaunchKernelGGL((k_transport_sm<LiteCounters, true>), dim3(blocks), dim3(block_size), lds, stream, d_P);
   else if (kind == MCPET_KERNEL_PERSISTENT)
-    hipLaunchKernelGGL((k_transport_sm<LiteCounters, false>), dim3(blocks), dim3(block_size), lds, stream, P);
+    hipLaunchKernelGGL((k_transport_sm<LiteCounters, false>), dim3(blocks), dim3(block_size), lds, stream, d_P);
   else
     hipLaunchKernelGGL(k_transport_plain, dim3(blocks), dim3(block_size), lds, stream, P);
+  return hipGetLastError();
+}
+
+hipError_t mcpet_launch_transport_batch(const KParams* d_params, int n, bool ext, int count_events, int blocks, int block_size,
+                                        hipStream_t stream)
+{
+  if (block_size > K_BLOCK || block_size % 64 != 0 || n < 1) return hipErrorInvalidValue;
+  if (count_events && ext)
+    hipLaunchKernelGGL((k_transport_batch<FullCounters, true>), dim3(blocks), dim3(block_size), 0, stream, d_params, n);
+  else if (count_events)
+    hipLaunchKernelGGL((k_transport_batch<FullCounters, false>), dim3(blocks), dim3(block_size), 0, stream, d_params, n);
+  else if (ext)
+    hipLaunchKernelGGL((k_transport_batch<LiteCounters, true>), dim3(blocks), dim3(block_size), 0, stream, d_params, n);
+  else
+    hipLaunchKernelGGL((k_transport_batch<LiteCounters, false>), dim3(blocks), dim3(block_size), 0, stream, d_params, n);
+  return hipGetLastError();
+}
+
+hipError_t mcpet_launch_fill_u32(void* p, size_t n_words, hipStream_t stream)
+{
+  if (n_words == 0) return hipSuccess;
+  const size_t per = 256 * 16;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n_words + per - 1) / per)), dim3(256), 0, stream, (unsigned*)p, n_words);
   return hipGetLastError();
 }
 

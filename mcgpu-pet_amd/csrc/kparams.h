@@ -88,6 +88,7 @@ struct KParams {
   int* final_seed;
   long long last_thread_id;      // reference thread id whose final seed is stored (K:454-455)
   int lds_spectrum;              // 1: privatise the spectrum in LDS
+  int lds_palette;               // 1: u8 palette copied to LDS (single-phantom kernels)
   // Decay branching (drawn per decay only when active) and the singles list.
   float positron_frac;           // < 1: a decay emits the annihilation pair with this probability
   float prompt_frac;             // > 0: a decay emits a prompt gamma with this probability
@@ -107,7 +108,11 @@ struct KParams {
 };
 
 // Launchers implemented in kernels.hip.
-hipError_t mcpet_launch_transport(const KParams& P, int kind, int count_events, int blocks, int block_size, hipStream_t stream);
+hipError_t mcpet_launch_transport(const KParams& P, const KParams* d_P, int kind, int count_events, int blocks, int block_size,
+                                  hipStream_t stream);
+hipError_t mcpet_launch_transport_batch(const KParams* d_params, int n, bool ext, int count_events, int blocks, int block_size,
+                                        hipStream_t stream);
+hipError_t mcpet_launch_fill_u32(void* p, size_t n_words, hipStream_t stream);
 hipError_t mcpet_launch_selftest_ranecu(int seed, const long long* d_ids, int nstreams, int n, float* d_out, double* d_out_d,
                                         int as_double, const uint32_t* j1, const uint32_t* j2, hipStream_t stream);
 hipError_t mcpet_launch_selftest_fmath(const double* d_x, const double* d_y, int n, double* d_od, float* d_of, hipStream_t stream);

@@ -54,6 +54,8 @@ struct mcpet_ctx {
   unsigned long long* d_mat_edep = nullptr;
   unsigned long long* d_counters = nullptr;  // KC_N counters + stream queue
   int* d_final_seed = nullptr;
+  KParams* d_params = nullptr;   // the run's parameter block (device copy)
+  KParams* h_params = nullptr;   // pinned staging copy
   // singles pipeline
   bool singles_mode = false;
   mcpet_single* d_singles = nullptr;
@@ -128,6 +130,8 @@ void free_all(mcpet_ctx* c)
   for (unsigned* p : c->d_scan)
     if (p) (void)hipFree(p);
   if (c->ev_c1) (void)hipEventDestroy(c->ev_c1);
+  if (c->d_params) (void)hipFree(c->d_params);
+  if (c->h_params) (void)hipHostFree(c->h_params);
   if (c->ev_k0) (void)hipEventDestroy(c->ev_k0);
   if (c->ev_k1) (void)hipEventDestroy(c->ev_k1);
   if (c->ev_z0) (void)hipEventDestroy(c->ev_z0);
@@ -358,6 +362,8 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
   if (voxel_dose) CK(dalloc(c, &c->d_voxel_edep, 2 * (size_t)c->n_roi));
   CK(dalloc(c, &c->d_mat_edep, 2 * MCPET_MAX_MATERIALS));
   CK(dalloc(c, &c->d_counters, KC_N + 1));
+  CK(dalloc(c, &c->d_params, 1));
+  HK(hipHostMalloc((void**)&c->h_params, sizeof(KParams), hipHostMallocDefault));
   c->singles_mode = (mode == MCPET_TALLY_SINGLES);
   if (c->singles_mode) {
     const long long cap = cfg->singles_capacity;
@@ -483,31 +489,50 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
   P.final_seed = c->d_final_seed;
   P.last_thread_id = (c->nvox - 1) * (long long)cfg->threads_per_block + (cfg->threads_per_block - 1);
   P.lds_spectrum = (cfg->ne > 0 && cfg->ne <= 8192) ? 1 : 0;
+  P.lds_palette = 1;
   *out = c;
 #undef CK
 #undef HK
   return MCPET_OK;
 }
 
-static mcpet_status zero_tallies(mcpet_ctx* c)
+// Clear the tallies (and with counters, the event counters and stream queue) on `stream`,
+// with memsets or, for a captured graph, with a fill kernel.
+static mcpet_status clear_buffers(mcpet_ctx* c, hipStream_t stream, bool fill, bool tallies, bool counters)
 {
-  const KParams& P = c->P;
-  HIPCHK(c, hipMemsetAsync(c->d_trues, 0, sizeof(int) * (size_t)c->cfg.nbins, c->stream));
-  HIPCHK(c, hipMemsetAsync(c->d_scatter, 0, sizeof(int) * (size_t)c->cfg.nbins, c->stream));
-  HIPCHK(c, hipMemsetAsync(c->d_img_t, 0, sizeof(int) * (size_t)c->nvox, c->stream));
-  HIPCHK(c, hipMemsetAsync(c->d_img_sc, 0, sizeof(int) * (size_t)c->nvox, c->stream));
-  HIPCHK(c, hipMemsetAsync(c->d_spectrum, 0, sizeof(int) * (size_t)std::max(1, c->cfg.ne), c->stream));
-  HIPCHK(c, hipMemsetAsync(c->d_psf_index, 0, sizeof(int), c->stream));
-  if (c->d_voxel_edep) HIPCHK(c, hipMemsetAsync(c->d_voxel_edep, 0, 16 * (size_t)c->n_roi, c->stream));
-  HIPCHK(c, hipMemsetAsync(c->d_mat_edep, 0, 16 * MCPET_MAX_MATERIALS, c->stream));
-  HIPCHK(c, hipMemsetAsync(c->d_final_seed, 0, sizeof(int), c->stream));
-  if (c->singles_mode) {
-    HIPCHK(c, hipMemsetAsync(c->d_randoms, 0, sizeof(int) * (size_t)c->cfg.nbins, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->d_spurious, 0, sizeof(int) * (size_t)c->cfg.nbins, c->stream));
+  struct Region {
+    void* p;
+    size_t bytes;
+  };
+  std::vector<Region> r;
+  if (tallies) {
+    r.push_back({c->d_trues, sizeof(int) * (size_t)c->cfg.nbins});
+    r.push_back({c->d_scatter, sizeof(int) * (size_t)c->cfg.nbins});
+    r.push_back({c->d_img_t, sizeof(int) * (size_t)c->nvox});
+    r.push_back({c->d_img_sc, sizeof(int) * (size_t)c->nvox});
+    r.push_back({c->d_spectrum, sizeof(int) * (size_t)std::max(1, c->cfg.ne)});
+    r.push_back({c->d_psf_index, sizeof(int)});
+    if (c->d_voxel_edep) r.push_back({c->d_voxel_edep, 16 * (size_t)c->n_roi});
+    r.push_back({c->d_mat_edep, 16 * MCPET_MAX_MATERIALS});
+    r.push_back({c->d_final_seed, sizeof(int)});
+    if (c->singles_mode) {
+      r.push_back({c->d_randoms, sizeof(int) * (size_t)c->cfg.nbins});
+      r.push_back({c->d_spurious, sizeof(int) * (size_t)c->cfg.nbins});
+    }
   }
-  (void)P;
+  if (counters) r.push_back({c->d_counters, sizeof(unsigned long long) * (KC_N + 1)});
+  for (const Region& x : r) {
+    if (fill) {
+      hipError_t e = mcpet_launch_fill_u32(x.p, x.bytes / 4, stream);
+      if (e != hipSuccess) return set_err(c, MCPET_E_HIP, std::string("clearing tallies: ") + hipGetErrorString(e));
+    } else {
+      HIPCHK(c, hipMemsetAsync(x.p, 0, x.bytes, stream));
+    }
+  }
   return MCPET_OK;
 }
+
+static mcpet_status zero_tallies(mcpet_ctx* c) { return clear_buffers(c, c->stream, false, true, false); }
 
 extern "C" mcpet_status mcpet_run(mcpet_ctx* c, const mcpet_run_params* rp)
 {
@@ -516,6 +541,8 @@ extern "C" mcpet_status mcpet_run(mcpet_ctx* c, const mcpet_run_params* rp)
   p.zero_tallies = 1;
   if (rp) p = *rp;
   HIPCHK(c, hipSetDevice(c->device));
+  // the parameter block's staging copy may still be read by the previous run's copy
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   KParams P = c->P;
   if (p.seed != 0) P.seed = p.seed;
   if (!seed_usable(P.seed)) return set_err(c, MCPET_E_ARG, seed_msg(P.seed));
@@ -553,7 +580,9 @@ extern "C" mcpet_status mcpet_run(mcpet_ctx* c, const mcpet_run_params* rp)
     if (blocks > 0x7fffffffLL) return set_err(c, MCPET_E_RANGE, "mcpet_run: too many streams for one launch");
   }
   if (P.n_streams > 0) {
-    hipError_t e = mcpet_launch_transport(P, kind, p.count_events, (int)blocks, bs, c->stream);
+    *c->h_params = P;
+    HIPCHK(c, hipMemcpyAsync(c->d_params, c->h_params, sizeof(KParams), hipMemcpyHostToDevice, c->stream));
+    hipError_t e = mcpet_launch_transport(P, c->d_params, kind, p.count_events, (int)blocks, bs, c->stream);
     if (e != hipSuccess) return set_err(c, MCPET_E_HIP, std::string("transport launch failed: ") + hipGetErrorString(e));
   }
   HIPCHK(c, hipEventRecord(c->ev_k1, c->stream));
@@ -651,6 +680,8 @@ extern "C" mcpet_status mcpet_fetch(mcpet_ctx* c, mcpet_tallies* t)
   return MCPET_OK;
 }
 
+static mcpet_status read_counts(mcpet_ctx* c, mcpet_stats* s);
+
 extern "C" mcpet_status mcpet_get_stats(mcpet_ctx* c, mcpet_stats* s)
 {
   if (!c || !s) return set_err(c, MCPET_E_ARG, "mcpet_get_stats: null argument");
@@ -663,6 +694,11 @@ extern "C" mcpet_status mcpet_get_stats(mcpet_ctx* c, mcpet_stats* s)
   HIPCHK(c, hipEventElapsedTime(&zms, c->ev_z0, c->ev_k0));
   s->kernel_ms = ms;
   s->zero_ms = zms;
+  return read_counts(c, s);
+}
+
+static mcpet_status read_counts(mcpet_ctx* c, mcpet_stats* s)
+{
   unsigned long long cnt[KC_N + 1];
   HIPCHK(c, hipMemcpy(cnt, c->d_counters, sizeof cnt, hipMemcpyDeviceToHost));
   s->decays = cnt[KC_DECAYS];
@@ -841,4 +877,241 @@ extern "C" mcpet_status mcpet_selftest_copy(size_t nbytes, int32_t iters, double
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   return rc;
+}
+
+// ---------------------------------------------------------------------------------- batches
+
+struct mcpet_batch {
+  std::vector<mcpet_ctx*> ctx;
+  int device = 0;
+  int num_cus = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  KParams* d_params = nullptr;
+  KParams* h_params = nullptr;   // pinned, n entries
+  int tuned_bs = 0, tuned_per_cu = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_key[3] = {-1, -1, -1};  // block size, blocks, count_events of the captured launch
+  bool ran = false;
+  std::string err;
+};
+
+namespace {
+
+mcpet_status bset_err(mcpet_batch* b, mcpet_status st, const std::string& m)
+{
+  if (b) b->err = m;
+  return set_err(nullptr, st, m);
+}
+
+void batch_free(mcpet_batch* b)
+{
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (mcpet_ctx* c : b->ctx) mcpet_destroy(c);
+  if (b->d_params) (void)hipFree(b->d_params);
+  if (b->h_params) (void)hipHostFree(b->h_params);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+}
+
+#define BHIP(b, call)                                                                                   \
+  do {                                                                                                  \
+    hipError_t e_ = (call);                                                                             \
+    if (e_ != hipSuccess) return bset_err(b, MCPET_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+}  // namespace
+
+extern "C" mcpet_status mcpet_batch_create(const mcpet_config* cfg, const mcpet_phantom* phantoms, int32_t n,
+                                           const mcpet_tables* tab, mcpet_batch** out)
+{
+  if (!out) return set_err(nullptr, MCPET_E_ARG, "mcpet_batch_create: null output pointer");
+  *out = nullptr;
+  if (!cfg || !phantoms || !tab || n < 1) return set_err(nullptr, MCPET_E_ARG, "mcpet_batch_create: bad arguments");
+  if (cfg->detector.tally_psf_sinogram == MCPET_TALLY_SINGLES)
+    return set_err(nullptr, MCPET_E_ARG, "mcpet_batch_create: batches support the pair tallies (modes 0-2) only");
+  mcpet_batch* b = new mcpet_batch();
+  for (int32_t k = 0; k < n; k++) {
+    mcpet_ctx* c = nullptr;
+    mcpet_status st = mcpet_create(cfg, &phantoms[k], tab, &c);
+    if (st != MCPET_OK) {
+      std::string m = std::string("mcpet_batch_create: phantom ") + std::to_string(k) + ": " + mcpet_last_error(nullptr);
+      batch_free(b);
+      delete b;
+      return set_err(nullptr, st, m);
+    }
+    b->ctx.push_back(c);
+  }
+  b->device = b->ctx[0]->device;
+  b->num_cus = b->ctx[0]->num_cus;
+  hipError_t e = hipSetDevice(b->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_params, sizeof(KParams) * (size_t)n);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_params, sizeof(KParams) * (size_t)n, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    std::string m = std::string("mcpet_batch_create: ") + hipGetErrorString(e);
+    batch_free(b);
+    delete b;
+    return set_err(nullptr, MCPET_E_HIP, m);
+  }
+  *out = b;
+  return MCPET_OK;
+}
+
+extern "C" mcpet_status mcpet_batch_run(mcpet_batch* b, const mcpet_run_params* rp)
+{
+  if (!b) return set_err(nullptr, MCPET_E_ARG, "mcpet_batch_run: null batch");
+  mcpet_run_params p{};
+  p.zero_tallies = 1;
+  if (rp) p = *rp;
+  BHIP(b, hipSetDevice(b->device));
+  BHIP(b, hipStreamSynchronize(b->stream));   // the staging copy of the parameters is reused
+  const int n = (int)b->ctx.size();
+  bool ext = false;
+  for (int k = 0; k < n; k++) {
+    KParams P = b->ctx[k]->P;
+    if (p.seed != 0) P.seed = p.seed;
+    if (!seed_usable(P.seed)) return bset_err(b, MCPET_E_ARG, seed_msg(P.seed));
+    if (p.acquisition_time_ps != 0) P.acq_ps = p.acquisition_time_ps;
+    P.sm_policy = (p.phase_policy == 2) ? 1 : 0;
+    static const int default_cost[MCPET_NPHASES] = {4, 6, 1, 3, 6, 3, 2, 4, 4, 1};
+    for (int i = 0; i < MCPET_NPHASES; i++) P.sm_cost[i] = p.phase_cost[i] > 0 ? p.phase_cost[i] : default_cost[i];
+    P.lds_spectrum = 0;   // per phantom: global atomics
+    P.lds_palette = 0;
+    ext = ext || P.prompt_frac > 0.0f;
+    b->h_params[k] = P;
+  }
+  const int bs = p.block_size > 0 ? p.block_size : (b->tuned_bs > 0 ? b->tuned_bs : 256);
+  if (bs % 64 != 0 || bs > 256) return bset_err(b, MCPET_E_RANGE, "mcpet_batch_run: block_size must be a multiple of 64, <= 256");
+  const int per_cu = p.blocks_per_cu > 0 ? p.blocks_per_cu : (b->tuned_per_cu > 0 ? b->tuned_per_cu : 8);
+  const int blocks = b->num_cus * per_cu;
+  BHIP(b, hipMemcpyAsync(b->d_params, b->h_params, sizeof(KParams) * (size_t)n, hipMemcpyHostToDevice, b->stream));
+  if (p.use_graph) {
+    const int key[3] = {bs, blocks, p.count_events ? 1 : 0};
+    if (!b->gexec || std::memcmp(key, b->graph_key, sizeof key) != 0 || !p.zero_tallies) {
+      if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+      if (b->graph) (void)hipGraphDestroy(b->graph);
+      b->gexec = nullptr;
+      b->graph = nullptr;
+      BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      mcpet_status st = MCPET_OK;
+      for (int k = 0; k < n && st == MCPET_OK; k++) st = clear_buffers(b->ctx[k], b->stream, true, p.zero_tallies != 0, true);
+      hipError_t e = hipSuccess;
+      if (st == MCPET_OK) e = mcpet_launch_transport_batch(b->d_params, n, ext, p.count_events, blocks, bs, b->stream);
+      hipGraph_t g = nullptr;
+      hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+      if (st != MCPET_OK) return st;
+      if (e != hipSuccess || e2 != hipSuccess)
+        return bset_err(b, MCPET_E_HIP, std::string("mcpet_batch_run: graph capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+      b->graph = g;
+      BHIP(b, hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+      std::memcpy(b->graph_key, key, sizeof key);
+    }
+    BHIP(b, hipEventRecord(b->ev0, b->stream));
+    BHIP(b, hipGraphLaunch(b->gexec, b->stream));
+    BHIP(b, hipEventRecord(b->ev1, b->stream));
+  } else {
+    for (int k = 0; k < n; k++) {
+      mcpet_status st = clear_buffers(b->ctx[k], b->stream, false, p.zero_tallies != 0, true);
+      if (st != MCPET_OK) return st;
+    }
+    BHIP(b, hipEventRecord(b->ev0, b->stream));
+    hipError_t e = mcpet_launch_transport_batch(b->d_params, n, ext, p.count_events, blocks, bs, b->stream);
+    if (e != hipSuccess) return bset_err(b, MCPET_E_HIP, std::string("batch launch failed: ") + hipGetErrorString(e));
+    BHIP(b, hipEventRecord(b->ev1, b->stream));
+  }
+  for (mcpet_ctx* c : b->ctx) c->ran = true;
+  b->ran = true;
+  return MCPET_OK;
+}
+
+extern "C" mcpet_status mcpet_batch_sync(mcpet_batch* b)
+{
+  if (!b) return set_err(nullptr, MCPET_E_ARG, "mcpet_batch_sync: null batch");
+  BHIP(b, hipSetDevice(b->device));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  return MCPET_OK;
+}
+
+extern "C" mcpet_status mcpet_batch_fetch(mcpet_batch* b, int32_t k, mcpet_tallies* out)
+{
+  if (!b || k < 0 || k >= (int32_t)b->ctx.size()) return bset_err(b, MCPET_E_ARG, "mcpet_batch_fetch: bad phantom index");
+  if (!b->ran) return bset_err(b, MCPET_E_STATE, "mcpet_batch_fetch: no run");
+  mcpet_status st = mcpet_batch_sync(b);
+  if (st != MCPET_OK) return st;
+  st = mcpet_fetch(b->ctx[k], out);
+  if (st != MCPET_OK) b->err = b->ctx[k]->err;
+  return st;
+}
+
+extern "C" mcpet_status mcpet_batch_get_stats(mcpet_batch* b, int32_t k, mcpet_stats* s)
+{
+  if (!b || !s || k < 0 || k >= (int32_t)b->ctx.size()) return bset_err(b, MCPET_E_ARG, "mcpet_batch_get_stats: bad arguments");
+  if (!b->ran) return bset_err(b, MCPET_E_STATE, "mcpet_batch_get_stats: no run");
+  mcpet_status st = mcpet_batch_sync(b);
+  if (st != MCPET_OK) return st;
+  std::memset(s, 0, sizeof *s);
+  float ms = 0.0f;
+  BHIP(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  s->kernel_ms = ms;
+  return read_counts(b->ctx[k], s);
+}
+
+extern "C" mcpet_status mcpet_batch_autotune(mcpet_batch* b, double fraction, mcpet_tune_result* results, int32_t max_results,
+                                             int32_t* n_results)
+{
+  if (!b || !(fraction > 0.0 && fraction <= 1.0)) return bset_err(b, MCPET_E_ARG, "mcpet_batch_autotune: bad arguments");
+  static const int shapes[][2] = {{256, 3}, {256, 4}, {256, 6}, {256, 8}, {128, 6}, {128, 8}, {128, 12}, {64, 12}, {64, 16}};
+  const int ns = (int)(sizeof shapes / sizeof shapes[0]);
+  double best = -1.0;
+  int best_i = 0, written = 0;
+  mcpet_run_params p{};
+  p.zero_tallies = 1;
+  p.acquisition_time_ps = (uint64_t)((double)b->ctx[0]->P.acq_ps * fraction);
+  if (p.acquisition_time_ps == 0) p.acquisition_time_ps = 1;
+  for (int i = 0; i < ns; i++) {
+    p.block_size = shapes[i][0];
+    p.blocks_per_cu = shapes[i][1];
+    mcpet_status st = mcpet_batch_run(b, &p);
+    if (st == MCPET_OK) st = mcpet_batch_sync(b);
+    if (st != MCPET_OK) return st;
+    float ms = 0.0f;
+    BHIP(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    unsigned long long decays = 0;
+    for (mcpet_ctx* c : b->ctx) {
+      unsigned long long d = 0;
+      BHIP(b, hipMemcpy(&d, c->d_counters + KC_DECAYS, sizeof d, hipMemcpyDeviceToHost));
+      decays += d;
+    }
+    const double rate = ms > 0.0f ? (double)decays / ms : 0.0;
+    if (results && written < max_results) results[written++] = {shapes[i][0], shapes[i][1], (double)ms, rate};
+    if (rate > best) {
+      best = rate;
+      best_i = i;
+    }
+  }
+  b->tuned_bs = shapes[best_i][0];
+  b->tuned_per_cu = shapes[best_i][1];
+  if (n_results) *n_results = written;
+  return MCPET_OK;
+}
+
+extern "C" void mcpet_batch_destroy(mcpet_batch* b)
+{
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  batch_free(b);
+  delete b;
+}
+
+extern "C" const char* mcpet_batch_last_error(const mcpet_batch* b)
+{
+  if (b && !b->err.empty()) return b->err.c_str();
+  return mcpet_last_error(nullptr);
 }

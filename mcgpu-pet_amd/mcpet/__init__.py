@@ -114,7 +114,11 @@ class RunParams(C.Structure):
     _fields_ = [("kernel", C.c_int32), ("block_size", C.c_int32), ("blocks_per_cu", C.c_int32),
                 ("zero_tallies", C.c_int32), ("seed", C.c_int32), ("acquisition_time_ps", C.c_uint64),
                 ("phase_policy", C.c_int32), ("phase_cost", C.c_int32 * NPHASES),
-                ("count_events", C.c_int32)]
+                ("count_events", C.c_int32), ("use_graph", C.c_int32)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("block_size", C.c_int32), ("blocks_per_cu", C.c_int32), ("ms", C.c_double), ("decays_per_ms", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -177,6 +181,15 @@ def lib():
     L.mcpet_version.restype = C.c_char_p
     L.mcpet_selftest_sort.argtypes = [P(C.c_uint64), C.c_int64, C.c_int32, P(C.c_uint32)]
     L.mcpet_struct_sizes.argtypes = [C.POINTER(C.c_size_t), C.c_int32]
+    L.mcpet_batch_create.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(C.c_void_p)]
+    L.mcpet_batch_run.argtypes = [C.c_void_p, C.c_void_p]
+    L.mcpet_batch_sync.argtypes = [C.c_void_p]
+    L.mcpet_batch_fetch.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
+    L.mcpet_batch_get_stats.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
+    L.mcpet_batch_autotune.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]
+    L.mcpet_batch_destroy.argtypes = [C.c_void_p]
+    L.mcpet_batch_last_error.argtypes = [C.c_void_p]
+    L.mcpet_batch_last_error.restype = C.c_char_p
     L.mcpet_struct_sizes.restype = C.c_int32
     _lib = L
     return L
@@ -319,11 +332,11 @@ class Simulation:
             raise McpetError(st, lib().mcpet_last_error(None).decode(errors="replace"))
 
     def run(self, kernel=KERNEL_AUTO, block_size=0, blocks_per_cu=0, zero_tallies=True, seed=0, acquisition_time_ps=0,
-            phase_policy=0, phase_cost=None, count_events=False):
+            phase_policy=0, phase_cost=None, count_events=False, use_graph=False):
         """Launch the transport (asynchronous). count_events selects the instrumented
         persistent kernel, which fills every event count of stats()."""
-        rp = RunParams(kernel, block_size, blocks_per_cu, 1 if zero_tallies else 0, seed, acquisition_time_ps,
-                       phase_policy, (C.c_int32 * NPHASES)(*(phase_cost or [0] * NPHASES)), 1 if count_events else 0)
+        rp = _run_params(kernel, block_size, blocks_per_cu, zero_tallies, seed, acquisition_time_ps, phase_policy,
+                         phase_cost, count_events, use_graph)
         _check(lib().mcpet_run(self.ctx, C.byref(rp)), self.ctx)
 
     def sync(self):
@@ -340,70 +353,138 @@ class Simulation:
         return d
 
     def fetch(self, sinograms=True, images=True, spectrum=True, psf=True, dose=True, singles=True):
-        cfg = self.cfg
-        nv = self.phantom.nvox
-        out = {}
-        t = Tallies()
-        if sinograms:
-            out["trues"] = np.zeros(cfg.nbins, np.int32)
-            out["scatter"] = np.zeros(cfg.nbins, np.int32)
-            t.trues = out["trues"].ctypes.data_as(C.POINTER(C.c_int32))
-            t.scatter = out["scatter"].ctypes.data_as(C.POINTER(C.c_int32))
-        if images:
-            out["image_trues"] = np.zeros(nv, np.int32)
-            out["image_scatter"] = np.zeros(nv, np.int32)
-            t.image_trues = out["image_trues"].ctypes.data_as(C.POINTER(C.c_int32))
-            t.image_scatter = out["image_scatter"].ctypes.data_as(C.POINTER(C.c_int32))
-        if spectrum and cfg.ne > 0:
-            out["spectrum"] = np.zeros(cfg.ne, np.int32)
-            t.spectrum = out["spectrum"].ctypes.data_as(C.POINTER(C.c_int32))
-        psf_arr = None
-        if psf and cfg.detector.tally_psf_sinogram in (0, 1) and cfg.detector.psf_size > 0:
-            psf_arr = np.zeros(cfg.detector.psf_size, PSF_DTYPE)
-            t.psf = psf_arr.ctypes.data
-            t.psf_capacity = cfg.detector.psf_size
-        if dose:
-            out["material_edep"] = np.zeros((MAX_MATERIALS, 2), np.uint64)
-            t.material_edep = out["material_edep"].ctypes.data_as(C.POINTER(C.c_uint64))
-            if cfg.voxel_dose and cfg.roi_x_max > -1:
-                n_roi = (cfg.roi_x_max - cfg.roi_x_min + 1) * (cfg.roi_y_max - cfg.roi_y_min + 1) * \
-                        (cfg.roi_z_max - cfg.roi_z_min + 1)
-                out["voxel_edep"] = np.zeros((n_roi, 2), np.uint64)
-                t.voxel_edep = out["voxel_edep"].ctypes.data_as(C.POINTER(C.c_uint64))
-        singles_arr = None
-        if cfg.detector.tally_psf_sinogram == TALLY_SINGLES:
-            if sinograms:
-                out["randoms"] = np.zeros(cfg.nbins, np.int32)
-                out["spurious"] = np.zeros(cfg.nbins, np.int32)
-                t.randoms = out["randoms"].ctypes.data_as(C.POINTER(C.c_int32))
-                t.spurious = out["spurious"].ctypes.data_as(C.POINTER(C.c_int32))
-            if singles:
-                singles_arr = np.zeros(cfg.singles_capacity, SINGLE_DTYPE)
-                t.singles = singles_arr.ctypes.data
-                t.singles_buffer = cfg.singles_capacity
-        _check(lib().mcpet_fetch(self.ctx, C.byref(t)), self.ctx)
-        if singles_arr is not None:
-            out["singles"] = singles_arr[:t.singles_count].copy()
-        if cfg.detector.tally_psf_sinogram == TALLY_SINGLES:
-            out["singles_count"] = t.singles_count
-            out["singles_requested"] = t.singles_requested
-            out["coincidences"] = dict(zip(("trues", "scatter", "randoms", "spurious", "multiples"),
-                                           [int(x) for x in t.coincidences]))
-        if psf_arr is not None:
-            out["psf"] = psf_arr[:t.psf_count].copy()
-        out["total_histories"] = t.total_histories
-        out["psf_count"] = t.psf_count
-        out["psf_requested"] = t.psf_requested
-        out["sum_trues"] = t.sum_trues
-        out["sum_scatter"] = t.sum_scatter
-        out["final_seed"] = t.final_seed
-        self._last_tallies = t
-        return out
+        return _fetch_tallies(lambda t: _check(lib().mcpet_fetch(self.ctx, C.byref(t)), self.ctx), self.cfg, self.phantom.nvox,
+                              sinograms, images, spectrum, psf, dose, singles)
 
     def close(self):
         if self.ctx:
             lib().mcpet_destroy(self.ctx)
             self.ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _run_params(kernel=KERNEL_AUTO, block_size=0, blocks_per_cu=0, zero_tallies=True, seed=0, acquisition_time_ps=0,
+                phase_policy=0, phase_cost=None, count_events=False, use_graph=False):
+    return RunParams(kernel, block_size, blocks_per_cu, 1 if zero_tallies else 0, seed, acquisition_time_ps, phase_policy,
+                     (C.c_int32 * NPHASES)(*(phase_cost or [0] * NPHASES)), 1 if count_events else 0,
+                     1 if use_graph else 0)
+
+
+def _fetch_tallies(fetch_fn, cfg, nv, sinograms=True, images=True, spectrum=True, psf=True, dose=True, singles=True):
+    """Allocate the result arrays for cfg, let fetch_fn fill a Tallies struct, return them."""
+    out = {}
+    t = Tallies()
+    if sinograms:
+        out["trues"] = np.zeros(cfg.nbins, np.int32)
+        out["scatter"] = np.zeros(cfg.nbins, np.int32)
+        t.trues = out["trues"].ctypes.data_as(C.POINTER(C.c_int32))
+        t.scatter = out["scatter"].ctypes.data_as(C.POINTER(C.c_int32))
+    if images:
+        out["image_trues"] = np.zeros(nv, np.int32)
+        out["image_scatter"] = np.zeros(nv, np.int32)
+        t.image_trues = out["image_trues"].ctypes.data_as(C.POINTER(C.c_int32))
+        t.image_scatter = out["image_scatter"].ctypes.data_as(C.POINTER(C.c_int32))
+    if spectrum and cfg.ne > 0:
+        out["spectrum"] = np.zeros(cfg.ne, np.int32)
+        t.spectrum = out["spectrum"].ctypes.data_as(C.POINTER(C.c_int32))
+    psf_arr = None
+    if psf and cfg.detector.tally_psf_sinogram in (0, 1) and cfg.detector.psf_size > 0:
+        psf_arr = np.zeros(cfg.detector.psf_size, PSF_DTYPE)
+        t.psf = psf_arr.ctypes.data
+        t.psf_capacity = cfg.detector.psf_size
+    if dose:
+        out["material_edep"] = np.zeros((MAX_MATERIALS, 2), np.uint64)
+        t.material_edep = out["material_edep"].ctypes.data_as(C.POINTER(C.c_uint64))
+        if cfg.voxel_dose and cfg.roi_x_max > -1:
+            n_roi = (cfg.roi_x_max - cfg.roi_x_min + 1) * (cfg.roi_y_max - cfg.roi_y_min + 1) * \
+                    (cfg.roi_z_max - cfg.roi_z_min + 1)
+            out["voxel_edep"] = np.zeros((n_roi, 2), np.uint64)
+            t.voxel_edep = out["voxel_edep"].ctypes.data_as(C.POINTER(C.c_uint64))
+    singles_arr = None
+    if cfg.detector.tally_psf_sinogram == TALLY_SINGLES:
+        if sinograms:
+            out["randoms"] = np.zeros(cfg.nbins, np.int32)
+            out["spurious"] = np.zeros(cfg.nbins, np.int32)
+            t.randoms = out["randoms"].ctypes.data_as(C.POINTER(C.c_int32))
+            t.spurious = out["spurious"].ctypes.data_as(C.POINTER(C.c_int32))
+        if singles:
+            singles_arr = np.zeros(cfg.singles_capacity, SINGLE_DTYPE)
+            t.singles = singles_arr.ctypes.data
+            t.singles_buffer = cfg.singles_capacity
+    fetch_fn(t)
+    if singles_arr is not None:
+        out["singles"] = singles_arr[:t.singles_count].copy()
+    if cfg.detector.tally_psf_sinogram == TALLY_SINGLES:
+        out["singles_count"] = t.singles_count
+        out["singles_requested"] = t.singles_requested
+        out["coincidences"] = dict(zip(("trues", "scatter", "randoms", "spurious", "multiples"),
+                                       [int(x) for x in t.coincidences]))
+    if psf_arr is not None:
+        out["psf"] = psf_arr[:t.psf_count].copy()
+    out["total_histories"] = t.total_histories
+    out["psf_count"] = t.psf_count
+    out["psf_requested"] = t.psf_requested
+    out["sum_trues"] = t.sum_trues
+    out["sum_scatter"] = t.sum_scatter
+    out["final_seed"] = t.final_seed
+    return out
+
+
+class Batch:
+    """n phantoms sharing scanner, tallies and tables, simulated in one persistent launch
+    (mcpet_batch_*). Phantom k's results equal a Simulation of phantoms[k] bit for bit."""
+
+    def __init__(self, cfg, phantoms, tables):
+        self.cfg = cfg
+        self.phantoms = list(phantoms)
+        self.tables = tables
+        arr = (Phantom * len(self.phantoms))(*[p.ph for p in self.phantoms])
+        self.b = C.c_void_p()
+        st = lib().mcpet_batch_create(C.byref(cfg), arr, len(self.phantoms), C.byref(tables.t), C.byref(self.b))
+        if st != 0:
+            raise McpetError(st, lib().mcpet_last_error(None).decode(errors="replace"))
+
+    def _check(self, st):
+        if st != 0:
+            raise McpetError(st, lib().mcpet_batch_last_error(self.b).decode(errors="replace"))
+
+    def run(self, **kw):
+        self._check(lib().mcpet_batch_run(self.b, C.byref(_run_params(**kw))))
+
+    def sync(self):
+        self._check(lib().mcpet_batch_sync(self.b))
+
+    def fetch(self, k, **kw):
+        return _fetch_tallies(lambda t: self._check(lib().mcpet_batch_fetch(self.b, k, C.byref(t))), self.cfg,
+                              self.phantoms[k].nvox, **kw)
+
+    def stats(self, k):
+        s = Stats()
+        self._check(lib().mcpet_batch_get_stats(self.b, k, C.byref(s)))
+        return s.as_dict()
+
+    def autotune(self, fraction=0.05):
+        res = (TuneResult * 32)()
+        n = C.c_int32()
+        self._check(lib().mcpet_batch_autotune(self.b, float(fraction), res, 32, C.byref(n)))
+        return [{"block_size": r.block_size, "blocks_per_cu": r.blocks_per_cu, "ms": r.ms, "decays_per_ms": r.decays_per_ms}
+                for r in res[:n.value]]
+
+    def close(self):
+        if self.b:
+            lib().mcpet_batch_destroy(self.b)
+            self.b = C.c_void_p()
 
     def __enter__(self):
         return self

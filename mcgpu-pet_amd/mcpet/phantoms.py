@@ -207,6 +207,41 @@ def head(seed=20261202, shape=(96, 128, 128), voxel_cm=0.2, n_decays=1.0e9, acqu
     return mat, dens, act, (voxel_cm,) * 3
 
 
+def random_phantoms(count=48, seed=20261202, voxel_cm=0.2, histories=(1.0e6, 1.0e8), xy=(32, 160), z=(32, 96),
+                    acquisition_time_s=10.0, mean_life=F18_MEAN_LIFE):
+    """A batch of varied phantoms (the many-phantom workload): nx, ny ~ U[xy], nz ~ U[z] at
+    voxel_cm, air background with 2-6 random ellipsoids of the head tissues (material slots
+    1..8 as HEAD_TISSUES, each with its density and activity weight), and a photon-history
+    budget log-uniform in `histories` (2 per decay), i.e. proportional to the phantom's total
+    activity. Returns a list of (mat, dens, act, voxel_size)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    frac = mean_life * (1.0 - np.exp(-acquisition_time_s / mean_life))
+    for _ in range(count):
+        nx, ny = (int(v) for v in rng.integers(xy[0], xy[1] + 1, 2))
+        nz = int(rng.integers(z[0], z[1] + 1))
+        shape = (nz, ny, nx)
+        size = np.array([nx, ny, nz], np.float32) * voxel_cm
+        mat = np.ones(shape, np.int32)
+        # a body ellipsoid of soft tissue, then inclusions of other tissues
+        mat[_ellipsoid(shape, voxel_cm, size / 2, size * rng.uniform(0.3, 0.48, 3).astype(np.float32))] = 2
+        for _ in range(int(rng.integers(2, 7))):
+            c = (size * rng.uniform(0.25, 0.75, 3)).astype(np.float32)
+            r = (size * rng.uniform(0.05, 0.2, 3)).astype(np.float32)
+            mat[_ellipsoid(shape, voxel_cm, c, r)] = int(rng.integers(2, 9))
+        dens = np.zeros(shape, np.float32)
+        w = np.zeros(shape, np.float64)
+        for slot, _, _, d, a in HEAD_TISSUES:
+            dens[mat == slot] = d
+            w[mat == slot] = a
+        if w.sum() == 0:
+            w[mat == 2] = 1.0
+        n_decays = 0.5 * float(np.exp(rng.uniform(np.log(histories[0]), np.log(histories[1]))))
+        act = (w * (n_decays / (w.sum() * frac))).astype(np.float32)
+        out.append((mat, dens, act, (voxel_cm,) * 3))
+    return out
+
+
 def head_materials(materials_dir):
     return [os.path.join(materials_dir, f) for (_, _, f, _, _) in HEAD_TISSUES]
 

@@ -1,0 +1,85 @@
+"""Batched persistent kernel (no reference counterpart; the reference runs one phantom per
+process): n phantoms in one launch with a stream queue per phantom. Pinned by equality: each
+phantom's tallies and event counts equal those of its own plain-kernel launch (which equals
+the oracle, see test_gpu_parity), bit for bit, whatever the launch shape, and a captured
+graph replays to the eager result."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import mcpet
+from conftest import MATERIALS
+from mcpet import phantoms
+
+KEYS = ("trues", "scatter", "image_trues", "image_scatter", "spectrum", "material_edep")
+
+
+def _batch_inputs(count=3, seed=5):
+    specs = phantoms.random_phantoms(count, seed=seed, xy=(16, 40), z=(16, 32), histories=(4e4, 4e5))
+    names = phantoms.head_materials(MATERIALS)
+    cfg = phantoms.make_config(seed=31337, threads_per_block=16, acquisition_time_s=10.0, materials=names,
+                               material_dose=True)
+    phs = [mcpet.phantom_from_arrays(*sp) for sp in specs]
+    dmax = np.max([p.density_max() for p in phs], axis=0)      # tables shared by the batch
+    tab = mcpet.load_materials(names, dmax)
+    return cfg, phs, tab
+
+
+def _same(a, b, counts=True):
+    for k in KEYS:
+        assert np.array_equal(a[k], b[k]), k
+    assert a["total_histories"] == b["total_histories"] and a["final_seed"] == b["final_seed"]
+
+
+def test_batch_rejects_bad_arguments_without_a_device():
+    cfg = mcpet.Config()
+    b = C.c_void_p()
+    assert mcpet.lib().mcpet_batch_create(C.byref(cfg), None, 0, None, C.byref(b)) == 1 and not b.value
+
+
+@pytest.mark.gpu
+def test_batch_equals_plain_launch_per_phantom():
+    cfg, phs, tab = _batch_inputs()
+    with mcpet.Batch(cfg, phs, tab) as b:
+        b.run(count_events=True)
+        got = [(b.fetch(k, psf=False), b.stats(k)) for k in range(len(phs))]
+    for k, ph in enumerate(phs):
+        with mcpet.Simulation(cfg, ph, tab) as sim:
+            sim.run(kernel=mcpet.KERNEL_PLAIN)
+            ref = sim.fetch(psf=False)
+            rs = sim.stats()
+        g, gs = got[k]
+        _same(g, ref)
+        for key in ("decays", "streams", "woodcock_steps", "material_changes", "compton", "compton_shell_evals",
+                    "rayleigh", "photoelectric", "pairs_tallied", "coincidences", "rn_draws"):
+            assert gs[key] == rs[key], (k, key, gs[key], rs[key])
+        assert gs["coincidences"] > 0
+
+
+@pytest.mark.gpu
+def test_batch_graph_replays_match_eager_runs():
+    cfg, phs, tab = _batch_inputs(count=3, seed=6)
+    with mcpet.Batch(cfg, phs, tab) as b:
+        b.run(seed=777)
+        eager = [b.fetch(k, psf=False) for k in range(len(phs))]
+        for rep in range(3):
+            b.run(seed=777, use_graph=True)
+            b.sync()
+            for k in range(len(phs)):
+                _same(b.fetch(k, psf=False), eager[k])
+            b.run(seed=778 + rep)        # eager work between the replays
+            b.sync()
+
+
+@pytest.mark.gpu
+def test_batch_autotune_keeps_results():
+    cfg, phs, tab = _batch_inputs(count=4, seed=7)
+    with mcpet.Batch(cfg, phs, tab) as b:
+        b.run(seed=99)
+        before = [b.fetch(k, psf=False) for k in range(len(phs))]
+        res = b.autotune(0.1)
+        assert len(res) >= 4 and all(r["decays_per_ms"] > 0 for r in res)
+        b.run(seed=99)
+        for k in range(len(phs)):
+            _same(b.fetch(k, psf=False), before[k])
