@@ -344,6 +344,48 @@ mcpet_status mcpet_batch_autotune(mcpet_batch* b, double acquisition_fraction, m
 void mcpet_batch_destroy(mcpet_batch* b);
 const char* mcpet_batch_last_error(const mcpet_batch* b);
 
+/* ------------------------------------------------------------------ serving loop
+ * No reference counterpart. A server owns the interaction tables and runs simulation
+ * requests (a config + phantom each, pair tallies) with continuous batching: every
+ * mcpet_server_step admits queued requests in submission order while the device memory of
+ * the admitted ones stays within the HBM budget, then advances every admitted request by
+ * up to `streams_per_step` of its decay streams in ONE batched launch; requests whose
+ * streams are all done are copied to host memory and release their device memory before the
+ * next step. Streams are independent with fixed ids, so a request's tallies equal its
+ * standalone mcpet_run bit for bit, whatever else it was batched with. */
+typedef struct mcpet_server mcpet_server;
+
+typedef enum mcpet_request_state {
+  MCPET_REQ_QUEUED = 0, MCPET_REQ_RUNNING = 1, MCPET_REQ_DONE = 2, MCPET_REQ_REJECTED = 3, MCPET_REQ_UNKNOWN = 4
+} mcpet_request_state;
+
+typedef struct mcpet_server_stats {
+  int64_t submitted, completed, rejected, steps;
+  int32_t queued, running;
+  size_t budget_bytes;
+  size_t peak_admitted_bytes;    /* largest device footprint of the admitted requests        */
+  int32_t peak_running;
+  double latency_p50_ms, latency_p99_ms, latency_max_ms;   /* submit -> done, completed requests */
+  double busy_ms;                /* time spent in steps                                       */
+} mcpet_server_stats;
+
+mcpet_status mcpet_server_create(const mcpet_tables* tab, size_t hbm_budget_bytes, int32_t max_running,
+                                 int64_t streams_per_step, mcpet_server** out);
+/* Queue a request (cfg and phantom are copied; seed 0: the config's). A request whose
+ * device footprint alone exceeds the budget is rejected (MCPET_E_CAPACITY). */
+mcpet_status mcpet_server_submit(mcpet_server* s, const mcpet_config* cfg, const mcpet_phantom* ph, int32_t seed,
+                                 int64_t* request_id);
+/* One scheduling step (admission, one batched launch, completions); blocks until done. */
+mcpet_status mcpet_server_step(mcpet_server* s, int32_t* running, int32_t* completed);
+/* Step until no request is queued or running. */
+mcpet_status mcpet_server_drain(mcpet_server* s);
+mcpet_status mcpet_server_poll(mcpet_server* s, int64_t request_id, int32_t* state);
+/* Results of a finished request (the sizes of its config / phantom); the request is then forgotten. */
+mcpet_status mcpet_server_fetch(mcpet_server* s, int64_t request_id, mcpet_tallies* out);
+mcpet_status mcpet_server_get_stats(mcpet_server* s, mcpet_server_stats* out);
+void mcpet_server_destroy(mcpet_server* s);
+const char* mcpet_server_last_error(const mcpet_server* s);
+
 /* ------------------------------------------------------------------ host-side output (boundary) */
 
 /* Write the reference's output files into directory dir (NULL = cwd):

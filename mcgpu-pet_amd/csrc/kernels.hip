@@ -1157,7 +1157,7 @@ __device__ __forceinline__ void sm_waves(const KParams& P, CN& C, const mcpet_co
     } else if (best == LS_DECAY) {
       for (;;) {
         if (!have) {
-          const long long sid = (long long)atomicAdd(P.queue, 1ull);
+          const long long sid = P.stream_base + (long long)atomicAdd(P.queue, 1ull);
           if (sid >= P.n_streams) {
             state = LS_DONE;
             break;

@@ -46,7 +46,8 @@ struct KParams {
   // decay streams: stream s -> emitting voxel active_vox[s / tpb], lane s % tpb
   const int* active_vox;
   const float* active_act;
-  long long n_streams;
+  long long n_streams;           // streams [stream_base, n_streams) are simulated by a launch
+  long long stream_base;
   int tpb;
   unsigned long long* queue;   // persistent kernel: next stream index
   // source

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "host_internal.h"
+#include "runtime_internal.h"
 #include "kparams.h"
 #include "mcpet.h"
 
@@ -54,6 +55,7 @@ struct mcpet_ctx {
   unsigned long long* d_mat_edep = nullptr;
   unsigned long long* d_counters = nullptr;  // KC_N counters + stream queue
   int* d_final_seed = nullptr;
+  size_t dev_bytes = 0;          // device memory held by the context
   KParams* d_params = nullptr;   // the run's parameter block (device copy)
   KParams* h_params = nullptr;   // pinned staging copy
   // singles pipeline
@@ -115,6 +117,7 @@ mcpet_status dalloc(mcpet_ctx* c, T** p, size_t n)
     *p = nullptr;
     return set_err(c, MCPET_E_NOMEM, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed: " + hipGetErrorString(e));
   }
+  c->dev_bytes += n * sizeof(T);
   return MCPET_OK;
 }
 
@@ -877,6 +880,45 @@ extern "C" mcpet_status mcpet_selftest_copy(size_t nbytes, int32_t iters, double
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   return rc;
+}
+
+// ---------------------------------------------------------------------------------- internals for the server
+
+size_t mcpet_ctx_device_bytes(const mcpet_ctx* c) { return c->dev_bytes; }
+long long mcpet_ctx_streams(const mcpet_ctx* c) { return c->P.n_streams; }
+int mcpet_ctx_device(const mcpet_ctx* c) { return c->device; }
+int mcpet_ctx_num_cus(const mcpet_ctx* c) { return c->num_cus; }
+
+mcpet_status mcpet_run_contexts(const CtxRun* runs, int n, hipStream_t stream, KParams* h_params, KParams* d_params,
+                                int block_size, int blocks, std::string* err)
+{
+  auto fail = [&](mcpet_status st, const std::string& m) {
+    if (err) *err = m;
+    return set_err(nullptr, st, m);
+  };
+  bool ext = false;
+  for (int k = 0; k < n; k++) {
+    mcpet_ctx* c = runs[k].c;
+    KParams P = c->P;
+    if (runs[k].seed != 0) P.seed = runs[k].seed;
+    if (!seed_usable(P.seed)) return fail(MCPET_E_ARG, seed_msg(P.seed));
+    P.stream_base = runs[k].begin;
+    P.n_streams = runs[k].end;
+    P.sm_policy = 0;
+    P.lds_spectrum = 0;
+    P.lds_palette = 0;
+    ext = ext || P.prompt_frac > 0.0f;
+    h_params[k] = P;
+    mcpet_status st = clear_buffers(c, stream, false, runs[k].clear != 0, runs[k].clear != 0);
+    if (st != MCPET_OK) return fail(st, c->err);
+    hipError_t e = hipMemsetAsync(c->d_counters + KC_N, 0, sizeof(unsigned long long), stream);   // stream queue
+    if (e != hipSuccess) return fail(MCPET_E_HIP, std::string("queue reset: ") + hipGetErrorString(e));
+    c->ran = true;
+  }
+  hipError_t e = hipMemcpyAsync(d_params, h_params, sizeof(KParams) * (size_t)n, hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess) e = mcpet_launch_transport_batch(d_params, n, ext, 0, blocks, block_size, stream);
+  if (e != hipSuccess) return fail(MCPET_E_HIP, std::string("batched launch: ") + hipGetErrorString(e));
+  return MCPET_OK;
 }
 
 // ---------------------------------------------------------------------------------- batches

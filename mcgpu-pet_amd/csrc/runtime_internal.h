@@ -1,0 +1,29 @@
+// runtime_internal.h — context internals shared by runtime.cpp and server.cpp.
+#ifndef MCPET_RUNTIME_INTERNAL_H
+#define MCPET_RUNTIME_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "kparams.h"
+#include "mcpet.h"
+
+// One context's share of a batched launch: streams [begin, end) of its phantom.
+struct CtxRun {
+  mcpet_ctx* c;
+  long long begin, end;
+  int clear;       // 1: clear the context's tallies and counters first
+  int seed;        // 0: the context's seed
+};
+
+size_t mcpet_ctx_device_bytes(const mcpet_ctx* c);
+long long mcpet_ctx_streams(const mcpet_ctx* c);
+int mcpet_ctx_device(const mcpet_ctx* c);
+int mcpet_ctx_num_cus(const mcpet_ctx* c);
+// Enqueue one batched launch over runs[0..n-1] on `stream` (h_params: pinned, n entries;
+// d_params: device, n entries). The contexts must share their interaction tables.
+mcpet_status mcpet_run_contexts(const CtxRun* runs, int n, hipStream_t stream, KParams* h_params, KParams* d_params,
+                                int block_size, int blocks, std::string* err);
+
+#endif

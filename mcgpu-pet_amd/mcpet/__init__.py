@@ -190,6 +190,16 @@ def lib():
     L.mcpet_batch_destroy.argtypes = [C.c_void_p]
     L.mcpet_batch_last_error.argtypes = [C.c_void_p]
     L.mcpet_batch_last_error.restype = C.c_char_p
+    L.mcpet_server_create.argtypes = [C.c_void_p, C.c_size_t, C.c_int32, C.c_int64, C.POINTER(C.c_void_p)]
+    L.mcpet_server_submit.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_int64)]
+    L.mcpet_server_step.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    L.mcpet_server_drain.argtypes = [C.c_void_p]
+    L.mcpet_server_poll.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int32)]
+    L.mcpet_server_fetch.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
+    L.mcpet_server_get_stats.argtypes = [C.c_void_p, C.c_void_p]
+    L.mcpet_server_destroy.argtypes = [C.c_void_p]
+    L.mcpet_server_last_error.argtypes = [C.c_void_p]
+    L.mcpet_server_last_error.restype = C.c_char_p
     L.mcpet_struct_sizes.restype = C.c_int32
     _lib = L
     return L
@@ -439,6 +449,82 @@ def _fetch_tallies(fetch_fn, cfg, nv, sinograms=True, images=True, spectrum=True
     out["sum_scatter"] = t.sum_scatter
     out["final_seed"] = t.final_seed
     return out
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("rejected", C.c_int64), ("steps", C.c_int64),
+                ("queued", C.c_int32), ("running", C.c_int32), ("budget_bytes", C.c_size_t),
+                ("peak_admitted_bytes", C.c_size_t), ("peak_running", C.c_int32), ("latency_p50_ms", C.c_double),
+                ("latency_p99_ms", C.c_double), ("latency_max_ms", C.c_double), ("busy_ms", C.c_double)]
+
+
+REQ_STATES = {0: "queued", 1: "running", 2: "done", 3: "rejected", 4: "unknown"}
+
+
+class Server:
+    """Serving loop (mcpet_server_*): continuous batching of simulation requests under an HBM
+    budget; each step advances every admitted request by streams_per_step decay streams in
+    one batched launch. A request's result equals its standalone Simulation bit for bit."""
+
+    def __init__(self, tables, budget_bytes, max_running=32, streams_per_step=1 << 20):
+        self.tables = tables
+        self.s = C.c_void_p()
+        self._meta = {}
+        st = lib().mcpet_server_create(C.byref(tables.t), int(budget_bytes), int(max_running), int(streams_per_step),
+                                       C.byref(self.s))
+        if st != 0:
+            raise McpetError(st, lib().mcpet_last_error(None).decode(errors="replace"))
+
+    def _check(self, st):
+        if st != 0:
+            raise McpetError(st, lib().mcpet_server_last_error(self.s).decode(errors="replace"))
+
+    def submit(self, cfg, phantom, seed=0):
+        rid = C.c_int64()
+        st = lib().mcpet_server_submit(self.s, C.byref(cfg), C.byref(phantom.ph), int(seed), C.byref(rid))
+        self._meta[rid.value] = (cfg, phantom.nvox)
+        self._check(st)
+        return rid.value
+
+    def step(self):
+        r, c = C.c_int32(), C.c_int32()
+        self._check(lib().mcpet_server_step(self.s, C.byref(r), C.byref(c)))
+        return r.value, c.value
+
+    def drain(self):
+        self._check(lib().mcpet_server_drain(self.s))
+
+    def poll(self, rid):
+        v = C.c_int32()
+        self._check(lib().mcpet_server_poll(self.s, int(rid), C.byref(v)))
+        return REQ_STATES[v.value]
+
+    def fetch(self, rid, **kw):
+        cfg, nv = self._meta.pop(rid)
+        return _fetch_tallies(lambda t: self._check(lib().mcpet_server_fetch(self.s, int(rid), C.byref(t))), cfg, nv,
+                              singles=False, **kw)
+
+    def stats(self):
+        x = ServerStats()
+        self._check(lib().mcpet_server_get_stats(self.s, C.byref(x)))
+        return {k: getattr(x, k) for k, _ in x._fields_}
+
+    def close(self):
+        if self.s:
+            lib().mcpet_server_destroy(self.s)
+            self.s = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Batch:
