@@ -337,8 +337,9 @@ mcpet_status mcpet_batch_sync(mcpet_batch* b);
 mcpet_status mcpet_batch_fetch(mcpet_batch* b, int32_t k, mcpet_tallies* out);
 /* Counts of phantom k; kernel_ms is the whole batch launch. */
 mcpet_status mcpet_batch_get_stats(mcpet_batch* b, int32_t k, mcpet_stats* out);
-/* Time the batch at a fraction of the acquisition for each launch shape (block size x
- * workgroups per CU), keep the fastest for later runs; writes up to max_results timings. */
+/* Time the batch on the first `fraction` of every phantom's decay streams (full acquisition
+ * per stream) for each launch shape (block size x workgroups per CU), keep the fastest for
+ * later runs; writes up to max_results timings. */
 mcpet_status mcpet_batch_autotune(mcpet_batch* b, double acquisition_fraction, mcpet_tune_result* results,
                                   int32_t max_results, int32_t* n_results);
 void mcpet_batch_destroy(mcpet_batch* b);

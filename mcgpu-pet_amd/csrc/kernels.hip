@@ -42,10 +42,10 @@
 
 // Occupancy target of the transport kernels (waves per SIMD; 0 leaves it to the compiler).
 // The persistent kernel is bound by the latency of dependent gathers and transcendental
-// chains; 3 waves per SIMD (168 VGPRs, a few dwords of spill) hides enough of it to beat
-// the compiler's own choice (2 waves at ~180 VGPRs) by 1.35x on the head workload.
+// chains; 4 waves per SIMD (128 VGPRs, ~10 dwords of spill) hides more of it than 3 (+17%
+// on the head workload) or 5 (spills grow), and than the compiler's own 2.
 #ifndef MCPET_WAVES_PER_EU
-#define MCPET_WAVES_PER_EU 3
+#define MCPET_WAVES_PER_EU 4
 #endif
 #if MCPET_WAVES_PER_EU > 0
 #define K_OCCUPANCY __attribute__((amdgpu_waves_per_eu(MCPET_WAVES_PER_EU, 8)))

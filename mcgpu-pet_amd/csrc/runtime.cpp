@@ -1006,7 +1006,13 @@ extern "C" mcpet_status mcpet_batch_create(const mcpet_config* cfg, const mcpet_
   return MCPET_OK;
 }
 
-extern "C" mcpet_status mcpet_batch_run(mcpet_batch* b, const mcpet_run_params* rp)
+static mcpet_status batch_run(mcpet_batch* b, const mcpet_run_params* rp, double stream_fraction);
+
+extern "C" mcpet_status mcpet_batch_run(mcpet_batch* b, const mcpet_run_params* rp) { return batch_run(b, rp, 1.0); }
+
+// stream_fraction < 1: simulate only the first streams of every phantom (the autotuner's
+// sample: full acquisition per stream, so the per-stream work is representative).
+static mcpet_status batch_run(mcpet_batch* b, const mcpet_run_params* rp, double stream_fraction)
 {
   if (!b) return set_err(nullptr, MCPET_E_ARG, "mcpet_batch_run: null batch");
   mcpet_run_params p{};
@@ -1021,6 +1027,7 @@ extern "C" mcpet_status mcpet_batch_run(mcpet_batch* b, const mcpet_run_params* 
     if (p.seed != 0) P.seed = p.seed;
     if (!seed_usable(P.seed)) return bset_err(b, MCPET_E_ARG, seed_msg(P.seed));
     if (p.acquisition_time_ps != 0) P.acq_ps = p.acquisition_time_ps;
+    if (stream_fraction < 1.0) P.n_streams = (long long)std::ceil((double)P.n_streams * stream_fraction);
     P.sm_policy = (p.phase_policy == 2) ? 1 : 0;
     static const int default_cost[MCPET_NPHASES] = {4, 6, 1, 3, 6, 3, 2, 4, 4, 1};
     for (int i = 0; i < MCPET_NPHASES; i++) P.sm_cost[i] = p.phase_cost[i] > 0 ? p.phase_cost[i] : default_cost[i];
@@ -1115,12 +1122,10 @@ extern "C" mcpet_status mcpet_batch_autotune(mcpet_batch* b, double fraction, mc
   int best_i = 0, written = 0;
   mcpet_run_params p{};
   p.zero_tallies = 1;
-  p.acquisition_time_ps = (uint64_t)((double)b->ctx[0]->P.acq_ps * fraction);
-  if (p.acquisition_time_ps == 0) p.acquisition_time_ps = 1;
   for (int i = 0; i < ns; i++) {
     p.block_size = shapes[i][0];
     p.blocks_per_cu = shapes[i][1];
-    mcpet_status st = mcpet_batch_run(b, &p);
+    mcpet_status st = batch_run(b, &p, fraction);
     if (st == MCPET_OK) st = mcpet_batch_sync(b);
     if (st != MCPET_OK) return st;
     float ms = 0.0f;
