@@ -387,6 +387,47 @@ mcpet_status mcpet_server_get_stats(mcpet_server* s, mcpet_server_stats* out);
 void mcpet_server_destroy(mcpet_server* s);
 const char* mcpet_server_last_error(const mcpet_server* s);
 
+/* ------------------------------------------------------------------ scatter surrogate (no reference counterpart)
+ *
+ * The rebinned scatter grid: sinogram izm of segment iseg holds the ring pairs with ring sum
+ * s = iz1 + iz2 = izm - ofseg(iseg) + ofz(incl) (the michelogram offsets of K:1749-1788), and
+ * single-slice rebinning puts it at plane s (0 .. 2 NROWS - 2). A D x H x W grid sums planes in
+ * groups of ceil(NZS / D), angles in groups of ceil(NANGLES / H) and radial bins in groups of
+ * ceil(NRAD / W). */
+
+/* plane_of_sinogram[izm] = s for every sinogram of cfg's michelogram (cfg->nsinos entries). */
+mcpet_status mcpet_sinogram_planes(const mcpet_config* cfg, int32_t* plane_of_sinogram);
+
+/* Rebin tally `which` (0 trues, 1 scatter, 2 randoms, 3 spurious) of the last run on the device
+ * into the D x H x W grid (out: host, D*H*W floats of counts). Waits for the run. */
+mcpet_status mcpet_rebin_sinogram(mcpet_ctx* c, int32_t which, int32_t D, int32_t H, int32_t W, float* out);
+mcpet_status mcpet_batch_rebin_sinogram(mcpet_batch* b, int32_t k, int32_t which, int32_t D, int32_t H, int32_t W, float* out);
+
+/* A 3D U-Net on the rebinned grid (D, H, W multiples of 4): in_channels -> width (two 3^3
+ * convolutions) -> stride-2 convolution to 2 width -> convolution -> stride-2 to 4 width ->
+ * convolution -> nearest upsampling, concatenation with the 2-width skip -> convolution to
+ * 2 width -> upsampling, concatenation with the width skip -> convolution to width -> 1^3
+ * convolution; ReLU after every 3^3 convolution; the output is input channel 0 plus that
+ * residual. Every convolution runs as an implicit GEMM on BF16 MFMA tiles with FP32
+ * accumulation; parameters, activations and optimiser state are FP32. Parameter layout: per
+ * layer W[Cout][Cin][k^3] then b[Cout], layers in the order above. */
+typedef struct mcpet_surrogate mcpet_surrogate;
+mcpet_status mcpet_surrogate_create(int32_t D, int32_t H, int32_t W, int32_t in_channels, int32_t width, uint64_t seed,
+                                    mcpet_surrogate** out);
+int64_t mcpet_surrogate_param_count(const mcpet_surrogate* s);
+mcpet_status mcpet_surrogate_get_params(mcpet_surrogate* s, float* out);
+mcpet_status mcpet_surrogate_set_params(mcpet_surrogate* s, const float* in);
+/* x: batch x in_channels x D x H x W (host); y: batch x D x H x W (host). gflop: forward GFLOP per sample. */
+mcpet_status mcpet_surrogate_forward(mcpet_surrogate* s, const float* x, int32_t batch, float* y, double* gflop);
+/* loss = mean over batch and voxels of (y - target)^2; grad (host, param_count floats; may be NULL). */
+mcpet_status mcpet_surrogate_gradients(mcpet_surrogate* s, const float* x, const float* target, int32_t batch, float* loss,
+                                       float* grad);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on that loss; loss is the value before the step. */
+mcpet_status mcpet_surrogate_train_step(mcpet_surrogate* s, const float* x, const float* target, int32_t batch, float lr,
+                                        float* loss);
+void mcpet_surrogate_destroy(mcpet_surrogate* s);
+const char* mcpet_surrogate_last_error(const mcpet_surrogate* s);
+
 /* ------------------------------------------------------------------ host-side output (boundary) */
 
 /* Write the reference's output files into directory dir (NULL = cwd):
@@ -419,6 +460,10 @@ mcpet_status mcpet_selftest_fmath(const double* x, const double* y, int32_t n, d
 /* Stable device radix sort of n 64-bit keys by their low 8*passes bits (passes 1..8), the
  * sort of the singles pipeline; perm receives the sorted order of the input indices. */
 mcpet_status mcpet_selftest_sort(const uint64_t* keys, int64_t n, int32_t passes, uint32_t* perm);
+
+/* C = A (MxK) x B (KxN), row-major FP32 in and out, computed by the surrogate's BF16 MFMA GEMM
+ * kernel (operands rounded to BF16, FP32 accumulation; split > 1: K split over workgroups). */
+mcpet_status mcpet_selftest_gemm_bf16(const float* a, const float* b, int32_t M, int32_t N, int32_t K, int32_t split, float* c);
 
 /* Device copy of nbytes (for HBM-counter calibration: FETCH_SIZE/WRITE_SIZE of a known copy). */
 mcpet_status mcpet_selftest_copy(size_t nbytes, int32_t iters, double* ms_out);

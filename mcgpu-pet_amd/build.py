@@ -56,6 +56,7 @@ def build_lib(jobs):
         ([HIPCC, "-x", "hip", *HIP_FLAGS, "-c", os.path.join(CSRC, "coinc.hip"), "-o", os.path.join(BUILD, "coinc.o")]),
         ([HIPCC, "-x", "hip", *HIP_FLAGS, "-c", os.path.join(CSRC, "runtime.cpp"), "-o", os.path.join(BUILD, "runtime.o")]),
         ([HIPCC, "-x", "hip", *HIP_FLAGS, "-c", os.path.join(CSRC, "server.cpp"), "-o", os.path.join(BUILD, "server.o")]),
+        ([HIPCC, "-x", "hip", *HIP_FLAGS, "-c", os.path.join(CSRC, "unet.hip"), "-o", os.path.join(BUILD, "unet.o")]),
         ([CXX, *HOST_FLAGS, "-c", os.path.join(CSRC, "host_io.cpp"), "-o", os.path.join(BUILD, "host_io.o")]),
     ]
     with ThreadPoolExecutor(max_workers=jobs) as ex:
@@ -64,7 +65,7 @@ def build_lib(jobs):
     tmp = so + ".tmp"
     run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp,
          os.path.join(BUILD, "kernels.o"), os.path.join(BUILD, "coinc.o"), os.path.join(BUILD, "runtime.o"),
-         os.path.join(BUILD, "server.o"),
+         os.path.join(BUILD, "server.o"), os.path.join(BUILD, "unet.o"),
          os.path.join(BUILD, "host_io.o"), "-lz"])
     os.replace(tmp, so)
     cli = os.path.join(BIN, "mcgpu_pet")

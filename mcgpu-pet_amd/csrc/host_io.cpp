@@ -11,6 +11,7 @@
 #include <zlib.h>
 #include <sys/time.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -121,6 +122,32 @@ extern "C" void mcpet_sinogram_sizes(mcpet_config* c)
   c->nsinos = nsinos;
   c->nbins = c->nrad * c->nangles * c->nsinos;
   c->nvoxs = c->res * c->res * c->nzs;
+}
+
+// Ring sum of every sinogram: segment iseg starts at ofseg(iseg) and its first sinogram holds
+// ring sum ofz(incl), incl = (iseg + 1) / 2 (the offsets of the binning, K:1749-1788).
+extern "C" mcpet_status mcpet_sinogram_planes(const mcpet_config* cfg, int32_t* plane)
+{
+  if (!cfg || !plane || cfg->span < 1 || cfg->nseg < 1 || cfg->nsinos < 1) {
+    mcpet_host_set_error("mcpet_sinogram_planes: bad arguments (config sizes not derived?)");
+    return MCPET_E_ARG;
+  }
+  const int span = cfg->span;
+  auto ofseg = [&](int iseg) {
+    int o = iseg * cfg->nzs;
+    for (int kka = 2; kka <= iseg; kka++) {
+      o -= span + 1;
+      if (kka > 3) o -= ((kka - 2) / 2) * 2 * span;
+    }
+    return o;
+  };
+  for (int iseg = 0; iseg < cfg->nseg; iseg++) {
+    const int incl = (iseg + 1) / 2;
+    const int ofz = incl > 0 ? (span + 1) / 2 + (incl - 1) * span : 0;
+    const int b = ofseg(iseg), e = std::min(ofseg(iseg + 1), cfg->nsinos);
+    for (int izm = b; izm < e; izm++) plane[izm] = izm - b + ofz;
+  }
+  return MCPET_OK;
 }
 
 extern "C" mcpet_status mcpet_read_input(const char* path, mcpet_config* c, char* err, size_t errlen)

@@ -131,4 +131,9 @@ hipError_t mcpet_radix_sort(long long n, int passes, unsigned long long* keys[2]
 hipError_t mcpet_launch_pairing(const KParams& P, const unsigned long long* keys, const unsigned* perm, long long n, hipStream_t st);
 hipError_t mcpet_launch_gather_singles(const mcpet_single* s, const unsigned* perm, long long n, mcpet_single* out, hipStream_t st);
 
+// Scatter surrogate (unet.hip): sum sinogram counts into a D x H x W grid (plane groups listed
+// in CSR form grp_off[D + 1] / grp_sino, angle groups of gh rows, radial groups of gw bins).
+hipError_t mcpet_launch_rebin(const int* sino, const int* grp_off, const int* grp_sino, int nangles, int nrad, int D, int H, int W,
+                              int gh, int gw, float* out, hipStream_t st);
+
 #endif

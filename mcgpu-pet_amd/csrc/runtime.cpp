@@ -76,6 +76,11 @@ struct mcpet_ctx {
   bool ran = false;
   bool timed = false;
   double zero_ms = 0.0;
+  // rebinning onto the surrogate's grid (allocated at the first rebin)
+  int* d_rb_off = nullptr;
+  int* d_rb_sino = nullptr;
+  float* d_rb_out = nullptr;
+  int rb_dims[3] = {0, 0, 0};
   std::string err;
 };
 
@@ -127,7 +132,7 @@ void free_all(mcpet_ctx* c)
                   c->d_trues, c->d_scatter, c->d_img_t, c->d_img_sc, c->d_spectrum, c->d_psf, c->d_psf_index,
                   c->d_voxel_edep, c->d_mat_edep, c->d_counters, c->d_final_seed, c->d_singles, c->d_singles_n,
                   c->d_keys[0], c->d_keys[1], c->d_vals[0], c->d_vals[1], c->d_hist, c->d_randoms, c->d_spurious,
-                  c->d_coinc};
+                  c->d_coinc, c->d_rb_off, c->d_rb_sino, c->d_rb_out};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (unsigned* p : c->d_scan)
@@ -741,6 +746,55 @@ extern "C" mcpet_status mcpet_get_detector(const mcpet_ctx* c, mcpet_detector* o
   return MCPET_OK;
 }
 
+// Single-slice rebinning of a sinogram tally onto a D x H x W grid (plane groups of
+// ceil(NZS / D) ring sums, angle groups of ceil(NANGLES / H), radial groups of ceil(NRAD / W)).
+static mcpet_status rebin(mcpet_ctx* c, int which, int D, int H, int W, float* out)
+{
+  if (!c || !out || D < 1 || H < 1 || W < 1 || W > 256) return set_err(c, MCPET_E_ARG, "rebin: bad arguments (W <= 256)");
+  if (!c->ran) return set_err(c, MCPET_E_STATE, "rebin: no run");
+  const int* src = which == 0 ? c->d_trues : which == 1 ? c->d_scatter : which == 2 ? c->d_randoms : which == 3 ? c->d_spurious : nullptr;
+  if (which < 0 || which > 3) return set_err(c, MCPET_E_ARG, "rebin: which must be 0 (trues), 1 (scatter), 2 (randoms), 3 (spurious)");
+  if (!src) return set_err(c, MCPET_E_STATE, "rebin: this tally is not kept in this mode");
+  const mcpet_config& cfg = c->cfg;
+  const int gz = (cfg.nzs + D - 1) / D, gh = (cfg.nangles + H - 1) / H, gw = (cfg.nrad + W - 1) / W;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (c->rb_dims[0] != D || c->rb_dims[1] != H || c->rb_dims[2] != W) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<int> plane(cfg.nsinos);
+    if (mcpet_sinogram_planes(&cfg, plane.data()) != MCPET_OK) return set_err(c, MCPET_E_ARG, mcpet_host_last_error());
+    std::vector<int> off(D + 1, 0), list;
+    for (int d = 0; d < D; d++) {
+      for (int izm = 0; izm < cfg.nsinos; izm++)
+        if (plane[izm] / gz == d) list.push_back(izm);
+      off[d + 1] = (int)list.size();
+    }
+    for (void* p : {(void*)c->d_rb_off, (void*)c->d_rb_sino, (void*)c->d_rb_out})
+      if (p) (void)hipFree(p);
+    c->d_rb_off = c->d_rb_sino = nullptr;
+    c->d_rb_out = nullptr;
+    c->rb_dims[0] = c->rb_dims[1] = c->rb_dims[2] = 0;
+    mcpet_status st = dalloc(c, &c->d_rb_off, (size_t)D + 1);
+    if (st == MCPET_OK) st = dalloc(c, &c->d_rb_sino, std::max<size_t>(1, list.size()));
+    if (st == MCPET_OK) st = dalloc(c, &c->d_rb_out, (size_t)D * H * W);
+    if (st != MCPET_OK) return st;
+    HIPCHK(c, hipMemcpy(c->d_rb_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
+    if (!list.empty()) HIPCHK(c, hipMemcpy(c->d_rb_sino, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice));
+    c->rb_dims[0] = D;
+    c->rb_dims[1] = H;
+    c->rb_dims[2] = W;
+  }
+  hipError_t e = mcpet_launch_rebin(src, c->d_rb_off, c->d_rb_sino, cfg.nangles, cfg.nrad, D, H, W, gh, gw, c->d_rb_out, c->stream);
+  if (e != hipSuccess) return set_err(c, MCPET_E_HIP, std::string("rebin launch failed: ") + hipGetErrorString(e));
+  HIPCHK(c, hipMemcpyAsync(out, c->d_rb_out, sizeof(float) * (size_t)D * H * W, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MCPET_OK;
+}
+
+extern "C" mcpet_status mcpet_rebin_sinogram(mcpet_ctx* c, int32_t which, int32_t D, int32_t H, int32_t W, float* out)
+{
+  return rebin(c, which, D, H, W, out);
+}
+
 extern "C" void mcpet_destroy(mcpet_ctx* c)
 {
   if (!c) return;
@@ -1095,6 +1149,17 @@ extern "C" mcpet_status mcpet_batch_fetch(mcpet_batch* b, int32_t k, mcpet_talli
   mcpet_status st = mcpet_batch_sync(b);
   if (st != MCPET_OK) return st;
   st = mcpet_fetch(b->ctx[k], out);
+  if (st != MCPET_OK) b->err = b->ctx[k]->err;
+  return st;
+}
+
+extern "C" mcpet_status mcpet_batch_rebin_sinogram(mcpet_batch* b, int32_t k, int32_t which, int32_t D, int32_t H, int32_t W,
+                                                   float* out)
+{
+  if (!b || k < 0 || k >= (int32_t)b->ctx.size()) return bset_err(b, MCPET_E_ARG, "mcpet_batch_rebin_sinogram: bad phantom index");
+  if (!b->ran) return bset_err(b, MCPET_E_STATE, "mcpet_batch_rebin_sinogram: no run");
+  mcpet_status st = mcpet_batch_sync(b);
+  if (st == MCPET_OK) st = rebin(b->ctx[k], which, D, H, W, out);
   if (st != MCPET_OK) b->err = b->ctx[k]->err;
   return st;
 }

@@ -1,0 +1,740 @@
+// unet.hip — learned scatter surrogate: a small 3D U-Net on the rebinned scatter sinogram,
+// trained on simulator output (no reference counterpart; the reference only runs Monte Carlo).
+//
+// Every convolution (forward, input gradient, weight gradient) is one implicit GEMM
+//   C[m][n] (+)= sum_k A(m, k) * B(k, n)
+// on BF16 MFMA tiles (v_mfma_f32_16x16x32_bf16: 16x16 outputs, K = 32 per instruction, FP32
+// accumulation). Operands are gathered from FP32 tensors by small functors and rounded to
+// BF16 on the fly (im2col is never materialised); the weights and activations stay FP32.
+//   forward      : A = W[co][ci*T + t],        B = X[ci][p shifted by tap t]      (m = co, n = p)
+//   input grad   : A = W[co][ci*T + t]^T,      B = dY[co][q with s q + d - 1 = p] (m = ci, n = p)
+//   weight grad  : A = dY[co][q],              B = X[ci][s q + d - 1]^T           (m = co, n = ci*T+t, K = voxels, split)
+// MFMA lane layout (16x16x32 BF16): lane l holds A row (l % 16), k = 8 (l / 16) .. +7; B column
+// (l % 16), the same k; the accumulator's 4 values are rows 4 (l / 16) .. +3 of column (l % 16).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "host_internal.h"
+#include "mcpet.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+__device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }   // round to nearest even
+
+// ------------------------------------------------------------------ operand functors
+
+struct Dense {      // row-major matrix, element (r, c) at p[r * ld + c]
+  const float* p;
+  int rows, cols, ld;
+  __device__ __forceinline__ float operator()(int r, int c) const { return (r < rows && c < cols) ? p[(size_t)r * ld + c] : 0.0f; }
+};
+struct DenseT {     // transposed view: (r, c) = p[c * ld + r]
+  const float* p;
+  int rows, cols, ld;
+  __device__ __forceinline__ float operator()(int r, int c) const { return (r < rows && c < cols) ? p[(size_t)c * ld + r] : 0.0f; }
+};
+
+struct Geom {       // a 3D activation volume and a 3x3x3 (ks = 3) or 1x1x1 (ks = 1) convolution
+  int C, D, H, W;           // channels and size of the tensor indexed by the functor
+  int oD, oH, oW;           // size of the convolution output
+  int ks, stride;
+};
+
+__device__ __forceinline__ void tap_offsets(int t, int ks, int& dz, int& dy, int& dx)
+{
+  if (ks == 1) {
+    dz = dy = dx = 0;
+    return;
+  }
+  dz = t / 9 - 1;
+  dy = (t / 3) % 3 - 1;
+  dx = t % 3 - 1;
+}
+
+// forward B: (k = ci*T + t, n = output voxel) -> X[ci][s*z + dz][s*y + dy][s*x + dx]
+struct Im2col {
+  const float* x;
+  Geom g;
+  int K, N;
+  __device__ __forceinline__ float operator()(int k, int n) const
+  {
+    if (k >= K || n >= N) return 0.0f;
+    const int T = g.ks * g.ks * g.ks;
+    const int ci = k / T, t = k - ci * T;
+    int dz, dy, dx;
+    tap_offsets(t, g.ks, dz, dy, dx);
+    const int ox = n % g.oW, oy = (n / g.oW) % g.oH, oz = n / (g.oW * g.oH);
+    const int z = oz * g.stride + dz, y = oy * g.stride + dy, xx = ox * g.stride + dx;
+    if (z < 0 || y < 0 || xx < 0 || z >= g.D || y >= g.H || xx >= g.W) return 0.0f;
+    return x[(((size_t)ci * g.D + z) * g.H + y) * g.W + xx];
+  }
+};
+
+// input-gradient B: (k = co*T + t, n = input voxel p) -> dY[co][q] where s*q + d = p
+struct Col2im {
+  const float* dy;   // [Cout][oD][oH][oW]
+  Geom g;            // D, H, W: input size; oD.. output size
+  int K, N;
+  __device__ __forceinline__ float operator()(int k, int n) const
+  {
+    if (k >= K || n >= N) return 0.0f;
+    const int T = g.ks * g.ks * g.ks;
+    const int co = k / T, t = k - co * T;
+    int dz, dy_, dx;
+    tap_offsets(t, g.ks, dz, dy_, dx);
+    const int px = n % g.W, py = (n / g.W) % g.H, pz = n / (g.W * g.H);
+    const int nz = pz - dz, ny = py - dy_, nx = px - dx;
+    if (nz < 0 || ny < 0 || nx < 0) return 0.0f;
+    if (nz % g.stride || ny % g.stride || nx % g.stride) return 0.0f;
+    const int qz = nz / g.stride, qy = ny / g.stride, qx = nx / g.stride;
+    if (qz >= g.oD || qy >= g.oH || qx >= g.oW) return 0.0f;
+    return dy[(((size_t)co * g.oD + qz) * g.oH + qy) * g.oW + qx];
+  }
+};
+
+// weight-gradient B: (k = output voxel q, n = ci*T + t) -> X[ci][s*q + d]
+struct Im2colT {
+  const float* x;
+  Geom g;
+  int K, N;
+  __device__ __forceinline__ float operator()(int k, int n) const
+  {
+    if (k >= K || n >= N) return 0.0f;
+    const int T = g.ks * g.ks * g.ks;
+    const int ci = n / T, t = n - ci * T;
+    int dz, dy, dx;
+    tap_offsets(t, g.ks, dz, dy, dx);
+    const int ox = k % g.oW, oy = (k / g.oW) % g.oH, oz = k / (g.oW * g.oH);
+    const int z = oz * g.stride + dz, y = oy * g.stride + dy, xx = ox * g.stride + dx;
+    if (z < 0 || y < 0 || xx < 0 || z >= g.D || y >= g.H || xx >= g.W) return 0.0f;
+    return x[(((size_t)ci * g.D + z) * g.H + y) * g.W + xx];
+  }
+};
+
+// ------------------------------------------------------------------ the GEMM
+
+// Block: 4 waves; block tile 32 (m) x 64 (n); wave w owns columns [16 w, 16 w + 16) and both
+// 16-row halves. Grid: (ceil(N/64), ceil(M/32), split). Split > 1 accumulates partial sums
+// over K ranges with atomics (the weight gradient, whose K is the voxel count).
+template <class AOp, class BOp>
+__global__ __launch_bounds__(256) void k_gemm(AOp A, BOp B, float* __restrict__ C, int M, int N, int K, int ldc,
+                                              const float* __restrict__ bias, int k_per_split, int accumulate)
+{
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = lane & 15, kg = lane >> 4;
+  const int m0 = blockIdx.y * 32, n0 = blockIdx.x * 64 + wave * 16;
+  const int kb = blockIdx.z * k_per_split;
+  const int ke = min(K, kb + k_per_split);
+  floatx4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+  for (int k0 = kb; k0 < ke; k0 += 32) {
+    bf16x8 a0, a1, b;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int k = k0 + kg * 8 + j;
+      const bool in = k < ke;
+      a0[j] = to_bf16(in ? A(m0 + r, k) : 0.0f);
+      a1[j] = to_bf16(in ? A(m0 + 16 + r, k) : 0.0f);
+      b[j] = to_bf16(in ? B(k, n0 + r) : 0.0f);
+    }
+    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b, acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b, acc1, 0, 0, 0);
+  }
+  const int col = n0 + r;
+  if (col >= N) return;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int row0 = m0 + kg * 4 + i, row1 = row0 + 16;
+    if (row0 < M) {
+      float v = acc0[i];
+      if (accumulate) atomicAdd(&C[(size_t)row0 * ldc + col], v);
+      else C[(size_t)row0 * ldc + col] = v + (bias ? bias[row0] : 0.0f);
+    }
+    if (row1 < M) {
+      float v = acc1[i];
+      if (accumulate) atomicAdd(&C[(size_t)row1 * ldc + col], v);
+      else C[(size_t)row1 * ldc + col] = v + (bias ? bias[row1] : 0.0f);
+    }
+  }
+}
+
+// accumulate: add into C with atomics (C holds earlier partial sums); else store (+ bias).
+template <class AOp, class BOp>
+hipError_t gemm(AOp A, BOp B, float* C, int M, int N, int K, int ldc, const float* bias, int split, bool accumulate,
+                hipStream_t st)
+{
+  if (M <= 0 || N <= 0) return hipSuccess;
+  if (split < 1) split = 1;
+  if (split > 1) accumulate = true;
+  int kps = (K + split - 1) / split;
+  kps = (kps + 31) / 32 * 32;
+  split = (K + kps - 1) / kps;
+  if (split < 1) split = 1;
+  dim3 grid((N + 63) / 64, (M + 31) / 32, split);
+  hipLaunchKernelGGL((k_gemm<AOp, BOp>), grid, dim3(256), 0, st, A, B, C, M, N, K, ldc, bias, kps, accumulate ? 1 : 0);
+  return hipGetLastError();
+}
+
+// input-gradient A: (m = ci, k = co*T + t) -> W[co][ci*T + t]
+struct WeightT {
+  const float* w;
+  int Cin, Cout, T;
+  __device__ __forceinline__ float operator()(int m, int k) const
+  {
+    if (m >= Cin || k >= Cout * T) return 0.0f;
+    const int co = k / T, t = k - co * T;
+    return w[(size_t)co * Cin * T + (size_t)m * T + t];
+  }
+};
+
+// ------------------------------------------------------------------ elementwise kernels
+
+__global__ void k_relu(float* __restrict__ x, size_t n)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && !(x[i] > 0.0f)) x[i] = 0.0f;
+}
+// g *= (y > 0): ReLU backward from its output
+__global__ void k_relu_bwd(float* __restrict__ g, const float* __restrict__ y, size_t n)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && !(y[i] > 0.0f)) g[i] = 0.0f;
+}
+// nearest x2 upsampling of [C][d][h][w] into [C][2d][2h][2w]
+__global__ void k_upsample(const float* __restrict__ x, float* __restrict__ y, int C, int d, int h, int w)
+{
+  const size_t n = (size_t)C * 8 * d * h * w;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int W2 = 2 * w, H2 = 2 * h, D2 = 2 * d;
+  const int xx = (int)(i % W2), yy = (int)((i / W2) % H2), zz = (int)((i / ((size_t)W2 * H2)) % D2);
+  const int c = (int)(i / ((size_t)W2 * H2 * D2));
+  y[i] = x[(((size_t)c * d + zz / 2) * h + yy / 2) * w + xx / 2];
+}
+__global__ void k_upsample_bwd(const float* __restrict__ gy, float* __restrict__ gx, int C, int d, int h, int w)
+{
+  const size_t n = (size_t)C * d * h * w;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int xx = (int)(i % w), yy = (int)((i / w) % h), zz = (int)((i / ((size_t)w * h)) % d);
+  const int c = (int)(i / ((size_t)w * h * d));
+  const int W2 = 2 * w, H2 = 2 * h, D2 = 2 * d;
+  float s = 0.0f;
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++)
+      for (int e = 0; e < 2; e++) s += gy[(((size_t)c * D2 + 2 * zz + a) * H2 + 2 * yy + b) * W2 + 2 * xx + e];
+  gx[i] = s;
+}
+__global__ void k_add(float* __restrict__ y, const float* __restrict__ x, size_t n)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += x[i];
+}
+// per-channel sum (bias gradient): one workgroup per channel
+__global__ void k_channel_sum(const float* __restrict__ g, float* __restrict__ out, int vox)
+{
+  __shared__ float sh[256];
+  const float* p = g + (size_t)blockIdx.x * vox;
+  float s = 0.0f;
+  for (int i = threadIdx.x; i < vox; i += blockDim.x) s += p[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] += sh[0];
+}
+// out = x0 + r (residual output) and, with a target, the loss gradient and squared error
+__global__ void k_output(const float* __restrict__ x0, const float* __restrict__ r, float* __restrict__ out,
+                         const float* __restrict__ target, float* __restrict__ g, float scale, float* __restrict__ sse, int n)
+{
+  __shared__ float sh[256];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  float e2 = 0.0f;
+  if (i < n) {
+    const float o = x0[i] + r[i];
+    out[i] = o;
+    if (target) {
+      const float d = o - target[i];
+      g[i] = 2.0f * d * scale;
+      e2 = d * d;
+    }
+  }
+  sh[threadIdx.x] = e2;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && target) atomicAdd(sse, sh[0]);
+}
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v, size_t n,
+                       float lr, float b1, float b2, float eps, float c1, float c2)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+inline unsigned blocks_for(size_t n) { return (unsigned)((n + 255) / 256); }
+
+// ------------------------------------------------------------------ the network
+
+// One convolution layer: W [Cout][Cin*T] and b [Cout] at offsets of the flat parameter vector.
+struct Conv {
+  int cin, cout, ks, stride;
+  size_t w, b;   // offsets into the parameter vector
+};
+
+struct Vol {
+  int C, D, H, W;
+  size_t vox() const { return (size_t)D * H * W; }
+  size_t size() const { return (size_t)C * vox(); }
+};
+
+}  // namespace
+
+// 3D U-Net: enc1 (2 convs, width c) -> stride-2 conv (2c) -> conv -> stride-2 conv (4c) -> conv
+// -> upsample, concat with the enc2 skip -> conv (2c) -> upsample, concat with enc1 -> conv (c)
+// -> 1x1x1 conv -> residual added to input channel 0. ReLU after every 3x3x3 convolution.
+struct mcpet_surrogate {
+  int D, H, W, cin, c;
+  Conv L[9];
+  size_t n_params = 0;
+  float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
+  // activations (post-ReLU) and their gradients; concat buffers hold the skip tensors
+  Vol vx, va1, vcat1, vd1, vcat2, vd2, vb3, ve2, ve1, vr;
+  float *x0 = nullptr, *a1 = nullptr, *cat1 = nullptr, *d1 = nullptr, *cat2 = nullptr, *d2 = nullptr, *b3 = nullptr,
+        *e2 = nullptr, *e1 = nullptr, *r = nullptr, *out = nullptr, *tgt = nullptr;
+  float *ga1 = nullptr, *gcat1 = nullptr, *gd1 = nullptr, *gcat2 = nullptr, *gd2 = nullptr, *gb3 = nullptr, *ge2 = nullptr,
+        *ge1 = nullptr, *gr = nullptr, *tmp1 = nullptr, *tmp2 = nullptr, *sse = nullptr;
+  long long t_adam = 0;
+  hipStream_t st = nullptr;
+  std::vector<void*> allocs;
+  std::string err;
+  double flops_fwd = 0.0;
+};
+
+namespace {
+
+mcpet_status serr(mcpet_surrogate* s, mcpet_status st, const std::string& m)
+{
+  if (s) s->err = m;
+  mcpet_host_set_error(m.c_str());
+  return st;
+}
+
+#define SHIP(s, call)                                                                                  \
+  do {                                                                                                 \
+    hipError_t e_ = (call);                                                                            \
+    if (e_ != hipSuccess) return serr(s, MCPET_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+hipError_t dalloc_f(mcpet_surrogate* s, float** p, size_t n)
+{
+  hipError_t e = hipMalloc((void**)p, sizeof(float) * std::max<size_t>(1, n));
+  if (e == hipSuccess) {
+    s->allocs.push_back(*p);
+    e = hipMemset(*p, 0, sizeof(float) * std::max<size_t>(1, n));
+  }
+  return e;
+}
+
+Geom geom_of(const Vol& in, const Vol& out, const Conv& l) { return Geom{in.C, in.D, in.H, in.W, out.D, out.H, out.W, l.ks, l.stride}; }
+
+// y = conv(x) + b (no activation)
+hipError_t conv_fwd(mcpet_surrogate* s, const Conv& l, const Vol& vin, const float* x, const Vol& vout, float* y)
+{
+  const int T = l.ks * l.ks * l.ks;
+  const int K = l.cin * T, N = (int)vout.vox();
+  s->flops_fwd += 2.0 * l.cout * (double)K * N;
+  return gemm(Dense{s->p + l.w, l.cout, K, K}, Im2col{x, geom_of(vin, vout, l), K, N}, y, l.cout, N, K, N, s->p + l.b, 1, false,
+              s->st);
+}
+// gx = dL/dx (overwritten); gW, gb accumulated
+hipError_t conv_bwd(mcpet_surrogate* s, const Conv& l, const Vol& vin, const float* x, const Vol& vout, const float* gy, float* gx)
+{
+  const int T = l.ks * l.ks * l.ks;
+  const Geom g = geom_of(vin, vout, l);
+  const int Nout = (int)vout.vox(), Nin = (int)vin.vox();
+  hipError_t e = hipSuccess;
+  // weight gradient: M = cout, N = cin*T, K = output voxels (split over workgroups)
+  const int split = std::max(1, Nout / 1024);
+  e = gemm(Dense{gy, l.cout, Nout, Nout}, Im2colT{x, g, Nout, l.cin * T}, s->g + l.w, l.cout, l.cin * T, Nout, l.cin * T, nullptr,
+           split, true, s->st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_channel_sum, dim3(l.cout), dim3(256), 0, s->st, gy, s->g + l.b, Nout);
+  e = hipGetLastError();
+  if (e != hipSuccess || !gx) return e;
+  // input gradient: M = cin, N = input voxels, K = cout*T
+  return gemm(WeightT{s->p + l.w, l.cin, l.cout, T}, Col2im{gy, g, l.cout * T, Nin}, gx, l.cin, Nin, l.cout * T, Nin, nullptr, 1,
+              false, s->st);
+}
+hipError_t relu(mcpet_surrogate* s, float* x, size_t n)
+{
+  hipLaunchKernelGGL(k_relu, dim3(blocks_for(n)), dim3(256), 0, s->st, x, n);
+  return hipGetLastError();
+}
+hipError_t relu_bwd(mcpet_surrogate* s, float* gx, const float* y, size_t n)
+{
+  hipLaunchKernelGGL(k_relu_bwd, dim3(blocks_for(n)), dim3(256), 0, s->st, gx, y, n);
+  return hipGetLastError();
+}
+
+// Forward of one sample already in s->x0; r = residual, out = x0[0] + r.
+hipError_t forward(mcpet_surrogate* s)
+{
+  hipError_t e = hipSuccess;
+  const Vol& vx = s->vx;
+  Vol vb1{s->c, s->D, s->H, s->W}, vb2{2 * s->c, s->D / 2, s->H / 2, s->W / 2};
+  float* b1 = s->cat1 + (size_t)2 * s->c * vb1.vox();   // enc1 output lives in the concat buffer
+  float* b2 = s->cat2 + (size_t)4 * s->c * vb2.vox();
+  s->flops_fwd = 0.0;
+#define STEP(x) \
+  if (e == hipSuccess) e = (x)
+  STEP(conv_fwd(s, s->L[0], vx, s->x0, s->va1, s->a1));
+  STEP(relu(s, s->a1, s->va1.size()));
+  STEP(conv_fwd(s, s->L[1], s->va1, s->a1, vb1, b1));
+  STEP(relu(s, b1, vb1.size()));
+  STEP(conv_fwd(s, s->L[2], vb1, b1, s->vd1, s->d1));
+  STEP(relu(s, s->d1, s->vd1.size()));
+  STEP(conv_fwd(s, s->L[3], s->vd1, s->d1, vb2, b2));
+  STEP(relu(s, b2, vb2.size()));
+  STEP(conv_fwd(s, s->L[4], vb2, b2, s->vd2, s->d2));
+  STEP(relu(s, s->d2, s->vd2.size()));
+  STEP(conv_fwd(s, s->L[5], s->vd2, s->d2, s->vb3, s->b3));
+  STEP(relu(s, s->b3, s->vb3.size()));
+  if (e == hipSuccess) {   // upsample b3 into the first 4c channels of cat2
+    hipLaunchKernelGGL(k_upsample, dim3(blocks_for(s->vb3.size() * 8)), dim3(256), 0, s->st, s->b3, s->cat2, s->vb3.C, s->vb3.D,
+                       s->vb3.H, s->vb3.W);
+    e = hipGetLastError();
+  }
+  STEP(conv_fwd(s, s->L[6], s->vcat2, s->cat2, s->ve2, s->e2));
+  STEP(relu(s, s->e2, s->ve2.size()));
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_upsample, dim3(blocks_for(s->ve2.size() * 8)), dim3(256), 0, s->st, s->e2, s->cat1, s->ve2.C, s->ve2.D,
+                       s->ve2.H, s->ve2.W);
+    e = hipGetLastError();
+  }
+  STEP(conv_fwd(s, s->L[7], s->vcat1, s->cat1, s->ve1, s->e1));
+  STEP(relu(s, s->e1, s->ve1.size()));
+  STEP(conv_fwd(s, s->L[8], s->ve1, s->e1, s->vr, s->r));
+#undef STEP
+  return e;
+}
+
+// Backward of one sample from gr (= dL/dr); accumulates parameter gradients.
+hipError_t backward(mcpet_surrogate* s)
+{
+  hipError_t e = hipSuccess;
+  Vol vb1{s->c, s->D, s->H, s->W}, vb2{2 * s->c, s->D / 2, s->H / 2, s->W / 2};
+  float* b1 = s->cat1 + (size_t)2 * s->c * vb1.vox();
+  float* b2 = s->cat2 + (size_t)4 * s->c * vb2.vox();
+  float* gb1 = s->gcat1 + (size_t)2 * s->c * vb1.vox();
+  float* gb2 = s->gcat2 + (size_t)4 * s->c * vb2.vox();
+#define STEP(x) \
+  if (e == hipSuccess) e = (x)
+  STEP(conv_bwd(s, s->L[8], s->ve1, s->e1, s->vr, s->gr, s->ge1));
+  STEP(relu_bwd(s, s->ge1, s->e1, s->ve1.size()));
+  STEP(conv_bwd(s, s->L[7], s->vcat1, s->cat1, s->ve1, s->ge1, s->gcat1));   // -> g(up1), g(b1 skip)
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_upsample_bwd, dim3(blocks_for(s->ve2.size())), dim3(256), 0, s->st, s->gcat1, s->ge2, s->ve2.C, s->ve2.D,
+                       s->ve2.H, s->ve2.W);
+    e = hipGetLastError();
+  }
+  STEP(relu_bwd(s, s->ge2, s->e2, s->ve2.size()));
+  STEP(conv_bwd(s, s->L[6], s->vcat2, s->cat2, s->ve2, s->ge2, s->gcat2));   // -> g(up2), g(b2 skip)
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_upsample_bwd, dim3(blocks_for(s->vb3.size())), dim3(256), 0, s->st, s->gcat2, s->gb3, s->vb3.C, s->vb3.D,
+                       s->vb3.H, s->vb3.W);
+    e = hipGetLastError();
+  }
+  STEP(relu_bwd(s, s->gb3, s->b3, s->vb3.size()));
+  STEP(conv_bwd(s, s->L[5], s->vd2, s->d2, s->vb3, s->gb3, s->gd2));
+  STEP(relu_bwd(s, s->gd2, s->d2, s->vd2.size()));
+  STEP(conv_bwd(s, s->L[4], vb2, b2, s->vd2, s->gd2, s->tmp2));             // main path into b2
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_add, dim3(blocks_for(vb2.size())), dim3(256), 0, s->st, gb2, s->tmp2, vb2.size());
+    e = hipGetLastError();
+  }
+  STEP(relu_bwd(s, gb2, b2, vb2.size()));
+  STEP(conv_bwd(s, s->L[3], s->vd1, s->d1, vb2, gb2, s->gd1));
+  STEP(relu_bwd(s, s->gd1, s->d1, s->vd1.size()));
+  STEP(conv_bwd(s, s->L[2], vb1, b1, s->vd1, s->gd1, s->tmp1));             // main path into b1
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_add, dim3(blocks_for(vb1.size())), dim3(256), 0, s->st, gb1, s->tmp1, vb1.size());
+    e = hipGetLastError();
+  }
+  STEP(relu_bwd(s, gb1, b1, vb1.size()));
+  STEP(conv_bwd(s, s->L[1], s->va1, s->a1, vb1, gb1, s->ga1));
+  STEP(relu_bwd(s, s->ga1, s->a1, s->va1.size()));
+  STEP(conv_bwd(s, s->L[0], s->vx, s->x0, s->va1, s->ga1, nullptr));
+#undef STEP
+  return e;
+}
+
+}  // namespace
+
+extern "C" mcpet_status mcpet_surrogate_create(int32_t D, int32_t H, int32_t W, int32_t in_channels, int32_t width, uint64_t seed,
+                                               mcpet_surrogate** out)
+{
+  if (!out) return serr(nullptr, MCPET_E_ARG, "mcpet_surrogate_create: null output pointer");
+  *out = nullptr;
+  if (D < 4 || H < 4 || W < 4 || D % 4 || H % 4 || W % 4 || in_channels < 1 || width < 1)
+    return serr(nullptr, MCPET_E_ARG, "mcpet_surrogate_create: sizes must be multiples of 4, channels >= 1");
+  mcpet_surrogate* s = new mcpet_surrogate();
+  s->D = D;
+  s->H = H;
+  s->W = W;
+  s->cin = in_channels;
+  s->c = width;
+  const int c = width;
+  const int spec[9][4] = {{in_channels, c, 3, 1}, {c, c, 3, 1},         {c, 2 * c, 3, 2},     {2 * c, 2 * c, 3, 1}, {2 * c, 4 * c, 3, 2},
+                          {4 * c, 4 * c, 3, 1},    {6 * c, 2 * c, 3, 1}, {3 * c, c, 3, 1},     {c, 1, 1, 1}};
+  size_t off = 0;
+  for (int i = 0; i < 9; i++) {
+    Conv& l = s->L[i];
+    l.cin = spec[i][0];
+    l.cout = spec[i][1];
+    l.ks = spec[i][2];
+    l.stride = spec[i][3];
+    const int T = l.ks * l.ks * l.ks;
+    l.w = off;
+    off += (size_t)l.cout * l.cin * T;
+    l.b = off;
+    off += (size_t)l.cout;
+  }
+  s->n_params = off;
+  s->vx = {in_channels, D, H, W};
+  s->va1 = {c, D, H, W};
+  s->vcat1 = {3 * c, D, H, W};
+  s->vd1 = {2 * c, D / 2, H / 2, W / 2};
+  s->vcat2 = {6 * c, D / 2, H / 2, W / 2};
+  s->vd2 = {4 * c, D / 4, H / 4, W / 4};
+  s->vb3 = {4 * c, D / 4, H / 4, W / 4};
+  s->ve2 = {2 * c, D / 2, H / 2, W / 2};
+  s->ve1 = {c, D, H, W};
+  s->vr = {1, D, H, W};
+  hipError_t e = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking);
+  struct A {
+    float** p;
+    size_t n;
+  } al[] = {{&s->p, off},          {&s->g, off},          {&s->m, off},           {&s->v, off},
+            {&s->x0, s->vx.size()}, {&s->a1, s->va1.size()}, {&s->cat1, s->vcat1.size()}, {&s->d1, s->vd1.size()},
+            {&s->cat2, s->vcat2.size()}, {&s->d2, s->vd2.size()}, {&s->b3, s->vb3.size()}, {&s->e2, s->ve2.size()},
+            {&s->e1, s->ve1.size()}, {&s->r, s->vr.size()}, {&s->out, s->vr.size()}, {&s->tgt, s->vr.size()},
+            {&s->ga1, s->va1.size()}, {&s->gcat1, s->vcat1.size()}, {&s->gd1, s->vd1.size()}, {&s->gcat2, s->vcat2.size()},
+            {&s->gd2, s->vd2.size()}, {&s->gb3, s->vb3.size()}, {&s->ge2, s->ve2.size()}, {&s->ge1, s->ve1.size()},
+            {&s->gr, s->vr.size()}, {&s->tmp1, s->va1.size()}, {&s->tmp2, s->vd1.size()}, {&s->sse, 1}};
+  for (auto& a : al)
+    if (e == hipSuccess) e = dalloc_f(s, a.p, a.n);
+  if (e != hipSuccess) {
+    std::string m = std::string("mcpet_surrogate_create: ") + hipGetErrorString(e);
+    mcpet_surrogate_destroy(s);
+    return serr(nullptr, MCPET_E_NOMEM, m);
+  }
+  // He-normal weights from a seeded generator (splitmix64 + Box-Muller), zero biases
+  std::vector<float> hp(off, 0.0f);
+  uint64_t z = seed ? seed : 0x9E3779B97F4A7C15ull;
+  auto next = [&]() {
+    z += 0x9E3779B97F4A7C15ull;
+    uint64_t x = z;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    x ^= x >> 31;
+    return (double)((x >> 11) + 1) * 0x1.0p-53;
+  };
+  for (int i = 0; i < 9; i++) {
+    const Conv& l = s->L[i];
+    const int T = l.ks * l.ks * l.ks;
+    const double sd = std::sqrt(2.0 / (l.cin * T)) * (i == 8 ? 0.1 : 1.0);
+    for (size_t k = 0; k < (size_t)l.cout * l.cin * T; k++)
+      hp[l.w + k] = (float)(sd * std::sqrt(-2.0 * std::log(next())) * std::cos(6.283185307179586 * next()));
+  }
+  e = hipMemcpy(s->p, hp.data(), sizeof(float) * off, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    mcpet_surrogate_destroy(s);
+    return serr(nullptr, MCPET_E_HIP, "mcpet_surrogate_create: upload");
+  }
+  *out = s;
+  return MCPET_OK;
+}
+
+extern "C" int64_t mcpet_surrogate_param_count(const mcpet_surrogate* s) { return s ? (int64_t)s->n_params : 0; }
+
+extern "C" mcpet_status mcpet_surrogate_get_params(mcpet_surrogate* s, float* out)
+{
+  if (!s || !out) return serr(s, MCPET_E_ARG, "mcpet_surrogate_get_params: null argument");
+  SHIP(s, hipMemcpy(out, s->p, sizeof(float) * s->n_params, hipMemcpyDeviceToHost));
+  return MCPET_OK;
+}
+extern "C" mcpet_status mcpet_surrogate_set_params(mcpet_surrogate* s, const float* in)
+{
+  if (!s || !in) return serr(s, MCPET_E_ARG, "mcpet_surrogate_set_params: null argument");
+  SHIP(s, hipMemcpy(s->p, in, sizeof(float) * s->n_params, hipMemcpyHostToDevice));
+  return MCPET_OK;
+}
+
+// x: batch x cin x D x H x W, y (out): batch x D x H x W; returns the forward GFLOP per sample.
+extern "C" mcpet_status mcpet_surrogate_forward(mcpet_surrogate* s, const float* x, int32_t batch, float* y, double* gflop)
+{
+  if (!s || !x || !y || batch < 1) return serr(s, MCPET_E_ARG, "mcpet_surrogate_forward: bad arguments");
+  const size_t nx = s->vx.size(), nv = s->vr.size();
+  for (int b = 0; b < batch; b++) {
+    SHIP(s, hipMemcpyAsync(s->x0, x + b * nx, sizeof(float) * nx, hipMemcpyHostToDevice, s->st));
+    SHIP(s, forward(s));
+    hipLaunchKernelGGL(k_output, dim3(blocks_for(nv)), dim3(256), 0, s->st, s->x0, s->r, s->out, (const float*)nullptr,
+                       (float*)nullptr, 0.0f, (float*)nullptr, (int)nv);
+    SHIP(s, hipGetLastError());
+    SHIP(s, hipMemcpyAsync(y + b * nv, s->out, sizeof(float) * nv, hipMemcpyDeviceToHost, s->st));
+  }
+  SHIP(s, hipStreamSynchronize(s->st));
+  if (gflop) *gflop = s->flops_fwd * 1e-9;
+  return MCPET_OK;
+}
+
+// Mean-squared-error gradients of the batch (loss = mean over batch and voxels); grad (host,
+// n_params) may be NULL. Leaves the gradients in the device buffer for an optimiser step.
+static mcpet_status batch_gradients(mcpet_surrogate* s, const float* x, const float* target, int32_t batch, float* loss)
+{
+  const size_t nx = s->vx.size(), nv = s->vr.size();
+  SHIP(s, hipMemsetAsync(s->g, 0, sizeof(float) * s->n_params, s->st));
+  SHIP(s, hipMemsetAsync(s->sse, 0, sizeof(float), s->st));
+  const float scale = 1.0f / ((float)nv * (float)batch);
+  for (int b = 0; b < batch; b++) {
+    SHIP(s, hipMemcpyAsync(s->x0, x + b * nx, sizeof(float) * nx, hipMemcpyHostToDevice, s->st));
+    SHIP(s, hipMemcpyAsync(s->tgt, target + b * nv, sizeof(float) * nv, hipMemcpyHostToDevice, s->st));
+    SHIP(s, forward(s));
+    hipLaunchKernelGGL(k_output, dim3(blocks_for(nv)), dim3(256), 0, s->st, s->x0, s->r, s->out, s->tgt, s->gr, scale, s->sse,
+                       (int)nv);
+    SHIP(s, hipGetLastError());
+    SHIP(s, backward(s));
+  }
+  float sse = 0.0f;
+  SHIP(s, hipMemcpyAsync(&sse, s->sse, sizeof(float), hipMemcpyDeviceToHost, s->st));
+  SHIP(s, hipStreamSynchronize(s->st));
+  if (loss) *loss = sse * scale;
+  return MCPET_OK;
+}
+
+extern "C" mcpet_status mcpet_surrogate_gradients(mcpet_surrogate* s, const float* x, const float* target, int32_t batch,
+                                                  float* loss, float* grad)
+{
+  if (!s || !x || !target || batch < 1) return serr(s, MCPET_E_ARG, "mcpet_surrogate_gradients: bad arguments");
+  mcpet_status st = batch_gradients(s, x, target, batch, loss);
+  if (st != MCPET_OK || !grad) return st;
+  SHIP(s, hipMemcpy(grad, s->g, sizeof(float) * s->n_params, hipMemcpyDeviceToHost));
+  return MCPET_OK;
+}
+
+// One Adam step (beta 0.9, 0.999, eps 1e-8) on the batch's MSE.
+extern "C" mcpet_status mcpet_surrogate_train_step(mcpet_surrogate* s, const float* x, const float* target, int32_t batch,
+                                                   float lr, float* loss)
+{
+  if (!s || !x || !target || batch < 1 || !(lr > 0.0f)) return serr(s, MCPET_E_ARG, "mcpet_surrogate_train_step: bad arguments");
+  mcpet_status st = batch_gradients(s, x, target, batch, loss);
+  if (st != MCPET_OK) return st;
+  s->t_adam++;
+  const float c1 = 1.0f - std::pow(0.9f, (float)s->t_adam), c2 = 1.0f - std::pow(0.999f, (float)s->t_adam);
+  hipLaunchKernelGGL(k_adam, dim3(blocks_for(s->n_params)), dim3(256), 0, s->st, s->p, s->g, s->m, s->v, s->n_params, lr, 0.9f,
+                     0.999f, 1e-8f, c1, c2);
+  SHIP(s, hipGetLastError());
+  SHIP(s, hipStreamSynchronize(s->st));
+  return MCPET_OK;
+}
+
+extern "C" void mcpet_surrogate_destroy(mcpet_surrogate* s)
+{
+  if (!s) return;
+  if (s->st) (void)hipStreamSynchronize(s->st);
+  for (void* p : s->allocs) (void)hipFree(p);
+  if (s->st) (void)hipStreamDestroy(s->st);
+  delete s;
+}
+
+extern "C" const char* mcpet_surrogate_last_error(const mcpet_surrogate* s)
+{
+  if (s && !s->err.empty()) return s->err.c_str();
+  return mcpet_last_error(nullptr);
+}
+
+namespace {
+
+// ------------------------------------------------------------------ sinogram rebinning
+
+// One workgroup per (angle group h, plane group d): the sinograms of the plane group (CSR list)
+// times the group's angle rows are read as whole radial rows (coalesced), each count added to
+// the LDS sum of its radial group. out[(d H + h) W + w] = float(sum).
+__global__ __launch_bounds__(256) void k_rebin(const int* __restrict__ sino, const int* __restrict__ grp_off,
+                                               const int* __restrict__ grp_sino, int nangles, int nrad, int H, int W, int gh,
+                                               int gw, float* __restrict__ out)
+{
+  __shared__ unsigned long long acc[256];
+  const int h = blockIdx.x, d = blockIdx.y;
+  for (int i = threadIdx.x; i < W; i += blockDim.x) acc[i] = 0ull;
+  __syncthreads();
+  const int a0 = h * gh, rows = max(0, min(gh, nangles - a0));
+  const int j0 = grp_off[d], nj = grp_off[d + 1] - j0;
+  const long long items = (long long)nj * rows * nrad;
+  for (long long it = threadIdx.x; it < items; it += blockDim.x) {
+    const int r = (int)(it % nrad);
+    const long long q = it / nrad;
+    const int a = (int)(q % rows), j = (int)(q / rows);
+    const int v = sino[((size_t)grp_sino[j0 + j] * nangles + a0 + a) * nrad + r];
+    if (v) atomicAdd(&acc[r / gw], (unsigned long long)(unsigned)v);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < W; i += blockDim.x) out[((size_t)d * H + h) * W + i] = (float)acc[i];
+}
+
+}  // namespace
+
+hipError_t mcpet_launch_rebin(const int* sino, const int* grp_off, const int* grp_sino, int nangles, int nrad, int D, int H, int W,
+                              int gh, int gw, float* out, hipStream_t st)
+{
+  if (W > 256) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_rebin, dim3(H, D), dim3(256), 0, st, sino, grp_off, grp_sino, nangles, nrad, H, W, gh, gw, out);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ self test of the GEMM
+
+extern "C" mcpet_status mcpet_selftest_gemm_bf16(const float* a, const float* b, int32_t M, int32_t N, int32_t K, int32_t split,
+                                                 float* c)
+{
+  if (!a || !b || !c || M < 1 || N < 1 || K < 1 || split < 1) {
+    mcpet_host_set_error("selftest_gemm_bf16: bad args");
+    return MCPET_E_ARG;
+  }
+  float *da = nullptr, *db = nullptr, *dc = nullptr;
+  hipError_t e = hipMalloc((void**)&da, 4 * (size_t)M * K);
+  if (e == hipSuccess) e = hipMalloc((void**)&db, 4 * (size_t)K * N);
+  if (e == hipSuccess) e = hipMalloc((void**)&dc, 4 * (size_t)M * N);
+  if (e == hipSuccess) e = hipMemcpy(da, a, 4 * (size_t)M * K, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(db, b, 4 * (size_t)K * N, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(dc, 0, 4 * (size_t)M * N);
+  if (e == hipSuccess) e = gemm(Dense{da, M, K, K}, Dense{db, K, N, N}, dc, M, N, K, N, nullptr, split, false, 0);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(c, dc, 4 * (size_t)M * N, hipMemcpyDeviceToHost);
+  if (da) (void)hipFree(da);
+  if (db) (void)hipFree(db);
+  if (dc) (void)hipFree(dc);
+  if (e != hipSuccess) {
+    mcpet_host_set_error((std::string("selftest_gemm_bf16: ") + hipGetErrorString(e)).c_str());
+    return MCPET_E_HIP;
+  }
+  return MCPET_OK;
+}

@@ -180,6 +180,8 @@ def lib():
     L.mcpet_stream_seeds.argtypes = [C.c_int32, P(C.c_int64), C.c_int32, P(C.c_int32), P(C.c_int32)]
     L.mcpet_version.restype = C.c_char_p
     L.mcpet_selftest_sort.argtypes = [P(C.c_uint64), C.c_int64, C.c_int32, P(C.c_uint32)]
+    L.mcpet_selftest_gemm_bf16.argtypes = [P(C.c_float), P(C.c_float), C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                           P(C.c_float)]
     L.mcpet_struct_sizes.argtypes = [C.POINTER(C.c_size_t), C.c_int32]
     L.mcpet_batch_create.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(C.c_void_p)]
     L.mcpet_batch_run.argtypes = [C.c_void_p, C.c_void_p]
@@ -201,6 +203,25 @@ def lib():
     L.mcpet_server_last_error.argtypes = [C.c_void_p]
     L.mcpet_server_last_error.restype = C.c_char_p
     L.mcpet_struct_sizes.restype = C.c_int32
+    L.mcpet_sinogram_planes.argtypes = [P(Config), P(C.c_int32)]
+    L.mcpet_rebin_sinogram.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P(C.c_float)]
+    L.mcpet_batch_rebin_sinogram.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                             P(C.c_float)]
+    L.mcpet_surrogate_create.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64,
+                                         P(C.c_void_p)]
+    L.mcpet_surrogate_param_count.argtypes = [C.c_void_p]
+    L.mcpet_surrogate_param_count.restype = C.c_int64
+    L.mcpet_surrogate_get_params.argtypes = [C.c_void_p, P(C.c_float)]
+    L.mcpet_surrogate_set_params.argtypes = [C.c_void_p, P(C.c_float)]
+    L.mcpet_surrogate_forward.argtypes = [C.c_void_p, P(C.c_float), C.c_int32, P(C.c_float), P(C.c_double)]
+    L.mcpet_surrogate_gradients.argtypes = [C.c_void_p, P(C.c_float), P(C.c_float), C.c_int32, P(C.c_float),
+                                            P(C.c_float)]
+    L.mcpet_surrogate_train_step.argtypes = [C.c_void_p, P(C.c_float), P(C.c_float), C.c_int32, C.c_float,
+                                             P(C.c_float)]
+    L.mcpet_surrogate_destroy.argtypes = [C.c_void_p]
+    L.mcpet_surrogate_destroy.restype = None
+    L.mcpet_surrogate_last_error.argtypes = [C.c_void_p]
+    L.mcpet_surrogate_last_error.restype = C.c_char_p
     _lib = L
     return L
 
@@ -226,6 +247,16 @@ def read_input(path):
 def sinogram_sizes(cfg):
     lib().mcpet_sinogram_sizes(C.byref(cfg))
     return cfg
+
+
+def sinogram_planes(cfg):
+    """Ring sum (single-slice rebinning plane) of every sinogram of cfg's michelogram."""
+    out = np.zeros(cfg.nsinos, np.int32)
+    _check(lib().mcpet_sinogram_planes(C.byref(cfg), out.ctypes.data_as(C.POINTER(C.c_int32))))
+    return out
+
+
+TALLY_INDEX = {"trues": 0, "scatter": 1, "randoms": 2, "spurious": 3}
 
 
 def load_voxels(path, cfg=None):
@@ -365,6 +396,13 @@ class Simulation:
     def fetch(self, sinograms=True, images=True, spectrum=True, psf=True, dose=True, singles=True):
         return _fetch_tallies(lambda t: _check(lib().mcpet_fetch(self.ctx, C.byref(t)), self.ctx), self.cfg, self.phantom.nvox,
                               sinograms, images, spectrum, psf, dose, singles)
+
+    def rebin(self, which, grid):
+        """Tally `which` ("trues", "scatter", ...) summed on the device into a (D, H, W) grid."""
+        D, H, W = grid
+        out = np.zeros((D, H, W), np.float32)
+        _check(lib().mcpet_rebin_sinogram(self.ctx, TALLY_INDEX[which], D, H, W, _fptr(out)), self.ctx)
+        return out
 
     def close(self):
         if self.ctx:
@@ -560,6 +598,12 @@ class Batch:
         self._check(lib().mcpet_batch_get_stats(self.b, k, C.byref(s)))
         return s.as_dict()
 
+    def rebin(self, k, which, grid):
+        D, H, W = grid
+        out = np.zeros((D, H, W), np.float32)
+        self._check(lib().mcpet_batch_rebin_sinogram(self.b, k, TALLY_INDEX[which], D, H, W, _fptr(out)))
+        return out
+
     def autotune(self, fraction=0.05):
         res = (TuneResult * 32)()
         n = C.c_int32()
@@ -657,6 +701,99 @@ def selftest_sort(keys, passes=8):
     _check(lib().mcpet_selftest_sort(k.ctypes.data_as(C.POINTER(C.c_uint64)), len(k), int(passes),
                                      perm.ctypes.data_as(C.POINTER(C.c_uint32))))
     return perm
+
+
+def selftest_gemm_bf16(a, b, split=1):
+    """C = a @ b on the device's BF16 MFMA GEMM (FP32 accumulation)."""
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    M, K = a.shape
+    N = b.shape[1]
+    c = np.zeros((M, N), np.float32)
+    _check(lib().mcpet_selftest_gemm_bf16(_fptr(a), _fptr(b), M, N, K, int(split), _fptr(c)))
+    return c
+
+
+class Surrogate:
+    """The scatter surrogate's 3D U-Net on the device (mcpet_surrogate_*): BF16 MFMA
+    convolutions with FP32 accumulation, FP32 parameters and Adam state."""
+
+    def __init__(self, grid, in_channels=2, width=16, seed=1):
+        self.grid = tuple(int(v) for v in grid)
+        self.in_channels = int(in_channels)
+        self.width = int(width)
+        self.s = C.c_void_p()
+        D, H, W = self.grid
+        _check(lib().mcpet_surrogate_create(D, H, W, self.in_channels, self.width, int(seed), C.byref(self.s)))
+        self.n_params = int(lib().mcpet_surrogate_param_count(self.s))
+        self.gflop = 0.0
+
+    def _check(self, st):
+        if st != 0:
+            raise McpetError(st, lib().mcpet_surrogate_last_error(self.s).decode(errors="replace"))
+
+    def _x(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        if x.ndim == 4:
+            x = x[None]
+        if x.shape[1:] != (self.in_channels,) + self.grid:
+            raise ValueError(f"input shape {x.shape} does not match (batch, {self.in_channels}, {self.grid})")
+        return x
+
+    def params(self):
+        p = np.zeros(self.n_params, np.float32)
+        self._check(lib().mcpet_surrogate_get_params(self.s, _fptr(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != self.n_params:
+            raise ValueError("parameter vector size mismatch")
+        self._check(lib().mcpet_surrogate_set_params(self.s, _fptr(p)))
+
+    def forward(self, x):
+        x = self._x(x)
+        y = np.zeros((x.shape[0],) + self.grid, np.float32)
+        g = C.c_double()
+        self._check(lib().mcpet_surrogate_forward(self.s, _fptr(x), x.shape[0], _fptr(y), C.byref(g)))
+        self.gflop = g.value
+        return y
+
+    def _target(self, x, t):
+        t = np.ascontiguousarray(t, np.float32).reshape((x.shape[0],) + self.grid)
+        return t
+
+    def gradients(self, x, target):
+        x = self._x(x)
+        t = self._target(x, target)
+        g = np.zeros(self.n_params, np.float32)
+        loss = C.c_float()
+        self._check(lib().mcpet_surrogate_gradients(self.s, _fptr(x), _fptr(t), x.shape[0], C.byref(loss), _fptr(g)))
+        return loss.value, g
+
+    def train_step(self, x, target, lr=1e-3):
+        x = self._x(x)
+        t = self._target(x, target)
+        loss = C.c_float()
+        self._check(lib().mcpet_surrogate_train_step(self.s, _fptr(x), _fptr(t), x.shape[0], float(lr), C.byref(loss)))
+        return loss.value
+
+    def close(self):
+        if self.s:
+            lib().mcpet_surrogate_destroy(self.s)
+            self.s = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def selftest_copy(nbytes, iters=10):
