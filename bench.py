@@ -124,6 +124,106 @@ def bench_batch(a):
     print(json.dumps(out))
 
 
+def bench_surrogate(a):
+    """Scatter surrogate workload: time to a 5 %-NRMSE scatter sinogram on the 32 x 56 x 56
+    grid, Monte Carlo against the surrogate (low-count run + device rebinning + U-Net), on
+    evaluation phantoms drawn with a seed disjoint from the training phantoms. Per phantom: a
+    Monte Carlo reference accumulated in runs of 4e9 decays until its Poisson floor is <= 1 %
+    (or 8 runs); N5 from the reference; Monte Carlo at N5 and the surrogate each timed over
+    a.steps runs (median) after a.warmup untimed runs, wall clock from the run call to the
+    estimate on the host."""
+    import mcpet
+    from mcpet import phantoms
+    from mcpet import surrogate as S
+    weights = a.surrogate_weights or S.WEIGHTS
+    if not os.path.exists(weights):
+        raise SystemExit(f"bench.py: surrogate weights {weights} missing (train with python -m mcpet.surrogate)")
+    names = S._materials()
+    specs = phantoms.scatter_phantoms(a.eval_phantoms, a.eval_seed, n_decays=S.DECAYS_PER_SECOND, acquisition_time_s=1.0)
+    est = S.ScatterEstimator(weights)
+    rows = []
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    for i, sp in enumerate(specs):
+        ph = mcpet.phantom_from_arrays(*sp)
+        tab = mcpet.load_materials(names, ph.density_max())
+        active = int((sp[2] > 0).sum())
+        # reference
+        acc = np.zeros(S.GRID)
+        ref_decays = 0
+        with mcpet.Simulation(S.config(names, S.streams_per_voxel(4e9, active)), ph, tab) as sim:
+            for j in range(8):
+                sim.run(seed=9000 + 17 * i + j, acquisition_time_ps=S.acquisition_ps(4e9))
+                acc += sim.rebin("scatter", S.GRID)
+                ref_decays += sim.stats()["decays"]
+                if S.poisson_floor(acc) <= 0.01:
+                    break
+        ref = acc / ref_decays
+        floor = S.poisson_floor(acc)
+        n5 = S.decays_for(ref)
+        log(f"surrogate bench: phantom {i}: reference {ref_decays:.3g} decays, floor {floor:.4f}, N5 {n5:.3g}")
+        # Monte Carlo at N5
+        with mcpet.Simulation(S.config(names, S.streams_per_voxel(n5, active)), ph, tab) as sim:
+            for w in range(a.warmup):
+                S.monte_carlo(sim, n5, seed=100 + w)
+            mc_t, mc_e = [], []
+            for s_ in range(a.steps):
+                e, d, dt, _ = S.monte_carlo(sim, n5, seed=200 + s_)
+                mc_t.append(dt)
+                mc_e.append(S.nrmse(e, ref))
+        # surrogate
+        with mcpet.Simulation(S.config(names, S.streams_per_voxel(S.N_LOW, active)), ph, tab) as sim:
+            for w in range(a.warmup):
+                est.estimate(sim, seed=300 + w)
+            su_t, su_e, su_z, su_mc, su_net, raw_e = [], [], [], [], [], []
+            for s_ in range(a.steps):
+                e, info = est.estimate(sim, seed=400 + s_)
+                su_t.append(info["seconds"])
+                su_mc.append(info["mc_seconds"])
+                su_net.append(info["net_seconds"])
+                su_e.append(S.nrmse(e, ref))
+                su_z.append(info["z"])
+        ph.close()
+        tab.close()
+        t_mc, t_su = float(np.median(mc_t)), float(np.median(su_t))
+        ok = [(err <= S.TARGET and z <= est.z_max) for err, z in zip(su_e, su_z)]
+        fallback = sum(1 for o in ok if not o) / len(ok)
+        t_eff = t_su + fallback * t_mc
+        rows.append({"phantom": i, "reference_decays": ref_decays, "reference_floor": floor, "n5_decays": n5,
+                     "mc_seconds": t_mc, "mc_nrmse": float(np.median(mc_e)), "surrogate_seconds": t_su,
+                     "surrogate_mc_seconds": float(np.median(su_mc)), "surrogate_net_seconds": float(np.median(su_net)),
+                     "surrogate_nrmse": float(np.median(su_e)), "surrogate_nrmse_max": float(np.max(su_e)),
+                     "consistency_z": float(np.median(su_z)), "fallback_fraction": fallback,
+                     "effective_seconds": t_eff, "speedup": t_mc / t_eff})
+        log(f"surrogate bench: phantom {i}: MC {t_mc * 1e3:.1f} ms (nrmse {np.median(mc_e):.4f}), surrogate "
+            f"{t_su * 1e3:.2f} ms (nrmse {np.median(su_e):.4f}, z {np.median(su_z):.2f}), speedup {t_mc / t_eff:.1f}")
+    # U-Net forward throughput (host-timed forward of one sample)
+    x = np.random.default_rng(0).random((1, S.IN_CHANNELS) + S.GRID).astype(np.float32)
+    est.net.forward(x)
+    t0 = time.perf_counter()
+    for _ in range(20):
+        est.net.forward(x)
+    fwd_s = (time.perf_counter() - t0) / 20
+    gflop = est.net.gflop
+    est.close()
+    tot_mc = sum(r["mc_seconds"] for r in rows)
+    tot_eff = sum(r["effective_seconds"] for r in rows)
+    out = {"metric": "time to a 5%-NRMSE scatter sinogram, Monte Carlo / surrogate", "value": tot_mc / tot_eff,
+           "unit": "x (speedup)", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": 1e3 * tot_eff / len(rows), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "bf16 MFMA (fp32 accumulate) + f32 transport", "data": "synthetic",
+           "config": {"workload": f"{len(rows)} evaluation phantoms 128x128x64 at 2 mm (scatter_phantoms seed "
+                                  f"{a.eval_seed}), scatter on the 32x56x56 rebinned grid, surrogate input "
+                                  f"{S.N_LOW:.0e} decays, U-Net width {S.WIDTH}", "parallelism": "single GPU"},
+           "mc_seconds_mean": tot_mc / len(rows), "surrogate_seconds_mean": tot_eff / len(rows),
+           "surrogate_nrmse_max": max(r["surrogate_nrmse_max"] for r in rows),
+           "fallback_fraction": float(np.mean([r["fallback_fraction"] for r in rows])),
+           "roofline": {"bound": "mfma", "achieved": gflop / fwd_s / 1e3, "peak": 2500.0, "unit": "TFLOP/s",
+                        "frac": gflop / fwd_s / 1e3 / 2500.0, "traffic": None,
+                        "kernel": "U-Net forward (host-timed, one sample)", "gflop": gflop, "ms": fwd_s * 1e3},
+           "phantoms": rows}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,8 +237,12 @@ def main():
     ap.add_argument("--phase-policy", type=int, default=0, help="0 auto, 1 most lanes, 2 least waste")
     ap.add_argument("--phase-cost", type=str, default="", help="comma-separated relative phase costs (DECAY, SOURCE2, STEP, INTERACT, TALLY, CTAU, CPZ)")
     ap.add_argument("--voxel-format", type=int, default=0, help="0 auto (palette indices), 1 8-byte voxel words")
-    ap.add_argument("--workload", choices=["head", "batch"], default="head",
-                    help="head: the 1e9-decay head phantom; batch: 48 varied phantoms in one launch")
+    ap.add_argument("--workload", choices=["head", "batch", "surrogate"], default="head",
+                    help="head: the 1e9-decay head phantom; batch: 48 varied phantoms in one launch; "
+                         "surrogate: time to a 5%%-NRMSE scatter sinogram, Monte Carlo vs the learned surrogate")
+    ap.add_argument("--eval-phantoms", type=int, default=6)
+    ap.add_argument("--eval-seed", type=int, default=777)
+    ap.add_argument("--surrogate-weights", default="", help="U-Net parameters (.npy); default: the packaged ones")
     ap.add_argument("--batch-size", type=int, default=48)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -160,6 +264,8 @@ def main():
     from mcpet import phantoms
     if a.workload == "batch":
         return bench_batch(a)
+    if a.workload == "surrogate":
+        return bench_surrogate(a)
     mat, dens, act, vs = phantoms.head(n_decays=a.decays)
     names = phantoms.head_materials(MATERIALS)
     cfg = phantoms.make_config(seed=20261202, threads_per_block=a.tpb, acquisition_time_s=10.0, nrad=168,

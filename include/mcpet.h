@@ -410,7 +410,8 @@ mcpet_status mcpet_batch_rebin_sinogram(mcpet_batch* b, int32_t k, int32_t which
  * convolution; ReLU after every 3^3 convolution; the output is input channel 0 plus that
  * residual. Every convolution runs as an implicit GEMM on BF16 MFMA tiles with FP32
  * accumulation; parameters, activations and optimiser state are FP32. Parameter layout: per
- * layer W[Cout][Cin][k^3] then b[Cout], layers in the order above. */
+ * layer W[Cout][Cin][k^3] then b[Cout], layers in the order above. Created with He-normal
+ * weights from `seed`, zero biases and a zero output layer (the identity on channel 0). */
 typedef struct mcpet_surrogate mcpet_surrogate;
 mcpet_status mcpet_surrogate_create(int32_t D, int32_t H, int32_t W, int32_t in_channels, int32_t width, uint64_t seed,
                                     mcpet_surrogate** out);

@@ -182,31 +182,241 @@ hipError_t gemm(AOp A, BOp B, float* C, int M, int N, int K, int ldc, const floa
   return hipGetLastError();
 }
 
-// input-gradient A: (m = ci, k = co*T + t) -> W[co][ci*T + t]
-struct WeightT {
-  const float* w;
-  int Cin, Cout, T;
-  __device__ __forceinline__ float operator()(int m, int k) const
-  {
-    if (m >= Cin || k >= Cout * T) return 0.0f;
-    const int co = k / T, t = k - co * T;
-    return w[(size_t)co * Cin * T + (size_t)m * T + t];
-  }
+// ------------------------------------------------------------------ convolution kernels
+//
+// Each convolution product is one implicit GEMM with a tap-major reduction index
+// k = t * Cp + c (t: kernel tap, 27 for 3x3x3 or 1 for 1x1x1; c: channel, padded to
+// Cp = roundup(C, 8)), so a lane's 8 consecutive k share one tap: one address and bounds
+// computation, then 8 loads at channel stride.
+//   FWD   : Y[co][p]        = act(sum_k Wf[co][k] X[c][s p + d_t] + b[co])    m = co, n = output voxel
+//   BDATA : dX[ci][p]      (+)= mask(sum_k Wb[ci][k] dY[c][q], s q + d_t = p)  m = ci, n = input voxel
+//   BWGT  : G[co][ci T + t] += sum_q dY[co][q] X[ci][s q + d_t]              m = co, n = t Cin + ci, k = q
+// Wf, Wb: BF16 copies of the FP32 weights in those layouts (zero padded). A workgroup is four
+// waves on one 16 MT x 16 output tile that split the k-steps between them and reduce through
+// LDS; BWGT also splits its (voxel) reduction over workgroups and adds with atomics.
+
+enum { FWD = 0, BDATA = 1, BWGT = 2 };
+
+struct ConvK {
+  const __bf16* A;     // FWD, BDATA: BF16 weights [rows][Kp]
+  const float* Ad;     // BWGT: dY [M][K]
+  const float* X;      // FWD, BWGT: input activations [Cin][D][H][W]; BDATA: dY [Cout][oD][oH][oW]
+  float* C;            // FWD: Y [M][N]; BDATA: dX [M][N]; BWGT: weight gradient [Cout][Cin][T]
+  const float* bias;   // FWD
+  const float* mask;   // BDATA: the layer input (gradient zero where it is <= 0: the ReLU before it); may be null
+  int relu, beta;      // FWD: ReLU on the output; BDATA: add to C
+  int M, N, K, Kp;     // GEMM sizes; Kp: row length of A
+  int Cb, Cbp;         // channels of the B tensor (FWD, BWGT: Cin; BDATA: Cout) and Cb rounded up to 8
+  int D, H, W, oD, oH, oW, ks, stride, T;
+  int k_per_split;     // BWGT: voxels per workgroup
+  int vec;             // BWGT: K % 4 == 0 (dY rows are float4-aligned)
 };
+
+__device__ __forceinline__ void tap3(int t, int ks, int& dz, int& dy, int& dx)
+{
+  if (ks == 1) {
+    dz = dy = dx = 0;
+  } else {
+    dz = t / 9 - 1;
+    dy = (t / 3) % 3 - 1;
+    dx = t % 3 - 1;
+  }
+}
+
+// q with s q + d = p (stride 1 or 2); false if none
+__device__ __forceinline__ bool src_of(int p, int d, int s, int n, int& q)
+{
+  const int v = p - d;
+  if (v < 0) return false;
+  if (s == 2) {
+    if (v & 1) return false;
+    q = v >> 1;
+  } else {
+    q = v;
+  }
+  return q < n;
+}
+
+template <int MODE, int MT>
+__global__ __launch_bounds__(256) void k_conv(ConvK a)
+{
+  __shared__ floatx4 red[4][MT][64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = lane & 15, kg = lane >> 4;
+  const int m0 = blockIdx.y * (16 * MT);
+  const int n = blockIdx.x * 16 + r;                 // this lane's B column
+  const bool nvalid = n < a.N;
+  const size_t vin = (size_t)a.D * a.H * a.W, vout = (size_t)a.oD * a.oH * a.oW;
+  floatx4 acc[MT];
+#pragma unroll
+  for (int i = 0; i < MT; i++) acc[i] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  // per-lane column coordinates
+  int cz = 0, cy = 0, cx = 0, bt = 0, bci = 0;
+  if (MODE == FWD) {
+    cx = n % a.oW;
+    cy = (n / a.oW) % a.oH;
+    cz = n / (a.oW * a.oH);
+  } else if (MODE == BDATA) {
+    cx = n % a.W;
+    cy = (n / a.W) % a.H;
+    cz = n / (a.W * a.H);
+  } else {
+    bt = n / a.Cb;
+    bci = n - bt * a.Cb;
+  }
+  int kb = 0, ke = a.K;
+  if (MODE == BWGT) {
+    kb = blockIdx.z * a.k_per_split;
+    ke = min(a.K, kb + a.k_per_split);
+  }
+  for (int k0 = kb + wave * 32; k0 < ke; k0 += 128) {
+    const int k = k0 + kg * 8;
+    bf16x8 b;
+    if (MODE == FWD || MODE == BDATA) {
+      const int t = k / a.Cbp, c0 = k - t * a.Cbp;
+      int dz, dy, dx;
+      tap3(t, a.ks, dz, dy, dx);
+      bool in = nvalid && k < ke;
+      const float* base = nullptr;
+      size_t cstride;
+      if (MODE == FWD) {
+        const int z = cz * a.stride + dz, y = cy * a.stride + dy, x = cx * a.stride + dx;
+        in = in && z >= 0 && y >= 0 && x >= 0 && z < a.D && y < a.H && x < a.W;
+        base = a.X + (size_t)c0 * vin + ((size_t)z * a.H + y) * a.W + x;
+        cstride = vin;
+      } else {
+        int qz = 0, qy = 0, qx = 0;
+        in = in && src_of(cz, dz, a.stride, a.oD, qz) && src_of(cy, dy, a.stride, a.oH, qy) && src_of(cx, dx, a.stride, a.oW, qx);
+        base = a.X + (size_t)c0 * vout + ((size_t)qz * a.oH + qy) * a.oW + qx;
+        cstride = vout;
+      }
+#pragma unroll
+      for (int e = 0; e < 8; e++) b[e] = to_bf16((in && c0 + e < a.Cb) ? base[e * cstride] : 0.0f);
+#pragma unroll
+      for (int i = 0; i < MT; i++) {
+        const bf16x8 av = *(const bf16x8*)(a.A + (size_t)(m0 + 16 * i + r) * a.Kp + k);
+        acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b, acc[i], 0, 0, 0);
+      }
+    } else {
+      // B: X[ci][s q + d_t] for q = k .. k + 7
+      int dz, dy, dx;
+      tap3(bt, a.ks, dz, dy, dx);
+      int ox = k % a.oW, oy = (k / a.oW) % a.oH, oz = k / (a.oW * a.oH);
+      const float* xc = a.X + (size_t)bci * vin;
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const int z = oz * a.stride + dz, y = oy * a.stride + dy, x = ox * a.stride + dx;
+        const bool in = nvalid && k + e < ke && z >= 0 && y >= 0 && x >= 0 && z < a.D && y < a.H && x < a.W;
+        b[e] = to_bf16(in ? xc[((size_t)z * a.H + y) * a.W + x] : 0.0f);
+        if (++ox == a.oW) {
+          ox = 0;
+          if (++oy == a.oH) {
+            oy = 0;
+            ++oz;
+          }
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < MT; i++) {
+        const int m = m0 + 16 * i + r;
+        bf16x8 av;
+        if (m < a.M && a.vec && k + 8 <= ke) {
+          const float4* p4 = (const float4*)(a.Ad + (size_t)m * a.K + k);
+          const float4 u = p4[0], w = p4[1];
+          av[0] = to_bf16(u.x); av[1] = to_bf16(u.y); av[2] = to_bf16(u.z); av[3] = to_bf16(u.w);
+          av[4] = to_bf16(w.x); av[5] = to_bf16(w.y); av[6] = to_bf16(w.z); av[7] = to_bf16(w.w);
+        } else {
+#pragma unroll
+          for (int e = 0; e < 8; e++) av[e] = to_bf16((m < a.M && k + e < ke) ? a.Ad[(size_t)m * a.K + k + e] : 0.0f);
+        }
+        acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b, acc[i], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < MT; i++) red[wave][i][lane] = acc[i];
+  __syncthreads();
+  if (wave != 0) return;
+#pragma unroll
+  for (int i = 0; i < MT; i++) {
+    const floatx4 s = red[0][i][lane] + red[1][i][lane] + red[2][i][lane] + red[3][i][lane];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int row = m0 + 16 * i + kg * 4 + v;
+      if (row >= a.M || !nvalid) continue;
+      if (MODE == FWD) {
+        float o = s[v] + a.bias[row];
+        if (a.relu) o = fmaxf(o, 0.0f);
+        a.C[(size_t)row * a.N + n] = o;
+      } else if (MODE == BDATA) {
+        const size_t idx = (size_t)row * a.N + n;
+        float o = s[v] + (a.beta ? a.C[idx] : 0.0f);
+        if (a.mask && !(a.mask[idx] > 0.0f)) o = 0.0f;
+        a.C[idx] = o;
+      } else {
+        atomicAdd(&a.C[((size_t)row * a.Cb + bci) * a.T + bt], s[v]);
+      }
+    }
+  }
+}
+
+inline int mt_for(int M)
+{
+  const int mt = (M + 15) / 16;
+  if (mt <= 4) return mt;
+  for (int c = 4; c >= 2; c--)
+    if (mt % c == 0) return c;
+  return 4;
+}
+inline int rows_for(int M)
+{
+  const int t = 16 * mt_for(M);
+  return (M + t - 1) / t * t;
+}
+inline int rup(int x, int m) { return (x + m - 1) / m * m; }
+
+template <int MODE>
+hipError_t launch_conv(const ConvK& a, hipStream_t st)
+{
+  const int mt = mt_for(a.M);
+  dim3 grid((a.N + 15) / 16, (a.M + 16 * mt - 1) / (16 * mt), 1);
+  ConvK b = a;
+  if (MODE == BWGT) {
+    const long long xy = (long long)grid.x * grid.y;
+    int split = (int)std::max<long long>(1, std::min<long long>(2048 / std::max<long long>(1, xy), (a.K + 255) / 256));
+    int kps = (a.K + split - 1) / split;
+    kps = rup(kps, 128);
+    split = (a.K + kps - 1) / kps;
+    grid.z = split;
+    b.k_per_split = kps;
+    b.vec = (a.K % 4 == 0) ? 1 : 0;
+  }
+  switch (mt) {
+    case 1: hipLaunchKernelGGL((k_conv<MODE, 1>), grid, dim3(256), 0, st, b); break;
+    case 2: hipLaunchKernelGGL((k_conv<MODE, 2>), grid, dim3(256), 0, st, b); break;
+    case 3: hipLaunchKernelGGL((k_conv<MODE, 3>), grid, dim3(256), 0, st, b); break;
+    default: hipLaunchKernelGGL((k_conv<MODE, 4>), grid, dim3(256), 0, st, b); break;
+  }
+  return hipGetLastError();
+}
+
+// BF16 weight copies: out[m][t * Cp + c] = W[co][ci][t] with (co, ci) = (m, c) (forward) or
+// (c, m) (transposed, for the input gradient); zero outside the real channels.
+__global__ void k_derive(const float* __restrict__ w, __bf16* __restrict__ out, int rows, int Kp, int Cout, int Cin, int T, int Cp,
+                         int transpose)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)rows * Kp) return;
+  const int m = (int)(i / Kp), k = (int)(i % Kp);
+  const int t = k / Cp, c = k - t * Cp;
+  const int co = transpose ? c : m, ci = transpose ? m : c;
+  const bool ok = t < T && co < Cout && ci < Cin;
+  out[i] = to_bf16(ok ? w[((size_t)co * Cin + ci) * T + t] : 0.0f);
+}
 
 // ------------------------------------------------------------------ elementwise kernels
 
-__global__ void k_relu(float* __restrict__ x, size_t n)
-{
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && !(x[i] > 0.0f)) x[i] = 0.0f;
-}
-// g *= (y > 0): ReLU backward from its output
-__global__ void k_relu_bwd(float* __restrict__ g, const float* __restrict__ y, size_t n)
-{
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && !(y[i] > 0.0f)) g[i] = 0.0f;
-}
 // nearest x2 upsampling of [C][d][h][w] into [C][2d][2h][2w]
 __global__ void k_upsample(const float* __restrict__ x, float* __restrict__ y, int C, int d, int h, int w)
 {
@@ -231,11 +441,6 @@ __global__ void k_upsample_bwd(const float* __restrict__ gy, float* __restrict__
     for (int b = 0; b < 2; b++)
       for (int e = 0; e < 2; e++) s += gy[(((size_t)c * D2 + 2 * zz + a) * H2 + 2 * yy + b) * W2 + 2 * xx + e];
   gx[i] = s;
-}
-__global__ void k_add(float* __restrict__ y, const float* __restrict__ x, size_t n)
-{
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] += x[i];
 }
 // per-channel sum (bias gradient): one workgroup per channel
 __global__ void k_channel_sum(const float* __restrict__ g, float* __restrict__ out, int vox)
@@ -288,15 +493,23 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
   v[i] = vi;
   p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
+__global__ void k_zero(float* __restrict__ p, size_t n)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0f;
+}
 
 inline unsigned blocks_for(size_t n) { return (unsigned)((n + 255) / 256); }
 
 // ------------------------------------------------------------------ the network
 
-// One convolution layer: W [Cout][Cin*T] and b [Cout] at offsets of the flat parameter vector.
+// One convolution layer: W [Cout][Cin][T] and b [Cout] at offsets of the flat parameter
+// vector; BF16 copies at offsets of the BF16 buffer.
 struct Conv {
   int cin, cout, ks, stride;
-  size_t w, b;   // offsets into the parameter vector
+  size_t w, b;                 // offsets into the parameter vector
+  size_t wf, wb;               // offsets of the forward / transposed BF16 copies
+  int rows_f, kp_f, rows_b, kp_b;
 };
 
 struct Vol {
@@ -313,14 +526,15 @@ struct Vol {
 struct mcpet_surrogate {
   int D, H, W, cin, c;
   Conv L[9];
-  size_t n_params = 0;
+  size_t n_params = 0, n_bf16 = 0;
   float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
+  __bf16* wbf = nullptr;
   // activations (post-ReLU) and their gradients; concat buffers hold the skip tensors
   Vol vx, va1, vcat1, vd1, vcat2, vd2, vb3, ve2, ve1, vr;
   float *x0 = nullptr, *a1 = nullptr, *cat1 = nullptr, *d1 = nullptr, *cat2 = nullptr, *d2 = nullptr, *b3 = nullptr,
         *e2 = nullptr, *e1 = nullptr, *r = nullptr, *out = nullptr, *tgt = nullptr;
   float *ga1 = nullptr, *gcat1 = nullptr, *gd1 = nullptr, *gcat2 = nullptr, *gd2 = nullptr, *gb3 = nullptr, *ge2 = nullptr,
-        *ge1 = nullptr, *gr = nullptr, *tmp1 = nullptr, *tmp2 = nullptr, *sse = nullptr;
+        *ge1 = nullptr, *gr = nullptr, *sse = nullptr;
   long long t_adam = 0;
   hipStream_t st = nullptr;
   std::vector<void*> allocs;
@@ -343,145 +557,169 @@ mcpet_status serr(mcpet_surrogate* s, mcpet_status st, const std::string& m)
     if (e_ != hipSuccess) return serr(s, MCPET_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-hipError_t dalloc_f(mcpet_surrogate* s, float** p, size_t n)
+template <class T>
+hipError_t dalloc_t(mcpet_surrogate* s, T** p, size_t n)
 {
-  hipError_t e = hipMalloc((void**)p, sizeof(float) * std::max<size_t>(1, n));
+  hipError_t e = hipMalloc((void**)p, sizeof(T) * std::max<size_t>(1, n));
   if (e == hipSuccess) {
     s->allocs.push_back(*p);
-    e = hipMemset(*p, 0, sizeof(float) * std::max<size_t>(1, n));
+    e = hipMemset(*p, 0, sizeof(T) * std::max<size_t>(1, n));
   }
   return e;
 }
 
-Geom geom_of(const Vol& in, const Vol& out, const Conv& l) { return Geom{in.C, in.D, in.H, in.W, out.D, out.H, out.W, l.ks, l.stride}; }
-
-// y = conv(x) + b (no activation)
-hipError_t conv_fwd(mcpet_surrogate* s, const Conv& l, const Vol& vin, const float* x, const Vol& vout, float* y)
+// refresh the BF16 weight copies from the FP32 parameters
+hipError_t derive(mcpet_surrogate* s)
 {
-  const int T = l.ks * l.ks * l.ks;
-  const int K = l.cin * T, N = (int)vout.vox();
-  s->flops_fwd += 2.0 * l.cout * (double)K * N;
-  return gemm(Dense{s->p + l.w, l.cout, K, K}, Im2col{x, geom_of(vin, vout, l), K, N}, y, l.cout, N, K, N, s->p + l.b, 1, false,
-              s->st);
+  for (int i = 0; i < 9; i++) {
+    const Conv& l = s->L[i];
+    const int T = l.ks * l.ks * l.ks;
+    size_t n = (size_t)l.rows_f * l.kp_f;
+    hipLaunchKernelGGL(k_derive, dim3(blocks_for(n)), dim3(256), 0, s->st, s->p + l.w, s->wbf + l.wf, l.rows_f, l.kp_f, l.cout,
+                       l.cin, T, rup(l.cin, 8), 0);
+    n = (size_t)l.rows_b * l.kp_b;
+    hipLaunchKernelGGL(k_derive, dim3(blocks_for(n)), dim3(256), 0, s->st, s->p + l.w, s->wbf + l.wb, l.rows_b, l.kp_b, l.cout,
+                       l.cin, T, rup(l.cout, 8), 1);
+  }
+  return hipGetLastError();
 }
-// gx = dL/dx (overwritten); gW, gb accumulated
-hipError_t conv_bwd(mcpet_surrogate* s, const Conv& l, const Vol& vin, const float* x, const Vol& vout, const float* gy, float* gx)
+
+ConvK conv_args(const Conv& l, const Vol& vin, const Vol& vout)
 {
-  const int T = l.ks * l.ks * l.ks;
-  const Geom g = geom_of(vin, vout, l);
-  const int Nout = (int)vout.vox(), Nin = (int)vin.vox();
-  hipError_t e = hipSuccess;
-  // weight gradient: M = cout, N = cin*T, K = output voxels (split over workgroups)
-  const int split = std::max(1, Nout / 1024);
-  e = gemm(Dense{gy, l.cout, Nout, Nout}, Im2colT{x, g, Nout, l.cin * T}, s->g + l.w, l.cout, l.cin * T, Nout, l.cin * T, nullptr,
-           split, true, s->st);
+  ConvK a{};
+  a.D = vin.D;
+  a.H = vin.H;
+  a.W = vin.W;
+  a.oD = vout.D;
+  a.oH = vout.H;
+  a.oW = vout.W;
+  a.ks = l.ks;
+  a.stride = l.stride;
+  a.T = l.ks * l.ks * l.ks;
+  return a;
+}
+
+// y = act(conv(x) + b)
+hipError_t conv_fwd(mcpet_surrogate* s, const Conv& l, const Vol& vin, const float* x, const Vol& vout, float* y, bool relu)
+{
+  ConvK a = conv_args(l, vin, vout);
+  a.A = s->wbf + l.wf;
+  a.X = x;
+  a.C = y;
+  a.bias = s->p + l.b;
+  a.relu = relu ? 1 : 0;
+  a.M = l.cout;
+  a.N = (int)vout.vox();
+  a.Cb = l.cin;
+  a.Cbp = rup(l.cin, 8);
+  a.K = a.T * a.Cbp;
+  a.Kp = l.kp_f;
+  s->flops_fwd += 2.0 * l.cout * (double)l.cin * a.T * a.N;
+  return launch_conv<FWD>(a, s->st);
+}
+// gx (+)= dL/dx of the layer input, zeroed where mask <= 0
+hipError_t conv_bwd_data(mcpet_surrogate* s, const Conv& l, const Vol& vin, const Vol& vout, const float* gy, float* gx,
+                         const float* mask, bool add)
+{
+  ConvK a = conv_args(l, vin, vout);
+  a.A = s->wbf + l.wb;
+  a.X = gy;
+  a.C = gx;
+  a.mask = mask;
+  a.beta = add ? 1 : 0;
+  a.M = l.cin;
+  a.N = (int)vin.vox();
+  a.Cb = l.cout;
+  a.Cbp = rup(l.cout, 8);
+  a.K = a.T * a.Cbp;
+  a.Kp = l.kp_b;
+  return launch_conv<BDATA>(a, s->st);
+}
+// weight and bias gradients accumulated from gy (dL/d pre-activation) and the layer input x
+hipError_t conv_bwd_weights(mcpet_surrogate* s, const Conv& l, const Vol& vin, const float* x, const Vol& vout, const float* gy)
+{
+  ConvK a = conv_args(l, vin, vout);
+  a.Ad = gy;
+  a.X = x;
+  a.C = s->g + l.w;
+  a.M = l.cout;
+  a.N = a.T * l.cin;
+  a.K = (int)vout.vox();
+  a.Cb = l.cin;
+  a.Cbp = l.cin;
+  hipError_t e = launch_conv<BWGT>(a, s->st);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_channel_sum, dim3(l.cout), dim3(256), 0, s->st, gy, s->g + l.b, Nout);
-  e = hipGetLastError();
-  if (e != hipSuccess || !gx) return e;
-  // input gradient: M = cin, N = input voxels, K = cout*T
-  return gemm(WeightT{s->p + l.w, l.cin, l.cout, T}, Col2im{gy, g, l.cout * T, Nin}, gx, l.cin, Nin, l.cout * T, Nin, nullptr, 1,
-              false, s->st);
-}
-hipError_t relu(mcpet_surrogate* s, float* x, size_t n)
-{
-  hipLaunchKernelGGL(k_relu, dim3(blocks_for(n)), dim3(256), 0, s->st, x, n);
+  hipLaunchKernelGGL(k_channel_sum, dim3(l.cout), dim3(256), 0, s->st, gy, s->g + l.b, (int)vout.vox());
   return hipGetLastError();
 }
-hipError_t relu_bwd(mcpet_surrogate* s, float* gx, const float* y, size_t n)
+hipError_t upsample(mcpet_surrogate* s, const Vol& v, const float* x, float* y)
 {
-  hipLaunchKernelGGL(k_relu_bwd, dim3(blocks_for(n)), dim3(256), 0, s->st, gx, y, n);
+  hipLaunchKernelGGL(k_upsample, dim3(blocks_for(v.size() * 8)), dim3(256), 0, s->st, x, y, v.C, v.D, v.H, v.W);
+  return hipGetLastError();
+}
+hipError_t upsample_bwd(mcpet_surrogate* s, const Vol& v, const float* gy, float* gx)
+{
+  hipLaunchKernelGGL(k_upsample_bwd, dim3(blocks_for(v.size())), dim3(256), 0, s->st, gy, gx, v.C, v.D, v.H, v.W);
   return hipGetLastError();
 }
 
-// Forward of one sample already in s->x0; r = residual, out = x0[0] + r.
+// Forward of one sample already in s->x0 (r = the residual).
 hipError_t forward(mcpet_surrogate* s)
 {
   hipError_t e = hipSuccess;
-  const Vol& vx = s->vx;
-  Vol vb1{s->c, s->D, s->H, s->W}, vb2{2 * s->c, s->D / 2, s->H / 2, s->W / 2};
+  const Vol vb1{s->c, s->D, s->H, s->W}, vb2{2 * s->c, s->D / 2, s->H / 2, s->W / 2};
   float* b1 = s->cat1 + (size_t)2 * s->c * vb1.vox();   // enc1 output lives in the concat buffer
   float* b2 = s->cat2 + (size_t)4 * s->c * vb2.vox();
   s->flops_fwd = 0.0;
 #define STEP(x) \
   if (e == hipSuccess) e = (x)
-  STEP(conv_fwd(s, s->L[0], vx, s->x0, s->va1, s->a1));
-  STEP(relu(s, s->a1, s->va1.size()));
-  STEP(conv_fwd(s, s->L[1], s->va1, s->a1, vb1, b1));
-  STEP(relu(s, b1, vb1.size()));
-  STEP(conv_fwd(s, s->L[2], vb1, b1, s->vd1, s->d1));
-  STEP(relu(s, s->d1, s->vd1.size()));
-  STEP(conv_fwd(s, s->L[3], s->vd1, s->d1, vb2, b2));
-  STEP(relu(s, b2, vb2.size()));
-  STEP(conv_fwd(s, s->L[4], vb2, b2, s->vd2, s->d2));
-  STEP(relu(s, s->d2, s->vd2.size()));
-  STEP(conv_fwd(s, s->L[5], s->vd2, s->d2, s->vb3, s->b3));
-  STEP(relu(s, s->b3, s->vb3.size()));
-  if (e == hipSuccess) {   // upsample b3 into the first 4c channels of cat2
-    hipLaunchKernelGGL(k_upsample, dim3(blocks_for(s->vb3.size() * 8)), dim3(256), 0, s->st, s->b3, s->cat2, s->vb3.C, s->vb3.D,
-                       s->vb3.H, s->vb3.W);
-    e = hipGetLastError();
-  }
-  STEP(conv_fwd(s, s->L[6], s->vcat2, s->cat2, s->ve2, s->e2));
-  STEP(relu(s, s->e2, s->ve2.size()));
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_upsample, dim3(blocks_for(s->ve2.size() * 8)), dim3(256), 0, s->st, s->e2, s->cat1, s->ve2.C, s->ve2.D,
-                       s->ve2.H, s->ve2.W);
-    e = hipGetLastError();
-  }
-  STEP(conv_fwd(s, s->L[7], s->vcat1, s->cat1, s->ve1, s->e1));
-  STEP(relu(s, s->e1, s->ve1.size()));
-  STEP(conv_fwd(s, s->L[8], s->ve1, s->e1, s->vr, s->r));
+  STEP(conv_fwd(s, s->L[0], s->vx, s->x0, s->va1, s->a1, true));
+  STEP(conv_fwd(s, s->L[1], s->va1, s->a1, vb1, b1, true));
+  STEP(conv_fwd(s, s->L[2], vb1, b1, s->vd1, s->d1, true));
+  STEP(conv_fwd(s, s->L[3], s->vd1, s->d1, vb2, b2, true));
+  STEP(conv_fwd(s, s->L[4], vb2, b2, s->vd2, s->d2, true));
+  STEP(conv_fwd(s, s->L[5], s->vd2, s->d2, s->vb3, s->b3, true));
+  STEP(upsample(s, s->vb3, s->b3, s->cat2));           // first 4c channels of cat2
+  STEP(conv_fwd(s, s->L[6], s->vcat2, s->cat2, s->ve2, s->e2, true));
+  STEP(upsample(s, s->ve2, s->e2, s->cat1));           // first 2c channels of cat1
+  STEP(conv_fwd(s, s->L[7], s->vcat1, s->cat1, s->ve1, s->e1, true));
+  STEP(conv_fwd(s, s->L[8], s->ve1, s->e1, s->vr, s->r, false));
 #undef STEP
   return e;
 }
 
-// Backward of one sample from gr (= dL/dr); accumulates parameter gradients.
+// Backward of one sample from gr (= dL/dr); accumulates the parameter gradients. The ReLU
+// derivative is applied where each gradient is produced (mask = the post-ReLU activation;
+// the upsampled and concatenated copies carry the same sign as their sources).
 hipError_t backward(mcpet_surrogate* s)
 {
   hipError_t e = hipSuccess;
-  Vol vb1{s->c, s->D, s->H, s->W}, vb2{2 * s->c, s->D / 2, s->H / 2, s->W / 2};
-  float* b1 = s->cat1 + (size_t)2 * s->c * vb1.vox();
-  float* b2 = s->cat2 + (size_t)4 * s->c * vb2.vox();
+  const Vol vb1{s->c, s->D, s->H, s->W}, vb2{2 * s->c, s->D / 2, s->H / 2, s->W / 2};
+  const float* b1 = s->cat1 + (size_t)2 * s->c * vb1.vox();
+  const float* b2 = s->cat2 + (size_t)4 * s->c * vb2.vox();
   float* gb1 = s->gcat1 + (size_t)2 * s->c * vb1.vox();
   float* gb2 = s->gcat2 + (size_t)4 * s->c * vb2.vox();
 #define STEP(x) \
   if (e == hipSuccess) e = (x)
-  STEP(conv_bwd(s, s->L[8], s->ve1, s->e1, s->vr, s->gr, s->ge1));
-  STEP(relu_bwd(s, s->ge1, s->e1, s->ve1.size()));
-  STEP(conv_bwd(s, s->L[7], s->vcat1, s->cat1, s->ve1, s->ge1, s->gcat1));   // -> g(up1), g(b1 skip)
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_upsample_bwd, dim3(blocks_for(s->ve2.size())), dim3(256), 0, s->st, s->gcat1, s->ge2, s->ve2.C, s->ve2.D,
-                       s->ve2.H, s->ve2.W);
-    e = hipGetLastError();
-  }
-  STEP(relu_bwd(s, s->ge2, s->e2, s->ve2.size()));
-  STEP(conv_bwd(s, s->L[6], s->vcat2, s->cat2, s->ve2, s->ge2, s->gcat2));   // -> g(up2), g(b2 skip)
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_upsample_bwd, dim3(blocks_for(s->vb3.size())), dim3(256), 0, s->st, s->gcat2, s->gb3, s->vb3.C, s->vb3.D,
-                       s->vb3.H, s->vb3.W);
-    e = hipGetLastError();
-  }
-  STEP(relu_bwd(s, s->gb3, s->b3, s->vb3.size()));
-  STEP(conv_bwd(s, s->L[5], s->vd2, s->d2, s->vb3, s->gb3, s->gd2));
-  STEP(relu_bwd(s, s->gd2, s->d2, s->vd2.size()));
-  STEP(conv_bwd(s, s->L[4], vb2, b2, s->vd2, s->gd2, s->tmp2));             // main path into b2
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_add, dim3(blocks_for(vb2.size())), dim3(256), 0, s->st, gb2, s->tmp2, vb2.size());
-    e = hipGetLastError();
-  }
-  STEP(relu_bwd(s, gb2, b2, vb2.size()));
-  STEP(conv_bwd(s, s->L[3], s->vd1, s->d1, vb2, gb2, s->gd1));
-  STEP(relu_bwd(s, s->gd1, s->d1, s->vd1.size()));
-  STEP(conv_bwd(s, s->L[2], vb1, b1, s->vd1, s->gd1, s->tmp1));             // main path into b1
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_add, dim3(blocks_for(vb1.size())), dim3(256), 0, s->st, gb1, s->tmp1, vb1.size());
-    e = hipGetLastError();
-  }
-  STEP(relu_bwd(s, gb1, b1, vb1.size()));
-  STEP(conv_bwd(s, s->L[1], s->va1, s->a1, vb1, gb1, s->ga1));
-  STEP(relu_bwd(s, s->ga1, s->a1, s->va1.size()));
-  STEP(conv_bwd(s, s->L[0], s->vx, s->x0, s->va1, s->ga1, nullptr));
+  STEP(conv_bwd_weights(s, s->L[8], s->ve1, s->e1, s->vr, s->gr));
+  STEP(conv_bwd_data(s, s->L[8], s->ve1, s->vr, s->gr, s->ge1, s->e1, false));
+  STEP(conv_bwd_weights(s, s->L[7], s->vcat1, s->cat1, s->ve1, s->ge1));
+  STEP(conv_bwd_data(s, s->L[7], s->vcat1, s->ve1, s->ge1, s->gcat1, s->cat1, false));   // g(up1) and g(b1 skip)
+  STEP(upsample_bwd(s, s->ve2, s->gcat1, s->ge2));
+  STEP(conv_bwd_weights(s, s->L[6], s->vcat2, s->cat2, s->ve2, s->ge2));
+  STEP(conv_bwd_data(s, s->L[6], s->vcat2, s->ve2, s->ge2, s->gcat2, s->cat2, false));   // g(up2) and g(b2 skip)
+  STEP(upsample_bwd(s, s->vb3, s->gcat2, s->gb3));
+  STEP(conv_bwd_weights(s, s->L[5], s->vd2, s->d2, s->vb3, s->gb3));
+  STEP(conv_bwd_data(s, s->L[5], s->vd2, s->vb3, s->gb3, s->gd2, s->d2, false));
+  STEP(conv_bwd_weights(s, s->L[4], vb2, b2, s->vd2, s->gd2));
+  STEP(conv_bwd_data(s, s->L[4], vb2, s->vd2, s->gd2, gb2, b2, true));                  // + the skip's gradient
+  STEP(conv_bwd_weights(s, s->L[3], s->vd1, s->d1, vb2, gb2));
+  STEP(conv_bwd_data(s, s->L[3], s->vd1, vb2, gb2, s->gd1, s->d1, false));
+  STEP(conv_bwd_weights(s, s->L[2], vb1, b1, s->vd1, s->gd1));
+  STEP(conv_bwd_data(s, s->L[2], vb1, s->vd1, s->gd1, gb1, b1, true));                  // + the skip's gradient
+  STEP(conv_bwd_weights(s, s->L[1], s->va1, s->a1, vb1, gb1));
+  STEP(conv_bwd_data(s, s->L[1], s->va1, vb1, gb1, s->ga1, s->a1, false));
+  STEP(conv_bwd_weights(s, s->L[0], s->vx, s->x0, s->va1, s->ga1));
 #undef STEP
   return e;
 }
@@ -493,8 +731,10 @@ extern "C" mcpet_status mcpet_surrogate_create(int32_t D, int32_t H, int32_t W, 
 {
   if (!out) return serr(nullptr, MCPET_E_ARG, "mcpet_surrogate_create: null output pointer");
   *out = nullptr;
-  if (D < 4 || H < 4 || W < 4 || D % 4 || H % 4 || W % 4 || in_channels < 1 || width < 1)
-    return serr(nullptr, MCPET_E_ARG, "mcpet_surrogate_create: sizes must be multiples of 4, channels >= 1");
+  if (D < 4 || H < 4 || W < 4 || D % 4 || H % 4 || W % 4 || in_channels < 1 || width < 1 || in_channels > 64 || width > 256 ||
+      (long long)D * H * W > (1LL << 28))
+    return serr(nullptr, MCPET_E_ARG, "mcpet_surrogate_create: sizes must be multiples of 4 (at most 2^28 voxels), "
+                                      "1 <= channels <= 64, 1 <= width <= 256");
   mcpet_surrogate* s = new mcpet_surrogate();
   s->D = D;
   s->H = H;
@@ -504,7 +744,7 @@ extern "C" mcpet_status mcpet_surrogate_create(int32_t D, int32_t H, int32_t W, 
   const int c = width;
   const int spec[9][4] = {{in_channels, c, 3, 1}, {c, c, 3, 1},         {c, 2 * c, 3, 2},     {2 * c, 2 * c, 3, 1}, {2 * c, 4 * c, 3, 2},
                           {4 * c, 4 * c, 3, 1},    {6 * c, 2 * c, 3, 1}, {3 * c, c, 3, 1},     {c, 1, 1, 1}};
-  size_t off = 0;
+  size_t off = 0, offb = 0;
   for (int i = 0; i < 9; i++) {
     Conv& l = s->L[i];
     l.cin = spec[i][0];
@@ -516,8 +756,17 @@ extern "C" mcpet_status mcpet_surrogate_create(int32_t D, int32_t H, int32_t W, 
     off += (size_t)l.cout * l.cin * T;
     l.b = off;
     off += (size_t)l.cout;
+    l.rows_f = rows_for(l.cout);
+    l.kp_f = rup(T * rup(l.cin, 8), 32);
+    l.rows_b = rows_for(l.cin);
+    l.kp_b = rup(T * rup(l.cout, 8), 32);
+    l.wf = offb;
+    offb += (size_t)l.rows_f * l.kp_f;
+    l.wb = offb;
+    offb += (size_t)l.rows_b * l.kp_b;
   }
   s->n_params = off;
+  s->n_bf16 = offb;
   s->vx = {in_channels, D, H, W};
   s->va1 = {c, D, H, W};
   s->vcat1 = {3 * c, D, H, W};
@@ -538,15 +787,17 @@ extern "C" mcpet_status mcpet_surrogate_create(int32_t D, int32_t H, int32_t W, 
             {&s->e1, s->ve1.size()}, {&s->r, s->vr.size()}, {&s->out, s->vr.size()}, {&s->tgt, s->vr.size()},
             {&s->ga1, s->va1.size()}, {&s->gcat1, s->vcat1.size()}, {&s->gd1, s->vd1.size()}, {&s->gcat2, s->vcat2.size()},
             {&s->gd2, s->vd2.size()}, {&s->gb3, s->vb3.size()}, {&s->ge2, s->ve2.size()}, {&s->ge1, s->ve1.size()},
-            {&s->gr, s->vr.size()}, {&s->tmp1, s->va1.size()}, {&s->tmp2, s->vd1.size()}, {&s->sse, 1}};
+            {&s->gr, s->vr.size()}, {&s->sse, 1}};
   for (auto& a : al)
-    if (e == hipSuccess) e = dalloc_f(s, a.p, a.n);
+    if (e == hipSuccess) e = dalloc_t(s, a.p, a.n);
+  if (e == hipSuccess) e = dalloc_t(s, &s->wbf, s->n_bf16);
   if (e != hipSuccess) {
     std::string m = std::string("mcpet_surrogate_create: ") + hipGetErrorString(e);
     mcpet_surrogate_destroy(s);
     return serr(nullptr, MCPET_E_NOMEM, m);
   }
-  // He-normal weights from a seeded generator (splitmix64 + Box-Muller), zero biases
+  // He-normal weights from a seeded generator (splitmix64 + Box-Muller), zero biases; the 1x1x1
+  // output layer is zero, so the untrained network returns its input channel 0
   std::vector<float> hp(off, 0.0f);
   uint64_t z = seed ? seed : 0x9E3779B97F4A7C15ull;
   auto next = [&]() {
@@ -560,11 +811,13 @@ extern "C" mcpet_status mcpet_surrogate_create(int32_t D, int32_t H, int32_t W, 
   for (int i = 0; i < 9; i++) {
     const Conv& l = s->L[i];
     const int T = l.ks * l.ks * l.ks;
-    const double sd = std::sqrt(2.0 / (l.cin * T)) * (i == 8 ? 0.1 : 1.0);
+    const double sd = i == 8 ? 0.0 : std::sqrt(2.0 / (l.cin * T));   // the output layer starts at zero: identity
     for (size_t k = 0; k < (size_t)l.cout * l.cin * T; k++)
       hp[l.w + k] = (float)(sd * std::sqrt(-2.0 * std::log(next())) * std::cos(6.283185307179586 * next()));
   }
   e = hipMemcpy(s->p, hp.data(), sizeof(float) * off, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = derive(s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->st);
   if (e != hipSuccess) {
     mcpet_surrogate_destroy(s);
     return serr(nullptr, MCPET_E_HIP, "mcpet_surrogate_create: upload");
@@ -578,13 +831,17 @@ extern "C" int64_t mcpet_surrogate_param_count(const mcpet_surrogate* s) { retur
 extern "C" mcpet_status mcpet_surrogate_get_params(mcpet_surrogate* s, float* out)
 {
   if (!s || !out) return serr(s, MCPET_E_ARG, "mcpet_surrogate_get_params: null argument");
+  SHIP(s, hipStreamSynchronize(s->st));
   SHIP(s, hipMemcpy(out, s->p, sizeof(float) * s->n_params, hipMemcpyDeviceToHost));
   return MCPET_OK;
 }
 extern "C" mcpet_status mcpet_surrogate_set_params(mcpet_surrogate* s, const float* in)
 {
   if (!s || !in) return serr(s, MCPET_E_ARG, "mcpet_surrogate_set_params: null argument");
+  SHIP(s, hipStreamSynchronize(s->st));
   SHIP(s, hipMemcpy(s->p, in, sizeof(float) * s->n_params, hipMemcpyHostToDevice));
+  SHIP(s, derive(s));
+  SHIP(s, hipStreamSynchronize(s->st));
   return MCPET_OK;
 }
 
@@ -606,13 +863,14 @@ extern "C" mcpet_status mcpet_surrogate_forward(mcpet_surrogate* s, const float*
   return MCPET_OK;
 }
 
-// Mean-squared-error gradients of the batch (loss = mean over batch and voxels); grad (host,
-// n_params) may be NULL. Leaves the gradients in the device buffer for an optimiser step.
+// Mean-squared-error gradients of the batch (loss = mean over batch and voxels); leaves them
+// in the device buffer for an optimiser step.
 static mcpet_status batch_gradients(mcpet_surrogate* s, const float* x, const float* target, int32_t batch, float* loss)
 {
   const size_t nx = s->vx.size(), nv = s->vr.size();
-  SHIP(s, hipMemsetAsync(s->g, 0, sizeof(float) * s->n_params, s->st));
-  SHIP(s, hipMemsetAsync(s->sse, 0, sizeof(float), s->st));
+  hipLaunchKernelGGL(k_zero, dim3(blocks_for(s->n_params)), dim3(256), 0, s->st, s->g, s->n_params);
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(256), 0, s->st, s->sse, (size_t)1);
+  SHIP(s, hipGetLastError());
   const float scale = 1.0f / ((float)nv * (float)batch);
   for (int b = 0; b < batch; b++) {
     SHIP(s, hipMemcpyAsync(s->x0, x + b * nx, sizeof(float) * nx, hipMemcpyHostToDevice, s->st));
@@ -652,6 +910,7 @@ extern "C" mcpet_status mcpet_surrogate_train_step(mcpet_surrogate* s, const flo
   hipLaunchKernelGGL(k_adam, dim3(blocks_for(s->n_params)), dim3(256), 0, s->st, s->p, s->g, s->m, s->v, s->n_params, lr, 0.9f,
                      0.999f, 1e-8f, c1, c2);
   SHIP(s, hipGetLastError());
+  SHIP(s, derive(s));
   SHIP(s, hipStreamSynchronize(s->st));
   return MCPET_OK;
 }

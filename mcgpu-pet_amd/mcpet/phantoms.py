@@ -8,6 +8,8 @@ per tissue):
   head         -- 128x128x96 voxels of 2 mm: seed-placed ellipsoids for scalp, fat,
                   skull (bone 1.85 g/cm3), CSF, grey and white matter, sinus air, in air;
                   18F activity grey 4 : white 1 : other tissue 0.5 : air 0; 1e9 decays in 10 s.
+  scatter_phantoms -- 128x128x64 voxels of 2 mm, random bodies, inclusions and lesions:
+                  the scatter surrogate's training and evaluation phantoms.
   sample_cube  -- the reference's own sample phantom (phantom_9x9x9cm.vox), written
                   byte for byte by write_sample_vox.
 """
@@ -237,6 +239,46 @@ def random_phantoms(count=48, seed=20261202, voxel_cm=0.2, histories=(1.0e6, 1.0
         if w.sum() == 0:
             w[mat == 2] = 1.0
         n_decays = 0.5 * float(np.exp(rng.uniform(np.log(histories[0]), np.log(histories[1]))))
+        act = (w * (n_decays / (w.sum() * frac))).astype(np.float32)
+        out.append((mat, dens, act, (voxel_cm,) * 3))
+    return out
+
+
+def scatter_phantoms(count, seed, shape=(64, 128, 128), voxel_cm=0.2, n_decays=1.0e9, acquisition_time_s=1.0,
+                     mean_life=F18_MEAN_LIFE):
+    """Phantoms of the scatter-surrogate workload: 128x128x64 voxels of 2 mm (the activity
+    and attenuation maps the surrogate's training draws from). Each is a body ellipsoid of
+    soft tissue (semi-axes 30-48 % of the field, offset up to 10 %) with 1-4 inclusions of
+    other tissues (bone, lung-like air, fat, brain tissue) and 0-3 hot lesions (3-10x the
+    local activity); tissue activity weights are the head weights times a log-normal factor
+    (sigma 0.5). Activity scaled to n_decays expected decays in acquisition_time_s. Returns a
+    list of (mat, dens, act, voxel_size)."""
+    rng = np.random.default_rng(seed)
+    frac = mean_life * (1.0 - np.exp(-acquisition_time_s / mean_life))
+    nz, ny, nx = shape
+    size = np.array([nx, ny, nz], np.float32) * voxel_cm
+    out = []
+    for _ in range(count):
+        mat = np.ones(shape, np.int32)
+        c0 = (size / 2 * (1.0 + rng.uniform(-0.1, 0.1, 3))).astype(np.float32)
+        body = (size * rng.uniform(0.3, 0.48, 3)).astype(np.float32)
+        body_mask = _ellipsoid(shape, voxel_cm, c0, body)
+        mat[body_mask] = 2
+        for _ in range(int(rng.integers(1, 5))):
+            c = (c0 + body * rng.uniform(-0.6, 0.6, 3)).astype(np.float32)
+            r = (body * rng.uniform(0.1, 0.4, 3)).astype(np.float32)
+            mat[_ellipsoid(shape, voxel_cm, c, r) & body_mask] = int(rng.choice([3, 4, 5, 6, 7, 8]))
+        dens = np.zeros(shape, np.float32)
+        w = np.zeros(shape, np.float64)
+        for slot, _, _, d, a in HEAD_TISSUES:
+            dens[mat == slot] = d
+            w[mat == slot] = a * float(np.exp(rng.normal(0.0, 0.5)))
+        for _ in range(int(rng.integers(0, 4))):
+            c = (c0 + body * rng.uniform(-0.5, 0.5, 3)).astype(np.float32)
+            lesion = _ellipsoid(shape, voxel_cm, c, np.full(3, rng.uniform(0.4, 1.2), np.float32)) & body_mask
+            w[lesion] *= rng.uniform(3.0, 10.0)
+        if w.sum() == 0:
+            w[mat == 2] = 1.0
         act = (w * (n_decays / (w.sum() * frac))).astype(np.float32)
         out.append((mat, dens, act, (voxel_cm,) * 3))
     return out

@@ -1,0 +1,262 @@
+"""Learned scatter surrogate: a low-count Monte Carlo run, rebinned on the device, turned into
+a scatter estimate by the 3D U-Net (mcpet_surrogate_*), with a consistency check that falls
+back to Monte Carlo. The reference only runs Monte Carlo (MCGPU-PET.cu:450-495); this module
+has no counterpart there.
+
+Grid (fixed before any training or measurement): the 168-angle x 168-radial x 1293-sinogram
+michelogram (NRAD 168, NANGLES 168, NZS 159, SPAN 11, MRD 79) rebinned single-slice to 159
+ring-sum planes, then summed in groups of 5 planes, 3 angles and 3 radial bins:
+32 x 56 x 56 = 100352 bins (about 8 mm axially, 3.2 degrees, 6.3 mm radially per bin; the finest
+grid on which a 1 %-noise Monte Carlo reference of each evaluation phantom stays affordable).
+
+NRMSE(estimate, reference) = ||estimate - reference||_2 / ||reference||_2 over that grid, both
+per simulated decay. The reference is a high-count Monte Carlo run whose own Poisson noise
+floor sqrt(sum counts) / ||counts|| is reported beside every figure.
+
+Time to 5 % NRMSE:
+  Monte Carlo: the decays N5 = sum(r) / (sum(r^2) 0.05^2) at which a Monte Carlo estimate's
+  expected NRMSE reaches 5 % (r = reference per decay; Poisson counts), simulated and timed
+  (run, wait, rebin), its NRMSE measured against the reference.
+  Surrogate: a Monte Carlo run of N_LOW decays, the device rebinning of its trues and
+  scatter, the U-Net forward; if the consistency check fails (or its NRMSE exceeds 5 %) the
+  time of the Monte Carlo fallback is added.
+"""
+import json
+import os
+import time
+
+import numpy as np
+
+from . import Batch, Simulation, Surrogate, load_materials, phantom_from_arrays, phantoms
+
+GRID = (32, 56, 56)
+N_LOW = 5.0e6              # decays: 1e7 photon histories
+TARGET = 0.05
+WIDTH = 32
+IN_CHANNELS = 2
+DECAYS_PER_SECOND = 1.0e9       # activity of the workload's phantoms: E[decays] = 1e9 per second
+DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+WEIGHTS = os.path.join(DATA_DIR, f"scatter_unet_w{WIDTH}.npy")
+
+
+def config(materials, threads_per_block=1, seed=20261202):
+    """The head scanner (168 x 168 x 1293 sinogram) for phantoms of 1e9 decays per second."""
+    return phantoms.make_config(seed=seed, threads_per_block=threads_per_block, acquisition_time_s=1.0, nrad=168,
+                                materials=materials)
+
+
+def acquisition_ps(n_decays):
+    return int(round(n_decays / DECAYS_PER_SECOND * 1e12))
+
+
+def streams_per_voxel(n_decays, active_voxels):
+    """Decay streams per emitting voxel for a run of n_decays: about 20 decays per stream,
+    1 to 32 streams (the same rule for the surrogate's input and for Monte Carlo)."""
+    return int(max(1, min(32, round(n_decays / (20.0 * max(1, active_voxels))))))
+
+
+def nrmse(est, ref):
+    return float(np.linalg.norm(est - ref) / np.linalg.norm(ref))
+
+
+def poisson_floor(counts):
+    """Expected NRMSE of a Poisson estimate with these expected counts: sqrt(sum) / ||counts||."""
+    counts = np.asarray(counts, np.float64)
+    return float(np.sqrt(counts.sum()) / np.linalg.norm(counts))
+
+
+def decays_for(ref_per_decay, target=TARGET):
+    r = np.asarray(ref_per_decay, np.float64)
+    return float(r.sum() / ((r * r).sum() * target * target))
+
+
+def normalize(scatter_low, trues_low, n_low):
+    """Network input (2 channels: scatter and trues per decay, scaled so that the trues
+    channel has mean 1) and the scale k."""
+    xs = np.asarray(scatter_low, np.float64) / n_low
+    xt = np.asarray(trues_low, np.float64) / n_low
+    k = 1.0 / max(xt.mean(), 1e-30)
+    return np.stack([xs * k, xt * k]).astype(np.float32), k
+
+
+def consistency_z(pred_per_decay, scatter_low, n_low, block=(4, 4, 4), min_mean=20.0):
+    """Chi-square z-score of the observed low-count scatter against the prediction, over
+    blocks of the grid with at least min_mean expected counts: z = (chi2 - dof) / sqrt(2 dof).
+    Large z: the prediction disagrees with the data it was made from (fall back)."""
+    mu = np.asarray(pred_per_decay, np.float64) * n_low
+    n = np.asarray(scatter_low, np.float64)
+    D, H, W = mu.shape
+    b = block
+    mu_b = mu[:D // b[0] * b[0], :H // b[1] * b[1], :W // b[2] * b[2]].reshape(D // b[0], b[0], H // b[1], b[1], W // b[2],
+                                                                                   b[2]).sum(axis=(1, 3, 5))
+    n_b = n[:D // b[0] * b[0], :H // b[1] * b[1], :W // b[2] * b[2]].reshape(D // b[0], b[0], H // b[1], b[1], W // b[2],
+                                                                                b[2]).sum(axis=(1, 3, 5))
+    m = mu_b >= min_mean
+    dof = int(m.sum())
+    if dof == 0:
+        return float("inf")
+    chi2 = float((((n_b - mu_b) ** 2) / mu_b)[m].sum())
+    return (chi2 - dof) / np.sqrt(2.0 * dof)
+
+
+class ScatterEstimator:
+    """The trained U-Net and the estimate / fallback policy."""
+
+    def __init__(self, weights=WEIGHTS, width=WIDTH, grid=GRID, z_max=5.0):
+        self.net = Surrogate(grid, in_channels=IN_CHANNELS, width=width, seed=1)
+        if isinstance(weights, str):
+            weights = np.load(weights, allow_pickle=False)
+        if weights is not None:
+            self.net.set_params(weights)
+        self.grid = tuple(grid)
+        self.z_max = z_max
+
+    def predict(self, scatter_low, trues_low, n_low):
+        x, k = normalize(scatter_low, trues_low, n_low)
+        y = self.net.forward(x)[0]
+        return np.maximum(y.astype(np.float64), 0.0) / k
+
+    def estimate(self, sim, n_low=N_LOW, seed=0):
+        """Scatter per decay on the grid from a run of n_low decays on `sim` (a Simulation of
+        the phantom). Returns (estimate, info); info["consistent"] False means: use Monte Carlo."""
+        t0 = time.perf_counter()
+        sim.run(seed=seed, acquisition_time_ps=acquisition_ps(n_low))
+        s = sim.rebin("scatter", self.grid)
+        t = sim.rebin("trues", self.grid)
+        decays = sim.stats()["decays"]
+        t1 = time.perf_counter()
+        est = self.predict(s, t, decays)
+        t2 = time.perf_counter()
+        z = consistency_z(est, s, decays)
+        return est, {"decays": decays, "z": z, "consistent": bool(z <= self.z_max), "mc_seconds": t1 - t0,
+                     "net_seconds": t2 - t1, "seconds": t2 - t0, "scatter_counts": float(s.sum())}
+
+    def close(self):
+        self.net.close()
+
+
+def monte_carlo(sim, n_decays, grid=GRID, seed=0):
+    """Scatter per decay on the grid from a Monte Carlo run of n_decays; (estimate, decays, seconds)."""
+    t0 = time.perf_counter()
+    sim.run(seed=seed, acquisition_time_ps=acquisition_ps(n_decays))
+    s = sim.rebin("scatter", grid)
+    decays = sim.stats()["decays"]
+    dt = time.perf_counter() - t0
+    return s.astype(np.float64) / decays, decays, dt, s
+
+
+# ------------------------------------------------------------------ training data and training
+
+def _materials():
+    here = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    return phantoms.head_materials(os.path.join(here, "tests", "golden", "sample_simulation", "materials"))
+
+
+def generate(n_phantoms, seed, n_target=5.0e8, n_low=N_LOW, inputs_per_phantom=4, chunk=16, log=print):
+    """Training pairs from scatter_phantoms(n_phantoms, seed): per phantom one target run of
+    n_target decays (scatter per decay) and inputs_per_phantom independent runs of n_low
+    decays (normalised scatter + trues). Returns X [n, 2, D, H, W], Y [n, D, H, W] (targets
+    in the inputs' normalisation) and the phantom index of each sample."""
+    names = _materials()
+    specs = phantoms.scatter_phantoms(n_phantoms, seed, n_decays=DECAYS_PER_SECOND, acquisition_time_s=1.0)
+    X, Y, idx = [], [], []
+    for c0 in range(0, n_phantoms, chunk):
+        part = specs[c0:c0 + chunk]
+        phs = [phantom_from_arrays(*sp) for sp in part]
+        dmax = np.max([p.density_max() for p in phs], axis=0)
+        tab = load_materials(names, dmax)
+        active = int(np.median([(sp[2] > 0).sum() for sp in part]))
+        targets = []
+        with Batch(config(names, streams_per_voxel(n_target, active)), phs, tab) as b:
+            b.run(seed=int(seed * 1000 + c0 + 1), acquisition_time_ps=acquisition_ps(n_target))
+            b.sync()
+            for k in range(len(phs)):
+                d = b.stats(k)["decays"]
+                targets.append(b.rebin(k, "scatter", GRID).astype(np.float64) / d)
+        with Batch(config(names, streams_per_voxel(n_low, active)), phs, tab) as b:
+            for r in range(inputs_per_phantom):
+                b.run(seed=int(seed * 1000 + c0 + 101 + r), acquisition_time_ps=acquisition_ps(n_low))
+                b.sync()
+                for k in range(len(phs)):
+                    d = b.stats(k)["decays"]
+                    x, kk = normalize(b.rebin(k, "scatter", GRID), b.rebin(k, "trues", GRID), d)
+                    X.append(x)
+                    Y.append((targets[k] * kk).astype(np.float32))
+                    idx.append(c0 + k)
+        for p in phs:
+            p.close()
+        tab.close()
+        log(f"generate: phantoms {c0}..{c0 + len(part) - 1} done")
+    return np.stack(X), np.stack(Y), np.array(idx)
+
+
+def train(X, Y, idx, width=WIDTH, epochs=150, batch=8, lr=5e-4, warmup=200, val_fraction=0.125, seed=1, log=print):
+    """Adam on the MSE, learning rate warmed up linearly over `warmup` steps then decayed on a
+    cosine; the parameters of the epoch with the lowest validation loss (phantoms held out by
+    index) are kept."""
+    rng = np.random.default_rng(seed)
+    phantom_ids = np.unique(idx)
+    n_val = max(1, int(len(phantom_ids) * val_fraction))
+    val_ids = set(phantom_ids[-n_val:].tolist())
+    val = np.array([i in val_ids for i in idx])
+    tr = np.flatnonzero(~val)
+    va = np.flatnonzero(val)
+    net = Surrogate(GRID, in_channels=IN_CHANNELS, width=width, seed=seed)
+    history = []
+    best = (np.inf, None, -1)
+    steps = epochs * max(1, len(tr) // batch)
+    step = 0
+    t0 = time.perf_counter()
+    for ep in range(epochs):
+        perm = rng.permutation(tr)
+        tl = []
+        for j in range(0, len(perm) - batch + 1, batch):
+            b = perm[j:j + batch]
+            cur = lr * min(1.0, (step + 1) / warmup) * 0.5 * (1.0 + np.cos(np.pi * step / steps))
+            tl.append(net.train_step(X[b], Y[b], lr=max(cur, lr * 0.01)))
+            step += 1
+            if ep == 0 and step % 10 == 0:
+                log(f"train: step {step} loss {tl[-1]:.5f}")
+        vl = float(np.mean([net.gradients(X[va[j:j + batch]], Y[va[j:j + batch]])[0]
+                            for j in range(0, len(va), batch)]))
+        vn = float(np.mean([nrmse(net.forward(X[i:i + 1])[0], Y[i]) for i in va]))
+        history.append({"epoch": ep, "train_loss": float(np.mean(tl)), "val_loss": vl, "val_nrmse": vn})
+        if vl < best[0]:
+            best = (vl, net.params(), ep)
+        if ep < 5 or ep % 10 == 0 or ep == epochs - 1:
+            log(f"train: epoch {ep} train {np.mean(tl):.5f} val {vl:.5f} val nrmse {vn:.4f} "
+                f"({time.perf_counter() - t0:.1f} s)")
+    net.close()
+    return best[1], {"best_epoch": best[2], "best_val_loss": best[0], "history": history,
+                     "train_samples": int(len(tr)), "val_samples": int(len(va)), "seconds": time.perf_counter() - t0}
+
+
+def main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser(description="train the scatter surrogate on simulator data")
+    ap.add_argument("--phantoms", type=int, default=96)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--n-target", type=float, default=5.0e8)
+    ap.add_argument("--inputs-per-phantom", type=int, default=4)
+    ap.add_argument("--epochs", type=int, default=150)
+    ap.add_argument("--width", type=int, default=WIDTH)
+    ap.add_argument("--lr", type=float, default=5e-4)
+    ap.add_argument("--out", default="gpurun_out/surrogate")
+    a = ap.parse_args(argv)
+    os.makedirs(a.out, exist_ok=True)
+    log = lambda m: print(m, flush=True)
+    t0 = time.perf_counter()
+    X, Y, idx = generate(a.phantoms, a.seed, n_target=a.n_target, inputs_per_phantom=a.inputs_per_phantom, log=log)
+    gen_s = time.perf_counter() - t0
+    log(f"generate: {len(X)} samples in {gen_s:.1f} s")
+    w, info = train(X, Y, idx, width=a.width, epochs=a.epochs, lr=a.lr, log=log)
+    np.save(os.path.join(a.out, f"scatter_unet_w{a.width}.npy"), w.astype(np.float32))
+    info.update({"phantoms": a.phantoms, "phantom_seed": a.seed, "n_target": a.n_target, "n_low": N_LOW,
+                 "inputs_per_phantom": a.inputs_per_phantom, "grid": GRID, "width": a.width, "generate_seconds": gen_s})
+    with open(os.path.join(a.out, f"scatter_unet_w{a.width}.json"), "w") as f:
+        json.dump(info, f)
+    log(f"saved {a.out}/scatter_unet_w{a.width}.npy (best epoch {info['best_epoch']})")
+
+
+if __name__ == "__main__":
+    main()

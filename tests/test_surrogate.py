@@ -191,13 +191,18 @@ def test_device_unet_forward_and_gradients_match_bf16_reference(grid, cin, c, ba
     with mcpet.Surrogate(grid, in_channels=cin, width=c, seed=7) as s:
         assert s.n_params == n_params(cin, c)
         p = s.params().astype(np.float64)
-        # He-normal weights, zero biases as created; give the biases values to check their gradients
-        for sw, sb in _layers(cin, c):
-            assert np.all(p[sb] == 0.0)
-            p[sb] = rng.standard_normal(sb.stop - sb.start) * 0.05
-        s.set_params(p.astype(np.float32))
         x = rng.random((batch, cin) + grid).astype(np.float32)
         t = rng.random((batch,) + grid).astype(np.float32)
+        # as created: He-normal weights, zero biases, zero output layer, so the output is input channel 0
+        layers = _layers(cin, c)
+        for sw, sb in layers:
+            assert np.all(p[sb] == 0.0)
+        assert np.all(p[layers[-1][0]] == 0.0) and np.array_equal(s.forward(x), x[:, 0])
+        # give the biases and the output layer values to check every gradient
+        for sw, sb in layers:
+            p[sb] = rng.standard_normal(sb.stop - sb.start) * 0.05
+        p[layers[-1][0]] = rng.standard_normal(layers[-1][0].stop - layers[-1][0].start) * 0.3
+        s.set_params(p.astype(np.float32))
         y = s.forward(x)
         loss, g = s.gradients(x, t)
         lr, gr, yr = unet_reference(p.astype(np.float32).astype(np.float64), x.astype(np.float64), t.astype(np.float64),
