@@ -146,13 +146,12 @@ def bench_surrogate(a):
     for i, sp in enumerate(specs):
         ph = mcpet.phantom_from_arrays(*sp)
         tab = mcpet.load_materials(names, ph.density_max())
-        active = int((sp[2] > 0).sum())
         # reference
         acc = np.zeros(S.GRID)
         ref_decays = 0
-        with mcpet.Simulation(S.config(names, S.streams_per_voxel(4e9, active)), ph, tab) as sim:
+        with mcpet.Simulation(S.config(names, 4e9), ph, tab) as sim:
             for j in range(8):
-                sim.run(seed=9000 + 17 * i + j, acquisition_time_ps=S.acquisition_ps(4e9))
+                sim.run(seed=9000 + 17 * i + j)
                 acc += sim.rebin("scatter", S.GRID)
                 ref_decays += sim.stats()["decays"]
                 if S.poisson_floor(acc) <= 0.01:
@@ -162,16 +161,16 @@ def bench_surrogate(a):
         n5 = S.decays_for(ref)
         log(f"surrogate bench: phantom {i}: reference {ref_decays:.3g} decays, floor {floor:.4f}, N5 {n5:.3g}")
         # Monte Carlo at N5
-        with mcpet.Simulation(S.config(names, S.streams_per_voxel(n5, active)), ph, tab) as sim:
+        with mcpet.Simulation(S.config(names, n5), ph, tab) as sim:
             for w in range(a.warmup):
-                S.monte_carlo(sim, n5, seed=100 + w)
+                S.monte_carlo(sim, seed=100 + w)
             mc_t, mc_e = [], []
             for s_ in range(a.steps):
-                e, d, dt, _ = S.monte_carlo(sim, n5, seed=200 + s_)
+                e, d, dt, _ = S.monte_carlo(sim, seed=200 + s_)
                 mc_t.append(dt)
                 mc_e.append(S.nrmse(e, ref))
         # surrogate
-        with mcpet.Simulation(S.config(names, S.streams_per_voxel(S.N_LOW, active)), ph, tab) as sim:
+        with mcpet.Simulation(S.config(names, S.N_LOW), ph, tab) as sim:
             for w in range(a.warmup):
                 est.estimate(sim, seed=300 + w)
             su_t, su_e, su_z, su_mc, su_net, raw_e = [], [], [], [], [], []
@@ -224,6 +223,41 @@ def bench_surrogate(a):
     print(json.dumps(out))
 
 
+def bench_unet(a):
+    """The surrogate's U-Net alone on the 32 x 56 x 56 grid (width 32, 2 input channels,
+    random inputs): a.steps forward passes of one sample and a.steps training steps of a
+    batch of 8, after a.warmup of each; GFLOP counted from the layer shapes."""
+    import mcpet
+    from mcpet import surrogate as S
+    rng = np.random.default_rng(0)
+    x = rng.random((8, S.IN_CHANNELS) + S.GRID).astype(np.float32)
+    t = rng.random((8,) + S.GRID).astype(np.float32)
+    with mcpet.Surrogate(S.GRID, in_channels=S.IN_CHANNELS, width=S.WIDTH, seed=1) as net:
+        for _ in range(a.warmup):
+            net.forward(x[:1])
+            net.train_step(x, t, lr=1e-4)
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            net.forward(x[:1])
+        fwd = (time.perf_counter() - t0) / a.steps
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            net.train_step(x, t, lr=1e-4)
+        trn = (time.perf_counter() - t0) / a.steps
+        gflop = net.gflop
+        n_params = net.n_params
+    tf = gflop / fwd / 1e3
+    out = {"metric": "U-Net forward throughput", "value": tf, "unit": "TFLOP/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": fwd * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "bf16 MFMA, fp32 accumulate", "data": "synthetic",
+           "config": {"workload": f"3D U-Net width {S.WIDTH}, {n_params} parameters, grid {S.GRID}, forward of one sample; "
+                                  "training step of 8 samples (forward, backward, Adam)", "parallelism": "single GPU"},
+           "forward_gflop": gflop, "train_step_ms": trn * 1e3, "train_tflops": 3 * 8 * gflop / trn / 1e3,
+           "roofline": {"bound": "mfma", "achieved": tf, "peak": 2500.0, "unit": "TFLOP/s", "frac": tf / 2500.0,
+                        "traffic": None, "kernel": "U-Net forward (host-timed)"}}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -237,9 +271,10 @@ def main():
     ap.add_argument("--phase-policy", type=int, default=0, help="0 auto, 1 most lanes, 2 least waste")
     ap.add_argument("--phase-cost", type=str, default="", help="comma-separated relative phase costs (DECAY, SOURCE2, STEP, INTERACT, TALLY, CTAU, CPZ)")
     ap.add_argument("--voxel-format", type=int, default=0, help="0 auto (palette indices), 1 8-byte voxel words")
-    ap.add_argument("--workload", choices=["head", "batch", "surrogate"], default="head",
+    ap.add_argument("--workload", choices=["head", "batch", "surrogate", "unet"], default="head",
                     help="head: the 1e9-decay head phantom; batch: 48 varied phantoms in one launch; "
-                         "surrogate: time to a 5%%-NRMSE scatter sinogram, Monte Carlo vs the learned surrogate")
+                         "surrogate: time to a 5%%-NRMSE scatter sinogram, Monte Carlo vs the learned surrogate; "
+                         "unet: the surrogate network alone")
     ap.add_argument("--eval-phantoms", type=int, default=6)
     ap.add_argument("--eval-seed", type=int, default=777)
     ap.add_argument("--surrogate-weights", default="", help="U-Net parameters (.npy); default: the packaged ones")
@@ -266,6 +301,8 @@ def main():
         return bench_batch(a)
     if a.workload == "surrogate":
         return bench_surrogate(a)
+    if a.workload == "unet":
+        return bench_unet(a)
     mat, dens, act, vs = phantoms.head(n_decays=a.decays)
     names = phantoms.head_materials(MATERIALS)
     cfg = phantoms.make_config(seed=20261202, threads_per_block=a.tpb, acquisition_time_s=10.0, nrad=168,

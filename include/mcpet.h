@@ -121,7 +121,12 @@ typedef struct mcpet_config {
   float positron_fraction;       /* fraction of decays emitting an annihilation pair (<= 0: 1) */
   float prompt_gamma_fraction;   /* fraction of decays emitting a prompt gamma (0: none)       */
   float prompt_gamma_energy;     /* eV, inside the interaction tables                          */
-  int32_t reserved0;
+  int32_t stream_decays;         /* 0: the reference's threads_per_block decay streams per emitting
+                                    voxel. > 0: voxel v gets max(1, ceil(E_v / stream_decays))
+                                    streams, E_v = activity * tau (1 - exp(-T / tau)) its expected
+                                    decays over acquisition_time_ps, each at activity / streams;
+                                    stream ids count the streams in voxel order. Balances the work
+                                    of hot voxels (statistically equal, other random numbers).  */
   int64_t singles_capacity;      /* records of the device singles list                         */
   uint64_t coincidence_window_ps;/* coincidence timing window (ps)                             */
 } mcpet_config;

@@ -878,18 +878,32 @@ __device__ bool decay(const KParams& P, Stream& st, CN& C, const mcpet_compton_t
 
 __device__ __forceinline__ void stream_init(const KParams& P, long long s, Stream& st)
 {
-  const int k = (int)(s / P.tpb);
-  const int t = (int)(s - (long long)k * P.tpb);
+  int k;
+  long long tid, per_voxel;
+  if (P.stream_start) {   // balanced layout: the voxel owning stream s (binary search)
+    int lo = 0, hi = P.n_active - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (P.stream_start[mid] <= s) lo = mid;
+      else hi = mid - 1;
+    }
+    k = lo;
+    per_voxel = P.stream_start[k + 1] - P.stream_start[k];
+    tid = P.stream_id[k] + (s - P.stream_start[k]);
+  } else {
+    k = (int)(s / P.tpb);
+    per_voxel = P.tpb;
+    tid = (long long)P.active_vox[k] * P.tpb + (s - (long long)k * P.tpb);
+  }
   st.absvox = P.active_vox[k];
   const float act = P.active_act[k];
   st.ix = st.absvox % P.nx;
   st.iy = (st.absvox / P.nx) % P.ny;
   st.iz = st.absvox / (P.nx * P.ny);
-  const long long tid = (long long)st.absvox * P.tpb + t;
   mcr_init_stream(tid, P.seed, P.j1, P.j2, &st.r.x, &st.r.y);
   st.event = (unsigned long long)tid << 32;
   st.remaining = P.acq_ps;
-  st.inv_act = ((double)P.tpb / (double)act);
+  st.inv_act = ((double)per_voxel / (double)act);
 }
 
 __device__ __forceinline__ void stream_finish(const KParams& P, const Stream& st)

@@ -43,9 +43,14 @@ struct KParams {
   int nx, ny, nz;
   float ivx, ivy, ivz;         // inverse voxel size
   float bx, by, bz;            // bounding box size
-  // decay streams: stream s -> emitting voxel active_vox[s / tpb], lane s % tpb
+  // decay streams: stream s -> emitting voxel active_vox[s / tpb], lane s % tpb; with a
+  // balanced layout (stream_start != null) voxel k owns streams [stream_start[k],
+  // stream_start[k + 1]) whose random-number stream ids start at stream_id[k]
   const int* active_vox;
   const float* active_act;
+  const long long* stream_start;
+  const long long* stream_id;
+  int n_active;
   long long n_streams;           // streams [stream_base, n_streams) are simulated by a launch
   long long stream_base;
   int tpb;

@@ -39,6 +39,8 @@ struct mcpet_ctx {
   float2* d_palette = nullptr;
   int* d_active_vox = nullptr;
   float* d_active_act = nullptr;
+  long long* d_stream_start = nullptr;   // balanced stream layout (stream_decays > 0)
+  long long* d_stream_id = nullptr;
   float2* d_woodcock = nullptr;
   float2* d_tot = nullptr;
   float4* d_cr = nullptr;
@@ -128,7 +130,7 @@ mcpet_status dalloc(mcpet_ctx* c, T** p, size_t n)
 
 void free_all(mcpet_ctx* c)
 {
-  void* ptrs[] = {c->d_vox, c->d_vox8, c->d_vox16, c->d_palette, c->d_active_vox, c->d_active_act, c->d_woodcock, c->d_tot, c->d_cr, c->d_ray, c->d_cmp,
+  void* ptrs[] = {c->d_vox, c->d_vox8, c->d_vox16, c->d_palette, c->d_active_vox, c->d_active_act, c->d_stream_start, c->d_stream_id, c->d_woodcock, c->d_tot, c->d_cr, c->d_ray, c->d_cmp,
                   c->d_trues, c->d_scatter, c->d_img_t, c->d_img_sc, c->d_spectrum, c->d_psf, c->d_psf_index,
                   c->d_voxel_edep, c->d_mat_edep, c->d_counters, c->d_final_seed, c->d_singles, c->d_singles_n,
                   c->d_keys[0], c->d_keys[1], c->d_vals[0], c->d_vals[1], c->d_hist, c->d_randoms, c->d_spurious,
@@ -319,6 +321,48 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
   std::vector<float> act_val(act_idx.size());
   for (size_t i = 0; i < act_idx.size(); i++) act_val[i] = ph->voxel_mat_dens[3 * (size_t)act_idx[i] + 2];
   c->n_active = (long long)act_idx.size();
+  // Balanced stream layout: voxel v gets max(1, ceil(E_v / stream_decays)) streams (E_v: its
+  // expected decays over the acquisition); random-number stream ids count the streams in voxel
+  // order, the device processes the voxels hottest first.
+  std::vector<long long> stream_start, stream_id;
+  long long balanced_streams = 0;
+  if (cfg->stream_decays > 0 && !act_idx.empty()) {
+    if (c->n_active > 0x7fffffffLL) {
+      free_all(c);
+      delete c;
+      return set_err(nullptr, MCPET_E_RANGE, "mcpet_create: too many emitting voxels for the balanced stream layout");
+    }
+    const double tau = (double)cfg->mean_life, T = (double)cfg->acquisition_time_ps * 1.0e-12;
+    const double teff = tau * (1.0 - std::exp(-T / tau));
+    auto streams_of = [&](float act) {
+      const double n = std::ceil((double)act * teff / (double)cfg->stream_decays);
+      return n < 1.0 ? 1LL : (n > 1.0e9 ? 1000000000LL : (long long)n);
+    };
+    std::vector<int> by_vox(act_idx);
+    std::sort(by_vox.begin(), by_vox.end());
+    std::unordered_map<int, long long> first_id;
+    first_id.reserve(by_vox.size() * 2);
+    long long id = 0;
+    for (int v : by_vox) {
+      first_id[v] = id;
+      id += streams_of(ph->voxel_mat_dens[3 * (size_t)v + 2]);
+    }
+    balanced_streams = id;
+    stream_start.resize(act_idx.size() + 1);
+    stream_id.resize(act_idx.size());
+    long long s0 = 0;
+    for (size_t k = 0; k < act_idx.size(); k++) {
+      stream_start[k] = s0;
+      stream_id[k] = first_id[act_idx[k]];
+      s0 += streams_of(act_val[k]);
+    }
+    stream_start[act_idx.size()] = s0;
+    if (cfg->detector.tally_psf_sinogram == MCPET_TALLY_SINGLES && balanced_streams >= (1LL << 32)) {
+      free_all(c);
+      delete c;
+      return set_err(nullptr, MCPET_E_RANGE, "mcpet_create: more than 2^32 decay streams (singles event ids)");
+    }
+  }
 
   // Tables: split (a,b) of the total cross section (read per Woodcock step) from the
   // Compton/Rayleigh pairs (read per real interaction).
@@ -355,6 +399,10 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
   if (c->P.vox_fmt != 0) CK(dalloc(c, &c->d_palette, palette.size()));
   CK(dalloc(c, &c->d_active_vox, std::max<size_t>(1, act_idx.size())));
   CK(dalloc(c, &c->d_active_act, std::max<size_t>(1, act_idx.size())));
+  if (!stream_start.empty()) {
+    CK(dalloc(c, &c->d_stream_start, stream_start.size()));
+    CK(dalloc(c, &c->d_stream_id, stream_id.size()));
+  }
   CK(dalloc(c, &c->d_woodcock, (size_t)N));
   CK(dalloc(c, &c->d_tot, htot.size()));
   CK(dalloc(c, &c->d_cr, hcr.size()));
@@ -406,6 +454,10 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
     HK(hipMemcpy(c->d_active_vox, act_idx.data(), act_idx.size() * sizeof(int), hipMemcpyHostToDevice));
     HK(hipMemcpy(c->d_active_act, act_val.data(), act_val.size() * sizeof(float), hipMemcpyHostToDevice));
   }
+  if (!stream_start.empty()) {
+    HK(hipMemcpy(c->d_stream_start, stream_start.data(), stream_start.size() * sizeof(long long), hipMemcpyHostToDevice));
+    HK(hipMemcpy(c->d_stream_id, stream_id.data(), stream_id.size() * sizeof(long long), hipMemcpyHostToDevice));
+  }
   HK(hipMemcpy(c->d_woodcock, hw.data(), hw.size() * sizeof(float2), hipMemcpyHostToDevice));
   HK(hipMemcpy(c->d_tot, htot.data(), htot.size() * sizeof(float2), hipMemcpyHostToDevice));
   HK(hipMemcpy(c->d_cr, hcr.data(), hcr.size() * sizeof(float4), hipMemcpyHostToDevice));
@@ -439,7 +491,10 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
   P.active_vox = c->d_active_vox;
   P.active_act = c->d_active_act;
   P.tpb = cfg->threads_per_block;
-  P.n_streams = c->n_active * (long long)cfg->threads_per_block;
+  P.stream_start = c->d_stream_start;
+  P.stream_id = c->d_stream_id;
+  P.n_active = (int)std::min<long long>(c->n_active, 0x7fffffffLL);
+  P.n_streams = c->d_stream_start ? balanced_streams : c->n_active * (long long)cfg->threads_per_block;
   P.queue = c->d_counters + KC_N;
   P.seed = cfg->seed;
   P.bg_mode = cfg->bg_mode;
@@ -495,7 +550,8 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
   P.mat_edep = c->d_mat_edep;
   P.counters = c->d_counters;
   P.final_seed = c->d_final_seed;
-  P.last_thread_id = (c->nvox - 1) * (long long)cfg->threads_per_block + (cfg->threads_per_block - 1);
+  P.last_thread_id = c->d_stream_start ? balanced_streams - 1
+                                        : (c->nvox - 1) * (long long)cfg->threads_per_block + (cfg->threads_per_block - 1);
   P.lds_spectrum = (cfg->ne > 0 && cfg->ne <= 8192) ? 1 : 0;
   P.lds_palette = 1;
   *out = c;

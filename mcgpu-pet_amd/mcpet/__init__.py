@@ -80,7 +80,7 @@ class Config(C.Structure):
                 ("nvoxs", C.c_int32), ("voxel_file", C.c_char * NAME_LEN),
                 ("material_files", (C.c_char * NAME_LEN) * MAX_MATERIALS), ("bg_mode", C.c_int32), ("voxel_format", C.c_int32),
                 ("positron_fraction", C.c_float), ("prompt_gamma_fraction", C.c_float),
-                ("prompt_gamma_energy", C.c_float), ("reserved0", C.c_int32), ("singles_capacity", C.c_int64),
+                ("prompt_gamma_energy", C.c_float), ("stream_decays", C.c_int32), ("singles_capacity", C.c_int64),
                 ("coincidence_window_ps", C.c_uint64)]
 
 

@@ -86,9 +86,10 @@ def make_config(*, seed=12345, threads_per_block=64, acquisition_time_s=1.0, mea
                 material_dose=False, voxel_dose=False, roi=None, e_resol=0.12, e_low=350000.0, e_high=600000.0,
                 fovz=25.0, nrows=80, ncrystals=336, nangles=168, nrad=147, nzs=159, res=128, ne=700, mrd=79, span=11,
                 materials=(), bg_mode=False, gpu_id=0, positron_fraction=0.0, prompt_gamma_fraction=0.0,
-                prompt_gamma_energy=0.0, singles_capacity=0, coincidence_window_ps=0):
+                prompt_gamma_energy=0.0, singles_capacity=0, coincidence_window_ps=0, stream_decays=0):
     """A Config equivalent to a MCGPU-PET.in file with these values (plus the library's
-    extensions: decay branching, prompt gamma and the singles pipeline, tally_psf_sinogram=3)."""
+    extensions: decay branching, prompt gamma and the singles pipeline, tally_psf_sinogram=3;
+    stream_decays > 0: the balanced decay-stream layout)."""
     c = Config()
     c.seed = seed
     c.gpu_id = gpu_id
@@ -123,6 +124,7 @@ def make_config(*, seed=12345, threads_per_block=64, acquisition_time_s=1.0, mea
     c.prompt_gamma_energy = prompt_gamma_energy
     c.singles_capacity = int(singles_capacity)
     c.coincidence_window_ps = int(coincidence_window_ps)
+    c.stream_decays = int(stream_decays)
     return c
 
 
@@ -140,7 +142,7 @@ def config_dict(c):
         else (32500, -32500, 32500, -32500, 32500, -32500),
         "positron_fraction": c.positron_fraction, "prompt_gamma_fraction": c.prompt_gamma_fraction,
         "prompt_gamma_energy": c.prompt_gamma_energy, "singles_capacity": c.singles_capacity,
-        "coincidence_window_ps": c.coincidence_window_ps,
+        "coincidence_window_ps": c.coincidence_window_ps, "stream_decays": c.stream_decays,
     }
 
 

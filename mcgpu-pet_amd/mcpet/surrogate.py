@@ -39,20 +39,18 @@ DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 WEIGHTS = os.path.join(DATA_DIR, f"scatter_unet_w{WIDTH}.npy")
 
 
-def config(materials, threads_per_block=1, seed=20261202):
-    """The head scanner (168 x 168 x 1293 sinogram) for phantoms of 1e9 decays per second."""
-    return phantoms.make_config(seed=seed, threads_per_block=threads_per_block, acquisition_time_s=1.0, nrad=168,
-                                materials=materials)
+def stream_decays(n_decays):
+    """Decays per stream of the balanced stream layout for a run of n_decays: about 2^20
+    streams beyond the one every emitting voxel gets, at least 4 decays each."""
+    return int(max(4, np.ceil(n_decays / 2.0 ** 20)))
 
 
-def acquisition_ps(n_decays):
-    return int(round(n_decays / DECAYS_PER_SECOND * 1e12))
-
-
-def streams_per_voxel(n_decays, active_voxels):
-    """Decay streams per emitting voxel for a run of n_decays: about 20 decays per stream,
-    1 to 32 streams (the same rule for the surrogate's input and for Monte Carlo)."""
-    return int(max(1, min(32, round(n_decays / (20.0 * max(1, active_voxels))))))
+def config(materials, n_decays, seed=20261202):
+    """The head scanner (168 x 168 x 1293 sinogram) and an acquisition of n_decays expected
+    decays for the workload's phantoms (1e9 decays per second), balanced stream layout (the
+    same rule for the surrogate's input and for Monte Carlo)."""
+    return phantoms.make_config(seed=seed, threads_per_block=1, acquisition_time_s=n_decays / DECAYS_PER_SECOND, nrad=168,
+                                materials=materials, stream_decays=stream_decays(n_decays))
 
 
 def nrmse(est, ref):
@@ -116,11 +114,12 @@ class ScatterEstimator:
         y = self.net.forward(x)[0]
         return np.maximum(y.astype(np.float64), 0.0) / k
 
-    def estimate(self, sim, n_low=N_LOW, seed=0):
-        """Scatter per decay on the grid from a run of n_low decays on `sim` (a Simulation of
-        the phantom). Returns (estimate, info); info["consistent"] False means: use Monte Carlo."""
+    def estimate(self, sim, seed=0):
+        """Scatter per decay on the grid from a run of `sim` (a Simulation of the phantom with
+        config(materials, N_LOW)). Returns (estimate, info); info["consistent"] False means: use
+        Monte Carlo."""
         t0 = time.perf_counter()
-        sim.run(seed=seed, acquisition_time_ps=acquisition_ps(n_low))
+        sim.run(seed=seed)
         s = sim.rebin("scatter", self.grid)
         t = sim.rebin("trues", self.grid)
         decays = sim.stats()["decays"]
@@ -135,10 +134,11 @@ class ScatterEstimator:
         self.net.close()
 
 
-def monte_carlo(sim, n_decays, grid=GRID, seed=0):
-    """Scatter per decay on the grid from a Monte Carlo run of n_decays; (estimate, decays, seconds)."""
+def monte_carlo(sim, grid=GRID, seed=0):
+    """Scatter per decay on the grid from a Monte Carlo run of `sim` (its configured
+    acquisition); (estimate, decays, seconds, counts)."""
     t0 = time.perf_counter()
-    sim.run(seed=seed, acquisition_time_ps=acquisition_ps(n_decays))
+    sim.run(seed=seed)
     s = sim.rebin("scatter", grid)
     decays = sim.stats()["decays"]
     dt = time.perf_counter() - t0
@@ -165,17 +165,16 @@ def generate(n_phantoms, seed, n_target=5.0e8, n_low=N_LOW, inputs_per_phantom=4
         phs = [phantom_from_arrays(*sp) for sp in part]
         dmax = np.max([p.density_max() for p in phs], axis=0)
         tab = load_materials(names, dmax)
-        active = int(np.median([(sp[2] > 0).sum() for sp in part]))
         targets = []
-        with Batch(config(names, streams_per_voxel(n_target, active)), phs, tab) as b:
-            b.run(seed=int(seed * 1000 + c0 + 1), acquisition_time_ps=acquisition_ps(n_target))
+        with Batch(config(names, n_target), phs, tab) as b:
+            b.run(seed=int(seed * 1000 + c0 + 1))
             b.sync()
             for k in range(len(phs)):
                 d = b.stats(k)["decays"]
                 targets.append(b.rebin(k, "scatter", GRID).astype(np.float64) / d)
-        with Batch(config(names, streams_per_voxel(n_low, active)), phs, tab) as b:
+        with Batch(config(names, n_low), phs, tab) as b:
             for r in range(inputs_per_phantom):
-                b.run(seed=int(seed * 1000 + c0 + 101 + r), acquisition_time_ps=acquisition_ps(n_low))
+                b.run(seed=int(seed * 1000 + c0 + 101 + r))
                 b.sync()
                 for k in range(len(phs)):
                     d = b.stats(k)["decays"]

@@ -54,7 +54,8 @@ class OrcSim(C.Structure):
         ("psf_index", C.c_int), ("total_histories", C.c_uint64), ("final_seed", C.c_int), ("final_seed_written", C.c_int),
         ("counters", C.c_uint64 * 16), ("nthreads", C.c_int), ("voxel_stride", C.c_int), ("voxel_offset", C.c_int),
         ("wall_seconds", C.c_double), ("positron_fraction", C.c_float), ("prompt_fraction", C.c_float),
-        ("prompt_energy", C.c_float), ("singles", C.c_void_p), ("singles_cap", C.c_longlong), ("singles_n", C.c_longlong)]
+        ("prompt_energy", C.c_float), ("singles", C.c_void_p), ("singles_cap", C.c_longlong), ("singles_n", C.c_longlong),
+        ("stream_decays", C.c_int), ("stream_first", C.c_void_p)]
 
 
 SINGLE_DTYPE = np.dtype([("time_ps", "<u8"), ("event", "<u8"), ("energy", "<f4"), ("z", "<f4"), ("phi", "<f4"),
@@ -213,6 +214,7 @@ class Oracle:
         s.prompt_fraction = p.get("prompt_gamma_fraction", 0.0)
         s.prompt_energy = p.get("prompt_gamma_energy", 0.0)
         s.singles_cap = int(p.get("singles_capacity", 0))
+        s.stream_decays = int(p.get("stream_decays", 0))
         self.s = s
 
     def locate(self, x, y, z):

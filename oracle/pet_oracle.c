@@ -690,15 +690,47 @@ static void o_tally_dose(orc_sim* s, float Edep, int mat, const float* pos)   /*
   }
 }
 
+/* Decay streams of a voxel and their random-number stream ids: the reference's layout
+ * (threads_per_block per voxel, id = voxel * threads_per_block + thread, K:251) or the
+ * balanced layout (stream_decays > 0: stream_first[v] .. stream_first[v + 1] - 1). */
+static long long o_streams_of_voxel(const orc_sim* s, int v)
+{
+  if (s->stream_decays > 0 && s->stream_first) return s->stream_first[v + 1] - s->stream_first[v];
+  return s->threads_per_block;
+}
+static long long o_stream_id(const orc_sim* s, int v, long long t)
+{
+  if (s->stream_decays > 0 && s->stream_first) return s->stream_first[v] + t;
+  return (long long)v * s->threads_per_block + t;
+}
+static int o_is_last_stream(const orc_sim* s, int v, long long t)
+{
+  const long long nvox = (long long)s->nx * s->ny * s->nz;
+  if (s->stream_decays > 0 && s->stream_first) return o_stream_id(s, v, t) == s->stream_first[nvox] - 1;
+  return v == nvox - 1 && t == s->threads_per_block - 1;
+}
+/* Streams of an emitting voxel in the balanced layout: max(1, ceil(E / stream_decays)), E the
+ * expected decays activity * tau (1 - exp(-T / tau)) over the acquisition T (at most 1e9). */
+static long long o_balanced_count(const orc_sim* s, float activity)
+{
+  const double tau = (double)s->mean_life, T = (double)s->acquisition_time_ps * 1.0e-12;
+  const double e = (double)activity * (tau * (1.0 - exp(-T / tau)));
+  const double n = ceil(e / (double)s->stream_decays);
+  if (n < 1.0) return 1;
+  if (n > 1.0e9) return 1000000000LL;
+  return (long long)n;
+}
+
 /* One reference GPU thread: K:223-458 for block (voxel absvox) and threadIdx tid. */
-static void o_thread(orc_sim* s, int absvox_block, int tid, o_local* L)
+static void o_thread(orc_sim* s, int absvox_block, long long tid, o_local* L)
 {
   const float* vmd = s->voxel_mat_dens;
   float voxel_activity = vmd[3 * (size_t)absvox_block + 2];
   unsigned long long acq_thread = s->acquisition_time_ps;
   unsigned long long histories_thread = 0;
   double inv_activity_thread;
-  if (voxel_activity > 1.0e-7f) inv_activity_thread = ((double)s->threads_per_block / (double)voxel_activity);
+  const long long per_voxel = o_streams_of_voxel(s, absvox_block);
+  if (voxel_activity > 1.0e-7f) inv_activity_thread = ((double)per_voxel / (double)voxel_activity);
   else return;
   const double inv_mean_life = 1.0 / (double)s->mean_life;
   const int ix = absvox_block % s->nx;
@@ -706,7 +738,7 @@ static void o_thread(orc_sim* s, int absvox_block, int tid, o_local* L)
   const int iz = absvox_block / (s->nx * s->ny);
   o_rng r;
   r.draws = 0;
-  long long thread_id = (long long)absvox_block * s->threads_per_block + tid;
+  long long thread_id = o_stream_id(s, absvox_block, tid);
   orc_init_PRNG(thread_id, 555555, s->seed, &r.x, &r.y);
   L->c[ORC_C_STREAMS]++;
 
@@ -830,7 +862,7 @@ static void o_thread(orc_sim* s, int absvox_block, int tid, o_local* L)
       }
     }
   }
-  if (absvox_block == s->nx * s->ny * s->nz - 1 && tid == s->threads_per_block - 1) {  /* K:454-455 */
+  if (o_is_last_stream(s, absvox_block, tid)) {  /* K:454-455 */
     s->final_seed = r.x;
     s->final_seed_written = 1;
   }
@@ -968,14 +1000,15 @@ static void o_record_single(orc_sim* s, const float* pos, const float* dir, floa
   }
 }
 
-static void o_thread_ext(orc_sim* s, int absvox_block, int tid, o_local* L)
+static void o_thread_ext(orc_sim* s, int absvox_block, long long tid, o_local* L)
 {
   const float* vmd = s->voxel_mat_dens;
   float voxel_activity = vmd[3 * (size_t)absvox_block + 2];
   unsigned long long acq_thread = s->acquisition_time_ps;
   unsigned long long histories_thread = 0;
   double inv_activity_thread;
-  if (voxel_activity > 1.0e-7f) inv_activity_thread = ((double)s->threads_per_block / (double)voxel_activity);
+  const long long per_voxel = o_streams_of_voxel(s, absvox_block);
+  if (voxel_activity > 1.0e-7f) inv_activity_thread = ((double)per_voxel / (double)voxel_activity);
   else return;
   const double inv_mean_life = 1.0 / (double)s->mean_life;
   const int ix = absvox_block % s->nx;
@@ -984,7 +1017,7 @@ static void o_thread_ext(orc_sim* s, int absvox_block, int tid, o_local* L)
   const int singles = s->tally_psf_sinogram == 3;
   o_rng r;
   r.draws = 0;
-  long long thread_id = (long long)absvox_block * s->threads_per_block + tid;
+  long long thread_id = o_stream_id(s, absvox_block, tid);
   orc_init_PRNG(thread_id, 555555, s->seed, &r.x, &r.y);
   L->c[ORC_C_STREAMS]++;
   unsigned long long event = (unsigned long long)thread_id << 32;
@@ -1057,7 +1090,7 @@ static void o_thread_ext(orc_sim* s, int absvox_block, int tid, o_local* L)
       if (index > -1 && singles) o_record_single(s, pos3, dir3, energy, travel, ss, 2, acq_thread, event, &r);
     }
   }
-  if (absvox_block == s->nx * s->ny * s->nz - 1 && tid == s->threads_per_block - 1) {  /* K:454-455 */
+  if (o_is_last_stream(s, absvox_block, tid)) {  /* K:454-455 */
     s->final_seed = r.x;
     s->final_seed_written = 1;
   }
@@ -1086,7 +1119,8 @@ static void* o_worker_main(void* arg)
     long long v = s->voxel_offset + k * stride;
     if (!(s->voxel_mat_dens[3 * v + 2] > 1.0e-7f)) continue;
     const int ext = s->tally_psf_sinogram == 3 || s->positron_fraction < 1.0f || s->prompt_fraction > 0.0f;
-    for (int t = 0; t < s->threads_per_block; t++) {
+    const long long nst = o_streams_of_voxel(s, (int)v);
+    for (long long t = 0; t < nst; t++) {
       if (ext) o_thread_ext(s, (int)v, t, &w->L);
       else o_thread(s, (int)v, t, &w->L);
     }
@@ -1107,6 +1141,20 @@ int orc_run(orc_sim* s)
   s->final_seed = 0;
   s->final_seed_written = 0;
   memset(s->counters, 0, sizeof s->counters);
+  long long* first = NULL;
+  if (s->stream_decays > 0) {   /* balanced layout: stream ids in voxel order */
+    const long long nvox = (long long)s->nx * s->ny * s->nz;
+    first = (long long*)malloc((size_t)(nvox + 1) * sizeof(long long));
+    if (!first) { free(ws); free(th); return -1; }
+    long long id = 0;
+    for (long long v = 0; v < nvox; v++) {
+      first[v] = id;
+      const float a = s->voxel_mat_dens[3 * v + 2];
+      if (a > 1.0e-7f) id += o_balanced_count(s, a);
+    }
+    first[nvox] = id;
+  }
+  s->stream_first = first;
   struct timespec t0, t1;
   clock_gettime(CLOCK_MONOTONIC, &t0);
   for (int i = 0; i < nt; i++) {
@@ -1123,6 +1171,8 @@ int orc_run(orc_sim* s)
     for (int c = 0; c < ORC_C_N; c++) s->counters[c] += ws[i].L.c[c];
   free(ws);
   free(th);
+  free(first);
+  s->stream_first = NULL;
   return 0;
 }
 

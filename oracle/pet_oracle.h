@@ -108,6 +108,12 @@ typedef struct orc_sim {
   unsigned char* singles;
   long long singles_cap;
   long long singles_n;
+  /* balanced decay-stream layout (the library's stream_decays extension): 0 = the reference's
+   * threads_per_block streams per emitting voxel; > 0: max(1, ceil(E_v / stream_decays))
+   * streams for voxel v, E_v = activity * tau (1 - exp(-T / tau)), stream ids numbered in voxel
+   * order. stream_first is filled by orc_run (NULL otherwise). */
+  int stream_decays;
+  long long* stream_first;               /* nvox + 1 entries while orc_run runs */
 } orc_sim;
 
 /* RANECU and skip-ahead (K:898-1083, C:2906-2935) */

@@ -62,10 +62,10 @@ def test_normalize_is_invariant_to_the_number_of_decays():
     assert abs(x1[1].mean() - 1) < 1e-6 and x1.dtype == np.float32 and x1.shape == (2,) + S.GRID
 
 
-def test_streams_per_voxel_rule():
-    assert S.streams_per_voxel(5e6, 400_000) == 1
-    assert S.streams_per_voxel(7e8, 400_000) == 32
-    assert S.streams_per_voxel(1.6e8, 400_000) == 20
+def test_workload_config():
+    c = S.config([], 5e6)
+    assert c.acquisition_time_ps == 5_000_000_000 and c.stream_decays == 5 and c.nrad == 168 and c.nsinos == 1293
+    assert S.config([], 7e8).stream_decays == 668 and S.stream_decays(1e3) == 4
 
 
 def test_scatter_phantoms_are_seeded_and_scaled():
