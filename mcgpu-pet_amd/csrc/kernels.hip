@@ -1088,11 +1088,22 @@ __device__ __forceinline__ int after_photon(const KParams& P, int photon_no, boo
 // and every lane of the wave has finished its stream. kExt: the singles list or prompt
 // gammas are in use (phases DETECT and SOURCE3 exist); otherwise the wave only ballots
 // over the seven phases of the reference's pair tallies.
+// Streams are claimed from the queue by the wave, K_CHUNK indices per atomic, and handed to
+// the lanes that need one in lane order; the wave's unused range [wq[0], wq[1]) lives in LDS
+// (a single global counter updated once per stream was the limit of short runs). The range
+// is read by every lane and written by the lowest one; volatile keeps each access an LDS
+// operation in program order, which a wave executes in order.
+#define K_CHUNK 64
 template <class CN, bool kExt>
 __device__ __forceinline__ void sm_waves(const KParams& P, CN& C, const mcpet_compton_table& s_cmp, int* s_spec,
-                                         unsigned long long* s_mat, const LaneSlots& L)
+                                         unsigned long long* s_mat, const LaneSlots& L, volatile long long* wq)
 {
   constexpr int kPhases = kExt ? LS_N : LS_CPZ + 1;
+  if (__lane_id() == 0) {
+    wq[0] = 0;
+    wq[1] = 0;
+  }
+  __builtin_amdgcn_wave_barrier();
   Stream st;
   Photon ph;
   bool have = false;
@@ -1171,7 +1182,29 @@ __device__ __forceinline__ void sm_waves(const KParams& P, CN& C, const mcpet_co
     } else if (best == LS_DECAY) {
       for (;;) {
         if (!have) {
-          const long long sid = P.stream_base + (long long)atomicAdd(P.queue, 1ull);
+          // the lanes here all need a stream: rank among them, claim a new chunk if short
+          const unsigned long long need = __ballot(1);
+          const int cnt = __popcll(need);
+          const int rank = __popcll(need & ((1ull << __lane_id()) - 1ull));
+          long long qn = wq[0];
+          const long long qe = wq[1];
+          long long idx;
+          if (qe - qn >= cnt) {
+            idx = qn + rank;
+            qn += cnt;
+            if (rank == 0) wq[0] = qn;
+          } else {
+            const long long avail = qe - qn;
+            unsigned long long base = 0;
+            if (rank == 0) base = atomicAdd(P.queue, (unsigned long long)K_CHUNK);
+            base = __builtin_amdgcn_readfirstlane((unsigned)base) | ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(base >> 32)) << 32);
+            idx = rank < avail ? qn + rank : (long long)base + (rank - avail);
+            if (rank == 0) {
+              wq[0] = (long long)base + (cnt - avail);
+              wq[1] = (long long)base + K_CHUNK;
+            }
+          }
+          const long long sid = P.stream_base + idx;
           if (sid >= P.n_streams) {
             state = LS_DONE;
             break;
@@ -1251,8 +1284,9 @@ __global__ __launch_bounds__(K_BLOCK) K_OCCUPANCY void k_transport_sm(const KPar
   const KParams& P = *Pd;
   K_PROLOGUE(CN)
   __shared__ float s_slots[LANE_SLOTS][K_BLOCK];
+  __shared__ long long s_wq[K_BLOCK / 64][2];
   const LaneSlots L{&s_slots[0][threadIdx.x]};
-  sm_waves<CN, kExt>(P, C, s_cmp, s_spec, s_mat, L);
+  sm_waves<CN, kExt>(P, C, s_cmp, s_spec, s_mat, L, s_wq[threadIdx.x >> 6]);
   K_EPILOGUE
 }
 
@@ -1273,6 +1307,7 @@ __global__ __launch_bounds__(K_BLOCK) K_OCCUPANCY void k_transport_batch(const K
   __shared__ unsigned long long s_matw[K_BLOCK / 64][2 * K_MAXMAT];
   __shared__ unsigned s_cntw[K_BLOCK / 64][KC_N];
   __shared__ float s_slots[LANE_SLOTS][K_BLOCK];
+  __shared__ long long s_wq[K_BLOCK / 64][2];
   {
     const int* src = reinterpret_cast<const int*>(Ps[0].cmp);
     int* dst = reinterpret_cast<int*>(&s_cmp);
@@ -1289,7 +1324,7 @@ __global__ __launch_bounds__(K_BLOCK) K_OCCUPANCY void k_transport_batch(const K
     const KParams& P = Ps[k];
     CN C;
     C.init(s_cntw[wave]);
-    sm_waves<CN, kExt>(P, C, s_cmp, nullptr, s_matw[wave], L);
+    sm_waves<CN, kExt>(P, C, s_cmp, nullptr, s_matw[wave], L, s_wq[wave]);
     C.flush(P, s_cntw[wave]);
     __builtin_amdgcn_wave_barrier();
     if (lane < KC_N && s_cntw[wave][lane]) {
