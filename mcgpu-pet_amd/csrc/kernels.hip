@@ -1089,11 +1089,49 @@ __device__ __forceinline__ int after_photon(const KParams& P, int photon_no, boo
 // gammas are in use (phases DETECT and SOURCE3 exist); otherwise the wave only ballots
 // over the seven phases of the reference's pair tallies.
 // Streams are claimed from the queue by the wave, K_CHUNK indices per atomic, and handed to
-// the lanes that need one in lane order; the wave's unused range [wq[0], wq[1]) lives in LDS
+// the lanes that need one in lane order; the wave's unused range (next index wq[0], count
+// wq[1]) lives in LDS
 // (a single global counter updated once per stream was the limit of short runs). The range
 // is read by every lane and written by the lowest one; volatile keeps each access an LDS
 // operation in program order, which a wave executes in order.
 #define K_CHUNK 64
+// Index (from the queue's origin) of the next stream for every calling lane. Not inlined:
+// inlined into the state machine it raised the register pressure of every phase (32 more
+// spilled VGPRs); as a call it costs a few register saves at each claim.
+__device__ __attribute__((noinline)) long long claim_stream(unsigned long long* queue, volatile long long* wq)
+{
+  // the lanes here all need a stream: rank among them, claim a new chunk if short
+  // (the range is wave-uniform: read into scalar registers)
+  const unsigned long long need = __ballot(1);
+  const int cnt = __popcll(need);
+  const int rank = __popcll(need & ((1ull << __lane_id()) - 1ull));
+  const long long q0 = wq[0];
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)q0);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)((unsigned long long)q0 >> 32));
+  const long long qn = (long long)(((unsigned long long)hi << 32) | lo);
+  const int avail = __builtin_amdgcn_readfirstlane((int)wq[1]);
+  long long idx;
+  if (avail >= cnt) {
+    idx = qn + rank;
+    if (rank == 0) {
+      wq[0] = qn + cnt;
+      wq[1] = avail - cnt;
+    }
+  } else {
+    unsigned long long base = 0;
+    if (rank == 0) base = atomicAdd(queue, (unsigned long long)K_CHUNK);
+    const unsigned blo = __builtin_amdgcn_readfirstlane((unsigned)base);
+    const unsigned bhi = __builtin_amdgcn_readfirstlane((unsigned)(base >> 32));
+    const long long b = (long long)(((unsigned long long)bhi << 32) | blo);
+    idx = rank < avail ? qn + rank : b + (rank - avail);
+    if (rank == 0) {
+      wq[0] = b + (cnt - avail);
+      wq[1] = K_CHUNK - (cnt - avail);
+    }
+  }
+  return idx;
+}
+
 template <class CN, bool kExt>
 __device__ __forceinline__ void sm_waves(const KParams& P, CN& C, const mcpet_compton_table& s_cmp, int* s_spec,
                                          unsigned long long* s_mat, const LaneSlots& L, volatile long long* wq)
@@ -1182,28 +1220,7 @@ __device__ __forceinline__ void sm_waves(const KParams& P, CN& C, const mcpet_co
     } else if (best == LS_DECAY) {
       for (;;) {
         if (!have) {
-          // the lanes here all need a stream: rank among them, claim a new chunk if short
-          const unsigned long long need = __ballot(1);
-          const int cnt = __popcll(need);
-          const int rank = __popcll(need & ((1ull << __lane_id()) - 1ull));
-          long long qn = wq[0];
-          const long long qe = wq[1];
-          long long idx;
-          if (qe - qn >= cnt) {
-            idx = qn + rank;
-            qn += cnt;
-            if (rank == 0) wq[0] = qn;
-          } else {
-            const long long avail = qe - qn;
-            unsigned long long base = 0;
-            if (rank == 0) base = atomicAdd(P.queue, (unsigned long long)K_CHUNK);
-            base = __builtin_amdgcn_readfirstlane((unsigned)base) | ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(base >> 32)) << 32);
-            idx = rank < avail ? qn + rank : (long long)base + (rank - avail);
-            if (rank == 0) {
-              wq[0] = (long long)base + (cnt - avail);
-              wq[1] = (long long)base + K_CHUNK;
-            }
-          }
+          const long long idx = claim_stream(P.queue, wq);
           const long long sid = P.stream_base + idx;
           if (sid >= P.n_streams) {
             state = LS_DONE;
