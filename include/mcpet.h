@@ -388,6 +388,13 @@ mcpet_status mcpet_server_drain(mcpet_server* s);
 mcpet_status mcpet_server_poll(mcpet_server* s, int64_t request_id, int32_t* state);
 /* Results of a finished request (the sizes of its config / phantom); the request is then forgotten. */
 mcpet_status mcpet_server_fetch(mcpet_server* s, int64_t request_id, mcpet_tallies* out);
+/* A request whose result is only its trues and scatter rebinned on the device onto a D x H x W
+ * grid (mcpet_rebin_sinogram) and its number of decays: no full tallies are copied to the host
+ * (the scatter surrogate's inputs and its Monte Carlo fallbacks). */
+mcpet_status mcpet_server_submit_rebinned(mcpet_server* s, const mcpet_config* cfg, const mcpet_phantom* ph, int32_t seed,
+                                          int32_t D, int32_t H, int32_t W, int64_t* request_id);
+/* trues, scatter: D*H*W floats each (may be NULL); decays (may be NULL). The request is then forgotten. */
+mcpet_status mcpet_server_fetch_rebinned(mcpet_server* s, int64_t request_id, float* trues, float* scatter, int64_t* decays);
 mcpet_status mcpet_server_get_stats(mcpet_server* s, mcpet_server_stats* out);
 void mcpet_server_destroy(mcpet_server* s);
 const char* mcpet_server_last_error(const mcpet_server* s);

@@ -40,6 +40,8 @@ struct Request {
   std::vector<uint64_t> mat_edep, voxel_edep;
   std::vector<mcpet_psf_record> psf;
   mcpet_tallies t{};
+  int grid[3] = {0, 0, 0};           // rebinned request: keep only these grids
+  std::vector<float> rb_trues, rb_scatter;
 };
 
 // Upper bound of the device memory mcpet_create allocates for (cfg, phantom, tables).
@@ -128,8 +130,8 @@ extern "C" mcpet_status mcpet_server_create(const mcpet_tables* tab, size_t budg
   return MCPET_OK;
 }
 
-extern "C" mcpet_status mcpet_server_submit(mcpet_server* s, const mcpet_config* cfg, const mcpet_phantom* ph, int32_t seed,
-                                            int64_t* request_id)
+static mcpet_status submit(mcpet_server* s, const mcpet_config* cfg, const mcpet_phantom* ph, int32_t seed, const int* grid,
+                           int64_t* request_id)
 {
   if (!s || !cfg || !ph || !ph->voxel_mat_dens) return sfail(s, MCPET_E_ARG, "mcpet_server_submit: null argument");
   if (cfg->detector.tally_psf_sinogram == MCPET_TALLY_SINGLES)
@@ -144,7 +146,13 @@ extern "C" mcpet_status mcpet_server_submit(mcpet_server* s, const mcpet_config*
   r->ph.voxel_mat_dens = r->vmd.data();
   r->ph.owned = 0;
   r->seed = seed;
-  r->footprint = footprint(r->cfg, r->ph, *s->tab);
+  if (grid) {
+    std::memcpy(r->grid, grid, sizeof r->grid);
+    r->footprint = footprint(r->cfg, r->ph, *s->tab) + 4 * ((size_t)grid[0] * grid[1] * grid[2] + grid[0] + 1) +
+                   4 * (size_t)r->cfg.nsinos;   // the rebinning's plane lists and output
+  } else {
+    r->footprint = footprint(r->cfg, r->ph, *s->tab);
+  }
   r->submitted = Clock::now();
   s->st.submitted++;
   if (request_id) *request_id = r->id;
@@ -159,8 +167,49 @@ extern "C" mcpet_status mcpet_server_submit(mcpet_server* s, const mcpet_config*
   return MCPET_OK;
 }
 
+extern "C" mcpet_status mcpet_server_submit(mcpet_server* s, const mcpet_config* cfg, const mcpet_phantom* ph, int32_t seed,
+                                            int64_t* request_id)
+{
+  return submit(s, cfg, ph, seed, nullptr, request_id);
+}
+
+extern "C" mcpet_status mcpet_server_submit_rebinned(mcpet_server* s, const mcpet_config* cfg, const mcpet_phantom* ph,
+                                                     int32_t seed, int32_t D, int32_t H, int32_t W, int64_t* request_id)
+{
+  if (D < 1 || H < 1 || W < 1 || W > 256) return sfail(s, MCPET_E_ARG, "mcpet_server_submit_rebinned: bad grid (W <= 256)");
+  const int grid[3] = {D, H, W};
+  return submit(s, cfg, ph, seed, grid, request_id);
+}
+
+static void complete(mcpet_server* s, Request& r)
+{
+  mcpet_destroy(r.ctx);
+  r.ctx = nullptr;
+  s->admitted -= r.footprint;
+  r.state = MCPET_REQ_DONE;
+  r.done = Clock::now();
+  s->latencies.push_back(std::chrono::duration<double, std::milli>(r.done - r.submitted).count());
+  s->st.completed++;
+}
+
+static mcpet_status finish_rebinned(mcpet_server* s, Request& r)
+{
+  const size_t n = (size_t)r.grid[0] * r.grid[1] * r.grid[2];
+  r.rb_trues.resize(n);
+  r.rb_scatter.resize(n);
+  mcpet_tallies& t = r.t;
+  std::memset(&t, 0, sizeof t);
+  mcpet_status st = mcpet_fetch(r.ctx, &t);   // counters only (no tally buffers given)
+  if (st == MCPET_OK) st = mcpet_rebin_sinogram(r.ctx, 0, r.grid[0], r.grid[1], r.grid[2], r.rb_trues.data());
+  if (st == MCPET_OK) st = mcpet_rebin_sinogram(r.ctx, 1, r.grid[0], r.grid[1], r.grid[2], r.rb_scatter.data());
+  if (st != MCPET_OK) return sfail(s, st, std::string("request ") + std::to_string(r.id) + ": " + mcpet_last_error(r.ctx));
+  complete(s, r);
+  return MCPET_OK;
+}
+
 static mcpet_status finish(mcpet_server* s, Request& r)
 {
+  if (r.grid[0] > 0) return finish_rebinned(s, r);
   const size_t nvox = (size_t)r.ph.nx * r.ph.ny * r.ph.nz;
   r.trues.resize((size_t)r.cfg.nbins);
   r.scatter.resize((size_t)r.cfg.nbins);
@@ -190,13 +239,7 @@ static mcpet_status finish(mcpet_server* s, Request& r)
   mcpet_status st = mcpet_fetch(r.ctx, &t);
   if (st != MCPET_OK) return sfail(s, st, std::string("request ") + std::to_string(r.id) + ": " + mcpet_last_error(r.ctx));
   r.psf.resize((size_t)t.psf_count);
-  mcpet_destroy(r.ctx);
-  r.ctx = nullptr;
-  s->admitted -= r.footprint;
-  r.state = MCPET_REQ_DONE;
-  r.done = Clock::now();
-  s->latencies.push_back(std::chrono::duration<double, std::milli>(r.done - r.submitted).count());
-  s->st.completed++;
+  complete(s, r);
   return MCPET_OK;
 }
 
@@ -286,6 +329,7 @@ extern "C" mcpet_status mcpet_server_fetch(mcpet_server* s, int64_t id, mcpet_ta
   if (it == s->req.end() || it->second->state != MCPET_REQ_DONE)
     return sfail(s, MCPET_E_STATE, "mcpet_server_fetch: request unknown or not finished");
   Request& r = *it->second;
+  if (r.grid[0] > 0) return sfail(s, MCPET_E_STATE, "mcpet_server_fetch: a rebinned request (use mcpet_server_fetch_rebinned)");
   const size_t nb = (size_t)r.cfg.nbins, nv = r.img_t.size();
   if (out->trues) std::memcpy(out->trues, r.trues.data(), 4 * nb);
   if (out->scatter) std::memcpy(out->scatter, r.scatter.data(), 4 * nb);
@@ -304,6 +348,20 @@ extern "C" mcpet_status mcpet_server_fetch(mcpet_server* s, int64_t id, mcpet_ta
   out->sum_trues = r.t.sum_trues;
   out->sum_scatter = r.t.sum_scatter;
   out->final_seed = r.t.final_seed;
+  s->req.erase(it);
+  return MCPET_OK;
+}
+
+extern "C" mcpet_status mcpet_server_fetch_rebinned(mcpet_server* s, int64_t id, float* trues, float* scatter, int64_t* decays)
+{
+  if (!s) return sfail(s, MCPET_E_ARG, "mcpet_server_fetch_rebinned: null server");
+  auto it = s->req.find(id);
+  if (it == s->req.end() || it->second->state != MCPET_REQ_DONE || it->second->grid[0] == 0)
+    return sfail(s, MCPET_E_STATE, "mcpet_server_fetch_rebinned: request unknown, not finished or not rebinned");
+  Request& r = *it->second;
+  if (trues) std::memcpy(trues, r.rb_trues.data(), 4 * r.rb_trues.size());
+  if (scatter) std::memcpy(scatter, r.rb_scatter.data(), 4 * r.rb_scatter.size());
+  if (decays) *decays = (int64_t)(r.t.total_histories / 2);
   s->req.erase(it);
   return MCPET_OK;
 }

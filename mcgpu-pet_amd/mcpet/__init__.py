@@ -200,6 +200,9 @@ def lib():
     L.mcpet_server_fetch.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
     L.mcpet_server_get_stats.argtypes = [C.c_void_p, C.c_void_p]
     L.mcpet_server_destroy.argtypes = [C.c_void_p]
+    L.mcpet_server_submit_rebinned.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
+                                               C.c_int32, C.POINTER(C.c_int64)]
+    L.mcpet_server_fetch_rebinned.argtypes = [C.c_void_p, C.c_int64, P(C.c_float), P(C.c_float), C.POINTER(C.c_int64)]
     L.mcpet_server_last_error.argtypes = [C.c_void_p]
     L.mcpet_server_last_error.restype = C.c_char_p
     L.mcpet_struct_sizes.restype = C.c_int32
@@ -520,7 +523,7 @@ class Server:
     def submit(self, cfg, phantom, seed=0):
         rid = C.c_int64()
         st = lib().mcpet_server_submit(self.s, C.byref(cfg), C.byref(phantom.ph), int(seed), C.byref(rid))
-        self._meta[rid.value] = (cfg, phantom.nvox)
+        self._meta[rid.value] = {"kind": "full", "cfg": cfg, "nvox": phantom.nvox}
         self._check(st)
         return rid.value
 
@@ -538,9 +541,36 @@ class Server:
         return REQ_STATES[v.value]
 
     def fetch(self, rid, **kw):
-        cfg, nv = self._meta.pop(rid)
-        return _fetch_tallies(lambda t: self._check(lib().mcpet_server_fetch(self.s, int(rid), C.byref(t))), cfg, nv,
-                              singles=False, **kw)
+        m = self._meta.get(rid)
+        if m is None or m["kind"] != "full":    # the library reports the error
+            self._check(lib().mcpet_server_fetch(self.s, int(rid), C.byref(Tallies())))
+        out = _fetch_tallies(lambda t: self._check(lib().mcpet_server_fetch(self.s, int(rid), C.byref(t))), m["cfg"],
+                             m["nvox"], singles=False, **kw)
+        self._meta.pop(rid)
+        return out
+
+    def submit_rebinned(self, cfg, phantom, grid, seed=0):
+        """A request whose result is its trues and scatter rebinned on the device onto grid."""
+        rid = C.c_int64()
+        D, H, W = grid
+        st = lib().mcpet_server_submit_rebinned(self.s, C.byref(cfg), C.byref(phantom.ph), int(seed), D, H, W,
+                                                C.byref(rid))
+        self._meta[rid.value] = {"kind": "rebinned", "grid": tuple(grid)}
+        self._check(st)
+        return rid.value
+
+    def fetch_rebinned(self, rid):
+        """(trues, scatter, decays) of a finished rebinned request."""
+        m = self._meta.get(rid)
+        if m is None or m["kind"] != "rebinned":    # the library reports the error
+            self._check(lib().mcpet_server_fetch_rebinned(self.s, int(rid), None, None, None))
+        grid = m["grid"]
+        t = np.zeros(grid, np.float32)
+        s = np.zeros(grid, np.float32)
+        d = C.c_int64()
+        self._check(lib().mcpet_server_fetch_rebinned(self.s, int(rid), _fptr(t), _fptr(s), C.byref(d)))
+        self._meta.pop(rid)
+        return t, s, d.value
 
     def stats(self):
         x = ServerStats()

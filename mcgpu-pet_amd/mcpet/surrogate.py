@@ -134,6 +134,83 @@ class ScatterEstimator:
         self.net.close()
 
 
+class ScatterService:
+    """Scatter estimates for many phantoms through the serving loop (mcpet_server_*): each
+    request's low-count run is a rebinned server request, batched with the other requests'
+    runs; when it completes the U-Net estimate is checked (consistency z-score) and, if it
+    fails, the request is resubmitted as a Monte Carlo run of N5 decays (N5 from the
+    prediction) whose scatter is the answer. A request's answer does not depend on the
+    other requests in flight."""
+
+    def __init__(self, tables, estimator, budget_bytes=16 << 30, max_running=16, streams_per_step=1 << 22,
+                 materials=None):
+        from . import Server
+        self.server = Server(tables, budget_bytes, max_running=max_running, streams_per_step=streams_per_step)
+        self.est = estimator
+        self.names = materials if materials is not None else _materials()
+        self.jobs = {}          # service id -> dict
+        self.by_rid = {}        # server request id -> service id
+        self.next_id = 1
+        self.latencies = []
+        self.fallbacks = 0
+
+    def submit(self, phantom, seed=1):
+        sid = self.next_id
+        self.next_id += 1
+        rid = self.server.submit_rebinned(config(self.names, N_LOW), phantom, self.est.grid, seed=seed)
+        self.jobs[sid] = {"phantom": phantom, "seed": seed, "stage": "surrogate", "t0": time.perf_counter(),
+                          "result": None, "info": None}
+        self.by_rid[rid] = sid
+        return sid
+
+    def _complete(self, rid):
+        sid = self.by_rid.pop(rid)
+        job = self.jobs[sid]
+        t, s, decays = self.server.fetch_rebinned(rid)
+        if job["stage"] == "surrogate":
+            est = self.est.predict(s, t, decays)
+            z = consistency_z(est, s, decays)
+            job["info"] = {"decays": decays, "z": z, "consistent": bool(z <= self.est.z_max)}
+            if job["info"]["consistent"]:
+                job["result"] = est
+            else:   # fall back: Monte Carlo to the decays a 5 % estimate needs
+                self.fallbacks += 1
+                n5 = decays_for(est) if est.sum() > 0 else 1e9
+                job["stage"] = "monte_carlo"
+                job["info"]["n5"] = n5
+                rid2 = self.server.submit_rebinned(config(self.names, n5), job["phantom"], self.est.grid, seed=job["seed"])
+                self.by_rid[rid2] = sid
+                return
+        else:
+            job["result"] = s.astype(np.float64) / decays
+            job["info"]["mc_decays"] = decays
+        job["seconds"] = time.perf_counter() - job["t0"]
+        self.latencies.append(job["seconds"])
+
+    def step(self):
+        self.server.step()
+        for rid in [r for r in list(self.by_rid) if self.server.poll(r) == "done"]:
+            self._complete(rid)
+
+    def drain(self):
+        while self.by_rid:
+            self.step()
+
+    def result(self, sid):
+        job = self.jobs.pop(sid)
+        return job["result"], job["info"]
+
+    def stats(self):
+        lat = np.array(self.latencies) * 1e3
+        out = {"completed": len(self.latencies), "fallbacks": self.fallbacks, "server": self.server.stats()}
+        if len(lat):
+            out.update({"latency_p50_ms": float(np.percentile(lat, 50)), "latency_p99_ms": float(np.percentile(lat, 99))})
+        return out
+
+    def close(self):
+        self.server.close()
+
+
 def monte_carlo(sim, grid=GRID, seed=0):
     """Scatter per decay on the grid from a Monte Carlo run of `sim` (its configured
     acquisition); (estimate, decays, seconds, counts)."""

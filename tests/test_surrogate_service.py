@@ -1,0 +1,89 @@
+"""Scatter surrogate served through the serving loop (no reference counterpart): rebinned
+server requests, the estimate of a request alone and in a mix of requests, the Monte Carlo
+fallback, and the HBM budget."""
+import numpy as np
+import pytest
+
+import mcpet
+from harness import water_cube_scenario
+from mcpet import phantoms
+from mcpet import surrogate as S
+
+
+@pytest.mark.gpu
+def test_server_rebinned_request_equals_simulation_rebinned():
+    sc = water_cube_scenario(n_decays=2e5, tpb=16)
+    grid = (8, 12, 12)
+    with mcpet.Simulation(sc.cfg, sc.phantom, sc.tables) as sim:
+        sim.run(seed=77)
+        t0, s0 = sim.rebin("trues", grid), sim.rebin("scatter", grid)
+        d0 = sim.stats()["decays"]
+    with mcpet.Server(sc.tables, 4 << 30, streams_per_step=1000) as srv:
+        a = srv.submit_rebinned(sc.cfg, sc.phantom, grid, seed=77)
+        b = srv.submit(sc.cfg, sc.phantom, seed=78)          # a full-tally request beside it
+        srv.drain()
+        with pytest.raises(mcpet.McpetError):
+            srv.fetch_rebinned(b)
+        with pytest.raises(mcpet.McpetError):
+            srv.fetch(a)
+        t, s, d = srv.fetch_rebinned(a)
+        full = srv.fetch(b)
+    assert d == d0 and np.array_equal(t, t0) and np.array_equal(s, s0) and t.sum() > 0
+    assert full["trues"].sum() > 0
+
+
+def _phantoms(n, seed):
+    return [mcpet.phantom_from_arrays(*sp) for sp in
+            phantoms.scatter_phantoms(n, seed, n_decays=S.DECAYS_PER_SECOND, acquisition_time_s=1.0)]
+
+
+@pytest.mark.gpu
+def test_scatter_service_alone_equals_in_mix_and_the_direct_estimate():
+    phs = _phantoms(3, 4242)
+    names = S._materials()
+    tab = mcpet.load_materials(names, np.max([p.density_max() for p in phs], axis=0))
+    est = S.ScatterEstimator()
+    alone = []
+    for k, ph in enumerate(phs):
+        svc = S.ScatterService(tab, est, budget_bytes=8 << 30)
+        sid = svc.submit(ph, seed=100 + k)
+        svc.drain()
+        alone.append(svc.result(sid))
+        svc.close()
+    svc = S.ScatterService(tab, est, budget_bytes=8 << 30, streams_per_step=1 << 18)
+    ids = [svc.submit(ph, seed=100 + k) for k, ph in enumerate(phs)]
+    svc.drain()
+    mix = [svc.result(i) for i in ids]
+    st = svc.stats()
+    svc.close()
+    for (ea, ia), (em, im) in zip(alone, mix):
+        assert np.array_equal(ea, em) and ia["decays"] == im["decays"]
+    # the direct path (Simulation + ScatterEstimator.estimate) gives the same estimate
+    with mcpet.Simulation(S.config(names, S.N_LOW), phs[0], tab) as sim:
+        e, info = est.estimate(sim, seed=100)
+    assert np.array_equal(e, alone[0][0]) and info["decays"] == alone[0][1]["decays"]
+    assert st["completed"] == 3 and st["server"]["peak_admitted_bytes"] <= 8 << 30
+    assert st["latency_p99_ms"] >= st["latency_p50_ms"] > 0
+    est.close()
+
+
+@pytest.mark.gpu
+def test_scatter_service_falls_back_to_monte_carlo():
+    """With a consistency threshold no estimate passes, every request is answered by a Monte
+    Carlo run of N5 decays (N5 from the prediction), equal to that run made directly."""
+    phs = _phantoms(1, 999)
+    names = S._materials()
+    tab = mcpet.load_materials(names, phs[0].density_max())
+    est = S.ScatterEstimator(z_max=-np.inf)
+    svc = S.ScatterService(tab, est, budget_bytes=8 << 30)
+    sid = svc.submit(phs[0], seed=5)
+    svc.drain()
+    res, info = svc.result(sid)
+    assert svc.stats()["fallbacks"] == 1 and not info["consistent"]
+    svc.close()
+    with mcpet.Simulation(S.config(names, info["n5"]), phs[0], tab) as sim:
+        sim.run(seed=5)
+        s = sim.rebin("scatter", S.GRID)
+        d = sim.stats()["decays"]
+    assert d == info["mc_decays"] and np.array_equal(res, s.astype(np.float64) / d)
+    est.close()
