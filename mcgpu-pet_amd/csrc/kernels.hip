@@ -44,6 +44,16 @@
 // The persistent kernel is bound by the latency of dependent gathers and transcendental
 // chains; 4 waves per SIMD (128 VGPRs, ~10 dwords of spill) hides more of it than 3 (+17%
 // on the head workload) or 5 (spills grow), and than the compiler's own 2.
+// Woodcock steps a lane may take in one STEP phase while its steps stay virtual, and GCOa
+// rejection trials (tau loop, p_z loop) in one CTAU / CPZ phase: the phase selection's ballots
+// are paid once for several. 2 and 2 measured best on the head workload (1, 2, 3, 4, 8 steps;
+// 1, 2, 3 trials); more inflate the register pressure of every phase.
+#ifndef MCPET_STEPS_PER_PHASE
+#define MCPET_STEPS_PER_PHASE 2
+#endif
+#ifndef MCPET_TRIALS_PER_PHASE
+#define MCPET_TRIALS_PER_PHASE 2
+#endif
 #ifndef MCPET_WAVES_PER_EU
 #define MCPET_WAVES_PER_EU 4
 #endif
@@ -1186,7 +1196,10 @@ __device__ __forceinline__ void sm_waves(const KParams& P, CN& C, const mcpet_co
 #endif
     if (state != best) {
     } else if (best == LS_STEP) {
-      const int o = woodcock_step(P, ph, st.r, C);
+      int o, k = 0;
+      do {
+        o = woodcock_step(P, ph, st.r, C);
+      } while (o == STEP_VIRTUAL && ++k < MCPET_STEPS_PER_PHASE);
       if (o == STEP_ESCAPED) state = after_photon(P, photon_no, prompt, ph, L);
       else if (o == STEP_REAL) state = LS_INTERACT;
     } else if (best == LS_INTERACT) {
@@ -1203,7 +1216,12 @@ __device__ __forceinline__ void sm_waves(const KParams& P, CN& C, const mcpet_co
     } else if (best == LS_CTAU) {
       ComptonState cs;
       cs_load(L, cs);
-      if (gcoa_tau_trial(ph.E, ph.mat0, st.r, C, s_cmp, cs)) {
+      bool ok;
+      int k = 0;
+      do {
+        ok = gcoa_tau_trial(ph.E, ph.mat0, st.r, C, s_cmp, cs);
+      } while (!ok && ++k < MCPET_TRIALS_PER_PHASE);
+      if (ok) {
         cs.trials = 0;
         state = LS_CPZ;
       }
@@ -1211,7 +1229,12 @@ __device__ __forceinline__ void sm_waves(const KParams& P, CN& C, const mcpet_co
     } else if (best == LS_CPZ) {
       ComptonState cs;
       cs_load(L, cs);
-      if (gcoa_pz_trial(ph.E, ph.mat0, st.r, C, s_cmp, cs)) {
+      bool ok;
+      int k = 0;
+      do {
+        ok = gcoa_pz_trial(ph.E, ph.mat0, st.r, C, s_cmp, cs);
+      } while (!ok && ++k < MCPET_TRIALS_PER_PHASE);
+      if (ok) {
         if (!compton_end(P, ph, st.r, C, cs, s_mat) || locate(P, ph.pos) < 0) state = after_photon(P, photon_no, prompt, ph, L);
         else state = LS_STEP;
       } else {
