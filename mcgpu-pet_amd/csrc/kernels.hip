@@ -402,7 +402,7 @@ struct Stream {
   Rng r;
   unsigned long long remaining;  // acquisition time left (ps)
   double inv_act;                // tpb / activity, decayed per emission
-  int absvox, ix, iy, iz;
+  int absvox;
   unsigned long long event;      // (reference thread id << 32) | number of the current decay
 };
 
@@ -766,9 +766,10 @@ __device__ __forceinline__ void source_first(const KParams& P, Stream& st, CN& C
   r1 = (K_EPS > r1 - K_EPS) ? K_EPS : r1 - K_EPS;
   r2 = (K_EPS > r2 - K_EPS) ? K_EPS : r2 - K_EPS;
   r3 = (K_EPS > r3 - K_EPS) ? K_EPS : r3 - K_EPS;
-  pos[0] = ((float)st.ix + r1) / P.ivx;
-  pos[1] = ((float)st.iy + r2) / P.ivy;
-  pos[2] = ((float)st.iz + r3) / P.ivz;
+  const int ix = st.absvox % P.nx, iy = (st.absvox / P.nx) % P.ny, iz = st.absvox / (P.nx * P.ny);
+  pos[0] = ((float)ix + r1) / P.ivx;
+  pos[1] = ((float)iy + r2) / P.ivy;
+  pos[2] = ((float)iz + r3) / P.ivz;
 }
 
 // Isotropic direction (K:711-720, the BG-mode second photon).
@@ -907,9 +908,6 @@ __device__ __forceinline__ void stream_init(const KParams& P, long long s, Strea
   }
   st.absvox = P.active_vox[k];
   const float act = P.active_act[k];
-  st.ix = st.absvox % P.nx;
-  st.iy = (st.absvox / P.nx) % P.ny;
-  st.iz = st.absvox / (P.nx * P.ny);
   mcr_init_stream(tid, P.seed, P.j1, P.j2, &st.r.x, &st.r.y);
   st.event = (unsigned long long)tid << 32;
   st.remaining = P.acq_ps;
@@ -1200,10 +1198,20 @@ __device__ __forceinline__ void sm_waves(const KParams& P, CN& C, const mcpet_co
       do {
         o = woodcock_step(P, ph, st.r, C);
       } while (o == STEP_VIRTUAL && ++k < MCPET_STEPS_PER_PHASE);
-      if (o == STEP_ESCAPED) state = after_photon(P, photon_no, prompt, ph, L);
-      else if (o == STEP_REAL) state = LS_INTERACT;
+      if (o == STEP_ESCAPED) {
+        state = after_photon(P, photon_no, prompt, ph, L);
+      } else if (o == STEP_REAL) {
+        // the interaction's inputs wait in LDS (Compton slots, free until GCOa starts)
+        L[CS_TAU] = ph.prob;
+        L[CS_S] = ph.mfp_density;
+        L[CS_CDTF] = ph.randno;
+        state = LS_INTERACT;
+      }
     } else if (best == LS_INTERACT) {
       ComptonState cs;
+      ph.prob = L[CS_TAU];
+      ph.mfp_density = L[CS_S];
+      ph.randno = L[CS_CDTF];
       const int o = interaction_begin(P, ph, st.r, C, s_cmp, cs, s_mat);
       if (o == IA_COMPTON) {
         cs_store(L, cs);
