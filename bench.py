@@ -124,6 +124,19 @@ def bench_batch(a):
     print(json.dumps(out))
 
 
+EVAL_FAMILIES = {
+    "scatter": "scatter_phantoms 128x128x64 at 2 mm, the training family",
+    "head": "head phantoms 128x128x96 at 2 mm, seed-jittered, not seen in training",
+    "batch": "random_phantoms 32-160 x 32-160 x 32-96 at 2 mm, not seen in training",
+}
+
+
+def eval_phantoms(family, n, seed):
+    """n phantoms of a family, activity scaled to 1e9 expected decays per second."""
+    from mcpet import surrogate as S
+    return S.family_phantoms(family, n, seed)
+
+
 def bench_surrogate(a):
     """Scatter surrogate workload: time to a 5 %-NRMSE scatter sinogram on the 32 x 56 x 56
     grid, Monte Carlo against the surrogate (low-count run + device rebinning + U-Net), on
@@ -139,7 +152,7 @@ def bench_surrogate(a):
     if not os.path.exists(weights):
         raise SystemExit(f"bench.py: surrogate weights {weights} missing (train with python -m mcpet.surrogate)")
     names = S._materials()
-    specs = phantoms.scatter_phantoms(a.eval_phantoms, a.eval_seed, n_decays=S.DECAYS_PER_SECOND, acquisition_time_s=1.0)
+    specs = eval_phantoms(a.eval_family, a.eval_phantoms, a.eval_seed)
     est = S.ScatterEstimator(weights)
     rows = []
     log = lambda m: print(m, file=sys.stderr, flush=True)
@@ -210,7 +223,7 @@ def bench_surrogate(a):
            "unit": "x (speedup)", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
            "ms_per_step": 1e3 * tot_eff / len(rows), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "bf16 MFMA (fp32 accumulate) + f32 transport", "data": "synthetic",
-           "config": {"workload": f"{len(rows)} evaluation phantoms 128x128x64 at 2 mm (scatter_phantoms seed "
+           "config": {"workload": f"{len(rows)} evaluation phantoms ({EVAL_FAMILIES[a.eval_family]}, seed "
                                   f"{a.eval_seed}), scatter on the 32x56x56 rebinned grid, surrogate input "
                                   f"{S.N_LOW:.0e} decays, U-Net width {S.WIDTH}", "parallelism": "single GPU"},
            "mc_seconds_mean": tot_mc / len(rows), "surrogate_seconds_mean": tot_eff / len(rows),
@@ -277,6 +290,8 @@ def main():
                          "unet: the surrogate network alone")
     ap.add_argument("--eval-phantoms", type=int, default=6)
     ap.add_argument("--eval-seed", type=int, default=777)
+    ap.add_argument("--eval-family", choices=sorted(EVAL_FAMILIES), default="scatter",
+                    help="surrogate workload: the phantom family of the evaluation")
     ap.add_argument("--surrogate-weights", default="", help="U-Net parameters (.npy); default: the packaged ones")
     ap.add_argument("--batch-size", type=int, default=48)
     ap.add_argument("--no-cpu-baseline", action="store_true")

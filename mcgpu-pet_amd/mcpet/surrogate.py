@@ -229,13 +229,35 @@ def _materials():
     return phantoms.head_materials(os.path.join(here, "tests", "golden", "sample_simulation", "materials"))
 
 
-def generate(n_phantoms, seed, n_target=5.0e8, n_low=N_LOW, inputs_per_phantom=4, chunk=16, log=print):
-    """Training pairs from scatter_phantoms(n_phantoms, seed): per phantom one target run of
-    n_target decays (scatter per decay) and inputs_per_phantom independent runs of n_low
-    decays (normalised scatter + trues). Returns X [n, 2, D, H, W], Y [n, D, H, W] (targets
-    in the inputs' normalisation) and the phantom index of each sample."""
+def family_phantoms(family, n, seed):
+    """n phantoms of a family with activity scaled to DECAYS_PER_SECOND expected decays per
+    second: "scatter" (scatter_phantoms, 128x128x64), "batch" (random_phantoms, 32-160 x 32-160 x
+    32-96 voxels, random ellipsoids), "head" (seed-jittered head phantoms, 128x128x96)."""
+    if family == "scatter":
+        return phantoms.scatter_phantoms(n, seed, n_decays=DECAYS_PER_SECOND, acquisition_time_s=1.0)
+    if family == "head":
+        return [phantoms.head(seed=seed + k, n_decays=DECAYS_PER_SECOND, acquisition_time_s=1.0) for k in range(n)]
+    if family == "batch":
+        frac = phantoms.F18_MEAN_LIFE * (1.0 - np.exp(-1.0 / phantoms.F18_MEAN_LIFE))
+        out = []
+        for mat, dens, act, vs in phantoms.random_phantoms(n, seed):
+            act = (act.astype(np.float64) * (DECAYS_PER_SECOND / (act.sum(dtype=np.float64) * frac))).astype(np.float32)
+            out.append((mat, dens, act, vs))
+        return out
+    raise ValueError(f"unknown phantom family {family}")
+
+
+def generate(n_phantoms, seed, n_target=5.0e8, n_low=N_LOW, inputs_per_phantom=4, chunk=16, families=("scatter",),
+             log=print):
+    """Training pairs from n_phantoms phantoms split evenly over `families` (family_phantoms,
+    seeds seed, seed + 1, ...): per phantom one target run of n_target decays (scatter per
+    decay) and inputs_per_phantom independent runs of n_low decays (normalised scatter +
+    trues). Returns X [n, 2, D, H, W], Y [n, D, H, W] (targets in the inputs' normalisation)
+    and the phantom index of each sample (phantoms interleaved over the families)."""
     names = _materials()
-    specs = phantoms.scatter_phantoms(n_phantoms, seed, n_decays=DECAYS_PER_SECOND, acquisition_time_s=1.0)
+    per = [n_phantoms // len(families) + (1 if i < n_phantoms % len(families) else 0) for i in range(len(families))]
+    lists = [family_phantoms(f, k, seed + i) for i, (f, k) in enumerate(zip(families, per))]
+    specs = [sp for group in zip(*[l + [None] * (max(per) - len(l)) for l in lists]) for sp in group if sp is not None]
     X, Y, idx = [], [], []
     for c0 in range(0, n_phantoms, chunk):
         part = specs[c0:c0 + chunk]
@@ -311,6 +333,7 @@ def main(argv=None):
     import argparse
     ap = argparse.ArgumentParser(description="train the scatter surrogate on simulator data")
     ap.add_argument("--phantoms", type=int, default=96)
+    ap.add_argument("--families", default="scatter", help="comma-separated phantom families (scatter, batch, head)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--n-target", type=float, default=5.0e8)
     ap.add_argument("--inputs-per-phantom", type=int, default=4)
@@ -322,12 +345,14 @@ def main(argv=None):
     os.makedirs(a.out, exist_ok=True)
     log = lambda m: print(m, flush=True)
     t0 = time.perf_counter()
-    X, Y, idx = generate(a.phantoms, a.seed, n_target=a.n_target, inputs_per_phantom=a.inputs_per_phantom, log=log)
+    fams = tuple(a.families.split(","))
+    X, Y, idx = generate(a.phantoms, a.seed, n_target=a.n_target, inputs_per_phantom=a.inputs_per_phantom, families=fams,
+                         log=log)
     gen_s = time.perf_counter() - t0
     log(f"generate: {len(X)} samples in {gen_s:.1f} s")
     w, info = train(X, Y, idx, width=a.width, epochs=a.epochs, lr=a.lr, log=log)
     np.save(os.path.join(a.out, f"scatter_unet_w{a.width}.npy"), w.astype(np.float32))
-    info.update({"phantoms": a.phantoms, "phantom_seed": a.seed, "n_target": a.n_target, "n_low": N_LOW,
+    info.update({"phantoms": a.phantoms, "families": fams, "phantom_seed": a.seed, "n_target": a.n_target, "n_low": N_LOW,
                  "inputs_per_phantom": a.inputs_per_phantom, "grid": GRID, "width": a.width, "generate_seconds": gen_s})
     with open(os.path.join(a.out, f"scatter_unet_w{a.width}.json"), "w") as f:
         json.dump(info, f)
