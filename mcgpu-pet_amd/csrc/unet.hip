@@ -191,16 +191,29 @@ hipError_t gemm(AOp A, BOp B, float* C, int M, int N, int K, int ldc, const floa
 //   FWD   : Y[co][p]        = act(sum_k Wf[co][k] X[c][s p + d_t] + b[co])    m = co, n = output voxel
 //   BDATA : dX[ci][p]      (+)= mask(sum_k Wb[ci][k] dY[c][q], s q + d_t = p)  m = ci, n = input voxel
 //   BWGT  : G[co][ci T + t] += sum_q dY[co][q] X[ci][s q + d_t]              m = co, n = t Cin + ci, k = q
-// Wf, Wb: BF16 copies of the FP32 weights in those layouts (zero padded). A workgroup is four
-// waves on one 16 MT x 16 output tile that split the k-steps between them and reduce through
-// LDS; BWGT also splits its (voxel) reduction over workgroups and adds with atomics.
+// Wf, Wb: BF16 copies of the FP32 weights in those layouts (zero padded). FWD and BDATA read
+// their B tensor from a channels-last BF16 copy ([voxel][Cp], padding zero), so a lane's 8
+// channels are one 16-byte load; their epilogues write such copies of what they produce (for
+// the next product) beside the FP32 channel-first tensor (which BWGT, the bias gradients and
+// the ReLU masks read). A workgroup is four waves on one 16 MT x 16 output tile that split the
+// k-steps between them and reduce through LDS; BWGT also splits its (voxel) reduction over
+// workgroups and adds with atomics.
 
 enum { FWD = 0, BDATA = 1, BWGT = 2 };
+
+// Where an epilogue writes a channels-last BF16 copy of its output: element (m, n) at
+// p[n * ld + off + m] (p null: no copy).
+struct ClOut {
+  __bf16* p;
+  int ld, off;
+};
 
 struct ConvK {
   const __bf16* A;     // FWD, BDATA: BF16 weights [rows][Kp]
   const float* Ad;     // BWGT: dY [M][K]
-  const float* X;      // FWD, BWGT: input activations [Cin][D][H][W]; BDATA: dY [Cout][oD][oH][oW]
+  const float* X;      // BWGT: input activations [Cin][D][H][W]
+  const __bf16* Xl;    // FWD: input activations [D][H][W][Cbp]; BDATA: dY [oD][oH][oW][Cbp] (channels-last BF16)
+  ClOut o1, o2;        // FWD, BDATA: channels-last copies of the output
   float* C;            // FWD: Y [M][N]; BDATA: dX [M][N]; BWGT: weight gradient [Cout][Cin][T]
   const float* bias;   // FWD
   const float* mask;   // BDATA: the layer input (gradient zero where it is <= 0: the ReLU before it); may be null
@@ -278,21 +291,22 @@ __global__ __launch_bounds__(256) void k_conv(ConvK a)
       int dz, dy, dx;
       tap3(t, a.ks, dz, dy, dx);
       bool in = nvalid && k < ke;
-      const float* base = nullptr;
-      size_t cstride;
+      size_t vox = 0;
       if (MODE == FWD) {
         const int z = cz * a.stride + dz, y = cy * a.stride + dy, x = cx * a.stride + dx;
         in = in && z >= 0 && y >= 0 && x >= 0 && z < a.D && y < a.H && x < a.W;
-        base = a.X + (size_t)c0 * vin + ((size_t)z * a.H + y) * a.W + x;
-        cstride = vin;
+        vox = ((size_t)z * a.H + y) * a.W + x;
       } else {
         int qz = 0, qy = 0, qx = 0;
         in = in && src_of(cz, dz, a.stride, a.oD, qz) && src_of(cy, dy, a.stride, a.oH, qy) && src_of(cx, dx, a.stride, a.oW, qx);
-        base = a.X + (size_t)c0 * vout + ((size_t)qz * a.oH + qy) * a.oW + qx;
-        cstride = vout;
+        vox = ((size_t)qz * a.oH + qy) * a.oW + qx;
       }
+      if (in) {
+        b = *(const bf16x8*)(a.Xl + vox * a.Cbp + c0);
+      } else {
 #pragma unroll
-      for (int e = 0; e < 8; e++) b[e] = to_bf16((in && c0 + e < a.Cb) ? base[e * cstride] : 0.0f);
+        for (int e = 0; e < 8; e++) b[e] = to_bf16(0.0f);
+      }
 #pragma unroll
       for (int i = 0; i < MT; i++) {
         const bf16x8 av = *(const bf16x8*)(a.A + (size_t)(m0 + 16 * i + r) * a.Kp + k);
@@ -345,15 +359,19 @@ __global__ __launch_bounds__(256) void k_conv(ConvK a)
     for (int v = 0; v < 4; v++) {
       const int row = m0 + 16 * i + kg * 4 + v;
       if (row >= a.M || !nvalid) continue;
-      if (MODE == FWD) {
-        float o = s[v] + a.bias[row];
-        if (a.relu) o = fmaxf(o, 0.0f);
-        a.C[(size_t)row * a.N + n] = o;
-      } else if (MODE == BDATA) {
+      if (MODE == FWD || MODE == BDATA) {
         const size_t idx = (size_t)row * a.N + n;
-        float o = s[v] + (a.beta ? a.C[idx] : 0.0f);
-        if (a.mask && !(a.mask[idx] > 0.0f)) o = 0.0f;
+        float o;
+        if (MODE == FWD) {
+          o = s[v] + a.bias[row];
+          if (a.relu) o = fmaxf(o, 0.0f);
+        } else {
+          o = s[v] + (a.beta ? a.C[idx] : 0.0f);
+          if (a.mask && !(a.mask[idx] > 0.0f)) o = 0.0f;
+        }
         a.C[idx] = o;
+        if (a.o1.p) a.o1.p[(size_t)n * a.o1.ld + a.o1.off + row] = to_bf16(o);
+        if (a.o2.p) a.o2.p[(size_t)n * a.o2.ld + a.o2.off + row] = to_bf16(o);
       } else {
         atomicAdd(&a.C[((size_t)row * a.Cb + bci) * a.T + bt], s[v]);
       }
@@ -417,18 +435,28 @@ __global__ void k_derive(const float* __restrict__ w, __bf16* __restrict__ out, 
 
 // ------------------------------------------------------------------ elementwise kernels
 
-// nearest x2 upsampling of [C][d][h][w] into [C][2d][2h][2w]
-__global__ void k_upsample(const float* __restrict__ x, float* __restrict__ y, int C, int d, int h, int w)
+// nearest x2 upsampling of [C][d][h][w] into [C][2d][2h][2w] (and its channels-last copy o)
+__global__ void k_upsample(const float* __restrict__ x, float* __restrict__ y, int C, int d, int h, int w, ClOut o)
 {
   const size_t n = (size_t)C * 8 * d * h * w;
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int W2 = 2 * w, H2 = 2 * h, D2 = 2 * d;
+  const size_t vox2 = (size_t)W2 * H2 * D2;
   const int xx = (int)(i % W2), yy = (int)((i / W2) % H2), zz = (int)((i / ((size_t)W2 * H2)) % D2);
-  const int c = (int)(i / ((size_t)W2 * H2 * D2));
-  y[i] = x[(((size_t)c * d + zz / 2) * h + yy / 2) * w + xx / 2];
+  const int c = (int)(i / vox2);
+  const float v = x[(((size_t)c * d + zz / 2) * h + yy / 2) * w + xx / 2];
+  y[i] = v;
+  if (o.p) o.p[(i % vox2) * o.ld + o.off + c] = to_bf16(v);
 }
-__global__ void k_upsample_bwd(const float* __restrict__ gy, float* __restrict__ gx, int C, int d, int h, int w)
+// channel-first FP32 [C][vox] -> channels-last BF16 copy
+__global__ void k_to_cl(const float* __restrict__ x, int C, size_t vox, ClOut o)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)C * vox) return;
+  o.p[(i % vox) * o.ld + o.off + i / vox] = to_bf16(x[i]);
+}
+__global__ void k_upsample_bwd(const float* __restrict__ gy, float* __restrict__ gx, int C, int d, int h, int w, ClOut o)
 {
   const size_t n = (size_t)C * d * h * w;
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -441,6 +469,7 @@ __global__ void k_upsample_bwd(const float* __restrict__ gy, float* __restrict__
     for (int b = 0; b < 2; b++)
       for (int e = 0; e < 2; e++) s += gy[(((size_t)c * D2 + 2 * zz + a) * H2 + 2 * yy + b) * W2 + 2 * xx + e];
   gx[i] = s;
+  if (o.p) o.p[(i % ((size_t)w * h * d)) * o.ld + o.off + c] = to_bf16(s);
 }
 // per-channel sum (bias gradient): one workgroup per channel
 __global__ void k_channel_sum(const float* __restrict__ g, float* __restrict__ out, int vox)
@@ -459,7 +488,8 @@ __global__ void k_channel_sum(const float* __restrict__ g, float* __restrict__ o
 }
 // out = x0 + r (residual output) and, with a target, the loss gradient and squared error
 __global__ void k_output(const float* __restrict__ x0, const float* __restrict__ r, float* __restrict__ out,
-                         const float* __restrict__ target, float* __restrict__ g, float scale, float* __restrict__ sse, int n)
+                         const float* __restrict__ target, float* __restrict__ g, float scale, float* __restrict__ sse, int n,
+                         ClOut gl)
 {
   __shared__ float sh[256];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -470,6 +500,7 @@ __global__ void k_output(const float* __restrict__ x0, const float* __restrict__
     if (target) {
       const float d = o - target[i];
       g[i] = 2.0f * d * scale;
+      if (gl.p) gl.p[(size_t)i * gl.ld] = to_bf16(g[i]);
       e2 = d * d;
     }
   }
@@ -535,6 +566,11 @@ struct mcpet_surrogate {
         *e2 = nullptr, *e1 = nullptr, *r = nullptr, *out = nullptr, *tgt = nullptr;
   float *ga1 = nullptr, *gcat1 = nullptr, *gd1 = nullptr, *gcat2 = nullptr, *gd2 = nullptr, *gb3 = nullptr, *ge2 = nullptr,
         *ge1 = nullptr, *gr = nullptr, *sse = nullptr;
+  // channels-last BF16 copies read as B operands (activations, then gradients)
+  __bf16 *x0l = nullptr, *a1l = nullptr, *b1l = nullptr, *cat1l = nullptr, *d1l = nullptr, *b2l = nullptr, *cat2l = nullptr,
+         *d2l = nullptr, *e1l = nullptr;
+  __bf16 *grl = nullptr, *ge1l = nullptr, *ge2l = nullptr, *gb3l = nullptr, *gd2l = nullptr, *gb2l = nullptr, *gd1l = nullptr,
+         *gb1l = nullptr;
   long long t_adam = 0;
   hipStream_t st = nullptr;
   std::vector<void*> allocs;
@@ -599,13 +635,18 @@ ConvK conv_args(const Conv& l, const Vol& vin, const Vol& vout)
   return a;
 }
 
-// y = act(conv(x) + b)
-hipError_t conv_fwd(mcpet_surrogate* s, const Conv& l, const Vol& vin, const float* x, const Vol& vout, float* y, bool relu)
+const ClOut kNoCl{nullptr, 0, 0};
+
+// y = act(conv(x) + b); x read from its channels-last copy xl; o1, o2: copies of y to write
+hipError_t conv_fwd(mcpet_surrogate* s, const Conv& l, const Vol& vin, const __bf16* xl, const Vol& vout, float* y, bool relu,
+                    ClOut o1, ClOut o2 = kNoCl)
 {
   ConvK a = conv_args(l, vin, vout);
   a.A = s->wbf + l.wf;
-  a.X = x;
+  a.Xl = xl;
   a.C = y;
+  a.o1 = o1;
+  a.o2 = o2;
   a.bias = s->p + l.b;
   a.relu = relu ? 1 : 0;
   a.M = l.cout;
@@ -617,14 +658,16 @@ hipError_t conv_fwd(mcpet_surrogate* s, const Conv& l, const Vol& vin, const flo
   s->flops_fwd += 2.0 * l.cout * (double)l.cin * a.T * a.N;
   return launch_conv<FWD>(a, s->st);
 }
-// gx (+)= dL/dx of the layer input, zeroed where mask <= 0
-hipError_t conv_bwd_data(mcpet_surrogate* s, const Conv& l, const Vol& vin, const Vol& vout, const float* gy, float* gx,
-                         const float* mask, bool add)
+// gx (+)= dL/dx of the layer input, zeroed where mask <= 0; gy read from its channels-last copy
+hipError_t conv_bwd_data(mcpet_surrogate* s, const Conv& l, const Vol& vin, const Vol& vout, const __bf16* gyl, float* gx,
+                         const float* mask, bool add, ClOut o1)
 {
   ConvK a = conv_args(l, vin, vout);
   a.A = s->wbf + l.wb;
-  a.X = gy;
+  a.Xl = gyl;
   a.C = gx;
+  a.o1 = o1;
+  a.o2 = kNoCl;
   a.mask = mask;
   a.beta = add ? 1 : 0;
   a.M = l.cin;
@@ -652,73 +695,79 @@ hipError_t conv_bwd_weights(mcpet_surrogate* s, const Conv& l, const Vol& vin, c
   hipLaunchKernelGGL(k_channel_sum, dim3(l.cout), dim3(256), 0, s->st, gy, s->g + l.b, (int)vout.vox());
   return hipGetLastError();
 }
-hipError_t upsample(mcpet_surrogate* s, const Vol& v, const float* x, float* y)
+hipError_t upsample(mcpet_surrogate* s, const Vol& v, const float* x, float* y, ClOut o)
 {
-  hipLaunchKernelGGL(k_upsample, dim3(blocks_for(v.size() * 8)), dim3(256), 0, s->st, x, y, v.C, v.D, v.H, v.W);
+  hipLaunchKernelGGL(k_upsample, dim3(blocks_for(v.size() * 8)), dim3(256), 0, s->st, x, y, v.C, v.D, v.H, v.W, o);
   return hipGetLastError();
 }
-hipError_t upsample_bwd(mcpet_surrogate* s, const Vol& v, const float* gy, float* gx)
+hipError_t upsample_bwd(mcpet_surrogate* s, const Vol& v, const float* gy, float* gx, ClOut o)
 {
-  hipLaunchKernelGGL(k_upsample_bwd, dim3(blocks_for(v.size())), dim3(256), 0, s->st, gy, gx, v.C, v.D, v.H, v.W);
+  hipLaunchKernelGGL(k_upsample_bwd, dim3(blocks_for(v.size())), dim3(256), 0, s->st, gy, gx, v.C, v.D, v.H, v.W, o);
   return hipGetLastError();
 }
+ClOut cl(__bf16* p, int channels, int off = 0) { return ClOut{p, rup(channels, 8), off}; }
 
 // Forward of one sample already in s->x0 (r = the residual).
 hipError_t forward(mcpet_surrogate* s)
 {
   hipError_t e = hipSuccess;
-  const Vol vb1{s->c, s->D, s->H, s->W}, vb2{2 * s->c, s->D / 2, s->H / 2, s->W / 2};
-  float* b1 = s->cat1 + (size_t)2 * s->c * vb1.vox();   // enc1 output lives in the concat buffer
-  float* b2 = s->cat2 + (size_t)4 * s->c * vb2.vox();
+  const int c = s->c;
+  const Vol vb1{c, s->D, s->H, s->W}, vb2{2 * c, s->D / 2, s->H / 2, s->W / 2};
+  float* b1 = s->cat1 + (size_t)2 * c * vb1.vox();   // enc1 output lives in the concat buffer
+  float* b2 = s->cat2 + (size_t)4 * c * vb2.vox();
   s->flops_fwd = 0.0;
+  hipLaunchKernelGGL(k_to_cl, dim3(blocks_for(s->vx.size())), dim3(256), 0, s->st, s->x0, s->cin, s->vx.vox(), cl(s->x0l, s->cin));
+  e = hipGetLastError();
 #define STEP(x) \
   if (e == hipSuccess) e = (x)
-  STEP(conv_fwd(s, s->L[0], s->vx, s->x0, s->va1, s->a1, true));
-  STEP(conv_fwd(s, s->L[1], s->va1, s->a1, vb1, b1, true));
-  STEP(conv_fwd(s, s->L[2], vb1, b1, s->vd1, s->d1, true));
-  STEP(conv_fwd(s, s->L[3], s->vd1, s->d1, vb2, b2, true));
-  STEP(conv_fwd(s, s->L[4], vb2, b2, s->vd2, s->d2, true));
-  STEP(conv_fwd(s, s->L[5], s->vd2, s->d2, s->vb3, s->b3, true));
-  STEP(upsample(s, s->vb3, s->b3, s->cat2));           // first 4c channels of cat2
-  STEP(conv_fwd(s, s->L[6], s->vcat2, s->cat2, s->ve2, s->e2, true));
-  STEP(upsample(s, s->ve2, s->e2, s->cat1));           // first 2c channels of cat1
-  STEP(conv_fwd(s, s->L[7], s->vcat1, s->cat1, s->ve1, s->e1, true));
-  STEP(conv_fwd(s, s->L[8], s->ve1, s->e1, s->vr, s->r, false));
+  STEP(conv_fwd(s, s->L[0], s->vx, s->x0l, s->va1, s->a1, true, cl(s->a1l, c)));
+  STEP(conv_fwd(s, s->L[1], s->va1, s->a1l, vb1, b1, true, cl(s->b1l, c), cl(s->cat1l, 3 * c, 2 * c)));
+  STEP(conv_fwd(s, s->L[2], vb1, s->b1l, s->vd1, s->d1, true, cl(s->d1l, 2 * c)));
+  STEP(conv_fwd(s, s->L[3], s->vd1, s->d1l, vb2, b2, true, cl(s->b2l, 2 * c), cl(s->cat2l, 6 * c, 4 * c)));
+  STEP(conv_fwd(s, s->L[4], vb2, s->b2l, s->vd2, s->d2, true, cl(s->d2l, 4 * c)));
+  STEP(conv_fwd(s, s->L[5], s->vd2, s->d2l, s->vb3, s->b3, true, kNoCl));
+  STEP(upsample(s, s->vb3, s->b3, s->cat2, cl(s->cat2l, 6 * c)));    // first 4c channels of cat2
+  STEP(conv_fwd(s, s->L[6], s->vcat2, s->cat2l, s->ve2, s->e2, true, kNoCl));
+  STEP(upsample(s, s->ve2, s->e2, s->cat1, cl(s->cat1l, 3 * c)));    // first 2c channels of cat1
+  STEP(conv_fwd(s, s->L[7], s->vcat1, s->cat1l, s->ve1, s->e1, true, cl(s->e1l, c)));
+  STEP(conv_fwd(s, s->L[8], s->ve1, s->e1l, s->vr, s->r, false, kNoCl));
 #undef STEP
   return e;
 }
 
-// Backward of one sample from gr (= dL/dr); accumulates the parameter gradients. The ReLU
-// derivative is applied where each gradient is produced (mask = the post-ReLU activation;
-// the upsampled and concatenated copies carry the same sign as their sources).
+// Backward of one sample from gr (= dL/dr, with its channels-last copy); accumulates the
+// parameter gradients. The ReLU derivative is applied where each gradient is produced (mask =
+// the post-ReLU activation; the upsampled and concatenated copies carry the same sign as their
+// sources). A gradient's channels-last copy is written by the product that completes it.
 hipError_t backward(mcpet_surrogate* s)
 {
   hipError_t e = hipSuccess;
-  const Vol vb1{s->c, s->D, s->H, s->W}, vb2{2 * s->c, s->D / 2, s->H / 2, s->W / 2};
-  const float* b1 = s->cat1 + (size_t)2 * s->c * vb1.vox();
-  const float* b2 = s->cat2 + (size_t)4 * s->c * vb2.vox();
-  float* gb1 = s->gcat1 + (size_t)2 * s->c * vb1.vox();
-  float* gb2 = s->gcat2 + (size_t)4 * s->c * vb2.vox();
+  const int c = s->c;
+  const Vol vb1{c, s->D, s->H, s->W}, vb2{2 * c, s->D / 2, s->H / 2, s->W / 2};
+  const float* b1 = s->cat1 + (size_t)2 * c * vb1.vox();
+  const float* b2 = s->cat2 + (size_t)4 * c * vb2.vox();
+  float* gb1 = s->gcat1 + (size_t)2 * c * vb1.vox();
+  float* gb2 = s->gcat2 + (size_t)4 * c * vb2.vox();
 #define STEP(x) \
   if (e == hipSuccess) e = (x)
   STEP(conv_bwd_weights(s, s->L[8], s->ve1, s->e1, s->vr, s->gr));
-  STEP(conv_bwd_data(s, s->L[8], s->ve1, s->vr, s->gr, s->ge1, s->e1, false));
+  STEP(conv_bwd_data(s, s->L[8], s->ve1, s->vr, s->grl, s->ge1, s->e1, false, cl(s->ge1l, c)));
   STEP(conv_bwd_weights(s, s->L[7], s->vcat1, s->cat1, s->ve1, s->ge1));
-  STEP(conv_bwd_data(s, s->L[7], s->vcat1, s->ve1, s->ge1, s->gcat1, s->cat1, false));   // g(up1) and g(b1 skip)
-  STEP(upsample_bwd(s, s->ve2, s->gcat1, s->ge2));
+  STEP(conv_bwd_data(s, s->L[7], s->vcat1, s->ve1, s->ge1l, s->gcat1, s->cat1, false, kNoCl));   // g(up1), g(b1 skip)
+  STEP(upsample_bwd(s, s->ve2, s->gcat1, s->ge2, cl(s->ge2l, 2 * c)));
   STEP(conv_bwd_weights(s, s->L[6], s->vcat2, s->cat2, s->ve2, s->ge2));
-  STEP(conv_bwd_data(s, s->L[6], s->vcat2, s->ve2, s->ge2, s->gcat2, s->cat2, false));   // g(up2) and g(b2 skip)
-  STEP(upsample_bwd(s, s->vb3, s->gcat2, s->gb3));
+  STEP(conv_bwd_data(s, s->L[6], s->vcat2, s->ve2, s->ge2l, s->gcat2, s->cat2, false, kNoCl));   // g(up2), g(b2 skip)
+  STEP(upsample_bwd(s, s->vb3, s->gcat2, s->gb3, cl(s->gb3l, 4 * c)));
   STEP(conv_bwd_weights(s, s->L[5], s->vd2, s->d2, s->vb3, s->gb3));
-  STEP(conv_bwd_data(s, s->L[5], s->vd2, s->vb3, s->gb3, s->gd2, s->d2, false));
+  STEP(conv_bwd_data(s, s->L[5], s->vd2, s->vb3, s->gb3l, s->gd2, s->d2, false, cl(s->gd2l, 4 * c)));
   STEP(conv_bwd_weights(s, s->L[4], vb2, b2, s->vd2, s->gd2));
-  STEP(conv_bwd_data(s, s->L[4], vb2, s->vd2, s->gd2, gb2, b2, true));                  // + the skip's gradient
+  STEP(conv_bwd_data(s, s->L[4], vb2, s->vd2, s->gd2l, gb2, b2, true, cl(s->gb2l, 2 * c)));   // + the skip's gradient
   STEP(conv_bwd_weights(s, s->L[3], s->vd1, s->d1, vb2, gb2));
-  STEP(conv_bwd_data(s, s->L[3], s->vd1, vb2, gb2, s->gd1, s->d1, false));
+  STEP(conv_bwd_data(s, s->L[3], s->vd1, vb2, s->gb2l, s->gd1, s->d1, false, cl(s->gd1l, 2 * c)));
   STEP(conv_bwd_weights(s, s->L[2], vb1, b1, s->vd1, s->gd1));
-  STEP(conv_bwd_data(s, s->L[2], vb1, s->vd1, s->gd1, gb1, b1, true));                  // + the skip's gradient
+  STEP(conv_bwd_data(s, s->L[2], vb1, s->vd1, s->gd1l, gb1, b1, true, cl(s->gb1l, c)));       // + the skip's gradient
   STEP(conv_bwd_weights(s, s->L[1], s->va1, s->a1, vb1, gb1));
-  STEP(conv_bwd_data(s, s->L[1], s->va1, vb1, gb1, s->ga1, s->a1, false));
+  STEP(conv_bwd_data(s, s->L[1], s->va1, vb1, s->gb1l, s->ga1, s->a1, false, kNoCl));
   STEP(conv_bwd_weights(s, s->L[0], s->vx, s->x0, s->va1, s->ga1));
 #undef STEP
   return e;
@@ -791,6 +840,20 @@ extern "C" mcpet_status mcpet_surrogate_create(int32_t D, int32_t H, int32_t W, 
   for (auto& a : al)
     if (e == hipSuccess) e = dalloc_t(s, a.p, a.n);
   if (e == hipSuccess) e = dalloc_t(s, &s->wbf, s->n_bf16);
+  {
+    const size_t v0 = s->vx.vox(), v1 = s->vd1.vox(), v2 = s->vd2.vox();
+    struct B {
+      __bf16** p;
+      size_t n;
+    } bl[] = {{&s->x0l, v0 * rup(in_channels, 8)}, {&s->a1l, v0 * rup(c, 8)},      {&s->b1l, v0 * rup(c, 8)},
+              {&s->cat1l, v0 * rup(3 * c, 8)},      {&s->d1l, v1 * rup(2 * c, 8)}, {&s->b2l, v1 * rup(2 * c, 8)},
+              {&s->cat2l, v1 * rup(6 * c, 8)},      {&s->d2l, v2 * rup(4 * c, 8)}, {&s->e1l, v0 * rup(c, 8)},
+              {&s->grl, v0 * 8},                    {&s->ge1l, v0 * rup(c, 8)},    {&s->ge2l, v1 * rup(2 * c, 8)},
+              {&s->gb3l, v2 * rup(4 * c, 8)},       {&s->gd2l, v2 * rup(4 * c, 8)}, {&s->gb2l, v1 * rup(2 * c, 8)},
+              {&s->gd1l, v1 * rup(2 * c, 8)},       {&s->gb1l, v0 * rup(c, 8)}};
+    for (auto& x : bl)
+      if (e == hipSuccess) e = dalloc_t(s, x.p, x.n);
+  }
   if (e != hipSuccess) {
     std::string m = std::string("mcpet_surrogate_create: ") + hipGetErrorString(e);
     mcpet_surrogate_destroy(s);
@@ -854,7 +917,7 @@ extern "C" mcpet_status mcpet_surrogate_forward(mcpet_surrogate* s, const float*
     SHIP(s, hipMemcpyAsync(s->x0, x + b * nx, sizeof(float) * nx, hipMemcpyHostToDevice, s->st));
     SHIP(s, forward(s));
     hipLaunchKernelGGL(k_output, dim3(blocks_for(nv)), dim3(256), 0, s->st, s->x0, s->r, s->out, (const float*)nullptr,
-                       (float*)nullptr, 0.0f, (float*)nullptr, (int)nv);
+                       (float*)nullptr, 0.0f, (float*)nullptr, (int)nv, kNoCl);
     SHIP(s, hipGetLastError());
     SHIP(s, hipMemcpyAsync(y + b * nv, s->out, sizeof(float) * nv, hipMemcpyDeviceToHost, s->st));
   }
@@ -877,7 +940,7 @@ static mcpet_status batch_gradients(mcpet_surrogate* s, const float* x, const fl
     SHIP(s, hipMemcpyAsync(s->tgt, target + b * nv, sizeof(float) * nv, hipMemcpyHostToDevice, s->st));
     SHIP(s, forward(s));
     hipLaunchKernelGGL(k_output, dim3(blocks_for(nv)), dim3(256), 0, s->st, s->x0, s->r, s->out, s->tgt, s->gr, scale, s->sse,
-                       (int)nv);
+                       (int)nv, ClOut{s->grl, 8, 0});
     SHIP(s, hipGetLastError());
     SHIP(s, backward(s));
   }
