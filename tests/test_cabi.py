@@ -50,3 +50,21 @@ def test_create_rejects_bad_arguments_without_touching_the_device():
     st = mcpet.lib().mcpet_create(ctypes.byref(cfg), ctypes.byref(ph), ctypes.byref(tab), ctypes.byref(ctx))
     assert st == 1 and not ctx.value
     assert b"null argument" in mcpet.lib().mcpet_last_error(None)
+
+
+def test_surrogate_and_rebinning_entry_points_reject_bad_arguments_without_the_device():
+    L = mcpet.lib()
+    s = ctypes.c_void_p()
+    for dims in ((30, 56, 56), (32, 0, 56), (2, 4, 4)):        # not multiples of 4 / empty / too small
+        assert L.mcpet_surrogate_create(*dims, 2, 32, 1, ctypes.byref(s)) == 1 and not s.value
+        assert b"multiples of 4" in L.mcpet_surrogate_last_error(None)
+    assert L.mcpet_surrogate_create(32, 56, 56, 0, 32, 1, ctypes.byref(s)) == 1          # no input channel
+    assert L.mcpet_surrogate_create(32, 56, 56, 2, 32, 1, None) == 1                      # no output pointer
+    out = (ctypes.c_float * 8)()
+    assert L.mcpet_rebin_sinogram(None, 1, 2, 2, 2, out) == 1
+    assert L.mcpet_surrogate_param_count(None) == 0
+    # the plane map needs a config whose sizes were derived
+    cfg = mcpet.Config()
+    planes = (ctypes.c_int32 * 4)()
+    assert L.mcpet_sinogram_planes(ctypes.byref(cfg), planes) == 1
+    assert L.mcpet_server_fetch_rebinned(None, 1, None, None, None) == 1
