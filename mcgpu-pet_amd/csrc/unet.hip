@@ -471,20 +471,21 @@ __global__ void k_upsample_bwd(const float* __restrict__ gy, float* __restrict__
   gx[i] = s;
   if (o.p) o.p[(i % ((size_t)w * h * d)) * o.ld + o.off + c] = to_bf16(s);
 }
-// per-channel sum (bias gradient): one workgroup per channel
-__global__ void k_channel_sum(const float* __restrict__ g, float* __restrict__ out, int vox)
+// per-channel sum (bias gradient): workgroups (channel, chunk of `per` voxels), added atomically
+__global__ void k_channel_sum(const float* __restrict__ g, float* __restrict__ out, int vox, int per)
 {
   __shared__ float sh[256];
   const float* p = g + (size_t)blockIdx.x * vox;
+  const int b = blockIdx.y * per, e = min(vox, b + per);
   float s = 0.0f;
-  for (int i = threadIdx.x; i < vox; i += blockDim.x) s += p[i];
+  for (int i = b + threadIdx.x; i < e; i += blockDim.x) s += p[i];
   sh[threadIdx.x] = s;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[blockIdx.x] += sh[0];
+  if (threadIdx.x == 0) atomicAdd(&out[blockIdx.x], sh[0]);
 }
 // out = x0 + r (residual output) and, with a target, the loss gradient and squared error
 __global__ void k_output(const float* __restrict__ x0, const float* __restrict__ r, float* __restrict__ out,
@@ -692,7 +693,8 @@ hipError_t conv_bwd_weights(mcpet_surrogate* s, const Conv& l, const Vol& vin, c
   a.Cbp = l.cin;
   hipError_t e = launch_conv<BWGT>(a, s->st);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_channel_sum, dim3(l.cout), dim3(256), 0, s->st, gy, s->g + l.b, (int)vout.vox());
+  const int vox = (int)vout.vox(), per = 4096;
+  hipLaunchKernelGGL(k_channel_sum, dim3(l.cout, (vox + per - 1) / per), dim3(256), 0, s->st, gy, s->g + l.b, vox, per);
   return hipGetLastError();
 }
 hipError_t upsample(mcpet_surrogate* s, const Vol& v, const float* x, float* y, ClOut o)
