@@ -93,7 +93,7 @@ def consistency_z(pred_per_decay, scatter_low, n_low, block=(4, 4, 4), min_mean=
     dof = int(m.sum())
     if dof == 0:
         return float("inf")
-    chi2 = float((((n_b - mu_b) ** 2) / mu_b)[m].sum())
+    chi2 = float((((n_b - mu_b)[m]) ** 2 / mu_b[m]).sum())
     return (chi2 - dof) / np.sqrt(2.0 * dof)
 
 
@@ -139,11 +139,19 @@ class ScatterService:
     request's low-count run is a rebinned server request, batched with the other requests'
     runs; when it completes the U-Net estimate is checked (consistency z-score) and, if it
     fails, the request is resubmitted as a Monte Carlo run of N5 decays (N5 from the
-    prediction) whose scatter is the answer. A request's answer does not depend on the
-    other requests in flight."""
+    prediction) whose scatter is the answer.
+
+    Spot checks (spot_check_every = k > 0): every k-th request's estimate is also compared with
+    a Monte Carlo run of N5 decays; the surrogate's own error is estimated as
+    sqrt(max(0, NRMSE(estimate, MC)^2 - floor(MC)^2)) (the Monte Carlo run's Poisson floor
+    removed). Above TARGET the request is answered by that Monte Carlo run, and after
+    `disable_after` consecutive failed spot checks the surrogate is switched off: later
+    requests are all answered by Monte Carlo (a prediction that merely copies its noisy input
+    passes the consistency test but not a spot check). A request's answer does not depend
+    on the other requests in flight."""
 
     def __init__(self, tables, estimator, budget_bytes=16 << 30, max_running=16, streams_per_step=1 << 22,
-                 materials=None):
+                 materials=None, spot_check_every=0, disable_after=2):
         from . import Server
         self.server = Server(tables, budget_bytes, max_running=max_running, streams_per_step=streams_per_step)
         self.est = estimator
@@ -153,15 +161,27 @@ class ScatterService:
         self.next_id = 1
         self.latencies = []
         self.fallbacks = 0
+        self.spot_every = int(spot_check_every)
+        self.disable_after = int(disable_after)
+        self.spot_checks = []   # estimated surrogate errors of the spot checks, in completion order
+        self.failed_in_row = 0
+        self.disabled = False
 
     def submit(self, phantom, seed=1):
         sid = self.next_id
         self.next_id += 1
         rid = self.server.submit_rebinned(config(self.names, N_LOW), phantom, self.est.grid, seed=seed)
+        spot = self.spot_every > 0 and sid % self.spot_every == 0
         self.jobs[sid] = {"phantom": phantom, "seed": seed, "stage": "surrogate", "t0": time.perf_counter(),
-                          "result": None, "info": None}
+                          "result": None, "info": None, "spot": spot}
         self.by_rid[rid] = sid
         return sid
+
+    def _monte_carlo(self, sid, job, n5, stage):
+        job["stage"] = stage
+        job["info"]["n5"] = n5
+        rid = self.server.submit_rebinned(config(self.names, n5), job["phantom"], self.est.grid, seed=job["seed"])
+        self.by_rid[rid] = sid
 
     def _complete(self, rid):
         sid = self.by_rid.pop(rid)
@@ -171,19 +191,36 @@ class ScatterService:
             est = self.est.predict(s, t, decays)
             z = consistency_z(est, s, decays)
             job["info"] = {"decays": decays, "z": z, "consistent": bool(z <= self.est.z_max)}
-            if job["info"]["consistent"]:
-                job["result"] = est
-            else:   # fall back: Monte Carlo to the decays a 5 % estimate needs
+            n5 = decays_for(est) if est.sum() > 0 else 1e9
+            if not job["info"]["consistent"] or self.disabled:
+                # fall back: Monte Carlo to the decays a 5 % estimate needs
                 self.fallbacks += 1
-                n5 = decays_for(est) if est.sum() > 0 else 1e9
-                job["stage"] = "monte_carlo"
-                job["info"]["n5"] = n5
-                rid2 = self.server.submit_rebinned(config(self.names, n5), job["phantom"], self.est.grid, seed=job["seed"])
-                self.by_rid[rid2] = sid
+                self._monte_carlo(sid, job, n5, "monte_carlo")
+                return
+            job["result"] = est
+            if job["spot"]:
+                self._monte_carlo(sid, job, n5, "spot_check")
                 return
         else:
-            job["result"] = s.astype(np.float64) / decays
+            mc = s.astype(np.float64) / decays
             job["info"]["mc_decays"] = decays
+            if job["stage"] == "spot_check":
+                e = nrmse(job["result"], mc)
+                floor = poisson_floor(s)
+                err = float(np.sqrt(max(0.0, e * e - floor * floor)))
+                job["info"].update({"spot_nrmse": e, "spot_floor": floor, "spot_error": err})
+                self.spot_checks.append(err)
+                if err > TARGET:
+                    job["result"] = mc
+                    job["info"]["spot_failed"] = True
+                    self.fallbacks += 1
+                    self.failed_in_row += 1
+                    if self.failed_in_row >= self.disable_after:
+                        self.disabled = True
+                else:
+                    self.failed_in_row = 0
+            else:
+                job["result"] = mc
         job["seconds"] = time.perf_counter() - job["t0"]
         self.latencies.append(job["seconds"])
 
@@ -202,7 +239,8 @@ class ScatterService:
 
     def stats(self):
         lat = np.array(self.latencies) * 1e3
-        out = {"completed": len(self.latencies), "fallbacks": self.fallbacks, "server": self.server.stats()}
+        out = {"completed": len(self.latencies), "fallbacks": self.fallbacks, "spot_checks": list(self.spot_checks),
+               "disabled": self.disabled, "server": self.server.stats()}
         if len(lat):
             out.update({"latency_p50_ms": float(np.percentile(lat, 50)), "latency_p99_ms": float(np.percentile(lat, 99))})
         return out

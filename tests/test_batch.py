@@ -83,3 +83,22 @@ def test_batch_autotune_keeps_results():
         b.run(seed=99)
         for k in range(len(phs)):
             _same(b.fetch(k, psf=False), before[k])
+
+
+@pytest.mark.gpu
+def test_batch_with_balanced_streams_equals_each_phantoms_own_run():
+    """The balanced stream layout (stream_decays > 0) in a batch: per-phantom stream ranges
+    and random-number ids as in each phantom's own persistent run, bit for bit."""
+    cfg, phs, tab = _batch_inputs(seed=9)
+    cfg.stream_decays = 7
+    with mcpet.Batch(cfg, phs, tab) as b:
+        b.run(seed=4242, use_graph=True)
+        got = [b.fetch(k, psf=False) for k in range(len(phs))]
+        st = [b.stats(k) for k in range(len(phs))]
+    for k, ph in enumerate(phs):
+        with mcpet.Simulation(cfg, ph, tab) as sim:
+            sim.run(seed=4242, kernel=mcpet.KERNEL_PERSISTENT)
+            ref = sim.fetch(psf=False)
+            rs = sim.stats()
+        _same(got[k], ref)
+        assert st[k]["streams"] == rs["streams"] and st[k]["decays"] == rs["decays"] > 0

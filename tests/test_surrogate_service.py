@@ -87,3 +87,37 @@ def test_scatter_service_falls_back_to_monte_carlo():
         d = sim.stats()["decays"]
     assert d == info["mc_decays"] and np.array_equal(res, s.astype(np.float64) / d)
     est.close()
+
+
+@pytest.mark.gpu
+def test_spot_checks_pass_for_the_trained_network_and_disable_a_network_that_copies_its_input():
+    """Spot checks against Monte Carlo at N5: the packaged network's estimated error stays
+    under 5 %; an untrained network (the identity: the estimate is the noisy low-count input)
+    passes the consistency test but fails the spot checks, is answered by Monte Carlo, and is
+    switched off after two failures."""
+    phs = _phantoms(3, 31)
+    names = S._materials()
+    tab = mcpet.load_materials(names, np.max([p.density_max() for p in phs], axis=0))
+    good = S.ScatterEstimator()
+    svc = S.ScatterService(tab, good, budget_bytes=8 << 30, spot_check_every=1)
+    ids = [svc.submit(ph, seed=20 + k) for k, ph in enumerate(phs[:1])]
+    svc.drain()
+    res, info = svc.result(ids[0])
+    st = svc.stats()
+    svc.close()
+    assert len(st["spot_checks"]) == 1 and st["spot_checks"][0] < S.TARGET and not st["disabled"]
+    assert info["spot_error"] < S.TARGET and st["fallbacks"] == 0
+    # an untrained network is the identity (its output layer starts at zero)
+    bad = S.ScatterEstimator(weights=None)
+    assert np.all(bad.net.params()[-(S.WIDTH + 1):] == 0.0)
+    svc = S.ScatterService(tab, bad, budget_bytes=8 << 30, spot_check_every=1, disable_after=2)
+    ids = [svc.submit(ph, seed=30 + k) for k, ph in enumerate(phs)]
+    svc.drain()
+    out = [svc.result(i) for i in ids]
+    st = svc.stats()
+    svc.close()
+    assert all(o[1]["consistent"] for o in out[:2])          # the identity passes the consistency test ...
+    assert all(o[1].get("spot_failed") for o in out[:2])     # ... but not the spot checks
+    assert st["disabled"] and st["fallbacks"] >= 2
+    good.close()
+    bad.close()
