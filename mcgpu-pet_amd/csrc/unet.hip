@@ -1,14 +1,13 @@
 // unet.hip — learned scatter surrogate: a small 3D U-Net on the rebinned scatter sinogram,
-// trained on simulator output (no reference counterpart; the reference only runs Monte Carlo).
+// trained on simulator output (no reference counterpart; the reference only runs Monte Carlo),
+// and the device rebinning of sinograms onto its grid.
 //
-// Every convolution (forward, input gradient, weight gradient) is one implicit GEMM
+// Every convolution product (forward, input gradient, weight gradient) is an implicit GEMM
 //   C[m][n] (+)= sum_k A(m, k) * B(k, n)
 // on BF16 MFMA tiles (v_mfma_f32_16x16x32_bf16: 16x16 outputs, K = 32 per instruction, FP32
-// accumulation). Operands are gathered from FP32 tensors by small functors and rounded to
-// BF16 on the fly (im2col is never materialised); the weights and activations stay FP32.
-//   forward      : A = W[co][ci*T + t],        B = X[ci][p shifted by tap t]      (m = co, n = p)
-//   input grad   : A = W[co][ci*T + t]^T,      B = dY[co][q with s q + d - 1 = p] (m = ci, n = p)
-//   weight grad  : A = dY[co][q],              B = X[ci][s q + d - 1]^T           (m = co, n = ci*T+t, K = voxels, split)
+// accumulation); operands are rounded to BF16, parameters, activations and gradients are kept
+// in FP32 (im2col is never materialised). k_conv holds the three products; k_gemm is a plain
+// GEMM on the same tiles, used by the self test of the MFMA operand layout.
 // MFMA lane layout (16x16x32 BF16): lane l holds A row (l % 16), k = 8 (l / 16) .. +7; B column
 // (l % 16), the same k; the accumulator's 4 values are rows 4 (l / 16) .. +3 of column (l % 16).
 #include <hip/hip_runtime.h>
@@ -29,101 +28,17 @@ namespace {
 
 __device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }   // round to nearest even
 
-// ------------------------------------------------------------------ operand functors
+// ------------------------------------------------------------------ the plain GEMM (self test)
 
 struct Dense {      // row-major matrix, element (r, c) at p[r * ld + c]
   const float* p;
   int rows, cols, ld;
   __device__ __forceinline__ float operator()(int r, int c) const { return (r < rows && c < cols) ? p[(size_t)r * ld + c] : 0.0f; }
 };
-struct DenseT {     // transposed view: (r, c) = p[c * ld + r]
-  const float* p;
-  int rows, cols, ld;
-  __device__ __forceinline__ float operator()(int r, int c) const { return (r < rows && c < cols) ? p[(size_t)c * ld + r] : 0.0f; }
-};
-
-struct Geom {       // a 3D activation volume and a 3x3x3 (ks = 3) or 1x1x1 (ks = 1) convolution
-  int C, D, H, W;           // channels and size of the tensor indexed by the functor
-  int oD, oH, oW;           // size of the convolution output
-  int ks, stride;
-};
-
-__device__ __forceinline__ void tap_offsets(int t, int ks, int& dz, int& dy, int& dx)
-{
-  if (ks == 1) {
-    dz = dy = dx = 0;
-    return;
-  }
-  dz = t / 9 - 1;
-  dy = (t / 3) % 3 - 1;
-  dx = t % 3 - 1;
-}
-
-// forward B: (k = ci*T + t, n = output voxel) -> X[ci][s*z + dz][s*y + dy][s*x + dx]
-struct Im2col {
-  const float* x;
-  Geom g;
-  int K, N;
-  __device__ __forceinline__ float operator()(int k, int n) const
-  {
-    if (k >= K || n >= N) return 0.0f;
-    const int T = g.ks * g.ks * g.ks;
-    const int ci = k / T, t = k - ci * T;
-    int dz, dy, dx;
-    tap_offsets(t, g.ks, dz, dy, dx);
-    const int ox = n % g.oW, oy = (n / g.oW) % g.oH, oz = n / (g.oW * g.oH);
-    const int z = oz * g.stride + dz, y = oy * g.stride + dy, xx = ox * g.stride + dx;
-    if (z < 0 || y < 0 || xx < 0 || z >= g.D || y >= g.H || xx >= g.W) return 0.0f;
-    return x[(((size_t)ci * g.D + z) * g.H + y) * g.W + xx];
-  }
-};
-
-// input-gradient B: (k = co*T + t, n = input voxel p) -> dY[co][q] where s*q + d = p
-struct Col2im {
-  const float* dy;   // [Cout][oD][oH][oW]
-  Geom g;            // D, H, W: input size; oD.. output size
-  int K, N;
-  __device__ __forceinline__ float operator()(int k, int n) const
-  {
-    if (k >= K || n >= N) return 0.0f;
-    const int T = g.ks * g.ks * g.ks;
-    const int co = k / T, t = k - co * T;
-    int dz, dy_, dx;
-    tap_offsets(t, g.ks, dz, dy_, dx);
-    const int px = n % g.W, py = (n / g.W) % g.H, pz = n / (g.W * g.H);
-    const int nz = pz - dz, ny = py - dy_, nx = px - dx;
-    if (nz < 0 || ny < 0 || nx < 0) return 0.0f;
-    if (nz % g.stride || ny % g.stride || nx % g.stride) return 0.0f;
-    const int qz = nz / g.stride, qy = ny / g.stride, qx = nx / g.stride;
-    if (qz >= g.oD || qy >= g.oH || qx >= g.oW) return 0.0f;
-    return dy[(((size_t)co * g.oD + qz) * g.oH + qy) * g.oW + qx];
-  }
-};
-
-// weight-gradient B: (k = output voxel q, n = ci*T + t) -> X[ci][s*q + d]
-struct Im2colT {
-  const float* x;
-  Geom g;
-  int K, N;
-  __device__ __forceinline__ float operator()(int k, int n) const
-  {
-    if (k >= K || n >= N) return 0.0f;
-    const int T = g.ks * g.ks * g.ks;
-    const int ci = n / T, t = n - ci * T;
-    int dz, dy, dx;
-    tap_offsets(t, g.ks, dz, dy, dx);
-    const int ox = k % g.oW, oy = (k / g.oW) % g.oH, oz = k / (g.oW * g.oH);
-    const int z = oz * g.stride + dz, y = oy * g.stride + dy, xx = ox * g.stride + dx;
-    if (z < 0 || y < 0 || xx < 0 || z >= g.D || y >= g.H || xx >= g.W) return 0.0f;
-    return x[(((size_t)ci * g.D + z) * g.H + y) * g.W + xx];
-  }
-};
-
-// ------------------------------------------------------------------ the GEMM
 
 // Block: 4 waves; block tile 32 (m) x 64 (n); wave w owns columns [16 w, 16 w + 16) and both
 // 16-row halves. Grid: (ceil(N/64), ceil(M/32), split). Split > 1 accumulates partial sums
-// over K ranges with atomics (the weight gradient, whose K is the voxel count).
+// over K ranges with atomics.
 template <class AOp, class BOp>
 __global__ __launch_bounds__(256) void k_gemm(AOp A, BOp B, float* __restrict__ C, int M, int N, int K, int ldc,
                                               const float* __restrict__ bias, int k_per_split, int accumulate)
