@@ -186,12 +186,10 @@ def bench_surrogate(a):
         with mcpet.Simulation(S.config(names, S.N_LOW), ph, tab) as sim:
             for w in range(a.warmup):
                 est.estimate(sim, seed=300 + w)
-            su_t, su_e, su_z, su_mc, su_net, raw_e = [], [], [], [], [], []
+            su_t, su_e, su_z = [], [], []
             for s_ in range(a.steps):
                 e, info = est.estimate(sim, seed=400 + s_)
                 su_t.append(info["seconds"])
-                su_mc.append(info["mc_seconds"])
-                su_net.append(info["net_seconds"])
                 su_e.append(S.nrmse(e, ref))
                 su_z.append(info["z"])
         ph.close()
@@ -202,7 +200,6 @@ def bench_surrogate(a):
         t_eff = t_su + fallback * t_mc
         rows.append({"phantom": i, "reference_decays": ref_decays, "reference_floor": floor, "n5_decays": n5,
                      "mc_seconds": t_mc, "mc_nrmse": float(np.median(mc_e)), "surrogate_seconds": t_su,
-                     "surrogate_mc_seconds": float(np.median(su_mc)), "surrogate_net_seconds": float(np.median(su_net)),
                      "surrogate_nrmse": float(np.median(su_e)), "surrogate_nrmse_max": float(np.max(su_e)),
                      "consistency_z": float(np.median(su_z)), "fallback_fraction": fallback,
                      "effective_seconds": t_eff, "speedup": t_mc / t_eff})

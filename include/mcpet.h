@@ -432,6 +432,14 @@ mcpet_status mcpet_surrogate_get_params(mcpet_surrogate* s, float* out);
 mcpet_status mcpet_surrogate_set_params(mcpet_surrogate* s, const float* in);
 /* x: batch x in_channels x D x H x W (host); y: batch x D x H x W (host). gflop: forward GFLOP per sample. */
 mcpet_status mcpet_surrogate_forward(mcpet_surrogate* s, const float* x, int32_t batch, float* y, double* gflop);
+/* One estimate from the last run of ctx (a 2-input-channel network; same device), all on the
+ * device: its scatter and trues rebinned onto the network's grid (mcpet_rebin_sinogram),
+ * normalised (counts times float(voxels / total trues), the total summed exactly in double),
+ * the network, and estimate = max(output, 0) times float(total trues / (voxels * decays)):
+ * scatter per decay (D*H*W floats, host). scatter_counts (may be NULL): the rebinned scatter;
+ * decays (may be NULL): the run's decays. */
+mcpet_status mcpet_surrogate_estimate(mcpet_surrogate* s, mcpet_ctx* ctx, float* estimate, float* scatter_counts,
+                                      int64_t* decays);
 /* loss = mean over batch and voxels of (y - target)^2; grad (host, param_count floats; may be NULL). */
 mcpet_status mcpet_surrogate_gradients(mcpet_surrogate* s, const float* x, const float* target, int32_t batch, float* loss,
                                        float* grad);

@@ -804,45 +804,80 @@ extern "C" mcpet_status mcpet_get_detector(const mcpet_ctx* c, mcpet_detector* o
 
 // Single-slice rebinning of a sinogram tally onto a D x H x W grid (plane groups of
 // ceil(NZS / D) ring sums, angle groups of ceil(NANGLES / H), radial groups of ceil(NRAD / W)).
-static mcpet_status rebin(mcpet_ctx* c, int which, int D, int H, int W, float* out)
+// rebin_setup builds (once per grid) the plane-group lists and the context's output buffer.
+static mcpet_status rebin_setup(mcpet_ctx* c, int D, int H, int W)
 {
-  if (!c || !out || D < 1 || H < 1 || W < 1 || W > 256) return set_err(c, MCPET_E_ARG, "rebin: bad arguments (W <= 256)");
+  if (!c || D < 1 || H < 1 || W < 1 || W > 256) return set_err(c, MCPET_E_ARG, "rebin: bad arguments (W <= 256)");
   if (!c->ran) return set_err(c, MCPET_E_STATE, "rebin: no run");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));   // the run (and any earlier rebinning) is complete
+  if (c->rb_dims[0] == D && c->rb_dims[1] == H && c->rb_dims[2] == W) return MCPET_OK;
+  const mcpet_config& cfg = c->cfg;
+  const int gz = (cfg.nzs + D - 1) / D;
+  std::vector<int> plane(cfg.nsinos);
+  if (mcpet_sinogram_planes(&cfg, plane.data()) != MCPET_OK) return set_err(c, MCPET_E_ARG, mcpet_host_last_error());
+  std::vector<int> off(D + 1, 0), list;
+  for (int d = 0; d < D; d++) {
+    for (int izm = 0; izm < cfg.nsinos; izm++)
+      if (plane[izm] / gz == d) list.push_back(izm);
+    off[d + 1] = (int)list.size();
+  }
+  for (void* p : {(void*)c->d_rb_off, (void*)c->d_rb_sino, (void*)c->d_rb_out})
+    if (p) (void)hipFree(p);
+  c->d_rb_off = c->d_rb_sino = nullptr;
+  c->d_rb_out = nullptr;
+  c->rb_dims[0] = c->rb_dims[1] = c->rb_dims[2] = 0;
+  mcpet_status e = dalloc(c, &c->d_rb_off, (size_t)D + 1);
+  if (e == MCPET_OK) e = dalloc(c, &c->d_rb_sino, std::max<size_t>(1, list.size()));
+  if (e == MCPET_OK) e = dalloc(c, &c->d_rb_out, (size_t)D * H * W);
+  if (e != MCPET_OK) return e;
+  HIPCHK(c, hipMemcpy(c->d_rb_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
+  if (!list.empty()) HIPCHK(c, hipMemcpy(c->d_rb_sino, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice));
+  c->rb_dims[0] = D;
+  c->rb_dims[1] = H;
+  c->rb_dims[2] = W;
+  return MCPET_OK;
+}
+
+// Enqueue the rebinning of tally `which` into the device buffer d_out on stream st (after
+// rebin_setup for this grid).
+static mcpet_status rebin_launch(mcpet_ctx* c, int which, int D, int H, int W, float* d_out, hipStream_t st)
+{
   const int* src = which == 0 ? c->d_trues : which == 1 ? c->d_scatter : which == 2 ? c->d_randoms : which == 3 ? c->d_spurious : nullptr;
   if (which < 0 || which > 3) return set_err(c, MCPET_E_ARG, "rebin: which must be 0 (trues), 1 (scatter), 2 (randoms), 3 (spurious)");
   if (!src) return set_err(c, MCPET_E_STATE, "rebin: this tally is not kept in this mode");
   const mcpet_config& cfg = c->cfg;
-  const int gz = (cfg.nzs + D - 1) / D, gh = (cfg.nangles + H - 1) / H, gw = (cfg.nrad + W - 1) / W;
-  HIPCHK(c, hipSetDevice(c->device));
-  if (c->rb_dims[0] != D || c->rb_dims[1] != H || c->rb_dims[2] != W) {
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    std::vector<int> plane(cfg.nsinos);
-    if (mcpet_sinogram_planes(&cfg, plane.data()) != MCPET_OK) return set_err(c, MCPET_E_ARG, mcpet_host_last_error());
-    std::vector<int> off(D + 1, 0), list;
-    for (int d = 0; d < D; d++) {
-      for (int izm = 0; izm < cfg.nsinos; izm++)
-        if (plane[izm] / gz == d) list.push_back(izm);
-      off[d + 1] = (int)list.size();
-    }
-    for (void* p : {(void*)c->d_rb_off, (void*)c->d_rb_sino, (void*)c->d_rb_out})
-      if (p) (void)hipFree(p);
-    c->d_rb_off = c->d_rb_sino = nullptr;
-    c->d_rb_out = nullptr;
-    c->rb_dims[0] = c->rb_dims[1] = c->rb_dims[2] = 0;
-    mcpet_status st = dalloc(c, &c->d_rb_off, (size_t)D + 1);
-    if (st == MCPET_OK) st = dalloc(c, &c->d_rb_sino, std::max<size_t>(1, list.size()));
-    if (st == MCPET_OK) st = dalloc(c, &c->d_rb_out, (size_t)D * H * W);
-    if (st != MCPET_OK) return st;
-    HIPCHK(c, hipMemcpy(c->d_rb_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
-    if (!list.empty()) HIPCHK(c, hipMemcpy(c->d_rb_sino, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice));
-    c->rb_dims[0] = D;
-    c->rb_dims[1] = H;
-    c->rb_dims[2] = W;
-  }
-  hipError_t e = mcpet_launch_rebin(src, c->d_rb_off, c->d_rb_sino, cfg.nangles, cfg.nrad, D, H, W, gh, gw, c->d_rb_out, c->stream);
+  const int gh = (cfg.nangles + H - 1) / H, gw = (cfg.nrad + W - 1) / W;
+  hipError_t e = mcpet_launch_rebin(src, c->d_rb_off, c->d_rb_sino, cfg.nangles, cfg.nrad, D, H, W, gh, gw, d_out, st);
   if (e != hipSuccess) return set_err(c, MCPET_E_HIP, std::string("rebin launch failed: ") + hipGetErrorString(e));
+  return MCPET_OK;
+}
+
+static mcpet_status rebin(mcpet_ctx* c, int which, int D, int H, int W, float* out)
+{
+  if (!out) return set_err(c, MCPET_E_ARG, "rebin: null output");
+  mcpet_status st = rebin_setup(c, D, H, W);
+  if (st == MCPET_OK) st = rebin_launch(c, which, D, H, W, c->d_rb_out, c->stream);
+  if (st != MCPET_OK) return st;
   HIPCHK(c, hipMemcpyAsync(out, c->d_rb_out, sizeof(float) * (size_t)D * H * W, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MCPET_OK;
+}
+
+mcpet_status mcpet_ctx_rebin_device(mcpet_ctx* c, int which, int D, int H, int W, float* d_out, hipStream_t st)
+{
+  if (!d_out) return set_err(c, MCPET_E_ARG, "rebin: null output");
+  mcpet_status e = rebin_setup(c, D, H, W);
+  if (e == MCPET_OK) e = rebin_launch(c, which, D, H, W, d_out, st);
+  return e;
+}
+
+mcpet_status mcpet_ctx_decays(mcpet_ctx* c, long long* decays)
+{
+  unsigned long long n = 0;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpy(&n, c->d_counters + KC_DECAYS, sizeof n, hipMemcpyDeviceToHost));
+  *decays = (long long)n;
   return MCPET_OK;
 }
 

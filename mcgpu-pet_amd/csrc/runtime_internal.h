@@ -26,4 +26,10 @@ int mcpet_ctx_num_cus(const mcpet_ctx* c);
 mcpet_status mcpet_run_contexts(const CtxRun* runs, int n, hipStream_t stream, KParams* h_params, KParams* d_params,
                                 int block_size, int blocks, std::string* err);
 
+// Rebinning of tally `which` of the context's last run into the device buffer d_out (D*H*W
+// floats), enqueued on `st` after the run has completed (mcpet_rebin_sinogram's grid).
+mcpet_status mcpet_ctx_rebin_device(mcpet_ctx* c, int which, int D, int H, int W, float* d_out, hipStream_t st);
+// Decays of the context's last run (its event counter; waits for nothing: call after the run).
+mcpet_status mcpet_ctx_decays(mcpet_ctx* c, long long* decays);
+
 #endif

@@ -20,6 +20,7 @@
 
 #include "host_internal.h"
 #include "mcpet.h"
+#include "runtime_internal.h"
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -446,6 +447,55 @@ __global__ void k_zero(float* __restrict__ p, size_t n)
   if (i < n) p[i] = 0.0f;
 }
 
+// Total of a float array in double, in a fixed order: 256 workgroups sum contiguous chunks,
+// then one workgroup sums their results. Integer-valued inputs (counts) give the exact total.
+__global__ void k_sum_partial(const float* __restrict__ x, size_t n, double* __restrict__ partial)
+{
+  __shared__ double sh[256];
+  const size_t chunk = (n + gridDim.x - 1) / gridDim.x, b = blockIdx.x * chunk, e = min(n, b + chunk);
+  double s = 0.0;
+  for (size_t i = b + threadIdx.x; i < e; i += blockDim.x) s += (double)x[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+__global__ void k_sum_final(const double* __restrict__ partial, int m, double* __restrict__ out)
+{
+  __shared__ double sh[256];
+  sh[threadIdx.x] = (int)threadIdx.x < m ? partial[threadIdx.x] : 0.0;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0];
+}
+// network input: scatter and trues counts times float(voxels / total trues)
+__global__ void k_normalize(const float* __restrict__ s, const float* __restrict__ t, float* __restrict__ x0, size_t n,
+                            const double* __restrict__ total)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double tot = total[0];
+  const float scale = tot > 0.0 ? (float)((double)n / tot) : 0.0f;
+  x0[i] = s[i] * scale;
+  x0[n + i] = t[i] * scale;
+}
+// scatter per decay: max(output, 0) times float(total trues / (voxels * decays))
+__global__ void k_estimate(const float* __restrict__ out, float* __restrict__ est, size_t n, const double* __restrict__ total,
+                           double decays)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double tot = total[0];
+  const float f = (tot > 0.0 && decays > 0.0) ? (float)(tot / ((double)n * decays)) : 0.0f;
+  est[i] = fmaxf(out[i], 0.0f) * f;
+}
+
 inline unsigned blocks_for(size_t n) { return (unsigned)((n + 255) / 256); }
 
 // ------------------------------------------------------------------ the network
@@ -487,7 +537,10 @@ struct mcpet_surrogate {
          *d2l = nullptr, *e1l = nullptr;
   __bf16 *grl = nullptr, *ge1l = nullptr, *ge2l = nullptr, *gb3l = nullptr, *gd2l = nullptr, *gb2l = nullptr, *gd1l = nullptr,
          *gb1l = nullptr;
+  float *raw_s = nullptr, *raw_t = nullptr, *est = nullptr;   // the fused estimate's buffers
+  double* sums = nullptr;                                       // 256 partial totals + the total
   long long t_adam = 0;
+  int device = 0;
   hipStream_t st = nullptr;
   std::vector<void*> allocs;
   std::string err;
@@ -743,7 +796,8 @@ extern "C" mcpet_status mcpet_surrogate_create(int32_t D, int32_t H, int32_t W, 
   s->ve2 = {2 * c, D / 2, H / 2, W / 2};
   s->ve1 = {c, D, H, W};
   s->vr = {1, D, H, W};
-  hipError_t e = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking);
+  hipError_t e = hipGetDevice(&s->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking);
   struct A {
     float** p;
     size_t n;
@@ -753,10 +807,12 @@ extern "C" mcpet_status mcpet_surrogate_create(int32_t D, int32_t H, int32_t W, 
             {&s->e1, s->ve1.size()}, {&s->r, s->vr.size()}, {&s->out, s->vr.size()}, {&s->tgt, s->vr.size()},
             {&s->ga1, s->va1.size()}, {&s->gcat1, s->vcat1.size()}, {&s->gd1, s->vd1.size()}, {&s->gcat2, s->vcat2.size()},
             {&s->gd2, s->vd2.size()}, {&s->gb3, s->vb3.size()}, {&s->ge2, s->ve2.size()}, {&s->ge1, s->ve1.size()},
-            {&s->gr, s->vr.size()}, {&s->sse, 1}};
+            {&s->gr, s->vr.size()}, {&s->sse, 1}, {&s->raw_s, s->vr.size()}, {&s->raw_t, s->vr.size()},
+            {&s->est, s->vr.size()}};
   for (auto& a : al)
     if (e == hipSuccess) e = dalloc_t(s, a.p, a.n);
   if (e == hipSuccess) e = dalloc_t(s, &s->wbf, s->n_bf16);
+  if (e == hipSuccess) e = dalloc_t(s, &s->sums, 257);
   {
     const size_t v0 = s->vx.vox(), v1 = s->vd1.vox(), v2 = s->vd2.vox();
     struct B {
@@ -840,6 +896,39 @@ extern "C" mcpet_status mcpet_surrogate_forward(mcpet_surrogate* s, const float*
   }
   SHIP(s, hipStreamSynchronize(s->st));
   if (gflop) *gflop = s->flops_fwd * 1e-9;
+  return MCPET_OK;
+}
+
+// One estimate from the last run of ctx, all on the device: rebinning of its scatter and
+// trues onto the network's grid, normalisation (counts times float(voxels / total trues), the
+// total exact in double), the network, and scatter per decay = max(output, 0) times
+// float(total trues / (voxels * decays)).
+extern "C" mcpet_status mcpet_surrogate_estimate(mcpet_surrogate* s, mcpet_ctx* ctx, float* estimate, float* scatter_counts,
+                                                 int64_t* decays)
+{
+  if (!s || !ctx || !estimate) return serr(s, MCPET_E_ARG, "mcpet_surrogate_estimate: null argument");
+  if (s->cin != 2) return serr(s, MCPET_E_ARG, "mcpet_surrogate_estimate: the network must take 2 input channels");
+  if (mcpet_ctx_device(ctx) != s->device) return serr(s, MCPET_E_ARG, "mcpet_surrogate_estimate: context on another device");
+  SHIP(s, hipSetDevice(s->device));
+  const size_t nv = s->vr.size();
+  mcpet_status st = mcpet_ctx_rebin_device(ctx, 1, s->D, s->H, s->W, s->raw_s, s->st);
+  if (st == MCPET_OK) st = mcpet_ctx_rebin_device(ctx, 0, s->D, s->H, s->W, s->raw_t, s->st);
+  long long n = 0;
+  if (st == MCPET_OK) st = mcpet_ctx_decays(ctx, &n);
+  if (st != MCPET_OK) return serr(s, st, std::string("mcpet_surrogate_estimate: ") + mcpet_last_error(ctx));
+  hipLaunchKernelGGL(k_sum_partial, dim3(256), dim3(256), 0, s->st, s->raw_t, nv, s->sums);
+  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, s->st, s->sums, 256, s->sums + 256);
+  hipLaunchKernelGGL(k_normalize, dim3(blocks_for(nv)), dim3(256), 0, s->st, s->raw_s, s->raw_t, s->x0, nv, s->sums + 256);
+  SHIP(s, hipGetLastError());
+  SHIP(s, forward(s));
+  hipLaunchKernelGGL(k_output, dim3(blocks_for(nv)), dim3(256), 0, s->st, s->x0, s->r, s->out, (const float*)nullptr,
+                     (float*)nullptr, 0.0f, (float*)nullptr, (int)nv, kNoCl);
+  hipLaunchKernelGGL(k_estimate, dim3(blocks_for(nv)), dim3(256), 0, s->st, s->out, s->est, nv, s->sums + 256, (double)n);
+  SHIP(s, hipGetLastError());
+  SHIP(s, hipMemcpyAsync(estimate, s->est, sizeof(float) * nv, hipMemcpyDeviceToHost, s->st));
+  if (scatter_counts) SHIP(s, hipMemcpyAsync(scatter_counts, s->raw_s, sizeof(float) * nv, hipMemcpyDeviceToHost, s->st));
+  SHIP(s, hipStreamSynchronize(s->st));
+  if (decays) *decays = n;
   return MCPET_OK;
 }
 

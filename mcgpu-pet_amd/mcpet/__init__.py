@@ -221,6 +221,7 @@ def lib():
                                             P(C.c_float)]
     L.mcpet_surrogate_train_step.argtypes = [C.c_void_p, P(C.c_float), P(C.c_float), C.c_int32, C.c_float,
                                              P(C.c_float)]
+    L.mcpet_surrogate_estimate.argtypes = [C.c_void_p, C.c_void_p, P(C.c_float), P(C.c_float), P(C.c_int64)]
     L.mcpet_surrogate_destroy.argtypes = [C.c_void_p]
     L.mcpet_surrogate_destroy.restype = None
     L.mcpet_surrogate_last_error.argtypes = [C.c_void_p]
@@ -788,6 +789,15 @@ class Surrogate:
         self._check(lib().mcpet_surrogate_forward(self.s, _fptr(x), x.shape[0], _fptr(y), C.byref(g)))
         self.gflop = g.value
         return y
+
+    def estimate(self, sim):
+        """Fused device estimate from the last run of Simulation `sim` (mcpet_surrogate_estimate):
+        (scatter per decay, rebinned scatter counts, decays)."""
+        est = np.zeros(self.grid, np.float32)
+        cnt = np.zeros(self.grid, np.float32)
+        d = C.c_int64()
+        self._check(lib().mcpet_surrogate_estimate(self.s, sim.ctx, _fptr(est), _fptr(cnt), C.byref(d)))
+        return est, cnt, d.value
 
     def _target(self, x, t):
         t = np.ascontiguousarray(t, np.float32).reshape((x.shape[0],) + self.grid)

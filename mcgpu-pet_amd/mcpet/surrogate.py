@@ -69,12 +69,17 @@ def decays_for(ref_per_decay, target=TARGET):
 
 
 def normalize(scatter_low, trues_low, n_low):
-    """Network input (2 channels: scatter and trues per decay, scaled so that the trues
-    channel has mean 1) and the scale k."""
-    xs = np.asarray(scatter_low, np.float64) / n_low
-    xt = np.asarray(trues_low, np.float64) / n_low
-    k = 1.0 / max(xt.mean(), 1e-30)
-    return np.stack([xs * k, xt * k]).astype(np.float32), k
+    """Network input (2 channels: scatter and trues scaled so that the trues channel has mean 1)
+    and the factor that turns the network's output into scatter per decay. The arithmetic is
+    the device path's (mcpet_surrogate_estimate): counts times float32(voxels / total trues), the
+    total exact in float64; factor float32(total trues / (voxels * decays))."""
+    s = np.asarray(scatter_low, np.float32)
+    t = np.asarray(trues_low, np.float32)
+    tot = float(np.sum(t, dtype=np.float64))
+    nvox = t.size
+    scale = np.float32(nvox / tot) if tot > 0 else np.float32(0.0)
+    factor = np.float32(tot / (nvox * float(n_low))) if tot > 0 and n_low > 0 else np.float32(0.0)
+    return np.stack([s * scale, t * scale]), factor
 
 
 def consistency_z(pred_per_decay, scatter_low, n_low, block=(4, 4, 4), min_mean=20.0):
@@ -110,25 +115,23 @@ class ScatterEstimator:
         self.z_max = z_max
 
     def predict(self, scatter_low, trues_low, n_low):
-        x, k = normalize(scatter_low, trues_low, n_low)
+        """Scatter per decay (float32) from rebinned low-count scatter and trues of n_low decays."""
+        x, factor = normalize(scatter_low, trues_low, n_low)
         y = self.net.forward(x)[0]
-        return np.maximum(y.astype(np.float64), 0.0) / k
+        return np.maximum(y, np.float32(0.0)) * factor
 
     def estimate(self, sim, seed=0):
         """Scatter per decay on the grid from a run of `sim` (a Simulation of the phantom with
-        config(materials, N_LOW)). Returns (estimate, info); info["consistent"] False means: use
-        Monte Carlo."""
+        config(materials, N_LOW)): the run, then the fused device estimate (rebinning,
+        normalisation, network). Returns (estimate, info); info["consistent"] False means: use
+        Monte Carlo. Equal to predict() on the rebinned tallies, bit for bit."""
         t0 = time.perf_counter()
         sim.run(seed=seed)
-        s = sim.rebin("scatter", self.grid)
-        t = sim.rebin("trues", self.grid)
-        decays = sim.stats()["decays"]
+        est, s, decays = self.net.estimate(sim)
         t1 = time.perf_counter()
-        est = self.predict(s, t, decays)
-        t2 = time.perf_counter()
         z = consistency_z(est, s, decays)
-        return est, {"decays": decays, "z": z, "consistent": bool(z <= self.z_max), "mc_seconds": t1 - t0,
-                     "net_seconds": t2 - t1, "seconds": t2 - t0, "scatter_counts": float(s.sum())}
+        return est, {"decays": decays, "z": z, "consistent": bool(z <= self.z_max), "seconds": t1 - t0,
+                     "scatter_counts": float(s.sum())}
 
     def close(self):
         self.net.close()
@@ -315,9 +318,9 @@ def generate(n_phantoms, seed, n_target=5.0e8, n_low=N_LOW, inputs_per_phantom=4
                 b.sync()
                 for k in range(len(phs)):
                     d = b.stats(k)["decays"]
-                    x, kk = normalize(b.rebin(k, "scatter", GRID), b.rebin(k, "trues", GRID), d)
+                    x, factor = normalize(b.rebin(k, "scatter", GRID), b.rebin(k, "trues", GRID), d)
                     X.append(x)
-                    Y.append((targets[k] * kk).astype(np.float32))
+                    Y.append((targets[k] / np.float64(factor)).astype(np.float32))
                     idx.append(c0 + k)
         for p in phs:
             p.close()

@@ -56,10 +56,12 @@ def test_normalize_is_invariant_to_the_number_of_decays():
     rng = np.random.default_rng(4)
     s = rng.poisson(3.0, S.GRID).astype(float)
     t = rng.poisson(9.0, S.GRID).astype(float)
-    x1, k1 = S.normalize(s, t, 5e6)
-    x2, k2 = S.normalize(2 * s, 2 * t, 1e7)
-    assert np.allclose(x1, x2) and np.isclose(k1, k2)
+    x1, f1 = S.normalize(s, t, 5e6)
+    x2, f2 = S.normalize(2 * s, 2 * t, 1e7)
+    assert np.array_equal(x1, x2) and f1 == f2           # counts x 2^k scale exactly
     assert abs(x1[1].mean() - 1) < 1e-6 and x1.dtype == np.float32 and x1.shape == (2,) + S.GRID
+    # the output factor turns normalised units back into counts per decay
+    assert abs(float(f1) * float(x1[1].sum()) * 5e6 / t.sum() - 1) < 1e-6
 
 
 def test_workload_config():

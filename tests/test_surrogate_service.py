@@ -32,6 +32,26 @@ def test_server_rebinned_request_equals_simulation_rebinned():
     assert full["trues"].sum() > 0
 
 
+@pytest.mark.gpu
+def test_fused_device_estimate_equals_the_host_path():
+    """mcpet_surrogate_estimate (rebinning, normalisation, network and scaling on the device)
+    equals predict() on the rebinned tallies bit for bit, with the same counts and decays."""
+    phs = _phantoms(1, 77)
+    names = S._materials()
+    tab = mcpet.load_materials(names, phs[0].density_max())
+    est = S.ScatterEstimator()
+    with mcpet.Simulation(S.config(names, S.N_LOW), phs[0], tab) as sim:
+        sim.run(seed=3)
+        e, cnt, d = est.net.estimate(sim)
+        s = sim.rebin("scatter", S.GRID)
+        t = sim.rebin("trues", S.GRID)
+        d0 = sim.stats()["decays"]
+    est2 = est.predict(s, t, d0)
+    assert d == d0 and np.array_equal(cnt, s)
+    assert e.dtype == np.float32 and np.array_equal(e, est2) and e.sum() > 0
+    est.close()
+
+
 def _phantoms(n, seed):
     return [mcpet.phantom_from_arrays(*sp) for sp in
             phantoms.scatter_phantoms(n, seed, n_decays=S.DECAYS_PER_SECOND, acquisition_time_s=1.0)]
