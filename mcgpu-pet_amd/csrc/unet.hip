@@ -1028,13 +1028,42 @@ __global__ __launch_bounds__(256) void k_rebin(const int* __restrict__ sino, con
   for (int i = threadIdx.x; i < W; i += blockDim.x) out[((size_t)d * H + h) * W + i] = (float)acc[i];
 }
 
+// The same sums for NRAD <= 256 without atomics: thread r owns radial bin r and adds it up
+// over the group's sinogram rows (each row read whole, coalesced), then the radial groups are
+// summed from LDS in order.
+__global__ __launch_bounds__(256) void k_rebin_rows(const int* __restrict__ sino, const int* __restrict__ grp_off,
+                                                    const int* __restrict__ grp_sino, int nangles, int nrad, int H, int W,
+                                                    int gh, int gw, float* __restrict__ out)
+{
+  __shared__ unsigned long long bin[256];
+  const int h = blockIdx.x, d = blockIdx.y, r = threadIdx.x;
+  const int a0 = h * gh, rows = max(0, min(gh, nangles - a0));
+  const int j0 = grp_off[d], nj = grp_off[d + 1] - j0;
+  unsigned long long acc = 0ull;
+  if (r < nrad)
+    for (int j = 0; j < nj; j++) {
+      const int* row = sino + ((size_t)grp_sino[j0 + j] * nangles + a0) * nrad + r;
+      for (int a = 0; a < rows; a++) acc += (unsigned)row[(size_t)a * nrad];
+    }
+  bin[r] = acc;
+  __syncthreads();
+  if (r < W) {
+    unsigned long long t = 0ull;
+    for (int i = r * gw; i < min(nrad, r * gw + gw); i++) t += bin[i];
+    out[((size_t)d * H + h) * W + r] = (float)t;
+  }
+}
+
 }  // namespace
 
 hipError_t mcpet_launch_rebin(const int* sino, const int* grp_off, const int* grp_sino, int nangles, int nrad, int D, int H, int W,
                               int gh, int gw, float* out, hipStream_t st)
 {
   if (W > 256) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_rebin, dim3(H, D), dim3(256), 0, st, sino, grp_off, grp_sino, nangles, nrad, H, W, gh, gw, out);
+  if (nrad <= 256)
+    hipLaunchKernelGGL(k_rebin_rows, dim3(H, D), dim3(256), 0, st, sino, grp_off, grp_sino, nangles, nrad, H, W, gh, gw, out);
+  else
+    hipLaunchKernelGGL(k_rebin, dim3(H, D), dim3(256), 0, st, sino, grp_off, grp_sino, nangles, nrad, H, W, gh, gw, out);
   return hipGetLastError();
 }
 

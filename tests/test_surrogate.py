@@ -267,3 +267,25 @@ def test_device_rebinning_equals_host_sum_of_the_fetched_sinograms():
                 ref = red.reshape(D, H, gh, W, gw).sum(axis=(2, 4))
                 assert np.array_equal(dev.astype(np.int64), ref), (which, grid)
                 assert dev.sum() == sino.sum() > 0
+
+
+@pytest.mark.gpu
+def test_device_rebinning_with_more_than_256_radial_bins():
+    """NRAD > 256 takes the LDS-atomic rebinning kernel (the row kernel holds one radial bin
+    per thread): equal to the host sum as well."""
+    from harness import water_cube_scenario
+    sc = water_cube_scenario(n_decays=1e5, tpb=16, nrad=300)
+    cfg = sc.cfg
+    planes = mcpet.sinogram_planes(cfg)
+    with mcpet.Simulation(cfg, sc.phantom, sc.tables) as sim:
+        sim.run(seed=5)
+        out = sim.fetch(images=False, psf=False, dose=False)
+        grid = (16, 12, 30)
+        D, H, W = grid
+        gz, gh, gw = -(-cfg.nzs // D), -(-cfg.nangles // H), -(-cfg.nrad // W)
+        dev = sim.rebin("trues", grid)
+    sino = out["trues"].reshape(cfg.nsinos, cfg.nangles, cfg.nrad).astype(np.int64)
+    red = np.zeros((D, cfg.nangles, cfg.nrad), np.int64)
+    np.add.at(red, planes // gz, sino)
+    red = np.pad(red, ((0, 0), (0, gh * H - cfg.nangles), (0, gw * W - cfg.nrad)))
+    assert np.array_equal(dev.astype(np.int64), red.reshape(D, H, gh, W, gw).sum(axis=(2, 4))) and dev.sum() > 0
