@@ -184,12 +184,19 @@ def bench_surrogate(a):
                 mc_e.append(S.nrmse(e, ref))
         # surrogate
         with mcpet.Simulation(S.config(names, S.N_LOW), ph, tab) as sim:
+            # the phantom's maps are computed once per context (then kept): their first
+            # computation is charged to every timed estimate
+            t_proj = 0.0
+            if est.in_channels == 4:
+                t0 = time.perf_counter()
+                sim.project(S.GRID, fetch=False)
+                t_proj = time.perf_counter() - t0
             for w in range(a.warmup):
                 est.estimate(sim, seed=300 + w)
             su_t, su_e, su_z = [], [], []
             for s_ in range(a.steps):
                 e, info = est.estimate(sim, seed=400 + s_)
-                su_t.append(info["seconds"])
+                su_t.append(info["seconds"] + t_proj)
                 su_e.append(S.nrmse(e, ref))
                 su_z.append(info["z"])
         ph.close()
@@ -201,12 +208,12 @@ def bench_surrogate(a):
         rows.append({"phantom": i, "reference_decays": ref_decays, "reference_floor": floor, "n5_decays": n5,
                      "mc_seconds": t_mc, "mc_nrmse": float(np.median(mc_e)), "surrogate_seconds": t_su,
                      "surrogate_nrmse": float(np.median(su_e)), "surrogate_nrmse_max": float(np.max(su_e)),
-                     "consistency_z": float(np.median(su_z)), "fallback_fraction": fallback,
+                     "consistency_z": float(np.median(su_z)), "fallback_fraction": fallback, "projection_seconds": t_proj,
                      "effective_seconds": t_eff, "speedup": t_mc / t_eff})
         log(f"surrogate bench: phantom {i}: MC {t_mc * 1e3:.1f} ms (nrmse {np.median(mc_e):.4f}), surrogate "
             f"{t_su * 1e3:.2f} ms (nrmse {np.median(su_e):.4f}, z {np.median(su_z):.2f}), speedup {t_mc / t_eff:.1f}")
     # U-Net forward throughput (host-timed forward of one sample)
-    x = np.random.default_rng(0).random((1, S.IN_CHANNELS) + S.GRID).astype(np.float32)
+    x = np.random.default_rng(0).random((1, est.in_channels) + S.GRID).astype(np.float32)
     est.net.forward(x)
     t0 = time.perf_counter()
     for _ in range(20):
@@ -222,7 +229,8 @@ def bench_surrogate(a):
            "dtype": "bf16 MFMA (fp32 accumulate) + f32 transport", "data": "synthetic",
            "config": {"workload": f"{len(rows)} evaluation phantoms ({EVAL_FAMILIES[a.eval_family]}, seed "
                                   f"{a.eval_seed}), scatter on the 32x56x56 rebinned grid, surrogate input "
-                                  f"{S.N_LOW:.0e} decays, U-Net width {S.WIDTH}", "parallelism": "single GPU"},
+                                  f"{S.N_LOW:.0e} decays + activity/attenuation maps, U-Net width {S.WIDTH}",
+                      "parallelism": "single GPU"},
            "mc_seconds_mean": tot_mc / len(rows), "surrogate_seconds_mean": tot_eff / len(rows),
            "surrogate_nrmse_max": max(r["surrogate_nrmse_max"] for r in rows),
            "fallback_fraction": float(np.mean([r["fallback_fraction"] for r in rows])),

@@ -390,11 +390,14 @@ mcpet_status mcpet_server_poll(mcpet_server* s, int64_t request_id, int32_t* sta
 mcpet_status mcpet_server_fetch(mcpet_server* s, int64_t request_id, mcpet_tallies* out);
 /* A request whose result is only its trues and scatter rebinned on the device onto a D x H x W
  * grid (mcpet_rebin_sinogram) and its number of decays: no full tallies are copied to the host
- * (the scatter surrogate's inputs and its Monte Carlo fallbacks). */
+ * (the scatter surrogate's inputs and its Monte Carlo fallbacks). with_maps: also the phantom's
+ * line integrals on that grid (mcpet_project_maps; D <= 256). */
 mcpet_status mcpet_server_submit_rebinned(mcpet_server* s, const mcpet_config* cfg, const mcpet_phantom* ph, int32_t seed,
-                                          int32_t D, int32_t H, int32_t W, int64_t* request_id);
-/* trues, scatter: D*H*W floats each (may be NULL); decays (may be NULL). The request is then forgotten. */
-mcpet_status mcpet_server_fetch_rebinned(mcpet_server* s, int64_t request_id, float* trues, float* scatter, int64_t* decays);
+                                          int32_t D, int32_t H, int32_t W, int32_t with_maps, int64_t* request_id);
+/* trues, scatter, act, att: D*H*W floats each (may be NULL; act and att need with_maps);
+ * decays (may be NULL). The request is then forgotten. */
+mcpet_status mcpet_server_fetch_rebinned(mcpet_server* s, int64_t request_id, float* trues, float* scatter, float* act,
+                                         float* att, int64_t* decays);
 mcpet_status mcpet_server_get_stats(mcpet_server* s, mcpet_server_stats* out);
 void mcpet_server_destroy(mcpet_server* s);
 const char* mcpet_server_last_error(const mcpet_server* s);
@@ -415,6 +418,17 @@ mcpet_status mcpet_sinogram_planes(const mcpet_config* cfg, int32_t* plane_of_si
 mcpet_status mcpet_rebin_sinogram(mcpet_ctx* c, int32_t which, int32_t D, int32_t H, int32_t W, float* out);
 mcpet_status mcpet_batch_rebin_sinogram(mcpet_batch* b, int32_t k, int32_t which, int32_t D, int32_t H, int32_t W, float* out);
 
+/* The phantom seen along the lines of response of the D x H x W grid (1 <= D <= 256): for every
+ * grid bin, the mean over the crystal pairs of its angle and radial groups (the pairs of the
+ * coincidence binning, K:1776-1786) of the line between the two crystal centres (angle
+ * 2 pi (k + 1/2) / NCRYSTALS - pi on the detector radius) in the axial plane of the bin's
+ * central ring sum, sampled every half transaxial voxel inside the phantom:
+ *   act: integral of the activity (Bq cm); att: exp(-integral of mu at 511 keV) (1 outside).
+ * Computed on the device at the first call for a grid and kept by the context (act and att may
+ * be NULL: compute and keep only); no run needed. */
+mcpet_status mcpet_project_maps(mcpet_ctx* c, int32_t D, int32_t H, int32_t W, float* act, float* att);
+mcpet_status mcpet_batch_project_maps(mcpet_batch* b, int32_t k, int32_t D, int32_t H, int32_t W, float* act, float* att);
+
 /* A 3D U-Net on the rebinned grid (D, H, W multiples of 4): in_channels -> width (two 3^3
  * convolutions) -> stride-2 convolution to 2 width -> convolution -> stride-2 to 4 width ->
  * convolution -> nearest upsampling, concatenation with the 2-width skip -> convolution to
@@ -432,12 +446,13 @@ mcpet_status mcpet_surrogate_get_params(mcpet_surrogate* s, float* out);
 mcpet_status mcpet_surrogate_set_params(mcpet_surrogate* s, const float* in);
 /* x: batch x in_channels x D x H x W (host); y: batch x D x H x W (host). gflop: forward GFLOP per sample. */
 mcpet_status mcpet_surrogate_forward(mcpet_surrogate* s, const float* x, int32_t batch, float* y, double* gflop);
-/* One estimate from the last run of ctx (a 2-input-channel network; same device), all on the
- * device: its scatter and trues rebinned onto the network's grid (mcpet_rebin_sinogram),
+/* One estimate from the last run of ctx (a 2- or 4-input-channel network; same device), all on
+ * the device: its scatter and trues rebinned onto the network's grid (mcpet_rebin_sinogram),
  * normalised (counts times float(voxels / total trues), the total summed exactly in double),
- * the network, and estimate = max(output, 0) times float(total trues / (voxels * decays)):
- * scatter per decay (D*H*W floats, host). scatter_counts (may be NULL): the rebinned scatter;
- * decays (may be NULL): the run's decays. */
+ * with 4 channels also the activity integrals times float(1 / their maximum) and the
+ * attenuation factors (mcpet_project_maps), the network, and estimate = max(output, 0) times
+ * float(total trues / (voxels * decays)): scatter per decay (D*H*W floats, host).
+ * scatter_counts (may be NULL): the rebinned scatter; decays (may be NULL): the run's decays. */
 mcpet_status mcpet_surrogate_estimate(mcpet_surrogate* s, mcpet_ctx* ctx, float* estimate, float* scatter_counts,
                                       int64_t* decays);
 /* loss = mean over batch and voxels of (y - target)^2; grad (host, param_count floats; may be NULL). */

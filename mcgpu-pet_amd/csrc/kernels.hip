@@ -1480,6 +1480,167 @@ hipError_t mcpet_launch_transport_batch(const KParams* d_params, int n, bool ext
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------------- projections
+
+// Dense per-voxel map for the projections: .y = mu(511 keV) = density x (a + E b) of the
+// voxel's material total-cross-section line (every voxel), .x = 0 (activity added next).
+__global__ void k_dense_mu(const KParams* __restrict__ Pd, float2* __restrict__ out, long long nvox)
+{
+  const KParams& P = *Pd;
+  const int idx = f2i_rd((K_E511 - P.e0) * P.ide);
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += (long long)gridDim.x * blockDim.x) {
+    // the palette lives in global memory here (LDS holds it only inside the transport)
+    const float2 md = P.vox_fmt == 1 ? P.palette[P.vox8[v]] : P.vox_fmt == 2 ? P.palette[P.vox16[v]] : P.vox[v];
+    const float2 tb = P.tot[idx * K_MAXMAT + (__float_as_int(md.y) & 0xff)];
+    out[v] = make_float2(0.0f, md.x * (tb.x + K_E511 * tb.y));
+  }
+}
+
+// Activity of the emitting-voxel list into the map's .x.
+__global__ void k_dense_activity(const int* __restrict__ vox, const float* __restrict__ act, long long n, float2* __restrict__ out)
+{
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[vox[i]].x = act[i];
+}
+
+// Line integrals through the phantom along the lines of response of a rebinned sinogram grid
+// (the scatter surrogate's phantom inputs). Workgroup (h, w) owns the crystal pairs
+// lors[off[h W + w] .. off[h W + w + 1]) of its transaxial bins; item i = d nl + l of the group
+// is pair l in plane group d (plane-major: a wave's lanes share a plane, whose lines all lie in
+// or all outside the phantom's axial extent), whose line runs in the plane z of the group's central ring sum
+// between the two crystal centres (angle 2 pi (k + 1/2) / N - pi at radius R around the
+// scanner axis) and is sampled at t0 + (j + 1/2) step < t1 inside the phantom's box [t0, t1]:
+// act_out = mean over the pairs of the activity integral (Bq cm), att_out = mean of
+// exp(-integral of mu(511 keV)) (map: the (activity, mu) of every voxel). One lane per item;
+// the per-plane sums run over the pairs in order (deterministic).
+__global__ __launch_bounds__(256) void k_project(const KParams* __restrict__ Pd, const float2* __restrict__ map,
+                                                 const int2* __restrict__ lors, const int* __restrict__ off, int D, int H,
+                                                 int W, int gz, float step, float* __restrict__ act_out,
+                                                 float* __restrict__ att_out)
+{
+  const KParams& P = *Pd;
+  __shared__ float s_act[256], s_att[256];
+  const int g = blockIdx.x, h = g / W, w = g - h * W;
+  const int l0 = off[g], nl = off[g + 1] - l0, n_items = nl * D;
+  const float twopi_n = 2.0f * (float)K_PI / (float)P.ncrystals;
+  float acc_a = 0.0f, acc_t = 0.0f;   // plane group threadIdx.x (< D)
+  for (int base = 0; base < n_items; base += 256) {
+    const int i = base + (int)threadIdx.x;
+    float ia = 0.0f, im = 0.0f;
+    if (i < n_items) {
+      const int d = i / nl, l = i - d * nl;
+      const float sc = fminf((float)(d * gz) + 0.5f * (float)(gz - 1), (float)(P.nzs - 1));
+      const float z = P.cz + ((0.5f * sc + 0.5f) / (float)P.nrows - 0.5f) * P.fovz;
+      const int2 pr = lors[l0 + l];
+      const float f1 = twopi_n * ((float)pr.x + 0.5f) - (float)K_PI, f2 = twopi_n * ((float)pr.y + 0.5f) - (float)K_PI;
+      const float ax = P.cx + P.radius * cosf(f1), ay = P.cy + P.radius * sinf(f1);
+      float dx = P.cx + P.radius * cosf(f2) - ax, dy = P.cy + P.radius * sinf(f2) - ay;
+      const float len = sqrtf(dx * dx + dy * dy);
+      if (len > 0.0f && z > 0.0f && z < P.bz) {
+        dx /= len;
+        dy /= len;
+        // clip to the box [0, bx] x [0, by]
+        float t0 = 0.0f, t1 = len;
+        const float o[2] = {ax, ay}, v[2] = {dx, dy}, hi[2] = {P.bx, P.by};
+        for (int k = 0; k < 2; k++) {
+          if (fabsf(v[k]) < 1e-12f) {
+            if (o[k] <= 0.0f || o[k] >= hi[k]) t1 = -1.0f;
+          } else {
+            const float ta = (0.0f - o[k]) / v[k], tb = (hi[k] - o[k]) / v[k];
+            t0 = fmaxf(t0, fminf(ta, tb));
+            t1 = fminf(t1, fmaxf(ta, tb));
+          }
+        }
+        // four samples per pass, their loads issued together
+        for (int j = 0; t0 + ((float)j + 0.5f) * step < t1; j += 4) {
+          float2 m[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const float t = t0 + ((float)(j + u) + 0.5f) * step;
+            const float p[3] = {ax + t * dx, ay + t * dy, z};
+            const int v0 = t < t1 ? locate(P, p) : -1;
+            m[u] = v0 >= 0 ? map[v0] : make_float2(0.0f, 0.0f);
+          }
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            ia += m[u].x;
+            im += m[u].y;
+          }
+        }
+      }
+    }
+    s_act[threadIdx.x] = ia * step;
+    s_att[threadIdx.x] = expf(-im * step);
+    __syncthreads();
+    if ((int)threadIdx.x < D) {
+      // items of plane d in this chunk: i = d nl + l, in increasing l
+      const int d = threadIdx.x, lo = max(base, d * nl), hi = min(base + 256, (d + 1) * nl);
+      for (int i = lo; i < hi; i++) {
+        acc_a += s_act[i - base];
+        acc_t += s_att[i - base];
+      }
+    }
+    __syncthreads();
+  }
+  if ((int)threadIdx.x < D) {
+    const size_t oi = ((size_t)threadIdx.x * H + h) * W + w;
+    act_out[oi] = nl > 0 ? acc_a / (float)nl : 0.0f;
+    att_out[oi] = nl > 0 ? acc_t / (float)nl : 1.0f;
+  }
+}
+
+// a[0..n) times float32(1 / max(a)) (a >= 0; unchanged when the maximum is 0), in two launches:
+// k_max_bits folds the block maxima into *mx (float bits, which order as unsigned integers for
+// a >= 0; *mx starts at 0), k_scale_by reads it. The maximum is order-independent, so the
+// result is exact and reproducible.
+__global__ __launch_bounds__(256) void k_max_bits(const float* __restrict__ a, int n, unsigned* __restrict__ mx)
+{
+  __shared__ float s[256];
+  float m = 0.0f;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) m = fmaxf(m, a[i]);
+  s[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] = fmaxf(s[threadIdx.x], s[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(mx, __float_as_uint(s[0]));
+}
+
+__global__ __launch_bounds__(256) void k_scale_by(float* __restrict__ a, int n, const unsigned* __restrict__ mx)
+{
+  const float m = __uint_as_float(*mx);
+  if (!(m > 0.0f)) return;
+  const float sc = (float)(1.0 / (double)m);
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) a[i] = a[i] * sc;
+}
+
+hipError_t mcpet_launch_project(const KParams* d_params, const int* active_vox, const float* active_act, long long n_active,
+                                float2* map, size_t nvox, const int2* lors, const int* off, int D, int H, int W, int gz,
+                                float step, float* act_out, float* att_out, hipStream_t stream)
+{
+  if (D < 1 || D > 256 || H < 1 || W < 1) return hipErrorInvalidValue;
+  const size_t nb = (nvox + 255) / 256;
+  const unsigned mb = nb < 1 ? 1u : nb > 8192 ? 8192u : (unsigned)nb;
+  hipLaunchKernelGGL(k_dense_mu, dim3(mb), dim3(256), 0, stream, d_params, map, (long long)nvox);
+  if (n_active > 0)
+    hipLaunchKernelGGL(k_dense_activity, dim3((unsigned)((n_active + 255) / 256)), dim3(256), 0, stream, active_vox, active_act,
+                       n_active, map);
+  hipLaunchKernelGGL(k_project, dim3(H * W), dim3(256), 0, stream, d_params, map, lors, off, D, H, W, gz, step, act_out,
+                     att_out);
+  return hipGetLastError();
+}
+
+hipError_t mcpet_launch_scale_by_max(float* a, int n, unsigned* mx, hipStream_t stream)
+{
+  hipError_t e = hipMemsetAsync(mx, 0, sizeof(unsigned), stream);
+  if (e != hipSuccess) return e;
+  const int nb = n < 1 ? 1 : (n + 255) / 256 > 256 ? 256 : (n + 255) / 256;
+  hipLaunchKernelGGL(k_max_bits, dim3(nb), dim3(256), 0, stream, a, n, mx);
+  hipLaunchKernelGGL(k_scale_by, dim3(nb), dim3(256), 0, stream, a, n, mx);
+  return hipGetLastError();
+}
+
 hipError_t mcpet_launch_fill_u32(void* p, size_t n_words, hipStream_t stream)
 {
   if (n_words == 0) return hipSuccess;

@@ -119,6 +119,14 @@ hipError_t mcpet_launch_transport(const KParams& P, const KParams* d_P, int kind
 hipError_t mcpet_launch_transport_batch(const KParams* d_params, int n, bool ext, int count_events, int blocks, int block_size,
                                         hipStream_t stream);
 hipError_t mcpet_launch_fill_u32(void* p, size_t n_words, hipStream_t stream);
+// Activity and attenuation line integrals on a rebinned sinogram grid (scatter surrogate
+// inputs): `map` is an nvox (activity, mu) scratch map; lors/off the crystal pairs of each
+// (angle, radial) group in CSR form; D <= 256.
+hipError_t mcpet_launch_project(const KParams* d_params, const int* active_vox, const float* active_act, long long n_active,
+                                float2* map, size_t nvox, const int2* lors, const int* off, int D, int H, int W, int gz,
+                                float step, float* act_out, float* att_out, hipStream_t stream);
+// a[0..n) times float32(1 / max(a)) (a >= 0), in place; mx: one device word of scratch.
+hipError_t mcpet_launch_scale_by_max(float* a, int n, unsigned* mx, hipStream_t stream);
 hipError_t mcpet_launch_selftest_ranecu(int seed, const long long* d_ids, int nstreams, int n, float* d_out, double* d_out_d,
                                         int as_double, const uint32_t* j1, const uint32_t* j2, hipStream_t stream);
 hipError_t mcpet_launch_selftest_fmath(const double* d_x, const double* d_y, int n, double* d_od, float* d_of, hipStream_t stream);

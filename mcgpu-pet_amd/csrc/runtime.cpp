@@ -7,9 +7,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <unordered_map>
@@ -83,6 +86,10 @@ struct mcpet_ctx {
   int* d_rb_sino = nullptr;
   float* d_rb_out = nullptr;
   int rb_dims[3] = {0, 0, 0};
+  // line integrals on the surrogate's grid (computed at the first request, kept)
+  KParams* d_pj_params = nullptr;
+  float* d_pj_out = nullptr;      // [3][D][H][W]: activity integral, the same / its maximum, attenuation factor; + 1 word
+  int pj_dims[3] = {0, 0, 0};
   std::string err;
 };
 
@@ -134,7 +141,7 @@ void free_all(mcpet_ctx* c)
                   c->d_trues, c->d_scatter, c->d_img_t, c->d_img_sc, c->d_spectrum, c->d_psf, c->d_psf_index,
                   c->d_voxel_edep, c->d_mat_edep, c->d_counters, c->d_final_seed, c->d_singles, c->d_singles_n,
                   c->d_keys[0], c->d_keys[1], c->d_vals[0], c->d_vals[1], c->d_hist, c->d_randoms, c->d_spurious,
-                  c->d_coinc, c->d_rb_off, c->d_rb_sino, c->d_rb_out};
+                  c->d_coinc, c->d_rb_off, c->d_rb_sino, c->d_rb_out, c->d_pj_params, c->d_pj_out};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (unsigned* p : c->d_scan)
@@ -881,6 +888,141 @@ mcpet_status mcpet_ctx_decays(mcpet_ctx* c, long long* decays)
   return MCPET_OK;
 }
 
+// ---------------------------------------------------------------- line integrals on the grid
+
+// Crystal pairs (k1 < k2) of every transaxial group of an H x W grid in CSR form (group
+// h W + w), grouped the way the rebinning groups sinogram bins: angle / ceil(NANGLES / H),
+// radial / ceil(NRAD / W), with the angle and radial bin of a pair from the binning of
+// coincidences (K:1776-1786).
+static void lor_groups_build(const KParams& P, int H, int W, std::vector<int>& off, std::vector<int2>& lors)
+{
+  const int N = P.ncrystals, gh = (P.nangles + H - 1) / H, gw = (P.nrad + W - 1) / W;
+  std::vector<std::vector<int2>> g((size_t)H * W);
+  for (int k1 = 0; k1 < N; k1++)
+    for (int k2 = k1 + 1; k2 < N; k2++) {
+      const int ith = ((k1 + k2 + P.nangles + 1) % N) / 2;
+      int ir = std::abs(k2 - k1 - P.nangles);
+      if (k1 < ith || k2 >= ith + P.nangles) ir = -ir;
+      ir += P.nrad / 2;
+      if (ir < 0 || ir >= P.nrad || ith < 0 || ith >= P.nangles) continue;
+      g[(size_t)(ith / gh) * W + ir / gw].push_back(int2{k1, k2});
+    }
+  off.assign((size_t)H * W + 1, 0);
+  lors.clear();
+  for (size_t i = 0; i < g.size(); i++) {
+    lors.insert(lors.end(), g[i].begin(), g[i].end());
+    off[i + 1] = (int)lors.size();
+  }
+}
+
+// lor_groups_build on the device, kept per (device, crystals, angles, radial bins, H, W) until
+// the process ends (every context of one scanner geometry on a device shares them).
+struct LorLists {
+  int2* lors = nullptr;
+  int* off = nullptr;
+};
+
+static mcpet_status lor_lists(mcpet_ctx* c, int H, int W, LorLists* out)
+{
+  static std::mutex mu;
+  static std::map<std::array<int, 6>, LorLists> cache;
+  const KParams& P = c->P;
+  const std::array<int, 6> key = {c->device, P.ncrystals, P.nangles, P.nrad, H, W};
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(key);
+  if (it != cache.end()) {
+    *out = it->second;
+    return MCPET_OK;
+  }
+  std::vector<int> off;
+  std::vector<int2> lors;
+  lor_groups_build(P, H, W, off, lors);
+  LorLists l;
+  hipError_t e = hipMalloc((void**)&l.lors, sizeof(int2) * std::max<size_t>(1, lors.size()));
+  if (e == hipSuccess) e = hipMalloc((void**)&l.off, sizeof(int) * off.size());
+  if (e == hipSuccess && !lors.empty()) e = hipMemcpy(l.lors, lors.data(), sizeof(int2) * lors.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(l.off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    if (l.lors) (void)hipFree(l.lors);
+    if (l.off) (void)hipFree(l.off);
+    return set_err(c, MCPET_E_NOMEM, std::string("project: crystal-pair lists: ") + hipGetErrorString(e));
+  }
+  cache[key] = l;
+  *out = l;
+  return MCPET_OK;
+}
+
+mcpet_status mcpet_ctx_project_device(mcpet_ctx* c, int D, int H, int W, const float** d_maps)
+{
+  if (!c || D < 1 || D > 256 || H < 1 || W < 1 || !d_maps)
+    return set_err(c, MCPET_E_ARG, "project: bad arguments (1 <= D <= 256, H, W >= 1)");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (c->pj_dims[0] == D && c->pj_dims[1] == H && c->pj_dims[2] == W) {
+    *d_maps = c->d_pj_out;
+    return MCPET_OK;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  LorLists l;
+  mcpet_status e = lor_lists(c, H, W, &l);
+  if (e != MCPET_OK) return e;
+  if (c->d_pj_out) (void)hipFree(c->d_pj_out);
+  c->d_pj_out = nullptr;
+  c->pj_dims[0] = c->pj_dims[1] = c->pj_dims[2] = 0;
+  const size_t n = (size_t)D * H * W;
+  if (!c->d_pj_params) e = dalloc(c, &c->d_pj_params, 1);
+  if (e == MCPET_OK) e = dalloc(c, &c->d_pj_out, 3 * n + 1);   // + the maximum's word
+  if (e != MCPET_OK) return e;
+  // the geometry, materials and voxels of the create-time parameter block (runs do not change them)
+  HIPCHK(c, hipMemcpy(c->d_pj_params, &c->P, sizeof(KParams), hipMemcpyHostToDevice));
+  // The (activity, mu) map is needed only while the projection runs: one per device, grown to
+  // the largest phantom projected and kept until the process ends; projections take turns on
+  // it (the lock is held until the stream has finished with it).
+  static std::mutex map_mu;
+  static std::map<int, std::pair<float2*, size_t>> maps;
+  std::lock_guard<std::mutex> lock(map_mu);
+  auto& m = maps[c->device];
+  if (m.second < (size_t)c->nvox) {
+    if (m.first) (void)hipFree(m.first);
+    m = {nullptr, 0};
+    HIPCHK(c, hipMalloc((void**)&m.first, sizeof(float2) * (size_t)c->nvox));
+    m.second = (size_t)c->nvox;
+  }
+  const int gz = (c->cfg.nzs + D - 1) / D;
+  const float step = 0.5f / std::max(c->P.ivx, c->P.ivy);   // half the smaller transaxial voxel side
+  hipError_t he = mcpet_launch_project(c->d_pj_params, c->d_active_vox, c->d_active_act, c->n_active, m.first,
+                                       (size_t)c->nvox, l.lors, l.off, D, H, W, gz, step, c->d_pj_out, c->d_pj_out + 2 * n,
+                                       c->stream);
+  if (he == hipSuccess)
+    he = hipMemcpyAsync(c->d_pj_out + n, c->d_pj_out, sizeof(float) * n, hipMemcpyDeviceToDevice, c->stream);
+  if (he == hipSuccess)
+    he = mcpet_launch_scale_by_max(c->d_pj_out + n, (int)n, reinterpret_cast<unsigned*>(c->d_pj_out + 3 * n), c->stream);
+  const hipError_t se = hipStreamSynchronize(c->stream);   // also on failure: the map is released after this
+  if (he == hipSuccess) he = se;
+  if (he != hipSuccess) return set_err(c, MCPET_E_HIP, std::string("projection failed: ") + hipGetErrorString(he));
+  c->pj_dims[0] = D;
+  c->pj_dims[1] = H;
+  c->pj_dims[2] = W;
+  *d_maps = c->d_pj_out;
+  return MCPET_OK;
+}
+
+static mcpet_status project_maps(mcpet_ctx* c, int D, int H, int W, float* act, float* att)
+{
+  const float* d = nullptr;
+  mcpet_status st = mcpet_ctx_project_device(c, D, H, W, &d);
+  if (st != MCPET_OK) return st;
+  const size_t n = (size_t)D * H * W;
+  if (act) HIPCHK(c, hipMemcpy(act, d, sizeof(float) * n, hipMemcpyDeviceToHost));
+  if (att) HIPCHK(c, hipMemcpy(att, d + 2 * n, sizeof(float) * n, hipMemcpyDeviceToHost));
+  return MCPET_OK;
+}
+
+extern "C" mcpet_status mcpet_project_maps(mcpet_ctx* c, int32_t D, int32_t H, int32_t W, float* act, float* att)
+{
+  if (!c) return set_err(c, MCPET_E_ARG, "mcpet_project_maps: null context");
+  return project_maps(c, D, H, W, act, att);
+}
+
 extern "C" mcpet_status mcpet_rebin_sinogram(mcpet_ctx* c, int32_t which, int32_t D, int32_t H, int32_t W, float* out)
 {
   return rebin(c, which, D, H, W, out);
@@ -1251,6 +1393,16 @@ extern "C" mcpet_status mcpet_batch_rebin_sinogram(mcpet_batch* b, int32_t k, in
   if (!b->ran) return bset_err(b, MCPET_E_STATE, "mcpet_batch_rebin_sinogram: no run");
   mcpet_status st = mcpet_batch_sync(b);
   if (st == MCPET_OK) st = rebin(b->ctx[k], which, D, H, W, out);
+  if (st != MCPET_OK) b->err = b->ctx[k]->err;
+  return st;
+}
+
+extern "C" mcpet_status mcpet_batch_project_maps(mcpet_batch* b, int32_t k, int32_t D, int32_t H, int32_t W, float* act,
+                                                 float* att)
+{
+  if (!b || k < 0 || k >= (int32_t)b->ctx.size()) return bset_err(b, MCPET_E_ARG, "mcpet_batch_project_maps: bad phantom index");
+  mcpet_status st = mcpet_batch_sync(b);
+  if (st == MCPET_OK) st = project_maps(b->ctx[k], D, H, W, act, att);
   if (st != MCPET_OK) b->err = b->ctx[k]->err;
   return st;
 }

@@ -31,5 +31,9 @@ mcpet_status mcpet_run_contexts(const CtxRun* runs, int n, hipStream_t stream, K
 mcpet_status mcpet_ctx_rebin_device(mcpet_ctx* c, int which, int D, int H, int W, float* d_out, hipStream_t st);
 // Decays of the context's last run (its event counter; waits for nothing: call after the run).
 mcpet_status mcpet_ctx_decays(mcpet_ctx* c, long long* decays);
+// Activity line integrals and attenuation factors on a D x H x W rebinned grid, on the device:
+// [3][D][H][W] = activity integrals, the same times float32(1 / their maximum), attenuation
+// factors; owned by the context, computed at the first call for this grid.
+mcpet_status mcpet_ctx_project_device(mcpet_ctx* c, int D, int H, int W, const float** d_maps);
 
 #endif

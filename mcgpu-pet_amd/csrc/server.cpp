@@ -41,7 +41,8 @@ struct Request {
   std::vector<mcpet_psf_record> psf;
   mcpet_tallies t{};
   int grid[3] = {0, 0, 0};           // rebinned request: keep only these grids
-  std::vector<float> rb_trues, rb_scatter;
+  int maps = 0;                      // and the line integrals of its phantom on that grid
+  std::vector<float> rb_trues, rb_scatter, rb_act, rb_att;
 };
 
 // Upper bound of the device memory mcpet_create allocates for (cfg, phantom, tables).
@@ -130,6 +131,7 @@ extern "C" mcpet_status mcpet_server_create(const mcpet_tables* tab, size_t budg
   return MCPET_OK;
 }
 
+// grid: NULL (a full request) or {D, H, W, maps} of a rebinned request.
 static mcpet_status submit(mcpet_server* s, const mcpet_config* cfg, const mcpet_phantom* ph, int32_t seed, const int* grid,
                            int64_t* request_id)
 {
@@ -148,8 +150,13 @@ static mcpet_status submit(mcpet_server* s, const mcpet_config* cfg, const mcpet
   r->seed = seed;
   if (grid) {
     std::memcpy(r->grid, grid, sizeof r->grid);
-    r->footprint = footprint(r->cfg, r->ph, *s->tab) + 4 * ((size_t)grid[0] * grid[1] * grid[2] + grid[0] + 1) +
+    r->maps = grid[3];
+    const size_t n = (size_t)grid[0] * grid[1] * grid[2];
+    r->footprint = footprint(r->cfg, r->ph, *s->tab) + 4 * (n + grid[0] + 1) +
                    4 * (size_t)r->cfg.nsinos;   // the rebinning's plane lists and output
+    if (r->maps)   // the device's (activity, mu) map, crystal-pair lists, the maps, parameter block
+      r->footprint += 8 * nvox + 4 * (size_t)r->cfg.ncrystals * r->cfg.ncrystals + 4 * ((size_t)grid[1] * grid[2] + 1) +
+                      8 * n + 64 * 1024;
   } else {
     r->footprint = footprint(r->cfg, r->ph, *s->tab);
   }
@@ -174,10 +181,12 @@ extern "C" mcpet_status mcpet_server_submit(mcpet_server* s, const mcpet_config*
 }
 
 extern "C" mcpet_status mcpet_server_submit_rebinned(mcpet_server* s, const mcpet_config* cfg, const mcpet_phantom* ph,
-                                                     int32_t seed, int32_t D, int32_t H, int32_t W, int64_t* request_id)
+                                                     int32_t seed, int32_t D, int32_t H, int32_t W, int32_t with_maps,
+                                                     int64_t* request_id)
 {
-  if (D < 1 || H < 1 || W < 1 || W > 256) return sfail(s, MCPET_E_ARG, "mcpet_server_submit_rebinned: bad grid (W <= 256)");
-  const int grid[3] = {D, H, W};
+  if (D < 1 || H < 1 || W < 1 || W > 256 || (with_maps && D > 256))
+    return sfail(s, MCPET_E_ARG, "mcpet_server_submit_rebinned: bad grid (W <= 256; D <= 256 with maps)");
+  const int grid[4] = {D, H, W, with_maps ? 1 : 0};
   return submit(s, cfg, ph, seed, grid, request_id);
 }
 
@@ -202,6 +211,11 @@ static mcpet_status finish_rebinned(mcpet_server* s, Request& r)
   mcpet_status st = mcpet_fetch(r.ctx, &t);   // counters only (no tally buffers given)
   if (st == MCPET_OK) st = mcpet_rebin_sinogram(r.ctx, 0, r.grid[0], r.grid[1], r.grid[2], r.rb_trues.data());
   if (st == MCPET_OK) st = mcpet_rebin_sinogram(r.ctx, 1, r.grid[0], r.grid[1], r.grid[2], r.rb_scatter.data());
+  if (st == MCPET_OK && r.maps) {
+    r.rb_act.resize(n);
+    r.rb_att.resize(n);
+    st = mcpet_project_maps(r.ctx, r.grid[0], r.grid[1], r.grid[2], r.rb_act.data(), r.rb_att.data());
+  }
   if (st != MCPET_OK) return sfail(s, st, std::string("request ") + std::to_string(r.id) + ": " + mcpet_last_error(r.ctx));
   complete(s, r);
   return MCPET_OK;
@@ -352,13 +366,18 @@ extern "C" mcpet_status mcpet_server_fetch(mcpet_server* s, int64_t id, mcpet_ta
   return MCPET_OK;
 }
 
-extern "C" mcpet_status mcpet_server_fetch_rebinned(mcpet_server* s, int64_t id, float* trues, float* scatter, int64_t* decays)
+extern "C" mcpet_status mcpet_server_fetch_rebinned(mcpet_server* s, int64_t id, float* trues, float* scatter, float* act,
+                                                    float* att, int64_t* decays)
 {
   if (!s) return sfail(s, MCPET_E_ARG, "mcpet_server_fetch_rebinned: null server");
   auto it = s->req.find(id);
   if (it == s->req.end() || it->second->state != MCPET_REQ_DONE || it->second->grid[0] == 0)
     return sfail(s, MCPET_E_STATE, "mcpet_server_fetch_rebinned: request unknown, not finished or not rebinned");
   Request& r = *it->second;
+  if ((act || att) && !r.maps)
+    return sfail(s, MCPET_E_STATE, "mcpet_server_fetch_rebinned: the request was submitted without maps");
+  if (act) std::memcpy(act, r.rb_act.data(), 4 * r.rb_act.size());
+  if (att) std::memcpy(att, r.rb_att.data(), 4 * r.rb_att.size());
   if (trues) std::memcpy(trues, r.rb_trues.data(), 4 * r.rb_trues.size());
   if (scatter) std::memcpy(scatter, r.rb_scatter.data(), 4 * r.rb_scatter.size());
   if (decays) *decays = (int64_t)(r.t.total_histories / 2);

@@ -907,7 +907,8 @@ extern "C" mcpet_status mcpet_surrogate_estimate(mcpet_surrogate* s, mcpet_ctx* 
                                                  int64_t* decays)
 {
   if (!s || !ctx || !estimate) return serr(s, MCPET_E_ARG, "mcpet_surrogate_estimate: null argument");
-  if (s->cin != 2) return serr(s, MCPET_E_ARG, "mcpet_surrogate_estimate: the network must take 2 input channels");
+  if (s->cin != 2 && s->cin != 4)
+    return serr(s, MCPET_E_ARG, "mcpet_surrogate_estimate: the network must take 2 or 4 input channels");
   if (mcpet_ctx_device(ctx) != s->device) return serr(s, MCPET_E_ARG, "mcpet_surrogate_estimate: context on another device");
   SHIP(s, hipSetDevice(s->device));
   const size_t nv = s->vr.size();
@@ -915,7 +916,11 @@ extern "C" mcpet_status mcpet_surrogate_estimate(mcpet_surrogate* s, mcpet_ctx* 
   if (st == MCPET_OK) st = mcpet_ctx_rebin_device(ctx, 0, s->D, s->H, s->W, s->raw_t, s->st);
   long long n = 0;
   if (st == MCPET_OK) st = mcpet_ctx_decays(ctx, &n);
+  const float* maps = nullptr;
+  if (st == MCPET_OK && s->cin == 4) st = mcpet_ctx_project_device(ctx, s->D, s->H, s->W, &maps);
   if (st != MCPET_OK) return serr(s, st, std::string("mcpet_surrogate_estimate: ") + mcpet_last_error(ctx));
+  if (maps)   // the scaled activity integrals and the attenuation factors
+    SHIP(s, hipMemcpyAsync(s->x0 + 2 * nv, maps + nv, 2 * sizeof(float) * nv, hipMemcpyDeviceToDevice, s->st));
   hipLaunchKernelGGL(k_sum_partial, dim3(256), dim3(256), 0, s->st, s->raw_t, nv, s->sums);
   hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, s->st, s->sums, 256, s->sums + 256);
   hipLaunchKernelGGL(k_normalize, dim3(blocks_for(nv)), dim3(256), 0, s->st, s->raw_s, s->raw_t, s->x0, nv, s->sums + 256);

@@ -201,8 +201,9 @@ def lib():
     L.mcpet_server_get_stats.argtypes = [C.c_void_p, C.c_void_p]
     L.mcpet_server_destroy.argtypes = [C.c_void_p]
     L.mcpet_server_submit_rebinned.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
-                                               C.c_int32, C.POINTER(C.c_int64)]
-    L.mcpet_server_fetch_rebinned.argtypes = [C.c_void_p, C.c_int64, P(C.c_float), P(C.c_float), C.POINTER(C.c_int64)]
+                                               C.c_int32, C.c_int32, C.POINTER(C.c_int64)]
+    L.mcpet_server_fetch_rebinned.argtypes = [C.c_void_p, C.c_int64, P(C.c_float), P(C.c_float), P(C.c_float),
+                                              P(C.c_float), C.POINTER(C.c_int64)]
     L.mcpet_server_last_error.argtypes = [C.c_void_p]
     L.mcpet_server_last_error.restype = C.c_char_p
     L.mcpet_struct_sizes.restype = C.c_int32
@@ -210,6 +211,9 @@ def lib():
     L.mcpet_rebin_sinogram.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P(C.c_float)]
     L.mcpet_batch_rebin_sinogram.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                              P(C.c_float)]
+    L.mcpet_project_maps.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, P(C.c_float), P(C.c_float)]
+    L.mcpet_batch_project_maps.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P(C.c_float),
+                                           P(C.c_float)]
     L.mcpet_surrogate_create.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64,
                                          P(C.c_void_p)]
     L.mcpet_surrogate_param_count.argtypes = [C.c_void_p]
@@ -408,6 +412,19 @@ class Simulation:
         _check(lib().mcpet_rebin_sinogram(self.ctx, TALLY_INDEX[which], D, H, W, _fptr(out)), self.ctx)
         return out
 
+    def project(self, grid, fetch=True):
+        """(activity integral, attenuation factor) of the phantom along the lines of response of
+        each (D, H, W) grid bin (mcpet_project_maps); fetch False: computed and kept on the
+        device only (for mcpet_surrogate_estimate), returns None."""
+        D, H, W = grid
+        if not fetch:
+            _check(lib().mcpet_project_maps(self.ctx, D, H, W, None, None), self.ctx)
+            return None
+        act = np.zeros((D, H, W), np.float32)
+        att = np.zeros((D, H, W), np.float32)
+        _check(lib().mcpet_project_maps(self.ctx, D, H, W, _fptr(act), _fptr(att)), self.ctx)
+        return act, att
+
     def close(self):
         if self.ctx:
             lib().mcpet_destroy(self.ctx)
@@ -550,28 +567,32 @@ class Server:
         self._meta.pop(rid)
         return out
 
-    def submit_rebinned(self, cfg, phantom, grid, seed=0):
-        """A request whose result is its trues and scatter rebinned on the device onto grid."""
+    def submit_rebinned(self, cfg, phantom, grid, seed=0, maps=False):
+        """A request whose result is its trues and scatter rebinned on the device onto grid (and,
+        with maps, the phantom's line integrals on that grid)."""
         rid = C.c_int64()
         D, H, W = grid
         st = lib().mcpet_server_submit_rebinned(self.s, C.byref(cfg), C.byref(phantom.ph), int(seed), D, H, W,
-                                                C.byref(rid))
-        self._meta[rid.value] = {"kind": "rebinned", "grid": tuple(grid)}
+                                                int(bool(maps)), C.byref(rid))
+        self._meta[rid.value] = {"kind": "rebinned", "grid": tuple(grid), "maps": bool(maps)}
         self._check(st)
         return rid.value
 
     def fetch_rebinned(self, rid):
-        """(trues, scatter, decays) of a finished rebinned request."""
+        """(trues, scatter, decays, maps) of a finished rebinned request; maps is (activity
+        integral, attenuation factor) or None."""
         m = self._meta.get(rid)
         if m is None or m["kind"] != "rebinned":    # the library reports the error
-            self._check(lib().mcpet_server_fetch_rebinned(self.s, int(rid), None, None, None))
+            self._check(lib().mcpet_server_fetch_rebinned(self.s, int(rid), None, None, None, None, None))
         grid = m["grid"]
         t = np.zeros(grid, np.float32)
         s = np.zeros(grid, np.float32)
+        maps = (np.zeros(grid, np.float32), np.zeros(grid, np.float32)) if m["maps"] else None
         d = C.c_int64()
-        self._check(lib().mcpet_server_fetch_rebinned(self.s, int(rid), _fptr(t), _fptr(s), C.byref(d)))
+        self._check(lib().mcpet_server_fetch_rebinned(self.s, int(rid), _fptr(t), _fptr(s), _fptr(maps[0]) if maps else None,
+                                                      _fptr(maps[1]) if maps else None, C.byref(d)))
         self._meta.pop(rid)
-        return t, s, d.value
+        return t, s, d.value, maps
 
     def stats(self):
         x = ServerStats()
@@ -634,6 +655,14 @@ class Batch:
         out = np.zeros((D, H, W), np.float32)
         self._check(lib().mcpet_batch_rebin_sinogram(self.b, k, TALLY_INDEX[which], D, H, W, _fptr(out)))
         return out
+
+    def project(self, k, grid):
+        """Simulation.project of phantom k."""
+        D, H, W = grid
+        act = np.zeros((D, H, W), np.float32)
+        att = np.zeros((D, H, W), np.float32)
+        self._check(lib().mcpet_batch_project_maps(self.b, k, D, H, W, _fptr(act), _fptr(att)))
+        return act, att
 
     def autotune(self, fraction=0.05):
         res = (TuneResult * 32)()

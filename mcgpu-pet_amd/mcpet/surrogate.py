@@ -1,6 +1,7 @@
-"""Learned scatter surrogate: a low-count Monte Carlo run, rebinned on the device, turned into
-a scatter estimate by the 3D U-Net (mcpet_surrogate_*), with a consistency check that falls
-back to Monte Carlo. The reference only runs Monte Carlo (MCGPU-PET.cu:450-495); this module
+"""Learned scatter surrogate: a low-count Monte Carlo run, rebinned on the device, and the
+phantom's activity and attenuation seen along the same lines of response (mcpet_project_maps)
+turned into a scatter estimate by the 3D U-Net (mcpet_surrogate_*), with a consistency check
+that falls back to Monte Carlo. The reference only runs Monte Carlo (MCGPU-PET.cu:450-495); this module
 has no counterpart there.
 
 Grid (fixed before any training or measurement): the 168-angle x 168-radial x 1293-sinogram
@@ -33,7 +34,7 @@ GRID = (32, 56, 56)
 N_LOW = 5.0e6              # decays: 1e7 photon histories
 TARGET = 0.05
 WIDTH = 32
-IN_CHANNELS = 2
+IN_CHANNELS = 4                 # low-count scatter, low-count trues, activity integrals, attenuation factors
 DECAYS_PER_SECOND = 1.0e9       # activity of the workload's phantoms: E[decays] = 1e9 per second
 DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 WEIGHTS = os.path.join(DATA_DIR, f"scatter_unet_w{WIDTH}.npy")
@@ -68,18 +69,26 @@ def decays_for(ref_per_decay, target=TARGET):
     return float(r.sum() / ((r * r).sum() * target * target))
 
 
-def normalize(scatter_low, trues_low, n_low):
-    """Network input (2 channels: scatter and trues scaled so that the trues channel has mean 1)
-    and the factor that turns the network's output into scatter per decay. The arithmetic is
-    the device path's (mcpet_surrogate_estimate): counts times float32(voxels / total trues), the
-    total exact in float64; factor float32(total trues / (voxels * decays))."""
+def normalize(scatter_low, trues_low, n_low, maps=None):
+    """Network input and the factor that turns the network's output into scatter per decay.
+    Channels: scatter and trues scaled so that the trues channel has mean 1; with maps =
+    (activity integrals, attenuation factors) also the activity integrals divided by their
+    maximum and the attenuation factors. The arithmetic is the device path's
+    (mcpet_surrogate_estimate): counts times float32(voxels / total trues), the total exact in
+    float64; activity times float32(1 / maximum); factor float32(total trues / (voxels * decays))."""
     s = np.asarray(scatter_low, np.float32)
     t = np.asarray(trues_low, np.float32)
     tot = float(np.sum(t, dtype=np.float64))
     nvox = t.size
     scale = np.float32(nvox / tot) if tot > 0 else np.float32(0.0)
     factor = np.float32(tot / (nvox * float(n_low))) if tot > 0 and n_low > 0 else np.float32(0.0)
-    return np.stack([s * scale, t * scale]), factor
+    ch = [s * scale, t * scale]
+    if maps is not None:
+        act = np.asarray(maps[0], np.float32)
+        att = np.asarray(maps[1], np.float32)
+        mx = float(act.max()) if act.size else 0.0
+        ch += [act * np.float32(1.0 / mx) if mx > 0 else act.copy(), att.copy()]
+    return np.stack(ch), factor
 
 
 def consistency_z(pred_per_decay, scatter_low, n_low, block=(4, 4, 4), min_mean=20.0):
@@ -102,29 +111,50 @@ def consistency_z(pred_per_decay, scatter_low, n_low, block=(4, 4, 4), min_mean=
     return (chi2 - dof) / np.sqrt(2.0 * dof)
 
 
-class ScatterEstimator:
-    """The trained U-Net and the estimate / fallback policy."""
+def param_count(in_channels, width):
+    """Parameters of the U-Net (mcpet_surrogate_create's layer list)."""
+    c = width
+    layers = [(in_channels, c, 3), (c, c, 3), (c, 2 * c, 3), (2 * c, 2 * c, 3), (2 * c, 4 * c, 3), (4 * c, 4 * c, 3),
+              (6 * c, 2 * c, 3), (3 * c, c, 3), (c, 1, 1)]
+    return sum(co * ci * k ** 3 + co for ci, co, k in layers)
 
-    def __init__(self, weights=WEIGHTS, width=WIDTH, grid=GRID, z_max=5.0):
-        self.net = Surrogate(grid, in_channels=IN_CHANNELS, width=width, seed=1)
+
+class ScatterEstimator:
+    """The trained U-Net and the estimate / fallback policy. in_channels None: 2 or 4, from the
+    size of the weights (IN_CHANNELS without weights)."""
+
+    def __init__(self, weights=WEIGHTS, width=WIDTH, grid=GRID, z_max=5.0, in_channels=None):
         if isinstance(weights, str):
             weights = np.load(weights, allow_pickle=False)
+        if in_channels is None:
+            in_channels = IN_CHANNELS
+            if weights is not None:
+                fits = [c for c in (2, 4) if param_count(c, width) == np.size(weights)]
+                if not fits:
+                    raise ValueError(f"{np.size(weights)} weights fit no 2- or 4-channel network of width {width}")
+                in_channels = fits[0]
+        self.in_channels = int(in_channels)
+        self.net = Surrogate(grid, in_channels=self.in_channels, width=width, seed=1)
         if weights is not None:
             self.net.set_params(weights)
         self.grid = tuple(grid)
         self.z_max = z_max
 
-    def predict(self, scatter_low, trues_low, n_low):
-        """Scatter per decay (float32) from rebinned low-count scatter and trues of n_low decays."""
-        x, factor = normalize(scatter_low, trues_low, n_low)
+    def predict(self, scatter_low, trues_low, n_low, maps=None):
+        """Scatter per decay (float32) from rebinned low-count scatter and trues of n_low decays
+        and (for a 4-channel network) the phantom's maps on the grid (Simulation.project)."""
+        if (maps is not None) != (self.in_channels == 4):
+            raise ValueError(f"a {self.in_channels}-channel network " + ("needs" if self.in_channels == 4 else "takes no")
+                             + " maps")
+        x, factor = normalize(scatter_low, trues_low, n_low, maps)
         y = self.net.forward(x)[0]
         return np.maximum(y, np.float32(0.0)) * factor
 
     def estimate(self, sim, seed=0):
         """Scatter per decay on the grid from a run of `sim` (a Simulation of the phantom with
-        config(materials, N_LOW)): the run, then the fused device estimate (rebinning,
+        config(materials, N_LOW)): the run, then the fused device estimate (rebinning, maps,
         normalisation, network). Returns (estimate, info); info["consistent"] False means: use
-        Monte Carlo. Equal to predict() on the rebinned tallies, bit for bit."""
+        Monte Carlo. Equal to predict() on the rebinned tallies and maps, bit for bit."""
         t0 = time.perf_counter()
         sim.run(seed=seed)
         est, s, decays = self.net.estimate(sim)
@@ -173,7 +203,8 @@ class ScatterService:
     def submit(self, phantom, seed=1):
         sid = self.next_id
         self.next_id += 1
-        rid = self.server.submit_rebinned(config(self.names, N_LOW), phantom, self.est.grid, seed=seed)
+        rid = self.server.submit_rebinned(config(self.names, N_LOW), phantom, self.est.grid, seed=seed,
+                                          maps=self.est.in_channels == 4)
         spot = self.spot_every > 0 and sid % self.spot_every == 0
         self.jobs[sid] = {"phantom": phantom, "seed": seed, "stage": "surrogate", "t0": time.perf_counter(),
                           "result": None, "info": None, "spot": spot}
@@ -189,9 +220,9 @@ class ScatterService:
     def _complete(self, rid):
         sid = self.by_rid.pop(rid)
         job = self.jobs[sid]
-        t, s, decays = self.server.fetch_rebinned(rid)
+        t, s, decays, maps = self.server.fetch_rebinned(rid)
         if job["stage"] == "surrogate":
-            est = self.est.predict(s, t, decays)
+            est = self.est.predict(s, t, decays, maps)
             z = consistency_z(est, s, decays)
             job["info"] = {"decays": decays, "z": z, "consistent": bool(z <= self.est.z_max)}
             n5 = decays_for(est) if est.sum() > 0 else 1e9
@@ -289,12 +320,13 @@ def family_phantoms(family, n, seed):
 
 
 def generate(n_phantoms, seed, n_target=5.0e8, n_low=N_LOW, inputs_per_phantom=4, chunk=16, families=("scatter",),
-             log=print):
+             in_channels=IN_CHANNELS, log=print):
     """Training pairs from n_phantoms phantoms split evenly over `families` (family_phantoms,
     seeds seed, seed + 1, ...): per phantom one target run of n_target decays (scatter per
     decay) and inputs_per_phantom independent runs of n_low decays (normalised scatter +
-    trues). Returns X [n, 2, D, H, W], Y [n, D, H, W] (targets in the inputs' normalisation)
-    and the phantom index of each sample (phantoms interleaved over the families)."""
+    trues, with in_channels 4 also the phantom's maps). Returns X [n, in_channels, D, H, W],
+    Y [n, D, H, W] (targets in the inputs' normalisation) and the phantom index of each sample
+    (phantoms interleaved over the families)."""
     names = _materials()
     per = [n_phantoms // len(families) + (1 if i < n_phantoms % len(families) else 0) for i in range(len(families))]
     lists = [family_phantoms(f, k, seed + i) for i, (f, k) in enumerate(zip(families, per))]
@@ -313,12 +345,13 @@ def generate(n_phantoms, seed, n_target=5.0e8, n_low=N_LOW, inputs_per_phantom=4
                 d = b.stats(k)["decays"]
                 targets.append(b.rebin(k, "scatter", GRID).astype(np.float64) / d)
         with Batch(config(names, n_low), phs, tab) as b:
+            maps = [b.project(k, GRID) if in_channels == 4 else None for k in range(len(phs))]
             for r in range(inputs_per_phantom):
                 b.run(seed=int(seed * 1000 + c0 + 101 + r))
                 b.sync()
                 for k in range(len(phs)):
                     d = b.stats(k)["decays"]
-                    x, factor = normalize(b.rebin(k, "scatter", GRID), b.rebin(k, "trues", GRID), d)
+                    x, factor = normalize(b.rebin(k, "scatter", GRID), b.rebin(k, "trues", GRID), d, maps[k])
                     X.append(x)
                     Y.append((targets[k] / np.float64(factor)).astype(np.float32))
                     idx.append(c0 + k)
@@ -330,6 +363,7 @@ def generate(n_phantoms, seed, n_target=5.0e8, n_low=N_LOW, inputs_per_phantom=4
 
 
 def train(X, Y, idx, width=WIDTH, epochs=150, batch=8, lr=5e-4, warmup=200, val_fraction=0.125, seed=1, log=print):
+    in_channels = X.shape[1]
     """Adam on the MSE, learning rate warmed up linearly over `warmup` steps then decayed on a
     cosine; the parameters of the epoch with the lowest validation loss (phantoms held out by
     index) are kept."""
@@ -340,7 +374,7 @@ def train(X, Y, idx, width=WIDTH, epochs=150, batch=8, lr=5e-4, warmup=200, val_
     val = np.array([i in val_ids for i in idx])
     tr = np.flatnonzero(~val)
     va = np.flatnonzero(val)
-    net = Surrogate(GRID, in_channels=IN_CHANNELS, width=width, seed=seed)
+    net = Surrogate(GRID, in_channels=in_channels, width=width, seed=seed)
     history = []
     best = (np.inf, None, -1)
     steps = epochs * max(1, len(tr) // batch)
@@ -381,6 +415,7 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=150)
     ap.add_argument("--width", type=int, default=WIDTH)
     ap.add_argument("--lr", type=float, default=5e-4)
+    ap.add_argument("--in-channels", type=int, default=IN_CHANNELS, choices=(2, 4))
     ap.add_argument("--out", default="gpurun_out/surrogate")
     a = ap.parse_args(argv)
     os.makedirs(a.out, exist_ok=True)
@@ -388,13 +423,14 @@ def main(argv=None):
     t0 = time.perf_counter()
     fams = tuple(a.families.split(","))
     X, Y, idx = generate(a.phantoms, a.seed, n_target=a.n_target, inputs_per_phantom=a.inputs_per_phantom, families=fams,
-                         log=log)
+                         in_channels=a.in_channels, log=log)
     gen_s = time.perf_counter() - t0
     log(f"generate: {len(X)} samples in {gen_s:.1f} s")
     w, info = train(X, Y, idx, width=a.width, epochs=a.epochs, lr=a.lr, log=log)
     np.save(os.path.join(a.out, f"scatter_unet_w{a.width}.npy"), w.astype(np.float32))
     info.update({"phantoms": a.phantoms, "families": fams, "phantom_seed": a.seed, "n_target": a.n_target, "n_low": N_LOW,
-                 "inputs_per_phantom": a.inputs_per_phantom, "grid": GRID, "width": a.width, "generate_seconds": gen_s})
+                 "inputs_per_phantom": a.inputs_per_phantom, "grid": GRID, "width": a.width, "in_channels": a.in_channels,
+                 "generate_seconds": gen_s})
     with open(os.path.join(a.out, f"scatter_unet_w{a.width}.json"), "w") as f:
         json.dump(info, f)
     log(f"saved {a.out}/scatter_unet_w{a.width}.npy (best epoch {info['best_epoch']})")

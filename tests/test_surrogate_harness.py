@@ -64,6 +64,23 @@ def test_normalize_is_invariant_to_the_number_of_decays():
     assert abs(float(f1) * float(x1[1].sum()) * 5e6 / t.sum() - 1) < 1e-6
 
 
+def test_normalize_with_maps_scales_activity_by_its_maximum():
+    """4-channel input: channels 0-1 as without maps; channel 2 the activity integrals times
+    float32(1 / max) (maximum exactly 1), channel 3 the attenuation factors unchanged."""
+    rng = np.random.default_rng(5)
+    s = rng.poisson(3.0, S.GRID).astype(np.float32)
+    t = rng.poisson(9.0, S.GRID).astype(np.float32)
+    act = (rng.random(S.GRID) * 3e4).astype(np.float32)
+    att = rng.random(S.GRID).astype(np.float32)
+    x2, f2 = S.normalize(s, t, 5e6)
+    x4, f4 = S.normalize(s, t, 5e6, (act, att))
+    assert x4.shape == (4,) + S.GRID and x4.dtype == np.float32 and f4 == f2
+    assert np.array_equal(x4[:2], x2) and np.array_equal(x4[3], att)
+    assert x4[2].max() == 1.0 and np.array_equal(x4[2], act * np.float32(1.0 / float(act.max())))
+    z, _ = S.normalize(s, t, 5e6, (np.zeros_like(act), att))
+    assert np.all(z[2] == 0)
+
+
 def test_workload_config():
     c = S.config([], 5e6)
     assert c.acquisition_time_ps == 5_000_000_000 and c.stream_decays == 5 and c.nrad == 168 and c.nsinos == 1293

@@ -26,7 +26,7 @@ def test_server_rebinned_request_equals_simulation_rebinned():
             srv.fetch_rebinned(b)
         with pytest.raises(mcpet.McpetError):
             srv.fetch(a)
-        t, s, d = srv.fetch_rebinned(a)
+        t, s, d, _ = srv.fetch_rebinned(a)
         full = srv.fetch(b)
     assert d == d0 and np.array_equal(t, t0) and np.array_equal(s, s0) and t.sum() > 0
     assert full["trues"].sum() > 0
@@ -34,8 +34,9 @@ def test_server_rebinned_request_equals_simulation_rebinned():
 
 @pytest.mark.gpu
 def test_fused_device_estimate_equals_the_host_path():
-    """mcpet_surrogate_estimate (rebinning, normalisation, network and scaling on the device)
-    equals predict() on the rebinned tallies bit for bit, with the same counts and decays."""
+    """mcpet_surrogate_estimate (rebinning, maps, normalisation, network and scaling on the
+    device) equals predict() on the rebinned tallies and maps bit for bit, with the same counts
+    and decays."""
     phs = _phantoms(1, 77)
     names = S._materials()
     tab = mcpet.load_materials(names, phs[0].density_max())
@@ -45,8 +46,9 @@ def test_fused_device_estimate_equals_the_host_path():
         e, cnt, d = est.net.estimate(sim)
         s = sim.rebin("scatter", S.GRID)
         t = sim.rebin("trues", S.GRID)
+        maps = sim.project(S.GRID)
         d0 = sim.stats()["decays"]
-    est2 = est.predict(s, t, d0)
+    est2 = est.predict(s, t, d0, maps)
     assert d == d0 and np.array_equal(cnt, s)
     assert e.dtype == np.float32 and np.array_equal(e, est2) and e.sum() > 0
     est.close()
