@@ -58,7 +58,8 @@ def cpu_baseline(cfg, ph, vs, names, threads, stride):
     r = o.run(nthreads=threads, voxel_stride=stride)
     c = r["counters"]
     return {"value": c["coincidences"] / r["wall_seconds"], "unit": "coincidences/s", "cores": threads, "kind": "port",
-            "sample": f"every {stride}th voxel of the same phantom, full 10 s acquisition: {c['decays']} decays, "
+            "sample": f"every {stride}th voxel of the same phantom, full 10 s acquisition: "
+                      f"{c['decays']} decays, "
                       f"{c['coincidences']} coincidences in {r['wall_seconds']:.1f} s (C restatement, libm, "
                       f"{threads} threads)"}
 
@@ -281,7 +282,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--decays", type=float, default=1.0e9)
+    ap.add_argument("--decays", type=float, default=0.0, help="expected decays (0: 1e9 head, 5e9 torso)")
     ap.add_argument("--tpb", type=int, default=32, help="decay streams per emitting voxel")
     ap.add_argument("--kernel", choices=["persistent", "plain"], default="persistent")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
@@ -289,8 +290,9 @@ def main():
     ap.add_argument("--phase-policy", type=int, default=0, help="0 auto, 1 most lanes, 2 least waste")
     ap.add_argument("--phase-cost", type=str, default="", help="comma-separated relative phase costs (DECAY, SOURCE2, STEP, INTERACT, TALLY, CTAU, CPZ)")
     ap.add_argument("--voxel-format", type=int, default=0, help="0 auto (palette indices), 1 8-byte voxel words")
-    ap.add_argument("--workload", choices=["head", "batch", "surrogate", "unet"], default="head",
-                    help="head: the 1e9-decay head phantom; batch: 48 varied phantoms in one launch; "
+    ap.add_argument("--workload", choices=["head", "torso", "batch", "surrogate", "unet"], default="head",
+                    help="head: the 1e9-decay head phantom; torso: the 5e9-decay torso phantom with beta+ "
+                         "branching, prompt gammas and dose tallies; batch: 48 varied phantoms in one launch; "
                          "surrogate: time to a 5%%-NRMSE scatter sinogram, Monte Carlo vs the learned surrogate; "
                          "unet: the surrogate network alone")
     ap.add_argument("--eval-phantoms", type=int, default=6)
@@ -301,7 +303,7 @@ def main():
     ap.add_argument("--batch-size", type=int, default=48)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--cpu-stride", type=int, default=5)
+    ap.add_argument("--cpu-stride", type=int, default=0, help="CPU baseline voxel stride (0: 5 head, 50 torso)")
     ap.add_argument("--copy-calibration", type=float, default=0,
                     help="only run the library's device copy of this many bytes (HBM counter calibration) and exit")
     a = ap.parse_args()
@@ -323,10 +325,31 @@ def main():
         return bench_surrogate(a)
     if a.workload == "unet":
         return bench_unet(a)
-    mat, dens, act, vs = phantoms.head(n_decays=a.decays)
-    names = phantoms.head_materials(MATERIALS)
-    cfg = phantoms.make_config(seed=20261202, threads_per_block=a.tpb, acquisition_time_s=10.0, nrad=168,
-                               materials=names)
+    torso = a.workload == "torso"
+    n_decays = a.decays or (5.0e9 if torso else 1.0e9)
+    stride = a.cpu_stride or (50 if torso else 5)
+    nd = f"{n_decays:.0e}".replace("e+0", "e").replace("e+", "e")
+    if torso:
+        # 50 % beta+ and a prompt gamma per decay (at 514 keV: the material files end at 515 keV),
+        # 64-bit dose tallies over every voxel, a 46 cm detector around the 64 x 64 cm box
+        mat, dens, act, vs = phantoms.torso(n_decays=n_decays)
+        names = phantoms.torso_materials(MATERIALS)
+        nz, ny, nx = mat.shape
+        cfg = phantoms.make_config(seed=20261202, threads_per_block=1, acquisition_time_s=10.0, nrad=168, radius=-46.0,
+                                   materials=names, positron_fraction=0.5, prompt_gamma_fraction=1.0,
+                                   prompt_gamma_energy=514.0e3, material_dose=True, voxel_dose=True,
+                                   roi=(0, nx - 1, 0, ny - 1, 0, nz - 1),
+                                   stream_decays=int(max(4, np.ceil(n_decays / 2.0 ** 20))))
+        workload = (f"torso phantom 256x256x200 at 2.5 mm, 7 tissues, 50% beta+ and a 514 keV prompt gamma per decay, "
+                    f"{nd} decays in 10 s, sinogram 168x168x1293, 64-bit voxel and material dose, fused "
+                    f"transport + tallies")
+    else:
+        mat, dens, act, vs = phantoms.head(n_decays=n_decays)
+        names = phantoms.head_materials(MATERIALS)
+        cfg = phantoms.make_config(seed=20261202, threads_per_block=a.tpb, acquisition_time_s=10.0, nrad=168,
+                                   materials=names)
+        workload = (f"head phantom 128x128x96 at 2 mm, 8 tissues, 18F, {nd} decays in 10 s, "
+                    f"sinogram 168x168x1293, fused transport + tallies")
     cfg.voxel_format = a.voxel_format
     ph = mcpet.phantom_from_arrays(mat, dens, act, vs)
     tab = mcpet.load_materials(names, ph.density_max())
@@ -368,14 +391,13 @@ def main():
     avg_ms = sum(kernel_ms) / len(kernel_ms)
     valu = ops / kt
     hbm = byts / kt
-    traffic = load_traffic()
+    traffic = None if torso else load_traffic()
     out = {
         "metric": "detected coincidences/s on 1 MI355X", "value": coinc / elapsed, "unit": "coincidences/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * elapsed / a.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "head phantom 128x128x96 at 2 mm, 8 tissues, 18F, 1e9 decays in 10 s, "
-                               "sinogram 168x168x1293, fused transport + tallies",
-                   "decays_per_step": decays / a.steps, "threads_per_block": a.tpb, "kernel": a.kernel,
+        "config": {"workload": workload,
+                   "decays_per_step": decays / a.steps, "threads_per_block": 1 if torso else a.tpb, "kernel": a.kernel,
                    "parallelism": "single GPU"},
         "decays_per_s": decays / elapsed, "photon_histories_per_s": 2 * decays / elapsed,
         "kernel_ms_avg": avg_ms,
@@ -395,7 +417,7 @@ def main():
                               for n, e, l, c in zip(("decay", "source2", "step", "interact", "tally", "ctau", "cpz", "detect", "source3"),
                                                     agg["phase_exec"], agg["phase_lanes"], agg["phase_cycles"])}
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(cfg, ph, vs, names, a.cpu_threads, a.cpu_stride)
+        out["cpu_baseline"] = cpu_baseline(cfg, ph, vs, names, a.cpu_threads, stride)
     print(json.dumps(out))
 
 

@@ -211,6 +211,71 @@ def head(seed=20261202, shape=(96, 128, 128), voxel_cm=0.2, n_decays=1.0e9, acqu
     return mat, dens, act, (voxel_cm,) * 3
 
 
+TORSO_TISSUES = [  # slot (1-based), name, material file, density g/cm3, activity weight
+    (1, "air", AIR, 0.0012, 0.0),
+    (2, "soft tissue", WATER, 1.03, 1.0),
+    (3, "fat", WATER, 0.95, 0.5),
+    (4, "bone", WATER, 1.85, 0.8),
+    (5, "lung", WATER, 0.30, 0.25),
+    (6, "liver", WATER, 1.06, 4.0),
+    (7, "lesion", WATER, 1.03, 12.0),
+]
+
+
+def torso(seed=20261202, shape=(200, 256, 256), voxel_cm=0.25, n_decays=5.0e9, acquisition_time_s=10.0,
+          mean_life=F18_MEAN_LIFE):
+    """Torso-size phantom (64 x 64 x 50 cm at 2.5 mm): an elliptic body (34 x 22 cm, 46 cm long)
+    of soft tissue under 1.5 cm of fat, two lungs (0.3 g/cm3), spine and ribs (bone 1.85),
+    liver, and two hot lesions, with seed-jittered sizes and positions. Lung, bone and liver
+    use the water tables at their densities (only air and water tables exist). Activity
+    weights as TORSO_TISSUES, scaled to n_decays expected decays. Returns material slots 1..7."""
+    rng = np.random.default_rng(seed)
+    nz, ny, nx = shape
+    vs = voxel_cm
+    cx, cy, cz = nx * vs / 2, ny * vs / 2, nz * vs / 2
+    j = lambda s: 1.0 + s * (rng.random() - 0.5)
+    z = (np.arange(nz, dtype=np.float32) + 0.5) * vs
+    y = (np.arange(ny, dtype=np.float32) + 0.5) * vs
+    x = (np.arange(nx, dtype=np.float32) + 0.5) * vs
+    X, Y = np.meshgrid(x, y)                                  # [ny, nx]
+    ax, ay = 17.0 * j(0.06), 11.0 * j(0.06)
+    body2 = ((X - cx) / ax) ** 2 + ((Y - cy) / ay) ** 2       # < 1 inside the body's outline
+    inner2 = ((X - cx) / (ax - 1.5)) ** 2 + ((Y - cy) / (ay - 1.5)) ** 2
+    ring2 = ((X - cx) / (ax - 2.0)) ** 2 + ((Y - cy) / (ay - 2.0)) ** 2
+    z0, z1 = cz - 23.0, cz + 23.0
+    mat = np.ones(shape, np.int32)
+    inz = (z > z0) & (z < z1)
+    mat[inz] = np.where(body2 < 1.0, 3, 1)[None]             # fat everywhere inside the outline
+    mat[inz] = np.where(inner2 < 1.0, 2, mat[inz])           # soft tissue under it
+    thorax = (z > cz + 2.0) & (z < z1 - 2.0)
+    for side in (-1, 1):                                      # lungs
+        c = np.array([cx + side * 7.5 * j(0.1), cy + 0.5 * j(0.5), cz + 12.0 * j(0.05)], np.float32)
+        mat[_ellipsoid(shape, vs, c, np.array([5.0, 6.5, 9.5], np.float32) * j(0.08))] = 5
+    for zr in np.arange(cz + 3.0, z1 - 2.0, 2.5):             # ribs: 0.75 cm bands of the ring
+        k = (z > zr) & (z < zr + 0.75) & thorax
+        band = (ring2 < 1.0) & (ring2 > 0.80)
+        mat[k] = np.where(band, 4, mat[k])
+    spine = ((X - cx) ** 2 + (Y - (cy - ay + 4.0)) ** 2) < (1.6 * j(0.1)) ** 2
+    mat[inz] = np.where(spine, 4, mat[inz])
+    cl = np.array([cx - 6.0 * j(0.2), cy + 1.0 * j(0.5), cz - 2.0 * j(0.2)], np.float32)
+    mat[_ellipsoid(shape, vs, cl, np.array([7.5, 6.0, 6.5], np.float32) * j(0.08))] = 6   # liver
+    for c in (cl + np.array([2.0, -1.0, 1.0], np.float32), np.array([cx + 7.0, cy, cz + 12.0], np.float32)):
+        mat[_ellipsoid(shape, vs, c * np.float32(j(0.01)), np.array([1.0, 1.0, 1.0], np.float32))] = 7
+    dens = np.zeros(shape, np.float32)
+    w = np.zeros(shape, np.float64)
+    for slot, _, _, d, a in TORSO_TISSUES:
+        m = mat == slot
+        dens[m] = d
+        w[m] = a
+    frac = mean_life * (1.0 - np.exp(-acquisition_time_s / mean_life))
+    act = (w * (n_decays / (w.sum() * frac))).astype(np.float32)
+    return mat, dens, act, (voxel_cm,) * 3
+
+
+def torso_materials(materials_dir):
+    return [os.path.join(materials_dir, f) for (_, _, f, _, _) in TORSO_TISSUES]
+
+
 def random_phantoms(count=48, seed=20261202, voxel_cm=0.2, histories=(1.0e6, 1.0e8), xy=(32, 160), z=(32, 96),
                     acquisition_time_s=10.0, mean_life=F18_MEAN_LIFE):
     """A batch of varied phantoms (the many-phantom workload): nx, ny ~ U[xy], nz ~ U[z] at

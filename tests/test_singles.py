@@ -134,8 +134,11 @@ def test_device_radix_sort_is_a_stable_sort(n, bits):
     assert np.array_equal(perm, np.argsort(keys, kind="stable").astype(np.uint32))
 
 
-def _check_against_oracle(sc):
-    o = sc.run_oracle("fmath", nthreads=8)
+def _check_against_oracle(sc, nthreads=8, keep=None):
+    """GPU singles pipeline vs the oracle on sc (keep: a dict that receives the oracle's result)."""
+    o = sc.run_oracle("fmath", nthreads=nthreads)
+    if keep is not None:
+        keep["oracle"] = o
     g = sc.run_gpu(kernel=mcpet.KERNEL_PERSISTENT)
     # singles list: the same records (the device list comes sorted by time)
     gs, os_ = g["singles"], o["singles"]
