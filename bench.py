@@ -243,7 +243,7 @@ def bench_surrogate(a):
 
 
 def bench_unet(a):
-    """The surrogate's U-Net alone on the 32 x 56 x 56 grid (width 32, 2 input channels,
+    """The surrogate's U-Net alone on the 32 x 56 x 56 grid (width 32, 4 input channels,
     random inputs): a.steps forward passes of one sample and a.steps training steps of a
     batch of 8, after a.warmup of each; GFLOP counted from the layer shapes."""
     import mcpet
@@ -277,6 +277,77 @@ def bench_unet(a):
     print(json.dumps(out))
 
 
+def bench_serve(a):
+    """Scatter estimates served to Poisson-arriving requests: a.serve_requests phantoms cycling
+    through the scatter, head and batch families (mixed sizes, seeds disjoint from training),
+    arriving at a.serve_rate per second, each answered by the serving loop (low-count run,
+    rebinning and maps on the device, the U-Net, the consistency check, Monte Carlo fallback,
+    every a.spot_every-th request spot-checked against Monte Carlo at N5) under an HBM budget.
+    Latency = completion - arrival; the phantoms are built before the clock starts."""
+    import mcpet
+    from mcpet import surrogate as S
+    fams = ("scatter", "head", "batch")
+    n = a.serve_requests
+    per = {f: S.family_phantoms(f, (n + 2) // 3, a.eval_seed + 11 * k) for k, f in enumerate(fams)}
+    specs = [per[fams[i % 3]][i // 3] for i in range(n)]
+    phs = [mcpet.phantom_from_arrays(*sp) for sp in specs]
+    names = S._materials()
+    tab = mcpet.load_materials(names, np.max([p.density_max() for p in phs], axis=0))
+    est = S.ScatterEstimator(a.surrogate_weights or S.WEIGHTS)
+    # warm-up: one request alone (kernels loaded, per-device buffers and crystal-pair lists built)
+    svc = S.ScatterService(tab, est, budget_bytes=a.serve_budget_gb << 30, max_running=16)
+    svc.submit(phs[0], seed=1)
+    svc.drain()
+    svc.close()
+    svc = S.ScatterService(tab, est, budget_bytes=a.serve_budget_gb << 30, max_running=16,
+                           spot_check_every=a.spot_every)
+    rng = np.random.default_rng(a.eval_seed)
+    arrival = np.cumsum(rng.exponential(1.0 / a.serve_rate, n))
+    sid_of = {}
+    i = 0
+    t0 = time.perf_counter()
+    while i < n or svc.by_rid:
+        now = time.perf_counter() - t0
+        while i < n and arrival[i] <= now:
+            sid_of[svc.submit(phs[i], seed=1000 + i)] = i
+            i += 1
+        if svc.by_rid:
+            svc.step()
+        elif i < n:
+            time.sleep(max(0.0, arrival[i] - (time.perf_counter() - t0)))
+    makespan = time.perf_counter() - t0
+    lat, kinds = [], []
+    for sid, k in sid_of.items():
+        job = svc.jobs[sid]
+        done = job["t0"] + job["seconds"] - t0
+        lat.append(1e3 * (done - arrival[k]))
+        kinds.append(job["stage"])
+    st = svc.stats()
+    svc.close()
+    est.close()
+    for p in phs:
+        p.close()
+    tab.close()
+    lat = np.array(lat)
+    surrogate_lat = lat[[k == "surrogate" for k in kinds]]
+    out = {"metric": "scatter-estimate requests served per second (Poisson arrivals)", "value": n / makespan,
+           "unit": "requests/s", "n_gpus": 1, "steps": n, "warmup": 1, "ms_per_step": 1e3 * makespan / n,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "bf16 MFMA (fp32 accumulate) + f32 transport", "data": "synthetic",
+           "config": {"workload": f"{n} requests, Poisson arrivals at {a.serve_rate:g}/s, phantoms cycling through the "
+                                  f"scatter / head / batch families (seed {a.eval_seed}), HBM budget "
+                                  f"{a.serve_budget_gb} GiB, 16 running, spot check every {a.spot_every}",
+                      "parallelism": "single GPU"},
+           "latency_p50_ms": float(np.percentile(lat, 50)), "latency_p99_ms": float(np.percentile(lat, 99)),
+           "latency_max_ms": float(lat.max()),
+           "surrogate_latency_p50_ms": float(np.percentile(surrogate_lat, 50)) if len(surrogate_lat) else None,
+           "surrogate_latency_p99_ms": float(np.percentile(surrogate_lat, 99)) if len(surrogate_lat) else None,
+           "answered_by": {k: kinds.count(k) for k in sorted(set(kinds))}, "fallbacks": st["fallbacks"],
+           "spot_check_errors": st["spot_checks"], "surrogate_disabled": st["disabled"],
+           "peak_admitted_bytes": st["server"].get("peak_admitted_bytes")}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -290,17 +361,21 @@ def main():
     ap.add_argument("--phase-policy", type=int, default=0, help="0 auto, 1 most lanes, 2 least waste")
     ap.add_argument("--phase-cost", type=str, default="", help="comma-separated relative phase costs (DECAY, SOURCE2, STEP, INTERACT, TALLY, CTAU, CPZ)")
     ap.add_argument("--voxel-format", type=int, default=0, help="0 auto (palette indices), 1 8-byte voxel words")
-    ap.add_argument("--workload", choices=["head", "torso", "batch", "surrogate", "unet"], default="head",
+    ap.add_argument("--workload", choices=["head", "torso", "batch", "surrogate", "unet", "serve"], default="head",
                     help="head: the 1e9-decay head phantom; torso: the 5e9-decay torso phantom with beta+ "
                          "branching, prompt gammas and dose tallies; batch: 48 varied phantoms in one launch; "
                          "surrogate: time to a 5%%-NRMSE scatter sinogram, Monte Carlo vs the learned surrogate; "
-                         "unet: the surrogate network alone")
+                         "unet: the surrogate network alone; serve: scatter estimates for Poisson-arriving requests")
     ap.add_argument("--eval-phantoms", type=int, default=6)
     ap.add_argument("--eval-seed", type=int, default=777)
     ap.add_argument("--eval-family", choices=sorted(EVAL_FAMILIES), default="scatter",
                     help="surrogate workload: the phantom family of the evaluation")
     ap.add_argument("--surrogate-weights", default="", help="U-Net parameters (.npy); default: the packaged ones")
     ap.add_argument("--batch-size", type=int, default=48)
+    ap.add_argument("--serve-requests", type=int, default=60)
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="serve workload: mean arrivals per second")
+    ap.add_argument("--serve-budget-gb", type=int, default=32)
+    ap.add_argument("--spot-every", type=int, default=20, help="serve workload: spot-check every k-th request")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-stride", type=int, default=0, help="CPU baseline voxel stride (0: 5 head, 50 torso)")
@@ -325,6 +400,8 @@ def main():
         return bench_surrogate(a)
     if a.workload == "unet":
         return bench_unet(a)
+    if a.workload == "serve":
+        return bench_serve(a)
     torso = a.workload == "torso"
     n_decays = a.decays or (5.0e9 if torso else 1.0e9)
     stride = a.cpu_stride or (50 if torso else 5)

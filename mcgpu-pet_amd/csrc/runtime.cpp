@@ -245,9 +245,19 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
 #define HK(x) CK(([&]() -> mcpet_status { HIPCHK(c, (x)); return MCPET_OK; })())
 
   HK(hipSetDevice(c->device));
-  hipDeviceProp_t prop;
-  HK(hipGetDeviceProperties(&prop, c->device));
-  c->num_cus = prop.multiProcessorCount;
+  {
+    // compute units per device, queried once per process (the property query is slow)
+    static std::mutex cu_mu;
+    static std::map<int, int> cus;
+    std::lock_guard<std::mutex> lock(cu_mu);
+    auto it = cus.find(c->device);
+    if (it == cus.end()) {
+      hipDeviceProp_t prop;
+      HK(hipGetDeviceProperties(&prop, c->device));
+      it = cus.emplace(c->device, prop.multiProcessorCount).first;
+    }
+    c->num_cus = it->second;
+  }
   HK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HK(hipEventCreate(&c->ev_k0));
   HK(hipEventCreate(&c->ev_k1));
@@ -279,7 +289,7 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
       }
     }
   }
-  // Voxels -> packed (density, material word) pairs; emitting voxels sorted by activity.
+  // Voxels -> packed (density, material word) pairs; the emitting voxels, in voxel order.
   std::vector<float2> hv((size_t)c->nvox);
   std::vector<int> act_idx;
   for (long long v = 0; v < c->nvox; v++) {
@@ -302,17 +312,25 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
   std::vector<uint16_t> pidx;
   if (cfg->voxel_format == 0) {
     std::unordered_map<uint64_t, int> seen;
+    uint64_t last_key = 0;
+    int last_idx = 0;
     pidx.resize((size_t)c->nvox);
     for (long long v = 0; v < c->nvox && palette.size() <= 65536; v++) {
       uint32_t d, m;
       std::memcpy(&d, &hv[(size_t)v].x, 4);
       std::memcpy(&m, &hv[(size_t)v].y, 4);
       const uint64_t key = ((uint64_t)m << 32) | d;
+      if (v > 0 && key == last_key) {   // runs of equal voxels (air, tissue interiors) skip the map
+        pidx[(size_t)v] = (uint16_t)last_idx;
+        continue;
+      }
       auto it = seen.find(key);
       if (it == seen.end()) {
         it = seen.emplace(key, (int)palette.size()).first;
         palette.push_back(hv[(size_t)v]);
       }
+      last_key = key;
+      last_idx = it->second;
       pidx[(size_t)v] = (uint16_t)it->second;
     }
     if (palette.size() > 65536) {
@@ -322,9 +340,28 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
   }
   c->P.vox_fmt = palette.empty() ? 0 : (palette.size() <= 256 ? 1 : 2);
   c->P.n_palette = (int)palette.size();
-  std::stable_sort(act_idx.begin(), act_idx.end(), [&](int a, int b) {
-    return ph->voxel_mat_dens[3 * (size_t)a + 2] > ph->voxel_mat_dens[3 * (size_t)b + 2];
-  });
+  // Device order of the emitting voxels: hottest first (long streams start early), by a
+  // counting sort on the top 12 bits of the activity's float representation (exponent and 3
+  // mantissa bits: classes within a factor 1.125), stable within a class. The order only
+  // balances the load: every stream draws the same random numbers in any order.
+  std::vector<int> act_vox(act_idx);   // voxel order
+  std::vector<size_t> src_of(act_idx.size());   // device position -> position in voxel order
+  {
+    std::vector<uint32_t> cls(act_idx.size());
+    std::vector<size_t> count(4097, 0);
+    for (size_t k = 0; k < act_idx.size(); k++) {
+      uint32_t b;
+      std::memcpy(&b, &ph->voxel_mat_dens[3 * (size_t)act_idx[k] + 2], 4);
+      cls[k] = 4095u - (b >> 20);   // activity > 0: larger bits = hotter; class 0 = hottest
+      count[cls[k] + 1]++;
+    }
+    for (size_t i = 1; i < count.size(); i++) count[i] += count[i - 1];
+    for (size_t k = 0; k < act_vox.size(); k++) {
+      const size_t dst = count[cls[k]]++;
+      act_idx[dst] = act_vox[k];
+      src_of[dst] = k;
+    }
+  }
   std::vector<float> act_val(act_idx.size());
   for (size_t i = 0; i < act_idx.size(); i++) act_val[i] = ph->voxel_mat_dens[3 * (size_t)act_idx[i] + 2];
   c->n_active = (long long)act_idx.size();
@@ -345,14 +382,12 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
       const double n = std::ceil((double)act * teff / (double)cfg->stream_decays);
       return n < 1.0 ? 1LL : (n > 1.0e9 ? 1000000000LL : (long long)n);
     };
-    std::vector<int> by_vox(act_idx);
-    std::sort(by_vox.begin(), by_vox.end());
-    std::unordered_map<int, long long> first_id;
-    first_id.reserve(by_vox.size() * 2);
+    // first stream id of every emitting voxel, counted in voxel order (act_vox)
+    std::vector<long long> first_id(act_vox.size());
     long long id = 0;
-    for (int v : by_vox) {
-      first_id[v] = id;
-      id += streams_of(ph->voxel_mat_dens[3 * (size_t)v + 2]);
+    for (size_t k = 0; k < act_vox.size(); k++) {
+      first_id[k] = id;
+      id += streams_of(ph->voxel_mat_dens[3 * (size_t)act_vox[k] + 2]);
     }
     balanced_streams = id;
     stream_start.resize(act_idx.size() + 1);
@@ -360,7 +395,7 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
     long long s0 = 0;
     for (size_t k = 0; k < act_idx.size(); k++) {
       stream_start[k] = s0;
-      stream_id[k] = first_id[act_idx[k]];
+      stream_id[k] = first_id[src_of[k]];
       s0 += streams_of(act_val[k]);
     }
     stream_start[act_idx.size()] = s0;
