@@ -300,7 +300,7 @@ def bench_serve(a):
     svc.drain()
     svc.close()
     svc = S.ScatterService(tab, est, budget_bytes=a.serve_budget_gb << 30, max_running=16,
-                           spot_check_every=a.spot_every)
+                           streams_per_step=a.serve_chunk, spot_check_every=a.spot_every)
     rng = np.random.default_rng(a.eval_seed)
     arrival = np.cumsum(rng.exponential(1.0 / a.serve_rate, n))
     sid_of = {}
@@ -336,7 +336,8 @@ def bench_serve(a):
            "dtype": "bf16 MFMA (fp32 accumulate) + f32 transport", "data": "synthetic",
            "config": {"workload": f"{n} requests, Poisson arrivals at {a.serve_rate:g}/s, phantoms cycling through the "
                                   f"scatter / head / batch families (seed {a.eval_seed}), HBM budget "
-                                  f"{a.serve_budget_gb} GiB, 16 running, spot check every {a.spot_every}",
+                                  f"{a.serve_budget_gb} GiB, 16 running, {a.serve_chunk} streams per request and step, spot "
+                                  f"check every {a.spot_every}",
                       "parallelism": "single GPU"},
            "latency_p50_ms": float(np.percentile(lat, 50)), "latency_p99_ms": float(np.percentile(lat, 99)),
            "latency_max_ms": float(lat.max()),
@@ -375,6 +376,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--serve-rate", type=float, default=20.0, help="serve workload: mean arrivals per second")
     ap.add_argument("--serve-budget-gb", type=int, default=32)
+    ap.add_argument("--serve-chunk", type=int, default=1 << 20,
+                    help="serve workload: decay streams per request and serving-loop step")
     ap.add_argument("--spot-every", type=int, default=20, help="serve workload: spot-check every k-th request")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
