@@ -481,11 +481,14 @@ def main():
                    "parallelism": "single GPU"},
         "decays_per_s": decays / elapsed, "photon_histories_per_s": 2 * decays / elapsed,
         "kernel_ms_avg": avg_ms,
-        "roofline": {"bound": "valu", "achieved": valu / 1e12, "peak": VALU_PEAK / 1e12, "unit": "Tlane-op/s",
-                     "frac": valu / VALU_PEAK, "traffic": traffic,
-                     "hbm": {"achieved": hbm / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": hbm / HBM_PEAK,
-                             "algorithmic_bytes_per_launch": byts / a.steps},
-                     "algorithmic_lane_ops_per_launch": ops / a.steps, "kernel": "k_transport (persistent)"},
+        # The transport uses no MFMA: its roofline is HBM, algorithmic bytes per launch over the
+        # kernel time. Its binding resource is VALU issue with dependent gathers; that figure
+        # (algorithmic lane operations over the non-packed VALU peak) is reported beside it.
+        "roofline": {"bound": "hbm", "achieved": hbm / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": hbm / HBM_PEAK, "traffic": traffic, "algorithmic_bytes_per_launch": byts / a.steps,
+                     "kernel": "k_transport_sm (persistent)",
+                     "valu": {"achieved": valu / 1e12, "peak": VALU_PEAK / 1e12, "unit": "Tlane-op/s",
+                              "frac": valu / VALU_PEAK, "algorithmic_lane_ops_per_launch": ops / a.steps}},
         "events_per_decay": {k: agg[k] / max(1, decays) for k in ("woodcock_steps", "material_changes", "compton",
                                                                    "rayleigh", "photoelectric", "coincidences",
                                                                    "rn_draws")},
