@@ -23,6 +23,7 @@
 #include "runtime_internal.h"
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 namespace {
@@ -267,15 +268,18 @@ __global__ __launch_bounds__(256) void k_conv(ConvK a)
 #pragma unroll
   for (int i = 0; i < MT; i++) red[wave][i][lane] = acc[i];
   __syncthreads();
-  if (wave != 0) return;
-#pragma unroll
-  for (int i = 0; i < MT; i++) {
+  // epilogue: wave w finishes the row blocks i = w, w + 4, ...
+  for (int i = wave; i < MT; i += 4) {
     const floatx4 s = red[0][i][lane] + red[1][i][lane] + red[2][i][lane] + red[3][i][lane];
+    const int row0 = m0 + 16 * i + kg * 4;   // this lane's four rows row0 .. row0 + 3
+    if (!nvalid) continue;
+    if (MODE == FWD || MODE == BDATA) {
+      float ov[4];
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-      const int row = m0 + 16 * i + kg * 4 + v;
-      if (row >= a.M || !nvalid) continue;
-      if (MODE == FWD || MODE == BDATA) {
+      for (int v = 0; v < 4; v++) {
+        const int row = row0 + v;
+        ov[v] = 0.0f;
+        if (row >= a.M) continue;
         const size_t idx = (size_t)row * a.N + n;
         float o;
         if (MODE == FWD) {
@@ -286,11 +290,30 @@ __global__ __launch_bounds__(256) void k_conv(ConvK a)
           if (a.mask && !(a.mask[idx] > 0.0f)) o = 0.0f;
         }
         a.C[idx] = o;
-        if (a.o1.p) a.o1.p[(size_t)n * a.o1.ld + a.o1.off + row] = to_bf16(o);
-        if (a.o2.p) a.o2.p[(size_t)n * a.o2.ld + a.o2.off + row] = to_bf16(o);
-      } else {
-        atomicAdd(&a.C[((size_t)row * a.Cb + bci) * a.T + bt], s[v]);
+        ov[v] = o;
       }
+      // channels-last copies: the four channels in one 8-byte store when they all exist
+      const ClOut* outs[2] = {&a.o1, &a.o2};
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        const ClOut& o = *outs[j];
+        if (!o.p) continue;
+        __bf16* dst = o.p + (size_t)n * o.ld + o.off + row0;
+        if (row0 + 3 < a.M && ((o.ld | o.off) & 3) == 0) {
+          bf16x4 q;
+#pragma unroll
+          for (int v = 0; v < 4; v++) q[v] = to_bf16(ov[v]);
+          *(bf16x4*)dst = q;
+        } else {
+#pragma unroll
+          for (int v = 0; v < 4; v++)
+            if (row0 + v < a.M) dst[v] = to_bf16(ov[v]);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        if (row0 + v < a.M) atomicAdd(&a.C[((size_t)(row0 + v) * a.Cb + bci) * a.T + bt], s[v]);
     }
   }
 }
