@@ -9,12 +9,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <deque>
 #include <map>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "host_internal.h"
@@ -262,16 +264,55 @@ extern "C" mcpet_status mcpet_server_step(mcpet_server* s, int32_t* running, int
   if (!s) return sfail(nullptr, MCPET_E_ARG, "mcpet_server_step: null server");
   const Clock::time_point t0 = Clock::now();
   if (hipSetDevice(s->device) != hipSuccess) return sfail(s, MCPET_E_HIP, "mcpet_server_step: hipSetDevice failed");
-  // 1. admission, in submission order, within the budget
-  while (!s->queue.empty() && (int)s->running.size() < s->max_running) {
-    Request& r = *s->req[s->queue.front()];
-    if (s->admitted + r.footprint > s->budget) break;
-    mcpet_status st = mcpet_create(&r.cfg, &r.ph, s->tab, &r.ctx);
-    if (st != MCPET_OK) return sfail(s, st, std::string("request ") + std::to_string(r.id) + ": " + mcpet_last_error(nullptr));
-    if (mcpet_ctx_device_bytes(r.ctx) > r.footprint) {
-      mcpet_destroy(r.ctx);
-      r.ctx = nullptr;
-      return sfail(s, MCPET_E_STATE, "mcpet_server_step: device footprint above its estimate");
+  // 1. admission, in submission order, within the budget. The admitted requests' contexts are
+  //    created concurrently (a context's creation is host work: voxel packing, stream layout,
+  //    uploads), then entered in submission order.
+  std::vector<int64_t> adm;
+  {
+    size_t used = s->admitted;
+    for (int64_t id : s->queue) {
+      if ((int)(s->running.size() + adm.size()) >= s->max_running) break;
+      const Request& r = *s->req[id];
+      if (used + r.footprint > s->budget) break;
+      used += r.footprint;
+      adm.push_back(id);
+    }
+  }
+  std::vector<mcpet_status> cst(adm.size(), MCPET_OK);
+  std::vector<std::string> cerr(adm.size());
+  {
+    std::atomic<size_t> next{0};
+    auto worker = [&]() {
+      for (size_t i = next++; i < adm.size(); i = next++) {
+        Request& r = *s->req[adm[i]];
+        cst[i] = mcpet_create(&r.cfg, &r.ph, s->tab, &r.ctx);
+        if (cst[i] != MCPET_OK) cerr[i] = mcpet_last_error(nullptr);
+      }
+    };
+    const size_t nt = std::min<size_t>(adm.size(), 8);
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < nt; t++) pool.emplace_back(worker);
+    if (nt > 0) worker();
+    for (auto& t : pool) t.join();
+  }
+  for (size_t i = 0; i < adm.size(); i++) {
+    Request& r = *s->req[adm[i]];
+    std::string why;
+    mcpet_status st = cst[i];
+    if (st != MCPET_OK) {
+      why = std::string("request ") + std::to_string(r.id) + ": " + cerr[i];
+    } else if (mcpet_ctx_device_bytes(r.ctx) > r.footprint) {
+      st = MCPET_E_STATE;
+      why = "mcpet_server_step: device footprint above its estimate";
+    }
+    if (st != MCPET_OK) {
+      // this request and the later ones stay queued; their contexts are released
+      for (size_t j = i; j < adm.size(); j++) {
+        Request& q = *s->req[adm[j]];
+        if (q.ctx) mcpet_destroy(q.ctx);
+        q.ctx = nullptr;
+      }
+      return sfail(s, st, why);
     }
     r.streams = mcpet_ctx_streams(r.ctx);
     r.next = 0;
