@@ -360,6 +360,7 @@ const char* mcpet_batch_last_error(const mcpet_batch* b);
  * next step. Streams are independent with fixed ids, so a request's tallies equal its
  * standalone mcpet_run bit for bit, whatever else it was batched with. */
 typedef struct mcpet_server mcpet_server;
+typedef struct mcpet_surrogate mcpet_surrogate;   /* the scatter network (below) */
 
 typedef enum mcpet_request_state {
   MCPET_REQ_QUEUED = 0, MCPET_REQ_RUNNING = 1, MCPET_REQ_DONE = 2, MCPET_REQ_REJECTED = 3, MCPET_REQ_UNKNOWN = 4
@@ -398,6 +399,15 @@ mcpet_status mcpet_server_submit_rebinned(mcpet_server* s, const mcpet_config* c
  * decays (may be NULL). The request is then forgotten. */
 mcpet_status mcpet_server_fetch_rebinned(mcpet_server* s, int64_t request_id, float* trues, float* scatter, float* act,
                                          float* att, int64_t* decays);
+/* A request answered on the device by a scatter network (mcpet_surrogate_estimate on the
+ * request's context when its run completes: rebinning, maps, normalisation, network, scaling).
+ * The network must outlive the request; the server's requests use it one at a time. */
+mcpet_status mcpet_server_submit_estimate(mcpet_server* s, const mcpet_config* cfg, const mcpet_phantom* ph, int32_t seed,
+                                          mcpet_surrogate* net, int64_t* request_id);
+/* estimate, scatter_counts: the network's grid (may be NULL); decays (may be NULL). The request
+ * is then forgotten. */
+mcpet_status mcpet_server_fetch_estimate(mcpet_server* s, int64_t request_id, float* estimate, float* scatter_counts,
+                                         int64_t* decays);
 mcpet_status mcpet_server_get_stats(mcpet_server* s, mcpet_server_stats* out);
 void mcpet_server_destroy(mcpet_server* s);
 const char* mcpet_server_last_error(const mcpet_server* s);
@@ -438,7 +448,6 @@ mcpet_status mcpet_batch_project_maps(mcpet_batch* b, int32_t k, int32_t D, int3
  * accumulation; parameters, activations and optimiser state are FP32. Parameter layout: per
  * layer W[Cout][Cin][k^3] then b[Cout], layers in the order above. Created with He-normal
  * weights from `seed`, zero biases and a zero output layer (the identity on channel 0). */
-typedef struct mcpet_surrogate mcpet_surrogate;
 mcpet_status mcpet_surrogate_create(int32_t D, int32_t H, int32_t W, int32_t in_channels, int32_t width, uint64_t seed,
                                     mcpet_surrogate** out);
 int64_t mcpet_surrogate_param_count(const mcpet_surrogate* s);

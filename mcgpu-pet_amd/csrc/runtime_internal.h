@@ -35,5 +35,7 @@ mcpet_status mcpet_ctx_decays(mcpet_ctx* c, long long* decays);
 // [3][D][H][W] = activity integrals, the same times float32(1 / their maximum), attenuation
 // factors; owned by the context, computed at the first call for this grid.
 mcpet_status mcpet_ctx_project_device(mcpet_ctx* c, int D, int H, int W, const float** d_maps);
+// The network's grid and input channels (unet.hip).
+void mcpet_surrogate_dims(const mcpet_surrogate* s, int* D, int* H, int* W, int* cin);
 
 #endif

@@ -45,6 +45,9 @@ struct Request {
   int grid[3] = {0, 0, 0};           // rebinned request: keep only these grids
   int maps = 0;                      // and the line integrals of its phantom on that grid
   std::vector<float> rb_trues, rb_scatter, rb_act, rb_att;
+  mcpet_surrogate* net = nullptr;    // estimate request: the network that answers it
+  std::vector<float> est;            // its scatter estimate per decay (rb_scatter: the counts)
+  int64_t est_decays = 0;
 };
 
 // Upper bound of the device memory mcpet_create allocates for (cfg, phantom, tables).
@@ -133,9 +136,10 @@ extern "C" mcpet_status mcpet_server_create(const mcpet_tables* tab, size_t budg
   return MCPET_OK;
 }
 
-// grid: NULL (a full request) or {D, H, W, maps} of a rebinned request.
+// grid: NULL (a full request) or {D, H, W, maps} of a rebinned request; net: the network that
+// answers an estimate request (rebinned on its grid).
 static mcpet_status submit(mcpet_server* s, const mcpet_config* cfg, const mcpet_phantom* ph, int32_t seed, const int* grid,
-                           int64_t* request_id)
+                           int64_t* request_id, mcpet_surrogate* net = nullptr)
 {
   if (!s || !cfg || !ph || !ph->voxel_mat_dens) return sfail(s, MCPET_E_ARG, "mcpet_server_submit: null argument");
   if (cfg->detector.tally_psf_sinogram == MCPET_TALLY_SINGLES)
@@ -150,6 +154,7 @@ static mcpet_status submit(mcpet_server* s, const mcpet_config* cfg, const mcpet
   r->ph.voxel_mat_dens = r->vmd.data();
   r->ph.owned = 0;
   r->seed = seed;
+  r->net = net;
   if (grid) {
     std::memcpy(r->grid, grid, sizeof r->grid);
     r->maps = grid[3];
@@ -158,7 +163,7 @@ static mcpet_status submit(mcpet_server* s, const mcpet_config* cfg, const mcpet
                    4 * (size_t)r->cfg.nsinos;   // the rebinning's plane lists and output
     if (r->maps)   // the device's (activity, mu) map, crystal-pair lists, the maps, parameter block
       r->footprint += 8 * nvox + 4 * (size_t)r->cfg.ncrystals * r->cfg.ncrystals + 4 * ((size_t)grid[1] * grid[2] + 1) +
-                      8 * n + 64 * 1024;
+                      12 * n + 4 + 64 * 1024;
   } else {
     r->footprint = footprint(r->cfg, r->ph, *s->tab);
   }
@@ -192,6 +197,18 @@ extern "C" mcpet_status mcpet_server_submit_rebinned(mcpet_server* s, const mcpe
   return submit(s, cfg, ph, seed, grid, request_id);
 }
 
+extern "C" mcpet_status mcpet_server_submit_estimate(mcpet_server* s, const mcpet_config* cfg, const mcpet_phantom* ph,
+                                                     int32_t seed, mcpet_surrogate* net, int64_t* request_id)
+{
+  if (!net) return sfail(s, MCPET_E_ARG, "mcpet_server_submit_estimate: null network");
+  int D = 0, H = 0, W = 0, cin = 0;
+  mcpet_surrogate_dims(net, &D, &H, &W, &cin);
+  if (cin != 2 && cin != 4) return sfail(s, MCPET_E_ARG, "mcpet_server_submit_estimate: the network must take 2 or 4 input channels");
+  if (W > 256 || D > 256) return sfail(s, MCPET_E_ARG, "mcpet_server_submit_estimate: network grid too large (D, W <= 256)");
+  const int grid[4] = {D, H, W, cin == 4 ? 1 : 0};
+  return submit(s, cfg, ph, seed, grid, request_id, net);
+}
+
 static void complete(mcpet_server* s, Request& r)
 {
   mcpet_destroy(r.ctx);
@@ -223,8 +240,25 @@ static mcpet_status finish_rebinned(mcpet_server* s, Request& r)
   return MCPET_OK;
 }
 
+// An estimate request: the network's fused device estimate on the request's context.
+static mcpet_status finish_estimate(mcpet_server* s, Request& r)
+{
+  const size_t n = (size_t)r.grid[0] * r.grid[1] * r.grid[2];
+  r.est.resize(n);
+  r.rb_scatter.resize(n);
+  mcpet_tallies& t = r.t;
+  std::memset(&t, 0, sizeof t);
+  mcpet_status st = mcpet_fetch(r.ctx, &t);   // counters only
+  if (st == MCPET_OK) st = mcpet_surrogate_estimate(r.net, r.ctx, r.est.data(), r.rb_scatter.data(), &r.est_decays);
+  if (st != MCPET_OK)
+    return sfail(s, st, std::string("request ") + std::to_string(r.id) + ": " + mcpet_surrogate_last_error(r.net));
+  complete(s, r);
+  return MCPET_OK;
+}
+
 static mcpet_status finish(mcpet_server* s, Request& r)
 {
+  if (r.net) return finish_estimate(s, r);
   if (r.grid[0] > 0) return finish_rebinned(s, r);
   const size_t nvox = (size_t)r.ph.nx * r.ph.ny * r.ph.nz;
   r.trues.resize((size_t)r.cfg.nbins);
@@ -412,7 +446,7 @@ extern "C" mcpet_status mcpet_server_fetch_rebinned(mcpet_server* s, int64_t id,
 {
   if (!s) return sfail(s, MCPET_E_ARG, "mcpet_server_fetch_rebinned: null server");
   auto it = s->req.find(id);
-  if (it == s->req.end() || it->second->state != MCPET_REQ_DONE || it->second->grid[0] == 0)
+  if (it == s->req.end() || it->second->state != MCPET_REQ_DONE || it->second->grid[0] == 0 || it->second->net)
     return sfail(s, MCPET_E_STATE, "mcpet_server_fetch_rebinned: request unknown, not finished or not rebinned");
   Request& r = *it->second;
   if ((act || att) && !r.maps)
@@ -422,6 +456,21 @@ extern "C" mcpet_status mcpet_server_fetch_rebinned(mcpet_server* s, int64_t id,
   if (trues) std::memcpy(trues, r.rb_trues.data(), 4 * r.rb_trues.size());
   if (scatter) std::memcpy(scatter, r.rb_scatter.data(), 4 * r.rb_scatter.size());
   if (decays) *decays = (int64_t)(r.t.total_histories / 2);
+  s->req.erase(it);
+  return MCPET_OK;
+}
+
+extern "C" mcpet_status mcpet_server_fetch_estimate(mcpet_server* s, int64_t id, float* estimate, float* scatter_counts,
+                                                    int64_t* decays)
+{
+  if (!s) return sfail(s, MCPET_E_ARG, "mcpet_server_fetch_estimate: null server");
+  auto it = s->req.find(id);
+  if (it == s->req.end() || it->second->state != MCPET_REQ_DONE || !it->second->net)
+    return sfail(s, MCPET_E_STATE, "mcpet_server_fetch_estimate: request unknown, not finished or not an estimate");
+  Request& r = *it->second;
+  if (estimate) std::memcpy(estimate, r.est.data(), 4 * r.est.size());
+  if (scatter_counts) std::memcpy(scatter_counts, r.rb_scatter.data(), 4 * r.rb_scatter.size());
+  if (decays) *decays = r.est_decays;
   s->req.erase(it);
   return MCPET_OK;
 }

@@ -926,6 +926,14 @@ extern "C" mcpet_status mcpet_surrogate_forward(mcpet_surrogate* s, const float*
 // trues onto the network's grid, normalisation (counts times float(voxels / total trues), the
 // total exact in double), the network, and scatter per decay = max(output, 0) times
 // float(total trues / (voxels * decays)).
+void mcpet_surrogate_dims(const mcpet_surrogate* s, int* D, int* H, int* W, int* cin)
+{
+  *D = s->D;
+  *H = s->H;
+  *W = s->W;
+  *cin = s->cin;
+}
+
 extern "C" mcpet_status mcpet_surrogate_estimate(mcpet_surrogate* s, mcpet_ctx* ctx, float* estimate, float* scatter_counts,
                                                  int64_t* decays)
 {

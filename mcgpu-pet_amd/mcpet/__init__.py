@@ -204,6 +204,9 @@ def lib():
                                                C.c_int32, C.c_int32, C.POINTER(C.c_int64)]
     L.mcpet_server_fetch_rebinned.argtypes = [C.c_void_p, C.c_int64, P(C.c_float), P(C.c_float), P(C.c_float),
                                               P(C.c_float), C.POINTER(C.c_int64)]
+    L.mcpet_server_submit_estimate.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
+                                               C.POINTER(C.c_int64)]
+    L.mcpet_server_fetch_estimate.argtypes = [C.c_void_p, C.c_int64, P(C.c_float), P(C.c_float), C.POINTER(C.c_int64)]
     L.mcpet_server_last_error.argtypes = [C.c_void_p]
     L.mcpet_server_last_error.restype = C.c_char_p
     L.mcpet_struct_sizes.restype = C.c_int32
@@ -593,6 +596,27 @@ class Server:
                                                       _fptr(maps[1]) if maps else None, C.byref(d)))
         self._meta.pop(rid)
         return t, s, d.value, maps
+
+    def submit_estimate(self, cfg, phantom, net, seed=0):
+        """A request answered on the device by the scatter network `net` (a Surrogate) when its
+        run completes (mcpet_server_submit_estimate); net must outlive the request."""
+        rid = C.c_int64()
+        st = lib().mcpet_server_submit_estimate(self.s, C.byref(cfg), C.byref(phantom.ph), int(seed), net.s, C.byref(rid))
+        self._meta[rid.value] = {"kind": "estimate", "grid": tuple(net.grid)}
+        self._check(st)
+        return rid.value
+
+    def fetch_estimate(self, rid):
+        """(scatter estimate per decay, rebinned scatter counts, decays) of a finished estimate request."""
+        m = self._meta.get(rid)
+        if m is None or m["kind"] != "estimate":    # the library reports the error
+            self._check(lib().mcpet_server_fetch_estimate(self.s, int(rid), None, None, None))
+        e = np.zeros(m["grid"], np.float32)
+        c = np.zeros(m["grid"], np.float32)
+        d = C.c_int64()
+        self._check(lib().mcpet_server_fetch_estimate(self.s, int(rid), _fptr(e), _fptr(c), C.byref(d)))
+        self._meta.pop(rid)
+        return e, c, d.value
 
     def stats(self):
         x = ServerStats()

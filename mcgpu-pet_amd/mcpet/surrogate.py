@@ -169,10 +169,11 @@ class ScatterEstimator:
 
 class ScatterService:
     """Scatter estimates for many phantoms through the serving loop (mcpet_server_*): each
-    request's low-count run is a rebinned server request, batched with the other requests'
-    runs; when it completes the U-Net estimate is checked (consistency z-score) and, if it
-    fails, the request is resubmitted as a Monte Carlo run of N5 decays (N5 from the
-    prediction) whose scatter is the answer.
+    request's low-count run is an estimate request (mcpet_server_submit_estimate), batched with
+    the other requests' runs and answered on the device by the U-Net when it completes; the
+    estimate is checked (consistency z-score) and, if it fails, the request is resubmitted as a
+    rebinned Monte Carlo request of N5 decays (N5 from the prediction) whose scatter is the
+    answer.
 
     Spot checks (spot_check_every = k > 0): every k-th request's estimate is also compared with
     a Monte Carlo run of N5 decays; the surrogate's own error is estimated as
@@ -203,8 +204,7 @@ class ScatterService:
     def submit(self, phantom, seed=1):
         sid = self.next_id
         self.next_id += 1
-        rid = self.server.submit_rebinned(config(self.names, N_LOW), phantom, self.est.grid, seed=seed,
-                                          maps=self.est.in_channels == 4)
+        rid = self.server.submit_estimate(config(self.names, N_LOW), phantom, self.est.net, seed=seed)
         spot = self.spot_every > 0 and sid % self.spot_every == 0
         self.jobs[sid] = {"phantom": phantom, "seed": seed, "stage": "surrogate", "t0": time.perf_counter(),
                           "result": None, "info": None, "spot": spot}
@@ -220,9 +220,8 @@ class ScatterService:
     def _complete(self, rid):
         sid = self.by_rid.pop(rid)
         job = self.jobs[sid]
-        t, s, decays, maps = self.server.fetch_rebinned(rid)
         if job["stage"] == "surrogate":
-            est = self.est.predict(s, t, decays, maps)
+            est, s, decays = self.server.fetch_estimate(rid)
             z = consistency_z(est, s, decays)
             job["info"] = {"decays": decays, "z": z, "consistent": bool(z <= self.est.z_max)}
             n5 = decays_for(est) if est.sum() > 0 else 1e9
@@ -236,6 +235,7 @@ class ScatterService:
                 self._monte_carlo(sid, job, n5, "spot_check")
                 return
         else:
+            _, s, decays, _ = self.server.fetch_rebinned(rid)
             mc = s.astype(np.float64) / decays
             job["info"]["mc_decays"] = decays
             if job["stage"] == "spot_check":

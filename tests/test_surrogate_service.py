@@ -33,6 +33,38 @@ def test_server_rebinned_request_equals_simulation_rebinned():
 
 
 @pytest.mark.gpu
+def test_server_estimate_request_equals_the_direct_estimate():
+    """An estimate request (run in chunks beside other requests, answered by the network on
+    the device at completion) equals the fused estimate of a Simulation run with the same
+    seed, bit for bit; its maps match a rebinned request's maps."""
+    phs = _phantoms(2, 88)
+    names = S._materials()
+    tab = mcpet.load_materials(names, np.max([p.density_max() for p in phs], axis=0))
+    est = S.ScatterEstimator()
+    cfg = S.config(names, S.N_LOW)
+    with mcpet.Simulation(cfg, phs[0], tab) as sim:
+        sim.run(seed=5)
+        e0, c0, d0 = est.net.estimate(sim)
+        maps0 = sim.project(S.GRID)
+    with mcpet.Server(tab, 8 << 30, streams_per_step=1 << 18) as srv:
+        a = srv.submit_estimate(cfg, phs[0], est.net, seed=5)
+        b = srv.submit_rebinned(cfg, phs[1], S.GRID, seed=6, maps=True)
+        c = srv.submit_rebinned(cfg, phs[0], S.GRID, seed=7, maps=True)
+        srv.drain()
+        with pytest.raises(mcpet.McpetError):
+            srv.fetch_rebinned(a)
+        e, cnt, d = srv.fetch_estimate(a)
+        srv.fetch_rebinned(b)
+        _, _, _, maps = srv.fetch_rebinned(c)
+    est.close()
+    assert d == d0 and np.array_equal(cnt, c0) and np.array_equal(e, e0) and e.sum() > 0
+    assert np.array_equal(maps[0], maps0[0]) and np.array_equal(maps[1], maps0[1])
+    for p in phs:
+        p.close()
+    tab.close()
+
+
+@pytest.mark.gpu
 def test_fused_device_estimate_equals_the_host_path():
     """mcpet_surrogate_estimate (rebinning, maps, normalisation, network and scaling on the
     device) equals predict() on the rebinned tallies and maps bit for bit, with the same counts
