@@ -65,6 +65,18 @@ def test_water_cube_bit_exact(kernel):
     assert_tallies_equal(g, o)
 
 
+def test_water_cube_full_workload_bit_exact():
+    """The small workload at its full size: the 9 x 9 x 9 water cube (1 cm voxels, three hot
+    voxels at 100x), 1e7 decays, sinogram, emission images, spectrum and material dose,
+    bit-exact against the oracle."""
+    _say("water cube 9x9x9, 1e7 decays (the small workload's full size), tpb 32, persistent kernel")
+    sc = water_cube_scenario(n_decays=1e7, tpb=32, material_dose=True)
+    g = sc.run_gpu(kernel=mcpet.KERNEL_PERSISTENT)
+    o = sc.run_oracle("fmath", nthreads=16)
+    assert o["trues"].sum() > 100000 and o["counters"]["decays"] > 9.9e6
+    assert_tallies_equal(g, o)
+
+
 def test_sample_simulation_bit_exact():
     _say("reference sample .in (R 9.05 cm), 2 s acquisition, PSF + sinogram + material dose")
     sc = sample_scenario(acq_s=2.0)
@@ -151,6 +163,25 @@ def test_head_full_size_invariants():
     for key in ("coincidences", "compton", "woodcock_steps"):
         rg, ro = st[key] / decays, o["counters"][key] / od
         assert abs(rg - ro) / ro < 0.05, f"{key}/decay gpu {rg:.4f} oracle-sample {ro:.4f}"
+
+
+def test_head_full_shape_slice_bit_exact():
+    """The head workload's phantom and scanner at their full size (128 x 128 x 96 at 2 mm, 8
+    tissues, 168 x 168 x 1293 sinogram, 32 streams per emitting voxel) over a 10 ms slice of
+    its acquisition (1e6 decays at the workload's rate), bit-exact against the oracle."""
+    from mcpet import phantoms
+    from conftest import MATERIALS
+    _say("head 128x128x96 full shape, 10 ms slice (1e6 decays), tpb 32")
+    mat, dens, act, vs = phantoms.head()                 # 1e9 decays in 10 s
+    names = phantoms.head_materials(MATERIALS)
+    cfg = phantoms.make_config(seed=4711, threads_per_block=32, acquisition_time_s=0.01, nrad=168, materials=names,
+                               material_dose=True)
+    from harness import Scenario
+    sc = Scenario(cfg, mat, dens, act, vs, names)
+    g = sc.run_gpu(kernel=mcpet.KERNEL_PERSISTENT)
+    o = sc.run_oracle("fmath", nthreads=16)
+    assert abs(o["counters"]["decays"] - 1e6) < 6e3 and o["trues"].sum() > 20000
+    assert_tallies_equal(g, o)
 
 
 def test_production_kernel_equals_instrumented_and_oracle():
