@@ -69,7 +69,7 @@ def bench_batch(a):
     32-96 voxels at 2 mm, 1e6-1e8 photon histories each, 10 s), sinogram 168x168x1293 each.
     Rates of three ways to run it: the plain kernel launched per phantom, the persistent
     kernel launched per phantom, and the batched persistent kernel (one launch for all
-    phantoms, replayed as a HIP graph with the autotuned launch shape)."""
+    phantoms, replayed as a HIP graph with the autotuned launch shape and phase policy)."""
     import mcpet
     from mcpet import phantoms
     specs = phantoms.random_phantoms(a.batch_size)

@@ -328,6 +328,7 @@ typedef struct mcpet_batch mcpet_batch;
 
 typedef struct mcpet_tune_result {
   int32_t block_size, blocks_per_cu;
+  int32_t phase_policy;          /* 1 most lanes, 2 least waste (mcpet_run_params)        */
   double ms;                     /* batch kernel time (HIP events)                        */
   double decays_per_ms;
 } mcpet_tune_result;

@@ -1253,7 +1253,7 @@ struct mcpet_batch {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   KParams* d_params = nullptr;
   KParams* h_params = nullptr;   // pinned, n entries
-  int tuned_bs = 0, tuned_per_cu = 0;
+  int tuned_bs = 0, tuned_per_cu = 0, tuned_policy = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int graph_key[3] = {-1, -1, -1};  // block size, blocks, count_events of the captured launch
@@ -1350,7 +1350,8 @@ static mcpet_status batch_run(mcpet_batch* b, const mcpet_run_params* rp, double
     if (!seed_usable(P.seed)) return bset_err(b, MCPET_E_ARG, seed_msg(P.seed));
     if (p.acquisition_time_ps != 0) P.acq_ps = p.acquisition_time_ps;
     if (stream_fraction < 1.0) P.n_streams = (long long)std::ceil((double)P.n_streams * stream_fraction);
-    P.sm_policy = (p.phase_policy == 2) ? 1 : 0;
+    const int policy = p.phase_policy > 0 ? p.phase_policy : (b->tuned_policy > 0 ? b->tuned_policy : 1);
+    P.sm_policy = (policy == 2) ? 1 : 0;
     static const int default_cost[MCPET_NPHASES] = {4, 6, 1, 3, 6, 3, 2, 4, 4, 1};
     for (int i = 0; i < MCPET_NPHASES; i++) P.sm_cost[i] = p.phase_cost[i] > 0 ? p.phase_cost[i] : default_cost[i];
     P.lds_spectrum = 0;   // per phantom: global atomics
@@ -1459,15 +1460,18 @@ extern "C" mcpet_status mcpet_batch_autotune(mcpet_batch* b, double fraction, mc
                                              int32_t* n_results)
 {
   if (!b || !(fraction > 0.0 && fraction <= 1.0)) return bset_err(b, MCPET_E_ARG, "mcpet_batch_autotune: bad arguments");
+  // launch shape x phase-selection policy (the tallies do not depend on either)
   static const int shapes[][2] = {{256, 3}, {256, 4}, {256, 6}, {256, 8}, {128, 6}, {128, 8}, {128, 12}, {64, 12}, {64, 16}};
   const int ns = (int)(sizeof shapes / sizeof shapes[0]);
   double best = -1.0;
-  int best_i = 0, written = 0;
+  int best_i = 0, best_pol = 1, written = 0;
   mcpet_run_params p{};
   p.zero_tallies = 1;
-  for (int i = 0; i < ns; i++) {
+  for (int t = 0; t < 2 * ns; t++) {
+    const int i = t % ns, pol = 1 + t / ns;
     p.block_size = shapes[i][0];
     p.blocks_per_cu = shapes[i][1];
+    p.phase_policy = pol;
     mcpet_status st = batch_run(b, &p, fraction);
     if (st == MCPET_OK) st = mcpet_batch_sync(b);
     if (st != MCPET_OK) return st;
@@ -1480,14 +1484,16 @@ extern "C" mcpet_status mcpet_batch_autotune(mcpet_batch* b, double fraction, mc
       decays += d;
     }
     const double rate = ms > 0.0f ? (double)decays / ms : 0.0;
-    if (results && written < max_results) results[written++] = {shapes[i][0], shapes[i][1], (double)ms, rate};
+    if (results && written < max_results) results[written++] = {shapes[i][0], shapes[i][1], pol, (double)ms, rate};
     if (rate > best) {
       best = rate;
       best_i = i;
+      best_pol = pol;
     }
   }
   b->tuned_bs = shapes[best_i][0];
   b->tuned_per_cu = shapes[best_i][1];
+  b->tuned_policy = best_pol;
   if (n_results) *n_results = written;
   return MCPET_OK;
 }

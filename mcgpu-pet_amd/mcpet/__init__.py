@@ -118,7 +118,8 @@ class RunParams(C.Structure):
 
 
 class TuneResult(C.Structure):
-    _fields_ = [("block_size", C.c_int32), ("blocks_per_cu", C.c_int32), ("ms", C.c_double), ("decays_per_ms", C.c_double)]
+    _fields_ = [("block_size", C.c_int32), ("blocks_per_cu", C.c_int32), ("phase_policy", C.c_int32), ("ms", C.c_double),
+                ("decays_per_ms", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -692,8 +693,8 @@ class Batch:
         res = (TuneResult * 32)()
         n = C.c_int32()
         self._check(lib().mcpet_batch_autotune(self.b, float(fraction), res, 32, C.byref(n)))
-        return [{"block_size": r.block_size, "blocks_per_cu": r.blocks_per_cu, "ms": r.ms, "decays_per_ms": r.decays_per_ms}
-                for r in res[:n.value]]
+        return [{"block_size": r.block_size, "blocks_per_cu": r.blocks_per_cu, "phase_policy": r.phase_policy, "ms": r.ms,
+                 "decays_per_ms": r.decays_per_ms} for r in res[:n.value]]
 
     def close(self):
         if self.b:

@@ -80,6 +80,7 @@ def test_batch_autotune_keeps_results():
         before = [b.fetch(k, psf=False) for k in range(len(phs))]
         res = b.autotune(0.1)
         assert len(res) >= 4 and all(r["decays_per_ms"] > 0 for r in res)
+        assert {r["phase_policy"] for r in res} == {1, 2}      # both phase-selection policies tried
         b.run(seed=99)
         for k in range(len(phs)):
             _same(b.fetch(k, psf=False), before[k])
