@@ -623,6 +623,19 @@ enum StepOutcome { STEP_ESCAPED = 0, STEP_VIRTUAL = 1, STEP_REAL = 2 };
 // Palette of (density, material) words for u8-indexed phantoms, filled by each
 // workgroup's prologue.
 __shared__ float2 k_palette[256];
+// The total-cross-section row (a, b) of every material slot at the annihilation energy's bin,
+// copied to LDS per workgroup: unscattered photons (most Woodcock steps) reload (a, b) from
+// here on a table change instead of from global memory. The same values as P.tot's row.
+__shared__ float2 k_tot511[K_MAXMAT];
+__shared__ int k_idx511;
+
+// Fill k_tot511 / k_idx511 (all lanes of the workgroup; a barrier must follow).
+__device__ __forceinline__ void load_tot511(const KParams& P)
+{
+  const int idx = f2i_rd((K_E511 - P.e0) * P.ide);
+  if (threadIdx.x < K_MAXMAT) k_tot511[threadIdx.x] = P.tot[idx * K_MAXMAT + threadIdx.x];
+  if (threadIdx.x == 0) k_idx511 = idx;
+}
 
 // The (density, material) word of voxel absvox in the device layout chosen at set-up.
 __device__ __forceinline__ float2 voxel_word(const KParams& P, int absvox)
@@ -650,7 +663,8 @@ __device__ __forceinline__ int woodcock_step(const KParams& P, Photon& ph, Rng& 
   const int w = __float_as_int(md.y);
   ph.mat0 = w & 0xff;
   if (w != ph.mat_old) {
-    if (ph.mat_old < 0 || (w >> 8) != (ph.mat_old >> 8)) ph.tab = P.tot[ph.index * K_MAXMAT + ph.mat0];
+    if (ph.mat_old < 0 || (w >> 8) != (ph.mat_old >> 8))
+      ph.tab = ph.index == k_idx511 ? k_tot511[ph.mat0] : P.tot[ph.index * K_MAXMAT + ph.mat0];
     ph.mat_old = w;
     C.inc(KC_MATCHANGE);
   }
@@ -935,6 +949,7 @@ __device__ __forceinline__ void stream_finish(const KParams& P, const Stream& st
     if (threadIdx.x < KC_N) s_cnt[threadIdx.x] = 0u;                                             \
     if (P.vox_fmt == 1 && P.lds_palette)                                                         \
       for (int i = threadIdx.x; i < P.n_palette; i += blockDim.x) k_palette[i] = P.palette[i];   \
+    load_tot511(P);                                                                              \
     if (P.lds_spectrum)                                                                          \
       for (int i = threadIdx.x; i < P.ne; i += blockDim.x) s_spec[i] = 0;                        \
   }                                                                                              \
@@ -1362,6 +1377,7 @@ __global__ __launch_bounds__(K_BLOCK) K_OCCUPANCY void k_transport_batch(const K
     for (int i = threadIdx.x; i < (int)(sizeof(mcpet_compton_table) / 4); i += blockDim.x) dst[i] = src[i];
     for (int i = threadIdx.x; i < (K_BLOCK / 64) * 2 * K_MAXMAT; i += blockDim.x) (&s_matw[0][0])[i] = 0ull;
     for (int i = threadIdx.x; i < (K_BLOCK / 64) * KC_N; i += blockDim.x) (&s_cntw[0][0])[i] = 0u;
+    load_tot511(Ps[0]);   // the phantoms of a batch share their interaction tables
   }
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
