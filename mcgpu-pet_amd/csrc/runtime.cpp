@@ -90,6 +90,7 @@ struct mcpet_ctx {
   KParams* d_pj_params = nullptr;
   float* d_pj_out = nullptr;      // [3][D][H][W]: activity integral, the same / its maximum, attenuation factor; + 1 word
   int pj_dims[3] = {0, 0, 0};
+  size_t pj_alloc = 0;            // grid size of d_pj_out's allocation
   std::string err;
 };
 
@@ -1000,13 +1001,18 @@ mcpet_status mcpet_ctx_project_device(mcpet_ctx* c, int D, int H, int W, const f
   LorLists l;
   mcpet_status e = lor_lists(c, H, W, &l);
   if (e != MCPET_OK) return e;
-  if (c->d_pj_out) (void)hipFree(c->d_pj_out);
+  if (c->d_pj_out) {   // maps of another grid
+    (void)hipFree(c->d_pj_out);
+    c->dev_bytes -= sizeof(float) * (3 * (size_t)c->pj_alloc + 1);
+  }
   c->d_pj_out = nullptr;
   c->pj_dims[0] = c->pj_dims[1] = c->pj_dims[2] = 0;
   const size_t n = (size_t)D * H * W;
+  c->pj_alloc = 0;
   if (!c->d_pj_params) e = dalloc(c, &c->d_pj_params, 1);
   if (e == MCPET_OK) e = dalloc(c, &c->d_pj_out, 3 * n + 1);   // + the maximum's word
   if (e != MCPET_OK) return e;
+  c->pj_alloc = n;
   // the geometry, materials and voxels of the create-time parameter block (runs do not change them)
   HIPCHK(c, hipMemcpy(c->d_pj_params, &c->P, sizeof(KParams), hipMemcpyHostToDevice));
   // The (activity, mu) map is needed only while the projection runs: one per device, grown to

@@ -249,7 +249,8 @@ static mcpet_status finish_estimate(mcpet_server* s, Request& r)
   mcpet_tallies& t = r.t;
   std::memset(&t, 0, sizeof t);
   mcpet_status st = mcpet_fetch(r.ctx, &t);   // counters only
-  if (st == MCPET_OK) st = mcpet_surrogate_estimate(r.net, r.ctx, r.est.data(), r.rb_scatter.data(), &r.est_decays);
+  if (st != MCPET_OK) return sfail(s, st, std::string("request ") + std::to_string(r.id) + ": " + mcpet_last_error(r.ctx));
+  st = mcpet_surrogate_estimate(r.net, r.ctx, r.est.data(), r.rb_scatter.data(), &r.est_decays);
   if (st != MCPET_OK)
     return sfail(s, st, std::string("request ") + std::to_string(r.id) + ": " + mcpet_surrogate_last_error(r.net));
   complete(s, r);
