@@ -456,6 +456,9 @@ mcpet_status mcpet_surrogate_get_params(mcpet_surrogate* s, float* out);
 mcpet_status mcpet_surrogate_set_params(mcpet_surrogate* s, const float* in);
 /* x: batch x in_channels x D x H x W (host); y: batch x D x H x W (host). gflop: forward GFLOP per sample. */
 mcpet_status mcpet_surrogate_forward(mcpet_surrogate* s, const float* x, int32_t batch, float* y, double* gflop);
+/* Device time (ms, HIP events on the network's stream) of the network proper in the last
+ * mcpet_surrogate_forward call (its last sample; no host copies). */
+mcpet_status mcpet_surrogate_forward_ms(mcpet_surrogate* s, double* ms);
 /* One estimate from the last run of ctx (a 2- or 4-input-channel network; same device), all on
  * the device: its scatter and trues rebinned onto the network's grid (mcpet_rebin_sinogram),
  * normalised (counts times float(voxels / total trues), the total summed exactly in double),

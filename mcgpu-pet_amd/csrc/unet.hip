@@ -565,6 +565,8 @@ struct mcpet_surrogate {
   long long t_adam = 0;
   int device = 0;
   hipStream_t st = nullptr;
+  hipEvent_t ev_f0 = nullptr, ev_f1 = nullptr;   // around the last forward of mcpet_surrogate_forward
+  bool fwd_timed = false;
   std::vector<void*> allocs;
   std::string err;
   double flops_fwd = 0.0;
@@ -821,6 +823,8 @@ extern "C" mcpet_status mcpet_surrogate_create(int32_t D, int32_t H, int32_t W, 
   s->vr = {1, D, H, W};
   hipError_t e = hipGetDevice(&s->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&s->ev_f0);
+  if (e == hipSuccess) e = hipEventCreate(&s->ev_f1);
   struct A {
     float** p;
     size_t n;
@@ -911,14 +915,30 @@ extern "C" mcpet_status mcpet_surrogate_forward(mcpet_surrogate* s, const float*
   const size_t nx = s->vx.size(), nv = s->vr.size();
   for (int b = 0; b < batch; b++) {
     SHIP(s, hipMemcpyAsync(s->x0, x + b * nx, sizeof(float) * nx, hipMemcpyHostToDevice, s->st));
+    const bool last = b == batch - 1;
+    if (last) SHIP(s, hipEventRecord(s->ev_f0, s->st));
     SHIP(s, forward(s));
     hipLaunchKernelGGL(k_output, dim3(blocks_for(nv)), dim3(256), 0, s->st, s->x0, s->r, s->out, (const float*)nullptr,
                        (float*)nullptr, 0.0f, (float*)nullptr, (int)nv, kNoCl);
     SHIP(s, hipGetLastError());
+    if (last) SHIP(s, hipEventRecord(s->ev_f1, s->st));
     SHIP(s, hipMemcpyAsync(y + b * nv, s->out, sizeof(float) * nv, hipMemcpyDeviceToHost, s->st));
   }
   SHIP(s, hipStreamSynchronize(s->st));
+  s->fwd_timed = true;
   if (gflop) *gflop = s->flops_fwd * 1e-9;
+  return MCPET_OK;
+}
+
+// Device time of the network in the last mcpet_surrogate_forward (its last sample: every kernel
+// from the input conversion to the output, HIP events on the network's stream).
+extern "C" mcpet_status mcpet_surrogate_forward_ms(mcpet_surrogate* s, double* ms)
+{
+  if (!s || !ms) return serr(s, MCPET_E_ARG, "mcpet_surrogate_forward_ms: null argument");
+  if (!s->fwd_timed) return serr(s, MCPET_E_STATE, "mcpet_surrogate_forward_ms: no forward yet");
+  float t = 0.0f;
+  SHIP(s, hipEventElapsedTime(&t, s->ev_f0, s->ev_f1));
+  *ms = t;
   return MCPET_OK;
 }
 
@@ -1025,6 +1045,8 @@ extern "C" void mcpet_surrogate_destroy(mcpet_surrogate* s)
   if (!s) return;
   if (s->st) (void)hipStreamSynchronize(s->st);
   for (void* p : s->allocs) (void)hipFree(p);
+  if (s->ev_f0) (void)hipEventDestroy(s->ev_f0);
+  if (s->ev_f1) (void)hipEventDestroy(s->ev_f1);
   if (s->st) (void)hipStreamDestroy(s->st);
   delete s;
 }

@@ -221,6 +221,7 @@ def lib():
     L.mcpet_surrogate_create.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64,
                                          P(C.c_void_p)]
     L.mcpet_surrogate_param_count.argtypes = [C.c_void_p]
+    L.mcpet_surrogate_forward_ms.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     L.mcpet_surrogate_param_count.restype = C.c_int64
     L.mcpet_surrogate_get_params.argtypes = [C.c_void_p, P(C.c_float)]
     L.mcpet_surrogate_set_params.argtypes = [C.c_void_p, P(C.c_float)]
@@ -843,6 +844,12 @@ class Surrogate:
         self._check(lib().mcpet_surrogate_forward(self.s, _fptr(x), x.shape[0], _fptr(y), C.byref(g)))
         self.gflop = g.value
         return y
+
+    def forward_ms(self):
+        """Device time (ms) of the network in the last forward() call (its last sample)."""
+        ms = C.c_double()
+        self._check(lib().mcpet_surrogate_forward_ms(self.s, C.byref(ms)))
+        return ms.value
 
     def estimate(self, sim):
         """Fused device estimate from the last run of Simulation `sim` (mcpet_surrogate_estimate):

@@ -71,4 +71,5 @@ def test_surrogate_and_rebinning_entry_points_reject_bad_arguments_without_the_d
     assert L.mcpet_project_maps(None, 4, 4, 4, None, None) == 1                       # no context
     assert L.mcpet_server_submit_estimate(None, None, None, 0, None, None) == 1        # no network
     assert L.mcpet_server_fetch_estimate(None, 1, None, None, None) == 1
+    assert L.mcpet_surrogate_forward_ms(None, None) == 1
     assert L.mcpet_batch_project_maps(None, 0, 4, 4, 4, None, None) == 1

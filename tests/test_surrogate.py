@@ -203,7 +203,12 @@ def test_device_unet_forward_and_gradients_match_bf16_reference(grid, cin, c, ba
             p[sb] = rng.standard_normal(sb.stop - sb.start) * 0.05
         p[layers[-1][0]] = rng.standard_normal(layers[-1][0].stop - layers[-1][0].start) * 0.3
         s.set_params(p.astype(np.float32))
+        with mcpet.Surrogate(grid, in_channels=cin, width=c, seed=1) as fresh:
+            with pytest.raises(mcpet.McpetError):
+                fresh.forward_ms()              # no forward yet
         y = s.forward(x)
+        ms = s.forward_ms()                     # HIP events around the last sample's network
+        assert 0.0 < ms < 1000.0
         loss, g = s.gradients(x, t)
         lr, gr, yr = unet_reference(p.astype(np.float32).astype(np.float64), x.astype(np.float64), t.astype(np.float64),
                                     cin, c, "bf16")
