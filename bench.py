@@ -8,7 +8,8 @@ cylindrical scanner R = 40 cm, H = 25 cm, 12% energy resolution, 350-600 keV win
 resident on the device before timing; every step clears the tallies and simulates all
 decays with a new seed.
 
-Prints one JSON line (see DESIGN.md for the field definitions).
+Prints one JSON line: metric, value, unit, timing, config, a roofline object for the dominant
+kernel and (head and torso workloads) the CPU baseline.
 """
 import argparse
 import json
@@ -395,7 +396,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one MI355X only; multi-GPU runs are not supported "
-              "(see DESIGN.md for how the path shards)", file=sys.stderr)
+              "(decays, phantoms and requests would shard by stream range, phantom and request)", file=sys.stderr)
         sys.exit(2)
 
     import mcpet

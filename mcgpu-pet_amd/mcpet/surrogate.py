@@ -416,7 +416,7 @@ def main(argv=None):
     ap.add_argument("--width", type=int, default=WIDTH)
     ap.add_argument("--lr", type=float, default=5e-4)
     ap.add_argument("--in-channels", type=int, default=IN_CHANNELS, choices=(2, 4))
-    ap.add_argument("--out", default="gpurun_out/surrogate")
+    ap.add_argument("--out", default="surrogate_out")
     a = ap.parse_args(argv)
     os.makedirs(a.out, exist_ok=True)
     log = lambda m: print(m, flush=True)
