@@ -566,6 +566,7 @@ struct mcpet_surrogate {
   int device = 0;
   hipStream_t st = nullptr;
   hipEvent_t ev_f0 = nullptr, ev_f1 = nullptr;   // around the last forward of mcpet_surrogate_forward
+  float* h_stage = nullptr;   // pinned host staging of the fused estimate's two outputs (2 x voxels)
   bool fwd_timed = false;
   std::vector<void*> allocs;
   std::string err;
@@ -825,6 +826,7 @@ extern "C" mcpet_status mcpet_surrogate_create(int32_t D, int32_t H, int32_t W, 
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&s->ev_f0);
   if (e == hipSuccess) e = hipEventCreate(&s->ev_f1);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_stage, 2 * sizeof(float) * (size_t)D * H * W, hipHostMallocDefault);
   struct A {
     float** p;
     size_t n;
@@ -981,9 +983,12 @@ extern "C" mcpet_status mcpet_surrogate_estimate(mcpet_surrogate* s, mcpet_ctx* 
                      (float*)nullptr, 0.0f, (float*)nullptr, (int)nv, kNoCl);
   hipLaunchKernelGGL(k_estimate, dim3(blocks_for(nv)), dim3(256), 0, s->st, s->out, s->est, nv, s->sums + 256, (double)n);
   SHIP(s, hipGetLastError());
-  SHIP(s, hipMemcpyAsync(estimate, s->est, sizeof(float) * nv, hipMemcpyDeviceToHost, s->st));
-  if (scatter_counts) SHIP(s, hipMemcpyAsync(scatter_counts, s->raw_s, sizeof(float) * nv, hipMemcpyDeviceToHost, s->st));
+  // through pinned staging (one DMA per output), then host copies into the caller's buffers
+  SHIP(s, hipMemcpyAsync(s->h_stage, s->est, sizeof(float) * nv, hipMemcpyDeviceToHost, s->st));
+  if (scatter_counts) SHIP(s, hipMemcpyAsync(s->h_stage + nv, s->raw_s, sizeof(float) * nv, hipMemcpyDeviceToHost, s->st));
   SHIP(s, hipStreamSynchronize(s->st));
+  std::memcpy(estimate, s->h_stage, sizeof(float) * nv);
+  if (scatter_counts) std::memcpy(scatter_counts, s->h_stage + nv, sizeof(float) * nv);
   if (decays) *decays = n;
   return MCPET_OK;
 }
@@ -1045,6 +1050,7 @@ extern "C" void mcpet_surrogate_destroy(mcpet_surrogate* s)
   if (!s) return;
   if (s->st) (void)hipStreamSynchronize(s->st);
   for (void* p : s->allocs) (void)hipFree(p);
+  if (s->h_stage) (void)hipHostFree(s->h_stage);
   if (s->ev_f0) (void)hipEventDestroy(s->ev_f0);
   if (s->ev_f1) (void)hipEventDestroy(s->ev_f1);
   if (s->st) (void)hipStreamDestroy(s->st);
