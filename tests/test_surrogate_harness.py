@@ -81,6 +81,20 @@ def test_normalize_with_maps_scales_activity_by_its_maximum():
     assert np.all(z[2] == 0)
 
 
+def test_packaged_weights_fit_the_four_channel_network():
+    """The packaged parameters are a width-32, 4-input-channel network's (the layer list of
+    mcpet_surrogate_create), and their training log says so."""
+    import json
+    import os
+    w = np.load(S.WEIGHTS, allow_pickle=False)
+    assert w.dtype == np.float32 and w.size == S.param_count(S.IN_CHANNELS, S.WIDTH) == 1275841
+    assert S.param_count(2, 32) == 1274113 and S.param_count(4, 56) == 3901969
+    with open(os.path.splitext(S.WEIGHTS)[0] + ".json") as f:
+        info = json.load(f)
+    assert info["in_channels"] == 4 and info["width"] == 32 and list(info["grid"]) == list(S.GRID)
+    assert np.all(np.isfinite(w))
+
+
 def test_workload_config():
     c = S.config([], 5e6)
     assert c.acquisition_time_ps == 5_000_000_000 and c.stream_decays == 5 and c.nrad == 168 and c.nsinos == 1293
