@@ -42,11 +42,12 @@ def algorithmic(stats):
     return ops, bytes_
 
 
-def load_traffic():
-    p = os.path.join(ROOT, "profiles", "transport_head_traffic.json")
+def load_traffic(name="transport_head_traffic.json", key="hbm_bytes_per_launch"):
+    """Measured HBM bytes (calibrated FETCH_SIZE + WRITE_SIZE) from a committed profile."""
+    p = os.path.join(ROOT, "profiles", name)
     if os.path.exists(p):
         with open(p) as f:
-            return json.load(f).get("hbm_bytes_per_launch")
+            return json.load(f).get(key)
     return None
 
 
@@ -240,7 +241,8 @@ def bench_surrogate(a):
            "surrogate_nrmse_max": max(r["surrogate_nrmse_max"] for r in rows),
            "fallback_fraction": float(np.mean([r["fallback_fraction"] for r in rows])),
            "roofline": {"bound": "mfma", "achieved": gflop / fwd_s / 1e3, "peak": 2500.0, "unit": "TFLOP/s",
-                        "frac": gflop / fwd_s / 1e3 / 2500.0, "traffic": None,
+                        "frac": gflop / fwd_s / 1e3 / 2500.0,
+                        "traffic": load_traffic("surrogate_unet_forward_traffic.json", "hbm_bytes_per_forward"),
                         "kernel": "U-Net forward (one sample; HIP events on the network's stream)", "gflop": gflop,
                         "ms": fwd_s * 1e3, "host_ms": fwd_host_s * 1e3},
            "phantoms": rows}
@@ -282,7 +284,8 @@ def bench_unet(a):
            "forward_gflop": gflop, "forward_host_ms": fwd_host * 1e3, "train_step_ms": trn * 1e3,
            "train_tflops": 3 * 8 * gflop / trn / 1e3,
            "roofline": {"bound": "mfma", "achieved": tf, "peak": 2500.0, "unit": "TFLOP/s", "frac": tf / 2500.0,
-                        "traffic": None, "kernel": "U-Net forward (HIP events on the network's stream)"}}
+                        "traffic": load_traffic("surrogate_unet_forward_traffic.json", "hbm_bytes_per_forward"),
+                        "kernel": "U-Net forward (HIP events on the network's stream)"}}
     print(json.dumps(out))
 
 
