@@ -388,6 +388,28 @@ __global__ void k_upsample(const float* __restrict__ x, float* __restrict__ y, i
   y[i] = v;
   if (o.p) o.p[(i % vox2) * o.ld + o.off + c] = to_bf16(v);
 }
+// k_upsample with each thread on 8 channels of one output voxel: the FP32 stores stay
+// coalesced across threads (consecutive voxels), and the channels-last copy of the 8 channels
+// is one 16-byte store (C % 8 == 0, o.ld and o.off multiples of 8).
+__global__ void k_upsample8(const float* __restrict__ x, float* __restrict__ y, int C, int d, int h, int w, ClOut o)
+{
+  const int W2 = 2 * w, H2 = 2 * h, D2 = 2 * d;
+  const size_t vox2 = (size_t)W2 * H2 * D2;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vox2 * (size_t)(C / 8)) return;
+  const size_t v = i % vox2;
+  const int c0 = (int)(i / vox2) * 8;
+  const int xx = (int)(v % W2), yy = (int)((v / W2) % H2), zz = (int)(v / ((size_t)W2 * H2));
+  const size_t src = ((size_t)(zz / 2) * h + yy / 2) * w + xx / 2, vin = (size_t)d * h * w;
+  bf16x8 q;
+#pragma unroll
+  for (int e = 0; e < 8; e++) {
+    const float val = x[(size_t)(c0 + e) * vin + src];
+    y[(size_t)(c0 + e) * vox2 + v] = val;
+    q[e] = to_bf16(val);
+  }
+  if (o.p) *(bf16x8*)(o.p + v * o.ld + o.off + c0) = q;
+}
 // channel-first FP32 [C][vox] -> channels-last BF16 copy
 __global__ void k_to_cl(const float* __restrict__ x, int C, size_t vox, ClOut o)
 {
@@ -693,7 +715,10 @@ hipError_t conv_bwd_weights(mcpet_surrogate* s, const Conv& l, const Vol& vin, c
 }
 hipError_t upsample(mcpet_surrogate* s, const Vol& v, const float* x, float* y, ClOut o)
 {
-  hipLaunchKernelGGL(k_upsample, dim3(blocks_for(v.size() * 8)), dim3(256), 0, s->st, x, y, v.C, v.D, v.H, v.W, o);
+  if (v.C % 8 == 0 && (!o.p || (o.ld % 8 == 0 && o.off % 8 == 0)))
+    hipLaunchKernelGGL(k_upsample8, dim3(blocks_for(v.size())), dim3(256), 0, s->st, x, y, v.C, v.D, v.H, v.W, o);
+  else
+    hipLaunchKernelGGL(k_upsample, dim3(blocks_for(v.size() * 8)), dim3(256), 0, s->st, x, y, v.C, v.D, v.H, v.W, o);
   return hipGetLastError();
 }
 hipError_t upsample_bwd(mcpet_surrogate* s, const Vol& v, const float* gy, float* gx, ClOut o)
