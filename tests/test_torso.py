@@ -57,6 +57,14 @@ def test_torso_phantom_shape_and_tissues():
     assert r.max() < 46.0
 
 
+def test_torso_phantom_is_seeded():
+    a = phantoms.torso(seed=3, shape=(40, 64, 64), voxel_cm=1.0)
+    b = phantoms.torso(seed=3, shape=(40, 64, 64), voxel_cm=1.0)
+    c = phantoms.torso(seed=4, shape=(40, 64, 64), voxel_cm=1.0)
+    assert all(np.array_equal(x, y) for x, y in zip(a[:3], b[:3]))
+    assert not np.array_equal(a[0], c[0])
+
+
 @pytest.mark.gpu
 def test_torso_slice_at_full_rate_bit_exact():
     sc, n = torso_scenario(0.002, tally_psf_sinogram=3, coincidence_window_ps=WINDOW, singles_capacity=4_000_000)
