@@ -285,11 +285,12 @@ __global__ __launch_bounds__(256) void k_conv(ConvK a)
         if (MODE == FWD) {
           o = s[v] + a.bias[row];
           if (a.relu) o = fmaxf(o, 0.0f);
+          if (a.C) a.C[idx] = o;   // null: no FP32 copy (inference)
         } else {
           o = s[v] + (a.beta ? a.C[idx] : 0.0f);
           if (a.mask && !(a.mask[idx] > 0.0f)) o = 0.0f;
+          a.C[idx] = o;
         }
-        a.C[idx] = o;
         ov[v] = o;
       }
       // channels-last copies: the four channels in one 8-byte store when they all exist
@@ -385,7 +386,7 @@ __global__ void k_upsample(const float* __restrict__ x, float* __restrict__ y, i
   const int xx = (int)(i % W2), yy = (int)((i / W2) % H2), zz = (int)((i / ((size_t)W2 * H2)) % D2);
   const int c = (int)(i / vox2);
   const float v = x[(((size_t)c * d + zz / 2) * h + yy / 2) * w + xx / 2];
-  y[i] = v;
+  if (y) y[i] = v;   // null: no FP32 copy (inference)
   if (o.p) o.p[(i % vox2) * o.ld + o.off + c] = to_bf16(v);
 }
 // k_upsample with each thread on 8 channels of one output voxel: the FP32 stores stay
@@ -405,7 +406,7 @@ __global__ void k_upsample8(const float* __restrict__ x, float* __restrict__ y, 
 #pragma unroll
   for (int e = 0; e < 8; e++) {
     const float val = x[(size_t)(c0 + e) * vin + src];
-    y[(size_t)(c0 + e) * vox2 + v] = val;
+    if (y) y[(size_t)(c0 + e) * vox2 + v] = val;   // null: no FP32 copy (inference)
     q[e] = to_bf16(val);
   }
   if (o.p) *(bf16x8*)(o.p + v * o.ld + o.off + c0) = q;
@@ -728,8 +729,11 @@ hipError_t upsample_bwd(mcpet_surrogate* s, const Vol& v, const float* gy, float
 }
 ClOut cl(__bf16* p, int channels, int off = 0) { return ClOut{p, rup(channels, 8), off}; }
 
-// Forward of one sample already in s->x0 (r = the residual).
-hipError_t forward(mcpet_surrogate* s)
+// Forward of one sample already in s->x0 (r = the residual). keep: write the FP32 copy of every
+// activation (training reads them); without it only the tensors the forward itself reads in
+// FP32 (the upsampling inputs b3 and e2, the residual r) are written, the layers passing their
+// outputs to the next one through the channels-last BF16 copies alone.
+hipError_t forward(mcpet_surrogate* s, bool keep)
 {
   hipError_t e = hipSuccess;
   const int c = s->c;
@@ -741,16 +745,17 @@ hipError_t forward(mcpet_surrogate* s)
   e = hipGetLastError();
 #define STEP(x) \
   if (e == hipSuccess) e = (x)
-  STEP(conv_fwd(s, s->L[0], s->vx, s->x0l, s->va1, s->a1, true, cl(s->a1l, c)));
-  STEP(conv_fwd(s, s->L[1], s->va1, s->a1l, vb1, b1, true, cl(s->b1l, c), cl(s->cat1l, 3 * c, 2 * c)));
-  STEP(conv_fwd(s, s->L[2], vb1, s->b1l, s->vd1, s->d1, true, cl(s->d1l, 2 * c)));
-  STEP(conv_fwd(s, s->L[3], s->vd1, s->d1l, vb2, b2, true, cl(s->b2l, 2 * c), cl(s->cat2l, 6 * c, 4 * c)));
-  STEP(conv_fwd(s, s->L[4], vb2, s->b2l, s->vd2, s->d2, true, cl(s->d2l, 4 * c)));
+  auto k = [keep](float* p) { return keep ? p : nullptr; };
+  STEP(conv_fwd(s, s->L[0], s->vx, s->x0l, s->va1, k(s->a1), true, cl(s->a1l, c)));
+  STEP(conv_fwd(s, s->L[1], s->va1, s->a1l, vb1, k(b1), true, cl(s->b1l, c), cl(s->cat1l, 3 * c, 2 * c)));
+  STEP(conv_fwd(s, s->L[2], vb1, s->b1l, s->vd1, k(s->d1), true, cl(s->d1l, 2 * c)));
+  STEP(conv_fwd(s, s->L[3], s->vd1, s->d1l, vb2, k(b2), true, cl(s->b2l, 2 * c), cl(s->cat2l, 6 * c, 4 * c)));
+  STEP(conv_fwd(s, s->L[4], vb2, s->b2l, s->vd2, k(s->d2), true, cl(s->d2l, 4 * c)));
   STEP(conv_fwd(s, s->L[5], s->vd2, s->d2l, s->vb3, s->b3, true, kNoCl));
-  STEP(upsample(s, s->vb3, s->b3, s->cat2, cl(s->cat2l, 6 * c)));    // first 4c channels of cat2
+  STEP(upsample(s, s->vb3, s->b3, k(s->cat2), cl(s->cat2l, 6 * c)));    // first 4c channels of cat2
   STEP(conv_fwd(s, s->L[6], s->vcat2, s->cat2l, s->ve2, s->e2, true, kNoCl));
-  STEP(upsample(s, s->ve2, s->e2, s->cat1, cl(s->cat1l, 3 * c)));    // first 2c channels of cat1
-  STEP(conv_fwd(s, s->L[7], s->vcat1, s->cat1l, s->ve1, s->e1, true, cl(s->e1l, c)));
+  STEP(upsample(s, s->ve2, s->e2, k(s->cat1), cl(s->cat1l, 3 * c)));    // first 2c channels of cat1
+  STEP(conv_fwd(s, s->L[7], s->vcat1, s->cat1l, s->ve1, k(s->e1), true, cl(s->e1l, c)));
   STEP(conv_fwd(s, s->L[8], s->ve1, s->e1l, s->vr, s->r, false, kNoCl));
 #undef STEP
   return e;
@@ -944,7 +949,7 @@ extern "C" mcpet_status mcpet_surrogate_forward(mcpet_surrogate* s, const float*
     SHIP(s, hipMemcpyAsync(s->x0, x + b * nx, sizeof(float) * nx, hipMemcpyHostToDevice, s->st));
     const bool last = b == batch - 1;
     if (last) SHIP(s, hipEventRecord(s->ev_f0, s->st));
-    SHIP(s, forward(s));
+    SHIP(s, forward(s, false));
     hipLaunchKernelGGL(k_output, dim3(blocks_for(nv)), dim3(256), 0, s->st, s->x0, s->r, s->out, (const float*)nullptr,
                        (float*)nullptr, 0.0f, (float*)nullptr, (int)nv, kNoCl);
     SHIP(s, hipGetLastError());
@@ -1003,7 +1008,7 @@ extern "C" mcpet_status mcpet_surrogate_estimate(mcpet_surrogate* s, mcpet_ctx* 
   hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, s->st, s->sums, 256, s->sums + 256);
   hipLaunchKernelGGL(k_normalize, dim3(blocks_for(nv)), dim3(256), 0, s->st, s->raw_s, s->raw_t, s->x0, nv, s->sums + 256);
   SHIP(s, hipGetLastError());
-  SHIP(s, forward(s));
+  SHIP(s, forward(s, false));
   hipLaunchKernelGGL(k_output, dim3(blocks_for(nv)), dim3(256), 0, s->st, s->x0, s->r, s->out, (const float*)nullptr,
                      (float*)nullptr, 0.0f, (float*)nullptr, (int)nv, kNoCl);
   hipLaunchKernelGGL(k_estimate, dim3(blocks_for(nv)), dim3(256), 0, s->st, s->out, s->est, nv, s->sums + 256, (double)n);
@@ -1030,7 +1035,7 @@ static mcpet_status batch_gradients(mcpet_surrogate* s, const float* x, const fl
   for (int b = 0; b < batch; b++) {
     SHIP(s, hipMemcpyAsync(s->x0, x + b * nx, sizeof(float) * nx, hipMemcpyHostToDevice, s->st));
     SHIP(s, hipMemcpyAsync(s->tgt, target + b * nv, sizeof(float) * nv, hipMemcpyHostToDevice, s->st));
-    SHIP(s, forward(s));
+    SHIP(s, forward(s, true));
     hipLaunchKernelGGL(k_output, dim3(blocks_for(nv)), dim3(256), 0, s->st, s->x0, s->r, s->out, s->tgt, s->gr, scale, s->sse,
                        (int)nv, ClOut{s->grl, 8, 0});
     SHIP(s, hipGetLastError());
