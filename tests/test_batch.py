@@ -103,3 +103,44 @@ def test_batch_with_balanced_streams_equals_each_phantoms_own_run():
             rs = sim.stats()
         _same(got[k], ref)
         assert st[k]["streams"] == rs["streams"] and st[k]["decays"] == rs["decays"] > 0
+
+
+@pytest.mark.gpu
+def test_48_phantom_workload_graph_replay_equals_each_persistent_run():
+    """The many-phantom workload at its stated size: 48 random phantoms (32-160 x 32-160 x
+    32-96 voxels at 2 mm, 1e6-1e8 photon histories each in proportion to their activity,
+    10 s), a 168 x 168 x 1293 sinogram per phantom, all in one batched persistent launch
+    captured and replayed as a HIP graph. After the replay, every phantom's tallies, final
+    seed and counters equal those of its own persistent launch, bit for bit."""
+    import sys
+    specs = phantoms.random_phantoms(48)
+    names = phantoms.head_materials(MATERIALS)
+    cfg = phantoms.make_config(seed=20261202, threads_per_block=32, acquisition_time_s=10.0, nrad=168,
+                               materials=names)
+    phs = [mcpet.phantom_from_arrays(*sp) for sp in specs]
+    tab = mcpet.load_materials(names, np.max([p.density_max() for p in phs], axis=0))
+    print(f"48-phantom batch: {sum(p.nvox for p in phs)} voxels, graph capture and replay", flush=True)
+    with mcpet.Batch(cfg, phs, tab) as b:
+        b.run(seed=11)                           # eager, then the graph (captured) and its replay
+        b.sync()
+        b.run(seed=4321, use_graph=True)
+        b.sync()
+        b.run(seed=4321, use_graph=True)
+        b.sync()
+        total = 0
+        for k, ph in enumerate(phs):
+            got, gs = b.fetch(k, psf=False), b.stats(k)
+            with mcpet.Simulation(cfg, ph, tab) as sim:
+                sim.run(seed=4321, kernel=mcpet.KERNEL_PERSISTENT)
+                ref, rs = sim.fetch(psf=False), sim.stats()
+            _same(got, ref)
+            for key in ("decays", "streams", "pairs_tallied", "coincidences"):
+                assert gs[key] == rs[key], (k, key, gs[key], rs[key])
+            total += gs["decays"]
+            del got, ref
+        print(f"48-phantom batch: {total} decays, every phantom equals its own run", flush=True)
+        sys.stdout.flush()
+    assert 1e8 < total < 2e9
+    for p in phs:
+        p.close()
+    tab.close()

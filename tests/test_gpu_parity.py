@@ -107,13 +107,12 @@ def test_head_small_bit_exact():
     assert_tallies_equal(g, o)
 
 
-def test_statistical_vs_libm_oracle():
-    _say("water cube 1e6 decays: GPU vs libm oracle on an independent seed, Poisson totals, spectrum chi2, image z")
-    sc = water_cube_scenario(n_decays=1e6, tpb=64, seed=99, material_dose=True)
-    g = sc.run_gpu()
-    sc_o = water_cube_scenario(n_decays=1e6, tpb=64, seed=1234567, material_dose=True)
-    o = sc_o.run_oracle("libm")
-    # the criteria: totals within 3 sigma, spectrum chi-square p > 0.01, per-bin z-scores
+def statistical_criteria(g, o, cfg, shape, image_block=1):
+    """The transport's statistical criteria between two independent simulations of the same
+    workload: totals of trues and scatter within 3 sigma, energy spectrum chi-square p > 0.01,
+    per-bin z-scores of the emission images (in blocks of image_block voxels per axis) and of the
+    sinograms (summed over planes, blocks of 8 angles x 3 radial bins) with mean within 0.2 of 0
+    and spread 0.8-1.25."""
     for k in ("trues", "scatter"):
         a, b = int(g[k].sum()), int(o[k].sum())
         z = (a - b) / np.sqrt(a + b)
@@ -125,25 +124,55 @@ def test_statistical_vs_libm_oracle():
     p = stats.chi2.sf(chi2, m.sum() - 1)
     _say(f"spectrum chi2={chi2:.1f} dof={m.sum() - 1} p={p:.3g}")
     assert p > 0.01, f"spectrum chi2={chi2:.1f} dof={m.sum() - 1} p={p:.2g}"
-    # per-bin z-scores of the emission image (coarse bins, well populated)
-    gi = g["image_trues"].astype(float)
-    oi = o["image_trues"].astype(float)
-    mm = (gi + oi) > 50
-    zz = (gi[mm] - oi[mm]) / np.sqrt(gi[mm] + oi[mm])
-    _say(f"image z: {mm.sum()} bins, mean {zz.mean():.3f} std {zz.std():.3f}")
-    assert abs(zz.mean()) < 0.2 and 0.8 < zz.std() < 1.25, f"image z mean {zz.mean():.3f} std {zz.std():.3f}"
-    # the sinogram itself, summed over its planes and in blocks of 8 angles x 3 radial bins
-    # (well-populated bins): per-bin z-scores ~ N(0, 1)
-    cfg = sc.cfg
+
+    def blocks(img):
+        a = img.astype(float).reshape(shape)
+        nb = [n // image_block * image_block for n in shape]
+        a = a[:nb[0], :nb[1], :nb[2]]
+        return a.reshape(nb[0] // image_block, image_block, nb[1] // image_block, image_block,
+                         nb[2] // image_block, image_block).sum(axis=(1, 3, 5)).ravel()
+
+    for which in ("image_trues", "image_scatter"):
+        gi, oi = blocks(g[which]), blocks(o[which])
+        mm = (gi + oi) > 50
+        zz = (gi[mm] - oi[mm]) / np.sqrt(gi[mm] + oi[mm])
+        _say(f"{which} z: {mm.sum()} bins, mean {zz.mean():.3f} std {zz.std():.3f}")
+        assert mm.sum() > 20 and abs(zz.mean()) < 0.2 and 0.8 < zz.std() < 1.25, (which, zz.mean(), zz.std())
+
     def coarse(t):
         a = t.astype(float).reshape(cfg.nsinos, cfg.nangles, cfg.nrad).sum(0)
         a = a[:cfg.nangles // 8 * 8, :cfg.nrad // 3 * 3]
         return a.reshape(cfg.nangles // 8, 8, cfg.nrad // 3, 3).sum(axis=(1, 3)).ravel()
-    gs, os_ = coarse(g["trues"]), coarse(o["trues"])
-    ms = (gs + os_) > 50
-    zs = (gs[ms] - os_[ms]) / np.sqrt(gs[ms] + os_[ms])
-    _say(f"sinogram z: {ms.sum()} bins, mean {zs.mean():.3f} std {zs.std():.3f}")
-    assert ms.sum() > 100 and abs(zs.mean()) < 0.2 and 0.8 < zs.std() < 1.25
+
+    for which in ("trues", "scatter"):
+        gs, os_ = coarse(g[which]), coarse(o[which])
+        ms = (gs + os_) > 50
+        zs = (gs[ms] - os_[ms]) / np.sqrt(gs[ms] + os_[ms])
+        _say(f"{which} sinogram z: {ms.sum()} bins, mean {zs.mean():.3f} std {zs.std():.3f}")
+        assert ms.sum() > 100 and abs(zs.mean()) < 0.2 and 0.8 < zs.std() < 1.25, (which, zs.mean(), zs.std())
+
+
+def test_statistical_vs_libm_oracle():
+    _say("water cube 1e6 decays: GPU vs libm oracle on an independent seed, Poisson totals, spectrum chi2, image z")
+    sc = water_cube_scenario(n_decays=1e6, tpb=64, seed=99, material_dose=True)
+    g = sc.run_gpu()
+    sc_o = water_cube_scenario(n_decays=1e6, tpb=64, seed=1234567, material_dose=True)
+    o = sc_o.run_oracle("libm")
+    statistical_criteria(g, o, sc.cfg, sc.phantom.shape)
+
+
+def test_statistical_vs_libm_oracle_on_the_head():
+    """The same criteria on heterogeneous geometry: the head phantom at 64 x 64 x 48 voxels of
+    4 mm (scalp, fat, skull bone at 1.85 g/cm3, grey and white matter, CSF, sinus air), 3e6
+    decays, where the tissue boundaries exercise material changes, Compton events in bone and
+    the air gaps; GPU against the oracle built on the system libm with an independent seed."""
+    _say("head 64x64x48 at 4 mm, 3e6 decays: GPU vs libm oracle on an independent seed")
+    sc = head_scenario(shape=(48, 64, 64), voxel_cm=0.4, n_decays=3e6, tpb=16, seed=2027)
+    g = sc.run_gpu(count_events=False)
+    sc_o = head_scenario(shape=(48, 64, 64), voxel_cm=0.4, n_decays=3e6, tpb=16, seed=8675309)
+    o = sc_o.run_oracle("libm", nthreads=16)
+    assert o["trues"].sum() > 50000 and o["scatter"].sum() > 20000
+    statistical_criteria(g, o, sc.cfg, sc.phantom.shape, image_block=8)
 
 
 def test_head_full_size_invariants():

@@ -180,13 +180,17 @@ def _layer_rel(a, b, cin, c):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("grid,cin,c,batch", [((8, 8, 8), 2, 4, 2), ((12, 8, 16), 2, 3, 1), ((4, 4, 4), 1, 2, 3)])
+@pytest.mark.parametrize("grid,cin,c,batch", [((8, 8, 8), 2, 4, 2), ((12, 8, 16), 2, 3, 1), ((4, 4, 4), 1, 2, 3),
+                                              ((8, 12, 12), 4, 32, 2), ((8, 8, 8), 4, 56, 1), ((4, 8, 8), 4, 16, 1)])
 def test_device_unet_forward_and_gradients_match_bf16_reference(grid, cin, c, batch):
     """Output, loss and every layer's weight and bias gradients of the device network against
     the BF16-operand CPU reference (float64 accumulation). The remaining differences come
     from float32 accumulation order and from activations whose float32 and float64 values
     round to different BF16 neighbours: bounded by 2e-4 (output, loss) and 2e-3 (gradients,
-    relative per layer)."""
+    relative per layer). Widths 2-4 use one 16-row tile per product; widths 16, 32 and 56 with
+    four input channels (the shipped network's layer shapes and a ~4 M-parameter one) take the
+    2-, 3- and 4-tile convolution kernels, several row blocks per product and the epilogue split
+    over the workgroup's waves, in the forward, input-gradient and weight-gradient modes."""
     rng = np.random.default_rng(sum(grid) + cin + c)
     with mcpet.Surrogate(grid, in_channels=cin, width=c, seed=7) as s:
         assert s.n_params == n_params(cin, c)
@@ -216,6 +220,25 @@ def test_device_unet_forward_and_gradients_match_bf16_reference(grid, cin, c, ba
     assert abs(loss - lr) <= 2e-4 * lr
     rel = _layer_rel(g.astype(np.float64), gr, cin, c)
     assert max(rel) <= 2e-3, rel
+
+
+@pytest.mark.gpu
+def test_packaged_network_forward_matches_bf16_reference():
+    """The shipped network (packaged weights, 4 input channels, width 32) on its full 32 x 56 x
+    56 grid: the device forward equals the BF16-operand CPU forward within 2e-4 (relative norm)."""
+    from mcpet import surrogate as sg
+    p = np.load(sg.WEIGHTS, allow_pickle=False).astype(np.float32)
+    rng = np.random.default_rng(12)
+    x = rng.random((1, sg.IN_CHANNELS) + sg.GRID).astype(np.float32)
+    with mcpet.Surrogate(sg.GRID, in_channels=sg.IN_CHANNELS, width=sg.WIDTH, seed=1) as s:
+        assert s.n_params == p.size == n_params(sg.IN_CHANNELS, sg.WIDTH)
+        s.set_params(p)
+        y = s.forward(x)
+    _, _, yr = unet_reference(p.astype(np.float64), x.astype(np.float64), np.zeros((1,) + sg.GRID), sg.IN_CHANNELS,
+                              sg.WIDTH, "bf16", grad=False)
+    rel = np.linalg.norm(y - yr) / np.linalg.norm(yr)
+    print(f"packaged network forward: relative difference {rel:.2e}", flush=True)
+    assert rel <= 2e-4, rel
 
 
 @pytest.mark.gpu
