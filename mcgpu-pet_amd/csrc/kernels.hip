@@ -176,6 +176,19 @@ __device__ __forceinline__ void rotate_double(float* d, double costh, double phi
   }
 }
 
+// The Rayleigh (GRAa) and Compton (GCOa) samplers below follow the PENELOPE 2006 algorithms
+// as the reference states them (MCGPU-PET_kernel.cu:1216-1228, 1318-1330), which carry this
+// notice:
+//   PENELOPE/PENGEOM (version 2006)
+//     Copyright (c) 2001-2006
+//     Universitat de Barcelona
+//   Permission to use, copy, modify, distribute and sell this software and its documentation
+//   for any purpose is hereby granted without fee, provided that the above copyright notice
+//   appears in all copies and that both that copyright notice and this permission notice
+//   appear in all supporting documentation. The Universitat de Barcelona makes no
+//   representations about the suitability of this software for any purpose. It is provided
+//   "as is" without express or implied warranty.
+
 // GRAa (K:1230-1296): Rayleigh angle by RITA inverse cumulative + Thomson rejection.
 template <class CN>
 K_RARE void graa(float energy, double* costh, int mat, float pmax_current, Rng& r, CN& C,

@@ -260,6 +260,19 @@ void orc_probe_rotate(int n, const float* dir_in, const double* costh, const dou
   }
 }
 
+/* The Rayleigh (GRAa) and Compton (GCOa) samplers below follow the PENELOPE 2006 algorithms
+ * as the reference states them (MCGPU-PET_kernel.cu:1216-1228, 1318-1330), which carry this
+ * notice:
+ *   PENELOPE/PENGEOM (version 2006)
+ *     Copyright (c) 2001-2006
+ *     Universitat de Barcelona
+ *   Permission to use, copy, modify, distribute and sell this software and its documentation
+ *   for any purpose is hereby granted without fee, provided that the above copyright notice
+ *   appears in all copies and that both that copyright notice and this permission notice
+ *   appear in all supporting documentation. The Universitat de Barcelona makes no
+ *   representations about the suitability of this software for any purpose. It is provided
+ *   "as is" without express or implied warranty. */
+
 static void o_GRAa(float energy, double* costh, int mat, float pmax_current, o_rng* r, const orc_rayleigh* cgra)  /* K:1230-1296 */
 {
   double xmax = ((double)energy) * 8.065535669099010e-5;
@@ -1282,8 +1295,43 @@ void orc_sinogram_sizes(int mrd, int span, int nzs, int nrad, int nangles, int* 
 
 void orc_free(void* p) { free(p); }
 
-int orc_load_material(const char* names, const float* density_max_in, float* density_nominal, int* num_values, float* e0,
-                      float* ide, float** woodcock, float** mfp_a, float** mfp_b, orc_rayleigh* ray, orc_compton* cmp)  /* C:1565-1831 */
+/* Library extension (no reference counterpart): the energy grid continued past the files'
+ * last point E_L up to e_max. Added points, as raw mean free paths: Compton scaled by the
+ * Klein-Nishina total cross section (sigma_KN(E_L) / sigma_KN(E)); Rayleigh and photoelectric
+ * as power laws whose exponents are the log-log slopes between E_L and the file point nearest
+ * E_L / 2; total = 1 / (sum of the inverse partial mean free paths); pmax = the last value. */
+static double o_klein_nishina(double e)
+{
+  double k = e / 510998.95, l = log(1.0 + 2.0 * k);
+  return (1.0 + k) / (k * k) * (2.0 * (1.0 + k) / (1.0 + 2.0 * k) - l / k) + l / (2.0 * k) -
+         (1.0 + 3.0 * k) / ((1.0 + 2.0 * k) * (1.0 + 2.0 * k));
+}
+
+static void o_extend_rows(double* rows, int n_file, int n_total, double e0, double delta_e)
+{
+  const double* last = rows + 6 * (size_t)(n_file - 1);
+  int h = (int)floor((0.5 * last[0] - e0) / delta_e + 0.5);
+  if (h > n_file - 2) h = n_file - 2;
+  if (h < 0) h = 0;
+  const double* half = rows + 6 * (size_t)h;
+  double s_ray = log(last[1] / half[1]) / log(last[0] / half[0]);
+  double s_pho = log(last[3] / half[3]) / log(last[0] / half[0]);
+  double kn_last = o_klein_nishina(last[0]);
+  for (int i = n_file; i < n_total; i++) {
+    double* r = rows + 6 * (size_t)i;
+    double e = e0 + i * delta_e;
+    r[0] = e;
+    r[1] = last[1] * pow(e / last[0], s_ray);
+    r[2] = last[2] * kn_last / o_klein_nishina(e);
+    r[3] = last[3] * pow(e / last[0], s_pho);
+    r[4] = 1.0 / (1.0 / r[1] + 1.0 / r[2] + 1.0 / r[3]);
+    r[5] = last[5];
+  }
+}
+
+int orc_load_material_ext(const char* names, const float* density_max_in, double e_max, float* density_nominal, int* num_values,
+                          float* e0, float* ide, float** woodcock, float** mfp_a, float** mfp_b, orc_rayleigh* ray,
+                          orc_compton* cmp)  /* C:1565-1831 (+ the grid extension above) */
 {
   char new_line[256];
   int mat, i, bin, input_num_values = 0, input_rayleigh_values = 0, input_num_shells = 0;
@@ -1292,7 +1340,8 @@ int orc_load_material(const char* names, const float* density_max_in, float* den
   float* W = NULL;
   float* Ta = NULL;
   float* Tb = NULL;
-  int N = 0;
+  double* rows = NULL;
+  int N = 0, NT = 0;
   for (mat = 0; mat < O_MAXMAT; mat++) {
     density_max[mat] = density_max_in[mat];
     density_nominal[mat] = -1.0f;
@@ -1317,38 +1366,51 @@ int orc_load_material(const char* names, const float* density_max_in, float* den
     sscanf(new_line, "# %d", &input_num_values);
     if (0 == mat) {
       N = input_num_values;
-      W = (float*)calloc(2 * (size_t)N, sizeof(float));
-      Ta = (float*)calloc(3 * (size_t)N * O_MAXMAT, sizeof(float));
-      Tb = (float*)calloc(3 * (size_t)N * O_MAXMAT, sizeof(float));
-      for (i = 0; i < N; i++) W[2 * i] = 99999999.99f;
     } else if (input_num_values != N) {
       gzclose(f);
       return -4;
     }
     gzgets(f, new_line, 250);
     gzgets(f, new_line, 250);
-    double d_energy, d_rayleigh, d_compton, d_photelectric, d_total_mfp, d_pmax, e_last = -1.0;
+    if (0 == mat) rows = (double*)calloc(6 * (size_t)N, sizeof(double));
+    double e_last = -1.0;
     for (i = 0; i < N; i++) {
+      double* r = rows + 6 * (size_t)i;
       gzgets(f, new_line, 250);
-      sscanf(new_line, "  %le  %le  %le  %le  %le  %le", &d_energy, &d_rayleigh, &d_compton, &d_photelectric, &d_total_mfp, &d_pmax);
+      sscanf(new_line, "  %le  %le  %le  %le  %le  %le", &r[0], &r[1], &r[2], &r[3], &r[4], &r[5]);
+      if (0 == i && 0 == mat) *e0 = r[0];
+      if (1 == i) delta_e = r[0] - e_last;
+      e_last = r[0];
+    }
+    if (0 == mat) {
+      NT = N;
+      if (e_max > e_last) NT = N + (int)ceil((e_max - e_last) / delta_e) + 1;
+      if (NT > ORC_MAX_ENERGYBINS_RAYLEIGH) { gzclose(f); free(rows); return -10; }
+      rows = (double*)realloc(rows, 6 * (size_t)NT * sizeof(double));
+      W = (float*)calloc(2 * (size_t)NT, sizeof(float));
+      Ta = (float*)calloc(3 * (size_t)NT * O_MAXMAT, sizeof(float));
+      Tb = (float*)calloc(3 * (size_t)NT * O_MAXMAT, sizeof(float));
+      for (i = 0; i < NT; i++) W[2 * i] = 99999999.99f;
+    }
+    if (NT > N) o_extend_rows(rows, N, NT, *e0, delta_e);
+    for (i = 0; i < NT; i++) {
+      const double* r = rows + 6 * (size_t)i;
+      double d_rayleigh = r[1], d_compton = r[2], d_total_mfp = r[4], d_pmax = r[5];
       float temp_mfp = d_total_mfp * (density_nominal[mat]) / (density_max[mat]);
       if (temp_mfp < W[2 * i]) W[2 * i] = temp_mfp;
       Ta[3 * (i * O_MAXMAT + mat) + 0] = 1.0 / (d_total_mfp * density_nominal[mat]);
       Ta[3 * (i * O_MAXMAT + mat) + 1] = 1.0 / (d_compton * density_nominal[mat]);
       Ta[3 * (i * O_MAXMAT + mat) + 2] = 1.0 / (d_rayleigh * density_nominal[mat]);
       ray->pmax[i * O_MAXMAT + mat] = d_pmax;
-      if (0 == i && 0 == mat) *e0 = d_energy;
-      if (1 == i) delta_e = d_energy - e_last;
-      e_last = d_energy;
     }
     *ide = 1.0f / delta_e;
-    for (i = 0; i < (N - 1); i++) {
+    for (i = 0; i < (NT - 1); i++) {
       bin = i * O_MAXMAT + mat;
       for (int c = 0; c < 3; c++) Tb[3 * bin + c] = (Ta[3 * (bin + O_MAXMAT) + c] - Ta[3 * bin + c]) / delta_e;
     }
-    for (int c = 0; c < 3; c++) Tb[3 * ((N - 1) * O_MAXMAT + mat) + c] = Tb[3 * ((N - 2) * O_MAXMAT + mat) + c];
-    for (i = 0; i < N; i++) {
-      d_energy = *e0 + i * delta_e;
+    for (int c = 0; c < 3; c++) Tb[3 * ((NT - 1) * O_MAXMAT + mat) + c] = Tb[3 * ((NT - 2) * O_MAXMAT + mat) + c];
+    for (i = 0; i < NT; i++) {
+      double d_energy = *e0 + i * delta_e;
       bin = i * O_MAXMAT + mat;
       for (int c = 0; c < 3; c++) Ta[3 * bin + c] = Ta[3 * bin + c] - d_energy * Tb[3 * bin + c];
     }
@@ -1385,15 +1447,23 @@ int orc_load_material(const char* names, const float* density_max_in, float* den
     }
     gzclose(f);
   }
+  free(rows);
   if (!W) return -9;
-  for (i = 0; i < (N - 1); i++) W[2 * i + 1] = (W[2 * (i + 1)] - W[2 * i]) / delta_e;
-  W[2 * (N - 1) + 1] = 0.0f;  /* not set by the reference (heap memory) */
-  for (i = 0; i < N; i++) W[2 * i] = W[2 * i] - (*e0 + i * delta_e) * W[2 * i + 1];
-  *num_values = N;
+  for (i = 0; i < (NT - 1); i++) W[2 * i + 1] = (W[2 * (i + 1)] - W[2 * i]) / delta_e;
+  W[2 * (NT - 1) + 1] = 0.0f;  /* not set by the reference (heap memory) */
+  for (i = 0; i < NT; i++) W[2 * i] = W[2 * i] - (*e0 + i * delta_e) * W[2 * i + 1];
+  *num_values = NT;
   *woodcock = W;
   *mfp_a = Ta;
   *mfp_b = Tb;
   return 0;
+}
+
+
+int orc_load_material(const char* names, const float* density_max_in, float* density_nominal, int* num_values, float* e0,
+                      float* ide, float** woodcock, float** mfp_a, float** mfp_b, orc_rayleigh* ray, orc_compton* cmp)  /* C:1565-1831 */
+{
+  return orc_load_material_ext(names, density_max_in, 0.0, density_nominal, num_values, e0, ide, woodcock, mfp_a, mfp_b, ray, cmp);
 }
 
 #ifdef ORACLE_SHARED_FMATH
