@@ -9,7 +9,10 @@ resident on the device before timing; every step clears the tallies and simulate
 decays with a new seed.
 
 Prints one JSON line: metric, value, unit, timing, config, a roofline object for the dominant
-kernel and (head and torso workloads) the CPU baseline.
+kernel and (head and torso workloads) the CPU baseline. The head line also carries the second
+half of the headline metric, `time_to_5pct_nrmse`: the time Monte Carlo and the learned
+surrogate take to a 5 %-NRMSE scatter sinogram on one phantom of each evaluation family, with
+the U-Net's MFMA roofline (--no-surrogate leaves it out).
 """
 import argparse
 import json
@@ -51,19 +54,52 @@ def load_traffic(name="transport_head_traffic.json", key="hbm_bytes_per_launch")
     return None
 
 
-def cpu_baseline(cfg, ph, vs, names, threads, stride):
+def host_description():
+    """The CPU baseline's host: CPUs this process may run on, the CPU model, the C compiler."""
+    import subprocess
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        gcc = subprocess.run(["gcc", "--version"], capture_output=True, text=True).stdout.splitlines()[0]
+    except (OSError, IndexError):
+        gcc = None
+    return {"affinity_cpus": len(os.sched_getaffinity(0)), "os_cpu_count": os.cpu_count(), "cpu_model": model,
+            "compiler": gcc, "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def baseline_threads(requested):
+    """Threads of the CPU baseline: the CPUs this process may run on, capped by the host's
+    per-job CPU share when the environment states one (OMP_NUM_THREADS)."""
+    if requested > 0:
+        return requested
+    n = len(os.sched_getaffinity(0))
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    if share.isdigit() and int(share) > 0:
+        n = min(n, int(share))
+    return max(1, n)
+
+
+def cpu_baseline(cfg, ph, vs, names, threads, stride, e_max=0.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py
     from mcpet import phantoms
-    tab = oracle_py.OracleTables(names + [""] * (15 - len(names)), ph.density_max(), "libm")
+    tab = oracle_py.OracleTables(names + [""] * (15 - len(names)), ph.density_max(), "libm", e_max=e_max)
     o = oracle_py.Oracle(phantoms.config_dict(cfg), ph.voxel_mat_dens(), ph.shape, vs, tab, "libm")
     r = o.run(nthreads=threads, voxel_stride=stride)
     c = r["counters"]
     return {"value": c["coincidences"] / r["wall_seconds"], "unit": "coincidences/s", "cores": threads, "kind": "port",
             "sample": f"every {stride}th voxel of the same phantom, full 10 s acquisition: "
                       f"{c['decays']} decays, "
-                      f"{c['coincidences']} coincidences in {r['wall_seconds']:.1f} s (C restatement, libm, "
-                      f"{threads} threads)"}
+                      f"{c['coincidences']} coincidences in {r['wall_seconds']:.1f} s (C restatement of the reference "
+                      f"transport built with gcc -O3 -march=x86-64-v3, system libm, {threads} threads)",
+            "host": host_description()}
 
 
 def bench_batch(a):
@@ -140,33 +176,25 @@ def eval_phantoms(family, n, seed):
     return S.family_phantoms(family, n, seed)
 
 
-def bench_surrogate(a):
-    """Scatter surrogate workload: time to a 5 %-NRMSE scatter sinogram on the 32 x 56 x 56
-    grid, Monte Carlo against the surrogate (low-count run + device rebinning + U-Net), on
-    evaluation phantoms drawn with a seed disjoint from the training phantoms. Per phantom: a
+def time_to_nrmse(specs, est, steps, warmup, max_ref_runs=8, log=None):
+    """Time to a 5 %-NRMSE scatter sinogram on the 32 x 56 x 56 grid, Monte Carlo against the
+    surrogate (low-count run + device rebinning + U-Net), per phantom of `specs`. Per phantom: a
     Monte Carlo reference accumulated in runs of 4e9 decays until its Poisson floor is <= 1 %
-    (or 8 runs); N5 from the reference; Monte Carlo at N5 and the surrogate each timed over
-    a.steps runs (median) after a.warmup untimed runs, wall clock from the run call to the
-    estimate on the host."""
+    (or max_ref_runs runs); N5 from the reference; Monte Carlo at N5 and the surrogate each timed
+    over `steps` runs (median) after `warmup` untimed runs, wall clock from the run call to the
+    estimate on the host. Returns one dict per phantom."""
     import mcpet
-    from mcpet import phantoms
     from mcpet import surrogate as S
-    weights = a.surrogate_weights or S.WEIGHTS
-    if not os.path.exists(weights):
-        raise SystemExit(f"bench.py: surrogate weights {weights} missing (train with python -m mcpet.surrogate)")
     names = S._materials()
-    specs = eval_phantoms(a.eval_family, a.eval_phantoms, a.eval_seed)
-    est = S.ScatterEstimator(weights)
     rows = []
-    log = lambda m: print(m, file=sys.stderr, flush=True)
+    log = log or (lambda m: print(m, file=sys.stderr, flush=True))
     for i, sp in enumerate(specs):
         ph = mcpet.phantom_from_arrays(*sp)
         tab = mcpet.load_materials(names, ph.density_max())
-        # reference
         acc = np.zeros(S.GRID)
         ref_decays = 0
         with mcpet.Simulation(S.config(names, 4e9), ph, tab) as sim:
-            for j in range(8):
+            for j in range(max_ref_runs):
                 sim.run(seed=9000 + 17 * i + j)
                 acc += sim.rebin("scatter", S.GRID)
                 ref_decays += sim.stats()["decays"]
@@ -175,17 +203,15 @@ def bench_surrogate(a):
         ref = acc / ref_decays
         floor = S.poisson_floor(acc)
         n5 = S.decays_for(ref)
-        log(f"surrogate bench: phantom {i}: reference {ref_decays:.3g} decays, floor {floor:.4f}, N5 {n5:.3g}")
-        # Monte Carlo at N5
+        log(f"time to 5 %: phantom {i}: reference {ref_decays:.3g} decays, floor {floor:.4f}, N5 {n5:.3g}")
         with mcpet.Simulation(S.config(names, n5), ph, tab) as sim:
-            for w in range(a.warmup):
+            for w in range(warmup):
                 S.monte_carlo(sim, seed=100 + w)
             mc_t, mc_e = [], []
-            for s_ in range(a.steps):
+            for s_ in range(steps):
                 e, d, dt, _ = S.monte_carlo(sim, seed=200 + s_)
                 mc_t.append(dt)
                 mc_e.append(S.nrmse(e, ref))
-        # surrogate
         with mcpet.Simulation(S.config(names, S.N_LOW), ph, tab) as sim:
             # the phantom's maps are computed once per context (then kept): their first
             # computation is charged to every timed estimate
@@ -194,10 +220,10 @@ def bench_surrogate(a):
                 t0 = time.perf_counter()
                 sim.project(S.GRID, fetch=False)
                 t_proj = time.perf_counter() - t0
-            for w in range(a.warmup):
+            for w in range(warmup):
                 est.estimate(sim, seed=300 + w)
             su_t, su_e, su_z = [], [], []
-            for s_ in range(a.steps):
+            for s_ in range(steps):
                 e, info = est.estimate(sim, seed=400 + s_)
                 su_t.append(info["seconds"] + t_proj)
                 su_e.append(S.nrmse(e, ref))
@@ -213,9 +239,15 @@ def bench_surrogate(a):
                      "surrogate_nrmse": float(np.median(su_e)), "surrogate_nrmse_max": float(np.max(su_e)),
                      "consistency_z": float(np.median(su_z)), "fallback_fraction": fallback, "projection_seconds": t_proj,
                      "effective_seconds": t_eff, "speedup": t_mc / t_eff})
-        log(f"surrogate bench: phantom {i}: MC {t_mc * 1e3:.1f} ms (nrmse {np.median(mc_e):.4f}), surrogate "
+        log(f"time to 5 %: phantom {i}: MC {t_mc * 1e3:.1f} ms (nrmse {np.median(mc_e):.4f}), surrogate "
             f"{t_su * 1e3:.2f} ms (nrmse {np.median(su_e):.4f}, z {np.median(su_z):.2f}), speedup {t_mc / t_eff:.1f}")
-    # U-Net forward throughput (host-timed forward of one sample)
+    return rows
+
+
+def unet_forward_roofline(est):
+    """The U-Net forward of one sample against the dense BF16 MFMA peak: HIP events around the
+    network on its stream (mean of 20 forwards after one untimed)."""
+    from mcpet import surrogate as S
     x = np.random.default_rng(0).random((1, est.in_channels) + S.GRID).astype(np.float32)
     est.net.forward(x)
     t0 = time.perf_counter()
@@ -224,8 +256,27 @@ def bench_surrogate(a):
         est.net.forward(x)
         dev_ms.append(est.net.forward_ms())
     fwd_host_s = (time.perf_counter() - t0) / 20
-    fwd_s = float(np.mean(dev_ms)) / 1e3     # HIP events around the network on its stream
+    fwd_s = float(np.mean(dev_ms)) / 1e3
     gflop = est.net.gflop
+    return {"bound": "mfma", "achieved": gflop / fwd_s / 1e3, "peak": 2500.0, "unit": "TFLOP/s",
+            "frac": gflop / fwd_s / 1e3 / 2500.0,
+            "traffic": load_traffic("surrogate_unet_forward_traffic.json", "hbm_bytes_per_forward"),
+            "kernel": "U-Net forward (one sample; HIP events on the network's stream)", "gflop": gflop,
+            "ms": fwd_s * 1e3, "host_ms": fwd_host_s * 1e3}
+
+
+def bench_surrogate(a):
+    """Scatter surrogate workload: time to a 5 %-NRMSE scatter sinogram (time_to_nrmse) on
+    a.eval_phantoms evaluation phantoms of one family, drawn with a seed disjoint from the
+    training phantoms, and the U-Net forward's roofline."""
+    from mcpet import surrogate as S
+    weights = a.surrogate_weights or S.WEIGHTS
+    if not os.path.exists(weights):
+        raise SystemExit(f"bench.py: surrogate weights {weights} missing (train with python -m mcpet.surrogate)")
+    specs = eval_phantoms(a.eval_family, a.eval_phantoms, a.eval_seed)
+    est = S.ScatterEstimator(weights)
+    rows = time_to_nrmse(specs, est, a.steps, a.warmup)
+    roof = unet_forward_roofline(est)
     est.close()
     tot_mc = sum(r["mc_seconds"] for r in rows)
     tot_eff = sum(r["effective_seconds"] for r in rows)
@@ -240,13 +291,33 @@ def bench_surrogate(a):
            "mc_seconds_mean": tot_mc / len(rows), "surrogate_seconds_mean": tot_eff / len(rows),
            "surrogate_nrmse_max": max(r["surrogate_nrmse_max"] for r in rows),
            "fallback_fraction": float(np.mean([r["fallback_fraction"] for r in rows])),
-           "roofline": {"bound": "mfma", "achieved": gflop / fwd_s / 1e3, "peak": 2500.0, "unit": "TFLOP/s",
-                        "frac": gflop / fwd_s / 1e3 / 2500.0,
-                        "traffic": load_traffic("surrogate_unet_forward_traffic.json", "hbm_bytes_per_forward"),
-                        "kernel": "U-Net forward (one sample; HIP events on the network's stream)", "gflop": gflop,
-                        "ms": fwd_s * 1e3, "host_ms": fwd_host_s * 1e3},
-           "phantoms": rows}
+           "roofline": roof, "phantoms": rows}
     print(json.dumps(out))
+
+
+def time_to_5pct_object(a):
+    """The second half of the headline metric beside the default line: time to a 5 %-NRMSE
+    scatter sinogram, Monte Carlo against the surrogate, on one evaluation phantom of each
+    family (scatter, head, batch; seed disjoint from training), with the U-Net roofline."""
+    from mcpet import surrogate as S
+    specs = [eval_phantoms(f, 1, a.eval_seed + k)[0] for k, f in enumerate(("scatter", "head", "batch"))]
+    est = S.ScatterEstimator(a.surrogate_weights or S.WEIGHTS)
+    steps = max(1, min(a.steps, 5))
+    rows = time_to_nrmse(specs, est, steps, 1, max_ref_runs=5)
+    roof = unet_forward_roofline(est)
+    est.close()
+    tot_mc = sum(r["mc_seconds"] for r in rows)
+    tot_eff = sum(r["effective_seconds"] for r in rows)
+    return {"metric": "time to a 5%-NRMSE scatter sinogram, Monte Carlo / surrogate", "value": tot_mc / tot_eff,
+            "unit": "x (speedup)", "higher_is_better": True,
+            "workload": f"one evaluation phantom per family (scatter 128x128x64, head 128x128x96, batch random; seeds "
+                        f"{a.eval_seed}-{a.eval_seed + 2}), scatter sinogram rebinned to 32x56x56, Monte Carlo "
+                        f"reference of up to 2e10 decays, surrogate = {S.N_LOW:.0e}-decay run + maps + U-Net width "
+                        f"{S.WIDTH}; medians of {steps} timed runs each",
+            "mc_seconds": [r["mc_seconds"] for r in rows], "surrogate_seconds": [r["effective_seconds"] for r in rows],
+            "mc_nrmse": [r["mc_nrmse"] for r in rows], "surrogate_nrmse": [r["surrogate_nrmse"] for r in rows],
+            "reference_floor": [r["reference_floor"] for r in rows],
+            "fallback_fraction": float(np.mean([r["fallback_fraction"] for r in rows])), "roofline": roof}
 
 
 def bench_unet(a):
@@ -392,7 +463,10 @@ def main():
                     help="serve workload: decay streams per request and serving-loop step")
     ap.add_argument("--spot-every", type=int, default=20, help="serve workload: spot-check every k-th request")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="CPU baseline threads (0: the CPUs this process may use, capped by OMP_NUM_THREADS)")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="head workload: leave out the time-to-5%%-NRMSE object")
     ap.add_argument("--cpu-stride", type=int, default=0, help="CPU baseline voxel stride (0: 5 head, 50 torso)")
     ap.add_argument("--copy-calibration", type=float, default=0,
                     help="only run the library's device copy of this many bytes (HBM counter calibration) and exit")
@@ -422,17 +496,18 @@ def main():
     stride = a.cpu_stride or (50 if torso else 5)
     nd = f"{n_decays:.0e}".replace("e+0", "e").replace("e+", "e")
     if torso:
-        # 50 % beta+ and a prompt gamma per decay (at 514 keV: the material files end at 515 keV),
-        # 64-bit dose tallies over every voxel, a 46 cm detector around the 64 x 64 cm box
+        # 50 % beta+ and a 603 keV prompt gamma per decay (the interaction tables continued past
+        # the material files' 515 keV), 64-bit dose tallies over every voxel, a 46 cm detector
+        # around the 64 x 64 cm box
         mat, dens, act, vs = phantoms.torso(n_decays=n_decays)
         names = phantoms.torso_materials(MATERIALS)
         nz, ny, nx = mat.shape
         cfg = phantoms.make_config(seed=20261202, threads_per_block=1, acquisition_time_s=10.0, nrad=168, radius=-46.0,
                                    materials=names, positron_fraction=0.5, prompt_gamma_fraction=1.0,
-                                   prompt_gamma_energy=514.0e3, material_dose=True, voxel_dose=True,
+                                   prompt_gamma_energy=603.0e3, material_dose=True, voxel_dose=True,
                                    roi=(0, nx - 1, 0, ny - 1, 0, nz - 1),
                                    stream_decays=int(max(4, np.ceil(n_decays / 2.0 ** 20))))
-        workload = (f"torso phantom 256x256x200 at 2.5 mm, 7 tissues, 50% beta+ and a 514 keV prompt gamma per decay, "
+        workload = (f"torso phantom 256x256x200 at 2.5 mm, 7 tissues, 50% beta+ and a 603 keV prompt gamma per decay, "
                     f"{nd} decays in 10 s, sinogram 168x168x1293, 64-bit voxel and material dose, fused "
                     f"transport + tallies")
     else:
@@ -444,7 +519,8 @@ def main():
                     f"sinogram 168x168x1293, fused transport + tallies")
     cfg.voxel_format = a.voxel_format
     ph = mcpet.phantom_from_arrays(mat, dens, act, vs)
-    tab = mcpet.load_materials(names, ph.density_max())
+    e_max = 610.0e3 if torso else 0.0
+    tab = mcpet.load_materials(names, ph.density_max(), e_max=e_max)
     kernel = mcpet.KERNEL_PERSISTENT if a.kernel == "persistent" else mcpet.KERNEL_PLAIN
     sim = mcpet.Simulation(cfg, ph, tab)
     run_kw = dict(kernel=kernel, block_size=a.block_size, blocks_per_cu=a.blocks_per_cu, phase_policy=a.phase_policy,
@@ -512,7 +588,9 @@ def main():
                               for n, e, l, c in zip(("decay", "source2", "step", "interact", "tally", "ctau", "cpz", "detect", "source3"),
                                                     agg["phase_exec"], agg["phase_lanes"], agg["phase_cycles"])}
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(cfg, ph, vs, names, a.cpu_threads, stride)
+        out["cpu_baseline"] = cpu_baseline(cfg, ph, vs, names, baseline_threads(a.cpu_threads), stride, e_max)
+    if not torso and not a.no_surrogate:
+        out["time_to_5pct_nrmse"] = time_to_5pct_object(a)
     print(json.dumps(out))
 
 

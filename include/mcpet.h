@@ -166,6 +166,8 @@ typedef struct mcpet_tables {
   mcpet_rayleigh_table* rayleigh;
   mcpet_compton_table* compton;
   int32_t owned;
+  int32_t num_file_values;       /* grid points read from the files; points beyond are extrapolated
+                                    (mcpet_load_materials_extended), 0 when none are   */
 } mcpet_tables;
 
 /* Results copied back by mcpet_fetch (reference D2H copies, MCGPU-PET.cu:499-535,642-652). */
@@ -277,6 +279,23 @@ void mcpet_free_phantom(mcpet_phantom* ph);
 mcpet_status mcpet_load_materials(const char names[MCPET_MAX_MATERIALS][MCPET_NAME_LEN],
                                   const float density_max[MCPET_MAX_MATERIALS], mcpet_tables* out,
                                   char* err, size_t errlen);
+/* The same tables with the energy grid continued past the files' last energy up to e_max_ev
+ * (no reference counterpart: the reference reads past its tables there, K:302,359, and only
+ * checks 511 keV, C:344-351). The grid keeps the files' first energy and step; every added
+ * point is built from the files' last point (E_L) as raw mean free paths, then through the same
+ * float arithmetic as the file points:
+ *   Compton        mfp(E) = mfp(E_L) sigma_KN(E_L) / sigma_KN(E)   (Klein-Nishina total cross
+ *                  section per electron; the files' Compton data follow its scaling within 2e-4
+ *                  between 450 and 515 keV)
+ *   Rayleigh, photoelectric: mfp(E) = mfp(E_L) (E / E_L)^s, s the log-log slope between E_L
+ *                  and the file point nearest E_L / 2 (about 1.99 and 2.9)
+ *   total          1 / (1/Rayleigh + 1/Compton + 1/photoelectric)  (as in the files below 1.022 MeV)
+ *   Rayleigh pmax  the last file value (the form-factor cumulative has saturated, 1 - 3e-8)
+ * e_max_ev <= the last file energy: the files' grid unchanged. At most
+ * MCPET_MAX_ENERGYBINS_RAYLEIGH points in all. */
+mcpet_status mcpet_load_materials_extended(const char names[MCPET_MAX_MATERIALS][MCPET_NAME_LEN],
+                                           const float density_max[MCPET_MAX_MATERIALS], double e_max_ev,
+                                           mcpet_tables* out, char* err, size_t errlen);
 void mcpet_free_tables(mcpet_tables* t);
 
 /* Sinogram sizes from (MRD, SPAN, NZS, NRAD, NANGLES): NSEG, NSINOS, NBINS (C:1176-1189). */
@@ -364,7 +383,8 @@ typedef struct mcpet_server mcpet_server;
 typedef struct mcpet_surrogate mcpet_surrogate;   /* the scatter network (below) */
 
 typedef enum mcpet_request_state {
-  MCPET_REQ_QUEUED = 0, MCPET_REQ_RUNNING = 1, MCPET_REQ_DONE = 2, MCPET_REQ_REJECTED = 3, MCPET_REQ_UNKNOWN = 4
+  MCPET_REQ_QUEUED = 0, MCPET_REQ_RUNNING = 1, MCPET_REQ_DONE = 2, MCPET_REQ_REJECTED = 3, MCPET_REQ_UNKNOWN = 4,
+  MCPET_REQ_FAILED = 5     /* admitted, but its context could not be created or its results not computed */
 } mcpet_request_state;
 
 typedef struct mcpet_server_stats {
@@ -375,12 +395,19 @@ typedef struct mcpet_server_stats {
   int32_t peak_running;
   double latency_p50_ms, latency_p99_ms, latency_max_ms;   /* submit -> done, completed requests */
   double busy_ms;                /* time spent in steps                                       */
+  int64_t failed;                /* requests that failed after admission                      */
 } mcpet_server_stats;
 
 mcpet_status mcpet_server_create(const mcpet_tables* tab, size_t hbm_budget_bytes, int32_t max_running,
                                  int64_t streams_per_step, mcpet_server** out);
-/* Queue a request (cfg and phantom are copied; seed 0: the config's). A request whose
- * device footprint alone exceeds the budget is rejected (MCPET_E_CAPACITY). */
+/* Queue a request (cfg and phantom are copied; seed 0: the config's). A request that
+ * mcpet_create would refuse (mcpet_create's argument and range checks, every voxel's material
+ * slot) or whose seed cannot be used is rejected with that status, and one whose device
+ * footprint alone exceeds the budget with MCPET_E_CAPACITY; a rejected request never enters the
+ * queue. A request that fails after admission (its context cannot be created, its results
+ * cannot be copied or computed) fails alone: its state becomes MCPET_REQ_FAILED, its device
+ * memory is released and the requests behind it proceed. Fetching a rejected or failed request
+ * returns its error status and message and forgets it. */
 mcpet_status mcpet_server_submit(mcpet_server* s, const mcpet_config* cfg, const mcpet_phantom* ph, int32_t seed,
                                  int64_t* request_id);
 /* One scheduling step (admission, one batched launch, completions); blocks until done. */
@@ -487,6 +514,26 @@ const char* mcpet_surrogate_last_error(const mcpet_surrogate* s);
 mcpet_status mcpet_write_outputs(const char* dir, const mcpet_config* cfg, const mcpet_phantom* ph,
                                  const mcpet_tables* tab, const mcpet_tallies* t,
                                  double time_elapsed_s, char* err, size_t errlen);
+
+/* The dose summary of report_voxels_dose (MCGPU-PET.cu:2476-2641) from the voxel dose tally of
+ * the ROI (t->voxel_edep: ROI voxels x fastest, (sum E, sum E^2) per voxel in eV): the energy
+ * deposited inside the ROI with voxels below 0.1 g/cm3 left out (C:2523-2527), the peak voxel
+ * dose over all materials, and per material slot the ROI mass, voxel count, energy (all ROI
+ * voxels of the slot, as C:2512-2515 sums them before the low-density exclusion), dose and
+ * peak voxel. Doses in eV/g per history, standard deviations as the reference computes them. */
+typedef struct mcpet_dose_report {
+  double total_energy_kev;                  /* inside the ROI, keV per history            */
+  double max_dose, max_dose_std_dev;        /* peak voxel dose of all materials           */
+  double mass[MCPET_MAX_MATERIALS];         /* g, ROI voxels of the slot                  */
+  double edep[MCPET_MAX_MATERIALS];         /* eV per history                             */
+  double dose[MCPET_MAX_MATERIALS], dose_std_dev[MCPET_MAX_MATERIALS];
+  double peak_dose[MCPET_MAX_MATERIALS], peak_std_dev[MCPET_MAX_MATERIALS];
+  int32_t peak_voxel[MCPET_MAX_MATERIALS][3];   /* 0-based voxel coordinates in the phantom  */
+  uint32_t voxels[MCPET_MAX_MATERIALS];     /* ROI voxels of the slot (0: not reported)    */
+} mcpet_dose_report;
+
+mcpet_status mcpet_voxel_dose_report(const mcpet_config* cfg, const mcpet_phantom* ph, const uint64_t* voxel_edep,
+                                     uint64_t total_histories, mcpet_dose_report* out);
 
 /* RANECU seeds (x, y) at the start of stream ids[i] for input seed `seed`, computed on the
  * host with the same jump-table skip-ahead as the kernels; equal to init_PRNG(ids[i], 555555,

@@ -49,6 +49,33 @@ static void report_materials_dose(const mcpet_config& cfg, const mcpet_phantom& 
   }
 }
 
+// The standard-output part of report_voxels_dose (MCGPU-PET.cu:2597-2641): energy in the ROI,
+// peak voxel dose and the per-material ROI dose table.
+static void report_voxels_dose(const mcpet_config& cfg, const mcpet_phantom& ph, const uint64_t* voxel_edep,
+                               uint64_t total_histories)
+{
+  mcpet_dose_report r;
+  if (mcpet_voxel_dose_report(&cfg, &ph, voxel_edep, total_histories, &r) != MCPET_OK) return;
+  printf("\n\n          *** VOXEL ROI DOSE TALLY REPORT ***\n\n");
+  printf("\n              Total energy absorved inside the dose deposition ROI: %.5lf keV/hist\n", r.total_energy_kev);
+  printf("              Maximum voxel dose (+-2 sigma): %lf +- %lf eV/g per history.\n", r.max_dose, r.max_dose_std_dev);
+  printf("              Dose deposited in the different materials inside the input ROI computed post-processing the 3D voxel dose results:\n\n");
+  printf("  [MAT]  [DOSE_ROI eV/g/hist]  [2*std_dev]  [Rel error %%]  [Peak voxel dose]  [2*std_dev]  [Rel error %%]  [Peak voxel coord]  [E_dep eV/hist]  [MASS_ROI g]  [NUM_VOXELS_ROI]\n");
+  printf(" ===============================================================================================================================================================================\n");
+  for (int i = 0; i < MCPET_MAX_MATERIALS; i++) {
+    if (!r.voxels[i]) continue;
+    double rel = 0.0, rel_peak = 0.0;
+    if (r.dose[i] > 0.0) {
+      rel = r.dose_std_dev[i] / r.dose[i];
+      rel_peak = r.peak_std_dev[i] / r.peak_dose[i];
+    }
+    printf("\t%d\t%.5lf\t%.5lf\t%.3lf\t\t%.5lf\t%.5lf\t%.3lf\t(%d,%d,%d)\t\t%.5lf\t%.5lf\t%u\n", i + 1, r.dose[i],
+           2.0 * r.dose_std_dev[i], 200.0 * rel, r.peak_dose[i], 2.0 * r.peak_std_dev[i], 200.0 * rel_peak, r.peak_voxel[i][0],
+           r.peak_voxel[i][1], r.peak_voxel[i][2], r.edep[i], r.mass[i], r.voxels[i]);
+  }
+  printf("\n");
+}
+
 int main(int argc, char** argv)
 {
   if (argc < 2) {
@@ -121,6 +148,7 @@ int main(int argc, char** argv)
            (long long)t.psf_requested, cfg.detector.psf_size);
   st = mcpet_write_outputs(nullptr, &cfg, &ph, &tab, &t, elapsed, err, sizeof err);
   if (st) { printf("\n\n   !!ERROR!! %s\n\n", err); return -3; }
+  if (n_roi > 0) report_voxels_dose(cfg, ph, vdose.data(), t.total_histories);
   if (cfg.material_dose) report_materials_dose(cfg, ph, tab, mdose.data(), t.total_histories);
   long long simt = 0, simsc = 0;
   for (long long v = 0; v < nvox; v++) { simt += imt[v]; simsc += imsc[v]; }

@@ -431,9 +431,49 @@ extern "C" void mcpet_free_tables(mcpet_tables* t)
   t->owned = 0;
 }
 
+// Klein-Nishina total cross section per electron, in units of 2 pi r_e^2, at photon energy
+// E (eV): k = E / (m_e c^2).
+static double klein_nishina(double e)
+{
+  const double k = e / 510998.95, l = std::log(1.0 + 2.0 * k);
+  return (1.0 + k) / (k * k) * (2.0 * (1.0 + k) / (1.0 + 2.0 * k) - l / k) + l / (2.0 * k) -
+         (1.0 + 3.0 * k) / ((1.0 + 2.0 * k) * (1.0 + 2.0 * k));
+}
+
+// Raw mean-free-path rows (energy, Rayleigh, Compton, photoelectric, total, pmax) of one
+// material past its file's last point: the continuation mcpet_load_materials_extended documents.
+static void extrapolate_rows(std::vector<double>& raw, int n_file, int n_total, double e0, double delta_e)
+{
+  const double* L = &raw[6 * (size_t)(n_file - 1)];
+  const double EL = L[0];
+  int h = (int)std::floor((0.5 * EL - e0) / delta_e + 0.5);
+  h = std::max(0, std::min(h, n_file - 2));
+  const double* Hh = &raw[6 * (size_t)h];
+  const double s_ray = std::log(L[1] / Hh[1]) / std::log(EL / Hh[0]);
+  const double s_pho = std::log(L[3] / Hh[3]) / std::log(EL / Hh[0]);
+  const double kn_last = klein_nishina(EL);
+  for (int i = n_file; i < n_total; i++) {
+    double* r = &raw[6 * (size_t)i];
+    const double e = e0 + i * delta_e;
+    r[0] = e;
+    r[1] = L[1] * std::pow(e / EL, s_ray);
+    r[2] = L[2] * kn_last / klein_nishina(e);
+    r[3] = L[3] * std::pow(e / EL, s_pho);
+    r[4] = 1.0 / (1.0 / r[1] + 1.0 / r[2] + 1.0 / r[3]);
+    r[5] = L[5];
+  }
+}
+
 extern "C" mcpet_status mcpet_load_materials(const char names[MCPET_MAX_MATERIALS][MCPET_NAME_LEN],
                                              const float density_max_in[MCPET_MAX_MATERIALS], mcpet_tables* T,
                                              char* err, size_t errlen)
+{
+  return mcpet_load_materials_extended(names, density_max_in, 0.0, T, err, errlen);
+}
+
+extern "C" mcpet_status mcpet_load_materials_extended(const char names[MCPET_MAX_MATERIALS][MCPET_NAME_LEN],
+                                                      const float density_max_in[MCPET_MAX_MATERIALS], double e_max,
+                                                      mcpet_tables* T, char* err, size_t errlen)
 {
   if (!names || !density_max_in || !T) return fail(MCPET_E_ARG, err, errlen, "mcpet_load_materials: null argument");
   memset(T, 0, sizeof *T);
@@ -452,7 +492,9 @@ extern "C" mcpet_status mcpet_load_materials(const char names[MCPET_MAX_MATERIAL
   }
   char line[MCPET_NAME_LEN + 2];
   double delta_e = -99999.0;
-  int N = 0;
+  int N = 0;        // file points
+  int NT = 0;       // file points + extrapolated points
+  std::vector<double> raw;
   for (int mat = 0; mat < MCPET_MAX_MATERIALS; mat++) {
     if (names[mat][0] == '\0' || names[mat][0] == '\n') continue;
     gzFile g = gzopen(names[mat], "rb");
@@ -475,36 +517,19 @@ extern "C" mcpet_status mcpet_load_materials(const char names[MCPET_MAX_MATERIAL
       if (nv > MCPET_MAX_ENERGYBINS_RAYLEIGH)
         BAD(MCPET_E_RANGE, "load_material: too many energy bins (%d > %d)", nv, MCPET_MAX_ENERGYBINS_RAYLEIGH);
       N = nv;
-      T->num_values = N;
-      T->woodcock = (float*)calloc(2 * (size_t)N, sizeof(float));
-      T->mfp_a = (float*)calloc(3 * (size_t)N * MCPET_MAX_MATERIALS, sizeof(float));
-      T->mfp_b = (float*)calloc(3 * (size_t)N * MCPET_MAX_MATERIALS, sizeof(float));
-      if (!T->woodcock || !T->mfp_a || !T->mfp_b) BAD(MCPET_E_NOMEM, "load_material: out of memory");
-      for (int i = 0; i < N; i++) T->woodcock[2 * i] = 99999999.99f;
     } else if (nv != N) {
       BAD(MCPET_E_FORMAT, "load_material: incorrect number of energy values in '%s': %d, expected %d", names[mat], nv, N);
     }
     rd.gets(line, MCPET_NAME_LEN);
     rd.gets(line, MCPET_NAME_LEN);
+    // the file's rows (raw doubles)
+    raw.assign(6 * (size_t)std::max(N, NT), 0.0);
     double e_last = -1.0;
-    float* A = T->mfp_a;
-    float* B = T->mfp_b;
-    const double dn = (double)T->density_nominal[mat];
     for (int i = 0; i < N; i++) {
-      double d_e, d_ray, d_com, d_pho, d_tot, d_pmax;
-      if (!rd.gets(line, MCPET_NAME_LEN) ||
-          sscanf(line, "  %le  %le  %le  %le  %le  %le", &d_e, &d_ray, &d_com, &d_pho, &d_tot, &d_pmax) != 6)
+      double* r = &raw[6 * (size_t)i];
+      if (!rd.gets(line, MCPET_NAME_LEN) || sscanf(line, "  %le  %le  %le  %le  %le  %le", &r[0], &r[1], &r[2], &r[3], &r[4], &r[5]) != 6)
         BAD(MCPET_E_FORMAT, "load_material: bad mean-free-path row %d in '%s'", i, names[mat]);
-      (void)d_pho;
-      // C:1682-1684: minimum total mfp over materials at their maximum density.
-      float temp_mfp = (float)(d_tot * dn / (double)density_max[mat]);
-      if (temp_mfp < T->woodcock[2 * i]) T->woodcock[2 * i] = temp_mfp;
-      // C:1688-1690: inverse mfp per unit nominal density.
-      const size_t bin = (size_t)i * MCPET_MAX_MATERIALS + mat;
-      A[3 * bin + 0] = (float)(1.0 / (d_tot * dn));
-      A[3 * bin + 1] = (float)(1.0 / (d_com * dn));
-      A[3 * bin + 2] = (float)(1.0 / (d_ray * dn));
-      T->rayleigh->pmax[bin] = (float)d_pmax;
+      const double d_e = r[0];
       if (i == 0 && mat == 0) T->e0 = (float)d_e;
       if (i == 0) {
         if (fabs(d_e - (double)T->e0) > 1.0e-9)
@@ -516,19 +541,53 @@ extern "C" mcpet_status mcpet_load_materials(const char names[MCPET_MAX_MATERIAL
       }
       e_last = d_e;
     }
+    if (mat == 0) {   // the grid: the files' points, continued up to e_max
+      NT = N;
+      if (e_max > e_last) {
+        const double extra = std::ceil((e_max - e_last) / delta_e);
+        if ((double)N + extra + 1.0 > (double)MCPET_MAX_ENERGYBINS_RAYLEIGH)
+          BAD(MCPET_E_RANGE, "load_material: %.1f eV needs more than %d energy points", e_max, MCPET_MAX_ENERGYBINS_RAYLEIGH);
+        NT = N + (int)extra + 1;   // one more point: Rayleigh sampling reads the next bin's pmax
+      }
+      T->num_values = NT;
+      T->num_file_values = NT > N ? N : 0;
+      T->woodcock = (float*)calloc(2 * (size_t)NT, sizeof(float));
+      T->mfp_a = (float*)calloc(3 * (size_t)NT * MCPET_MAX_MATERIALS, sizeof(float));
+      T->mfp_b = (float*)calloc(3 * (size_t)NT * MCPET_MAX_MATERIALS, sizeof(float));
+      if (!T->woodcock || !T->mfp_a || !T->mfp_b) BAD(MCPET_E_NOMEM, "load_material: out of memory");
+      for (int i = 0; i < NT; i++) T->woodcock[2 * i] = 99999999.99f;
+      raw.resize(6 * (size_t)NT);
+    }
+    if (NT > N) extrapolate_rows(raw, N, NT, (double)T->e0, delta_e);
+    float* A = T->mfp_a;
+    float* B = T->mfp_b;
+    const double dn = (double)T->density_nominal[mat];
+    for (int i = 0; i < NT; i++) {
+      const double* r = &raw[6 * (size_t)i];
+      const double d_ray = r[1], d_com = r[2], d_tot = r[4], d_pmax = r[5];
+      // C:1682-1684: minimum total mfp over materials at their maximum density.
+      float temp_mfp = (float)(d_tot * dn / (double)density_max[mat]);
+      if (temp_mfp < T->woodcock[2 * i]) T->woodcock[2 * i] = temp_mfp;
+      // C:1688-1690: inverse mfp per unit nominal density.
+      const size_t bin = (size_t)i * MCPET_MAX_MATERIALS + mat;
+      A[3 * bin + 0] = (float)(1.0 / (d_tot * dn));
+      A[3 * bin + 1] = (float)(1.0 / (d_com * dn));
+      A[3 * bin + 2] = (float)(1.0 / (d_ray * dn));
+      T->rayleigh->pmax[bin] = (float)d_pmax;
+    }
     T->ide = (float)(1.0 / delta_e);   // C:1725 (1.0f/delta_e evaluated in double)
     // C:1728-1736: slopes; the last bin repeats the previous slope.
-    for (int i = 0; i < N - 1; i++) {
+    for (int i = 0; i < NT - 1; i++) {
       const size_t bin = (size_t)i * MCPET_MAX_MATERIALS + mat;
       const size_t nxt = bin + MCPET_MAX_MATERIALS;
       for (int c = 0; c < 3; c++) B[3 * bin + c] = (float)((double)(A[3 * nxt + c] - A[3 * bin + c]) / delta_e);
     }
     {
-      const size_t last = (size_t)(N - 1) * MCPET_MAX_MATERIALS + mat, prev = last - MCPET_MAX_MATERIALS;
+      const size_t last = (size_t)(NT - 1) * MCPET_MAX_MATERIALS + mat, prev = last - MCPET_MAX_MATERIALS;
       for (int c = 0; c < 3; c++) B[3 * last + c] = B[3 * prev + c];
     }
     // C:1739-1746: re-base so that value = a + E*b.
-    for (int i = 0; i < N; i++) {
+    for (int i = 0; i < NT; i++) {
       const double e_i = (double)T->e0 + i * delta_e;
       const size_t bin = (size_t)i * MCPET_MAX_MATERIALS + mat;
       for (int c = 0; c < 3; c++) A[3 * bin + c] = (float)((double)A[3 * bin + c] - e_i * (double)B[3 * bin + c]);
@@ -579,10 +638,10 @@ extern "C" mcpet_status mcpet_load_materials(const char names[MCPET_MAX_MATERIAL
   }
   // Woodcock slope and re-base (C:1821-1829). The slope of the last point is not set by the
   // reference (heap memory); it is 0 here, which leaves the last point's value unchanged.
-  for (int i = 0; i < N - 1; i++)
+  for (int i = 0; i < NT - 1; i++)
     T->woodcock[2 * i + 1] = (float)((double)(T->woodcock[2 * (i + 1)] - T->woodcock[2 * i]) / delta_e);
-  T->woodcock[2 * (N - 1) + 1] = 0.0f;
-  for (int i = 0; i < N; i++)
+  T->woodcock[2 * (NT - 1) + 1] = 0.0f;
+  for (int i = 0; i < NT; i++)
     T->woodcock[2 * i] = (float)((double)T->woodcock[2 * i] - ((double)T->e0 + i * delta_e) * (double)T->woodcock[2 * i + 1]);
   return MCPET_OK;
 }
@@ -679,6 +738,95 @@ static mcpet_status write_psf(const std::string& path, const mcpet_config* cfg, 
   return MCPET_OK;
 }
 
+// One ROI voxel of report_voxels_dose (C:2505-2566): the voxel's dose and standard deviation
+// (eV/g per history) as the reference computes them, with voxels below 0.1 g/cm3 reporting 0.
+struct RoiVoxel {
+  int64_t geometry;    // voxel index in the phantom
+  int mat;             // material slot, 0-based
+  double mass;         // g
+  uint64_t e1, e2;     // tallied sum E and sum E^2 (eV), zeroed for low-density voxels
+  double dose, sd;
+};
+
+template <class F>
+static void for_each_roi_voxel(const mcpet_config* cfg, const mcpet_phantom* ph, const uint64_t* voxel_edep,
+                               uint64_t total_histories, F&& f)
+{
+  const int DX = cfg->roi_x_max - cfg->roi_x_min + 1, DY = cfg->roi_y_max - cfg->roi_y_min + 1,
+            DZ = cfg->roi_z_max - cfg->roi_z_min + 1;
+  const double inv_N = 1.0 / (double)(total_histories ? total_histories : 1);
+  const double voxel_volume = 1.0 / (((double)ph->inv_voxel_size[0]) * ((double)ph->inv_voxel_size[1]) * ((double)ph->inv_voxel_size[2]));
+  int64_t voxel = 0;
+  for (int k = 0; k < DZ; k++)
+    for (int j = 0; j < DY; j++)
+      for (int i = 0; i < DX; i++, voxel++) {
+        RoiVoxel v;
+        v.geometry = (int64_t)(i + cfg->roi_x_min) + (int64_t)(j + cfg->roi_y_min) * ph->nx +
+                     (int64_t)(k + cfg->roi_z_min) * ph->nx * ph->ny;
+        const float dens = ph->voxel_mat_dens[3 * v.geometry + 1];
+        v.mat = (int)ph->voxel_mat_dens[3 * v.geometry] - 1;
+        v.mass = dens * voxel_volume;
+        const double inv_mass = 1.0 / v.mass;
+        v.e1 = voxel_edep[2 * voxel];
+        v.e2 = voxel_edep[2 * voxel + 1];
+        const uint64_t raw1 = v.e1, raw2 = v.e2;
+        if (dens < 0.1f) { v.e1 = 0; v.e2 = 0; }   // C:2523-2527: no dose reported in low-density voxels
+        double dose = (double)v.e1 * inv_N;
+        double sd = ((double)v.e2 * inv_N - dose * dose) * inv_N * inv_mass;
+        if (sd > 0.0) sd = std::sqrt(sd);
+        v.dose = dose * inv_mass;
+        v.sd = sd;
+        f(i, j, k, v, raw1, raw2);
+      }
+}
+
+extern "C" mcpet_status mcpet_voxel_dose_report(const mcpet_config* cfg, const mcpet_phantom* ph, const uint64_t* voxel_edep,
+                                                uint64_t total_histories, mcpet_dose_report* out)
+{
+  if (!cfg || !ph || !ph->voxel_mat_dens || !voxel_edep || !out) return MCPET_E_ARG;
+  if (cfg->roi_x_min < 0 || cfg->roi_y_min < 0 || cfg->roi_z_min < 0 || cfg->roi_x_max >= ph->nx || cfg->roi_y_max >= ph->ny ||
+      cfg->roi_z_max >= ph->nz || cfg->roi_x_min > cfg->roi_x_max || cfg->roi_y_min > cfg->roi_y_max || cfg->roi_z_min > cfg->roi_z_max)
+    return MCPET_E_RANGE;
+  std::memset(out, 0, sizeof *out);
+  double edep2[MCPET_MAX_MATERIALS] = {0};
+  for (int m = 0; m < MCPET_MAX_MATERIALS; m++) {   // C:2488-2495
+    out->peak_dose[m] = -1.0;
+    out->peak_std_dev[m] = 1.0e-15;
+  }
+  uint64_t total = 0;
+  for_each_roi_voxel(cfg, ph, voxel_edep, total_histories, [&](int i, int j, int k, const RoiVoxel& v, uint64_t raw1, uint64_t raw2) {
+    if (v.mat < 0 || v.mat >= MCPET_MAX_MATERIALS) return;
+    out->mass[v.mat] += v.mass;                 // C:2512-2515: every ROI voxel of the slot, before the exclusion
+    out->edep[v.mat] += (double)raw1;
+    edep2[v.mat] += (double)raw2;
+    out->voxels[v.mat]++;
+    total += v.e1;
+    if (v.dose > out->peak_dose[v.mat]) {        // C:2541-2554
+      out->peak_dose[v.mat] = v.dose;
+      out->peak_std_dev[v.mat] = v.sd;
+      out->peak_voxel[v.mat][0] = i + cfg->roi_x_min;
+      out->peak_voxel[v.mat][1] = j + cfg->roi_y_min;
+      out->peak_voxel[v.mat][2] = k + cfg->roi_z_min;
+      if (v.dose > out->max_dose) {
+        out->max_dose = v.dose;
+        out->max_dose_std_dev = v.sd;
+      }
+    }
+  });
+  const double inv_N = 1.0 / (double)(total_histories ? total_histories : 1);
+  out->total_energy_kev = 0.001 * (double)total * inv_N;
+  for (int m = 0; m < MCPET_MAX_MATERIALS; m++) {   // C:2614-2636
+    if (!out->voxels[m]) continue;
+    const double E = out->edep[m] * inv_N;
+    double sd = (edep2[m] * inv_N - E * E) * inv_N;
+    if (sd > 0.0) sd = std::sqrt(sd);
+    out->edep[m] = E;
+    out->dose[m] = E / out->mass[m];
+    out->dose_std_dev[m] = sd / out->mass[m];
+  }
+  return MCPET_OK;
+}
+
 // report_voxels_dose (MCGPU-PET.cu:2389-2650): ASCII mid-plane, float dose .raw and 2-sigma % error .raw.
 static mcpet_status write_voxel_dose(const std::string& path, const mcpet_config* cfg, const mcpet_phantom* ph,
                                      const mcpet_tallies* t, double time_elapsed, char* err, size_t errlen)
@@ -699,6 +847,7 @@ static mcpet_status write_voxel_dose(const std::string& path, const mcpet_config
   fprintf(f, "#  The 3D dose deposition is reported in binary form in the .raw files (data given as 32-bit floats). \n");
   fprintf(f, "#  The %% relative error in the voxel dose at 2 standard deviations [=100*2*sigma/voxel_dose] is reported in the *_PercentRelError2sigma.raw file (32-bit floats). \n");
   fprintf(f, "#  This text output reports only the 2D dose at the Z plane z_coord = %d (z_coord in ROI = %d)\n#\n", z_plane, z_plane_roi);
+  fprintf(f, "#  The total dose deposited in each material inside the ROI is reported to the standard output.\n#\n");
   fprintf(f, "#  Voxel size:  %lf x %lf x %lf = %lf cm^3\n", 1.0 / (double)ph->inv_voxel_size[0], 1.0 / (double)ph->inv_voxel_size[1],
           1.0 / (double)ph->inv_voxel_size[2],
           1.0 / (double)(ph->inv_voxel_size[0] * ph->inv_voxel_size[1] * ph->inv_voxel_size[2]));
@@ -708,42 +857,31 @@ static mcpet_status write_voxel_dose(const std::string& path, const mcpet_config
   fprintf(f, "#  Voxel dose units: eV/g per history\n");
   fprintf(f, "#  X rows given first, then Y, then Z. One blank line separates the different Y, and two blanks the Z values (GNUPLOT format).\n# \n");
   fprintf(f, "#    [DOSE]   [2*standard_deviation]\n# =====================================\n");
-  const double inv_N = 1.0 / (double)(t->total_histories ? t->total_histories : 1);
-  const double voxel_volume = 1.0 / (((double)ph->inv_voxel_size[0]) * ((double)ph->inv_voxel_size[1]) * ((double)ph->inv_voxel_size[2]));
   uint64_t total_energy = 0;
-  int64_t voxel = 0;
-  for (int k = 0; k < DZ; k++) {
-    for (int j = 0; j < DY; j++) {
-      for (int i = 0; i < DX; i++) {
-        const int64_t vg = (int64_t)(i + cfg->roi_x_min) + (int64_t)(j + cfg->roi_y_min) * ph->nx +
-                           (int64_t)(k + cfg->roi_z_min) * ph->nx * ph->ny;
-        const float dens = ph->voxel_mat_dens[3 * vg + 1];
-        const double inv_mass = 1.0 / (dens * voxel_volume);
-        uint64_t e1 = t->voxel_edep[2 * voxel], e2 = t->voxel_edep[2 * voxel + 1];
-        if (dens < 0.1f) { e1 = 0; e2 = 0; }   // C:2523-2527: no dose reported in low-density voxels
-        double dose = (double)e1 * inv_N;
-        total_energy += e1;
-        double sd = ((double)e2 * inv_N - dose * dose) * inv_N * inv_mass;
-        if (sd > 0.0) sd = sqrt(sd);
-        dose *= inv_mass;
-        if (k == z_plane_roi) fprintf(f, "%.6lf %.6lf\n", dose, 2.0 * sd);
-        float df = (float)dose, rel = 0.0f;
-        if (dose > 0.0) rel = 200.0f * (float)(sd / dose);
-        fwrite(&df, sizeof(float), 1, fm);
-        fwrite(&rel, sizeof(float), 1, fs);
-        voxel++;
-      }
-      if (k == z_plane_roi) fprintf(f, "\n");
+  bool ok = true;
+  for_each_roi_voxel(cfg, ph, t->voxel_edep, t->total_histories, [&](int i, int j, int k, const RoiVoxel& v, uint64_t, uint64_t) {
+    total_energy += v.e1;
+    if (k == z_plane_roi) fprintf(f, "%.6lf %.6lf\n", v.dose, 2.0 * v.sd);
+    const float df = (float)v.dose;
+    const float rel = v.dose > 0.0 ? 200.0f * (float)(v.sd / v.dose) : 0.0f;
+    ok = ok && fwrite(&df, sizeof(float), 1, fm) == 1 && fwrite(&rel, sizeof(float), 1, fs) == 1;
+    if (k == z_plane_roi && i == DX - 1) {
+      fprintf(f, "\n");                       // Y rows apart
+      if (j == DY - 1) fprintf(f, "\n");      // then the plane
     }
-    if (k == z_plane_roi) fprintf(f, "\n");
-  }
-  fprintf(f, "#   ****** DOSE REPORT ******\n");
+  });
+  const double inv_N = 1.0 / (double)(t->total_histories ? t->total_histories : 1);
+  // C:2586-2595 (one projection: a PET acquisition)
+  fprintf(f, "#   ****** DOSE REPORT: TOTAL SIMULATION PERFORMANCE FOR ALL PROJECTIONS ******\n");
   fprintf(f, "#       Total number of simulated x rays: %llu\n", (unsigned long long)t->total_histories);
+  fprintf(f, "#       Simulated x rays per projection:  %llu\n", (unsigned long long)t->total_histories);
   fprintf(f, "#       Total simulation time [s]:  %.2f\n", time_elapsed);
+  if (time_elapsed > 0.000001) fprintf(f, "#       Total speed [x-rays/s]:  %.2f\n", (double)t->total_histories / time_elapsed);
   fprintf(f, "\n#       Total energy absorved inside the dose ROI: %.5lf keV/hist\n\n", 0.001 * (double)total_energy * inv_N);
   fclose(f);
-  fclose(fm);
-  fclose(fs);
+  ok = (fclose(fm) == 0) && ok;
+  ok = (fclose(fs) == 0) && ok;
+  if (!ok) return fail(MCPET_E_IO, err, errlen, "report_voxels_dose: write error on %s.raw", path.c_str());
   return MCPET_OK;
 }
 

@@ -158,13 +158,15 @@ void free_all(mcpet_ctx* c)
 
 }  // namespace
 
+bool mcpet_seed_usable(int seed) { return seed_usable(seed); }
+
 extern "C" const char* mcpet_version(void) { return "mcpet-mi355x 0.2 (gfx950)"; }
 
 extern "C" int32_t mcpet_struct_sizes(size_t* out, int32_t n)
 {
   const size_t sz[] = {sizeof(mcpet_config),      sizeof(mcpet_phantom), sizeof(mcpet_tables),
                        sizeof(mcpet_tallies),     sizeof(mcpet_run_params), sizeof(mcpet_stats),
-                       sizeof(mcpet_psf_record),  sizeof(mcpet_single)};
+                       sizeof(mcpet_psf_record),  sizeof(mcpet_single), sizeof(mcpet_server_stats)};
   int32_t k = 0;
   for (; out && k < n && k < (int32_t)(sizeof(sz) / sizeof(sz[0])); k++) out[k] = sz[k];
   return k;
@@ -178,10 +180,12 @@ extern "C" const char* mcpet_last_error(const mcpet_ctx* ctx)
   return g_err.c_str();
 }
 
-extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phantom* ph, const mcpet_tables* tab, mcpet_ctx** out)
+// The argument and range checks of mcpet_create that need no device and no allocation (the
+// serving loop runs them when a request is submitted, so that a request that cannot be created
+// is rejected then and never blocks the queue).
+mcpet_status mcpet_check_create_args(const mcpet_config* cfg, const mcpet_phantom* ph, const mcpet_tables* tab,
+                                     bool scan_voxels)
 {
-  if (!out) return set_err(nullptr, MCPET_E_ARG, "mcpet_create: null output pointer");
-  *out = nullptr;
   if (!cfg || !ph || !tab || !ph->voxel_mat_dens || !tab->woodcock || !tab->mfp_a || !tab->mfp_b || !tab->rayleigh || !tab->compton)
     return set_err(nullptr, MCPET_E_ARG, "mcpet_create: null argument");
   if (cfg->threads_per_block < 1) return set_err(nullptr, MCPET_E_RANGE, "mcpet_create: threads_per_block must be >= 1");
@@ -217,6 +221,29 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
       return set_err(nullptr, MCPET_E_RANGE, "mcpet_create: coincidence_window_ps must be > 0");
     if ((long long)ph->nx * ph->ny * ph->nz * (long long)cfg->threads_per_block >= (1LL << 32))
       return set_err(nullptr, MCPET_E_RANGE, "mcpet_create: voxels x threads_per_block must be < 2^32 for singles event ids");
+  }
+  if (cfg->voxel_dose && cfg->roi_x_max > -1 &&
+      (cfg->roi_x_min > cfg->roi_x_max || cfg->roi_y_min > cfg->roi_y_max || cfg->roi_z_min > cfg->roi_z_max ||
+       cfg->roi_x_min < 0 || cfg->roi_y_min < 0 || cfg->roi_z_min < 0 || cfg->roi_x_max >= ph->nx ||
+       cfg->roi_y_max >= ph->ny || cfg->roi_z_max >= ph->nz))
+    return set_err(nullptr, MCPET_E_RANGE, "mcpet_create: dose region of interest outside the phantom");
+  if (scan_voxels) {
+    const long long nvox = (long long)ph->nx * ph->ny * ph->nz;
+    for (long long v = 0; v < nvox; v++) {
+      const int m0 = (int)(ph->voxel_mat_dens[3 * v] - 1);
+      if (m0 < 0 || m0 >= MCPET_MAX_MATERIALS) return set_err(nullptr, MCPET_E_RANGE, "mcpet_create: voxel material out of range");
+    }
+  }
+  return MCPET_OK;
+}
+
+extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phantom* ph, const mcpet_tables* tab, mcpet_ctx** out)
+{
+  if (!out) return set_err(nullptr, MCPET_E_ARG, "mcpet_create: null output pointer");
+  *out = nullptr;
+  {
+    const mcpet_status st0 = mcpet_check_create_args(cfg, ph, tab, false);   // voxels are checked while packing
+    if (st0 != MCPET_OK) return st0;
   }
   mcpet_ctx* c = new mcpet_ctx();
   c->cfg = *cfg;
@@ -422,14 +449,7 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
 
   // ROI (dose) size.
   const bool voxel_dose = cfg->voxel_dose && cfg->roi_x_max > -1;
-  if (voxel_dose) {
-    if (cfg->roi_x_min > cfg->roi_x_max || cfg->roi_y_min > cfg->roi_y_max || cfg->roi_z_min > cfg->roi_z_max ||
-        cfg->roi_x_min < 0 || cfg->roi_y_min < 0 || cfg->roi_z_min < 0 || cfg->roi_x_max >= ph->nx || cfg->roi_y_max >= ph->ny ||
-        cfg->roi_z_max >= ph->nz) {
-      free_all(c);
-      delete c;
-      return set_err(nullptr, MCPET_E_RANGE, "mcpet_create: dose region of interest outside the phantom");
-    }
+  if (voxel_dose) {   // inside the phantom (checked above)
     c->n_roi = (long long)(cfg->roi_x_max - cfg->roi_x_min + 1) * (cfg->roi_y_max - cfg->roi_y_min + 1) *
                (cfg->roi_z_max - cfg->roi_z_min + 1);
   }

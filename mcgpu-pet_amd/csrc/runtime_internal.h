@@ -17,6 +17,13 @@ struct CtxRun {
   int seed;        // 0: the context's seed
 };
 
+// mcpet_create's checks that need neither the device nor an allocation (scan_voxels: also
+// every voxel's material slot); on failure the status and the library's last error are set.
+mcpet_status mcpet_check_create_args(const mcpet_config* cfg, const mcpet_phantom* ph, const mcpet_tables* tab,
+                                     bool scan_voxels);
+// Whether a run can use this seed (abMODm divides by zero for 0 and for values below -m2).
+bool mcpet_seed_usable(int seed);
+
 size_t mcpet_ctx_device_bytes(const mcpet_ctx* c);
 long long mcpet_ctx_streams(const mcpet_ctx* c);
 int mcpet_ctx_device(const mcpet_ctx* c);

@@ -33,6 +33,8 @@ struct Request {
   std::vector<float> vmd;            // the request's own copy of the voxels
   int32_t seed = 0;
   int state = MCPET_REQ_QUEUED;
+  mcpet_status fail_status = MCPET_OK;   // a rejected or failed request: why
+  std::string fail_msg;
   size_t footprint = 0;              // device bytes reserved at admission
   mcpet_ctx* ctx = nullptr;
   long long next = 0, streams = 0;
@@ -155,6 +157,14 @@ static mcpet_status submit(mcpet_server* s, const mcpet_config* cfg, const mcpet
   r->ph.owned = 0;
   r->seed = seed;
   r->net = net;
+  // what mcpet_create and the run would refuse is refused now, so the request never
+  // reaches the head of the queue (where a failing creation would be retried every step)
+  mcpet_status vst = mcpet_check_create_args(&r->cfg, &r->ph, s->tab, true);
+  std::string vmsg = vst != MCPET_OK ? std::string(mcpet_last_error(nullptr)) : std::string();
+  if (vst == MCPET_OK && !mcpet_seed_usable(seed != 0 ? seed : r->cfg.seed)) {
+    vst = MCPET_E_ARG;
+    vmsg = "mcpet_server_submit: unusable seed (0 or below -2147483398)";
+  }
   if (grid) {
     std::memcpy(r->grid, grid, sizeof r->grid);
     r->maps = grid[3];
@@ -170,11 +180,17 @@ static mcpet_status submit(mcpet_server* s, const mcpet_config* cfg, const mcpet
   r->submitted = Clock::now();
   s->st.submitted++;
   if (request_id) *request_id = r->id;
-  if (r->footprint > s->budget) {
+  if (vst == MCPET_OK && r->footprint > s->budget) {
+    vst = MCPET_E_CAPACITY;
+    vmsg = "mcpet_server_submit: the request needs more device memory than the budget";
+  }
+  if (vst != MCPET_OK) {
     r->state = MCPET_REQ_REJECTED;
+    r->fail_status = vst;
+    r->fail_msg = vmsg;
     s->st.rejected++;
     s->req[r->id] = std::move(r);
-    return sfail(s, MCPET_E_CAPACITY, "mcpet_server_submit: the request needs more device memory than the budget");
+    return sfail(s, vst, vmsg);
   }
   s->queue.push_back(r->id);
   s->req[r->id] = std::move(r);
@@ -218,6 +234,34 @@ static void complete(mcpet_server* s, Request& r)
   r.done = Clock::now();
   s->latencies.push_back(std::chrono::duration<double, std::milli>(r.done - r.submitted).count());
   s->st.completed++;
+}
+
+// A request that could not be created or finished: its context and device memory are released,
+// it leaves the queue or the running set, and its fetch reports why.
+static void fail_request(mcpet_server* s, Request& r, mcpet_status st, const std::string& why)
+{
+  if (r.ctx) mcpet_destroy(r.ctx);
+  r.ctx = nullptr;
+  if (r.state == MCPET_REQ_RUNNING) s->admitted -= r.footprint;
+  r.state = MCPET_REQ_FAILED;
+  r.fail_status = st;
+  r.fail_msg = why;
+  r.done = Clock::now();
+  s->st.failed++;
+  s->err = why;
+}
+
+// fetch of a rejected or failed request: its status and message; the request is forgotten.
+static bool fetch_refused(mcpet_server* s, std::map<int64_t, std::unique_ptr<Request>>::iterator it, mcpet_status* st)
+{
+  if (it == s->req.end()) return false;
+  const Request& r = *it->second;
+  if (r.state != MCPET_REQ_REJECTED && r.state != MCPET_REQ_FAILED) return false;
+  const std::string m = "request " + std::to_string(r.id) + (r.state == MCPET_REQ_REJECTED ? " rejected: " : " failed: ") +
+                        r.fail_msg;
+  *st = sfail(s, r.fail_status, m);
+  s->req.erase(it);
+  return true;
 }
 
 static mcpet_status finish_rebinned(mcpet_server* s, Request& r)
@@ -330,8 +374,9 @@ extern "C" mcpet_status mcpet_server_step(mcpet_server* s, int32_t* running, int
     if (nt > 0) worker();
     for (auto& t : pool) t.join();
   }
-  for (size_t i = 0; i < adm.size(); i++) {
+  for (size_t i = 0; i < adm.size(); i++) {   // adm is the queue's head, in order
     Request& r = *s->req[adm[i]];
+    s->queue.pop_front();
     std::string why;
     mcpet_status st = cst[i];
     if (st != MCPET_OK) {
@@ -340,21 +385,15 @@ extern "C" mcpet_status mcpet_server_step(mcpet_server* s, int32_t* running, int
       st = MCPET_E_STATE;
       why = "mcpet_server_step: device footprint above its estimate";
     }
-    if (st != MCPET_OK) {
-      // this request and the later ones stay queued; their contexts are released
-      for (size_t j = i; j < adm.size(); j++) {
-        Request& q = *s->req[adm[j]];
-        if (q.ctx) mcpet_destroy(q.ctx);
-        q.ctx = nullptr;
-      }
-      return sfail(s, st, why);
+    if (st != MCPET_OK) {   // only this request fails; the ones behind it are admitted as usual
+      fail_request(s, r, st, why);
+      continue;
     }
     r.streams = mcpet_ctx_streams(r.ctx);
     r.next = 0;
     r.state = MCPET_REQ_RUNNING;
     s->admitted += r.footprint;
     s->running.push_back(r.id);
-    s->queue.pop_front();
   }
   s->st.peak_admitted_bytes = std::max(s->st.peak_admitted_bytes, s->admitted);
   s->st.peak_running = std::max(s->st.peak_running, (int32_t)s->running.size());
@@ -373,15 +412,18 @@ extern "C" mcpet_status mcpet_server_step(mcpet_server* s, int32_t* running, int
     if (st != MCPET_OK) return sfail(s, st, e);
     if (hipStreamSynchronize(s->stream) != hipSuccess) return sfail(s, MCPET_E_HIP, "mcpet_server_step: launch failed");
     s->st.steps++;
-    // 3. completions free their device memory before the next admission
+    // 3. completions free their device memory before the next admission; a request whose
+    //    results cannot be copied or computed fails alone
     std::vector<int64_t> still;
     for (size_t i = 0; i < s->running.size(); i++) {
       Request& r = *s->req[s->running[i]];
       r.next = runs[i].end;
       if (r.next >= r.streams) {
-        st = finish(s, r);
-        if (st != MCPET_OK) return st;
-        ndone++;
+        const mcpet_status fst = finish(s, r);
+        if (fst == MCPET_OK)
+          ndone++;
+        else
+          fail_request(s, r, fst, s->err);
       } else {
         still.push_back(r.id);
       }
@@ -416,6 +458,8 @@ extern "C" mcpet_status mcpet_server_fetch(mcpet_server* s, int64_t id, mcpet_ta
 {
   if (!s || !out) return sfail(s, MCPET_E_ARG, "mcpet_server_fetch: null argument");
   auto it = s->req.find(id);
+  mcpet_status fst;
+  if (fetch_refused(s, it, &fst)) return fst;
   if (it == s->req.end() || it->second->state != MCPET_REQ_DONE)
     return sfail(s, MCPET_E_STATE, "mcpet_server_fetch: request unknown or not finished");
   Request& r = *it->second;
@@ -447,6 +491,8 @@ extern "C" mcpet_status mcpet_server_fetch_rebinned(mcpet_server* s, int64_t id,
 {
   if (!s) return sfail(s, MCPET_E_ARG, "mcpet_server_fetch_rebinned: null server");
   auto it = s->req.find(id);
+  mcpet_status fst;
+  if (fetch_refused(s, it, &fst)) return fst;
   if (it == s->req.end() || it->second->state != MCPET_REQ_DONE || it->second->grid[0] == 0 || it->second->net)
     return sfail(s, MCPET_E_STATE, "mcpet_server_fetch_rebinned: request unknown, not finished or not rebinned");
   Request& r = *it->second;
@@ -466,6 +512,8 @@ extern "C" mcpet_status mcpet_server_fetch_estimate(mcpet_server* s, int64_t id,
 {
   if (!s) return sfail(s, MCPET_E_ARG, "mcpet_server_fetch_estimate: null server");
   auto it = s->req.find(id);
+  mcpet_status fst;
+  if (fetch_refused(s, it, &fst)) return fst;
   if (it == s->req.end() || it->second->state != MCPET_REQ_DONE || !it->second->net)
     return sfail(s, MCPET_E_STATE, "mcpet_server_fetch_estimate: request unknown, not finished or not an estimate");
   Request& r = *it->second;

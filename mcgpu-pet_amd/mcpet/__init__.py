@@ -95,7 +95,8 @@ class Tables(C.Structure):
     _fields_ = [("num_values", C.c_int32), ("e0", C.c_float), ("ide", C.c_float),
                 ("density_nominal", C.c_float * MAX_MATERIALS), ("woodcock", C.POINTER(C.c_float)),
                 ("mfp_a", C.POINTER(C.c_float)), ("mfp_b", C.POINTER(C.c_float)),
-                ("rayleigh", C.POINTER(RayleighTable)), ("compton", C.POINTER(ComptonTable)), ("owned", C.c_int32)]
+                ("rayleigh", C.POINTER(RayleighTable)), ("compton", C.POINTER(ComptonTable)), ("owned", C.c_int32),
+                ("num_file_values", C.c_int32)]
 
 
 class Tallies(C.Structure):
@@ -158,6 +159,7 @@ def lib():
     L.mcpet_free_phantom.argtypes = [P(Phantom)]
     L.mcpet_free_phantom.restype = None
     L.mcpet_load_materials.argtypes = [C.c_void_p, P(C.c_float), P(Tables), C.c_char_p, C.c_size_t]
+    L.mcpet_load_materials_extended.argtypes = [C.c_void_p, P(C.c_float), C.c_double, P(Tables), C.c_char_p, C.c_size_t]
     L.mcpet_free_tables.argtypes = [P(Tables)]
     L.mcpet_free_tables.restype = None
     L.mcpet_sinogram_sizes.argtypes = [P(Config)]
@@ -172,6 +174,7 @@ def lib():
     L.mcpet_last_error.argtypes = [C.c_void_p]
     L.mcpet_last_error.restype = C.c_char_p
     L.mcpet_get_detector.argtypes = [C.c_void_p, P(Detector)]
+    L.mcpet_voxel_dose_report.argtypes = [P(Config), P(Phantom), P(C.c_uint64), C.c_uint64, C.c_void_p]
     L.mcpet_write_outputs.argtypes = [C.c_char_p, P(Config), P(Phantom), P(Tables), P(Tallies), C.c_double, C.c_char_p,
                                       C.c_size_t]
     L.mcpet_selftest_ranecu.argtypes = [C.c_int32, P(C.c_int64), C.c_int32, C.c_int32, P(C.c_float), P(C.c_double),
@@ -334,8 +337,9 @@ def _names_buffer(names):
     return buf
 
 
-def load_materials(names, density_max):
-    """names: up to 15 material file paths ('' for unused slots)."""
+def load_materials(names, density_max, e_max=0.0):
+    """names: up to 15 material file paths ('' for unused slots). e_max (eV) above the files'
+    last energy: the grid continued up to e_max (mcpet_load_materials_extended)."""
     if isinstance(names, Config):
         names = [names.material_files[i].value.decode() for i in range(MAX_MATERIALS)]
     names = list(names) + [""] * (MAX_MATERIALS - len(names))
@@ -343,7 +347,8 @@ def load_materials(names, density_max):
     t = Tables()
     err = C.create_string_buffer(1024)
     buf = _names_buffer(names)
-    _check(lib().mcpet_load_materials(C.cast(buf, C.c_void_p), _fptr(dm), C.byref(t), err, 1024), err=err)
+    _check(lib().mcpet_load_materials_extended(C.cast(buf, C.c_void_p), _fptr(dm), float(e_max), C.byref(t), err, 1024),
+           err=err)
     return TablesHandle(t)
 
 
@@ -360,6 +365,8 @@ class TablesHandle:
             "woodcock": np.ctypeslib.as_array(t.woodcock, shape=(n, 2)).copy(),
             "mfp_a": np.ctypeslib.as_array(t.mfp_a, shape=(n, MAX_MATERIALS, 3)).copy(),
             "mfp_b": np.ctypeslib.as_array(t.mfp_b, shape=(n, MAX_MATERIALS, 3)).copy(),
+            "pmax": np.ctypeslib.as_array(t.rayleigh.contents.pmax)[:n * MAX_MATERIALS].reshape(n, MAX_MATERIALS).copy(),
+            "num_file_values": t.num_file_values,
         }
 
     def close(self):
@@ -519,10 +526,11 @@ class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("rejected", C.c_int64), ("steps", C.c_int64),
                 ("queued", C.c_int32), ("running", C.c_int32), ("budget_bytes", C.c_size_t),
                 ("peak_admitted_bytes", C.c_size_t), ("peak_running", C.c_int32), ("latency_p50_ms", C.c_double),
-                ("latency_p99_ms", C.c_double), ("latency_max_ms", C.c_double), ("busy_ms", C.c_double)]
+                ("latency_p99_ms", C.c_double), ("latency_max_ms", C.c_double), ("busy_ms", C.c_double),
+                ("failed", C.c_int64)]
 
 
-REQ_STATES = {0: "queued", 1: "running", 2: "done", 3: "rejected", 4: "unknown"}
+REQ_STATES = {0: "queued", 1: "running", 2: "done", 3: "rejected", 4: "unknown", 5: "failed"}
 
 
 class Server:
@@ -746,6 +754,29 @@ def write_outputs(directory, cfg, phantom, tables, tallies, time_elapsed=0.0):
                                      C.byref(t), float(time_elapsed), err, 1024), err=err)
 
 
+class DoseReport(C.Structure):
+    _fields_ = [("total_energy_kev", C.c_double), ("max_dose", C.c_double), ("max_dose_std_dev", C.c_double),
+                ("mass", C.c_double * MAX_MATERIALS), ("edep", C.c_double * MAX_MATERIALS),
+                ("dose", C.c_double * MAX_MATERIALS), ("dose_std_dev", C.c_double * MAX_MATERIALS),
+                ("peak_dose", C.c_double * MAX_MATERIALS), ("peak_std_dev", C.c_double * MAX_MATERIALS),
+                ("peak_voxel", (C.c_int32 * 3) * MAX_MATERIALS), ("voxels", C.c_uint32 * MAX_MATERIALS)]
+
+
+def voxel_dose_report(cfg, phantom, voxel_edep, total_histories):
+    """The ROI dose summary of the reference's voxel dose report (mcpet_voxel_dose_report) as a
+    dict: total_energy_kev, max_dose, max_dose_std_dev and per material slot (lists of 15) mass,
+    edep, dose, dose_std_dev, peak_dose, peak_std_dev, peak_voxel, voxels."""
+    v = np.ascontiguousarray(voxel_edep, dtype=np.uint64)
+    r = DoseReport()
+    _check(lib().mcpet_voxel_dose_report(C.byref(cfg), C.byref(phantom.ph), v.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                         int(total_histories), C.byref(r)))
+    out = {k: getattr(r, k) for k in ("total_energy_kev", "max_dose", "max_dose_std_dev")}
+    for k in ("mass", "edep", "dose", "dose_std_dev", "peak_dose", "peak_std_dev", "voxels"):
+        out[k] = list(getattr(r, k))
+    out["peak_voxel"] = [tuple(r.peak_voxel[m]) for m in range(MAX_MATERIALS)]
+    return out
+
+
 def selftest_ranecu(seed, stream_ids, n, as_double=False):
     ids = np.ascontiguousarray(stream_ids, dtype=np.int64)
     if as_double:
@@ -915,8 +946,8 @@ TALLY_SINGLES = 3
 
 def struct_sizes():
     """sizeof of the C structs as compiled into the library (see mcpet_struct_sizes)."""
-    out = (C.c_size_t * 8)()
-    n = lib().mcpet_struct_sizes(out, 8)
+    out = (C.c_size_t * 16)()
+    n = lib().mcpet_struct_sizes(out, 16)
     return list(out)[:n]
 
 

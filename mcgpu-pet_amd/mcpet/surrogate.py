@@ -260,8 +260,28 @@ class ScatterService:
 
     def step(self):
         self.server.step()
-        for rid in [r for r in list(self.by_rid) if self.server.poll(r) == "done"]:
-            self._complete(rid)
+        for rid in list(self.by_rid):
+            state = self.server.poll(rid)
+            if state == "done":
+                self._complete(rid)
+            elif state in ("failed", "rejected"):
+                self._fail(rid)
+
+    def _fail(self, rid):
+        """A request the serving loop could not answer: the job ends with no result and the
+        library's error in its info."""
+        sid = self.by_rid.pop(rid)
+        job = self.jobs[sid]
+        from . import McpetError
+        try:
+            self.server.fetch_rebinned(rid) if job["stage"] != "surrogate" else self.server.fetch_estimate(rid)
+            err = "unknown"
+        except McpetError as e:
+            err = str(e)
+        job["result"] = None
+        job["info"] = dict(job["info"] or {}, error=err)
+        job["stage"] = "failed"
+        job["seconds"] = time.perf_counter() - job["t0"]
 
     def drain(self):
         while self.by_rid:

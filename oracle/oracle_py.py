@@ -89,6 +89,10 @@ def load(flavor="fmath"):
     L.orc_load_material.argtypes = [C.c_char_p, P(C.c_float), P(C.c_float), P(C.c_int), P(C.c_float), P(C.c_float),
                                     P(P(C.c_float)), P(P(C.c_float)), P(P(C.c_float)), P(OrcRayleigh), P(OrcCompton)]
     L.orc_load_material.restype = C.c_int
+    L.orc_load_material_ext.argtypes = [C.c_char_p, P(C.c_float), C.c_double, P(C.c_float), P(C.c_int), P(C.c_float),
+                                        P(C.c_float), P(P(C.c_float)), P(P(C.c_float)), P(P(C.c_float)), P(OrcRayleigh),
+                                        P(OrcCompton)]
+    L.orc_load_material_ext.restype = C.c_int
     L.orc_free.argtypes = [C.c_void_p]
     L.orc_free.restype = None
     L.orc_run.argtypes = [P(OrcSim)]
@@ -125,7 +129,7 @@ def _f(a):
 class OracleTables:
     """Interaction tables built by the oracle's own restatement of load_material (C:1565-1831)."""
 
-    def __init__(self, names, density_max, flavor="fmath"):
+    def __init__(self, names, density_max, flavor="fmath", e_max=0.0):
         L = load(flavor)
         buf = C.create_string_buffer(MAXMAT * 250)
         for i, n in enumerate(list(names)[:MAXMAT]):
@@ -141,8 +145,8 @@ class OracleTables:
         b = C.POINTER(C.c_float)()
         self.ray = OrcRayleigh()
         self.cmp = OrcCompton()
-        rc = L.orc_load_material(buf, _f(dm), _f(self.density_nominal), C.byref(n), C.byref(e0), C.byref(ide),
-                                 C.byref(w), C.byref(a), C.byref(b), C.byref(self.ray), C.byref(self.cmp))
+        rc = L.orc_load_material_ext(buf, _f(dm), float(e_max), _f(self.density_nominal), C.byref(n), C.byref(e0),
+                                     C.byref(ide), C.byref(w), C.byref(a), C.byref(b), C.byref(self.ray), C.byref(self.cmp))
         if rc != 0:
             raise RuntimeError(f"orc_load_material failed ({rc})")
         self.num_values = n.value

@@ -129,6 +129,10 @@ void orc_sinogram_sizes(int mrd, int span, int nzs, int nrad, int nangles, int* 
 int orc_load_material(const char* names /* 15*250 */, const float* density_max /* 15, modified */,
                       float* density_nominal, int* num_values, float* e0, float* ide, float** woodcock,
                       float** mfp_a, float** mfp_b, orc_rayleigh* ray, orc_compton* cmp);
+/* The same with the energy grid continued past the files' last point up to e_max (eV). */
+int orc_load_material_ext(const char* names, const float* density_max, double e_max, float* density_nominal,
+                          int* num_values, float* e0, float* ide, float** woodcock, float** mfp_a, float** mfp_b,
+                          orc_rayleigh* ray, orc_compton* cmp);
 void orc_free(void* p);
 
 /* Whole transport (K:153-463) */

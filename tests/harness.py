@@ -11,17 +11,21 @@ from conftest import AIR, WATER, MATERIALS, names15
 
 
 class Scenario:
-    def __init__(self, cfg, mat, dens, act, vs, names):
+    """e_max > 0: interaction tables continued up to e_max eV (library and oracle alike)."""
+
+    def __init__(self, cfg, mat, dens, act, vs, names, e_max=0.0):
         self.cfg = cfg
         self.names = list(names)
         self.phantom = mcpet.phantom_from_arrays(mat, dens, act, vs)
         self.vs = vs
-        self.tables = mcpet.load_materials(self.names, self.phantom.density_max())
+        self.e_max = e_max
+        self.tables = mcpet.load_materials(self.names, self.phantom.density_max(), e_max=e_max)
         self._otab = {}
 
     def oracle(self, flavor="fmath"):
         if flavor not in self._otab:
-            self._otab[flavor] = oracle_py.OracleTables(names15(self.names), self.phantom.density_max(), flavor)
+            self._otab[flavor] = oracle_py.OracleTables(names15(self.names), self.phantom.density_max(), flavor,
+                                                        e_max=self.e_max)
         return oracle_py.Oracle(phantoms.config_dict(self.cfg), self.phantom.voxel_mat_dens(), self.phantom.shape,
                                 self.vs, self._otab[flavor], flavor)
 

@@ -37,9 +37,9 @@ def test_struct_layouts():
 def test_struct_sizes_match_the_compiled_library():
     import ctypes as C
     mine = [C.sizeof(t) for t in (mcpet.Config, mcpet.Phantom, mcpet.Tables, mcpet.Tallies, mcpet.RunParams,
-                                  mcpet.Stats, mcpet.PsfRecord, mcpet.Single)]
+                                  mcpet.Stats, mcpet.PsfRecord, mcpet.Single, mcpet.ServerStats)]
     assert mcpet.struct_sizes() == mine
-    assert mine[-1] == 32 and mcpet.SINGLE_DTYPE.itemsize == 32
+    assert mine[-2] == 32 and mcpet.SINGLE_DTYPE.itemsize == 32
 
 
 def test_create_rejects_bad_arguments_without_touching_the_device():

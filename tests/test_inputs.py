@@ -150,6 +150,94 @@ def test_tables_bit_identical_to_oracle_and_closed_forms():
     assert np.max(np.abs(val[:-1] / wood[:-1] - 1)) < 2e-5
 
 
+def _klein_nishina(e):
+    """Klein-Nishina total cross section per electron in units of 2 pi r_e^2 (Heitler's form)."""
+    k = np.asarray(e, np.float64) / 510998.95
+    return ((1 + k) / k ** 2 * (2 * (1 + k) / (1 + 2 * k) - np.log1p(2 * k) / k) + np.log1p(2 * k) / (2 * k)
+            - (1 + 3 * k) / (1 + 2 * k) ** 2)
+
+
+def test_klein_nishina_helper_limits():
+    """The helper against its limits: Thomson (8/3 in units of pi r_e^2, i.e. 4/3 here) at low
+    energy, and the high-energy form pi r_e^2 (ln 2k + 1/2) / k."""
+    assert abs(_klein_nishina(5.0) / (4.0 / 3.0) - 1) < 1e-4
+    k = 1e4
+    assert abs(_klein_nishina(k * 510998.95) / ((np.log(2 * k) + 0.5) / (2 * k)) - 1) < 1e-3
+
+
+def test_material_files_follow_klein_nishina_near_their_last_energy():
+    """The premise of the table continuation: between 450 and 515 keV the files' Compton mean
+    free paths scale as 1 / sigma_KN within 3e-4 (binding effects have faded), for air and
+    water."""
+    for path in (AIR, WATER):
+        raw = _raw_mfp(path)
+        E, mc = raw[:, 0], raw[:, 2]
+        sel = E >= 450e3
+        ratio = (mc[sel] / mc[-1]) / (_klein_nishina(E[-1]) / _klein_nishina(E[sel]))
+        assert np.max(np.abs(ratio - 1)) < 3e-4, path
+
+
+def test_tables_continued_past_the_files():
+    """Tables continued to 700 keV: the files' points are unchanged (bins below the last file
+    point bit for bit), library and oracle agree bit for bit, and the added points follow their
+    definition in closed form: Compton 1/mfp proportional to sigma_KN (within float rounding),
+    Rayleigh and photoelectric power laws, total = sum of the partial inverse mean free paths,
+    Woodcock = the minimum over materials at their maximum densities."""
+    dm = np.full(15, -999.0, np.float32)
+    dm[0], dm[1], dm[2] = 0.0012, 1.85, 1.0
+    names = [AIR, WATER, WATER]
+    base = mcpet.load_materials(names, dm).arrays()
+    same = mcpet.load_materials(names, dm, e_max=514000.0).arrays()     # below the last point: unchanged
+    assert same["num_values"] == 10200 and same["num_file_values"] == 0
+    for k in ("woodcock", "mfp_a", "mfp_b"):
+        assert np.array_equal(same[k].view(np.uint32), base[k].view(np.uint32)), k
+    t = mcpet.load_materials(names, dm, e_max=700e3).arrays()
+    n_ext = 10200 + int(np.ceil((700e3 - 514950.0) / 50.0)) + 1
+    assert t["num_values"] == n_ext and t["num_file_values"] == 10200
+    for k in ("woodcock", "mfp_a", "mfp_b"):
+        assert np.array_equal(t[k][:10199].view(np.uint32), base[k][:10199].view(np.uint32)), k
+    for flavor in ("libm", "fmath"):
+        o = oracle_py.OracleTables(names15(names), dm, flavor, e_max=700e3)
+        assert o.num_values == n_ext
+        for k in ("woodcock", "mfp_a", "mfp_b"):
+            assert np.array_equal(t[k].view(np.uint32), getattr(o, k).view(np.uint32)), (flavor, k)
+    E = 5000.0 + 50.0 * np.arange(n_ext)
+    ext = slice(10200, n_ext)
+    for slot, path, rho in ((0, AIR, 0.00120479), (1, WATER, 1.0), (2, WATER, 1.0)):
+        raw = _raw_mfp(path)
+        EL = raw[-1, 0]
+        h = np.argmin(np.abs(raw[:, 0] - EL / 2))
+        inv = {}
+        for c in range(3):
+            inv[c] = t["mfp_a"][:, slot, c].astype(np.float64) + E * t["mfp_b"][:, slot, c]
+        # Compton: 1/mfp proportional to the Klein-Nishina cross section
+        want_c = _klein_nishina(E[ext]) / _klein_nishina(EL) / (raw[-1, 2] * rho)
+        assert np.max(np.abs(inv[1][ext] / want_c - 1)) < 2e-5, slot
+        # Rayleigh power law through the last point with the slope between E_L/2 and E_L
+        s_r = np.log(raw[-1, 1] / raw[h, 1]) / np.log(EL / raw[h, 0])
+        assert 1.9 < s_r < 2.1
+        want_r = 1.0 / (raw[-1, 1] * (E[ext] / EL) ** s_r * rho)
+        assert np.max(np.abs(inv[2][ext] / want_r - 1)) < 2e-5, slot
+        # total = Rayleigh + Compton + photoelectric (photoelectric by the same power-law rule)
+        s_p = np.log(raw[-1, 3] / raw[h, 3]) / np.log(EL / raw[h, 0])
+        pho = 1.0 / (raw[-1, 3] * (E[ext] / EL) ** s_p * rho)
+        assert np.max(np.abs(inv[0][ext] / (want_c + want_r + pho) - 1)) < 2e-5, slot
+        # continuous across the files' last point
+        assert abs(inv[1][10200] / inv[1][10199] - _klein_nishina(E[10200]) / _klein_nishina(E[10199])) < 2e-5
+    # the 603 keV attenuation of water: between the 511 keV value and the Klein-Nishina-scaled one
+    w = t["mfp_a"][:, 2, 0].astype(np.float64) + E * t["mfp_b"][:, 2, 0]
+    mu603 = w[np.searchsorted(E, 603e3)]
+    mu511 = w[np.searchsorted(E, 511e3)]
+    assert 0.88 < mu603 / mu511 < 0.95
+    wood = t["woodcock"][:, 0].astype(np.float64) + E * t["woodcock"][:, 1]
+    tot = [t["mfp_a"][:, k, 0].astype(np.float64) + E * t["mfp_b"][:, k, 0] for k in range(3)]
+    want_w = np.minimum.reduce([1.0 / (tot[0] * 0.0012 / 0.00120479), 1.0 / (tot[1] * 1.85), 1.0 / (tot[2] * 1.0)])
+    assert np.max(np.abs(wood[ext][:-1] / want_w[ext][:-1] - 1)) < 5e-5
+    assert np.all(t["pmax"][ext, 2] == np.float32(raw[-1, 5]))
+    with pytest.raises(mcpet.McpetError):
+        mcpet.load_materials(names, dm, e_max=2.0e6)          # more than the Rayleigh table's energy points
+
+
 @pytest.mark.parametrize("kind", ["missing", "nosection", "badyes", "roi"])
 def test_read_input_errors(tmp_path, kind):
     src = open(os.path.join(SAMPLE, "MCGPU-PET.in")).read()
@@ -237,3 +325,85 @@ def test_cli_missing_input_exit_code(tmp_path):
     assert r.returncode == 255 and "not found" in r.stdout
     r = subprocess.run([exe], capture_output=True, text=True)
     assert r.returncode == 255
+
+
+def _dose_by_the_reference_formulas(cfg, mat, dens, vs, edep, N):
+    """Per ROI voxel (dose, std_dev) of report_voxels_dose (MCGPU-PET.cu:2505-2566) in numpy:
+    <E> = E1/N, sigma^2 = (E2/N - <E>^2)/N/mass (then sqrt where positive), dose = <E>/mass,
+    voxels below 0.1 g/cm3 reported as 0."""
+    x0, x1, y0, y1, z0, z1 = cfg.roi_x_min, cfg.roi_x_max, cfg.roi_y_min, cfg.roi_y_max, cfg.roi_z_min, cfg.roi_z_max
+    d = dens[z0:z1 + 1, y0:y1 + 1, x0:x1 + 1].astype(np.float64).ravel()
+    m = mat[z0:z1 + 1, y0:y1 + 1, x0:x1 + 1].ravel()
+    vol = 1.0 / np.prod([np.float64(np.float32(1.0) / np.float32(v)) for v in vs])
+    mass = d * vol
+    e1 = edep[0::2].astype(np.float64)
+    e2 = edep[1::2].astype(np.float64)
+    low = d.astype(np.float32) < np.float32(0.1)
+    e1z, e2z = np.where(low, 0.0, e1), np.where(low, 0.0, e2)
+    mean = e1z / N
+    var = (e2z / N - mean * mean) / N / mass
+    sd = np.where(var > 0, np.sqrt(np.maximum(var, 0)), var)
+    return mean / mass, sd, m, mass, e1, e2, low
+
+
+def test_voxel_dose_file_and_report_known_answers(tmp_path):
+    """The voxel dose outputs (report_voxels_dose, MCGPU-PET.cu:2389-2650) on synthetic 64-bit
+    tallies of a 9 x 9 x 9 phantom with water, bone-density water and air voxels inside a ROI:
+    the .raw dose and 2-sigma %-error files, the ASCII mid-plane, the performance and ROI-energy
+    lines, and the per-material ROI dose and peak voxel summary. Closed-form cases: a voxel whose
+    every history deposits the same energy E has dose E/mass and zero deviation; an air voxel
+    reports 0."""
+    rng = np.random.default_rng(3)
+    mat, dens, act, vs = phantoms.water_cube(n_decays=10)
+    dens = dens.copy()
+    mat = mat.copy()
+    dens[2:5, 2:5, 2:5] = 1.85                       # bone-density region
+    mat[6:, :, :] = 1                                # air slab (slot 1) at z >= 6
+    dens[6:, :, :] = 0.0012
+    roi = (1, 7, 2, 8, 1, 7)                         # x 1-7, y 2-8, z 1-7 (0-based, inclusive)
+    cfg = phantoms.make_config(materials=[AIR, WATER], voxel_dose=True, roi=roi, tally_psf_sinogram=1)
+    ph = mcpet.phantom_from_arrays(mat, dens, act, vs)
+    tab = mcpet.load_materials([AIR, WATER], ph.density_max())
+    n_roi = 7 * 7 * 7
+    N = 1000
+    e1 = rng.integers(0, 5_000_000, n_roi).astype(np.uint64)
+    e2 = (e1.astype(np.float64) ** 2 / rng.uniform(1, 3, n_roi)).astype(np.uint64)   # sum E^2 >= (sum E)^2 / N
+    e1[0], e2[0] = N * 250_000, N * 250_000 ** 2     # every history deposits 250 keV
+    edep = np.stack([e1, e2], 1).ravel()
+    mcpet.write_outputs(str(tmp_path), cfg, ph, tab, {"voxel_edep": edep, "total_histories": N}, 2.0)
+    dose, sd, m, mass, E1, E2, low = _dose_by_the_reference_formulas(cfg, mat, dens, vs, edep, N)
+    got = np.fromfile(tmp_path / "mc-gpu_dose.dat.raw", np.float32)
+    rel = np.fromfile(tmp_path / "mc-gpu_dose.dat_PercentRelError2sigma.raw", np.float32)
+    assert got.size == n_roi and rel.size == n_roi
+    assert np.array_equal(got, dose.astype(np.float32))
+    want_rel = np.where(dose > 0, np.float32(200.0) * (sd / np.where(dose > 0, dose, 1)).astype(np.float32), 0)
+    assert np.array_equal(rel, want_rel.astype(np.float32))
+    assert got[0] == np.float32(250_000 / (1.0 * 1.0)) and rel[0] == 0.0      # closed form: mass 1 g, no spread
+    assert np.all(got[low] == 0) and low.sum() == 7 * 7 * 2
+    txt = open(tmp_path / "mc-gpu_dose.dat").read()
+    z_plane_roi = (roi[4] + roi[5]) // 2 - roi[4]
+    rows = [l for l in txt.splitlines() if l and not l.startswith("#")]
+    plane = slice(z_plane_roi * 49, (z_plane_roi + 1) * 49)
+    assert rows == ["%.6f %.6f" % (a, 2 * b) for a, b in zip(dose[plane], sd[plane])]
+    assert "Total number of simulated x rays: 1000" in txt and "Simulated x rays per projection:  1000" in txt
+    assert "Total speed [x-rays/s]:  500.00" in txt
+    tot = 0.001 * float(np.where(low, 0, E1).sum()) / N
+    assert f"Total energy absorved inside the dose ROI: {tot:.5f} keV/hist" in txt
+    # the per-material summary
+    r = mcpet.voxel_dose_report(cfg, ph, edep, N)
+    assert abs(r["total_energy_kev"] - tot) <= 1e-12 * tot
+    for slot in (1, 2):
+        sel = m == slot
+        k = slot - 1
+        assert r["voxels"][k] == sel.sum() and abs(r["mass"][k] - mass[sel].sum()) <= 1e-9 * mass[sel].sum()
+        E = E1[sel].sum() / N                        # every ROI voxel of the slot (before the exclusion)
+        v = (E2[sel].sum() / N - E * E) / N
+        assert abs(r["edep"][k] - E) <= 1e-12 * E
+        assert abs(r["dose"][k] - E / mass[sel].sum()) <= 1e-12 * r["dose"][k]
+        assert v > 0 and abs(r["dose_std_dev"][k] - np.sqrt(v) / mass[sel].sum()) <= 1e-9 * r["dose_std_dev"][k]
+        i = np.flatnonzero(sel)[np.argmax(dose[sel])]   # first voxel of the largest dose, in ROI order
+        assert r["peak_dose"][k] == dose[i] and r["peak_std_dev"][k] == sd[i]
+        zz, yy, xx = np.unravel_index(i, (7, 7, 7))
+        assert r["peak_voxel"][k] == (xx + roi[0], yy + roi[2], zz + roi[4])
+    assert r["peak_dose"][0] == 0.0                  # air: every dose reported as 0
+    assert r["max_dose"] == dose.max() and r["voxels"][2:] == [0] * 13

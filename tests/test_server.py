@@ -52,3 +52,37 @@ def test_oversized_request_is_rejected():
         with pytest.raises(mcpet.McpetError):
             srv.submit(cfg, phs[0])
         assert srv.stats()["rejected"] == 1
+
+
+def test_bad_requests_are_refused_at_submit_and_the_queue_proceeds():
+    """Requests that could not be created (a dose region of interest outside the phantom) or
+    run (an unusable seed) are rejected when submitted, with the library's reason; fetching one
+    reports it and forgets it; the valid requests behind them complete and equal their
+    standalone runs."""
+    cfg, phs, tab = _inputs(2, seed=13)
+    nz, ny, nx = phs[0].shape
+    bad = phantoms.make_config(seed=4711, threads_per_block=16, acquisition_time_s=10.0,
+                               materials=phantoms.head_materials(MATERIALS), material_dose=True, nrows=20, nzs=39,
+                               mrd=19, span=3, voxel_dose=True, roi=(0, nx, 0, ny - 1, 0, nz - 1))   # x one past the end
+    with mcpet.Server(tab, 4 << 30, max_running=4, streams_per_step=2000) as srv:
+        with pytest.raises(mcpet.McpetError, match="region of interest"):
+            srv.submit(bad, phs[0])
+        rid_bad = max(srv._meta)
+        assert srv.poll(rid_bad) == "rejected"
+        with pytest.raises(mcpet.McpetError, match="seed"):
+            srv.submit(cfg, phs[1], seed=-2147483399)            # below -m2: abMODm would divide by zero
+        good = [srv.submit(cfg, ph, seed=200 + k) for k, ph in enumerate(phs)]
+        srv.drain()
+        assert [srv.poll(i) for i in good] == ["done", "done"]
+        with pytest.raises(mcpet.McpetError, match="rejected"):
+            srv.fetch(rid_bad, psf=False)
+        assert srv.poll(rid_bad) == "unknown"
+        st = srv.stats()
+        assert st["rejected"] == 2 and st["completed"] == 2 and st["failed"] == 0
+        got = [srv.fetch(i, psf=False) for i in good]
+    for k, ph in enumerate(phs):
+        with mcpet.Simulation(cfg, ph, tab) as sim:
+            sim.run(seed=200 + k)
+            ref = sim.fetch(psf=False)
+        for key in KEYS:
+            assert np.array_equal(got[k][key], ref[key]), (k, key)
