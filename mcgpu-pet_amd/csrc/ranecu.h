@@ -105,13 +105,24 @@ MCR_HD void mcr_init_stream(int64_t id, int seed_input, const uint32_t* j1, cons
   }
 }
 
-/* One step of both MLCGs (Schrage decomposition), combined output in [1, m1-1]. */
+/* Products of operands below 2^24 whose result fits in 31 bits: one full-rate 24-bit
+ * multiply on the device instead of a quarter-rate 32-bit one; the same integer. */
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MCR_MUL24(a, b) __mul24((a), (b))
+#else
+#define MCR_MUL24(a, b) ((a) * (b))
+#endif
+
+/* One step of both MLCGs (Schrage decomposition), combined output in [1, m1-1].
+ * The states lie in [1, m-1], so the quotients are taken unsigned (no sign fix-up), and
+ * every product has operands below 2^16 (quotient < 40015, remainder < q) and a result
+ * below 2^31: 24-bit multiplies. */
 MCR_HD int mcr_step(int* sx, int* sy)
 {
-  int i1 = *sx / 53668;
-  *sx = 40014 * (*sx - i1 * 53668) - i1 * 12211;
-  int i2 = *sy / 52774;
-  *sy = 40692 * (*sy - i2 * 52774) - i2 * 3791;
+  int i1 = (int)((unsigned)*sx / 53668u);
+  *sx = MCR_MUL24(40014, *sx - MCR_MUL24(i1, 53668)) - MCR_MUL24(i1, 12211);
+  int i2 = (int)((unsigned)*sy / 52774u);
+  *sy = MCR_MUL24(40692, *sy - MCR_MUL24(i2, 52774)) - MCR_MUL24(i2, 3791);
   if (*sx < 0) *sx += 2147483563;
   if (*sy < 0) *sy += 2147483399;
   i2 = *sx - *sy;

@@ -163,7 +163,7 @@ static mcpet_status submit(mcpet_server* s, const mcpet_config* cfg, const mcpet
   std::string vmsg = vst != MCPET_OK ? std::string(mcpet_last_error(nullptr)) : std::string();
   if (vst == MCPET_OK && !mcpet_seed_usable(seed != 0 ? seed : r->cfg.seed)) {
     vst = MCPET_E_ARG;
-    vmsg = "mcpet_server_submit: unusable seed (0 or below -2147483398)";
+    vmsg = "mcpet_server_submit: unusable seed (0 or below -2147483399)";
   }
   if (grid) {
     std::memcpy(r->grid, grid, sizeof r->grid);

@@ -70,7 +70,7 @@ def test_bad_requests_are_refused_at_submit_and_the_queue_proceeds():
         rid_bad = max(srv._meta)
         assert srv.poll(rid_bad) == "rejected"
         with pytest.raises(mcpet.McpetError, match="seed"):
-            srv.submit(cfg, phs[1], seed=-2147483399)            # below -m2: abMODm would divide by zero
+            srv.submit(cfg, phs[1], seed=-2147483400)            # below -m2: abMODm would divide by zero
         good = [srv.submit(cfg, ph, seed=200 + k) for k, ph in enumerate(phs)]
         srv.drain()
         assert [srv.poll(i) for i in good] == ["done", "done"]
