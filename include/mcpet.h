@@ -220,6 +220,9 @@ typedef struct mcpet_run_params {
                                     streams, pairs_tallied and coincidences are counted. The
                                     plain kernel always counts every event.              */
   int32_t use_graph;             /* batches: replay the clearing + launch as a HIP graph  */
+  int32_t lds_layout;            /* persistent kernel, single context: 0 = the autotuned or
+                                    default layout; MCPET_LDS_SET | MCPET_LDS_* bits = that
+                                    layout (tallies are identical in every layout)        */
 } mcpet_run_params;
 
 /* Per-run statistics: HIP-event kernel time and algorithmic event counts (see
@@ -337,6 +340,48 @@ const char* mcpet_last_error(const mcpet_ctx* ctx);
 /* Effective detector after mcpet_create (centre/radius rule applied). */
 mcpet_status mcpet_get_detector(const mcpet_ctx* ctx, mcpet_detector* out);
 
+/* LDS layout of the persistent transport kernel (mcpet_tune_result.lds_layout bits): what each
+ * workgroup keeps in LDS rather than reading from global memory. No placement changes a tally. */
+#define MCPET_LDS_PALETTE 1      /* the u8 voxel palette (phantoms with <= 256 distinct voxel words) */
+#define MCPET_LDS_SPECTRUM 2     /* the energy spectrum, privatised (else global atomics)            */
+#define MCPET_LDS_XS511 4        /* the 511 keV total-cross-section row of every material            */
+#define MCPET_LDS_SET 8          /* mcpet_run_params.lds_layout holds an explicit layout             */
+
+typedef struct mcpet_tune_result {
+  int32_t block_size, blocks_per_cu;
+  int32_t phase_policy;          /* 1 most lanes, 2 least waste (mcpet_run_params)        */
+  double ms;                     /* kernel time of the sample (HIP events)                */
+  double decays_per_ms;
+  int32_t lds_layout;            /* MCPET_LDS_* bits                                      */
+  int64_t stream_decays;         /* decays per stream of the layout (0: threads_per_block
+                                    streams per emitting voxel, the reference's layout)   */
+} mcpet_tune_result;
+
+/* Autotuner of one context. No reference counterpart: the reference fixes its one launch from
+ * threads_per_block (MCGPU-PET.in line 14, MCGPU-PET.cu:452-487). Each candidate runs a sample
+ * of the context's decay streams (every k-th stream over the whole acquisition, k =
+ * round(1 / stream_fraction)) and is timed with HIP events; the fastest is kept for later
+ * mcpet_run calls whose block_size, blocks_per_cu and phase_policy are 0. Searched in turn:
+ *   1. launch shape (workgroup size x workgroups per CU) x phase-selection policy;
+ *   2. tune_layout: the LDS layout (MCPET_LDS_* placements moved to global memory one at a
+ *      time, kept when faster); tallies are identical in every layout;
+ *   3. tune_streams: the decays per stream (the reference's histories per thread), among
+ *      stream_decays[0..n_stream_decays) (0 = threads_per_block streams per emitting voxel).
+ *      Another layout draws other random-number streams (statistically equal tallies, not the
+ *      same bits), so it is opt-in; the chosen value is applied to the context and reported,
+ *      and a context created with mcpet_config.stream_decays = that value reproduces its runs.
+ * results receives every timed candidate, then the choice (up to max_results entries);
+ * n_results the number written. The context's tallies afterwards are not a run's. */
+typedef struct mcpet_tune_options {
+  double stream_fraction;        /* (0, 1]: share of the streams each candidate simulates */
+  int32_t tune_layout;           /* 1: search the LDS layout                              */
+  int32_t tune_streams;          /* 1: search the decays per stream (opt-in, see above)   */
+  int32_t n_stream_decays;       /* 0..8 candidates in stream_decays                     */
+  int64_t stream_decays[8];
+} mcpet_tune_options;
+mcpet_status mcpet_autotune(mcpet_ctx* ctx, const mcpet_tune_options* opt, mcpet_tune_result* results, int32_t max_results,
+                            int32_t* n_results);
+
 /* ------------------------------------------------------------------ batches of phantoms
  * No reference counterpart (the reference runs one phantom per process). A batch simulates
  * n phantoms that share the scanner, tally settings and interaction tables (cfg, tab), each
@@ -345,12 +390,6 @@ mcpet_status mcpet_get_detector(const mcpet_ctx* ctx, mcpet_detector* out);
  * mcpet_run with the same run parameters, bit for bit. The pair tallies (modes 0-2) only. */
 typedef struct mcpet_batch mcpet_batch;
 
-typedef struct mcpet_tune_result {
-  int32_t block_size, blocks_per_cu;
-  int32_t phase_policy;          /* 1 most lanes, 2 least waste (mcpet_run_params)        */
-  double ms;                     /* batch kernel time (HIP events)                        */
-  double decays_per_ms;
-} mcpet_tune_result;
 
 mcpet_status mcpet_batch_create(const mcpet_config* cfg, const mcpet_phantom* phantoms, int32_t n, const mcpet_tables* tab,
                                 mcpet_batch** out);

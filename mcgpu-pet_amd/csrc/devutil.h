@@ -7,6 +7,41 @@
 
 #define DU_PI 3.14159265358979323846
 
+// Accesses through pointers that address global memory, as global (not flat) instructions.
+// Pointers read from a parameter block in memory are generic to the compiler, which then emits
+// flat loads and atomics; a flat access counts against both the vector-memory and the LDS wait
+// counters, so every later wait for an LDS result would also wait for it.
+#define DU_GLOBAL __attribute__((address_space(1)))
+template <class T>
+static __device__ __forceinline__ T gld(const T* p, size_t i)
+{
+  return ((const DU_GLOBAL T*)p)[i];
+}
+// HIP's float2/float4 are classes whose members take generic references: load them as
+// builtin vectors.
+static __device__ __forceinline__ float2 gld(const float2* p, size_t i)
+{
+  typedef float v2 __attribute__((ext_vector_type(2)));
+  const v2 v = ((const DU_GLOBAL v2*)p)[i];
+  return make_float2(v.x, v.y);
+}
+static __device__ __forceinline__ float4 gld(const float4* p, size_t i)
+{
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  const v4 v = ((const DU_GLOBAL v4*)p)[i];
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+template <class T>
+static __device__ __forceinline__ T gatomic_add(T* p, T v)
+{
+  return __hip_atomic_fetch_add((DU_GLOBAL T*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+static __device__ __forceinline__ T gatomic_min(T* p, T v)
+{
+  return __hip_atomic_fetch_min((DU_GLOBAL T*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // __float2int_rd with NaN -> INT_MIN and saturation (matches the host checker).
 static __device__ __forceinline__ int f2i_rd(float x)
 {

@@ -35,6 +35,9 @@
 #define K_INV_C (1.0 / 29979245800e-12)
 #define K_E511 510998.95f
 #define K_MAXMAT 15
+// LDS address space: the wave's stream-queue range is passed to a non-inlined function, which
+// would otherwise see a generic pointer and use flat accesses.
+#define K_LDS __attribute__((address_space(3)))
 // Upper bound on the iterations of each rejection loop (GRAa, GCOa). Valid tables accept
 // within a few trials; the bound only guarantees that no lane can loop forever on
 // corrupt input. The CPU checker applies the same bound.
@@ -100,7 +103,7 @@ struct FullCounters {
   {
 #pragma unroll
     for (int i = 0; i < KC_N; i++)
-      if (c[i]) atomicAdd(&P.counters[i], (unsigned long long)c[i]);
+      if (c[i]) gatomic_add(&P.counters[i], (unsigned long long)c[i]);
   }
 };
 struct LiteCounters {
@@ -192,8 +195,9 @@ __device__ __forceinline__ void rotate_double(float* d, double costh, double phi
 // GRAa (K:1230-1296): Rayleigh angle by RITA inverse cumulative + Thomson rejection.
 template <class CN>
 K_RARE void graa(float energy, double* costh, int mat, float pmax_current, Rng& r, CN& C,
-                     const mcpet_rayleigh_table* __restrict__ cgra)
+                     const mcpet_rayleigh_table* __restrict__ cgra_in)
 {
+  const DU_GLOBAL mcpet_rayleigh_table* cgra = (const DU_GLOBAL mcpet_rayleigh_table*)cgra_in;
   double xmax = ((double)energy) * 8.065535669099010e-5;
   double x2lim = (double)cgra->xco[(mat + 1) * MCPET_NP_RAYLEIGH - 1];
   double x2max = (xmax * xmax) < x2lim ? (xmax * xmax) : x2lim;
@@ -431,8 +435,8 @@ __device__ __forceinline__ float cyl_dist(const KParams& P, float x0, float y0, 
 
 __device__ __forceinline__ void spectrum_add(const KParams& P, int* s_spec, int iE)
 {
-  if (P.lds_spectrum) atomicAdd(&s_spec[iE], 1);
-  else atomicAdd(&P.spectrum[iE], 1);
+  if (P.lds_spectrum) __hip_atomic_fetch_add((K_LDS int*)s_spec + iE, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  else gatomic_add(&P.spectrum[iE], 1);
 }
 
 // tally_Sinogram (K:1623-1836).
@@ -473,19 +477,19 @@ K_RARE void tally_sinogram(const KParams& P, float energy, float energyFirst, co
   C.inc(KC_PAIRS);
   if ((Energia1) > (P.e_low) && (Energia1) < (P.e_high) && (Energia2) > (P.e_low) && (Energia2) < (P.e_high)) {
     if (ss == 0 && ssf == 0) {
-      atomicAdd(&P.trues[ibin], 1);
-      atomicAdd(&P.img_t[st.absvox], 1);
+      gatomic_add(&P.trues[ibin], 1);
+      gatomic_add(&P.img_t[st.absvox], 1);
       C.inc(KC_COINC);
     }
     if (ss > 0 || ssf > 0) {
-      atomicAdd(&P.scatter[ibin], 1);
-      atomicAdd(&P.img_sc[st.absvox], 1);
+      gatomic_add(&P.scatter[ibin], 1);
+      gatomic_add(&P.img_sc[st.absvox], 1);
       C.inc(KC_COINC);
     }
   }
 }
 
-__device__ __forceinline__ void write_psf(mcpet_psf_record* rec, unsigned long long t_ps, float travel_ps, int vox, float e,
+__device__ __forceinline__ void write_psf(DU_GLOBAL mcpet_psf_record* rec, unsigned long long t_ps, float travel_ps, int vox, float e,
                                           float z, float phi, const float* v, int idx1)
 {
   rec->emission_time_ps = t_ps;
@@ -532,8 +536,8 @@ K_RARE void tally_psf(const KParams& P, float energy, float energyFirst, const f
   float Energia2 = energyFirst + ((P.e_resol) * (1.0f / 2.35f)) * energyFirst * g2;
   if (fabsf(positionFirst[2]) < 0.5f * P.height && (Energia1 > P.e_low) && (Energia1 < P.e_high) && (Energia2 > P.e_low) &&
       (Energia2 < P.e_high)) {
-    int index = atomicAdd(P.psf_index, 2);
-    if (index > 2000000000) atomicMin(P.psf_index, 2000000000);
+    int index = gatomic_add(P.psf_index, 2);
+    if (index > 2000000000) gatomic_min(P.psf_index, 2000000000);
     if (index < P.psf_size && P.psf) {
       int i0, i1;
       if (travel < travelFirst) {
@@ -548,9 +552,9 @@ K_RARE void tally_psf(const KParams& P, float energy, float energyFirst, const f
       const float phi = mcf_atan2f(position[1], position[0]);
       const float phi1 = mcf_atan2f(positionFirst[1], positionFirst[0]);
       if (i0 < P.psf_size)
-        write_psf(P.psf + i0, t_ps, (float)(travel * K_INV_C), st.absvox, Energia1, position[2], phi, dir, ss);
+        write_psf((DU_GLOBAL mcpet_psf_record*)(P.psf + i0), t_ps, (float)(travel * K_INV_C), st.absvox, Energia1, position[2], phi, dir, ss);
       if (i1 < P.psf_size)
-        write_psf(P.psf + i1, t_ps, (float)(travelFirst * K_INV_C), st.absvox, Energia2, positionFirst[2], phi1, dirf, ssf);
+        write_psf((DU_GLOBAL mcpet_psf_record*)(P.psf + i1), t_ps, (float)(travelFirst * K_INV_C), st.absvox, Energia2, positionFirst[2], phi1, dirf, ssf);
     }
   }
 }
@@ -578,12 +582,12 @@ __device__ __forceinline__ void record_single(const KParams& P, const float* pos
   const float phi = mcf_atan2f(y0 + dist * dir[1], x0 + dist * dir[0]);
   const unsigned long long flight = d2ull_rn((double)(travel + dist) * K_INV_C);
   const unsigned long long t = (P.acq_ps - st.remaining) + (flight < (1ull << 40) ? flight : (1ull << 40) - 1ull);
-  const unsigned long long k = atomicAdd(P.singles_n, 1ull);
+  const unsigned long long k = gatomic_add(P.singles_n, 1ull);
   if ((long long)k < P.singles_cap) {
-    uint4* rec = reinterpret_cast<uint4*>(P.singles + k);
-    rec[0] = make_uint4((unsigned)t, (unsigned)(t >> 32), (unsigned)st.event, (unsigned)(st.event >> 32));
-    rec[1] = make_uint4(__float_as_uint(e), __float_as_uint(z), __float_as_uint(phi),
-                        ((unsigned)ss & 0xffffu) | ((unsigned)kind << 16));
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+    DU_GLOBAL u32x4* rec = (DU_GLOBAL u32x4*)(P.singles + k);
+    rec[0] = u32x4{(unsigned)t, (unsigned)(t >> 32), (unsigned)st.event, (unsigned)(st.event >> 32)};
+    rec[1] = u32x4{__float_as_uint(e), __float_as_uint(z), __float_as_uint(phi), ((unsigned)ss & 0xffffu) | ((unsigned)kind << 16)};
   }
 }
 
@@ -600,8 +604,8 @@ __device__ __forceinline__ void tally_dose(const KParams& P, float Edep, int mat
     if ((vx < P.roi[0]) || (vx > P.roi[1]) || (vy < P.roi[2]) || (vy > P.roi[3]) || (vz < P.roi[4]) || (vz > P.roi[5])) return;
     int DX = 1 + (int)(P.roi[1] - P.roi[0]);
     int num = (int)(vx - P.roi[0]) + ((int)(vy - P.roi[2])) * DX + ((int)(vz - P.roi[4])) * DX * (1 + (int)(P.roi[3] - P.roi[2]));
-    atomicAdd(&P.voxel_edep[2 * (size_t)num], f2ull_rn(Edep * 1.0f));
-    atomicAdd(&P.voxel_edep[2 * (size_t)num + 1], f2ull_rn(Edep * Edep));
+    gatomic_add(&P.voxel_edep[2 * (size_t)num], f2ull_rn(Edep * 1.0f));
+    gatomic_add(&P.voxel_edep[2 * (size_t)num + 1], f2ull_rn(Edep * Edep));
   }
 }
 
@@ -626,7 +630,7 @@ __device__ __forceinline__ void photon_start(const KParams& P, Photon& ph, float
   ph.ss = 0;
   ph.travel = 0.0f;
   ph.index = f2i_rd((ph.E - P.e0) * P.ide);
-  const float2 w = P.woodcock[ph.index];
+  const float2 w = gld(P.woodcock, ph.index);
   ph.mfpW = w.x + ph.E * w.y;
   ph.mat_old = -1;
 }
@@ -646,16 +650,19 @@ __shared__ int k_idx511;
 __device__ __forceinline__ void load_tot511(const KParams& P)
 {
   const int idx = f2i_rd((K_E511 - P.e0) * P.ide);
-  if (threadIdx.x < K_MAXMAT) k_tot511[threadIdx.x] = P.tot[idx * K_MAXMAT + threadIdx.x];
-  if (threadIdx.x == 0) k_idx511 = idx;
+  if (threadIdx.x < K_MAXMAT) k_tot511[threadIdx.x] = gld(P.tot, idx * K_MAXMAT + threadIdx.x);
+  if (threadIdx.x == 0) k_idx511 = P.lds_xs511 ? idx : -1;   // -1: no energy bin matches, rows from global memory
 }
 
 // The (density, material) word of voxel absvox in the device layout chosen at set-up.
 __device__ __forceinline__ float2 voxel_word(const KParams& P, int absvox)
 {
-  if (P.vox_fmt == 1) return P.lds_palette ? k_palette[P.vox8[absvox]] : P.palette[P.vox8[absvox]];
-  if (P.vox_fmt == 2) return P.palette[P.vox16[absvox]];
-  return P.vox[absvox];
+  if (P.vox_fmt == 1) {
+    const unsigned i = gld(P.vox8, absvox);
+    return P.lds_palette ? k_palette[i] : gld(P.palette, i);
+  }
+  if (P.vox_fmt == 2) return gld(P.palette, gld(P.vox16, absvox));
+  return gld(P.vox, absvox);
 }
 
 template <class CN>
@@ -677,7 +684,7 @@ __device__ __forceinline__ int woodcock_step(const KParams& P, Photon& ph, Rng& 
   ph.mat0 = w & 0xff;
   if (w != ph.mat_old) {
     if (ph.mat_old < 0 || (w >> 8) != (ph.mat_old >> 8))
-      ph.tab = ph.index == k_idx511 ? k_tot511[ph.mat0] : P.tot[ph.index * K_MAXMAT + ph.mat0];
+      ph.tab = ph.index == k_idx511 ? k_tot511[ph.mat0] : gld(P.tot, ph.index * K_MAXMAT + ph.mat0);
     ph.mat_old = w;
     C.inc(KC_MATCHANGE);
   }
@@ -697,7 +704,7 @@ template <class CN>
 __device__ __forceinline__ int interaction_begin(const KParams& P, Photon& ph, Rng& r, CN& C,
                                                  const mcpet_compton_table& cg, ComptonState& cs, unsigned long long* s_mat)
 {
-  const float4 cr = P.cr[ph.index * K_MAXMAT + ph.mat0];
+  const float4 cr = gld(P.cr, ph.index * K_MAXMAT + ph.mat0);
   float prob = ph.prob + ph.mfp_density * (cr.x + ph.E * cr.y);
   const float randno = ph.randno;
   if (randno < prob) {
@@ -709,7 +716,7 @@ __device__ __forceinline__ int interaction_begin(const KParams& P, Photon& ph, R
   if (randno < prob) {
     double costh_Rayleigh;
     C.inc(KC_RAYLEIGH);
-    const float pmax_current = P.ray->pmax[(ph.index + 1) * K_MAXMAT + ph.mat0];
+    const float pmax_current = gld(P.ray->pmax, (ph.index + 1) * K_MAXMAT + ph.mat0);
     graa(ph.E, &costh_Rayleigh, ph.mat0, pmax_current, r, C, P.ray);
     rotate_double(ph.dir, costh_Rayleigh, 6.28318530717958647693 * rnd(r, C));
     ph.ss = (ph.ss == 0) ? 2 : 3;
@@ -735,7 +742,7 @@ __device__ __forceinline__ bool compton_end(const KParams& P, Photon& ph, Rng& r
   const float randno = ph.E - e_before;
   ph.index = f2i_rd((ph.E - P.e0) * P.ide);
   if (ph.index > -1) {
-    const float2 w = P.woodcock[ph.index];
+    const float2 w = gld(P.woodcock, ph.index);
     ph.mfpW = w.x + ph.E * w.y;
     ph.mat_old = -2;
     ph.ss = (ph.ss == 0) ? 1 : 3;
@@ -922,19 +929,20 @@ __device__ __forceinline__ void stream_init(const KParams& P, long long s, Strea
     int lo = 0, hi = P.n_active - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
-      if (P.stream_start[mid] <= s) lo = mid;
+      if (gld(P.stream_start, mid) <= s) lo = mid;
       else hi = mid - 1;
     }
     k = lo;
-    per_voxel = P.stream_start[k + 1] - P.stream_start[k];
-    tid = P.stream_id[k] + (s - P.stream_start[k]);
+    const long long s0 = gld(P.stream_start, k);
+    per_voxel = gld(P.stream_start, k + 1) - s0;
+    tid = gld(P.stream_id, k) + (s - s0);
   } else {
     k = (int)(s / P.tpb);
     per_voxel = P.tpb;
-    tid = (long long)P.active_vox[k] * P.tpb + (s - (long long)k * P.tpb);
+    tid = (long long)gld(P.active_vox, k) * P.tpb + (s - (long long)k * P.tpb);
   }
-  st.absvox = P.active_vox[k];
-  const float act = P.active_act[k];
+  st.absvox = gld(P.active_vox, k);
+  const float act = gld(P.active_act, k);
   mcr_init_stream(tid, P.seed, P.j1, P.j2, &st.r.x, &st.r.y);
   st.event = (unsigned long long)tid << 32;
   st.remaining = P.acq_ps;
@@ -943,7 +951,7 @@ __device__ __forceinline__ void stream_init(const KParams& P, long long s, Strea
 
 __device__ __forceinline__ void stream_finish(const KParams& P, const Stream& st)
 {
-  if ((long long)(st.event >> 32) == P.last_thread_id) *P.final_seed = st.r.x;
+  if ((long long)(st.event >> 32) == P.last_thread_id) *(DU_GLOBAL int*)P.final_seed = st.r.x;
 }
 
 // Workgroup prologue/epilogue shared by the kernels: Compton table, spectrum, material
@@ -957,11 +965,11 @@ __device__ __forceinline__ void stream_finish(const KParams& P, const Stream& st
     const int* src = reinterpret_cast<const int*>(P.cmp);                                        \
     int* dst = reinterpret_cast<int*>(&s_cmp);                                                   \
     for (int i = threadIdx.x; i < (int)(sizeof(mcpet_compton_table) / 4); i += blockDim.x)       \
-      dst[i] = src[i];                                                                           \
+      dst[i] = gld(src, i);                                                                      \
     if (threadIdx.x < 2 * K_MAXMAT) s_mat[threadIdx.x] = 0ull;                                   \
     if (threadIdx.x < KC_N) s_cnt[threadIdx.x] = 0u;                                             \
     if (P.vox_fmt == 1 && P.lds_palette)                                                         \
-      for (int i = threadIdx.x; i < P.n_palette; i += blockDim.x) k_palette[i] = P.palette[i];   \
+      for (int i = threadIdx.x; i < P.n_palette; i += blockDim.x) k_palette[i] = gld(P.palette, i); \
     load_tot511(P);                                                                              \
     if (P.lds_spectrum)                                                                          \
       for (int i = threadIdx.x; i < P.ne; i += blockDim.x) s_spec[i] = 0;                        \
@@ -974,11 +982,11 @@ __device__ __forceinline__ void stream_finish(const KParams& P, const Stream& st
   C.flush(P, s_cnt);                                                                             \
   __syncthreads();                                                                               \
   if (threadIdx.x < KC_N && s_cnt[threadIdx.x])                                                  \
-    atomicAdd(&P.counters[threadIdx.x], (unsigned long long)s_cnt[threadIdx.x]);                 \
+    gatomic_add(&P.counters[threadIdx.x], (unsigned long long)s_cnt[threadIdx.x]);               \
   if (P.lds_spectrum)                                                                            \
     for (int i = threadIdx.x; i < P.ne; i += blockDim.x)                                         \
-      if (s_spec[i]) atomicAdd(&P.spectrum[i], s_spec[i]);                                       \
-  if (threadIdx.x < 2 * K_MAXMAT && s_mat[threadIdx.x]) atomicAdd(&P.mat_edep[threadIdx.x], s_mat[threadIdx.x]);
+      if (s_spec[i]) gatomic_add(&P.spectrum[i], s_spec[i]);                                     \
+  if (threadIdx.x < 2 * K_MAXMAT && s_mat[threadIdx.x]) gatomic_add(&P.mat_edep[threadIdx.x], s_mat[threadIdx.x]);
 
 // Plain kernel: lane i of the grid runs stream i to the end (nested loops, no queue).
 __global__ __launch_bounds__(256) K_OCCUPANCY void k_transport_plain(KParams P)
@@ -1134,7 +1142,7 @@ __device__ __forceinline__ int after_photon(const KParams& P, int photon_no, boo
 // Index (from the queue's origin) of the next stream for every calling lane. Not inlined:
 // inlined into the state machine it raised the register pressure of every phase (32 more
 // spilled VGPRs); as a call it costs a few register saves at each claim.
-__device__ __attribute__((noinline)) long long claim_stream(unsigned long long* queue, volatile long long* wq)
+__device__ __attribute__((noinline)) long long claim_stream(unsigned long long* queue, volatile K_LDS long long* wq)
 {
   // the lanes here all need a stream: rank among them, claim a new chunk if short
   // (the range is wave-uniform: read into scalar registers)
@@ -1155,7 +1163,7 @@ __device__ __attribute__((noinline)) long long claim_stream(unsigned long long* 
     }
   } else {
     unsigned long long base = 0;
-    if (rank == 0) base = atomicAdd(queue, (unsigned long long)K_CHUNK);
+    if (rank == 0) base = gatomic_add(queue, (unsigned long long)K_CHUNK);
     const unsigned blo = __builtin_amdgcn_readfirstlane((unsigned)base);
     const unsigned bhi = __builtin_amdgcn_readfirstlane((unsigned)(base >> 32));
     const long long b = (long long)(((unsigned long long)bhi << 32) | blo);
@@ -1170,7 +1178,7 @@ __device__ __attribute__((noinline)) long long claim_stream(unsigned long long* 
 
 template <class CN, bool kExt>
 __device__ __forceinline__ void sm_waves(const KParams& P, CN& C, const mcpet_compton_table& s_cmp, int* s_spec,
-                                         unsigned long long* s_mat, const LaneSlots& L, volatile long long* wq)
+                                         unsigned long long* s_mat, const LaneSlots& L, volatile K_LDS long long* wq)
 {
   constexpr int kPhases = kExt ? LS_N : LS_CPZ + 1;
   if (__lane_id() == 0) {
@@ -1280,7 +1288,7 @@ __device__ __forceinline__ void sm_waves(const KParams& P, CN& C, const mcpet_co
       for (;;) {
         if (!have) {
           const long long idx = claim_stream(P.queue, wq);
-          const long long sid = P.stream_base + idx;
+          const long long sid = P.stream_base + (P.stream_stride > 1 ? idx * P.stream_stride : idx);
           if (sid >= P.n_streams) {
             state = LS_DONE;
             break;
@@ -1362,7 +1370,7 @@ __global__ __launch_bounds__(K_BLOCK) K_OCCUPANCY void k_transport_sm(const KPar
   __shared__ float s_slots[LANE_SLOTS][K_BLOCK];
   __shared__ long long s_wq[K_BLOCK / 64][2];
   const LaneSlots L{&s_slots[0][threadIdx.x]};
-  sm_waves<CN, kExt>(P, C, s_cmp, s_spec, s_mat, L, s_wq[threadIdx.x >> 6]);
+  sm_waves<CN, kExt>(P, C, s_cmp, s_spec, s_mat, L, (volatile K_LDS long long*)s_wq[threadIdx.x >> 6]);
   K_EPILOGUE
 }
 
@@ -1387,7 +1395,7 @@ __global__ __launch_bounds__(K_BLOCK) K_OCCUPANCY void k_transport_batch(const K
   {
     const int* src = reinterpret_cast<const int*>(Ps[0].cmp);
     int* dst = reinterpret_cast<int*>(&s_cmp);
-    for (int i = threadIdx.x; i < (int)(sizeof(mcpet_compton_table) / 4); i += blockDim.x) dst[i] = src[i];
+    for (int i = threadIdx.x; i < (int)(sizeof(mcpet_compton_table) / 4); i += blockDim.x) dst[i] = gld(src, i);
     for (int i = threadIdx.x; i < (K_BLOCK / 64) * 2 * K_MAXMAT; i += blockDim.x) (&s_matw[0][0])[i] = 0ull;
     for (int i = threadIdx.x; i < (K_BLOCK / 64) * KC_N; i += blockDim.x) (&s_cntw[0][0])[i] = 0u;
     load_tot511(Ps[0]);   // the phantoms of a batch share their interaction tables
@@ -1401,15 +1409,15 @@ __global__ __launch_bounds__(K_BLOCK) K_OCCUPANCY void k_transport_batch(const K
     const KParams& P = Ps[k];
     CN C;
     C.init(s_cntw[wave]);
-    sm_waves<CN, kExt>(P, C, s_cmp, nullptr, s_matw[wave], L, s_wq[wave]);
+    sm_waves<CN, kExt>(P, C, s_cmp, nullptr, s_matw[wave], L, (volatile K_LDS long long*)s_wq[wave]);
     C.flush(P, s_cntw[wave]);
     __builtin_amdgcn_wave_barrier();
     if (lane < KC_N && s_cntw[wave][lane]) {
-      atomicAdd(&P.counters[lane], (unsigned long long)s_cntw[wave][lane]);
+      gatomic_add(&P.counters[lane], (unsigned long long)s_cntw[wave][lane]);
       s_cntw[wave][lane] = 0u;
     }
     if (lane < 2 * K_MAXMAT && s_matw[wave][lane]) {
-      atomicAdd(&P.mat_edep[lane], s_matw[wave][lane]);
+      gatomic_add(&P.mat_edep[lane], s_matw[wave][lane]);
       s_matw[wave][lane] = 0ull;
     }
     __builtin_amdgcn_wave_barrier();

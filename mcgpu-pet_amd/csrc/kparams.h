@@ -95,6 +95,8 @@ struct KParams {
   long long last_thread_id;      // reference thread id whose final seed is stored (K:454-455)
   int lds_spectrum;              // 1: privatise the spectrum in LDS
   int lds_palette;               // 1: u8 palette copied to LDS (single-phantom kernels)
+  int lds_xs511;                 // 1: the 511 keV total-cross-section row read from LDS
+  long long stream_stride;       // > 1: the k-th claimed stream is stream_base + k * stride (sampling)
   // Decay branching (drawn per decay only when active) and the singles list.
   float positron_frac;           // < 1: a decay emits the annihilation pair with this probability
   float prompt_frac;             // > 0: a decay emits a prompt gamma with this probability

@@ -91,6 +91,8 @@ struct mcpet_ctx {
   float* d_pj_out = nullptr;      // [3][D][H][W]: activity integral, the same / its maximum, attenuation factor; + 1 word
   int pj_dims[3] = {0, 0, 0};
   size_t pj_alloc = 0;            // grid size of d_pj_out's allocation
+  // mcpet_autotune's choices for runs that leave them 0 (0 / -1: none)
+  int tuned_bs = 0, tuned_per_cu = 0, tuned_policy = 0, tuned_lds = -1;
   std::string err;
 };
 
@@ -235,6 +237,46 @@ mcpet_status mcpet_check_create_args(const mcpet_config* cfg, const mcpet_phanto
     }
   }
   return MCPET_OK;
+}
+
+// Balanced stream layout: voxel v gets max(1, ceil(E_v / stream_decays)) streams (E_v: its
+// expected decays over the acquisition, teff = tau (1 - exp(-T / tau)) times its activity);
+// random-number stream ids count the streams in voxel order, the device processes the voxels
+// in its own (hottest first) order. act_dev: activities in device order; src_of[k]: the
+// position of device entry k in voxel order. Fills stream_start (n + 1) and stream_id (n) in
+// device order and returns the number of streams.
+static long long balanced_layout(const std::vector<float>& act_dev, const std::vector<size_t>& src_of, double teff,
+                                 long long stream_decays, std::vector<long long>& stream_start, std::vector<long long>& stream_id)
+{
+  const size_t n = act_dev.size();
+  auto streams_of = [&](float act) {
+    const double m = std::ceil((double)act * teff / (double)stream_decays);
+    return m < 1.0 ? 1LL : (m > 1.0e9 ? 1000000000LL : (long long)m);
+  };
+  std::vector<long long> first_id(n);   // in voxel order
+  for (size_t k = 0; k < n; k++) first_id[src_of[k]] = streams_of(act_dev[k]);
+  long long id = 0;
+  for (size_t j = 0; j < n; j++) {
+    const long long m = first_id[j];
+    first_id[j] = id;
+    id += m;
+  }
+  stream_start.resize(n + 1);
+  stream_id.resize(n);
+  long long s0 = 0;
+  for (size_t k = 0; k < n; k++) {
+    stream_start[k] = s0;
+    stream_id[k] = first_id[src_of[k]];
+    s0 += streams_of(act_dev[k]);
+  }
+  stream_start[n] = s0;
+  return id;
+}
+
+static double effective_time(const mcpet_config* cfg)
+{
+  const double tau = (double)cfg->mean_life, T = (double)cfg->acquisition_time_ps * 1.0e-12;
+  return tau * (1.0 - std::exp(-T / tau));
 }
 
 extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phantom* ph, const mcpet_tables* tab, mcpet_ctx** out)
@@ -404,29 +446,7 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
       delete c;
       return set_err(nullptr, MCPET_E_RANGE, "mcpet_create: too many emitting voxels for the balanced stream layout");
     }
-    const double tau = (double)cfg->mean_life, T = (double)cfg->acquisition_time_ps * 1.0e-12;
-    const double teff = tau * (1.0 - std::exp(-T / tau));
-    auto streams_of = [&](float act) {
-      const double n = std::ceil((double)act * teff / (double)cfg->stream_decays);
-      return n < 1.0 ? 1LL : (n > 1.0e9 ? 1000000000LL : (long long)n);
-    };
-    // first stream id of every emitting voxel, counted in voxel order (act_vox)
-    std::vector<long long> first_id(act_vox.size());
-    long long id = 0;
-    for (size_t k = 0; k < act_vox.size(); k++) {
-      first_id[k] = id;
-      id += streams_of(ph->voxel_mat_dens[3 * (size_t)act_vox[k] + 2]);
-    }
-    balanced_streams = id;
-    stream_start.resize(act_idx.size() + 1);
-    stream_id.resize(act_idx.size());
-    long long s0 = 0;
-    for (size_t k = 0; k < act_idx.size(); k++) {
-      stream_start[k] = s0;
-      stream_id[k] = first_id[src_of[k]];
-      s0 += streams_of(act_val[k]);
-    }
-    stream_start[act_idx.size()] = s0;
+    balanced_streams = balanced_layout(act_val, src_of, effective_time(cfg), cfg->stream_decays, stream_start, stream_id);
     if (cfg->detector.tally_psf_sinogram == MCPET_TALLY_SINGLES && balanced_streams >= (1LL << 32)) {
       free_all(c);
       delete c;
@@ -617,6 +637,8 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
                                         : (c->nvox - 1) * (long long)cfg->threads_per_block + (cfg->threads_per_block - 1);
   P.lds_spectrum = (cfg->ne > 0 && cfg->ne <= 8192) ? 1 : 0;
   P.lds_palette = 1;
+  P.lds_xs511 = 1;
+  P.stream_stride = 1;
   *out = c;
 #undef CK
 #undef HK
@@ -676,6 +698,16 @@ extern "C" mcpet_status mcpet_run(mcpet_ctx* c, const mcpet_run_params* rp)
   if (p.acquisition_time_ps != 0) P.acq_ps = p.acquisition_time_ps;
   // Phase selection (persistent kernel): most lanes by default (measured best on the head
   // workload); policy 2 weighs the phases by relative cost (DECAY, SOURCE2, STEP, INTERACT, TALLY).
+  // Launch shape, policy and LDS layout left 0 take mcpet_autotune's choices when it ran.
+  if (p.block_size <= 0 && c->tuned_bs > 0) p.block_size = c->tuned_bs;
+  if (p.blocks_per_cu <= 0 && c->tuned_per_cu > 0) p.blocks_per_cu = c->tuned_per_cu;
+  if (p.phase_policy <= 0 && c->tuned_policy > 0) p.phase_policy = c->tuned_policy;
+  const int lds = (p.lds_layout & MCPET_LDS_SET) ? p.lds_layout : c->tuned_lds;
+  if (lds >= 0) {
+    P.lds_palette = (lds & MCPET_LDS_PALETTE) ? 1 : 0;
+    P.lds_spectrum = (lds & MCPET_LDS_SPECTRUM) && c->P.lds_spectrum ? 1 : 0;   // only where the spectrum fits
+    P.lds_xs511 = (lds & MCPET_LDS_XS511) ? 1 : 0;
+  }
   P.sm_policy = (p.phase_policy == 2) ? 1 : 0;
   static const int default_cost[MCPET_NPHASES] = {4, 6, 1, 3, 6, 3, 2, 4, 4, 1};
   for (int i = 0; i < MCPET_NPHASES; i++) P.sm_cost[i] = p.phase_cost[i] > 0 ? p.phase_cost[i] : default_cost[i];
@@ -730,6 +762,146 @@ extern "C" mcpet_status mcpet_run(mcpet_ctx* c, const mcpet_run_params* rp)
   if (e == hipSuccess) e = mcpet_launch_pairing(P, c->d_keys[c->sorted], c->d_vals[c->sorted], c->n_singles, c->stream);
   if (e != hipSuccess) return set_err(c, MCPET_E_HIP, std::string("coincidence sorting failed: ") + hipGetErrorString(e));
   HIPCHK(c, hipEventRecord(c->ev_c1, c->stream));
+  return MCPET_OK;
+}
+
+// ---------------------------------------------------------------------------------- autotuner
+
+// The context's streams in another layout (stream_decays 0: the reference's threads_per_block
+// streams per emitting voxel; > 0: the balanced layout). The emitting voxels' device order and
+// activities are read back from the device; balanced arrays stay allocated once made.
+static mcpet_status apply_stream_layout(mcpet_ctx* c, long long stream_decays)
+{
+  KParams& P = c->P;
+  const long long n = c->n_active;
+  const long long tpb = c->cfg.threads_per_block;
+  if (stream_decays <= 0 || n <= 0) {
+    P.stream_start = nullptr;
+    P.stream_id = nullptr;
+    P.n_streams = n * tpb;
+    P.last_thread_id = (c->nvox - 1) * tpb + (tpb - 1);
+    c->cfg.stream_decays = 0;
+    return MCPET_OK;
+  }
+  std::vector<int> vox((size_t)n);
+  std::vector<float> act((size_t)n);
+  HIPCHK(c, hipMemcpy(vox.data(), c->d_active_vox, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(act.data(), c->d_active_act, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost));
+  std::vector<size_t> order((size_t)n), src_of((size_t)n);
+  for (size_t k = 0; k < order.size(); k++) order[k] = k;
+  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return vox[a] < vox[b]; });
+  for (size_t j = 0; j < order.size(); j++) src_of[order[j]] = j;   // voxel order = ascending voxel index
+  std::vector<long long> start, id;
+  const long long total = balanced_layout(act, src_of, effective_time(&c->cfg), stream_decays, start, id);
+  if (c->singles_mode && total >= (1LL << 32))
+    return set_err(c, MCPET_E_RANGE, "mcpet_autotune: more than 2^32 decay streams (singles event ids)");
+  if (!c->d_stream_start) {
+    mcpet_status st = dalloc(c, &c->d_stream_start, start.size());
+    if (st == MCPET_OK) st = dalloc(c, &c->d_stream_id, id.size());
+    if (st != MCPET_OK) return st;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(c->d_stream_start, start.data(), sizeof(long long) * start.size(), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->d_stream_id, id.data(), sizeof(long long) * id.size(), hipMemcpyHostToDevice));
+  P.stream_start = c->d_stream_start;
+  P.stream_id = c->d_stream_id;
+  P.n_streams = total;
+  P.last_thread_id = total - 1;
+  c->cfg.stream_decays = (int32_t)stream_decays;
+  return MCPET_OK;
+}
+
+// One timed sample: every stride-th decay stream over the whole acquisition, with the given
+// launch shape, phase policy and LDS layout; decays per millisecond of kernel time.
+static mcpet_status tune_sample(mcpet_ctx* c, long long stride, int bs, int per_cu, int policy, int lds, mcpet_tune_result* r)
+{
+  mcpet_run_params p{};
+  p.kernel = MCPET_KERNEL_PERSISTENT;
+  p.zero_tallies = 1;
+  p.block_size = bs;
+  p.blocks_per_cu = per_cu;
+  p.phase_policy = policy;
+  const int saved_lds = c->tuned_lds;
+  c->tuned_lds = lds;
+  c->P.stream_stride = stride;
+  mcpet_status st = mcpet_run(c, &p);
+  c->P.stream_stride = 1;
+  c->tuned_lds = saved_lds;
+  mcpet_stats s{};
+  if (st == MCPET_OK) st = mcpet_get_stats(c, &s);
+  if (st != MCPET_OK) return st;
+  r->block_size = bs;
+  r->blocks_per_cu = per_cu;
+  r->phase_policy = policy;
+  r->ms = s.kernel_ms;
+  r->decays_per_ms = s.kernel_ms > 0.0 ? (double)s.decays / s.kernel_ms : 0.0;
+  r->lds_layout = lds;
+  r->stream_decays = c->cfg.stream_decays;
+  return MCPET_OK;
+}
+
+extern "C" mcpet_status mcpet_autotune(mcpet_ctx* c, const mcpet_tune_options* opt, mcpet_tune_result* results,
+                                       int32_t max_results, int32_t* n_results)
+{
+  if (!c || !opt || !(opt->stream_fraction > 0.0 && opt->stream_fraction <= 1.0) || opt->n_stream_decays < 0 ||
+      opt->n_stream_decays > 8)
+    return set_err(c, MCPET_E_ARG, "mcpet_autotune: bad arguments");
+  for (int i = 0; i < opt->n_stream_decays; i++)
+    if (opt->stream_decays[i] < 0 || opt->stream_decays[i] > 0x7fffffffLL)
+      return set_err(c, MCPET_E_ARG, "mcpet_autotune: decays per stream outside [0, 2^31)");
+  if (c->P.n_streams <= 0) return set_err(c, MCPET_E_STATE, "mcpet_autotune: the phantom has no emitting voxel");
+  const long long stride = std::max(1LL, (long long)std::llround(1.0 / opt->stream_fraction));
+  int written = 0;
+  auto keep = [&](const mcpet_tune_result& r) {
+    if (results && written < max_results) results[written] = r;
+    written++;
+  };
+  // the layout's default: everything in LDS that applies to the phantom
+  int lds = MCPET_LDS_XS511 | (c->P.vox_fmt == 1 ? MCPET_LDS_PALETTE : 0) | (c->P.lds_spectrum ? MCPET_LDS_SPECTRUM : 0);
+  mcpet_tune_result r{}, best{};
+  best.decays_per_ms = -1.0;
+  mcpet_status st = tune_sample(c, stride, 256, 4, 1, lds, &r);   // warm-up (code load, caches)
+  if (st != MCPET_OK) return st;
+  // 1. launch shape x phase policy
+  static const int shapes[][2] = {{256, 4}, {256, 8}, {128, 8}};
+  for (const auto& sh : shapes)
+    for (int pol = 1; pol <= 2; pol++) {
+      if ((st = tune_sample(c, stride, sh[0], sh[1], pol, lds, &r)) != MCPET_OK) return st;
+      keep(r);
+      if (r.decays_per_ms > best.decays_per_ms) best = r;
+    }
+  // 2. LDS layout: each placement moved to global memory in turn, kept when faster
+  if (opt->tune_layout) {
+    for (int bit : {MCPET_LDS_PALETTE, MCPET_LDS_SPECTRUM, MCPET_LDS_XS511}) {
+      if (!(lds & bit)) continue;
+      if ((st = tune_sample(c, stride, best.block_size, best.blocks_per_cu, best.phase_policy, best.lds_layout & ~bit, &r)) !=
+          MCPET_OK)
+        return st;
+      keep(r);
+      if (r.decays_per_ms > best.decays_per_ms) best = r;
+    }
+  }
+  // 3. decays per stream (the reference's histories per thread): opt-in, it changes the
+  //    random-number streams
+  const long long own = c->cfg.stream_decays;
+  if (opt->tune_streams && opt->n_stream_decays > 0) {
+    for (int i = 0; i < opt->n_stream_decays; i++) {
+      if ((st = apply_stream_layout(c, opt->stream_decays[i])) != MCPET_OK) return st;
+      if ((st = tune_sample(c, stride, best.block_size, best.blocks_per_cu, best.phase_policy, best.lds_layout, &r)) != MCPET_OK)
+        return st;
+      keep(r);
+      if (r.decays_per_ms > best.decays_per_ms) best = r;
+    }
+  }
+  if ((st = apply_stream_layout(c, opt->tune_streams && opt->n_stream_decays > 0 ? best.stream_decays : own)) != MCPET_OK)
+    return st;
+  c->tuned_bs = best.block_size;
+  c->tuned_per_cu = best.blocks_per_cu;
+  c->tuned_policy = best.phase_policy;
+  c->tuned_lds = best.lds_layout;
+  c->ran = false;   // the samples' tallies are not a run's
+  keep(best);       // the choice, after the timings
+  if (n_results) *n_results = std::min(written, std::max(0, max_results));
   return MCPET_OK;
 }
 
@@ -1510,7 +1682,8 @@ extern "C" mcpet_status mcpet_batch_autotune(mcpet_batch* b, double fraction, mc
       decays += d;
     }
     const double rate = ms > 0.0f ? (double)decays / ms : 0.0;
-    if (results && written < max_results) results[written++] = {shapes[i][0], shapes[i][1], pol, (double)ms, rate};
+    if (results && written < max_results)
+      results[written++] = {shapes[i][0], shapes[i][1], pol, (double)ms, rate, MCPET_LDS_XS511, b->ctx[0]->cfg.stream_decays};
     if (rate > best) {
       best = rate;
       best_i = i;

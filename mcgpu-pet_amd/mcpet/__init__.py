@@ -115,12 +115,23 @@ class RunParams(C.Structure):
     _fields_ = [("kernel", C.c_int32), ("block_size", C.c_int32), ("blocks_per_cu", C.c_int32),
                 ("zero_tallies", C.c_int32), ("seed", C.c_int32), ("acquisition_time_ps", C.c_uint64),
                 ("phase_policy", C.c_int32), ("phase_cost", C.c_int32 * NPHASES),
-                ("count_events", C.c_int32), ("use_graph", C.c_int32)]
+                ("count_events", C.c_int32), ("use_graph", C.c_int32), ("lds_layout", C.c_int32)]
+
+
+LDS_PALETTE, LDS_SPECTRUM, LDS_XS511 = 1, 2, 4
 
 
 class TuneResult(C.Structure):
     _fields_ = [("block_size", C.c_int32), ("blocks_per_cu", C.c_int32), ("phase_policy", C.c_int32), ("ms", C.c_double),
-                ("decays_per_ms", C.c_double)]
+                ("decays_per_ms", C.c_double), ("lds_layout", C.c_int32), ("stream_decays", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class TuneOptions(C.Structure):
+    _fields_ = [("stream_fraction", C.c_double), ("tune_layout", C.c_int32), ("tune_streams", C.c_int32),
+                ("n_stream_decays", C.c_int32), ("stream_decays", C.c_int64 * 8)]
 
 
 class Stats(C.Structure):
@@ -184,6 +195,7 @@ def lib():
     L.mcpet_stream_seeds.argtypes = [C.c_int32, P(C.c_int64), C.c_int32, P(C.c_int32), P(C.c_int32)]
     L.mcpet_version.restype = C.c_char_p
     L.mcpet_selftest_sort.argtypes = [P(C.c_uint64), C.c_int64, C.c_int32, P(C.c_uint32)]
+    L.mcpet_autotune.argtypes = [C.c_void_p, P(TuneOptions), P(TuneResult), C.c_int32, P(C.c_int32)]
     L.mcpet_selftest_gemm_bf16.argtypes = [P(C.c_float), P(C.c_float), C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                            P(C.c_float)]
     L.mcpet_struct_sizes.argtypes = [C.POINTER(C.c_size_t), C.c_int32]
@@ -393,12 +405,33 @@ class Simulation:
             raise McpetError(st, lib().mcpet_last_error(None).decode(errors="replace"))
 
     def run(self, kernel=KERNEL_AUTO, block_size=0, blocks_per_cu=0, zero_tallies=True, seed=0, acquisition_time_ps=0,
-            phase_policy=0, phase_cost=None, count_events=False, use_graph=False):
+            phase_policy=0, phase_cost=None, count_events=False, use_graph=False, lds_layout=None):
         """Launch the transport (asynchronous). count_events selects the instrumented
-        persistent kernel, which fills every event count of stats()."""
+        persistent kernel, which fills every event count of stats(). lds_layout: LDS_* bits of
+        an explicit LDS layout (None: the autotuned or default one)."""
         rp = _run_params(kernel, block_size, blocks_per_cu, zero_tallies, seed, acquisition_time_ps, phase_policy,
-                         phase_cost, count_events, use_graph)
+                         phase_cost, count_events, use_graph, lds_layout)
         _check(lib().mcpet_run(self.ctx, C.byref(rp)), self.ctx)
+
+    def autotune(self, fraction=0.02, layout=True, stream_decays=None):
+        """Time launch shapes, phase policies, the LDS layout and (if stream_decays lists
+        candidates, 0 = threads_per_block streams per voxel) the decays per stream on every
+        round(1/fraction)-th stream; later runs use the fastest. Returns the timed candidates,
+        the last entry being the choice."""
+        o = TuneOptions()
+        o.stream_fraction = float(fraction)
+        o.tune_layout = 1 if layout else 0
+        cands = list(stream_decays or [])
+        if len(cands) > 8:
+            raise ValueError("at most 8 stream-layout candidates")
+        o.tune_streams = 1 if cands else 0
+        o.n_stream_decays = len(cands)
+        for i, v in enumerate(cands):
+            o.stream_decays[i] = int(v)
+        res = (TuneResult * 32)()
+        n = C.c_int32()
+        _check(lib().mcpet_autotune(self.ctx, C.byref(o), res, 32, C.byref(n)), self.ctx)
+        return [r.as_dict() for r in res[:n.value]]
 
     def sync(self):
         _check(lib().mcpet_sync(self.ctx), self.ctx)
@@ -455,11 +488,14 @@ class Simulation:
             pass
 
 
+LDS_SET = 8
+
+
 def _run_params(kernel=KERNEL_AUTO, block_size=0, blocks_per_cu=0, zero_tallies=True, seed=0, acquisition_time_ps=0,
-                phase_policy=0, phase_cost=None, count_events=False, use_graph=False):
+                phase_policy=0, phase_cost=None, count_events=False, use_graph=False, lds_layout=None):
     return RunParams(kernel, block_size, blocks_per_cu, 1 if zero_tallies else 0, seed, acquisition_time_ps, phase_policy,
                      (C.c_int32 * NPHASES)(*(phase_cost or [0] * NPHASES)), 1 if count_events else 0,
-                     1 if use_graph else 0)
+                     1 if use_graph else 0, 0 if lds_layout is None else (LDS_SET | int(lds_layout)))
 
 
 def _fetch_tallies(fetch_fn, cfg, nv, sinograms=True, images=True, spectrum=True, psf=True, dose=True, singles=True):
@@ -702,8 +738,7 @@ class Batch:
         res = (TuneResult * 32)()
         n = C.c_int32()
         self._check(lib().mcpet_batch_autotune(self.b, float(fraction), res, 32, C.byref(n)))
-        return [{"block_size": r.block_size, "blocks_per_cu": r.blocks_per_cu, "phase_policy": r.phase_policy, "ms": r.ms,
-                 "decays_per_ms": r.decays_per_ms} for r in res[:n.value]]
+        return [r.as_dict() for r in res[:n.value]]
 
     def close(self):
         if self.b:
