@@ -596,6 +596,14 @@ mcpet_status mcpet_selftest_fmath(const double* x, const double* y, int32_t n, d
  * sort of the singles pipeline; perm receives the sorted order of the input indices. */
 mcpet_status mcpet_selftest_sort(const uint64_t* keys, int64_t n, int32_t passes, uint32_t* perm);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the same sort as
+ * mcpet_selftest_sort, same arguments and results, calling no hipMalloc/hipFree. Its device
+ * buffers come from one static device array of about 24.3 MiB that each device the library runs
+ * on holds until the process ends; n is limited to 2^20 keys (MCPET_E_CAPACITY beyond). Calls
+ * that overlap on one device share that array and race. */
+mcpet_status mcpet_selftest_sort_noalloc(const uint64_t* keys, int64_t n, int32_t passes, uint32_t* perm);
+
 /* C = A (MxK) x B (KxN), row-major FP32 in and out, computed by the surrogate's BF16 MFMA GEMM
  * kernel (operands rounded to BF16, FP32 accumulation; split > 1: K split over workgroups). */
 mcpet_status mcpet_selftest_gemm_bf16(const float* a, const float* b, int32_t M, int32_t N, int32_t K, int32_t split, float* c);

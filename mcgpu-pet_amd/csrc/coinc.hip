@@ -20,10 +20,14 @@
 // singles with equal times in the order of (event, kind)).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "devutil.h"
+#include "host_internal.h"
 #include "kparams.h"
+#include "mcpet.h"
 
 #define RS_THREADS 256
 #define RS_ROUNDS 16
@@ -278,4 +282,83 @@ hipError_t mcpet_launch_gather_singles(const mcpet_single* s, const unsigned* pe
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_gather_singles, dim3(nblocks(n, 256)), dim3(256), 0, st, s, perm, n, out);
   return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------- allocation-free sort
+// mcpet_selftest_sort_noalloc: the same sort as mcpet_selftest_sort with its device buffers (both
+// key/payload buffers, the digit histogram table and the scan's per-level tile sums) carved out
+// of one static device array, for callers that cannot allocate device memory. The array is
+// loaded with the code object, so every device the library runs on holds it until the process
+// ends. Its contents are not cleared between calls: nothing in it needs to start at zero (the
+// keys and payload are written first, every histogram entry is written by k_rs_hist, every scan
+// sum by k_scan_tile), and two calls that overlap on one device race on it.
+#define NA_MAX_KEYS (1LL << 20)
+#define NA_HIST_WORDS (256LL * ((NA_MAX_KEYS + RS_TILE - 1) / RS_TILE))
+#define NA_LEVEL_WORDS 1024LL
+#define NA_BYTES (2 * 8 * NA_MAX_KEYS + 2 * 4 * NA_MAX_KEYS + 4 * NA_HIST_WORDS + 4 * NA_LEVEL_WORDS)
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+namespace {
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mutex;
+void* g_na_ptr[kNaMaxDevices] = {};   // device address of `scratch` on each device
+
+mcpet_status na_fail(mcpet_status st, const std::string& msg)
+{
+  mcpet_host_set_error(msg.c_str());
+  return st;
+}
+}  // namespace
+
+extern "C" mcpet_status mcpet_selftest_sort_noalloc(const uint64_t* keys, int64_t n, int32_t passes, uint32_t* perm)
+{
+  if (!keys || !perm || n < 0 || passes < 1 || passes > 8)
+    return na_fail(MCPET_E_ARG, "mcpet_selftest_sort_noalloc: bad arguments");
+  if (n > NA_MAX_KEYS)
+    return na_fail(MCPET_E_CAPACITY, "mcpet_selftest_sort_noalloc: " + std::to_string(n) + " keys exceed the static scratch's " +
+                                         std::to_string(NA_MAX_KEYS) + " (use mcpet_selftest_sort)");
+  if (n == 0) return MCPET_OK;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return na_fail(MCPET_E_HIP, std::string("mcpet_selftest_sort_noalloc: hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= kNaMaxDevices) return na_fail(MCPET_E_RANGE, "mcpet_selftest_sort_noalloc: device index out of range");
+  unsigned char* base = nullptr;
+  {
+    // first call on this device (current): the array's address on this device
+    std::lock_guard<std::mutex> lock(g_na_mutex);
+    if (!g_na_ptr[dev]) {
+      void* p = nullptr;
+      e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p)
+        return na_fail(MCPET_E_HIP, std::string("mcpet_selftest_sort_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+      g_na_ptr[dev] = p;
+    }
+    base = static_cast<unsigned char*>(g_na_ptr[dev]);
+  }
+  const long long m = 256LL * mcpet_rs_tiles(n);
+  std::vector<long long> levels;
+  mcpet_scan_levels(m, levels);
+  long long level_words = 0;
+  for (long long l : levels) level_words += l;
+  if (m > NA_HIST_WORDS || level_words > NA_LEVEL_WORDS)
+    return na_fail(MCPET_E_CAPACITY, "mcpet_selftest_sort_noalloc: scan tables exceed the static scratch");
+  unsigned long long* k[2] = {reinterpret_cast<unsigned long long*>(base),
+                              reinterpret_cast<unsigned long long*>(base + 8 * NA_MAX_KEYS)};
+  unsigned* v[2] = {reinterpret_cast<unsigned*>(base + 16 * NA_MAX_KEYS), reinterpret_cast<unsigned*>(base + 20 * NA_MAX_KEYS)};
+  unsigned* hist = reinterpret_cast<unsigned*>(base + 24 * NA_MAX_KEYS);
+  std::vector<unsigned*> scan(levels.size());
+  unsigned* lp = hist + NA_HIST_WORDS;
+  for (size_t i = 0; i < levels.size(); i++) {
+    scan[i] = lp;
+    lp += levels[i];
+  }
+  int which = 0;
+  e = hipMemcpy(k[0], keys, 8 * (size_t)n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = mcpet_launch_iota(v[0], n, 0);
+  if (e == hipSuccess) e = mcpet_radix_sort(n, passes, k, v, hist, scan.data(), &which, 0);
+  if (e == hipSuccess) e = hipStreamSynchronize(0);
+  if (e == hipSuccess) e = hipMemcpy(perm, v[which], 4 * (size_t)n, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return na_fail(MCPET_E_HIP, std::string("mcpet_selftest_sort_noalloc: ") + hipGetErrorString(e));
+  return MCPET_OK;
 }

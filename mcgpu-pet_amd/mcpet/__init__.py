@@ -196,6 +196,7 @@ def lib():
     L.mcpet_version.restype = C.c_char_p
     L.mcpet_selftest_sort.argtypes = [P(C.c_uint64), C.c_int64, C.c_int32, P(C.c_uint32)]
     L.mcpet_autotune.argtypes = [C.c_void_p, P(TuneOptions), P(TuneResult), C.c_int32, P(C.c_int32)]
+    L.mcpet_selftest_sort_noalloc.argtypes = [P(C.c_uint64), C.c_int64, C.c_int32, P(C.c_uint32)]
     L.mcpet_selftest_gemm_bf16.argtypes = [P(C.c_float), P(C.c_float), C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                            P(C.c_float)]
     L.mcpet_struct_sizes.argtypes = [C.POINTER(C.c_size_t), C.c_int32]
@@ -846,12 +847,13 @@ def selftest_fmath(x, y):
     return od, of
 
 
-def selftest_sort(keys, passes=8):
-    """Device radix sort of uint64 keys (low 8*passes bits); returns the stable sorted order."""
+def selftest_sort(keys, passes=8, noalloc=False):
+    """Device radix sort of uint64 keys (low 8*passes bits); returns the stable sorted order.
+    noalloc: the allocation-free entry point (static device scratch, at most 2^20 keys)."""
     k = np.ascontiguousarray(keys, dtype=np.uint64)
     perm = np.zeros(len(k), np.uint32)
-    _check(lib().mcpet_selftest_sort(k.ctypes.data_as(C.POINTER(C.c_uint64)), len(k), int(passes),
-                                     perm.ctypes.data_as(C.POINTER(C.c_uint32))))
+    fn = lib().mcpet_selftest_sort_noalloc if noalloc else lib().mcpet_selftest_sort
+    _check(fn(k.ctypes.data_as(C.POINTER(C.c_uint64)), len(k), int(passes), perm.ctypes.data_as(C.POINTER(C.c_uint32))))
     return perm
 
 
