@@ -102,6 +102,26 @@ def cpu_baseline(cfg, ph, vs, names, threads, stride, e_max=0.0):
             "host": host_description()}
 
 
+def singles_roofline(steps, acq_ps):
+    """HBM roofline of the singles pipeline (time sort + pairing, coinc.hip) from its HIP-event
+    time. Algorithmic bytes per single: the key/index build reads the 32-byte record and writes
+    an 8-byte key and a 4-byte index (44 B); every radix pass reads the keys for the digit
+    histogram (8 B) and reads and writes key and index in the scatter (2 x 12 B), 32 B a pass;
+    the pairing reads a single's key and index and its neighbours' keys (about 20 B)."""
+    bits = int(acq_ps + (1 << 40)).bit_length()
+    passes = (bits + 7) // 8
+    per_single = 44 + 32 * passes + 20
+    n = sum(x for x, _, _ in steps) / len(steps)
+    ms = sum(t for _, t, _ in steps) / len(steps)
+    achieved = n * per_single / (ms * 1e-3)
+    classes = {k: sum(c[k] for _, _, c in steps) / len(steps) for k in steps[0][2]}
+    return {"singles_per_step": n, "coincidences_per_step": classes, "sort_and_pairing_ms": ms,
+            "singles_per_s": n / (ms * 1e-3), "radix_passes": passes,
+            "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK, "algorithmic_bytes_per_single": per_single,
+                         "kernels": "k_keys_from_singles, k_rs_hist, k_scan_tile, k_scan_add, k_rs_scatter, k_pair"}}
+
+
 def bench_batch(a):
     """Many-phantom workload: a.batch_size varied phantoms (random_phantoms: 32-160 x 32-160 x
     32-96 voxels at 2 mm, 1e6-1e8 photon histories each, 10 s), sinogram 168x168x1293 each.
@@ -274,7 +294,7 @@ def bench_surrogate(a):
     if not os.path.exists(weights):
         raise SystemExit(f"bench.py: surrogate weights {weights} missing (train with python -m mcpet.surrogate)")
     specs = eval_phantoms(a.eval_family, a.eval_phantoms, a.eval_seed)
-    est = S.ScatterEstimator(weights)
+    est = S.ScatterEstimator(weights, width=None)
     rows = time_to_nrmse(specs, est, a.steps, a.warmup)
     roof = unet_forward_roofline(est)
     est.close()
@@ -286,7 +306,7 @@ def bench_surrogate(a):
            "dtype": "bf16 MFMA (fp32 accumulate) + f32 transport", "data": "synthetic",
            "config": {"workload": f"{len(rows)} evaluation phantoms ({EVAL_FAMILIES[a.eval_family]}, seed "
                                   f"{a.eval_seed}), scatter on the 32x56x56 rebinned grid, surrogate input "
-                                  f"{S.N_LOW:.0e} decays + activity/attenuation maps, U-Net width {S.WIDTH}",
+                                  f"{S.N_LOW:.0e} decays + activity/attenuation maps, U-Net width {est.width}",
                       "parallelism": "single GPU"},
            "mc_seconds_mean": tot_mc / len(rows), "surrogate_seconds_mean": tot_eff / len(rows),
            "surrogate_nrmse_max": max(r["surrogate_nrmse_max"] for r in rows),
@@ -301,10 +321,11 @@ def time_to_5pct_object(a):
     family (scatter, head, batch; seed disjoint from training), with the U-Net roofline."""
     from mcpet import surrogate as S
     specs = [eval_phantoms(f, 1, a.eval_seed + k)[0] for k, f in enumerate(("scatter", "head", "batch"))]
-    est = S.ScatterEstimator(a.surrogate_weights or S.WEIGHTS)
+    est = S.ScatterEstimator(a.surrogate_weights or S.WEIGHTS, width=None)
     steps = max(1, min(a.steps, 5))
     rows = time_to_nrmse(specs, est, steps, 1, max_ref_runs=5)
     roof = unet_forward_roofline(est)
+    width = est.width
     est.close()
     tot_mc = sum(r["mc_seconds"] for r in rows)
     tot_eff = sum(r["effective_seconds"] for r in rows)
@@ -313,7 +334,7 @@ def time_to_5pct_object(a):
             "workload": f"one evaluation phantom per family (scatter 128x128x64, head 128x128x96, batch random; seeds "
                         f"{a.eval_seed}-{a.eval_seed + 2}), scatter sinogram rebinned to 32x56x56, Monte Carlo "
                         f"reference of up to 2e10 decays, surrogate = {S.N_LOW:.0e}-decay run + maps + U-Net width "
-                        f"{S.WIDTH}; medians of {steps} timed runs each",
+                        f"{width}; medians of {steps} timed runs each",
             "mc_seconds": [r["mc_seconds"] for r in rows], "surrogate_seconds": [r["effective_seconds"] for r in rows],
             "mc_nrmse": [r["mc_nrmse"] for r in rows], "surrogate_nrmse": [r["surrogate_nrmse"] for r in rows],
             "reference_floor": [r["reference_floor"] for r in rows],
@@ -376,7 +397,7 @@ def bench_serve(a):
     phs = [mcpet.phantom_from_arrays(*sp) for sp in specs]
     names = S._materials()
     tab = mcpet.load_materials(names, np.max([p.density_max() for p in phs], axis=0))
-    est = S.ScatterEstimator(a.surrogate_weights or S.WEIGHTS)
+    est = S.ScatterEstimator(a.surrogate_weights or S.WEIGHTS, width=None)
     # warm-up: one request alone (kernels loaded, per-device buffers and crystal-pair lists built)
     svc = S.ScatterService(tab, est, budget_bytes=a.serve_budget_gb << 30, max_running=16)
     svc.submit(phs[0], seed=1)
@@ -462,6 +483,9 @@ def main():
     ap.add_argument("--serve-chunk", type=int, default=1 << 20,
                     help="serve workload: decay streams per request and serving-loop step")
     ap.add_argument("--spot-every", type=int, default=20, help="serve workload: spot-check every k-th request")
+    ap.add_argument("--singles", action="store_true",
+                    help="torso workload: the singles pipeline (time sort, 4.5 ns coincidence window, randoms) "
+                         "instead of the fused pair tallies")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="CPU baseline threads (0: the CPUs this process may use, capped by OMP_NUM_THREADS)")
@@ -502,14 +526,19 @@ def main():
         mat, dens, act, vs = phantoms.torso(n_decays=n_decays)
         names = phantoms.torso_materials(MATERIALS)
         nz, ny, nx = mat.shape
+        # singles pipeline: about 0.35 detected singles per decay; room for 0.43 (the list's
+        # 32-bit indices allow 2^31 - 1 records)
+        singles_kw = dict(tally_psf_sinogram=3, singles_capacity=int(min(2 ** 31 - 1, 0.5 * n_decays + (1 << 20))),
+                          coincidence_window_ps=4500) if a.singles else {}
         cfg = phantoms.make_config(seed=20261202, threads_per_block=1, acquisition_time_s=10.0, nrad=168, radius=-46.0,
                                    materials=names, positron_fraction=0.5, prompt_gamma_fraction=1.0,
                                    prompt_gamma_energy=603.0e3, material_dose=True, voxel_dose=True,
                                    roi=(0, nx - 1, 0, ny - 1, 0, nz - 1),
-                                   stream_decays=int(max(4, np.ceil(n_decays / 2.0 ** 20))))
+                                   stream_decays=int(max(4, np.ceil(n_decays / 2.0 ** 20))), **singles_kw)
         workload = (f"torso phantom 256x256x200 at 2.5 mm, 7 tissues, 50% beta+ and a 603 keV prompt gamma per decay, "
-                    f"{nd} decays in 10 s, sinogram 168x168x1293, 64-bit voxel and material dose, fused "
-                    f"transport + tallies")
+                    f"{nd} decays in 10 s, sinogram 168x168x1293, 64-bit voxel and material dose, " +
+                    ("singles list, time sort and 4.5 ns coincidence window (trues, scatter, randoms, spurious)"
+                     if a.singles else "fused transport + tallies"))
     else:
         mat, dens, act, vs = phantoms.head(n_decays=n_decays)
         names = phantoms.head_materials(MATERIALS)
@@ -532,6 +561,7 @@ def main():
     coinc = decays = 0
     kernel_ms = []
     per_step = []
+    singles = []
     t0 = time.perf_counter()
     for s in range(a.steps):
         sim.run(seed=2000 + s, **run_kw)
@@ -540,6 +570,10 @@ def main():
         decays += st["decays"]
         kernel_ms.append(st["kernel_ms"])
         per_step.append((st["decays"], st["coincidences"]))
+        if a.singles:
+            cc = sim.fetch(sinograms=False, images=False, spectrum=False, psf=False, dose=False,
+                           singles=False)["coincidences"]
+            singles.append((st["singles"], st["coinc_ms"], cc))
     sim.sync()
     elapsed = time.perf_counter() - t0
     # Algorithmic event counts of the timed steps: the same seeds replayed (untimed) with the
@@ -587,7 +621,9 @@ def main():
                                   "cycle_share": c / tot_cyc, "cycles_per_exec": c / max(1, e)}
                               for n, e, l, c in zip(("decay", "source2", "step", "interact", "tally", "ctau", "cpz", "detect", "source3"),
                                                     agg["phase_exec"], agg["phase_lanes"], agg["phase_cycles"])}
-    if not a.no_cpu_baseline:
+    if singles:
+        out["singles_pipeline"] = singles_roofline(singles, cfg.acquisition_time_ps)
+    if not a.no_cpu_baseline and not a.singles:
         out["cpu_baseline"] = cpu_baseline(cfg, ph, vs, names, baseline_threads(a.cpu_threads), stride, e_max)
     if not torso and not a.no_surrogate:
         out["time_to_5pct_nrmse"] = time_to_5pct_object(a)

@@ -42,13 +42,14 @@ static __device__ __forceinline__ T gatomic_min(T* p, T v)
   return __hip_atomic_fetch_min((DU_GLOBAL T*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// __float2int_rd with NaN -> INT_MIN and saturation (matches the host checker).
+// __float2int_rd with NaN -> INT_MIN and saturation (matches the host checker), without
+// branches: floor, clamp into int range (a NaN clamps to -2^31), convert, and select INT_MAX
+// for values from 2^31 up (2147483520 is the largest float below 2^31).
 static __device__ __forceinline__ int f2i_rd(float x)
 {
-  float f = floorf(x);
-  if (!(f >= -2147483648.0f)) return INT_MIN;
-  if (f >= 2147483648.0f) return INT_MAX;
-  return (int)f;
+  const float f = floorf(x);
+  const int r = (int)fminf(fmaxf(f, -2147483648.0f), 2147483520.0f);
+  return (f >= 2147483648.0f) ? INT_MAX : r;
 }
 static __device__ __forceinline__ unsigned long long d2ull_rn(double x)
 {

@@ -131,17 +131,19 @@ __device__ __forceinline__ double rnd(Rng& r, CN& C)
 }
 
 // locate_voxel (K:1097-1124). NaN positions count as outside and indices are clamped
-// to the grid (the reference would read out of bounds in both cases).
+// to the grid (the reference would read out of bounds in both cases). Evaluated without
+// branches: the six bound tests (a NaN coordinate fails them) combine into one select, and
+// inside the box every floor(p * inverse size) lies in [0, n], so it converts directly after
+// clamping to [0, n - 1]; the same index as the reference's early returns and
+// __float2int_rd.
 __device__ __forceinline__ int locate(const KParams& P, const float* p)
 {
-  if ((p[1] < K_EPS) || (p[1] > (P.by - K_EPS)) || (p[0] < K_EPS) || (p[0] > (P.bx - K_EPS)) || (p[2] < K_EPS) ||
-      (p[2] > (P.bz - K_EPS)))
-    return -100;
-  if (!(p[0] == p[0] && p[1] == p[1] && p[2] == p[2])) return -100;
-  int vx = min(f2i_rd(p[0] * P.ivx), P.nx - 1);
-  int vy = min(f2i_rd(p[1] * P.ivy), P.ny - 1);
-  int vz = min(f2i_rd(p[2] * P.ivz), P.nz - 1);
-  return (vx + vy * (P.nx) + vz * (P.nx) * (P.ny));
+  const bool inside = (p[1] >= K_EPS) & (p[1] <= (P.by - K_EPS)) & (p[0] >= K_EPS) & (p[0] <= (P.bx - K_EPS)) &
+                      (p[2] >= K_EPS) & (p[2] <= (P.bz - K_EPS));
+  const int vx = (int)fminf(fmaxf(floorf(p[0] * P.ivx), 0.0f), (float)(P.nx - 1));
+  const int vy = (int)fminf(fmaxf(floorf(p[1] * P.ivy), 0.0f), (float)(P.ny - 1));
+  const int vz = (int)fminf(fmaxf(floorf(p[2] * P.ivz), 0.0f), (float)(P.nz - 1));
+  return inside ? (vx + vy * (P.nx) + vz * (P.nx) * (P.ny)) : -100;
 }
 
 // rotate_double (K:1159-1200): PENELOPE DIRECT in double precision on a float vector.

@@ -119,13 +119,30 @@ def param_count(in_channels, width):
     return sum(co * ci * k ** 3 + co for ci, co, k in layers)
 
 
+def network_shape(n_params):
+    """(in_channels, width) of the U-Net with n_params parameters (2 or 4 input channels,
+    width 1..256); None if none fits."""
+    for width in range(1, 257):
+        for c in (IN_CHANNELS, 2):
+            if param_count(c, width) == n_params:
+                return c, width
+    return None
+
+
 class ScatterEstimator:
     """The trained U-Net and the estimate / fallback policy. in_channels None: 2 or 4, from the
-    size of the weights (IN_CHANNELS without weights)."""
+    size of the weights (IN_CHANNELS without weights); width None: from the size of the weights
+    as well."""
 
     def __init__(self, weights=WEIGHTS, width=WIDTH, grid=GRID, z_max=5.0, in_channels=None):
         if isinstance(weights, str):
             weights = np.load(weights, allow_pickle=False)
+        if width is None:
+            shape = network_shape(np.size(weights)) if weights is not None else (IN_CHANNELS, WIDTH)
+            if shape is None:
+                raise ValueError(f"{np.size(weights)} weights fit no U-Net of width 1-256")
+            width = shape[1]
+        self.width = int(width)
         if in_channels is None:
             in_channels = IN_CHANNELS
             if weights is not None:
@@ -382,11 +399,12 @@ def generate(n_phantoms, seed, n_target=5.0e8, n_low=N_LOW, inputs_per_phantom=4
     return np.stack(X), np.stack(Y), np.array(idx)
 
 
-def train(X, Y, idx, width=WIDTH, epochs=150, batch=8, lr=5e-4, warmup=200, val_fraction=0.125, seed=1, log=print):
-    in_channels = X.shape[1]
+def train(X, Y, idx, width=WIDTH, epochs=150, batch=8, lr=5e-4, warmup=200, val_fraction=0.125, seed=1, log=print,
+          checkpoint=None):
     """Adam on the MSE, learning rate warmed up linearly over `warmup` steps then decayed on a
     cosine; the parameters of the epoch with the lowest validation loss (phantoms held out by
-    index) are kept."""
+    index) are kept, and written to `checkpoint` (a .npy path) whenever they improve."""
+    in_channels = X.shape[1]
     rng = np.random.default_rng(seed)
     phantom_ids = np.unique(idx)
     n_val = max(1, int(len(phantom_ids) * val_fraction))
@@ -416,7 +434,9 @@ def train(X, Y, idx, width=WIDTH, epochs=150, batch=8, lr=5e-4, warmup=200, val_
         history.append({"epoch": ep, "train_loss": float(np.mean(tl)), "val_loss": vl, "val_nrmse": vn})
         if vl < best[0]:
             best = (vl, net.params(), ep)
-        if ep < 5 or ep % 10 == 0 or ep == epochs - 1:
+            if checkpoint:
+                np.save(checkpoint, best[1].astype(np.float32))
+        if ep < 5 or ep % 5 == 0 or ep == epochs - 1:
             log(f"train: epoch {ep} train {np.mean(tl):.5f} val {vl:.5f} val nrmse {vn:.4f} "
                 f"({time.perf_counter() - t0:.1f} s)")
     net.close()
@@ -446,7 +466,8 @@ def main(argv=None):
                          in_channels=a.in_channels, log=log)
     gen_s = time.perf_counter() - t0
     log(f"generate: {len(X)} samples in {gen_s:.1f} s")
-    w, info = train(X, Y, idx, width=a.width, epochs=a.epochs, lr=a.lr, log=log)
+    w, info = train(X, Y, idx, width=a.width, epochs=a.epochs, lr=a.lr, log=log,
+                    checkpoint=os.path.join(a.out, f"scatter_unet_w{a.width}.npy"))
     np.save(os.path.join(a.out, f"scatter_unet_w{a.width}.npy"), w.astype(np.float32))
     info.update({"phantoms": a.phantoms, "families": fams, "phantom_seed": a.seed, "n_target": a.n_target, "n_low": N_LOW,
                  "inputs_per_phantom": a.inputs_per_phantom, "grid": GRID, "width": a.width, "in_channels": a.in_channels,
