@@ -64,23 +64,52 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const unsigned long long
   hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// Stable scatter of one tile by digit.
+// Stable scatter of one tile by digit, through LDS: every key's place in the tile ordered by
+// digit (stable: by round, wave and lane, the input order) is found with eight ballots per wave
+// and the counts of earlier waves and rounds; the keys and indices are written there in LDS, and
+// the tile is then written out in that order, so that consecutive lanes store consecutive
+// elements of one digit's run (its bucket in the output) instead of up to 64 different buckets.
+// offs is the digit-major table of global offsets (exclusive scan of the tile histograms): the
+// tile's count of digit d is the next entry minus this one.
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const unsigned long long* __restrict__ kin,
                                                            const unsigned* __restrict__ vin, unsigned long long* __restrict__ kout,
                                                            unsigned* __restrict__ vout, long long n, int shift,
                                                            const unsigned* __restrict__ offs, int ntiles)
 {
-  __shared__ unsigned s_base[256];
+  __shared__ unsigned long long s_key[RS_TILE];
+  __shared__ unsigned s_val[RS_TILE];
+  __shared__ unsigned s_gbase[256];                 // output offset of digit d for this tile
+  __shared__ unsigned s_start[256];                 // start of digit d in the tile's sorted order
+  __shared__ unsigned s_run[256];                   // keys of digit d placed so far
   __shared__ unsigned s_wcnt[RS_THREADS / 64][256];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const unsigned long long lt = (1ull << lane) - 1ull;
-  s_base[tid] = offs[(size_t)tid * ntiles + blockIdx.x];
   const long long base = (long long)blockIdx.x * RS_TILE;
+  const int tile_n = (int)min((long long)RS_TILE, n - base);
+  {
+    const size_t e = (size_t)tid * ntiles + blockIdx.x;
+    const unsigned g = offs[e];
+    const unsigned next = (e + 1 < (size_t)256 * ntiles) ? offs[e + 1] : (unsigned)n;
+    s_gbase[tid] = g;
+    s_start[tid] = next - g;                        // this tile's count of digit tid, scanned below
+    s_run[tid] = 0u;
+  }
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {         // inclusive scan of the counts over digits
+    const unsigned x = tid >= off ? s_start[tid - off] : 0u;
+    __syncthreads();
+    s_start[tid] += x;
+    __syncthreads();
+  }
+  const unsigned excl = tid > 0 ? s_start[tid - 1] : 0u;   // exclusive: the inclusive scan shifted by one digit
+  __syncthreads();
+  s_start[tid] = excl;
+  __syncthreads();
   for (int r = 0; r < RS_ROUNDS; r++) {
-    const long long i = base + (long long)r * RS_THREADS + tid;
-    const bool valid = i < n;
-    const unsigned long long key = valid ? kin[i] : 0ull;
-    const unsigned val = valid ? vin[i] : 0u;
+    const int j = r * RS_THREADS + tid;
+    const bool valid = j < tile_n;
+    const unsigned long long key = valid ? kin[base + j] : 0ull;
+    const unsigned val = valid ? vin[base + j] : 0u;
     const unsigned d = (unsigned)(key >> shift) & 255u;
     unsigned long long m = __ballot(valid);
 #pragma unroll
@@ -96,17 +125,25 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const unsigned long l
     if (valid && rank == 0u) s_wcnt[wave][d] = (unsigned)__popcll(m);
     __syncthreads();
     if (valid) {
-      unsigned pos = s_base[d] + rank;
+      unsigned pos = s_start[d] + s_run[d] + rank;
       for (int w = 0; w < wave; w++) pos += s_wcnt[w][d];
-      kout[pos] = key;
-      vout[pos] = val;
+      s_key[pos] = key;
+      s_val[pos] = val;
     }
     __syncthreads();
     unsigned add = 0u;
 #pragma unroll
     for (int w = 0; w < RS_THREADS / 64; w++) add += s_wcnt[w][tid];
-    s_base[tid] += add;
+    s_run[tid] += add;
     __syncthreads();
+  }
+  // the tile in digit order: element j of digit d goes to s_gbase[d] + (j - s_start[d])
+  for (int j = tid; j < tile_n; j += RS_THREADS) {
+    const unsigned long long key = s_key[j];
+    const unsigned d = (unsigned)(key >> shift) & 255u;
+    const unsigned pos = s_gbase[d] + ((unsigned)j - s_start[d]);
+    kout[pos] = key;
+    vout[pos] = s_val[j];
   }
 }
 
