@@ -49,10 +49,10 @@
 // on the head workload) or 5 (spills grow), and than the compiler's own 2.
 // Woodcock steps a lane may take in one STEP phase while its steps stay virtual, and GCOa
 // rejection trials (tau loop, p_z loop) in one CTAU / CPZ phase: the phase selection's ballots
-// are paid once for several. 2 and 2 measured best on the head workload (1, 2, 3, 4, 8 steps;
-// 1, 2, 3 trials); more inflate the register pressure of every phase.
+// are paid once for several. 3 and 2 measured best on the head workload (1-4 steps, 1-3
+// trials); more inflate the register pressure of every phase.
 #ifndef MCPET_STEPS_PER_PHASE
-#define MCPET_STEPS_PER_PHASE 2
+#define MCPET_STEPS_PER_PHASE 3
 #endif
 #ifndef MCPET_TRIALS_PER_PHASE
 #define MCPET_TRIALS_PER_PHASE 2
@@ -786,6 +786,25 @@ __device__ __forceinline__ void transport_photon(const KParams& P, Photon& ph, S
 }
 
 // Pair emission at a uniform point of the stream's voxel (K:665-773).
+// q = n / d for 0 <= n < 2^31 and 1 <= d < 2^20 with inv = 1/d in double: the product's
+// error is below 2^-21 < 1/d, so its truncation is q, or q - 1 when n / d is an integer and the
+// product rounds below it; one comparison fixes that. Replaces a 32-bit integer division (tens
+// of instructions). Grid dimensions are far below 2^20.
+__device__ __forceinline__ int div_by_inv(int n, int d, double inv)
+{
+  int q = (int)((double)n * inv);
+  if (n - q * d >= d) q++;
+  return q;
+}
+// (x, y, z) of voxel index v (x fastest), the reference's v % nx, (v / nx) % ny, v / (nx ny).
+__device__ __forceinline__ void voxel_coords(const KParams& P, int v, int& ix, int& iy, int& iz)
+{
+  const int q = div_by_inv(v, P.nx, P.inv_nx);
+  iz = div_by_inv(q, P.ny, P.inv_ny);
+  iy = q - iz * P.ny;
+  ix = v - q * P.nx;
+}
+
 template <class CN>
 __device__ __forceinline__ void source_first(const KParams& P, Stream& st, CN& C, float* pos, float* dir)
 {
@@ -802,7 +821,8 @@ __device__ __forceinline__ void source_first(const KParams& P, Stream& st, CN& C
   r1 = (K_EPS > r1 - K_EPS) ? K_EPS : r1 - K_EPS;
   r2 = (K_EPS > r2 - K_EPS) ? K_EPS : r2 - K_EPS;
   r3 = (K_EPS > r3 - K_EPS) ? K_EPS : r3 - K_EPS;
-  const int ix = st.absvox % P.nx, iy = (st.absvox / P.nx) % P.ny, iz = st.absvox / (P.nx * P.ny);
+  int ix, iy, iz;
+  voxel_coords(P, st.absvox, ix, iy, iz);
   pos[0] = ((float)ix + r1) / P.ivx;
   pos[1] = ((float)iy + r2) / P.ivy;
   pos[2] = ((float)iz + r3) / P.ivz;

@@ -41,6 +41,7 @@ struct KParams {
   const float2* palette;
   int vox_fmt, n_palette;
   int nx, ny, nz;
+  double inv_nx, inv_ny;       // 1 / nx, 1 / ny: voxel coordinates by multiplication (src_coords)
   float ivx, ivy, ivz;         // inverse voxel size
   float bx, by, bz;            // bounding box size
   // decay streams: stream s -> emitting voxel active_vox[s / tpb], lane s % tpb; with a

@@ -583,6 +583,8 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
   P.bg_mode = cfg->bg_mode;
   P.acq_ps = cfg->acquisition_time_ps;
   P.inv_mean_life = 1.0 / (double)cfg->mean_life;
+  P.inv_nx = 1.0 / (double)ph->nx;
+  P.inv_ny = 1.0 / (double)ph->ny;
   mcr_build_jump_table(MCR_A1, MCR_M1, P.j1);
   mcr_build_jump_table(MCR_A2, MCR_M2, P.j2);
   P.woodcock = c->d_woodcock;
@@ -883,7 +885,6 @@ extern "C" mcpet_status mcpet_autotune(mcpet_ctx* c, const mcpet_tune_options* o
   }
   // 3. decays per stream (the reference's histories per thread): opt-in, it changes the
   //    random-number streams
-  const long long own = c->cfg.stream_decays;
   if (opt->tune_streams && opt->n_stream_decays > 0) {
     for (int i = 0; i < opt->n_stream_decays; i++) {
       if ((st = apply_stream_layout(c, opt->stream_decays[i])) != MCPET_OK) return st;
@@ -892,9 +893,8 @@ extern "C" mcpet_status mcpet_autotune(mcpet_ctx* c, const mcpet_tune_options* o
       keep(r);
       if (r.decays_per_ms > best.decays_per_ms) best = r;
     }
+    if ((st = apply_stream_layout(c, best.stream_decays)) != MCPET_OK) return st;   // the chosen layout
   }
-  if ((st = apply_stream_layout(c, opt->tune_streams && opt->n_stream_decays > 0 ? best.stream_decays : own)) != MCPET_OK)
-    return st;
   c->tuned_bs = best.block_size;
   c->tuned_per_cu = best.blocks_per_cu;
   c->tuned_policy = best.phase_policy;
