@@ -319,6 +319,114 @@ __global__ __launch_bounds__(256) void k_conv(ConvK a)
   }
 }
 
+// Forward and input-gradient products of a stride-1 3x3x3 convolution whose B tensor (the
+// input, or the output gradient) has a multiple of 32 (padded) channels, on a tile of 4 rows x
+// 16 voxels of one z-plane per workgroup (wave w: row y0 + w, the 16 voxels as its MFMA
+// columns). The B tile with its one-voxel halo (3 x 6 x 18 voxels) is staged in LDS 32 channels
+// at a time, zero outside the volume, so a k-step is one LDS read at a compile-time offset per
+// tap (no tap decode, no bounds test, no global gather) and MT MFMAs; the weights are read as in
+// k_conv and reused across the tile's 64 voxels. FWD reads voxel p + d_t for tap t, BDATA
+// (stride 1: q = p - d_t) voxel p - d_t; the epilogues are k_conv's.
+#define FT_Y 4
+#define FT_X 16
+#define FT_HY (FT_Y + 2)
+#define FT_HX (FT_X + 2)
+#define FT_HALO (3 * FT_HY * FT_HX)
+
+template <int MODE, int MT>
+__global__ __launch_bounds__(256) void k_conv_tiled(ConvK a)
+{
+  __shared__ bf16x8 s_halo[FT_HALO * 4];              // [halo voxel][channel group of 8], 20.7 KB
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = lane & 15, kg = lane >> 4;
+  const int tiles_x = (a.W + FT_X - 1) / FT_X, tiles_y = (a.H + FT_Y - 1) / FT_Y;
+  int b = blockIdx.x;
+  const int x0 = (b % tiles_x) * FT_X;
+  b /= tiles_x;
+  const int y0 = (b % tiles_y) * FT_Y;
+  const int z = b / tiles_y;
+  const int m0 = blockIdx.y * (16 * MT);
+  floatx4 acc[MT];
+#pragma unroll
+  for (int i = 0; i < MT; i++) acc[i] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const __bf16* arow[MT];
+#pragma unroll
+  for (int i = 0; i < MT; i++) arow[i] = a.A + (size_t)(m0 + 16 * i + r) * a.Kp + kg * 8;
+  bf16x8 zero;
+#pragma unroll
+  for (int e = 0; e < 8; e++) zero[e] = to_bf16(0.0f);
+  for (int c0 = 0; c0 < a.Cbp; c0 += 32) {
+    for (int i = threadIdx.x; i < FT_HALO * 4; i += 256) {
+      const int hv = i >> 2, q = i & 3;
+      const int hx = hv % FT_HX, hy = (hv / FT_HX) % FT_HY, hz = hv / (FT_HX * FT_HY);
+      const int gz = z + hz - 1, gy = y0 + hy - 1, gx = x0 + hx - 1;
+      bf16x8 v = zero;
+      if (gz >= 0 && gz < a.D && gy >= 0 && gy < a.H && gx >= 0 && gx < a.W)
+        v = *(const bf16x8*)(a.Xl + (((size_t)gz * a.H + gy) * a.W + gx) * a.Cbp + c0 + q * 8);
+      s_halo[i] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 27; t++) {
+      const int dz = MODE == FWD ? t / 9 : 2 - t / 9;            // halo coordinates of p + d_t or p - d_t
+      const int dy = MODE == FWD ? (t / 3) % 3 : 2 - (t / 3) % 3;
+      const int dx = MODE == FWD ? t % 3 : 2 - t % 3;
+      const bf16x8 bv = s_halo[((dz * FT_HY + wave + dy) * FT_HX + r + dx) * 4 + kg];
+      const int k = t * a.Cbp + c0;
+#pragma unroll
+      for (int i = 0; i < MT; i++) {
+        const bf16x8 av = *(const bf16x8*)(arow[i] + k);
+        acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[i], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  // epilogue (as k_conv's): this lane's column is voxel (z, y0 + wave, x0 + r)
+  const int x = x0 + r, y = y0 + wave;
+  if (x >= a.W || y >= a.H) return;
+  const int n = (z * a.H + y) * a.W + x;
+#pragma unroll
+  for (int i = 0; i < MT; i++) {
+    const int row0 = m0 + 16 * i + kg * 4;
+    float ov[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int row = row0 + v;
+      ov[v] = 0.0f;
+      if (row >= a.M) continue;
+      const size_t idx = (size_t)row * a.N + n;
+      float o;
+      if (MODE == FWD) {
+        o = acc[i][v] + a.bias[row];
+        if (a.relu) o = fmaxf(o, 0.0f);
+        if (a.C) a.C[idx] = o;
+      } else {
+        o = acc[i][v] + (a.beta ? a.C[idx] : 0.0f);
+        if (a.mask && !(a.mask[idx] > 0.0f)) o = 0.0f;
+        a.C[idx] = o;
+      }
+      ov[v] = o;
+    }
+    const ClOut* outs[2] = {&a.o1, &a.o2};
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const ClOut& o = *outs[j];
+      if (!o.p) continue;
+      __bf16* dst = o.p + (size_t)n * o.ld + o.off + row0;
+      if (row0 + 3 < a.M && ((o.ld | o.off) & 3) == 0) {
+        bf16x4 q;
+#pragma unroll
+        for (int v = 0; v < 4; v++) q[v] = to_bf16(ov[v]);
+        *(bf16x4*)dst = q;
+      } else {
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+          if (row0 + v < a.M) dst[v] = to_bf16(ov[v]);
+      }
+    }
+  }
+}
+
 inline int mt_for(int M)
 {
   const int mt = (M + 15) / 16;
@@ -355,6 +463,37 @@ hipError_t launch_conv(const ConvK& a, hipStream_t st)
     case 2: hipLaunchKernelGGL((k_conv<MODE, 2>), grid, dim3(256), 0, st, b); break;
     case 3: hipLaunchKernelGGL((k_conv<MODE, 3>), grid, dim3(256), 0, st, b); break;
     default: hipLaunchKernelGGL((k_conv<MODE, 4>), grid, dim3(256), 0, st, b); break;
+  }
+  return hipGetLastError();
+}
+
+// Stride-1 3x3x3 FWD / BDATA products with Cbp % 32 == 0 (k_conv_tiled); the row blocks as
+// k_conv's, so the A rows read stay inside the padded weight copy.
+inline long long tiled_blocks(const ConvK& a)
+{
+  return (long long)((a.W + FT_X - 1) / FT_X) * ((a.H + FT_Y - 1) / FT_Y) * a.D * ((a.M + 16 * mt_for(a.M) - 1) / (16 * mt_for(a.M)));
+}
+// The tiled products on volumes whose tiles fill the chip (at least four workgroups per CU);
+// smaller volumes (the coarser U-Net levels) keep k_conv's 16-voxel tiles and in-workgroup k split.
+inline bool use_tiled(const ConvK& a)
+{
+  return a.ks == 3 && a.stride == 1 && a.Cbp % 32 == 0 && tiled_blocks(a) >= 1024;
+}
+
+template <int MODE>
+hipError_t launch_conv_tiled(const ConvK& a, hipStream_t st)
+{
+  const int mt = mt_for(a.M);
+  const long long tiles = (long long)((a.W + FT_X - 1) / FT_X) * ((a.H + FT_Y - 1) / FT_Y) * a.D;
+  if (a.ks != 3 || a.stride != 1 || a.D != a.oD || a.H != a.oH || a.W != a.oW || a.Cbp % 32 != 0 || a.Kp < 27 * a.Cbp ||
+      tiles > 0x7fffffffLL)
+    return hipErrorInvalidValue;
+  dim3 grid((unsigned)tiles, (a.M + 16 * mt - 1) / (16 * mt), 1);
+  switch (mt) {
+    case 1: hipLaunchKernelGGL((k_conv_tiled<MODE, 1>), grid, dim3(256), 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_conv_tiled<MODE, 2>), grid, dim3(256), 0, st, a); break;
+    case 3: hipLaunchKernelGGL((k_conv_tiled<MODE, 3>), grid, dim3(256), 0, st, a); break;
+    default: hipLaunchKernelGGL((k_conv_tiled<MODE, 4>), grid, dim3(256), 0, st, a); break;
   }
   return hipGetLastError();
 }
@@ -674,6 +813,7 @@ hipError_t conv_fwd(mcpet_surrogate* s, const Conv& l, const Vol& vin, const __b
   a.K = a.T * a.Cbp;
   a.Kp = l.kp_f;
   s->flops_fwd += 2.0 * l.cout * (double)l.cin * a.T * a.N;
+  if (use_tiled(a)) return launch_conv_tiled<FWD>(a, s->st);
   return launch_conv<FWD>(a, s->st);
 }
 // gx (+)= dL/dx of the layer input, zeroed where mask <= 0; gy read from its channels-last copy
@@ -694,6 +834,7 @@ hipError_t conv_bwd_data(mcpet_surrogate* s, const Conv& l, const Vol& vin, cons
   a.Cbp = rup(l.cout, 8);
   a.K = a.T * a.Cbp;
   a.Kp = l.kp_b;
+  if (use_tiled(a)) return launch_conv_tiled<BDATA>(a, s->st);
   return launch_conv<BDATA>(a, s->st);
 }
 // weight and bias gradients accumulated from gy (dL/d pre-activation) and the layer input x
