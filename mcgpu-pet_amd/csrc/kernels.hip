@@ -957,6 +957,11 @@ __device__ __forceinline__ void stream_init(const KParams& P, long long s, Strea
   long long tid, per_voxel;
   if (P.stream_start) {   // balanced layout: the voxel owning stream s (binary search)
     int lo = 0, hi = P.n_active - 1;
+    if (P.stream_hint) {  // narrowed to the owners of the first streams of s's block and the next
+      const long long b = s >> P.hint_shift;
+      lo = gld(P.stream_hint, b);
+      hi = gld(P.stream_hint, b + 1);
+    }
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
       if (gld(P.stream_start, mid) <= s) lo = mid;

@@ -51,6 +51,11 @@ struct KParams {
   const float* active_act;
   const long long* stream_start;
   const long long* stream_id;
+  // balanced layout: stream_hint[b] = the voxel owning stream b << hint_shift (b = 0 .. last
+  // block; one more entry = n_active - 1), so the owner of stream s lies between
+  // stream_hint[s >> hint_shift] and the next entry (null: search all voxels)
+  const int* stream_hint;
+  int hint_shift;
   int n_active;
   long long n_streams;           // streams [stream_base, n_streams) are simulated by a launch
   long long stream_base;

@@ -44,6 +44,8 @@ struct mcpet_ctx {
   float* d_active_act = nullptr;
   long long* d_stream_start = nullptr;   // balanced stream layout (stream_decays > 0)
   long long* d_stream_id = nullptr;
+  int* d_stream_hint = nullptr;
+  size_t hint_cap = 0;                   // entries allocated for d_stream_hint
   float2* d_woodcock = nullptr;
   float2* d_tot = nullptr;
   float4* d_cr = nullptr;
@@ -140,7 +142,7 @@ mcpet_status dalloc(mcpet_ctx* c, T** p, size_t n)
 
 void free_all(mcpet_ctx* c)
 {
-  void* ptrs[] = {c->d_vox, c->d_vox8, c->d_vox16, c->d_palette, c->d_active_vox, c->d_active_act, c->d_stream_start, c->d_stream_id, c->d_woodcock, c->d_tot, c->d_cr, c->d_ray, c->d_cmp,
+  void* ptrs[] = {c->d_vox, c->d_vox8, c->d_vox16, c->d_palette, c->d_active_vox, c->d_active_act, c->d_stream_start, c->d_stream_id, c->d_stream_hint, c->d_woodcock, c->d_tot, c->d_cr, c->d_ray, c->d_cmp,
                   c->d_trues, c->d_scatter, c->d_img_t, c->d_img_sc, c->d_spectrum, c->d_psf, c->d_psf_index,
                   c->d_voxel_edep, c->d_mat_edep, c->d_counters, c->d_final_seed, c->d_singles, c->d_singles_n,
                   c->d_keys[0], c->d_keys[1], c->d_vals[0], c->d_vals[1], c->d_hist, c->d_randoms, c->d_spurious,
@@ -271,6 +273,27 @@ static long long balanced_layout(const std::vector<float>& act_dev, const std::v
   }
   stream_start[n] = s0;
   return id;
+}
+
+// Owner hints of a balanced layout (start: n + 1 stream offsets, strictly increasing): with
+// g the smallest shift leaving at most n blocks of 2^g streams, hint[b] = the voxel owning
+// stream b 2^g and hint[last + 1] = n - 1. The device's owner search then starts from the
+// voxels between two consecutive hints instead of all n.
+static int stream_hints(const std::vector<long long>& start, std::vector<int>& hint)
+{
+  const long long n = (long long)start.size() - 1, total = start.back();
+  int g = 0;
+  while ((total >> g) > n) g++;
+  const long long nb = (total - 1) >> g;
+  hint.resize((size_t)nb + 2);
+  long long k = 0;
+  for (long long b = 0; b <= nb; b++) {
+    const long long s = b << g;
+    while (start[(size_t)k + 1] <= s) k++;
+    hint[(size_t)b] = (int)k;
+  }
+  hint[(size_t)nb + 1] = (int)(n - 1);
+  return g;
 }
 
 static double effective_time(const mcpet_config* cfg)
@@ -482,9 +505,14 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
   if (c->P.vox_fmt != 0) CK(dalloc(c, &c->d_palette, palette.size()));
   CK(dalloc(c, &c->d_active_vox, std::max<size_t>(1, act_idx.size())));
   CK(dalloc(c, &c->d_active_act, std::max<size_t>(1, act_idx.size())));
+  std::vector<int> stream_hint;
+  int hint_shift = 0;
   if (!stream_start.empty()) {
+    hint_shift = stream_hints(stream_start, stream_hint);
     CK(dalloc(c, &c->d_stream_start, stream_start.size()));
     CK(dalloc(c, &c->d_stream_id, stream_id.size()));
+    CK(dalloc(c, &c->d_stream_hint, stream_hint.size()));
+    c->hint_cap = stream_hint.size();
   }
   CK(dalloc(c, &c->d_woodcock, (size_t)N));
   CK(dalloc(c, &c->d_tot, htot.size()));
@@ -540,6 +568,7 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
   if (!stream_start.empty()) {
     HK(hipMemcpy(c->d_stream_start, stream_start.data(), stream_start.size() * sizeof(long long), hipMemcpyHostToDevice));
     HK(hipMemcpy(c->d_stream_id, stream_id.data(), stream_id.size() * sizeof(long long), hipMemcpyHostToDevice));
+    HK(hipMemcpy(c->d_stream_hint, stream_hint.data(), stream_hint.size() * sizeof(int), hipMemcpyHostToDevice));
   }
   HK(hipMemcpy(c->d_woodcock, hw.data(), hw.size() * sizeof(float2), hipMemcpyHostToDevice));
   HK(hipMemcpy(c->d_tot, htot.data(), htot.size() * sizeof(float2), hipMemcpyHostToDevice));
@@ -576,6 +605,8 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
   P.tpb = cfg->threads_per_block;
   P.stream_start = c->d_stream_start;
   P.stream_id = c->d_stream_id;
+  P.stream_hint = c->d_stream_hint;
+  P.hint_shift = hint_shift;
   P.n_active = (int)std::min<long long>(c->n_active, 0x7fffffffLL);
   P.n_streams = c->d_stream_start ? balanced_streams : c->n_active * (long long)cfg->threads_per_block;
   P.queue = c->d_counters + KC_N;
@@ -780,6 +811,7 @@ static mcpet_status apply_stream_layout(mcpet_ctx* c, long long stream_decays)
   if (stream_decays <= 0 || n <= 0) {
     P.stream_start = nullptr;
     P.stream_id = nullptr;
+    P.stream_hint = nullptr;
     P.n_streams = n * tpb;
     P.last_thread_id = (c->nvox - 1) * tpb + (tpb - 1);
     c->cfg.stream_decays = 0;
@@ -797,16 +829,32 @@ static mcpet_status apply_stream_layout(mcpet_ctx* c, long long stream_decays)
   const long long total = balanced_layout(act, src_of, effective_time(&c->cfg), stream_decays, start, id);
   if (c->singles_mode && total >= (1LL << 32))
     return set_err(c, MCPET_E_RANGE, "mcpet_autotune: more than 2^32 decay streams (singles event ids)");
+  std::vector<int> hint;
+  const int shift = stream_hints(start, hint);
   if (!c->d_stream_start) {
     mcpet_status st = dalloc(c, &c->d_stream_start, start.size());
     if (st == MCPET_OK) st = dalloc(c, &c->d_stream_id, id.size());
     if (st != MCPET_OK) return st;
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (hint.size() > c->hint_cap) {   // at most n + 2 entries: allocated once at that size
+    if (c->d_stream_hint) {
+      (void)hipFree(c->d_stream_hint);
+      c->dev_bytes -= c->hint_cap * sizeof(int);
+      c->d_stream_hint = nullptr;
+      c->hint_cap = 0;
+    }
+    mcpet_status st = dalloc(c, &c->d_stream_hint, (size_t)n + 2);
+    if (st != MCPET_OK) return st;
+    c->hint_cap = (size_t)n + 2;
+  }
   HIPCHK(c, hipMemcpy(c->d_stream_start, start.data(), sizeof(long long) * start.size(), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->d_stream_id, id.data(), sizeof(long long) * id.size(), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->d_stream_hint, hint.data(), sizeof(int) * hint.size(), hipMemcpyHostToDevice));
   P.stream_start = c->d_stream_start;
   P.stream_id = c->d_stream_id;
+  P.stream_hint = c->d_stream_hint;
+  P.hint_shift = shift;
   P.n_streams = total;
   P.last_thread_id = total - 1;
   c->cfg.stream_decays = (int32_t)stream_decays;

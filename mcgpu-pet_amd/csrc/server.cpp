@@ -62,6 +62,7 @@ size_t footprint(const mcpet_config& cfg, const mcpet_phantom& ph, const mcpet_t
   b += 2 * 4 * nvox;                                    // emission images
   b += 8 * nvox + 512 * 1024;                           // voxel words (or indices + palette)
   b += 8 * nvox;                                        // emitting voxels and activities
+  b += 20 * nvox + 32;                                  // balanced stream layout: offsets, ids, owner hints
   b += N * 8 + N * 15 * 8 + N * 15 * 16;                // Woodcock, total, Compton/Rayleigh tables
   b += sizeof(mcpet_rayleigh_table) + sizeof(mcpet_compton_table);
   b += 4 * (size_t)std::max(1, cfg.ne);
