@@ -498,6 +498,257 @@ hipError_t launch_conv_tiled(const ConvK& a, hipStream_t st)
   return hipGetLastError();
 }
 
+// Weight gradient of a 3x3x3 convolution of stride S (1 or 2) from channels-last BF16 copies of
+// its input X and of its output gradient dY:
+//   G[co][ci][t] = sum_q dY[co][q] X[ci][S q + d_t]      (MFMA m = co, n = ci, k = q)
+// A workgroup owns one block of 32 output and 32 input channels and a slab of output-voxel tiles
+// (4 rows x 16 voxels of one z-plane, 64 voxels = two 32-voxel k-steps); six waves split the
+// accumulators by tap plane dz (3) and input-channel half (2), each holding the 3 x 3 (dy, dx)
+// taps x 2 output-channel halves = 18 MFMA tiles. Per tile the workgroup stages in LDS, both
+// transposed to voxel-contiguous rows: dY of the tile's 64 voxels (32 x 64) and X of the input
+// region the tile's taps read (32 channels x 3 planes x 6 rows x 18 voxels for S = 1; 3 planes x
+// 9 rows x 33 voxels for S = 2, each row split into its even and odd voxels), so that each 8-voxel
+// k-group of a fragment is one aligned 16-byte LDS read: S = 1 voxel x at index x - x0 + 8, the
+// dx = -1 / +1 taps that read shifted by one voxel with v_alignbit across the neighbouring words;
+// S = 2 the even voxels 2 (x0 + j) at E[j] (dx = 0), the odd voxels 2 (x0 + j) - 1 at O[j] (dx = -1,
+// and shifted by one, dx = +1). Each workgroup writes its slab's sums to part[slab][t][co][ci];
+// k_bwgt_reduce adds the slabs in a fixed order into the gradient (deterministic, no atomics).
+#define BW_Y 4
+#define BW_X 16
+#define BW_THREADS 384
+
+struct BwK {
+  const __bf16* Xl;   // input, channels-last [vox][ldx] (channels >= ldx: zero)
+  const __bf16* Gl;   // output gradient, channels-last [ovox][ldg]
+  float* part;        // [n_slabs][27][32 cob][32 cib]
+  int ldx, ldg, Cin, Cout, D, H, W, oD, oH, oW;
+  int tiles_x, tiles_y, n_tiles, per_slab, cob, cib;
+};
+
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ bf16x8 as_bf16x8(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+// words of v shifted by one BF16 element towards lower indices, the last one from `next`
+__device__ __forceinline__ u32x4 shift_down1(u32x4 v, unsigned next)
+{
+  u32x4 o;
+  o[0] = __builtin_amdgcn_alignbit(v[1], v[0], 16);
+  o[1] = __builtin_amdgcn_alignbit(v[2], v[1], 16);
+  o[2] = __builtin_amdgcn_alignbit(v[3], v[2], 16);
+  o[3] = __builtin_amdgcn_alignbit(next, v[3], 16);
+  return o;
+}
+
+template <int S>
+struct BwLds;
+template <>
+struct BwLds<1> {
+  __bf16 x[32][3][6][32];   // 36 KB: [channel][plane][row][x - x0 + 8]
+};
+template <>
+struct BwLds<2> {
+  __bf16 e[32][3][9][16];   // even input voxels 2 (x0 + j), j = 0..15
+  __bf16 o[32][3][9][24];   // odd input voxels 2 (x0 + j) - 1, j = 0..16
+};
+
+template <int S>
+__global__ __launch_bounds__(BW_THREADS) void k_bwgt_tiled(BwK a)
+{
+  constexpr int HY = S == 1 ? 6 : 9, HX = S == 1 ? 18 : 33;   // input region: 3 x HY x HX voxels
+  __shared__ __attribute__((aligned(16))) BwLds<S> L;
+  __shared__ __attribute__((aligned(16))) __bf16 gs[32][64];   // 4 KB
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int lr = lane & 15, g = lane >> 4;
+  const int dz = wave % 3, nj = wave / 3;   // tap plane and input-channel half of this wave
+  const int slab = blockIdx.x, co0 = blockIdx.y * 32, ci0 = blockIdx.z * 32;
+  floatx4 acc[3][3][2];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+      for (int m = 0; m < 2; m++) acc[i][j][m] = floatx4{0.f, 0.f, 0.f, 0.f};
+  bf16x8 zero;
+#pragma unroll
+  for (int e = 0; e < 8; e++) zero[e] = to_bf16(0.0f);
+  const int t_end = min(a.n_tiles, (slab + 1) * a.per_slab);
+  for (int tile = slab * a.per_slab; tile < t_end; tile++) {
+    int b = tile;
+    const int x0 = (b % a.tiles_x) * BW_X;
+    b /= a.tiles_x;
+    const int y0 = (b % a.tiles_y) * BW_Y;
+    const int z0 = b / a.tiles_y;
+    // input region: voxel (S z0 - 1 + hz, S y0 - 1 + hy, S x0 - 1 + hx), 4 groups of 8 channels
+    for (int i = threadIdx.x; i < 3 * HY * HX * 4; i += BW_THREADS) {
+      const int q = i & 3, hv = i >> 2;
+      const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+      const int gz = S * z0 + hz - 1, gy = S * y0 + hy - 1, gx = S * x0 + hx - 1;
+      const int c = ci0 + q * 8;
+      bf16x8 v = zero;
+      if (gz >= 0 && gz < a.D && gy >= 0 && gy < a.H && gx >= 0 && gx < a.W && c < a.ldx)
+        v = *(const bf16x8*)(a.Xl + (((size_t)gz * a.H + gy) * a.W + gx) * a.ldx + c);
+      if constexpr (S == 1) {
+#pragma unroll
+        for (int e = 0; e < 8; e++) L.x[q * 8 + e][hz][hy][hx + 7] = v[e];
+      } else if (hx & 1) {
+#pragma unroll
+        for (int e = 0; e < 8; e++) L.e[q * 8 + e][hz][hy][hx >> 1] = v[e];
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; e++) L.o[q * 8 + e][hz][hy][hx >> 1] = v[e];
+      }
+    }
+    // dY of the tile's 64 output voxels, transposed into gs[c][q] (q = row * 16 + x)
+    for (int i = threadIdx.x; i < 64 * 4; i += BW_THREADS) {
+      const int q4 = i & 3, qv = i >> 2;
+      const int gy = y0 + (qv >> 4), gx = x0 + (qv & 15);
+      const int c = co0 + q4 * 8;
+      bf16x8 v = zero;
+      if (gy < a.oH && gx < a.oW && c < a.ldg)
+        v = *(const bf16x8*)(a.Gl + (((size_t)z0 * a.oH + gy) * a.oW + gx) * a.ldg + c);
+#pragma unroll
+      for (int e = 0; e < 8; e++) gs[q4 * 8 + e][qv] = v[e];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < 2; ks++) {
+      const int r = 2 * ks + (g >> 1), xh = (g & 1) * 8;   // this lane's 8 voxels: row r, x = xh .. xh + 7
+      bf16x8 av[2];
+#pragma unroll
+      for (int m = 0; m < 2; m++) av[m] = *(const bf16x8*)&gs[16 * m + lr][r * 16 + xh];
+#pragma unroll
+      for (int dy = 0; dy < 3; dy++) {
+        bf16x8 bx[3];   // the dx = -1, 0, +1 fragments
+        if constexpr (S == 1) {
+          const __bf16* row = &L.x[16 * nj + lr][dz][r + dy][0];
+          const u32x4 bm = *(const u32x4*)(row + xh + 8);          // x = xh .. xh + 7
+          const unsigned lo = *(const unsigned*)(row + xh + 6);    // x = xh - 2, xh - 1
+          const unsigned hi = *(const unsigned*)(row + xh + 16);   // x = xh + 8, xh + 9
+          u32x4 bl;
+          bl[0] = __builtin_amdgcn_alignbit(bm[0], lo, 16);
+          bl[1] = __builtin_amdgcn_alignbit(bm[1], bm[0], 16);
+          bl[2] = __builtin_amdgcn_alignbit(bm[2], bm[1], 16);
+          bl[3] = __builtin_amdgcn_alignbit(bm[3], bm[2], 16);
+          bx[0] = as_bf16x8(bl);
+          bx[1] = as_bf16x8(bm);
+          bx[2] = as_bf16x8(shift_down1(bm, hi));
+        } else {
+          const int yy = 2 * r + dy;
+          const __bf16* orow = &L.o[16 * nj + lr][dz][yy][0];
+          const u32x4 ov = *(const u32x4*)(orow + xh);             // O[xh .. xh + 7]: x = 2 j - 1
+          const unsigned hi = *(const unsigned*)(orow + xh + 8);   // O[xh + 8], O[xh + 9]
+          bx[0] = as_bf16x8(ov);
+          bx[1] = *(const bf16x8*)&L.e[16 * nj + lr][dz][yy][xh];  // x = 2 j
+          bx[2] = as_bf16x8(shift_down1(ov, hi));                  // x = 2 j + 1
+        }
+#pragma unroll
+        for (int dx = 0; dx < 3; dx++)
+#pragma unroll
+          for (int m = 0; m < 2; m++)
+            acc[dy][dx][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bx[dx], acc[dy][dx][m], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  // this lane: columns ci = ci0 + 16 nj + lr, rows co = co0 + 16 m + 4 g + v of each tap's tile
+  const int CoutR = 32 * a.cob, CinR = 32 * a.cib;
+#pragma unroll
+  for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+    for (int dx = 0; dx < 3; dx++) {
+      const int t = dz * 9 + dy * 3 + dx;
+      float* p = a.part + ((size_t)slab * 27 + t) * CoutR * CinR + (ci0 + 16 * nj + lr);
+#pragma unroll
+      for (int m = 0; m < 2; m++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) p[(size_t)(co0 + 16 * m + 4 * g + v) * CinR] = acc[dy][dx][m][v];
+    }
+}
+
+// G[co][ci][t] += sum over slabs of part[slab][t][co][ci]. A workgroup takes 64 consecutive
+// outputs in the partial sums' (t, co, ci) order (each slab's reads contiguous across them) and
+// four slab phases: thread (j, ph) adds slabs ph, ph + 4, ... of output j, and the four phase
+// sums are added in a fixed order (deterministic).
+__global__ __launch_bounds__(256) void k_bwgt_reduce(const float* __restrict__ part, float* __restrict__ G, int Cout, int Cin,
+                                                     int CoutR, int CinR, int n_slabs)
+{
+  __shared__ float ps[4][64];
+  const long long n = (long long)Cout * Cin * 27;
+  const int j = threadIdx.x & 63, ph = threadIdx.x >> 6;
+  const long long i = (long long)blockIdx.x * 64 + j;
+  float s = 0.0f;
+  int ci = 0, co = 0, t = 0;
+  if (i < n) {
+    ci = (int)(i % Cin);
+    co = (int)((i / Cin) % Cout);
+    t = (int)(i / ((long long)Cin * Cout));
+    const size_t stride = (size_t)27 * CoutR * CinR;
+    const float* p = part + ((size_t)t * CoutR + co) * CinR + ci;
+#pragma unroll 8
+    for (int k = ph; k < n_slabs; k += 4) s += p[(size_t)k * stride];
+  }
+  ps[ph][j] = s;
+  __syncthreads();
+  if (ph == 0 && i < n) G[((size_t)co * Cin + ci) * 27 + t] += ((ps[0][j] + ps[1][j]) + ps[2][j]) + ps[3][j];
+}
+
+// Workgroups of the tiled weight gradient: about BW_WGS in total, the output-voxel tiles dealt
+// to slabs of equal length; the partial sums then fit in BW_WGS x 27 x 1024 floats.
+#define BW_WGS 512
+inline void bwgt_shape(int oD, int oH, int oW, int Cin, int Cout, BwK& b, int& n_slabs)
+{
+  b.tiles_x = (oW + BW_X - 1) / BW_X;
+  b.tiles_y = (oH + BW_Y - 1) / BW_Y;
+  b.n_tiles = b.tiles_x * b.tiles_y * oD;
+  b.cob = (Cout + 31) / 32;
+  b.cib = (Cin + 31) / 32;
+  n_slabs = std::max(1, std::min(b.n_tiles, BW_WGS / (b.cob * b.cib)));
+  b.per_slab = (b.n_tiles + n_slabs - 1) / n_slabs;
+  n_slabs = (b.n_tiles + b.per_slab - 1) / b.per_slab;
+}
+constexpr size_t kBwPartFloats = (size_t)BW_WGS * 27 * 1024;
+
+// true when the tiled weight gradient takes this layer (its slab sums fit the buffer)
+inline bool bwgt_fits(int oD, int oH, int oW, int Cin, int Cout)
+{
+  BwK b{};
+  int n_slabs = 0;
+  bwgt_shape(oD, oH, oW, Cin, Cout, b, n_slabs);
+  return (size_t)n_slabs * 27 * (32 * b.cob) * (32 * b.cib) <= kBwPartFloats;
+}
+
+// stride 1 (input and output volumes equal) or 2 (output (D/2, H/2, W/2) rounded down or up)
+hipError_t launch_bwgt_tiled(int stride, const __bf16* xl, int ldx, const __bf16* gyl, int ldg, int Cin, int Cout, int D, int H,
+                             int W, int oD, int oH, int oW, float* part, float* G, hipStream_t st)
+{
+  BwK b{};
+  int n_slabs = 0;
+  bwgt_shape(oD, oH, oW, Cin, Cout, b, n_slabs);
+  if ((size_t)n_slabs * 27 * (32 * b.cob) * (32 * b.cib) > kBwPartFloats || ldx % 8 != 0 || ldg % 8 != 0 ||
+      (stride != 1 && stride != 2))
+    return hipErrorInvalidValue;
+  b.Xl = xl;
+  b.Gl = gyl;
+  b.part = part;
+  b.ldx = ldx;
+  b.ldg = ldg;
+  b.Cin = Cin;
+  b.Cout = Cout;
+  b.D = D;
+  b.H = H;
+  b.W = W;
+  b.oD = oD;
+  b.oH = oH;
+  b.oW = oW;
+  if (stride == 1) hipLaunchKernelGGL(k_bwgt_tiled<1>, dim3(n_slabs, b.cob, b.cib), dim3(BW_THREADS), 0, st, b);
+  else hipLaunchKernelGGL(k_bwgt_tiled<2>, dim3(n_slabs, b.cob, b.cib), dim3(BW_THREADS), 0, st, b);
+  const long long n = (long long)Cout * Cin * 27;
+  hipLaunchKernelGGL(k_bwgt_reduce, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, st, (const float*)part, G, Cout, Cin,
+                     32 * b.cob, 32 * b.cib, n_slabs);
+  return hipGetLastError();
+}
+
 // BF16 weight copies: out[m][t * Cp + c] = W[co][ci][t] with (co, ci) = (m, c) (forward) or
 // (c, m) (transposed, for the input gradient); zero outside the real channels.
 __global__ void k_derive(const float* __restrict__ w, __bf16* __restrict__ out, int rows, int Kp, int Cout, int Cin, int T, int Cp,
@@ -724,6 +975,7 @@ struct mcpet_surrogate {
          *gb1l = nullptr;
   float *raw_s = nullptr, *raw_t = nullptr, *est = nullptr;   // the fused estimate's buffers
   double* sums = nullptr;                                       // 256 partial totals + the total
+  float* bw_part = nullptr;                                     // slab sums of the tiled weight gradient
   long long t_adam = 0;
   int device = 0;
   hipStream_t st = nullptr;
@@ -837,9 +1089,21 @@ hipError_t conv_bwd_data(mcpet_surrogate* s, const Conv& l, const Vol& vin, cons
   if (use_tiled(a)) return launch_conv_tiled<BDATA>(a, s->st);
   return launch_conv<BDATA>(a, s->st);
 }
-// weight and bias gradients accumulated from gy (dL/d pre-activation) and the layer input x
-hipError_t conv_bwd_weights(mcpet_surrogate* s, const Conv& l, const Vol& vin, const float* x, const Vol& vout, const float* gy)
+// weight and bias gradients accumulated from gy (dL/d pre-activation) and the layer input x;
+// xl, gyl: their channels-last BF16 copies (ClOut ld = row length) or null. 3x3x3 layers with
+// both copies take the LDS-tiled product, the others k_conv's.
+hipError_t conv_bwd_weights(mcpet_surrogate* s, const Conv& l, const Vol& vin, const float* x, const Vol& vout, const float* gy,
+                            ClOut xl = ClOut{nullptr, 0, 0}, ClOut gyl = ClOut{nullptr, 0, 0})
 {
+  const int vox = (int)vout.vox(), per = 4096;
+  if (l.ks == 3 && (l.stride == 1 || l.stride == 2) && xl.p && gyl.p && xl.off == 0 && gyl.off == 0 && xl.ld % 8 == 0 &&
+      gyl.ld % 8 == 0 && bwgt_fits(vout.D, vout.H, vout.W, l.cin, l.cout)) {
+    hipError_t e = launch_bwgt_tiled(l.stride, xl.p, xl.ld, gyl.p, gyl.ld, l.cin, l.cout, vin.D, vin.H, vin.W, vout.D, vout.H,
+                                     vout.W, s->bw_part, s->g + l.w, s->st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_channel_sum, dim3(l.cout, (vox + per - 1) / per), dim3(256), 0, s->st, gy, s->g + l.b, vox, per);
+    return hipGetLastError();
+  }
   ConvK a = conv_args(l, vin, vout);
   a.Ad = gy;
   a.X = x;
@@ -851,7 +1115,6 @@ hipError_t conv_bwd_weights(mcpet_surrogate* s, const Conv& l, const Vol& vin, c
   a.Cbp = l.cin;
   hipError_t e = launch_conv<BWGT>(a, s->st);
   if (e != hipSuccess) return e;
-  const int vox = (int)vout.vox(), per = 4096;
   hipLaunchKernelGGL(k_channel_sum, dim3(l.cout, (vox + per - 1) / per), dim3(256), 0, s->st, gy, s->g + l.b, vox, per);
   return hipGetLastError();
 }
@@ -919,21 +1182,21 @@ hipError_t backward(mcpet_surrogate* s)
   if (e == hipSuccess) e = (x)
   STEP(conv_bwd_weights(s, s->L[8], s->ve1, s->e1, s->vr, s->gr));
   STEP(conv_bwd_data(s, s->L[8], s->ve1, s->vr, s->grl, s->ge1, s->e1, false, cl(s->ge1l, c)));
-  STEP(conv_bwd_weights(s, s->L[7], s->vcat1, s->cat1, s->ve1, s->ge1));
+  STEP(conv_bwd_weights(s, s->L[7], s->vcat1, s->cat1, s->ve1, s->ge1, cl(s->cat1l, 3 * c), cl(s->ge1l, c)));
   STEP(conv_bwd_data(s, s->L[7], s->vcat1, s->ve1, s->ge1l, s->gcat1, s->cat1, false, kNoCl));   // g(up1), g(b1 skip)
   STEP(upsample_bwd(s, s->ve2, s->gcat1, s->ge2, cl(s->ge2l, 2 * c)));
-  STEP(conv_bwd_weights(s, s->L[6], s->vcat2, s->cat2, s->ve2, s->ge2));
+  STEP(conv_bwd_weights(s, s->L[6], s->vcat2, s->cat2, s->ve2, s->ge2, cl(s->cat2l, 6 * c), cl(s->ge2l, 2 * c)));
   STEP(conv_bwd_data(s, s->L[6], s->vcat2, s->ve2, s->ge2l, s->gcat2, s->cat2, false, kNoCl));   // g(up2), g(b2 skip)
   STEP(upsample_bwd(s, s->vb3, s->gcat2, s->gb3, cl(s->gb3l, 4 * c)));
-  STEP(conv_bwd_weights(s, s->L[5], s->vd2, s->d2, s->vb3, s->gb3));
+  STEP(conv_bwd_weights(s, s->L[5], s->vd2, s->d2, s->vb3, s->gb3, cl(s->d2l, 4 * c), cl(s->gb3l, 4 * c)));
   STEP(conv_bwd_data(s, s->L[5], s->vd2, s->vb3, s->gb3l, s->gd2, s->d2, false, cl(s->gd2l, 4 * c)));
-  STEP(conv_bwd_weights(s, s->L[4], vb2, b2, s->vd2, s->gd2));
+  STEP(conv_bwd_weights(s, s->L[4], vb2, b2, s->vd2, s->gd2, cl(s->b2l, 2 * c), cl(s->gd2l, 4 * c)));
   STEP(conv_bwd_data(s, s->L[4], vb2, s->vd2, s->gd2l, gb2, b2, true, cl(s->gb2l, 2 * c)));   // + the skip's gradient
-  STEP(conv_bwd_weights(s, s->L[3], s->vd1, s->d1, vb2, gb2));
+  STEP(conv_bwd_weights(s, s->L[3], s->vd1, s->d1, vb2, gb2, cl(s->d1l, 2 * c), cl(s->gb2l, 2 * c)));
   STEP(conv_bwd_data(s, s->L[3], s->vd1, vb2, s->gb2l, s->gd1, s->d1, false, cl(s->gd1l, 2 * c)));
-  STEP(conv_bwd_weights(s, s->L[2], vb1, b1, s->vd1, s->gd1));
+  STEP(conv_bwd_weights(s, s->L[2], vb1, b1, s->vd1, s->gd1, cl(s->b1l, c), cl(s->gd1l, 2 * c)));
   STEP(conv_bwd_data(s, s->L[2], vb1, s->vd1, s->gd1l, gb1, b1, true, cl(s->gb1l, c)));       // + the skip's gradient
-  STEP(conv_bwd_weights(s, s->L[1], s->va1, s->a1, vb1, gb1));
+  STEP(conv_bwd_weights(s, s->L[1], s->va1, s->a1, vb1, gb1, cl(s->a1l, c), cl(s->gb1l, c)));
   STEP(conv_bwd_data(s, s->L[1], s->va1, vb1, s->gb1l, s->ga1, s->a1, false, kNoCl));
   STEP(conv_bwd_weights(s, s->L[0], s->vx, s->x0, s->va1, s->ga1));
 #undef STEP
@@ -1013,6 +1276,7 @@ extern "C" mcpet_status mcpet_surrogate_create(int32_t D, int32_t H, int32_t W, 
     if (e == hipSuccess) e = dalloc_t(s, a.p, a.n);
   if (e == hipSuccess) e = dalloc_t(s, &s->wbf, s->n_bf16);
   if (e == hipSuccess) e = dalloc_t(s, &s->sums, 257);
+  if (e == hipSuccess) e = dalloc_t(s, &s->bw_part, kBwPartFloats);
   {
     const size_t v0 = s->vx.vox(), v1 = s->vd1.vox(), v2 = s->vd2.vox();
     struct B {

@@ -192,7 +192,9 @@ def test_device_unet_forward_and_gradients_match_bf16_reference(grid, cin, c, ba
     relative per layer). Widths 2-4 use one 16-row tile per product; widths 16, 32 and 56 with
     four input channels (the shipped network's layer shapes and a ~4 M-parameter one) take the
     2-, 3- and 4-tile convolution kernels, several row blocks per product and the epilogue split
-    over the workgroup's waves, in the forward, input-gradient and weight-gradient modes."""
+    over the workgroup's waves, in the forward, input-gradient and weight-gradient modes; the
+    stride-1 layers' weight gradients take the LDS-tiled product (32-channel blocks, partial
+    blocks at widths 16 and 56, voxel tiles cut by the grid's edges)."""
     rng = np.random.default_rng(sum(grid) + cin + c)
     with mcpet.Surrogate(grid, in_channels=cin, width=c, seed=7) as s:
         assert s.n_params == n_params(cin, c)
