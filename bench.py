@@ -30,18 +30,21 @@ MATERIALS = os.path.join(ROOT, "tests", "golden", "sample_simulation", "material
 # Algorithmic work units (VALU lane-ops, HBM bytes) per event, counted from the reference
 # source: RANECU draw 25 ops; Woodcock step 83 ops incl. its 2 draws, 4 B packed voxel word;
 # material/energy change 24 B of (a,b); Compton 40 + 35 per shell evaluation; detected
-# coincidence 32 B of atomic read-modify-write.
+# coincidence 32 B of atomic read-modify-write; with the voxel dose tally, 32 B (two u64
+# read-modify-writes of the voxel's 16-byte record) per Compton or photoelectric deposit.
 U_DRAW, U_STEP, U_COMPTON, U_SHELL = 25, 83, 40, 35
-B_STEP, B_MATCHANGE, B_COINC = 4, 24, 32
+B_STEP, B_MATCHANGE, B_COINC, B_DOSE = 4, 24, 32, 32
 VALU_PEAK = 39.3e12      # 256 CU x 4 SIMD x 16 lanes x 2.4 GHz, non-packed lane-ops/s
 HBM_PEAK = 8.0e12        # B/s
 
 
-def algorithmic(stats):
+def algorithmic(stats, voxel_dose=False):
     steps = stats["woodcock_steps"]
     other_draws = max(0, stats["rn_draws"] - 2 * steps)
     ops = (U_STEP * steps + U_DRAW * other_draws + U_COMPTON * stats["compton"] + U_SHELL * stats["compton_shell_evals"])
     bytes_ = B_STEP * steps + B_MATCHANGE * stats["material_changes"] + B_COINC * stats["coincidences"]
+    if voxel_dose:
+        bytes_ += B_DOSE * (stats["compton"] + stats["photoelectric"])
     return ops, bytes_
 
 
@@ -588,12 +591,15 @@ def main():
                                       [x + y for x, y in zip(agg[k], st[k])] for k in agg}
     sim.close()
 
-    ops, byts = algorithmic(agg)
+    ops, byts = algorithmic(agg, voxel_dose=torso)
     kt = sum(kernel_ms) / 1e3
     avg_ms = sum(kernel_ms) / len(kernel_ms)
     valu = ops / kt
     hbm = byts / kt
-    traffic = None if torso else load_traffic()
+    # measured HBM bytes per launch of this workload at its default size (a profile of that run)
+    traffic = None
+    if not a.decays and not a.singles:
+        traffic = load_traffic("transport_torso_traffic.json" if torso else "transport_head_traffic.json")
     out = {
         "metric": "detected coincidences/s on 1 MI355X", "value": coinc / elapsed, "unit": "coincidences/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * elapsed / a.steps,

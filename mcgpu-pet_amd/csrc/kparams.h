@@ -156,5 +156,8 @@ hipError_t mcpet_launch_gather_singles(const mcpet_single* s, const unsigned* pe
 // in CSR form grp_off[D + 1] / grp_sino, angle groups of gh rows, radial groups of gw bins).
 hipError_t mcpet_launch_rebin(const int* sino, const int* grp_off, const int* grp_sino, int nangles, int nrad, int D, int H, int W,
                               int gh, int gw, float* out, hipStream_t st);
+// the same for two tallies (sa -> oa, sb -> ob) in one launch
+hipError_t mcpet_launch_rebin2(const int* sa, const int* sb, const int* grp_off, const int* grp_sino, int nangles, int nrad, int D,
+                               int H, int W, int gh, int gw, float* oa, float* ob, hipStream_t st);
 
 #endif

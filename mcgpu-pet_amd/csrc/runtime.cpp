@@ -1124,9 +1124,14 @@ static mcpet_status rebin_setup(mcpet_ctx* c, int D, int H, int W)
 
 // Enqueue the rebinning of tally `which` into the device buffer d_out on stream st (after
 // rebin_setup for this grid).
+static const int* rebin_source(const mcpet_ctx* c, int which)
+{
+  return which == 0 ? c->d_trues : which == 1 ? c->d_scatter : which == 2 ? c->d_randoms : which == 3 ? c->d_spurious : nullptr;
+}
+
 static mcpet_status rebin_launch(mcpet_ctx* c, int which, int D, int H, int W, float* d_out, hipStream_t st)
 {
-  const int* src = which == 0 ? c->d_trues : which == 1 ? c->d_scatter : which == 2 ? c->d_randoms : which == 3 ? c->d_spurious : nullptr;
+  const int* src = rebin_source(c, which);
   if (which < 0 || which > 3) return set_err(c, MCPET_E_ARG, "rebin: which must be 0 (trues), 1 (scatter), 2 (randoms), 3 (spurious)");
   if (!src) return set_err(c, MCPET_E_STATE, "rebin: this tally is not kept in this mode");
   const mcpet_config& cfg = c->cfg;
@@ -1153,6 +1158,22 @@ mcpet_status mcpet_ctx_rebin_device(mcpet_ctx* c, int which, int D, int H, int W
   mcpet_status e = rebin_setup(c, D, H, W);
   if (e == MCPET_OK) e = rebin_launch(c, which, D, H, W, d_out, st);
   return e;
+}
+
+mcpet_status mcpet_ctx_rebin2_device(mcpet_ctx* c, int wa, int wb, int D, int H, int W, float* d_a, float* d_b, hipStream_t st)
+{
+  if (!d_a || !d_b) return set_err(c, MCPET_E_ARG, "rebin: null output");
+  if (wa < 0 || wa > 3 || wb < 0 || wb > 3)
+    return set_err(c, MCPET_E_ARG, "rebin: which must be 0 (trues), 1 (scatter), 2 (randoms), 3 (spurious)");
+  const int *sa = rebin_source(c, wa), *sb = rebin_source(c, wb);
+  if (!sa || !sb) return set_err(c, MCPET_E_STATE, "rebin: this tally is not kept in this mode");
+  mcpet_status e = rebin_setup(c, D, H, W);
+  if (e != MCPET_OK) return e;
+  const mcpet_config& cfg = c->cfg;
+  const int gh = (cfg.nangles + H - 1) / H, gw = (cfg.nrad + W - 1) / W;
+  hipError_t he = mcpet_launch_rebin2(sa, sb, c->d_rb_off, c->d_rb_sino, cfg.nangles, cfg.nrad, D, H, W, gh, gw, d_a, d_b, st);
+  if (he != hipSuccess) return set_err(c, MCPET_E_HIP, std::string("rebin launch failed: ") + hipGetErrorString(he));
+  return MCPET_OK;
 }
 
 mcpet_status mcpet_ctx_decays(mcpet_ctx* c, long long* decays)

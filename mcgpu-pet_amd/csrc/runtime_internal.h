@@ -36,6 +36,8 @@ mcpet_status mcpet_run_contexts(const CtxRun* runs, int n, hipStream_t stream, K
 // Rebinning of tally `which` of the context's last run into the device buffer d_out (D*H*W
 // floats), enqueued on `st` after the run has completed (mcpet_rebin_sinogram's grid).
 mcpet_status mcpet_ctx_rebin_device(mcpet_ctx* c, int which, int D, int H, int W, float* d_out, hipStream_t st);
+// Tallies wa and wb rebinned together (one pass over both sinograms) into d_a and d_b.
+mcpet_status mcpet_ctx_rebin2_device(mcpet_ctx* c, int wa, int wb, int D, int H, int W, float* d_a, float* d_b, hipStream_t st);
 // Decays of the context's last run (its event counter; waits for nothing: call after the run).
 mcpet_status mcpet_ctx_decays(mcpet_ctx* c, long long* decays);
 // Activity line integrals and attenuation factors on a D x H x W rebinned grid, on the device:

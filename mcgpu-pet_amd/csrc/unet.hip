@@ -475,9 +475,12 @@ inline long long tiled_blocks(const ConvK& a)
 }
 // The tiled products on volumes whose tiles fill the chip (at least four workgroups per CU);
 // smaller volumes (the coarser U-Net levels) keep k_conv's 16-voxel tiles and in-workgroup k split.
+#ifndef MCPET_TILED_MIN
+#define MCPET_TILED_MIN 1024
+#endif
 inline bool use_tiled(const ConvK& a)
 {
-  return a.ks == 3 && a.stride == 1 && a.Cbp % 32 == 0 && tiled_blocks(a) >= 1024;
+  return a.ks == 3 && a.stride == 1 && a.Cbp % 32 == 0 && tiled_blocks(a) >= MCPET_TILED_MIN;
 }
 
 template <int MODE>
@@ -1400,8 +1403,7 @@ extern "C" mcpet_status mcpet_surrogate_estimate(mcpet_surrogate* s, mcpet_ctx* 
   if (mcpet_ctx_device(ctx) != s->device) return serr(s, MCPET_E_ARG, "mcpet_surrogate_estimate: context on another device");
   SHIP(s, hipSetDevice(s->device));
   const size_t nv = s->vr.size();
-  mcpet_status st = mcpet_ctx_rebin_device(ctx, 1, s->D, s->H, s->W, s->raw_s, s->st);
-  if (st == MCPET_OK) st = mcpet_ctx_rebin_device(ctx, 0, s->D, s->H, s->W, s->raw_t, s->st);
+  mcpet_status st = mcpet_ctx_rebin2_device(ctx, 1, 0, s->D, s->H, s->W, s->raw_s, s->raw_t, s->st);
   long long n = 0;
   if (st == MCPET_OK) st = mcpet_ctx_decays(ctx, &n);
   const float* maps = nullptr;
@@ -1553,7 +1555,54 @@ __global__ __launch_bounds__(256) void k_rebin_rows(const int* __restrict__ sino
   }
 }
 
+// k_rebin_rows for two tallies of the same geometry in one pass (the plane-group lists read
+// once, twice the loads in flight): the same integer sums as two k_rebin_rows launches.
+__global__ __launch_bounds__(256) void k_rebin_rows2(const int* __restrict__ sa, const int* __restrict__ sb,
+                                                     const int* __restrict__ grp_off, const int* __restrict__ grp_sino,
+                                                     int nangles, int nrad, int H, int W, int gh, int gw, float* __restrict__ oa,
+                                                     float* __restrict__ ob)
+{
+  __shared__ unsigned long long bin[2][256];
+  const int h = blockIdx.x, d = blockIdx.y, r = threadIdx.x;
+  const int a0 = h * gh, rows = max(0, min(gh, nangles - a0));
+  const int j0 = grp_off[d], nj = grp_off[d + 1] - j0;
+  unsigned long long acc_a = 0ull, acc_b = 0ull;
+  if (r < nrad) {
+#pragma unroll 4
+    for (int j = 0; j < nj; j++) {
+      const size_t base = ((size_t)grp_sino[j0 + j] * nangles + a0) * nrad + r;
+      for (int a = 0; a < rows; a++) {
+        acc_a += (unsigned)sa[base + (size_t)a * nrad];
+        acc_b += (unsigned)sb[base + (size_t)a * nrad];
+      }
+    }
+  }
+  bin[0][r] = acc_a;
+  bin[1][r] = acc_b;
+  __syncthreads();
+  if (r < W) {
+    unsigned long long ta = 0ull, tb = 0ull;
+    for (int i = r * gw; i < min(nrad, r * gw + gw); i++) {
+      ta += bin[0][i];
+      tb += bin[1][i];
+    }
+    oa[((size_t)d * H + h) * W + r] = (float)ta;
+    ob[((size_t)d * H + h) * W + r] = (float)tb;
+  }
+}
+
 }  // namespace
+
+hipError_t mcpet_launch_rebin2(const int* sa, const int* sb, const int* grp_off, const int* grp_sino, int nangles, int nrad, int D,
+                               int H, int W, int gh, int gw, float* oa, float* ob, hipStream_t st)
+{
+  if (W > 256 || nrad > 256) {
+    hipError_t e = mcpet_launch_rebin(sa, grp_off, grp_sino, nangles, nrad, D, H, W, gh, gw, oa, st);
+    return e == hipSuccess ? mcpet_launch_rebin(sb, grp_off, grp_sino, nangles, nrad, D, H, W, gh, gw, ob, st) : e;
+  }
+  hipLaunchKernelGGL(k_rebin_rows2, dim3(H, D), dim3(256), 0, st, sa, sb, grp_off, grp_sino, nangles, nrad, H, W, gh, gw, oa, ob);
+  return hipGetLastError();
+}
 
 hipError_t mcpet_launch_rebin(const int* sino, const int* grp_off, const int* grp_sino, int nangles, int nrad, int D, int H, int W,
                               int gh, int gw, float* out, hipStream_t st)
