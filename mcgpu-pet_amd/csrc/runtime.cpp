@@ -1088,13 +1088,17 @@ extern "C" mcpet_status mcpet_get_detector(const mcpet_ctx* c, mcpet_detector* o
 // Single-slice rebinning of a sinogram tally onto a D x H x W grid (plane groups of
 // ceil(NZS / D) ring sums, angle groups of ceil(NANGLES / H), radial groups of ceil(NRAD / W)).
 // rebin_setup builds (once per grid) the plane-group lists and the context's output buffer.
-static mcpet_status rebin_setup(mcpet_ctx* c, int D, int H, int W)
+// wait_run: the host waits for the run (and any earlier rebinning) first; without it the
+// caller orders its launches after the run itself, and the host waits only when the grid's
+// plane lists have to be (re)built.
+static mcpet_status rebin_setup(mcpet_ctx* c, int D, int H, int W, bool wait_run = true)
 {
   if (!c || D < 1 || H < 1 || W < 1 || W > 256) return set_err(c, MCPET_E_ARG, "rebin: bad arguments (W <= 256)");
   if (!c->ran) return set_err(c, MCPET_E_STATE, "rebin: no run");
   HIPCHK(c, hipSetDevice(c->device));
-  HIPCHK(c, hipStreamSynchronize(c->stream));   // the run (and any earlier rebinning) is complete
-  if (c->rb_dims[0] == D && c->rb_dims[1] == H && c->rb_dims[2] == W) return MCPET_OK;
+  const bool cached = c->rb_dims[0] == D && c->rb_dims[1] == H && c->rb_dims[2] == W;
+  if (wait_run || !cached) HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (cached) return MCPET_OK;
   const mcpet_config& cfg = c->cfg;
   const int gz = (cfg.nzs + D - 1) / D;
   std::vector<int> plane(cfg.nsinos);
@@ -1167,14 +1171,19 @@ mcpet_status mcpet_ctx_rebin2_device(mcpet_ctx* c, int wa, int wb, int D, int H,
     return set_err(c, MCPET_E_ARG, "rebin: which must be 0 (trues), 1 (scatter), 2 (randoms), 3 (spurious)");
   const int *sa = rebin_source(c, wa), *sb = rebin_source(c, wb);
   if (!sa || !sb) return set_err(c, MCPET_E_STATE, "rebin: this tally is not kept in this mode");
-  mcpet_status e = rebin_setup(c, D, H, W);
+  mcpet_status e = rebin_setup(c, D, H, W, false);
   if (e != MCPET_OK) return e;
+  // after the last mcpet_run's transport on the device (no host wait); a context run by the
+  // serving loop's batched launches has completed before its estimate is taken
+  HIPCHK(c, hipStreamWaitEvent(st, c->ev_k1, 0));
   const mcpet_config& cfg = c->cfg;
   const int gh = (cfg.nangles + H - 1) / H, gw = (cfg.nrad + W - 1) / W;
   hipError_t he = mcpet_launch_rebin2(sa, sb, c->d_rb_off, c->d_rb_sino, cfg.nangles, cfg.nrad, D, H, W, gh, gw, d_a, d_b, st);
   if (he != hipSuccess) return set_err(c, MCPET_E_HIP, std::string("rebin launch failed: ") + hipGetErrorString(he));
   return MCPET_OK;
 }
+
+const unsigned long long* mcpet_ctx_decay_counter(const mcpet_ctx* c) { return c->d_counters + KC_DECAYS; }
 
 mcpet_status mcpet_ctx_decays(mcpet_ctx* c, long long* decays)
 {

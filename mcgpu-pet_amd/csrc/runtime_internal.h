@@ -36,10 +36,14 @@ mcpet_status mcpet_run_contexts(const CtxRun* runs, int n, hipStream_t stream, K
 // Rebinning of tally `which` of the context's last run into the device buffer d_out (D*H*W
 // floats), enqueued on `st` after the run has completed (mcpet_rebin_sinogram's grid).
 mcpet_status mcpet_ctx_rebin_device(mcpet_ctx* c, int which, int D, int H, int W, float* d_out, hipStream_t st);
-// Tallies wa and wb rebinned together (one pass over both sinograms) into d_a and d_b.
+// Tallies wa and wb rebinned together (one pass over both sinograms) into d_a and d_b, enqueued
+// on `st` behind the context's last mcpet_run without a host wait (the host waits only when
+// the grid's plane lists are built, at the first call for a grid).
 mcpet_status mcpet_ctx_rebin2_device(mcpet_ctx* c, int wa, int wb, int D, int H, int W, float* d_a, float* d_b, hipStream_t st);
 // Decays of the context's last run (its event counter; waits for nothing: call after the run).
 mcpet_status mcpet_ctx_decays(mcpet_ctx* c, long long* decays);
+// Device address of the context's decay counter (u64; final once its last run has completed).
+const unsigned long long* mcpet_ctx_decay_counter(const mcpet_ctx* c);
 // Activity line integrals and attenuation factors on a D x H x W rebinned grid, on the device:
 // [3][D][H][W] = activity integrals, the same times float32(1 / their maximum), attenuation
 // factors; owned by the context, computed at the first call for this grid.

@@ -926,11 +926,11 @@ __global__ void k_normalize(const float* __restrict__ s, const float* __restrict
 }
 // scatter per decay: max(output, 0) times float(total trues / (voxels * decays))
 __global__ void k_estimate(const float* __restrict__ out, float* __restrict__ est, size_t n, const double* __restrict__ total,
-                           double decays)
+                           const unsigned long long* __restrict__ decay_counter)
 {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double tot = total[0];
+  const double tot = total[0], decays = (double)(long long)*decay_counter;
   const float f = (tot > 0.0 && decays > 0.0) ? (float)(tot / ((double)n * decays)) : 0.0f;
   est[i] = fmaxf(out[i], 0.0f) * f;
 }
@@ -984,6 +984,7 @@ struct mcpet_surrogate {
   hipStream_t st = nullptr;
   hipEvent_t ev_f0 = nullptr, ev_f1 = nullptr;   // around the last forward of mcpet_surrogate_forward
   float* h_stage = nullptr;   // pinned host staging of the fused estimate's two outputs (2 x voxels)
+  unsigned long long* h_decays = nullptr;   // pinned: the estimated run's decays
   bool fwd_timed = false;
   std::vector<void*> allocs;
   std::string err;
@@ -1264,6 +1265,7 @@ extern "C" mcpet_status mcpet_surrogate_create(int32_t D, int32_t H, int32_t W, 
   if (e == hipSuccess) e = hipEventCreate(&s->ev_f0);
   if (e == hipSuccess) e = hipEventCreate(&s->ev_f1);
   if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_stage, 2 * sizeof(float) * (size_t)D * H * W, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_decays, sizeof(unsigned long long), hipHostMallocDefault);
   struct A {
     float** p;
     size_t n;
@@ -1403,9 +1405,10 @@ extern "C" mcpet_status mcpet_surrogate_estimate(mcpet_surrogate* s, mcpet_ctx* 
   if (mcpet_ctx_device(ctx) != s->device) return serr(s, MCPET_E_ARG, "mcpet_surrogate_estimate: context on another device");
   SHIP(s, hipSetDevice(s->device));
   const size_t nv = s->vr.size();
+  // everything below is enqueued behind the context's run on the device: the host waits once,
+  // for the outputs
   mcpet_status st = mcpet_ctx_rebin2_device(ctx, 1, 0, s->D, s->H, s->W, s->raw_s, s->raw_t, s->st);
-  long long n = 0;
-  if (st == MCPET_OK) st = mcpet_ctx_decays(ctx, &n);
+  const unsigned long long* d_decays = mcpet_ctx_decay_counter(ctx);
   const float* maps = nullptr;
   if (st == MCPET_OK && s->cin == 4) st = mcpet_ctx_project_device(ctx, s->D, s->H, s->W, &maps);
   if (st != MCPET_OK) return serr(s, st, std::string("mcpet_surrogate_estimate: ") + mcpet_last_error(ctx));
@@ -1418,15 +1421,16 @@ extern "C" mcpet_status mcpet_surrogate_estimate(mcpet_surrogate* s, mcpet_ctx* 
   SHIP(s, forward(s, false));
   hipLaunchKernelGGL(k_output, dim3(blocks_for(nv)), dim3(256), 0, s->st, s->x0, s->r, s->out, (const float*)nullptr,
                      (float*)nullptr, 0.0f, (float*)nullptr, (int)nv, kNoCl);
-  hipLaunchKernelGGL(k_estimate, dim3(blocks_for(nv)), dim3(256), 0, s->st, s->out, s->est, nv, s->sums + 256, (double)n);
+  hipLaunchKernelGGL(k_estimate, dim3(blocks_for(nv)), dim3(256), 0, s->st, s->out, s->est, nv, s->sums + 256, d_decays);
   SHIP(s, hipGetLastError());
   // through pinned staging (one DMA per output), then host copies into the caller's buffers
   SHIP(s, hipMemcpyAsync(s->h_stage, s->est, sizeof(float) * nv, hipMemcpyDeviceToHost, s->st));
   if (scatter_counts) SHIP(s, hipMemcpyAsync(s->h_stage + nv, s->raw_s, sizeof(float) * nv, hipMemcpyDeviceToHost, s->st));
+  SHIP(s, hipMemcpyAsync(s->h_decays, d_decays, sizeof(unsigned long long), hipMemcpyDeviceToHost, s->st));
   SHIP(s, hipStreamSynchronize(s->st));
   std::memcpy(estimate, s->h_stage, sizeof(float) * nv);
   if (scatter_counts) std::memcpy(scatter_counts, s->h_stage + nv, sizeof(float) * nv);
-  if (decays) *decays = n;
+  if (decays) *decays = (int64_t)*s->h_decays;
   return MCPET_OK;
 }
 
@@ -1488,6 +1492,7 @@ extern "C" void mcpet_surrogate_destroy(mcpet_surrogate* s)
   if (s->st) (void)hipStreamSynchronize(s->st);
   for (void* p : s->allocs) (void)hipFree(p);
   if (s->h_stage) (void)hipHostFree(s->h_stage);
+  if (s->h_decays) (void)hipHostFree(s->h_decays);
   if (s->ev_f0) (void)hipEventDestroy(s->ev_f0);
   if (s->ev_f1) (void)hipEventDestroy(s->ev_f1);
   if (s->st) (void)hipStreamDestroy(s->st);
