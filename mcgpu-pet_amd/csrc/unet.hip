@@ -200,36 +200,60 @@ __global__ __launch_bounds__(256) void k_conv(ConvK a)
     kb = blockIdx.z * a.k_per_split;
     ke = min(a.K, kb + a.k_per_split);
   }
-  for (int k0 = kb + wave * 32; k0 < ke; k0 += 128) {
-    const int k = k0 + kg * 8;
-    bf16x8 b;
-    if (MODE == FWD || MODE == BDATA) {
-      const int t = k / a.Cbp, c0 = k - t * a.Cbp;
-      int dz, dy, dx;
-      tap3(t, a.ks, dz, dy, dx);
+  if constexpr (MODE == FWD || MODE == BDATA) {
+    // The lane's k = k0 + 8 kg advances by 128 per step: its tap t and channel c0 (k = t Cbp +
+    // c0) are carried forward (one division before the loop), the tap's offsets come from two
+    // multiplications (exact for t < 27: t / 9 = (7282 t) >> 16, t / 3 = (21846 t) >> 16), the
+    // voxel index is 32-bit and the weight rows' addresses are fixed.
+    const int Cbp = a.Cbp, dt = 128 / Cbp, dc = 128 - dt * Cbp;
+    int k = wave * 32 + kg * 8;
+    int t = k / Cbp, c0 = k - t * Cbp;
+    const __bf16* arow[MT];
+#pragma unroll
+    for (int i = 0; i < MT; i++) arow[i] = a.A + (size_t)(m0 + 16 * i + r) * a.Kp;
+    bf16x8 zero;
+#pragma unroll
+    for (int e = 0; e < 8; e++) zero[e] = to_bf16(0.0f);
+    for (int k0 = wave * 32; k0 < ke; k0 += 128, k += 128) {
+      int dz = -1, dy = -1, dx = -1;   // kernel size 1: tap 0, no offset
+      if (a.ks == 3) {
+        const int q9 = (t * 7282) >> 16, q3 = (t * 21846) >> 16;
+        dz += q9;
+        dy += q3 - 3 * q9;
+        dx += t - 3 * q3;
+      } else {
+        dz = dy = dx = 0;
+      }
       bool in = nvalid && k < ke;
-      size_t vox = 0;
+      int vox = 0;
       if (MODE == FWD) {
         const int z = cz * a.stride + dz, y = cy * a.stride + dy, x = cx * a.stride + dx;
         in = in && z >= 0 && y >= 0 && x >= 0 && z < a.D && y < a.H && x < a.W;
-        vox = ((size_t)z * a.H + y) * a.W + x;
+        vox = (z * a.H + y) * a.W + x;
       } else {
         int qz = 0, qy = 0, qx = 0;
         in = in && src_of(cz, dz, a.stride, a.oD, qz) && src_of(cy, dy, a.stride, a.oH, qy) && src_of(cx, dx, a.stride, a.oW, qx);
-        vox = ((size_t)qz * a.oH + qy) * a.oW + qx;
+        vox = (qz * a.oH + qy) * a.oW + qx;
       }
-      if (in) {
-        b = *(const bf16x8*)(a.Xl + vox * a.Cbp + c0);
-      } else {
-#pragma unroll
-        for (int e = 0; e < 8; e++) b[e] = to_bf16(0.0f);
-      }
+      bf16x8 b = zero;
+      if (in) b = *(const bf16x8*)(a.Xl + (size_t)vox * Cbp + c0);
 #pragma unroll
       for (int i = 0; i < MT; i++) {
-        const bf16x8 av = *(const bf16x8*)(a.A + (size_t)(m0 + 16 * i + r) * a.Kp + k);
+        const bf16x8 av = *(const bf16x8*)(arow[i] + k);
         acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b, acc[i], 0, 0, 0);
       }
-    } else {
+      c0 += dc;
+      t += dt;
+      if (c0 >= Cbp) {
+        c0 -= Cbp;
+        t++;
+      }
+    }
+  } else
+  for (int k0 = kb + wave * 32; k0 < ke; k0 += 128) {
+    const int k = k0 + kg * 8;
+    bf16x8 b;
+    {
       // B: X[ci][s q + d_t] for q = k .. k + 7
       int dz, dy, dx;
       tap3(bt, a.ks, dz, dy, dx);
