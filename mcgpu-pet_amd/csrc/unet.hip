@@ -344,14 +344,19 @@ __global__ __launch_bounds__(256) void k_conv(ConvK a)
 }
 
 // Forward and input-gradient products of a stride-1 3x3x3 convolution whose B tensor (the
-// input, or the output gradient) has a multiple of 32 (padded) channels, on a tile of 4 rows x
-// 16 voxels of one z-plane per workgroup (wave w: row y0 + w, the 16 voxels as its MFMA
-// columns). The B tile with its one-voxel halo (3 x 6 x 18 voxels) is staged in LDS 32 channels
-// at a time, zero outside the volume, so a k-step is one LDS read at a compile-time offset per
-// tap (no tap decode, no bounds test, no global gather) and MT MFMAs; the weights are read as in
-// k_conv and reused across the tile's 64 voxels. FWD reads voxel p + d_t for tap t, BDATA
+// input, or the output gradient) has a multiple of 32 (padded) channels, on a tile of 4 FT_R
+// rows x 16 voxels of one z-plane per workgroup (wave w: rows y0 + FT_R w .. + FT_R - 1, the 16
+// voxels of each as its MFMA columns). The B tile with its one-voxel halo (3 x (4 FT_R + 2) x 18
+// voxels) is staged in LDS 32 channels at a time, zero outside the volume, so a k-step is one
+// LDS read per row at a compile-time offset per tap (no tap decode, no bounds test, no global
+// gather) and MT MFMAs per row; each weight fragment, read as in k_conv, serves the wave's FT_R
+// rows (the weights stream from L2 once per FT_R rows). FWD reads voxel p + d_t for tap t, BDATA
 // (stride 1: q = p - d_t) voxel p - d_t; the epilogues are k_conv's.
-#define FT_Y 4
+#ifndef MCPET_TILED_ROWS
+#define MCPET_TILED_ROWS 2
+#endif
+#define FT_R MCPET_TILED_ROWS
+#define FT_Y (4 * FT_R)
 #define FT_X 16
 #define FT_HY (FT_Y + 2)
 #define FT_HX (FT_X + 2)
@@ -360,7 +365,7 @@ __global__ __launch_bounds__(256) void k_conv(ConvK a)
 template <int MODE, int MT>
 __global__ __launch_bounds__(256) void k_conv_tiled(ConvK a)
 {
-  __shared__ bf16x8 s_halo[FT_HALO * 4];              // [halo voxel][channel group of 8], 20.7 KB
+  __shared__ bf16x8 s_halo[FT_HALO * 4];              // [halo voxel][channel group of 8]
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r = lane & 15, kg = lane >> 4;
   const int tiles_x = (a.W + FT_X - 1) / FT_X, tiles_y = (a.H + FT_Y - 1) / FT_Y;
@@ -370,9 +375,11 @@ __global__ __launch_bounds__(256) void k_conv_tiled(ConvK a)
   const int y0 = (b % tiles_y) * FT_Y;
   const int z = b / tiles_y;
   const int m0 = blockIdx.y * (16 * MT);
-  floatx4 acc[MT];
+  floatx4 acc[FT_R][MT];
 #pragma unroll
-  for (int i = 0; i < MT; i++) acc[i] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int j = 0; j < FT_R; j++)
+#pragma unroll
+    for (int i = 0; i < MT; i++) acc[j][i] = floatx4{0.f, 0.f, 0.f, 0.f};
   const __bf16* arow[MT];
 #pragma unroll
   for (int i = 0; i < MT; i++) arow[i] = a.A + (size_t)(m0 + 16 * i + r) * a.Kp + kg * 8;
@@ -395,57 +402,64 @@ __global__ __launch_bounds__(256) void k_conv_tiled(ConvK a)
       const int dz = MODE == FWD ? t / 9 : 2 - t / 9;            // halo coordinates of p + d_t or p - d_t
       const int dy = MODE == FWD ? (t / 3) % 3 : 2 - (t / 3) % 3;
       const int dx = MODE == FWD ? t % 3 : 2 - t % 3;
-      const bf16x8 bv = s_halo[((dz * FT_HY + wave + dy) * FT_HX + r + dx) * 4 + kg];
       const int k = t * a.Cbp + c0;
+      bf16x8 av[MT];
 #pragma unroll
-      for (int i = 0; i < MT; i++) {
-        const bf16x8 av = *(const bf16x8*)(arow[i] + k);
-        acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[i], 0, 0, 0);
+      for (int i = 0; i < MT; i++) av[i] = *(const bf16x8*)(arow[i] + k);
+#pragma unroll
+      for (int j = 0; j < FT_R; j++) {
+        const bf16x8 bv = s_halo[((dz * FT_HY + FT_R * wave + j + dy) * FT_HX + r + dx) * 4 + kg];
+#pragma unroll
+        for (int i = 0; i < MT; i++) acc[j][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv, acc[j][i], 0, 0, 0);
       }
     }
     __syncthreads();
   }
-  // epilogue (as k_conv's): this lane's column is voxel (z, y0 + wave, x0 + r)
-  const int x = x0 + r, y = y0 + wave;
-  if (x >= a.W || y >= a.H) return;
-  const int n = (z * a.H + y) * a.W + x;
+  // epilogue (as k_conv's): this lane's columns are voxels (z, y0 + FT_R wave + j, x0 + r)
+  const int x = x0 + r;
 #pragma unroll
-  for (int i = 0; i < MT; i++) {
-    const int row0 = m0 + 16 * i + kg * 4;
-    float ov[4];
+  for (int j = 0; j < FT_R; j++) {
+    const int y = y0 + FT_R * wave + j;
+    if (x >= a.W || y >= a.H) continue;
+    const int n = (z * a.H + y) * a.W + x;
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-      const int row = row0 + v;
-      ov[v] = 0.0f;
-      if (row >= a.M) continue;
-      const size_t idx = (size_t)row * a.N + n;
-      float o;
-      if (MODE == FWD) {
-        o = acc[i][v] + a.bias[row];
-        if (a.relu) o = fmaxf(o, 0.0f);
-        if (a.C) a.C[idx] = o;
-      } else {
-        o = acc[i][v] + (a.beta ? a.C[idx] : 0.0f);
-        if (a.mask && !(a.mask[idx] > 0.0f)) o = 0.0f;
-        a.C[idx] = o;
+    for (int i = 0; i < MT; i++) {
+      const int row0 = m0 + 16 * i + kg * 4;
+      float ov[4];
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int row = row0 + v;
+        ov[v] = 0.0f;
+        if (row >= a.M) continue;
+        const size_t idx = (size_t)row * a.N + n;
+        float o;
+        if (MODE == FWD) {
+          o = acc[j][i][v] + a.bias[row];
+          if (a.relu) o = fmaxf(o, 0.0f);
+          if (a.C) a.C[idx] = o;
+        } else {
+          o = acc[j][i][v] + (a.beta ? a.C[idx] : 0.0f);
+          if (a.mask && !(a.mask[idx] > 0.0f)) o = 0.0f;
+          a.C[idx] = o;
+        }
+        ov[v] = o;
       }
-      ov[v] = o;
-    }
-    const ClOut* outs[2] = {&a.o1, &a.o2};
+      const ClOut* outs[2] = {&a.o1, &a.o2};
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
-      const ClOut& o = *outs[j];
-      if (!o.p) continue;
-      __bf16* dst = o.p + (size_t)n * o.ld + o.off + row0;
-      if (row0 + 3 < a.M && ((o.ld | o.off) & 3) == 0) {
-        bf16x4 q;
+      for (int jj = 0; jj < 2; jj++) {
+        const ClOut& o = *outs[jj];
+        if (!o.p) continue;
+        __bf16* dst = o.p + (size_t)n * o.ld + o.off + row0;
+        if (row0 + 3 < a.M && ((o.ld | o.off) & 3) == 0) {
+          bf16x4 q;
 #pragma unroll
-        for (int v = 0; v < 4; v++) q[v] = to_bf16(ov[v]);
-        *(bf16x4*)dst = q;
-      } else {
+          for (int v = 0; v < 4; v++) q[v] = to_bf16(ov[v]);
+          *(bf16x4*)dst = q;
+        } else {
 #pragma unroll
-        for (int v = 0; v < 4; v++)
-          if (row0 + v < a.M) dst[v] = to_bf16(ov[v]);
+          for (int v = 0; v < 4; v++)
+            if (row0 + v < a.M) dst[v] = to_bf16(ov[v]);
+        }
       }
     }
   }
@@ -492,12 +506,12 @@ hipError_t launch_conv(const ConvK& a, hipStream_t st)
 }
 
 // Stride-1 3x3x3 FWD / BDATA products with Cbp % 32 == 0 (k_conv_tiled); the row blocks as
-// k_conv's, so the A rows read stay inside the padded weight copy.
+// k_conv's, so the A rows read stay inside the padded weight copy. Counted in 4-row tiles.
 inline long long tiled_blocks(const ConvK& a)
 {
-  return (long long)((a.W + FT_X - 1) / FT_X) * ((a.H + FT_Y - 1) / FT_Y) * a.D * ((a.M + 16 * mt_for(a.M) - 1) / (16 * mt_for(a.M)));
+  return (long long)((a.W + FT_X - 1) / FT_X) * ((a.H + 3) / 4) * a.D * ((a.M + 16 * mt_for(a.M) - 1) / (16 * mt_for(a.M)));
 }
-// The tiled products on volumes whose tiles fill the chip (at least four workgroups per CU);
+// The tiled products on volumes with at least 1024 four-row tiles (the full-resolution level);
 // smaller volumes (the coarser U-Net levels) keep k_conv's 16-voxel tiles and in-workgroup k split.
 #ifndef MCPET_TILED_MIN
 #define MCPET_TILED_MIN 1024

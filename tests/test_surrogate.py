@@ -183,7 +183,8 @@ def _layer_rel(a, b, cin, c):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("grid,cin,c,batch", [((8, 8, 8), 2, 4, 2), ((12, 8, 16), 2, 3, 1), ((4, 4, 4), 1, 2, 3),
-                                              ((8, 12, 12), 4, 32, 2), ((8, 8, 8), 4, 56, 1), ((4, 8, 8), 4, 16, 1)])
+                                              ((8, 12, 12), 4, 32, 2), ((8, 8, 8), 4, 56, 1), ((4, 8, 8), 4, 16, 1),
+                                              ((16, 64, 64), 4, 32, 1)])
 def test_device_unet_forward_and_gradients_match_bf16_reference(grid, cin, c, batch):
     """Output, loss and every layer's weight and bias gradients of the device network against
     the BF16-operand CPU reference (float64 accumulation). The remaining differences come
@@ -194,7 +195,9 @@ def test_device_unet_forward_and_gradients_match_bf16_reference(grid, cin, c, ba
     2-, 3- and 4-tile convolution kernels, several row blocks per product and the epilogue split
     over the workgroup's waves, in the forward, input-gradient and weight-gradient modes; the
     stride-1 layers' weight gradients take the LDS-tiled product (32-channel blocks, partial
-    blocks at widths 16 and 56, voxel tiles cut by the grid's edges)."""
+    blocks at widths 16 and 56, voxel tiles cut by the grid's edges). On the 16 x 64 x 64 grid
+    (1024 four-row tiles per plane set) the full-resolution forward and input-gradient products
+    run on their LDS tiles (`k_conv_tiled`), as on the shipped 32 x 56 x 56 grid."""
     rng = np.random.default_rng(sum(grid) + cin + c)
     with mcpet.Surrogate(grid, in_channels=cin, width=c, seed=7) as s:
         assert s.n_params == n_params(cin, c)
