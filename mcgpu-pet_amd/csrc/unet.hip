@@ -343,6 +343,135 @@ __global__ __launch_bounds__(256) void k_conv(ConvK a)
   }
 }
 
+// Forward and input-gradient products as k_conv's, with NC 16-voxel column tiles per workgroup:
+// each wave's weight fragments (MT per k-step) serve the NC tiles, so the weights stream from
+// L2 once per 16 NC voxels; the four waves still split the k-steps and reduce through LDS.
+#ifndef MCPET_CONV_COLS
+#define MCPET_CONV_COLS 2
+#endif
+template <int MODE, int MT, int NC>
+__global__ __launch_bounds__(256) void k_conv_cols(ConvK a)
+{
+  static_assert(MODE == FWD || MODE == BDATA, "forward and input-gradient products only");
+  __shared__ floatx4 red[4][NC * MT][64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = lane & 15, kg = lane >> 4;
+  const int m0 = blockIdx.y * (16 * MT);
+  int nn[NC], cz[NC], cy[NC], cx[NC];
+  bool nvalid[NC];
+#pragma unroll
+  for (int j = 0; j < NC; j++) {
+    const int n = (blockIdx.x * NC + j) * 16 + r;   // this lane's column in tile j
+    nn[j] = n;
+    nvalid[j] = n < a.N;
+    const int w = MODE == FWD ? a.oW : a.W, h = MODE == FWD ? a.oH : a.H;
+    cx[j] = n % w;
+    cy[j] = (n / w) % h;
+    cz[j] = n / (w * h);
+  }
+  floatx4 acc[NC][MT];
+#pragma unroll
+  for (int j = 0; j < NC; j++)
+#pragma unroll
+    for (int i = 0; i < MT; i++) acc[j][i] = floatx4{0.f, 0.f, 0.f, 0.f};
+  // tap and channel carried across k-steps as in k_conv
+  const int Cbp = a.Cbp, dt = 128 / Cbp, dc = 128 - dt * Cbp, ke = a.K;
+  int k = wave * 32 + kg * 8;
+  int t = k / Cbp, c0 = k - t * Cbp;
+  const __bf16* arow[MT];
+#pragma unroll
+  for (int i = 0; i < MT; i++) arow[i] = a.A + (size_t)(m0 + 16 * i + r) * a.Kp;
+  bf16x8 zero;
+#pragma unroll
+  for (int e = 0; e < 8; e++) zero[e] = to_bf16(0.0f);
+  for (int k0 = wave * 32; k0 < ke; k0 += 128, k += 128) {
+    int dz = 0, dy = 0, dx = 0;
+    if (a.ks == 3) {
+      const int q9 = (t * 7282) >> 16, q3 = (t * 21846) >> 16;
+      dz = q9 - 1;
+      dy = q3 - 3 * q9 - 1;
+      dx = t - 3 * q3 - 1;
+    }
+    bf16x8 av[MT];
+#pragma unroll
+    for (int i = 0; i < MT; i++) av[i] = *(const bf16x8*)(arow[i] + k);
+#pragma unroll
+    for (int j = 0; j < NC; j++) {
+      bool in = nvalid[j] && k < ke;
+      int vox = 0;
+      if (MODE == FWD) {
+        const int z = cz[j] * a.stride + dz, y = cy[j] * a.stride + dy, x = cx[j] * a.stride + dx;
+        in = in && z >= 0 && y >= 0 && x >= 0 && z < a.D && y < a.H && x < a.W;
+        vox = (z * a.H + y) * a.W + x;
+      } else {
+        int qz = 0, qy = 0, qx = 0;
+        in = in && src_of(cz[j], dz, a.stride, a.oD, qz) && src_of(cy[j], dy, a.stride, a.oH, qy) &&
+             src_of(cx[j], dx, a.stride, a.oW, qx);
+        vox = (qz * a.oH + qy) * a.oW + qx;
+      }
+      bf16x8 b = zero;
+      if (in) b = *(const bf16x8*)(a.Xl + (size_t)vox * Cbp + c0);
+#pragma unroll
+      for (int i = 0; i < MT; i++) acc[j][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], b, acc[j][i], 0, 0, 0);
+    }
+    c0 += dc;
+    t += dt;
+    if (c0 >= Cbp) {
+      c0 -= Cbp;
+      t++;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NC; j++)
+#pragma unroll
+    for (int i = 0; i < MT; i++) red[wave][j * MT + i][lane] = acc[j][i];
+  __syncthreads();
+  // epilogue (as k_conv's): wave w finishes the (tile, row block) pairs w, w + 4, ...
+  for (int q = wave; q < NC * MT; q += 4) {
+    const int j = q / MT, i = q - (q / MT) * MT;
+    const floatx4 s = red[0][q][lane] + red[1][q][lane] + red[2][q][lane] + red[3][q][lane];
+    const int row0 = m0 + 16 * i + kg * 4;
+    const int n = nn[j];
+    if (!nvalid[j]) continue;
+    float ov[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int row = row0 + v;
+      ov[v] = 0.0f;
+      if (row >= a.M) continue;
+      const size_t idx = (size_t)row * a.N + n;
+      float o;
+      if (MODE == FWD) {
+        o = s[v] + a.bias[row];
+        if (a.relu) o = fmaxf(o, 0.0f);
+        if (a.C) a.C[idx] = o;
+      } else {
+        o = s[v] + (a.beta ? a.C[idx] : 0.0f);
+        if (a.mask && !(a.mask[idx] > 0.0f)) o = 0.0f;
+        a.C[idx] = o;
+      }
+      ov[v] = o;
+    }
+    const ClOut* outs[2] = {&a.o1, &a.o2};
+#pragma unroll
+    for (int jj = 0; jj < 2; jj++) {
+      const ClOut& o = *outs[jj];
+      if (!o.p) continue;
+      __bf16* dst = o.p + (size_t)n * o.ld + o.off + row0;
+      if (row0 + 3 < a.M && ((o.ld | o.off) & 3) == 0) {
+        bf16x4 qv;
+#pragma unroll
+        for (int v = 0; v < 4; v++) qv[v] = to_bf16(ov[v]);
+        *(bf16x4*)dst = qv;
+      } else {
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+          if (row0 + v < a.M) dst[v] = to_bf16(ov[v]);
+      }
+    }
+  }
+}
+
 // Forward and input-gradient products of a stride-1 3x3x3 convolution whose B tensor (the
 // input, or the output gradient) has a multiple of 32 (padded) channels, on a tile of 4 FT_R
 // rows x 16 voxels of one z-plane per workgroup (wave w: rows y0 + FT_R w .. + FT_R - 1, the 16
@@ -484,6 +613,17 @@ template <int MODE>
 hipError_t launch_conv(const ConvK& a, hipStream_t st)
 {
   const int mt = mt_for(a.M);
+  if constexpr (MODE != BWGT && MCPET_CONV_COLS > 1) {
+    constexpr int NC = MCPET_CONV_COLS;
+    dim3 g((a.N + 16 * NC - 1) / (16 * NC), (a.M + 16 * mt - 1) / (16 * mt), 1);
+    switch (mt) {
+      case 1: hipLaunchKernelGGL((k_conv_cols<MODE, 1, NC>), g, dim3(256), 0, st, a); break;
+      case 2: hipLaunchKernelGGL((k_conv_cols<MODE, 2, NC>), g, dim3(256), 0, st, a); break;
+      case 3: hipLaunchKernelGGL((k_conv_cols<MODE, 3, NC>), g, dim3(256), 0, st, a); break;
+      default: hipLaunchKernelGGL((k_conv_cols<MODE, 4, NC>), g, dim3(256), 0, st, a); break;
+    }
+    return hipGetLastError();
+  }
   dim3 grid((a.N + 15) / 16, (a.M + 16 * mt - 1) / (16 * mt), 1);
   ConvK b = a;
   if (MODE == BWGT) {
