@@ -1738,37 +1738,46 @@ __global__ __launch_bounds__(256) void k_rebin_rows(const int* __restrict__ sino
   }
 }
 
-// k_rebin_rows for two tallies of the same geometry in one pass (the plane-group lists read
-// once, twice the loads in flight): the same integer sums as two k_rebin_rows launches.
-__global__ __launch_bounds__(256) void k_rebin_rows2(const int* __restrict__ sa, const int* __restrict__ sb,
-                                                     const int* __restrict__ grp_off, const int* __restrict__ grp_sino,
-                                                     int nangles, int nrad, int H, int W, int gh, int gw, float* __restrict__ oa,
-                                                     float* __restrict__ ob)
+// k_rebin_rows for two tallies of the same geometry in one pass, with the group's sinogram rows
+// dealt to RB_SLICES slices of RB_LANES threads (thread r of slice s adds rows j = s, s + RB_SLICES,
+// ... of radial bin r), so five times as many rows are read at once; the slices' integer sums
+// are added in LDS in slice order. The same integer sums as two k_rebin_rows launches.
+#define RB_LANES 192
+#define RB_SLICES 5
+__global__ __launch_bounds__(RB_LANES * RB_SLICES) void k_rebin_rows2(const int* __restrict__ sa, const int* __restrict__ sb,
+                                                                      const int* __restrict__ grp_off,
+                                                                      const int* __restrict__ grp_sino, int nangles, int nrad,
+                                                                      int H, int W, int gh, int gw, float* __restrict__ oa,
+                                                                      float* __restrict__ ob)
 {
-  __shared__ unsigned long long bin[2][256];
-  const int h = blockIdx.x, d = blockIdx.y, r = threadIdx.x;
+  __shared__ unsigned long long bin[RB_SLICES][2][RB_LANES];
+  const int h = blockIdx.x, d = blockIdx.y;
+  const int r = threadIdx.x % RB_LANES, sl = threadIdx.x / RB_LANES;
   const int a0 = h * gh, rows = max(0, min(gh, nangles - a0));
   const int j0 = grp_off[d], nj = grp_off[d + 1] - j0;
   unsigned long long acc_a = 0ull, acc_b = 0ull;
-  if (r < nrad) {
+  if (r < nrad && rows > 0) {
+    // rows (j, a) of the group, a fastest, dealt round-robin to the slices; the loop is unrolled
+    // so that several rows' loads are in flight per thread
+    const int items = nj * rows;
 #pragma unroll 4
-    for (int j = 0; j < nj; j++) {
-      const size_t base = ((size_t)grp_sino[j0 + j] * nangles + a0) * nrad + r;
-      for (int a = 0; a < rows; a++) {
-        acc_a += (unsigned)sa[base + (size_t)a * nrad];
-        acc_b += (unsigned)sb[base + (size_t)a * nrad];
-      }
+    for (int it = sl; it < items; it += RB_SLICES) {
+      const int j = it / rows, a = it - j * rows;
+      const size_t off = ((size_t)grp_sino[j0 + j] * nangles + a0 + a) * nrad + r;
+      acc_a += (unsigned)sa[off];
+      acc_b += (unsigned)sb[off];
     }
   }
-  bin[0][r] = acc_a;
-  bin[1][r] = acc_b;
+  bin[sl][0][r] = acc_a;
+  bin[sl][1][r] = acc_b;
   __syncthreads();
-  if (r < W) {
+  if (sl == 0 && r < W) {
     unsigned long long ta = 0ull, tb = 0ull;
-    for (int i = r * gw; i < min(nrad, r * gw + gw); i++) {
-      ta += bin[0][i];
-      tb += bin[1][i];
-    }
+    for (int i = r * gw; i < min(nrad, r * gw + gw); i++)
+      for (int k = 0; k < RB_SLICES; k++) {
+        ta += bin[k][0][i];
+        tb += bin[k][1][i];
+      }
     oa[((size_t)d * H + h) * W + r] = (float)ta;
     ob[((size_t)d * H + h) * W + r] = (float)tb;
   }
@@ -1779,11 +1788,12 @@ __global__ __launch_bounds__(256) void k_rebin_rows2(const int* __restrict__ sa,
 hipError_t mcpet_launch_rebin2(const int* sa, const int* sb, const int* grp_off, const int* grp_sino, int nangles, int nrad, int D,
                                int H, int W, int gh, int gw, float* oa, float* ob, hipStream_t st)
 {
-  if (W > 256 || nrad > 256) {
+  if (W > RB_LANES || nrad > RB_LANES) {
     hipError_t e = mcpet_launch_rebin(sa, grp_off, grp_sino, nangles, nrad, D, H, W, gh, gw, oa, st);
     return e == hipSuccess ? mcpet_launch_rebin(sb, grp_off, grp_sino, nangles, nrad, D, H, W, gh, gw, ob, st) : e;
   }
-  hipLaunchKernelGGL(k_rebin_rows2, dim3(H, D), dim3(256), 0, st, sa, sb, grp_off, grp_sino, nangles, nrad, H, W, gh, gw, oa, ob);
+  hipLaunchKernelGGL(k_rebin_rows2, dim3(H, D), dim3(RB_LANES * RB_SLICES), 0, st, sa, sb, grp_off, grp_sino, nangles, nrad, H,
+                     W, gh, gw, oa, ob);
   return hipGetLastError();
 }
 
