@@ -28,6 +28,7 @@ struct mcpet_ctx {
   int num_cus = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev_k0 = nullptr, ev_k1 = nullptr, ev_z0 = nullptr;
+  hipEvent_t ev_done = nullptr;   // recorded behind every launch that runs this context (any stream)
   mcpet_config cfg{};
   mcpet_detector det{};
   int nx = 0, ny = 0, nz = 0;
@@ -156,6 +157,7 @@ void free_all(mcpet_ctx* c)
   if (c->h_params) (void)hipHostFree(c->h_params);
   if (c->ev_k0) (void)hipEventDestroy(c->ev_k0);
   if (c->ev_k1) (void)hipEventDestroy(c->ev_k1);
+  if (c->ev_done) (void)hipEventDestroy(c->ev_done);
   if (c->ev_z0) (void)hipEventDestroy(c->ev_z0);
   if (c->stream) (void)hipStreamDestroy(c->stream);
 }
@@ -354,6 +356,7 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
   HK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HK(hipEventCreate(&c->ev_k0));
   HK(hipEventCreate(&c->ev_k1));
+  HK(hipEventCreateWithFlags(&c->ev_done, hipEventDisableTiming));
   HK(hipEventCreate(&c->ev_z0));
 
   // Detector: negative radius -> centre on the bounding box (C:316-322).
@@ -780,7 +783,10 @@ extern "C" mcpet_status mcpet_run(mcpet_ctx* c, const mcpet_run_params* rp)
   HIPCHK(c, hipEventRecord(c->ev_k1, c->stream));
   c->ran = true;
   c->timed = true;
-  if (!c->singles_mode) return MCPET_OK;
+  if (!c->singles_mode) {
+    HIPCHK(c, hipEventRecord(c->ev_done, c->stream));
+    return MCPET_OK;
+  }
   // Singles pipeline: the sort needs the number of singles, so wait for the transport.
   HIPCHK(c, hipMemcpyAsync(&c->singles_req, c->d_singles_n, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -795,6 +801,7 @@ extern "C" mcpet_status mcpet_run(mcpet_ctx* c, const mcpet_run_params* rp)
   if (e == hipSuccess) e = mcpet_launch_pairing(P, c->d_keys[c->sorted], c->d_vals[c->sorted], c->n_singles, c->stream);
   if (e != hipSuccess) return set_err(c, MCPET_E_HIP, std::string("coincidence sorting failed: ") + hipGetErrorString(e));
   HIPCHK(c, hipEventRecord(c->ev_c1, c->stream));
+  HIPCHK(c, hipEventRecord(c->ev_done, c->stream));
   return MCPET_OK;
 }
 
@@ -1173,9 +1180,8 @@ mcpet_status mcpet_ctx_rebin2_device(mcpet_ctx* c, int wa, int wb, int D, int H,
   if (!sa || !sb) return set_err(c, MCPET_E_STATE, "rebin: this tally is not kept in this mode");
   mcpet_status e = rebin_setup(c, D, H, W, false);
   if (e != MCPET_OK) return e;
-  // after the last mcpet_run's transport on the device (no host wait); a context run by the
-  // serving loop's batched launches has completed before its estimate is taken
-  HIPCHK(c, hipStreamWaitEvent(st, c->ev_k1, 0));
+  // after the context's last run on the device, whichever launch ran it (no host wait)
+  HIPCHK(c, hipStreamWaitEvent(st, c->ev_done, 0));
   const mcpet_config& cfg = c->cfg;
   const int gh = (cfg.nangles + H - 1) / H, gw = (cfg.nrad + W - 1) / W;
   hipError_t he = mcpet_launch_rebin2(sa, sb, c->d_rb_off, c->d_rb_sino, cfg.nangles, cfg.nrad, D, H, W, gh, gw, d_a, d_b, st);
@@ -1515,6 +1521,7 @@ mcpet_status mcpet_run_contexts(const CtxRun* runs, int n, hipStream_t stream, K
   }
   hipError_t e = hipMemcpyAsync(d_params, h_params, sizeof(KParams) * (size_t)n, hipMemcpyHostToDevice, stream);
   if (e == hipSuccess) e = mcpet_launch_transport_batch(d_params, n, ext, 0, blocks, block_size, stream);
+  for (int k = 0; k < n && e == hipSuccess; k++) e = hipEventRecord(runs[k].c->ev_done, stream);
   if (e != hipSuccess) return fail(MCPET_E_HIP, std::string("batched launch: ") + hipGetErrorString(e));
   return MCPET_OK;
 }
@@ -1674,7 +1681,10 @@ static mcpet_status batch_run(mcpet_batch* b, const mcpet_run_params* rp, double
     if (e != hipSuccess) return bset_err(b, MCPET_E_HIP, std::string("batch launch failed: ") + hipGetErrorString(e));
     BHIP(b, hipEventRecord(b->ev1, b->stream));
   }
-  for (mcpet_ctx* c : b->ctx) c->ran = true;
+  for (mcpet_ctx* c : b->ctx) {
+    BHIP(b, hipEventRecord(c->ev_done, b->stream));
+    c->ran = true;
+  }
   b->ran = true;
   return MCPET_OK;
 }
