@@ -531,7 +531,9 @@ mcpet_status mcpet_surrogate_forward_ms(mcpet_surrogate* s, double* ms);
  * with 4 channels also the activity integrals times float(1 / their maximum) and the
  * attenuation factors (mcpet_project_maps), the network, and estimate = max(output, 0) times
  * float(total trues / (voxels * decays)): scatter per decay (D*H*W floats, host).
- * scatter_counts (may be NULL): the rebinned scatter; decays (may be NULL): the run's decays. */
+ * scatter_counts (may be NULL): the rebinned scatter; decays (may be NULL): the run's decays.
+ * Call it right after mcpet_run: its device work is ordered behind the run on the device, and
+ * the host waits once, for the outputs. */
 mcpet_status mcpet_surrogate_estimate(mcpet_surrogate* s, mcpet_ctx* ctx, float* estimate, float* scatter_counts,
                                       int64_t* decays);
 /* loss = mean over batch and voxels of (y - target)^2; grad (host, param_count floats; may be NULL). */
