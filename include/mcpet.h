@@ -493,6 +493,13 @@ mcpet_status mcpet_sinogram_planes(const mcpet_config* cfg, int32_t* plane_of_si
 /* Rebin tally `which` (0 trues, 1 scatter, 2 randoms, 3 spurious) of the last run on the device
  * into the D x H x W grid (out: host, D*H*W floats of counts). Waits for the run. */
 mcpet_status mcpet_rebin_sinogram(mcpet_ctx* c, int32_t which, int32_t D, int32_t H, int32_t W, float* out);
+/* From the next run on, every detected trues or scatter coincidence is also added to its bin
+ * of the D x H x W rebinning grid (the grid of mcpet_rebin_sinogram), so that the fused
+ * estimate (mcpet_surrogate_estimate) on that grid reads these counts instead of rebinning the
+ * sinograms after the run: the same integer sums. The sinograms are tallied as before.
+ * D = H = W = 0 turns it off. Not in the singles mode (MCPET_E_STATE). No reference
+ * counterpart: an extension for the scatter surrogate's input runs. */
+mcpet_status mcpet_tally_rebinned(mcpet_ctx* c, int32_t D, int32_t H, int32_t W);
 mcpet_status mcpet_batch_rebin_sinogram(mcpet_batch* b, int32_t k, int32_t which, int32_t D, int32_t H, int32_t W, float* out);
 
 /* The phantom seen along the lines of response of the D x H x W grid (1 <= D <= 256): for every

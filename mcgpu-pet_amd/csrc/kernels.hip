@@ -496,6 +496,13 @@ K_RARE void tally_sinogram(const KParams& P, float energy, float energyFirst, co
       gatomic_add(&P.img_sc[st.absvox], 1);
       C.inc(KC_COINC);
     }
+    if (P.rb_trues) {   // the same event on the rebinning grid (sinogram, angle, radial bin of ibin)
+      const int per = P.nangles * P.nrad;
+      const int izm = ibin / per, rem = ibin - izm * per;
+      const int ith = rem / P.nrad, ir = rem - ith * P.nrad;
+      const int cb = (P.rb_plane[izm] * P.rb_H + ith / P.rb_gh) * P.rb_W + ir / P.rb_gw;
+      gatomic_add((ss == 0 && ssf == 0) ? &P.rb_trues[cb] : &P.rb_scatter[cb], 1);
+    }
   }
 }
 

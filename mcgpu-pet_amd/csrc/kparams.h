@@ -89,6 +89,12 @@ struct KParams {
   // tallies
   int* trues;
   int* scatter;
+  // rebinned tallies (null: off): trues and scatter also added on the D x H x W grid of the
+  // sinogram rebinning, bin (rb_plane[sinogram] H + angle / rb_gh) W + radial / rb_gw
+  int* rb_trues;
+  int* rb_scatter;
+  const int* rb_plane;
+  int rb_H, rb_W, rb_gh, rb_gw;
   int* img_t;
   int* img_sc;
   int* spectrum;
@@ -156,6 +162,8 @@ hipError_t mcpet_launch_gather_singles(const mcpet_single* s, const unsigned* pe
 // in CSR form grp_off[D + 1] / grp_sino, angle groups of gh rows, radial groups of gw bins).
 hipError_t mcpet_launch_rebin(const int* sino, const int* grp_off, const int* grp_sino, int nangles, int nrad, int D, int H, int W,
                               int gh, int gw, float* out, hipStream_t st);
+// oa[i] = float(a[i]), ob[i] = float(b[i]) for i < n (rebinned tallies to the network's input)
+hipError_t mcpet_launch_i2f2(const int* a, const int* b, float* oa, float* ob, size_t n, hipStream_t st);
 // the same for two tallies (sa -> oa, sb -> ob) in one launch
 hipError_t mcpet_launch_rebin2(const int* sa, const int* sb, const int* grp_off, const int* grp_sino, int nangles, int nrad, int D,
                                int H, int W, int gh, int gw, float* oa, float* ob, hipStream_t st);

@@ -46,6 +46,10 @@ struct mcpet_ctx {
   long long* d_stream_start = nullptr;   // balanced stream layout (stream_decays > 0)
   long long* d_stream_id = nullptr;
   int* d_stream_hint = nullptr;
+  int* d_rbt = nullptr;                  // rebinned trues and scatter (mcpet_tally_rebinned), 2 x D*H*W
+  int* d_rbt_plane = nullptr;            // their plane group of every sinogram
+  int rbt_dims[3] = {0, 0, 0};
+  bool rbt_ran = false;                  // the last run added to them
   size_t hint_cap = 0;                   // entries allocated for d_stream_hint
   float2* d_woodcock = nullptr;
   float2* d_tot = nullptr;
@@ -143,7 +147,7 @@ mcpet_status dalloc(mcpet_ctx* c, T** p, size_t n)
 
 void free_all(mcpet_ctx* c)
 {
-  void* ptrs[] = {c->d_vox, c->d_vox8, c->d_vox16, c->d_palette, c->d_active_vox, c->d_active_act, c->d_stream_start, c->d_stream_id, c->d_stream_hint, c->d_woodcock, c->d_tot, c->d_cr, c->d_ray, c->d_cmp,
+  void* ptrs[] = {c->d_vox, c->d_vox8, c->d_vox16, c->d_palette, c->d_active_vox, c->d_active_act, c->d_stream_start, c->d_stream_id, c->d_stream_hint, c->d_rbt, c->d_rbt_plane, c->d_woodcock, c->d_tot, c->d_cr, c->d_ray, c->d_cmp,
                   c->d_trues, c->d_scatter, c->d_img_t, c->d_img_sc, c->d_spectrum, c->d_psf, c->d_psf_index,
                   c->d_voxel_edep, c->d_mat_edep, c->d_counters, c->d_final_seed, c->d_singles, c->d_singles_n,
                   c->d_keys[0], c->d_keys[1], c->d_vals[0], c->d_vals[1], c->d_hist, c->d_randoms, c->d_spurious,
@@ -698,6 +702,7 @@ static mcpet_status clear_buffers(mcpet_ctx* c, hipStream_t stream, bool fill, b
     r.push_back({c->d_spectrum, sizeof(int) * (size_t)std::max(1, c->cfg.ne)});
     r.push_back({c->d_psf_index, sizeof(int)});
     if (c->d_voxel_edep) r.push_back({c->d_voxel_edep, 16 * (size_t)c->n_roi});
+    if (c->d_rbt) r.push_back({c->d_rbt, 2 * sizeof(int) * (size_t)c->rbt_dims[0] * c->rbt_dims[1] * c->rbt_dims[2]});
     r.push_back({c->d_mat_edep, 16 * MCPET_MAX_MATERIALS});
     r.push_back({c->d_final_seed, sizeof(int)});
     if (c->singles_mode) {
@@ -774,6 +779,7 @@ extern "C" mcpet_status mcpet_run(mcpet_ctx* c, const mcpet_run_params* rp)
     blocks = std::max(1LL, (P.n_streams + bs - 1) / bs);
     if (blocks > 0x7fffffffLL) return set_err(c, MCPET_E_RANGE, "mcpet_run: too many streams for one launch");
   }
+  c->rbt_ran = P.rb_trues != nullptr;
   if (P.n_streams > 0) {
     *c->h_params = P;
     HIPCHK(c, hipMemcpyAsync(c->d_params, c->h_params, sizeof(KParams), hipMemcpyHostToDevice, c->stream));
@@ -1178,6 +1184,19 @@ mcpet_status mcpet_ctx_rebin2_device(mcpet_ctx* c, int wa, int wb, int D, int H,
     return set_err(c, MCPET_E_ARG, "rebin: which must be 0 (trues), 1 (scatter), 2 (randoms), 3 (spurious)");
   const int *sa = rebin_source(c, wa), *sb = rebin_source(c, wb);
   if (!sa || !sb) return set_err(c, MCPET_E_STATE, "rebin: this tally is not kept in this mode");
+  const bool pair = (wa == 0 && wb == 1) || (wa == 1 && wb == 0);
+  if (pair && c->rbt_ran && c->rbt_dims[0] == D && c->rbt_dims[1] == H && c->rbt_dims[2] == W) {
+    // the run tallied them on this grid: the same integer sums, no sinogram read
+    if (!c->ran) return set_err(c, MCPET_E_STATE, "rebin: no run");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamWaitEvent(st, c->ev_done, 0));
+    const size_t n = (size_t)D * H * W;
+    const int* ra = wa == 0 ? c->d_rbt : c->d_rbt + n;
+    const int* rb = wb == 0 ? c->d_rbt : c->d_rbt + n;
+    hipError_t he = mcpet_launch_i2f2(ra, rb, d_a, d_b, n, st);
+    if (he != hipSuccess) return set_err(c, MCPET_E_HIP, std::string("rebinned tallies: ") + hipGetErrorString(he));
+    return MCPET_OK;
+  }
   mcpet_status e = rebin_setup(c, D, H, W, false);
   if (e != MCPET_OK) return e;
   // after the context's last run on the device, whichever launch ran it (no host wait)
@@ -1338,6 +1357,58 @@ extern "C" mcpet_status mcpet_project_maps(mcpet_ctx* c, int32_t D, int32_t H, i
 {
   if (!c) return set_err(c, MCPET_E_ARG, "mcpet_project_maps: null context");
   return project_maps(c, D, H, W, act, att);
+}
+
+extern "C" mcpet_status mcpet_tally_rebinned(mcpet_ctx* c, int32_t D, int32_t H, int32_t W)
+{
+  if (!c) return set_err(nullptr, MCPET_E_ARG, "mcpet_tally_rebinned: null context");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  KParams& P = c->P;
+  if (D == 0 && H == 0 && W == 0) {
+    P.rb_trues = P.rb_scatter = nullptr;
+    P.rb_plane = nullptr;
+    return MCPET_OK;
+  }
+  const mcpet_config& cfg = c->cfg;
+  if (D < 1 || H < 1 || W < 1 || (long long)D * H * W > (1LL << 28))
+    return set_err(c, MCPET_E_ARG, "mcpet_tally_rebinned: bad grid");
+  if (c->singles_mode)
+    return set_err(c, MCPET_E_STATE, "mcpet_tally_rebinned: the singles pipeline bins its coincidences after the transport");
+  if (c->rbt_dims[0] != D || c->rbt_dims[1] != H || c->rbt_dims[2] != W) {
+    // plane group of every sinogram as the rebinning's: plane / ceil(NZS / D)
+    std::vector<int> plane((size_t)cfg.nsinos);
+    if (mcpet_sinogram_planes(&cfg, plane.data()) != MCPET_OK) return set_err(c, MCPET_E_ARG, mcpet_host_last_error());
+    const int gz = (cfg.nzs + D - 1) / D;
+    for (int& v : plane) v /= gz;
+    for (void* p : {(void*)c->d_rbt, (void*)c->d_rbt_plane})
+      if (p) (void)hipFree(p);
+    if (c->d_rbt) c->dev_bytes -= 2 * sizeof(int) * (size_t)c->rbt_dims[0] * c->rbt_dims[1] * c->rbt_dims[2];
+    if (c->d_rbt_plane) c->dev_bytes -= sizeof(int) * (size_t)cfg.nsinos;
+    c->d_rbt = c->d_rbt_plane = nullptr;
+    c->rbt_dims[0] = c->rbt_dims[1] = c->rbt_dims[2] = 0;
+    P.rb_trues = P.rb_scatter = nullptr;
+    P.rb_plane = nullptr;
+    const size_t n = (size_t)D * H * W;
+    mcpet_status e = dalloc(c, &c->d_rbt, 2 * n);
+    if (e == MCPET_OK) e = dalloc(c, &c->d_rbt_plane, plane.size());
+    if (e != MCPET_OK) return e;
+    HIPCHK(c, hipMemcpy(c->d_rbt_plane, plane.data(), sizeof(int) * plane.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(c->d_rbt, 0, 2 * sizeof(int) * n));
+    c->rbt_dims[0] = D;
+    c->rbt_dims[1] = H;
+    c->rbt_dims[2] = W;
+  }
+  const size_t n = (size_t)D * H * W;
+  P.rb_trues = c->d_rbt;
+  P.rb_scatter = c->d_rbt + n;
+  P.rb_plane = c->d_rbt_plane;
+  P.rb_H = H;
+  P.rb_W = W;
+  P.rb_gh = (cfg.nangles + H - 1) / H;
+  P.rb_gw = (cfg.nrad + W - 1) / W;
+  c->rbt_ran = false;   // the next run fills them
+  return MCPET_OK;
 }
 
 extern "C" mcpet_status mcpet_rebin_sinogram(mcpet_ctx* c, int32_t which, int32_t D, int32_t H, int32_t W, float* out)
@@ -1513,6 +1584,7 @@ mcpet_status mcpet_run_contexts(const CtxRun* runs, int n, hipStream_t stream, K
     P.lds_palette = 0;
     ext = ext || P.prompt_frac > 0.0f;
     h_params[k] = P;
+    c->rbt_ran = P.rb_trues != nullptr;
     mcpet_status st = clear_buffers(c, stream, false, runs[k].clear != 0, runs[k].clear != 0);
     if (st != MCPET_OK) return fail(st, c->err);
     hipError_t e = hipMemsetAsync(c->d_counters + KC_N, 0, sizeof(unsigned long long), stream);   // stream queue
@@ -1684,6 +1756,7 @@ static mcpet_status batch_run(mcpet_batch* b, const mcpet_run_params* rp, double
   for (mcpet_ctx* c : b->ctx) {
     BHIP(b, hipEventRecord(c->ev_done, b->stream));
     c->ran = true;
+    c->rbt_ran = c->P.rb_trues != nullptr;
   }
   b->ran = true;
   return MCPET_OK;

@@ -876,7 +876,9 @@ __global__ __launch_bounds__(256) void k_bwgt_reduce(const float* __restrict__ p
 
 // Workgroups of the tiled weight gradient: about BW_WGS in total, the output-voxel tiles dealt
 // to slabs of equal length; the partial sums then fit in BW_WGS x 27 x 1024 floats.
+#ifndef BW_WGS
 #define BW_WGS 512
+#endif
 inline void bwgt_shape(int oD, int oH, int oW, int Cin, int Cout, BwK& b, int& n_slabs)
 {
   b.tiles_x = (oW + BW_X - 1) / BW_X;
@@ -1783,7 +1785,22 @@ __global__ __launch_bounds__(RB_LANES * RB_SLICES) void k_rebin_rows2(const int*
   }
 }
 
+__global__ void k_i2f2(const int* __restrict__ a, const int* __restrict__ b, float* __restrict__ oa, float* __restrict__ ob,
+                       size_t n)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  oa[i] = (float)a[i];
+  ob[i] = (float)b[i];
+}
+
 }  // namespace
+
+hipError_t mcpet_launch_i2f2(const int* a, const int* b, float* oa, float* ob, size_t n, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_i2f2, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, b, oa, ob, n);
+  return hipGetLastError();
+}
 
 hipError_t mcpet_launch_rebin2(const int* sa, const int* sb, const int* grp_off, const int* grp_sino, int nangles, int nrad, int D,
                                int H, int W, int gh, int gw, float* oa, float* ob, hipStream_t st)

@@ -229,6 +229,7 @@ def lib():
     L.mcpet_struct_sizes.restype = C.c_int32
     L.mcpet_sinogram_planes.argtypes = [P(Config), P(C.c_int32)]
     L.mcpet_rebin_sinogram.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P(C.c_float)]
+    L.mcpet_tally_rebinned.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]
     L.mcpet_batch_rebin_sinogram.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                              P(C.c_float)]
     L.mcpet_project_maps.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, P(C.c_float), P(C.c_float)]
@@ -450,6 +451,12 @@ class Simulation:
     def fetch(self, sinograms=True, images=True, spectrum=True, psf=True, dose=True, singles=True):
         return _fetch_tallies(lambda t: _check(lib().mcpet_fetch(self.ctx, C.byref(t)), self.ctx), self.cfg, self.phantom.nvox,
                               sinograms, images, spectrum, psf, dose, singles)
+
+    def tally_rebinned(self, grid):
+        """From the next run on, also tally trues and scatter directly on the (D, H, W) rebinning
+        grid (mcpet_tally_rebinned; the fused estimate then reads them); None turns it off."""
+        D, H, W = grid if grid is not None else (0, 0, 0)
+        _check(lib().mcpet_tally_rebinned(self.ctx, D, H, W), self.ctx)
 
     def rebin(self, which, grid):
         """Tally `which` ("trues", "scatter", ...) summed on the device into a (D, H, W) grid."""
