@@ -24,6 +24,7 @@
 //     between the two photons, the Compton sampler between rejection trials) lives in
 //     LDS slots, so that registers hold only the stream and the current photon.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <climits>
 
 #include "devutil.h"
@@ -1542,6 +1543,28 @@ __global__ void k_fill_u32(unsigned* __restrict__ p, size_t n)
     if (base + (size_t)k * 256 < n) p[base + (size_t)k * 256] = 0u;
 }
 
+// Zero up to K_FILL_MAX regions of 4-byte words in one launch: block row y clears region y,
+// its blocks grid-striding over the region's words (16-byte stores where the region allows).
+struct FillRegions {
+  unsigned* p[K_FILL_MAX];
+  unsigned long long n[K_FILL_MAX];   // words
+};
+__global__ __launch_bounds__(256) void k_fill_regions(FillRegions r)
+{
+  unsigned* p = r.p[blockIdx.y];
+  const size_t n = r.n[blockIdx.y];
+  const size_t stride = (size_t)gridDim.x * 256;
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (((uintptr_t)p & 15) == 0) {
+    uint4* q = (uint4*)p;
+    const size_t n4 = n / 4;
+    for (size_t k = i; k < n4; k += stride) q[k] = uint4{0u, 0u, 0u, 0u};
+    for (size_t k = n4 * 4 + i; k < n; k += stride) p[k] = 0u;
+  } else {
+    for (size_t k = i; k < n; k += stride) p[k] = 0u;
+  }
+}
+
 __global__ void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16)
 {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1744,6 +1767,26 @@ hipError_t mcpet_launch_scale_by_max(float* a, int n, unsigned* mx, hipStream_t 
   hipLaunchKernelGGL(k_max_bits, dim3(nb), dim3(256), 0, stream, a, n, mx);
   hipLaunchKernelGGL(k_scale_by, dim3(nb), dim3(256), 0, stream, a, n, mx);
   return hipGetLastError();
+}
+
+hipError_t mcpet_launch_fill_regions(void* const* p, const size_t* n_words, int n, hipStream_t stream)
+{
+  for (int i0 = 0; i0 < n; i0 += K_FILL_MAX) {
+    FillRegions r{};
+    const int m = std::min(K_FILL_MAX, n - i0);
+    size_t most = 0;
+    for (int k = 0; k < m; k++) {
+      r.p[k] = (unsigned*)p[i0 + k];
+      r.n[k] = n_words[i0 + k];
+      most = std::max(most, n_words[i0 + k]);
+    }
+    if (most == 0) continue;
+    const unsigned bx = (unsigned)std::min<size_t>(2048, (most / 4 + 255) / 256 + 1);
+    hipLaunchKernelGGL(k_fill_regions, dim3(bx, m), dim3(256), 0, stream, r);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t mcpet_launch_fill_u32(void* p, size_t n_words, hipStream_t stream)

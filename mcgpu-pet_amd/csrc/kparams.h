@@ -133,6 +133,9 @@ hipError_t mcpet_launch_transport(const KParams& P, const KParams* d_P, int kind
 hipError_t mcpet_launch_transport_batch(const KParams* d_params, int n, bool ext, int count_events, int blocks, int block_size,
                                         hipStream_t stream);
 hipError_t mcpet_launch_fill_u32(void* p, size_t n_words, hipStream_t stream);
+// zero n regions of n_words[i] 4-byte words at p[i], K_FILL_MAX regions per launch
+#define K_FILL_MAX 16
+hipError_t mcpet_launch_fill_regions(void* const* p, const size_t* n_words, int n, hipStream_t stream);
 // Activity and attenuation line integrals on a rebinned sinogram grid (scatter surrogate
 // inputs): `map` is an nvox (activity, mu) scratch map; lors/off the crystal pairs of each
 // (angle, radial) group in CSR form; D <= 256.

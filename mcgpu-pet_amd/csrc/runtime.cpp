@@ -685,8 +685,9 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
   return MCPET_OK;
 }
 
-// Clear the tallies (and with counters, the event counters and stream queue) on `stream`,
-// with memsets or, for a captured graph, with a fill kernel.
+// Clear the tallies (and with counters, the event counters and stream queue) on `stream` with
+// one fill kernel for all of them (one launch instead of a memset per buffer, and no memset
+// nodes when a graph captures it; `fill` is kept for the callers that capture).
 static mcpet_status clear_buffers(mcpet_ctx* c, hipStream_t stream, bool fill, bool tallies, bool counters)
 {
   struct Region {
@@ -711,18 +712,18 @@ static mcpet_status clear_buffers(mcpet_ctx* c, hipStream_t stream, bool fill, b
     }
   }
   if (counters) r.push_back({c->d_counters, sizeof(unsigned long long) * (KC_N + 1)});
-  for (const Region& x : r) {
-    if (fill) {
-      hipError_t e = mcpet_launch_fill_u32(x.p, x.bytes / 4, stream);
-      if (e != hipSuccess) return set_err(c, MCPET_E_HIP, std::string("clearing tallies: ") + hipGetErrorString(e));
-    } else {
-      HIPCHK(c, hipMemsetAsync(x.p, 0, x.bytes, stream));
+  (void)fill;   // one fill kernel for every region (no memset nodes when a graph captures it)
+  std::vector<void*> ptrs;
+  std::vector<size_t> words;
+  for (const Region& x : r)
+    if (x.p && x.bytes) {
+      ptrs.push_back(x.p);
+      words.push_back(x.bytes / 4);
     }
-  }
+  const hipError_t e = mcpet_launch_fill_regions(ptrs.data(), words.data(), (int)ptrs.size(), stream);
+  if (e != hipSuccess) return set_err(c, MCPET_E_HIP, std::string("clearing tallies: ") + hipGetErrorString(e));
   return MCPET_OK;
 }
-
-static mcpet_status zero_tallies(mcpet_ctx* c) { return clear_buffers(c, c->stream, false, true, false); }
 
 extern "C" mcpet_status mcpet_run(mcpet_ctx* c, const mcpet_run_params* rp)
 {
@@ -759,11 +760,12 @@ extern "C" mcpet_status mcpet_run(mcpet_ctx* c, const mcpet_run_params* rp)
   int bs = p.block_size > 0 ? p.block_size : 256;
   if (bs % 64 != 0 || bs > 256) return set_err(c, MCPET_E_RANGE, "mcpet_run: block_size must be a multiple of 64, <= 256");
   HIPCHK(c, hipEventRecord(c->ev_z0, c->stream));
-  if (p.zero_tallies) {
-    mcpet_status st = zero_tallies(c);
+  if (p.zero_tallies) {   // tallies and counters in one fill launch
+    mcpet_status st = clear_buffers(c, c->stream, false, true, true);
     if (st != MCPET_OK) return st;
+  } else {
+    HIPCHK(c, hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * (KC_N + 1), c->stream));
   }
-  HIPCHK(c, hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * (KC_N + 1), c->stream));
   if (c->singles_mode) {
     HIPCHK(c, hipMemsetAsync(c->d_singles_n, 0, sizeof(unsigned long long), c->stream));
     HIPCHK(c, hipMemsetAsync(c->d_coinc, 0, 5 * sizeof(unsigned long long), c->stream));
