@@ -687,8 +687,8 @@ extern "C" mcpet_status mcpet_create(const mcpet_config* cfg, const mcpet_phanto
 
 // Clear the tallies (and with counters, the event counters and stream queue) on `stream` with
 // one fill kernel for all of them (one launch instead of a memset per buffer, and no memset
-// nodes when a graph captures it; `fill` is kept for the callers that capture).
-static mcpet_status clear_buffers(mcpet_ctx* c, hipStream_t stream, bool fill, bool tallies, bool counters)
+// nodes when a graph captures it).
+static mcpet_status clear_buffers(mcpet_ctx* c, hipStream_t stream, bool tallies, bool counters)
 {
   struct Region {
     void* p;
@@ -712,7 +712,6 @@ static mcpet_status clear_buffers(mcpet_ctx* c, hipStream_t stream, bool fill, b
     }
   }
   if (counters) r.push_back({c->d_counters, sizeof(unsigned long long) * (KC_N + 1)});
-  (void)fill;   // one fill kernel for every region (no memset nodes when a graph captures it)
   std::vector<void*> ptrs;
   std::vector<size_t> words;
   for (const Region& x : r)
@@ -761,7 +760,7 @@ extern "C" mcpet_status mcpet_run(mcpet_ctx* c, const mcpet_run_params* rp)
   if (bs % 64 != 0 || bs > 256) return set_err(c, MCPET_E_RANGE, "mcpet_run: block_size must be a multiple of 64, <= 256");
   HIPCHK(c, hipEventRecord(c->ev_z0, c->stream));
   if (p.zero_tallies) {   // tallies and counters in one fill launch
-    mcpet_status st = clear_buffers(c, c->stream, false, true, true);
+    mcpet_status st = clear_buffers(c, c->stream, true, true);
     if (st != MCPET_OK) return st;
   } else {
     HIPCHK(c, hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * (KC_N + 1), c->stream));
@@ -1587,7 +1586,7 @@ mcpet_status mcpet_run_contexts(const CtxRun* runs, int n, hipStream_t stream, K
     ext = ext || P.prompt_frac > 0.0f;
     h_params[k] = P;
     c->rbt_ran = P.rb_trues != nullptr;
-    mcpet_status st = clear_buffers(c, stream, false, runs[k].clear != 0, runs[k].clear != 0);
+    mcpet_status st = clear_buffers(c, stream, runs[k].clear != 0, runs[k].clear != 0);
     if (st != MCPET_OK) return fail(st, c->err);
     hipError_t e = hipMemsetAsync(c->d_counters + KC_N, 0, sizeof(unsigned long long), stream);   // stream queue
     if (e != hipSuccess) return fail(MCPET_E_HIP, std::string("queue reset: ") + hipGetErrorString(e));
@@ -1730,11 +1729,12 @@ static mcpet_status batch_run(mcpet_batch* b, const mcpet_run_params* rp, double
       b->graph = nullptr;
       BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
       mcpet_status st = MCPET_OK;
-      for (int k = 0; k < n && st == MCPET_OK; k++) st = clear_buffers(b->ctx[k], b->stream, true, p.zero_tallies != 0, true);
+      for (int k = 0; k < n && st == MCPET_OK; k++) st = clear_buffers(b->ctx[k], b->stream, p.zero_tallies != 0, true);
       hipError_t e = hipSuccess;
       if (st == MCPET_OK) e = mcpet_launch_transport_batch(b->d_params, n, ext, p.count_events, blocks, bs, b->stream);
       hipGraph_t g = nullptr;
       hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+      if ((st != MCPET_OK || e != hipSuccess || e2 != hipSuccess) && g) (void)hipGraphDestroy(g);
       if (st != MCPET_OK) return st;
       if (e != hipSuccess || e2 != hipSuccess)
         return bset_err(b, MCPET_E_HIP, std::string("mcpet_batch_run: graph capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
@@ -1747,7 +1747,7 @@ static mcpet_status batch_run(mcpet_batch* b, const mcpet_run_params* rp, double
     BHIP(b, hipEventRecord(b->ev1, b->stream));
   } else {
     for (int k = 0; k < n; k++) {
-      mcpet_status st = clear_buffers(b->ctx[k], b->stream, false, p.zero_tallies != 0, true);
+      mcpet_status st = clear_buffers(b->ctx[k], b->stream, p.zero_tallies != 0, true);
       if (st != MCPET_OK) return st;
     }
     BHIP(b, hipEventRecord(b->ev0, b->stream));
