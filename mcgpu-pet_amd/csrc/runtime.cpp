@@ -780,7 +780,8 @@ extern "C" mcpet_status mcpet_run(mcpet_ctx* c, const mcpet_run_params* rp)
     blocks = std::max(1LL, (P.n_streams + bs - 1) / bs);
     if (blocks > 0x7fffffffLL) return set_err(c, MCPET_E_RANGE, "mcpet_run: too many streams for one launch");
   }
-  c->rbt_ran = P.rb_trues != nullptr;
+  // the rebinned tallies hold the sinograms' counts only if every run since the last clear added to them
+  c->rbt_ran = P.rb_trues != nullptr && (p.zero_tallies || c->rbt_ran);
   if (P.n_streams > 0) {
     *c->h_params = P;
     HIPCHK(c, hipMemcpyAsync(c->d_params, c->h_params, sizeof(KParams), hipMemcpyHostToDevice, c->stream));
@@ -1585,7 +1586,7 @@ mcpet_status mcpet_run_contexts(const CtxRun* runs, int n, hipStream_t stream, K
     P.lds_palette = 0;
     ext = ext || P.prompt_frac > 0.0f;
     h_params[k] = P;
-    c->rbt_ran = P.rb_trues != nullptr;
+    c->rbt_ran = P.rb_trues != nullptr && (runs[k].clear != 0 || c->rbt_ran);
     mcpet_status st = clear_buffers(c, stream, runs[k].clear != 0, runs[k].clear != 0);
     if (st != MCPET_OK) return fail(st, c->err);
     hipError_t e = hipMemsetAsync(c->d_counters + KC_N, 0, sizeof(unsigned long long), stream);   // stream queue
@@ -1758,7 +1759,7 @@ static mcpet_status batch_run(mcpet_batch* b, const mcpet_run_params* rp, double
   for (mcpet_ctx* c : b->ctx) {
     BHIP(b, hipEventRecord(c->ev_done, b->stream));
     c->ran = true;
-    c->rbt_ran = c->P.rb_trues != nullptr;
+    c->rbt_ran = c->P.rb_trues != nullptr && (p.zero_tallies || c->rbt_ran);
   }
   b->ran = true;
   return MCPET_OK;

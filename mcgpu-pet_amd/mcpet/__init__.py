@@ -401,6 +401,7 @@ class Simulation:
         self.cfg = cfg
         self.phantom = phantom
         self.tables = tables
+        self.tally_grid = None   # the grid of mcpet_tally_rebinned, None while off
         self.ctx = C.c_void_p()
         st = lib().mcpet_create(C.byref(cfg), C.byref(phantom.ph), C.byref(tables.t), C.byref(self.ctx))
         if st != 0:
@@ -457,6 +458,7 @@ class Simulation:
         grid (mcpet_tally_rebinned; the fused estimate then reads them); None turns it off."""
         D, H, W = grid if grid is not None else (0, 0, 0)
         _check(lib().mcpet_tally_rebinned(self.ctx, D, H, W), self.ctx)
+        self.tally_grid = tuple(grid) if grid is not None else None
 
     def rebin(self, which, grid):
         """Tally `which` ("trues", "scatter", ...) summed on the device into a (D, H, W) grid."""

@@ -173,9 +173,8 @@ class ScatterEstimator:
         normalisation, network). Returns (estimate, info); info["consistent"] False means: use
         Monte Carlo. Equal to predict() on the rebinned tallies and maps, bit for bit."""
         t0 = time.perf_counter()
-        if getattr(sim, "_tally_grid", None) != self.grid:   # the run tallies on the network's grid
+        if getattr(sim, "tally_grid", None) != self.grid:   # the run tallies on the network's grid
             sim.tally_rebinned(self.grid)
-            sim._tally_grid = self.grid
         sim.run(seed=seed)
         est, s, decays = self.net.estimate(sim)
         t1 = time.perf_counter()

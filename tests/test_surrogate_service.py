@@ -116,6 +116,39 @@ def test_rebinned_tally_equals_the_rebinned_sinograms():
     tab.close()
 
 
+@pytest.mark.gpu
+def test_rebinned_tally_follows_the_sinograms_across_accumulating_runs():
+    """A run that adds to the sinograms of a run made with the rebinned tally off holds rebinned
+    counts of its own run only: the fused estimate must then rebin the sinograms. Runs that add
+    to each other with the tally on keep the two in step. Either way the counts the estimate
+    uses equal the rebinning of the sinograms."""
+    phs = _phantoms(1, 93)
+    names = S._materials()
+    tab = mcpet.load_materials(names, phs[0].density_max())
+    est = S.ScatterEstimator()
+    with mcpet.Simulation(S.config(names, S.N_LOW), phs[0], tab) as sim:
+        sim.run(seed=5)                        # rebinned tally off
+        sim.tally_rebinned(S.GRID)
+        sim.run(seed=6, zero_tallies=False)    # adds to the first run's sinograms
+        _, cnt1, _ = est.net.estimate(sim)
+        s1 = sim.rebin("scatter", S.GRID)
+        sim.run(seed=7)                        # cleared, tally on
+        sim.run(seed=8, zero_tallies=False)
+        e2, cnt2, d2 = est.net.estimate(sim)
+        s2 = sim.rebin("scatter", S.GRID)
+        sim.tally_rebinned(None)
+        sim.run(seed=7)
+        sim.run(seed=8, zero_tallies=False)
+        e3, cnt3, d3 = est.net.estimate(sim)
+    assert s1.sum() > 0 and s2.sum() > s1.sum() / 4
+    assert np.array_equal(cnt1, s1)
+    assert np.array_equal(cnt2, s2) and np.array_equal(cnt3, s2)
+    assert np.array_equal(e2, e3) and d2 == d3
+    est.close()
+    phs[0].close()
+    tab.close()
+
+
 def _phantoms(n, seed):
     return [mcpet.phantom_from_arrays(*sp) for sp in
             phantoms.scatter_phantoms(n, seed, n_decays=S.DECAYS_PER_SECOND, acquisition_time_s=1.0)]
