@@ -496,7 +496,9 @@ mcpet_status mcpet_rebin_sinogram(mcpet_ctx* c, int32_t which, int32_t D, int32_
 /* From the next run on, every detected trues or scatter coincidence is also added to its bin
  * of the D x H x W rebinning grid (the grid of mcpet_rebin_sinogram), so that the fused
  * estimate (mcpet_surrogate_estimate) on that grid reads these counts instead of rebinning the
- * sinograms after the run: the same integer sums. The sinograms are tallied as before.
+ * sinograms after the run: the same integer sums. The sinograms are tallied as before. The
+ * estimate reads the counts only while every run since the last clear of the tallies added to
+ * them (a run with zero_tallies = 0 after one with the tally off: it rebins the sinograms).
  * D = H = W = 0 turns it off. Not in the singles mode (MCPET_E_STATE). No reference
  * counterpart: an extension for the scatter surrogate's input runs. */
 mcpet_status mcpet_tally_rebinned(mcpet_ctx* c, int32_t D, int32_t H, int32_t W);
