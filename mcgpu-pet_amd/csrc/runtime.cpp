@@ -1230,7 +1230,10 @@ mcpet_status mcpet_ctx_decays(mcpet_ctx* c, long long* decays)
 static void lor_groups_build(const KParams& P, int H, int W, std::vector<int>& off, std::vector<int2>& lors)
 {
   const int N = P.ncrystals, gh = (P.nangles + H - 1) / H, gw = (P.nrad + W - 1) / W;
-  std::vector<std::vector<int2>> g((size_t)H * W);
+  // group of every pair (-1: outside the sinogram), counted, then placed by a counting sort that
+  // keeps the (k1, k2) order inside each group
+  std::vector<int> gid((size_t)N * N, -1);
+  off.assign((size_t)H * W + 1, 0);
   for (int k1 = 0; k1 < N; k1++)
     for (int k2 = k1 + 1; k2 < N; k2++) {
       const int ith = ((k1 + k2 + P.nangles + 1) % N) / 2;
@@ -1238,14 +1241,18 @@ static void lor_groups_build(const KParams& P, int H, int W, std::vector<int>& o
       if (k1 < ith || k2 >= ith + P.nangles) ir = -ir;
       ir += P.nrad / 2;
       if (ir < 0 || ir >= P.nrad || ith < 0 || ith >= P.nangles) continue;
-      g[(size_t)(ith / gh) * W + ir / gw].push_back(int2{k1, k2});
+      const int g = (ith / gh) * W + ir / gw;
+      gid[(size_t)k1 * N + k2] = g;
+      off[g + 1]++;
     }
-  off.assign((size_t)H * W + 1, 0);
-  lors.clear();
-  for (size_t i = 0; i < g.size(); i++) {
-    lors.insert(lors.end(), g[i].begin(), g[i].end());
-    off[i + 1] = (int)lors.size();
-  }
+  for (size_t i = 0; i + 1 < off.size(); i++) off[i + 1] += off[i];
+  lors.resize((size_t)off.back());
+  std::vector<int> next(off.begin(), off.end() - 1);
+  for (int k1 = 0; k1 < N; k1++)
+    for (int k2 = k1 + 1; k2 < N; k2++) {
+      const int g = gid[(size_t)k1 * N + k2];
+      if (g >= 0) lors[(size_t)next[g]++] = int2{k1, k2};
+    }
 }
 
 // lor_groups_build on the device, kept per (device, crystals, angles, radial bins, H, W) until
@@ -1270,16 +1277,21 @@ static mcpet_status lor_lists(mcpet_ctx* c, int H, int W, LorLists* out)
   std::vector<int> off;
   std::vector<int2> lors;
   lor_groups_build(P, H, W, off, lors);
-  LorLists l;
-  hipError_t e = hipMalloc((void**)&l.lors, sizeof(int2) * std::max<size_t>(1, lors.size()));
-  if (e == hipSuccess) e = hipMalloc((void**)&l.off, sizeof(int) * off.size());
-  if (e == hipSuccess && !lors.empty()) e = hipMemcpy(l.lors, lors.data(), sizeof(int2) * lors.size(), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(l.off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice);
+  // one allocation and one copy: the pairs, then the offsets
+  const size_t lb = sizeof(int2) * lors.size(), ob = sizeof(int) * off.size();
+  std::vector<char> host(lb + ob);
+  if (lb) std::memcpy(host.data(), lors.data(), lb);
+  std::memcpy(host.data() + lb, off.data(), ob);
+  char* d = nullptr;
+  hipError_t e = hipMalloc((void**)&d, host.size());
+  if (e == hipSuccess) e = hipMemcpy(d, host.data(), host.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
-    if (l.lors) (void)hipFree(l.lors);
-    if (l.off) (void)hipFree(l.off);
+    if (d) (void)hipFree(d);
     return set_err(c, MCPET_E_NOMEM, std::string("project: crystal-pair lists: ") + hipGetErrorString(e));
   }
+  LorLists l;
+  l.lors = reinterpret_cast<int2*>(d);
+  l.off = reinterpret_cast<int*>(d + lb);
   cache[key] = l;
   *out = l;
   return MCPET_OK;
